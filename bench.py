@@ -1,0 +1,235 @@
+"""Space-charge kick throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload flattop_10m_127]
+
+Workload (default): 10M macroparticles, 1 nC, 500 MeV, flat-top 2 mm with
+0.2 mm Gaussian edges and chirp -10 1/m, mesh 127^3.  One step = a 0.1 m drift
+followed by one space-charge kick with dz = 0.1 m (the kick of
+SpaceCharge.apply, ocelot/cpbd/sc.py:208-251), mixed precision, the step
+captured once in a HIP graph and replayed K times; the bunch is resident in
+HBM before the timed region starts.  The same K steps are then timed in the
+plain FP64 form, a few steps are re-run with per-stage HIP events for the
+roofline of the dominant kernel, and the FP64 oracle is timed on the host
+cores on one kick of a particle sample of the same bunch.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "SC macroparticle-kicks/s at 10M, 127³ mesh (mixed vs FP64); field RMS err"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="flattop_10m_127")
+    ap.add_argument("--particles", type=int, default=None, help="override the particle count")
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000,
+                    help="particles in the CPU-baseline kick (0: skip)")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--deposit-xcd", default="auto", choices=["0", "1", "auto"],
+                    help="charge deposit into one grid (0), per-XCD copies (1), or pick the faster after "
+                         "checking both give identical kicks (auto)")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(n, nmesh, prec):
+    """Compulsory HBM bytes of one kick (particle passes A-D + pruned mesh
+    passes), and of the kick kernel alone.  See DESIGN.md for the derivation."""
+    nx, ny, nz = nmesh
+    mx, my, mz = (1 << int(np.ceil(np.log2(2 * v - 1))) for v in nmesh)
+    hz = mz // 2 + 1
+    c = 8 if prec == "mixed" else 16            # complex element
+    f = 4 if prec == "mixed" else 8             # real element
+    w = 8 if prec == "mixed" else 16            # fixed-point charge words
+    part = (24 + 56 + 56 + 96) * n
+    nf = 3 if prec == "mixed" else 1            # mixed: three field grids, FP64: the potential
+    nc = nx * ny * nz
+    mesh = (2 * w * nc                                         # zero + read charge grid
+            + c * nx * ny * hz + 2 * c * nx * my * hz            # z pass write, y pass read/write
+            + c * nx * my * hz + f * (mx // 2 + 1) * (my // 2 + 1) * hz  # x pass read + spectrum
+            + (2 if nf == 3 else 1) * c * nx * my * hz           # x pass write (Qx, P)
+            + (2 if nf == 3 else 1) * c * nx * my * hz + nf * c * nx * ny * hz  # inverse y
+            + nf * c * nx * ny * hz + nf * f * nc)               # inverse z -> fields
+    kick = 96 * n + nf * f * nc
+    return part + mesh, kick
+
+
+def cpu_baseline(name, n_sample, seed=0):
+    """One FP64 oracle kick (numpy/scipy, the reference's fallback library
+    calls) on a particle sample of the workload, BLAS limited to one thread."""
+    from oracle import sc_oracle as orc
+    from ocelot_amd.workloads import make_bunch
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    # same distribution and total charge, fewer macroparticles
+    r, q, E, nmesh, dz = make_bunch(name, n=n_sample, seed=seed)
+    if threadpool_limits is not None:
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            orc.apply(r, q, E, dz, nmesh)
+            dt = time.perf_counter() - t0
+    else:
+        t0 = time.perf_counter()
+        orc.apply(r, q, E, dz, nmesh)
+        dt = time.perf_counter() - t0
+    return dict(value=n_sample / dt, unit="macroparticle-kicks/s", cores=1, kind="port",
+                sample=f"{name}: one FP64 oracle kick of {n_sample} particles of the same distribution on "
+                       f"{nmesh[0]}^3, {dt:.2f} s")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; the multi-GPU design "
+                                   "(particle sharding + one charge-grid all-reduce) is described in "
+                                   "DESIGN.md and not built", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+    from ocelot_amd import DeviceParticleArray, ParticleArray, SpaceCharge, track_drift
+    from ocelot_amd import _lib
+    from ocelot_amd.workloads import make_bunch
+
+    r, q, E, nmesh, dz = make_bunch(a.workload, n=a.particles)
+    n = r.shape[1]
+    host = ParticleArray()
+    host.rparticles, host.q_array, host.E = r, q, E
+    dev = torch.cuda.current_device()
+    L = dz
+
+    def pick_deposit():
+        """Autotune the deposit mode: both modes must give bit-identical kicks;
+        keep the faster."""
+        if a.deposit_xcd != "auto":
+            return int(a.deposit_xcd), {}
+        res, times = [], {}
+        for mode in (0, 1):
+            _lib.set_option(dev, "deposit_xcd", mode)
+            sc = SpaceCharge(nmesh_xyz=list(nmesh), precision="mixed")
+            d = DeviceParticleArray.from_host(host, dev)
+            sc.apply(d, dz)
+            res.append(d.rparticles.clone())
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                sc.apply(d, dz)
+            torch.cuda.synchronize()
+            times[mode] = (time.perf_counter() - t0) / 3
+        same = bool(torch.equal(res[0], res[1]))
+        mode = 1 if (same and times[1] < times[0]) else 0
+        return mode, {"identical": same, "ms_per_kick": {str(k): round(v * 1e3, 4) for k, v in times.items()}}
+
+    deposit_mode, deposit_tune = pick_deposit()
+    _lib.set_option(dev, "deposit_xcd", deposit_mode)
+    print(f"# deposit_xcd={deposit_mode} {deposit_tune}", file=sys.stderr, flush=True)
+
+    def run(prec, steps, warmup):
+        sc = SpaceCharge(nmesh_xyz=list(nmesh), precision=prec)
+        d = DeviceParticleArray.from_host(host, dev)
+        track_drift(d, sc, L, dz, warmup, use_graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        track_drift(d, sc, L, dz, steps, use_graph=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        return (t1 - t0) / steps, d
+
+    print(f"# workload {a.workload}: {n} particles, mesh {nmesh}", file=sys.stderr, flush=True)
+    t_mix, _ = run("mixed", a.steps, a.warmup)
+    print(f"# mixed: {t_mix * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+    t_64 = None
+    if not a.no_fp64:
+        t_64, _ = run("fp64", a.steps, a.warmup)
+        print(f"# fp64: {t_64 * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # field error of one kick, mixed vs FP64, at full size (kick deltas are the
+    # lab-frame field times fixed factors)
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        d = DeviceParticleArray.from_host(host, dev)
+        SpaceCharge(nmesh_xyz=list(nmesh), precision=prec).apply(d, dz)
+        outs[prec] = d.rparticles
+    r0 = torch.as_tensor(r, device=outs["fp64"].device)
+    errs = []
+    for i in (1, 3, 5):
+        ref = outs["fp64"][i] - r0[i]
+        got = outs["mixed"][i] - r0[i]
+        errs.append(float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref)))
+    field_rms = max(errs)
+    del outs, r0
+
+    # per-stage HIP events over a few eager steps (same kick, same stream)
+    lib = _lib.load()
+    ctx = _lib.context(dev)
+    stages = {}
+    prof = {}
+    for prec in ("mixed", "fp64"):
+        sc = SpaceCharge(nmesh_xyz=list(nmesh), precision=prec)
+        d = DeviceParticleArray.from_host(host, dev)
+        sc.apply(d, dz)
+        _lib.check(lib.ocsc_set_profiling(ctx.handle, 1))
+        for _ in range(a.profile_steps):
+            sc.apply(d, dz)
+        import ctypes
+        ms = (ctypes.c_double * 16)()
+        calls = ctypes.c_int64()
+        ns = lib.ocsc_stage_times(ctx.handle, ms, 16, ctypes.byref(calls))
+        _lib.check(lib.ocsc_set_profiling(ctx.handle, 0))
+        stages[prec] = {lib.ocsc_stage_name(i).decode(): round(ms[i], 4) for i in range(ns)}
+        prof[prec] = [ms[i] for i in range(ns)]
+    total_b, kick_b = algorithmic_bytes(n, nmesh, "mixed")
+    total_b64, _ = algorithmic_bytes(n, nmesh, "fp64")
+    kick_ms = stages["mixed"]["kick"]
+    dom = max(stages["mixed"], key=lambda k: stages["mixed"][k])
+    achieved = kick_b / (kick_ms * 1e-3) / 1e9
+    roofline = {"kernel": "k_kick<float> (gather + kick + write-back)", "bound": "hbm",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel_ms": kick_ms, "algorithmic_bytes": kick_b, "largest_stage": dom,
+                "kick_total_bytes": total_b,
+                "kick_total_GBs": round(total_b / (sum(prof["mixed"]) * 1e-3) / 1e9, 1),
+                "kick_total_bytes_fp64": total_b64,
+                "kick_total_GBs_fp64": round(total_b64 / (sum(prof["fp64"]) * 1e-3) / 1e9, 1)}
+
+    cpu = None
+    if a.cpu_sample > 0:
+        cpu = cpu_baseline(a.workload, min(a.cpu_sample, n))
+
+    value = n / t_mix
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "macroparticle-kicks/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t_mix * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 mesh/FFT/gather + fp64 frame/Green/momentum",
+        "data": "synthetic (seeded flat-top bunch, numpy default_rng)",
+        "config": {"workload": a.workload, "particles": n, "nmesh": list(nmesh), "kick_dz_m": dz,
+                   "drift_m": L, "precision": "mixed", "hip_graph": True, "deposit_xcd": deposit_mode},
+        "deposit_autotune": deposit_tune,
+        "fp64": None if t_64 is None else {"value": round(n / t_64, 1), "ms_per_step": round(t_64 * 1e3, 4)},
+        "mixed_over_fp64": None if t_64 is None else round(t_64 / t_mix, 3),
+        "field_rms_rel_err": field_rms,
+        "stages_ms": stages,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": None if cpu is None else round(value / cpu["value"], 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

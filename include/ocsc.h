@@ -1,0 +1,130 @@
+/*
+ * ocsc -- MI355X (gfx950) 3D space-charge kick, C ABI.
+ *
+ * Drop-in compute for Ocelot's SpaceCharge physics process
+ * (ocelot/cpbd/sc.py:76-258).  The Python class ocelot_amd.SpaceCharge binds
+ * these entry points with ctypes; any other host language binds them the
+ * same way.  No PyTorch or HIP types appear in the signatures: particle
+ * buffers are plain device pointers (allocated by the caller, e.g. as torch
+ * tensors), streams are passed as void* (a hipStream_t, NULL = default).
+ *
+ * Particle layout is Ocelot's ParticleArray.rparticles
+ * (ocelot/cpbd/beam/particle.py:79-84): six FP64 rows x, x', y, y', tau,
+ * delta, row i starting at rparticles + i*ld (ld >= n; ld = n for a
+ * contiguous (6, n) array).  Charges q_array are FP64 [C].
+ *
+ * Return value of every entry point: 0 on success, otherwise one of the
+ * OCSC_E* codes; ocsc_last_error() gives a message for the calling thread.
+ * OCSC_EVALUE marks inputs for which the reference raises ValueError (empty
+ * bunch, zero total charge, non-finite coordinates); rparticles is left
+ * untouched in that case, as in the reference.
+ *
+ * Contexts are independent and may live on different devices; one context
+ * must not be used from two host threads at once.
+ */
+#ifndef OCSC_H
+#define OCSC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCSC_OK 0
+#define OCSC_EINVAL 1 /* bad argument (null pointer, nmesh < 4, n < 0, ...) */
+#define OCSC_EVALUE 2 /* input the reference rejects with ValueError */
+#define OCSC_EHIP 3   /* HIP runtime error */
+#define OCSC_ENOMEM 4 /* device allocation failed */
+
+#define OCSC_FP64 0  /* FP64 everywhere */
+#define OCSC_MIXED 1 /* FP32 mesh, FFT and gather; FP64 frame, Green's function, momentum update */
+
+typedef struct ocsc_ctx ocsc_ctx;
+
+/* Per-process state of one SpaceCharge instance on one device: workspaces,
+ * twiddle tables, the kernel-spectrum cache.  Replaces the implicit state of
+ * SpaceCharge.__init__ (ocelot/cpbd/sc.py:92-102), which holds no buffers. */
+int ocsc_create(ocsc_ctx** ctx, int device);
+int ocsc_destroy(ocsc_ctx* ctx);
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* ocsc_last_error(void);
+
+/* One space-charge kick, SpaceCharge.apply(p_array, zstep)
+ * (ocelot/cpbd/sc.py:208-251).  All six rows of rparticles are rewritten in
+ * place.  zstep == 0 is a no-op, as in the reference (sc.py:210-212).
+ * random_mesh != 0 applies the two host draws of the reference
+ * (sc.py:174-175, 184-185): the extent factor mesh_scale ~ U(1,1.1) and the
+ * offset mesh_shift ~ U(-0.5,0.5); the caller draws them in that order.
+ * Asynchronous on stream except when an input error must be reported: the
+ * call then synchronises to read the device error flags (sync != 0), or
+ * defers them to ocsc_check (sync == 0). */
+int ocsc_apply(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,
+               const int32_t nmesh[3], double E_GeV, double zstep, int random_mesh, double mesh_scale,
+               double mesh_shift, int precision, int sync, void* stream);
+
+/* Error flags left by kicks issued with sync == 0 (synchronises stream);
+ * returns OCSC_EVALUE if any kick saw an invalid bunch and clears the flag. */
+int ocsc_check(ocsc_ctx* ctx, void* stream);
+
+/* SpaceCharge.el_field (ocelot/cpbd/sc.py:170-205): lab-frame field (V/m) at
+ * rest-frame positions xyz (three FP64 rows x, y, z with z not yet scaled by
+ * gamma, row stride ld).  Exyz: three FP64 rows, row stride lde.  cells
+ * (optional, may be NULL): the flat C-order NGP cell index of each particle
+ * (sc.py:191-192), int32.  Synchronous. */
+int ocsc_el_field(ocsc_ctx* ctx, const double* xyz, int64_t ld, const double* q_array, int64_t n,
+                  const int32_t nmesh[3], double gamma, int random_mesh, double mesh_scale, double mesh_shift,
+                  int precision, double* Exyz, int64_t lde, int32_t* cells, void* stream);
+
+/* SpaceCharge.sym_kernel (ocelot/cpbd/sc.py:109-133): integrated Green's
+ * function on nmesh[0] x nmesh[1] x nmesh[2] cells of size h, FP64, written
+ * to the device array K (C order).  Synchronous. */
+int ocsc_sym_kernel(ocsc_ctx* ctx, const int32_t nmesh[3], const double h[3], double* K, void* stream);
+
+/* SpaceCharge.potential (ocelot/cpbd/sc.py:135-168): potential of the device
+ * charge grid rho (FP64, C order) with cell sizes h.  phi is FP64 for
+ * precision OCSC_FP64 and FP32 for OCSC_MIXED.  Synchronous. */
+int ocsc_potential(ocsc_ctx* ctx, const double* rho, const int32_t nmesh[3], const double h[3], void* phi,
+                   int precision, void* stream);
+
+/* First-order drift of length L (ocelot/cpbd/r_matrix.py:41-91 with
+ * k1 = hx = 0, applied as in ocelot/cpbd/transformations/transfer_map.py:40-50):
+ * x += L x', y += L y', tau += r56 delta, r56 = -L / (beta^2 gamma^2). */
+int ocsc_drift(ocsc_ctx* ctx, double* rparticles, int64_t ld, int64_t n, double L, double E_GeV, void* stream);
+
+/* nsteps x [drift(L); kick(zstep)] on a device-resident bunch, the tracking
+ * loop of ocelot/cpbd/track.py:470-477 restricted to drifts.  use_graph != 0
+ * captures one step in a HIP graph once and replays it.  Error flags are
+ * checked once at the end (synchronises). */
+int ocsc_track_drift(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,
+                     const int32_t nmesh[3], double E_GeV, double zstep, double L, int nsteps, int precision,
+                     int use_graph, void* stream);
+
+/* Tuning options of a context (the autotuner's knobs; no reference
+ * counterpart).  "deposit_xcd": 0 = one charge grid, device-scope atomics;
+ * 1 = one grid copy per XCD with atomics that complete in that XCD's L2,
+ * summed exactly in the first FFT pass.  Both give bit-identical results. */
+int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
+int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
+
+/* Stage timing for performance work: with profiling on, every kick issued
+ * outside a graph capture records HIP events between its stages and waits
+ * for them (this serialises the host with the device).  ocsc_stage_times
+ * writes the mean milliseconds per stage since profiling was switched on and
+ * returns the number of stages; ocsc_stage_name names stage i.  No reference
+ * counterpart (the reference logs the FFT wall time, ocelot/cpbd/sc.py:150-166). */
+int ocsc_set_profiling(ocsc_ctx* ctx, int on);
+int ocsc_stage_times(ocsc_ctx* ctx, double* ms, int nout, int64_t* calls);
+const char* ocsc_stage_name(int i);
+
+/* Scalars of the last kick, for diagnostics and parity tests (synchronises):
+ * out[0] gamma0, [1] Pav, [2..10] T (row-major, T[j][k]), [11..13] steps,
+ * [14..16] X_off, [17] total charge, [18] error bits, [19] kernel rebuilt.
+ * Writes min(nout, 20) values. */
+int ocsc_last_stats(ocsc_ctx* ctx, double* out, int nout, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCSC_H */

@@ -1,0 +1,13 @@
+"""MI355X-native 3D space-charge kick, a drop-in for Ocelot's SpaceCharge.
+
+    from ocelot_amd import SpaceCharge, ParticleArray, DeviceParticleArray
+
+The compute path is libocsc.so (HIP kernels for gfx950, C ABI in
+include/ocsc.h); importing this package does not load it, the first call
+does, and raises if it is missing.
+"""
+from .physproc import PhysProc
+from .beam import ParticleArray, DeviceParticleArray
+from .sc import SpaceCharge, drift, track_drift
+
+__all__ = ["PhysProc", "ParticleArray", "DeviceParticleArray", "SpaceCharge", "drift", "track_drift"]

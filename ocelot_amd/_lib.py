@@ -1,0 +1,138 @@
+"""ctypes binding of libocsc.so (include/ocsc.h).
+
+The library is the product path: there is no CPU fallback.  If it is missing
+or fails to load, every entry point raises.
+"""
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("OCSC_LIB", os.path.join(HERE, "libocsc.so"))
+
+OCSC_OK = 0
+OCSC_EINVAL = 1
+OCSC_EVALUE = 2
+OCSC_EHIP = 3
+OCSC_ENOMEM = 4
+OCSC_FP64 = 0
+OCSC_MIXED = 1
+
+_c_int = ctypes.c_int
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+# name -> (restype, argtypes); the list every header declaration must match
+SIGNATURES = {
+    "ocsc_create": (_c_int, [ctypes.POINTER(_vp), _c_int]),
+    "ocsc_destroy": (_c_int, [_vp]),
+    "ocsc_last_error": (ctypes.c_char_p, []),
+    "ocsc_apply": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _c_int, _dbl, _dbl, _c_int, _c_int, _vp]),
+    "ocsc_check": (_c_int, [_vp, _vp]),
+    "ocsc_el_field": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _c_int, _dbl, _dbl, _c_int, _vp, _i64, _vp,
+                               _vp]),
+    "ocsc_sym_kernel": (_c_int, [_vp, _i32p, _dp, _vp, _vp]),
+    "ocsc_potential": (_c_int, [_vp, _vp, _i32p, _dp, _vp, _c_int, _vp]),
+    "ocsc_drift": (_c_int, [_vp, _vp, _i64, _i64, _dbl, _dbl, _vp]),
+    "ocsc_track_drift": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int, _c_int, _vp]),
+    "ocsc_last_stats": (_c_int, [_vp, _dp, _c_int, _vp]),
+    "ocsc_set_option": (_c_int, [_vp, ctypes.c_char_p, _i64]),
+    "ocsc_get_option": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_i64)]),
+    "ocsc_set_profiling": (_c_int, [_vp, _c_int]),
+    "ocsc_stage_times": (_c_int, [_vp, _dp, _c_int, ctypes.POINTER(_i64)]),
+    "ocsc_stage_name": (ctypes.c_char_p, [_c_int]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path=None):
+    """Load libocsc.so once and declare the C signatures."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise RuntimeError(f"libocsc.so not found at {p}: build it with `python -m ocelot_amd.build` "
+                               "(the space-charge kick has no CPU fallback)")
+        lib = ctypes.CDLL(p)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+class OcscError(RuntimeError):
+    pass
+
+
+def check(rc):
+    """Raise for a non-zero status: ValueError where the reference raises
+    ValueError (OCSC_EVALUE), RuntimeError otherwise."""
+    if rc == OCSC_OK:
+        return
+    msg = load().ocsc_last_error().decode(errors="replace")
+    if rc == OCSC_EVALUE:
+        raise ValueError(msg)
+    if rc == OCSC_EINVAL:
+        raise ValueError("invalid argument: " + msg)
+    raise OcscError(f"ocsc error {rc}: {msg}")
+
+
+def i32x3(v):
+    a = (ctypes.c_int32 * 3)(*[int(x) for x in v])
+    return a
+
+
+def d3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+class Context:
+    """One library context (device workspaces and caches) per device."""
+
+    def __init__(self, device):
+        lib = load()
+        h = _vp()
+        check(lib.ocsc_create(ctypes.byref(h), int(device)))
+        self.handle = h
+        self.device = int(device)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load().ocsc_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def set_option(device, name, value):
+    check(load().ocsc_set_option(context(device).handle, name.encode(), int(value)))
+
+
+def get_option(device, name):
+    v = _i64()
+    check(load().ocsc_get_option(context(device).handle, name.encode(), ctypes.byref(v)))
+    return v.value
+
+
+_contexts = {}
+
+
+def context(device):
+    """Process-wide context of a device (created lazily, never pickled)."""
+    device = int(device)
+    with _lock:
+        ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _lock:
+            _contexts[device] = ctx
+    return ctx

@@ -1,0 +1,147 @@
+// LDS-resident radix-2/4/8/16 Stockham FFT for power-of-two lengths 8..1024.
+//
+// A workgroup holds T columns of length M in LDS as s[m*S + t] (S = T + 1:
+// the pad keeps column-stride accesses off a single bank).  Each pass reads
+// R inputs with stride M/R, twiddles them, does an R-point DFT in registers
+// and writes the outputs in Stockham (self-sorting) order, so the result is
+// in natural order after the last pass.  Twiddles W_M^k come from a table
+// computed on the host in double precision.
+#pragma once
+#include "common.hpp"
+
+namespace ocsc {
+
+// cos / sin of 2 pi k / 16, k = 0..15
+__device__ __forceinline__ constexpr double w16c(int k) {
+  constexpr double c[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                            0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                            -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                            0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+  return c[k & 15];
+}
+__device__ __forceinline__ constexpr double w16s(int k) { return w16c(k - 4); }
+
+// y = x * exp(DIR * 2 pi i * k / R) with constant k/R; DIR = -1 forward
+template <int R, int K, int DIR, typename F>
+__device__ __forceinline__ cpx<F> twc(cpx<F> v) {
+  constexpr int k16 = ((K * 16 / R) % 16 + 16) % 16;
+  if constexpr (k16 == 0) {
+    return v;
+  } else if constexpr (k16 == 8) {
+    return {-v.x, -v.y};
+  } else if constexpr ((k16 == 4 && DIR > 0) || (k16 == 12 && DIR < 0)) {
+    return {-v.y, v.x};  // * i
+  } else if constexpr ((k16 == 4 && DIR < 0) || (k16 == 12 && DIR > 0)) {
+    return {v.y, -v.x};  // * -i
+  } else {
+    const F c = (F)w16c(k16);
+    const F s = (F)(DIR * w16s(k16));
+    return cmul(v, cpx<F>{c, s});
+  }
+}
+
+// In-register R-point DFT (radix-2 decimation in time, fully unrolled).
+template <int R, int DIR, typename F>
+struct Dft {
+  template <int K>
+  __device__ __forceinline__ static void combine(cpx<F>* v, const cpx<F>* e, const cpx<F>* o) {
+    if constexpr (K < R / 2) {
+      cpx<F> t = twc<R, K, DIR>(o[K]);
+      v[K] = cadd(e[K], t);
+      v[K + R / 2] = csub(e[K], t);
+      combine<K + 1>(v, e, o);
+    }
+  }
+  __device__ __forceinline__ static void run(cpx<F>* v) {
+    if constexpr (R == 2) {
+      cpx<F> a = v[0], b = v[1];
+      v[0] = cadd(a, b);
+      v[1] = csub(a, b);
+    } else {
+      cpx<F> e[R / 2], o[R / 2];
+#pragma unroll
+      for (int i = 0; i < R / 2; ++i) {
+        e[i] = v[2 * i];
+        o[i] = v[2 * i + 1];
+      }
+      Dft<R / 2, DIR, F>::run(e);
+      Dft<R / 2, DIR, F>::run(o);
+      combine<0>(v, e, o);
+    }
+  }
+};
+
+// Radix plan of a length: passes of radix 16/8/4/2, largest first.
+template <int M>
+struct Plan {
+  static constexpr int L = ilog2(M);
+  // number of radix-16 passes, then one remainder pass
+  static constexpr int n16 = L / 4;
+  static constexpr int rem = L % 4;  // 0, 1 (radix 2), 2 (radix 4), 3 (radix 8)
+  static constexpr int npass = n16 + (rem ? 1 : 0);
+  static constexpr int radix(int p) { return p < n16 ? 16 : (1 << rem); }
+  // smallest radix decides the butterfly count per column
+  static constexpr int rmin = rem ? (1 << rem) : 16;
+};
+
+// One Stockham pass of radix R over T columns (LDS row stride S), NT threads.
+template <typename F, int M, int R, int T, int NT, int DIR, int S>
+__device__ __forceinline__ void stockham_pass(cpx<F>* s, int Ns, const cpx<F>* __restrict__ tw) {
+  constexpr int NB = M / R;            // butterflies per column
+  constexpr int TOT = NB * T;          // butterflies per workgroup
+  constexpr int C = (TOT + NT - 1) / NT;  // per thread
+  cpx<F> v[C][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int w = tid + c * NT;
+    if (TOT % NT == 0 || w < TOT) {
+      const int t = w % T;
+      const int j = w / T;
+      const int jm = j & (Ns - 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[c][r] = s[(j + r * NB) * S + t];
+      // twiddle exponent jm*r/(Ns*R) of a turn -> table index jm*r*(M/(Ns*R))
+      const int step = jm * (M / (Ns * R));
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cpx<F> w_ = tw[(step * r) & (M - 1)];
+        if (DIR > 0) w_.y = -w_.y;
+        v[c][r] = cmul(v[c][r], w_);
+      }
+      Dft<R, DIR, F>::run(v[c]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int w = tid + c * NT;
+    if (TOT % NT == 0 || w < TOT) {
+      const int t = w % T;
+      const int j = w / T;
+      const int jm = j & (Ns - 1);
+      const int base = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[(base + r * Ns) * S + t] = v[c][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename F, int M, int T, int NT, int DIR, int S, int P = 0>
+__device__ __forceinline__ void lds_fft_from(cpx<F>* s, int Ns, const cpx<F>* __restrict__ tw) {
+  if constexpr (P < Plan<M>::npass) {
+    constexpr int R = Plan<M>::radix(P);
+    stockham_pass<F, M, R, T, NT, DIR, S>(s, Ns, tw);
+    lds_fft_from<F, M, T, NT, DIR, S, P + 1>(s, Ns * R, tw);
+  }
+}
+
+// In-place FFT of T columns held in LDS as s[m*S + t] (caller synchronises
+// before).  DIR = -1: forward exp(-i...), DIR = +1: inverse without 1/M.
+template <typename F, int M, int T, int NT, int DIR, int S = T + 1>
+__device__ __forceinline__ void lds_fft(cpx<F>* s, const cpx<F>* __restrict__ tw) {
+  lds_fft_from<F, M, T, NT, DIR, S, 0>(s, 1, tw);
+}
+
+}  // namespace ocsc

@@ -1,0 +1,478 @@
+// Open-boundary Poisson solve on the rest-frame mesh (Hockney doubled grid).
+//
+// Reference: ocelot/cpbd/sc.py:109-168 (sym_kernel, potential).  The reference
+// convolves on (2N-1)^3 with numpy/pyfftw; here every axis uses the power of
+// two M >= 2N-1 (the circular convolution restricted to [0,N)^3 is the same
+// linear convolution for any M >= 2N-1, with the even kernel laid out as
+// K2[M-m] = K[m]).  The transforms are pruned: the charge has N non-zeros per
+// line and only N outputs per line are kept, so no pass touches the zero
+// padding in HBM.
+//
+// Layouts (C order, last index fastest; Hz = Mz/2+1):
+//   rho/phi  [nx][ny][nz]      real
+//   A        [nx][ny][Hz]      complex   (after the z pass)
+//   B        [nx][My][Hz]      complex   (after the y pass; x pass in place)
+//   G        [Hx][Hy][Hz]      real      kernel spectrum, even in every axis,
+//                                        scaled by 1/(Mx My Mz 4 pi eps0 hx hy hz)
+#pragma once
+#include "fft.hpp"
+
+namespace ocsc {
+
+constexpr int kNT = 256;  // threads per workgroup for the mesh kernels
+
+// columns per workgroup: LDS footprint M*(T+1)*sizeof(cpx<F>) stays <= ~72 KB
+template <typename F, int M>
+struct Tile {
+  static constexpr int cap = sizeof(F) == 4 ? 64 : 32;
+  static constexpr int want = (sizeof(F) == 4 ? 4096 : 2048) / M;
+  static constexpr int T = want < 2 ? 2 : (want > cap ? cap : want);
+  static constexpr int S = T + 1;
+  static constexpr size_t lds_bytes = (size_t)M * S * sizeof(cpx<F>);
+};
+
+struct MeshDims {
+  int nx, ny, nz;
+  int mx, my, mz;
+  int hx, hy, hz;  // M/2 + 1 per axis
+};
+
+enum ChargeIn : int { kChargeFix1 = 0, kChargeFix2 = 1, kChargeReal = 2 };
+
+// charge of cell i from the fixed-point accumulators (see deposit kernel);
+// ncopy grid copies (one per XCD, or 1) of NWORD words are summed exactly
+template <int CIN, typename F>
+__device__ __forceinline__ double charge_at(const void* src, int64_t i, int64_t ncell, double inv_qscale,
+                                            int ncopy) {
+  if constexpr (CIN == kChargeReal) {
+    return ((const double*)src)[i];
+  } else if constexpr (CIN == kChargeFix1) {
+    const long long* a = (const long long*)src;
+    unsigned long long hi = 0;
+    for (int c = 0; c < ncopy; ++c) hi += (unsigned long long)a[c * ncell + i];
+    return (double)(long long)hi * inv_qscale;
+  } else {
+    const long long* a = (const long long*)src;
+    unsigned long long hi = 0, lo = 0;
+    for (int c = 0; c < ncopy; ++c) {
+      hi += (unsigned long long)a[2 * c * ncell + i];
+      lo += (unsigned long long)a[(2 * c + 1) * ncell + i];
+    }
+    return ((double)(long long)hi + (double)(long long)lo * 0x1p-30) * inv_qscale;
+  }
+}
+
+// Spectral forward difference along one axis: (phi[i] - phi[i+1]) / h has
+// the transform (1 - exp(+2 pi i k / M)) / h times that of phi.  Evaluated
+// in FP64 from the FP64 twiddle table (1 - cos is taken there, where it does
+// not cancel) and rounded once.
+template <typename F>
+__device__ __forceinline__ cpx<F> dfac(const cpx<double>* __restrict__ twd, int k, double inv_h) {
+  const cpx<double> w = twd[k];  // (cos, -sin) of 2 pi k / M
+  return cpx<F>{(F)((1.0 - w.x) * inv_h), (F)(w.y * inv_h)};
+}
+
+// ---------------------------------------------------------------------------
+// z pass: pairs of real rows -> one complex FFT of length Mz -> two half spectra
+template <typename F, int M, int CIN>
+__global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d,
+                                              const KickScalars* __restrict__ sc,
+                                              const cpx<F>* __restrict__ tw, int ncopy = 1) {
+  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  if (sc->err) return;
+  const int nz = d.nz, Hz = d.hz;
+  const int64_t R = (int64_t)d.nx * d.ny;
+  const int64_t ncell = R * nz;
+  const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
+  const double iq = sc->inv_qscale;
+  // zero the padding rows m in [nz, M)
+  for (int i = threadIdx.x; i < (M - nz) * T; i += kNT) {
+    const int t = i % T, m = nz + i / T;
+    s[m * S + t] = cpx<F>{0, 0};
+  }
+  for (int i = threadIdx.x; i < 2 * T * nz; i += kNT) {
+    const int rl = i / nz, m = i - rl * nz;
+    const int64_t row = row0 + rl;
+    F v = 0;
+    if (row < R) v = (F)charge_at<CIN, F>(rho, row * nz + m, ncell, iq, ncopy);
+    F* p = (F*)&s[m * S + (rl >> 1)];
+    p[rl & 1] = v;
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, -1>(s, tw);
+  for (int i = threadIdx.x; i < T * Hz; i += kNT) {
+    const int t = i / Hz, k = i - t * Hz;
+    const int64_t ra = row0 + 2 * t, rb = ra + 1;
+    if (ra >= R) continue;
+    const cpx<F> z = s[k * S + t];
+    const cpx<F> w = s[((M - k) & (M - 1)) * S + t];
+    const F h = (F)0.5;
+    A[ra * Hz + k] = cpx<F>{(z.x + w.x) * h, (z.y - w.y) * h};
+    if (rb < R) A[rb * Hz + k] = cpx<F>{(z.y + w.y) * h, (w.x - z.x) * h};
+  }
+}
+
+// y pass forward: A[x][y<ny][kz] -> B[x][ky<My][kz]
+template <typename F, int M>
+__global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d,
+                                              const KickScalars* __restrict__ sc,
+                                              const cpx<F>* __restrict__ tw) {
+  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  if (sc->err) return;
+  const int Hz = d.hz, ny = d.ny;
+  const int kz0 = blockIdx.x * T;
+  const int x = blockIdx.y;
+  for (int i = threadIdx.x; i < M * T; i += kNT) {
+    const int t = i % T, y = i / T;
+    const int kz = kz0 + t;
+    cpx<F> v{0, 0};
+    if (y < ny && kz < Hz) v = A[((int64_t)x * ny + y) * Hz + kz];
+    s[y * S + t] = v;
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, -1>(s, tw);
+  for (int i = threadIdx.x; i < M * T; i += kNT) {
+    const int t = i % T, ky = i / T;
+    const int kz = kz0 + t;
+    if (kz < Hz) B[((int64_t)x * M + ky) * Hz + kz] = s[ky * S + t];
+  }
+}
+
+// x pass: forward FFT along x, multiply by the kernel spectrum, inverse FFT,
+// keep x < nx.  In place on B.
+template <typename F, int M>
+__global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* __restrict__ G, MeshDims d,
+                                               const KickScalars* __restrict__ sc,
+                                               const cpx<F>* __restrict__ tw) {
+  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  if (sc->err) return;
+  const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
+  const int kz0 = blockIdx.x * T;
+  const int ky = blockIdx.y;
+  const int64_t cs = (int64_t)My * Hz;  // stride between x planes
+  for (int i = threadIdx.x; i < M * T; i += kNT) {
+    const int t = i % T, x = i / T;
+    const int kz = kz0 + t;
+    cpx<F> v{0, 0};
+    if (x < nx && kz < Hz) v = B[x * cs + (int64_t)ky * Hz + kz];
+    s[x * S + t] = v;
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, -1>(s, tw);
+  const int fy = ky <= My - ky ? ky : My - ky;
+  for (int i = threadIdx.x; i < M * T; i += kNT) {
+    const int t = i % T, kx = i / T;
+    const int kz = kz0 + t;
+    const int fx = kx <= M - kx ? kx : M - kx;
+    const F g = kz < Hz ? G[((int64_t)fx * Hy + fy) * Hz + kz] : (F)0;
+    s[kx * S + t] = cscale(s[kx * S + t], g);
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, +1>(s, tw);
+  for (int i = threadIdx.x; i < nx * T; i += kNT) {
+    const int t = i % T, x = i / T;
+    const int kz = kz0 + t;
+    if (kz < Hz) B[x * cs + (int64_t)ky * Hz + kz] = s[x * S + t];
+  }
+}
+
+// y pass inverse: B[x][ky][kz] -> A[x][y<ny][kz]
+template <typename F, int M>
+__global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<F>* __restrict__ A, MeshDims d,
+                                              const KickScalars* __restrict__ sc,
+                                              const cpx<F>* __restrict__ tw) {
+  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  if (sc->err) return;
+  const int Hz = d.hz, ny = d.ny;
+  const int kz0 = blockIdx.x * T;
+  const int x = blockIdx.y;
+  for (int i = threadIdx.x; i < M * T; i += kNT) {
+    const int t = i % T, ky = i / T;
+    const int kz = kz0 + t;
+    cpx<F> v{0, 0};
+    if (kz < Hz) v = B[((int64_t)x * M + ky) * Hz + kz];
+    s[ky * S + t] = v;
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, +1>(s, tw);
+  for (int i = threadIdx.x; i < ny * T; i += kNT) {
+    const int t = i % T, y = i / T;
+    const int kz = kz0 + t;
+    if (kz < Hz) A[((int64_t)x * ny + y) * Hz + kz] = s[y * S + t];
+  }
+}
+
+// z pass inverse: two Hermitian half spectra -> one complex inverse FFT ->
+// two real rows of the potential, z < nz
+template <typename F, int M, bool DERIV = false>
+__global__ void __launch_bounds__(kNT) k_zinv(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d,
+                                              const KickScalars* __restrict__ sc,
+                                              const cpx<F>* __restrict__ tw,
+                                              const cpx<double>* __restrict__ twd = nullptr) {
+  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  if (sc->err) return;
+  const int nz = d.nz, Hz = d.hz;
+  const int64_t R = (int64_t)d.nx * d.ny;
+  const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
+  for (int i = threadIdx.x; i < T * Hz; i += kNT) {
+    const int t = i / Hz, k = i - t * Hz;
+    const int64_t ra = row0 + 2 * t, rb = ra + 1;
+    cpx<F> a{0, 0}, b{0, 0};
+    if (ra < R) a = A[ra * Hz + k];
+    if (rb < R) b = A[rb * Hz + k];
+    if constexpr (DERIV) {
+      const cpx<F> f = dfac<F>(twd, k, 1.0 / sc->steps[2]);
+      a = cmul(a, f);
+      b = cmul(b, f);
+    }
+    s[k * S + t] = cpx<F>{a.x - b.y, a.y + b.x};
+    if (k > 0 && k < M / 2) s[(M - k) * S + t] = cpx<F>{a.x + b.y, b.x - a.y};
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, +1>(s, tw);
+  for (int i = threadIdx.x; i < 2 * T * nz; i += kNT) {
+    const int rl = i / nz, z = i - rl * nz;
+    const int64_t row = row0 + rl;
+    if (row >= R) continue;
+    const cpx<F> v = s[z * S + (rl >> 1)];
+    phi[row * nz + z] = (rl & 1) ? v.y : v.x;
+  }
+}
+
+// columns per workgroup of the kernels that carry two or three fields
+template <int M>
+struct FieldTile {
+  static constexpr int T2 = Tile<float, M>::T;  // x pass: T forward, 2T inverse
+  static constexpr int S2 = 2 * T2 + 1;
+  static constexpr size_t lds2 = (size_t)M * S2 * sizeof(cpx<float>);
+  static constexpr int T3 = Tile<float, M>::T / 2 < 1 ? 1 : Tile<float, M>::T / 2;  // y inverse: 3 fields
+  static constexpr int S3 = 3 * T3 + 1;
+  static constexpr size_t lds3 = (size_t)M * S3 * sizeof(cpx<float>);
+};
+
+// Mixed form, x pass: forward FFT along x, times the kernel spectrum, then
+// two inverse FFTs: Qx = IFFT(S * Dx) (the x field) and P = IFFT(S) (the
+// potential, from which the y and z fields follow by their own factors).
+template <int M>
+__global__ void __launch_bounds__(kNT) k_xconv_field(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
+                                                     cpx<float>* __restrict__ P, const float* __restrict__ G,
+                                                     MeshDims d, const KickScalars* __restrict__ sc,
+                                                     const cpx<float>* __restrict__ tw,
+                                                     const cpx<double>* __restrict__ twd) {
+  using F = float;
+  constexpr int T = FieldTile<M>::T2, S = FieldTile<M>::S2;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  if (sc->err) return;
+  const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
+  const int kz0 = blockIdx.x * T;
+  const int ky = blockIdx.y;
+  const int64_t cs = (int64_t)My * Hz;
+  for (int i = threadIdx.x; i < M * T; i += kNT) {
+    const int t = i % T, x = i / T;
+    const int kz = kz0 + t;
+    cpx<F> v{0, 0};
+    if (x < nx && kz < Hz) v = B[x * cs + (int64_t)ky * Hz + kz];
+    s[x * S + t] = v;
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, -1, S>(s, tw);
+  const int fy = ky <= My - ky ? ky : My - ky;
+  const double inv_hx = 1.0 / sc->steps[0];
+  for (int i = threadIdx.x; i < M * T; i += kNT) {
+    const int t = i % T, kx = i / T;
+    const int kz = kz0 + t;
+    const int fx = kx <= M - kx ? kx : M - kx;
+    const F g = kz < Hz ? G[((int64_t)fx * Hy + fy) * Hz + kz] : (F)0;
+    const cpx<F> v = cscale(s[kx * S + t], g);
+    s[kx * S + t] = cmul(v, dfac<F>(twd, kx, inv_hx));
+    s[kx * S + T + t] = v;
+  }
+  __syncthreads();
+  lds_fft<F, M, 2 * T, kNT, +1, S>(s, tw);
+  for (int i = threadIdx.x; i < nx * T; i += kNT) {
+    const int t = i % T, x = i / T;
+    const int kz = kz0 + t;
+    if (kz < Hz) {
+      const int64_t o = x * cs + (int64_t)ky * Hz + kz;
+      Qx[o] = s[x * S + t];
+      P[o] = s[x * S + T + t];
+    }
+  }
+}
+
+// Mixed form, y inverse: Ex' = IFFT_y(Qx), Ey' = IFFT_y(P * Dy), Phi' = IFFT_y(P)
+// (all [nx][ny][Hz]); the z field is Dz * Phi', applied in the z inverse.
+template <int M>
+__global__ void __launch_bounds__(kNT) k_yinv_field(const cpx<float>* __restrict__ Qx,
+                                                    const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax,
+                                                    cpx<float>* __restrict__ Ay, cpx<float>* __restrict__ Ap,
+                                                    MeshDims d, const KickScalars* __restrict__ sc,
+                                                    const cpx<float>* __restrict__ tw,
+                                                    const cpx<double>* __restrict__ twd) {
+  using F = float;
+  constexpr int T = FieldTile<M>::T3, S = FieldTile<M>::S3;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  if (sc->err) return;
+  const int Hz = d.hz, ny = d.ny;
+  const int kz0 = blockIdx.x * T;
+  const int x = blockIdx.y;
+  const double inv_hy = 1.0 / sc->steps[1];
+  for (int i = threadIdx.x; i < M * T; i += kNT) {
+    const int t = i % T, ky = i / T;
+    const int kz = kz0 + t;
+    cpx<F> q{0, 0}, p{0, 0};
+    if (kz < Hz) {
+      const int64_t o = ((int64_t)x * M + ky) * Hz + kz;
+      q = Qx[o];
+      p = P[o];
+    }
+    s[ky * S + t] = q;
+    s[ky * S + T + t] = cmul(p, dfac<F>(twd, ky, inv_hy));
+    s[ky * S + 2 * T + t] = p;
+  }
+  __syncthreads();
+  lds_fft<F, M, 3 * T, kNT, +1, S>(s, tw);
+  for (int i = threadIdx.x; i < ny * T; i += kNT) {
+    const int t = i % T, y = i / T;
+    const int kz = kz0 + t;
+    if (kz < Hz) {
+      const int64_t o = ((int64_t)x * ny + y) * Hz + kz;
+      Ax[o] = s[y * S + t];
+      Ay[o] = s[y * S + T + t];
+      Ap[o] = s[y * S + 2 * T + t];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Integrated Green's function (ocelot/cpbd/sc.py:109-133), FP64.
+// K[i,j,k] = 8-corner alternating difference of F at the cell faces
+// x_i = i*hx - hx/2 (i = 0..N).  A workgroup computes a 4x8x8 tile of cells
+// from a 5x9x9 tile of F values held in LDS.
+__device__ __forceinline__ double igf_F(double x, double y, double z) {
+  const double r = __builtin_sqrt(x * x + y * y + z * z);
+  return -x * x * 0.5 * atan(y * z / (x * r)) + y * z * log(x + r) - y * y * 0.5 * atan(z * x / (y * r)) +
+         z * x * log(y + r) - z * z * 0.5 * atan(x * y / (z * r)) + x * y * log(z + r);
+}
+
+constexpr int kIgfTx = 4, kIgfTy = 8, kIgfTz = 8;
+
+__global__ void __launch_bounds__(256) k_igf(double* __restrict__ K, MeshDims d, const KickScalars* __restrict__ sc,
+                                             int force) {
+  __shared__ double F[(kIgfTx + 1) * (kIgfTy + 1) * (kIgfTz + 1)];
+  if (sc->err || !(force || sc->rebuild)) return;
+  const double hx = sc->steps[0], hy = sc->steps[1], hz = sc->steps[2];
+  const int bz = (d.nz + kIgfTz - 1) / kIgfTz, by = (d.ny + kIgfTy - 1) / kIgfTy;
+  const int b = blockIdx.x;
+  const int i0 = (b / (bz * by)) * kIgfTx, j0 = ((b / bz) % by) * kIgfTy, k0 = (b % bz) * kIgfTz;
+  constexpr int FX = kIgfTx + 1, FY = kIgfTy + 1, FZ = kIgfTz + 1;
+  for (int f = threadIdx.x; f < FX * FY * FZ; f += 256) {
+    const int a = f / (FY * FZ), bb = (f / FZ) % FY, c = f % FZ;
+    const int i = i0 + a, j = j0 + bb, k = k0 + c;
+    double v = 0;
+    if (i <= d.nx && j <= d.ny && k <= d.nz) v = igf_F(hx * i - hx / 2, hy * j - hy / 2, hz * k - hz / 2);
+    F[f] = v;
+  }
+  __syncthreads();
+  const int a = threadIdx.x / (kIgfTy * kIgfTz), bb = (threadIdx.x / kIgfTz) % kIgfTy, c = threadIdx.x % kIgfTz;
+  const int i = i0 + a, j = j0 + bb, k = k0 + c;
+  if (i >= d.nx || j >= d.ny || k >= d.nz) return;
+#define FAT(p, q, r) F[((a + (p)) * FY + (bb + (q))) * FZ + (c + (r))]
+  const double kv = FAT(1, 1, 1) - FAT(0, 1, 1) - FAT(1, 0, 1) + FAT(0, 0, 1) - FAT(1, 1, 0) + FAT(0, 1, 0) +
+                    FAT(1, 0, 0) - FAT(0, 0, 0);
+#undef FAT
+  K[((int64_t)i * d.ny + j) * d.nz + k] = kv;
+}
+
+// Even transform along z of the real kernel, two rows per complex FFT:
+// e[m] = K[m] (m < nz), e[M-m] = K[m] (1 <= m < nz); both spectra are real,
+// so row a is the real part and row b the imaginary part of the output.
+template <int M>
+__global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, double* __restrict__ out, MeshDims d,
+                                              const KickScalars* __restrict__ sc,
+                                              const cpx<double>* __restrict__ tw, int force) {
+  using F = double;
+  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  if (sc->err || !(force || sc->rebuild)) return;
+  const int nz = d.nz, Hz = d.hz;
+  const int64_t R = (int64_t)d.nx * d.ny;
+  const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
+  for (int i = threadIdx.x; i < M * T; i += kNT) s[(i / T) * S + (i % T)] = cpx<F>{0, 0};
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2 * T * nz; i += kNT) {
+    const int rl = i / nz, m = i - rl * nz;
+    const int64_t row = row0 + rl;
+    const F v = row < R ? K[row * nz + m] : 0;
+    ((F*)&s[m * S + (rl >> 1)])[rl & 1] = v;
+    if (m > 0) ((F*)&s[(M - m) * S + (rl >> 1)])[rl & 1] = v;
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, -1>(s, tw);
+  for (int i = threadIdx.x; i < 2 * T * Hz; i += kNT) {
+    const int rl = i / Hz, k = i - rl * Hz;
+    const int64_t row = row0 + rl;
+    if (row >= R) continue;
+    const cpx<F> v = s[k * S + (rl >> 1)];
+    out[row * Hz + k] = (rl & 1) ? v.y : v.x;
+  }
+}
+
+// Even transform along a strided axis of a real array [outer][n][inner] ->
+// [outer][H][inner]; adjacent inner columns are paired into one complex FFT.
+// The last (x) pass scales by gscale and writes the FP32 or FP64 spectrum.
+template <int M, typename FO>
+__global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO* __restrict__ out, int n, int inner,
+                                              int H, const KickScalars* __restrict__ sc,
+                                              const cpx<double>* __restrict__ tw, int force, int last,
+                                              double inv_m3) {
+  using F = double;
+  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  if (sc->err || !(force || sc->rebuild)) return;
+  const int c0 = blockIdx.x * 2 * T;  // first real inner column of this tile
+  const int o = blockIdx.y;            // outer index
+  for (int i = threadIdx.x; i < M * T; i += kNT) s[(i / T) * S + (i % T)] = cpx<F>{0, 0};
+  __syncthreads();
+  for (int i = threadIdx.x; i < n * 2 * T; i += kNT) {
+    const int cl = i % (2 * T), m = i / (2 * T);
+    const int col = c0 + cl;
+    const F v = col < inner ? in[((int64_t)o * n + m) * inner + col] : 0;
+    ((F*)&s[m * S + (cl >> 1)])[cl & 1] = v;
+    if (m > 0) ((F*)&s[(M - m) * S + (cl >> 1)])[cl & 1] = v;
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, -1>(s, tw);
+  double g = 1.0;
+  if (last) {
+    // potential scale 1/(4 pi eps0 hx hy hz) (sc.py:167) and the 1/(Mx My Mz)
+    // of the unnormalised inverse transform
+    const double hx = sc->steps[0], hy = sc->steps[1], hz = sc->steps[2];
+    const double denom = 4 * kPi * kEps0 * hx * hy * hz;
+    g = inv_m3 / denom;
+  }
+  for (int i = threadIdx.x; i < H * 2 * T; i += kNT) {
+    const int cl = i % (2 * T), k = i / (2 * T);
+    const int col = c0 + cl;
+    if (col >= inner) continue;
+    const cpx<F> v = s[k * S + (cl >> 1)];
+    const double r = (cl & 1) ? v.y : v.x;
+    out[((int64_t)o * H + k) * inner + col] = (FO)(r * g);
+  }
+}
+
+}  // namespace ocsc

@@ -1,0 +1,826 @@
+// C ABI of the MI355X space-charge kick: contexts, workspaces, launch
+// sequences.  Declarations and contracts are in include/ocsc.h.
+#include "ocsc.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "common.hpp"
+#include "fft.hpp"
+#include "mesh.hpp"
+#include "particles.hpp"
+
+using namespace ocsc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                            \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) return fail(OCSC_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK(what)                                                                      \
+  do {                                                                                       \
+    hipError_t e_ = hipGetLastError();                                                       \
+    if (e_ != hipSuccess) return fail(OCSC_EHIP, std::string(what) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+int ensure(Buf& b, size_t bytes) {
+  if (b.cap >= bytes) return OCSC_OK;
+  if (b.p) {
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(OCSC_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  b.cap = bytes;
+  return OCSC_OK;
+}
+
+int next_pow2(int v) {
+  int m = 1;
+  while (m < v) m <<= 1;
+  return m;
+}
+
+// host-side inputs of one kick, passed to the device by kernel argument
+struct KickInputs {
+  double gamref, betaref, pref, gbref, cdT, mesh_scale, mesh_shift, gamma0;
+  double steps[3];
+  int32_t random_mesh, mode;  // mode 0: kick; 1: el_field (gamma0 given); 2: steps given
+  int32_t nmesh[3];
+  int64_t n;
+};
+
+__global__ void k_setup(KickScalars* __restrict__ s, KickInputs in) {
+  if (threadIdx.x != 0) return;
+  s->gamref = in.gamref;
+  s->betaref = in.betaref;
+  s->pref = in.pref;
+  s->gbref = in.gbref;
+  s->cdT = in.cdT;
+  s->mesh_scale = in.mesh_scale;
+  s->mesh_shift = in.mesh_shift;
+  s->random_mesh = in.random_mesh;
+  for (int k = 0; k < 3; ++k) s->nmesh[k] = in.nmesh[k];
+  s->n = in.n;
+  s->err_sticky |= s->err;
+  s->err = 0;
+  if (in.mode == 1) {
+    s->gamma0 = in.gamma0;
+    for (int k = 0; k < 9; ++k) s->T[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  }
+  if (in.mode == 2) {
+    for (int k = 0; k < 3; ++k) {
+      s->steps[k] = in.steps[k];
+      s->key_steps[k] = -1.0;  // forced builds never satisfy a later cache lookup
+    }
+    s->key_prec = -1;
+    s->inv_qscale = 1.0;
+  }
+}
+
+// x += L x', y += L y', tau += r56 delta (numpy.dot(R, X) accumulates with fma)
+__global__ void __launch_bounds__(256) k_drift(double* __restrict__ r, int64_t ld, int64_t n, double L, double r56) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  r[p] = __builtin_fma(L, r[ld + p], r[p]);
+  r[2 * ld + p] = __builtin_fma(L, r[3 * ld + p], r[2 * ld + p]);
+  r[4 * ld + p] = __builtin_fma(r56, r[5 * ld + p], r[4 * ld + p]);
+}
+
+}  // namespace
+
+// stages of one kick, in launch order (names in ocsc_stage_name)
+enum Stage : int {
+  kStFrame = 0,   // setup, pass A, frame
+  kStExtent,      // pass B, grid placement
+  kStDeposit,     // clear + deposit
+  kStKernel,      // IGF + kernel spectrum (device-skipped on a cache hit)
+  kStZfwd,
+  kStYfwd,
+  kStXconv,
+  kStYinv,
+  kStZinv,
+  kStKick,        // gather + kick + write-back
+  kNumStages
+};
+const char* const kStageNames[kNumStages] = {"frame", "extent", "deposit", "green", "fft_z", "fft_y",
+                                             "conv_x", "ifft_y", "ifft_z", "kick"};
+
+struct ocsc_ctx {
+  int device = 0;
+  KickScalars* sc = nullptr;       // device
+  KickScalars* sc_host = nullptr;  // pinned mirror
+  double* part = nullptr;          // per-block partial sums
+  cpx<float>* twf = nullptr;       // W_M^k for M at offset M (M = 8..1024)
+  cpx<double>* twd = nullptr;
+  hipStream_t cap_stream = nullptr;  // stream used for graph capture
+  Buf grid, A, B, phi, G;
+  // stage timing (ocsc_set_profiling): events between the stages of a kick
+  int profiling = 0;
+  // tuning options (ocsc_set_option)
+  int deposit_xcd = 0;  // 1: per-XCD grid copies with L2-local atomics
+  hipEvent_t ev[kNumStages + 1] = {};
+  double stage_ms[kNumStages] = {};
+  int64_t stage_calls = 0;
+};
+
+namespace {
+
+struct Geometry {
+  MeshDims d;
+  int64_t ncell;
+};
+
+int make_geometry(const int32_t* nmesh, Geometry& g) {
+  for (int k = 0; k < 3; ++k) {
+    if (nmesh[k] < 4) return fail(OCSC_EINVAL, "nmesh must be >= 4 on every axis (steps = extent/(n-3))");
+    if (nmesh[k] > 512) return fail(OCSC_EINVAL, "nmesh must be <= 512 on every axis");
+  }
+  MeshDims& d = g.d;
+  d.nx = nmesh[0];
+  d.ny = nmesh[1];
+  d.nz = nmesh[2];
+  d.mx = next_pow2(2 * d.nx - 1);
+  d.my = next_pow2(2 * d.ny - 1);
+  d.mz = next_pow2(2 * d.nz - 1);
+  d.hx = d.mx / 2 + 1;
+  d.hy = d.my / 2 + 1;
+  d.hz = d.mz / 2 + 1;
+  g.ncell = (int64_t)d.nx * d.ny * d.nz;
+  return OCSC_OK;
+}
+
+template <typename F>
+int ensure_mesh(ocsc_ctx* c, const Geometry& g, int nword) {
+  const MeshDims& d = g.d;
+  int rc;
+  if ((rc = ensure(c->grid, (size_t)(c->deposit_xcd ? kNumXcd : 1) * nword * g.ncell * 8))) return rc;
+  size_t a = (size_t)d.nx * d.ny * d.hz * sizeof(cpx<F>);
+  size_t b = (size_t)d.nx * d.my * d.hz * sizeof(cpx<F>);
+  // the kernel-spectrum build borrows A and B as FP64 scratch
+  a = std::max(a, (size_t)d.nx * d.ny * d.hz * 8);
+  b = std::max(b, std::max((size_t)g.ncell * 8, (size_t)d.nx * d.hy * d.hz * 8));
+  if constexpr (sizeof(F) == 4) {
+    // mixed form carries three fields through the inverse passes
+    a = std::max(a, 3 * (size_t)d.nx * d.ny * d.hz * sizeof(cpx<float>));
+    b = std::max(b, 2 * (size_t)d.nx * d.my * d.hz * sizeof(cpx<float>));
+  }
+  if ((rc = ensure(c->A, a))) return rc;
+  if ((rc = ensure(c->B, b))) return rc;
+  if ((rc = ensure(c->phi, (size_t)g.ncell * std::max(sizeof(double), 3 * sizeof(float))))) return rc;
+  if ((rc = ensure(c->G, (size_t)d.hx * d.hy * d.hz * sizeof(double)))) return rc;
+  return OCSC_OK;
+}
+
+template <typename F>
+const cpx<F>* tw_for(ocsc_ctx* c, int M) {
+  if constexpr (sizeof(F) == 4)
+    return c->twf + M;
+  else
+    return c->twd + M;
+}
+
+#define OCSC_CASE_M(V, ...) \
+  case V: {                 \
+    constexpr int MM = V;   \
+    __VA_ARGS__;            \
+  } break;
+#define OCSC_SWITCH_M(MVAR, ...)                                   \
+  switch (MVAR) {                                                  \
+    OCSC_CASE_M(8, __VA_ARGS__)                                    \
+    OCSC_CASE_M(16, __VA_ARGS__)                                   \
+    OCSC_CASE_M(32, __VA_ARGS__)                                   \
+    OCSC_CASE_M(64, __VA_ARGS__)                                   \
+    OCSC_CASE_M(128, __VA_ARGS__)                                  \
+    OCSC_CASE_M(256, __VA_ARGS__)                                  \
+    OCSC_CASE_M(512, __VA_ARGS__)                                  \
+    OCSC_CASE_M(1024, __VA_ARGS__)                                 \
+    default:                                                       \
+      return fail(OCSC_EINVAL, "unsupported FFT length");          \
+  }
+
+// kernel spectrum G from the current cell sizes (skipped on device when the
+// cache key matches, unless force)
+template <typename F>
+int launch_kernel_spectrum(ocsc_ctx* c, const Geometry& g, hipStream_t st, int force) {
+  const MeshDims& d = g.d;
+  double* K = (double*)c->B.p;
+  double* Kz = (double*)c->A.p;
+  double* Kzy = (double*)c->B.p;
+  const int nb_igf = ((d.nx + kIgfTx - 1) / kIgfTx) * ((d.ny + kIgfTy - 1) / kIgfTy) * ((d.nz + kIgfTz - 1) / kIgfTz);
+  k_igf<<<nb_igf, 256, 0, st>>>(K, d, c->sc, force);
+  LAUNCHCHK("k_igf");
+  const int64_t R = (int64_t)d.nx * d.ny;
+  OCSC_SWITCH_M(d.mz, {
+    using TL = Tile<double, MM>;
+    const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
+    k_zcos<MM><<<nb, kNT, TL::lds_bytes, st>>>(K, Kz, d, c->sc, c->twd + MM, force);
+  });
+  LAUNCHCHK("k_zcos");
+  OCSC_SWITCH_M(d.my, {
+    using TL = Tile<double, MM>;
+    dim3 grid((d.hz + 2 * TL::T - 1) / (2 * TL::T), d.nx);
+    k_scos<MM, double><<<grid, kNT, TL::lds_bytes, st>>>(Kz, Kzy, d.ny, d.hz, d.hy, c->sc, c->twd + MM, force, 0, 1.0);
+  });
+  LAUNCHCHK("k_scos(y)");
+  const double inv_m3 = 1.0 / ((double)d.mx * (double)d.my * (double)d.mz);
+  OCSC_SWITCH_M(d.mx, {
+    using TL = Tile<double, MM>;
+    const int inner = d.hy * d.hz;
+    dim3 grid((inner + 2 * TL::T - 1) / (2 * TL::T), 1);
+    k_scos<MM, F><<<grid, kNT, TL::lds_bytes, st>>>(Kzy, (F*)c->G.p, d.nx, inner, d.hx, c->sc, c->twd + MM, force, 1,
+                                                     inv_m3);
+  });
+  LAUNCHCHK("k_scos(x)");
+  return OCSC_OK;
+}
+
+// phi = G * rho on the doubled grid; rho read through charge_at<CIN>
+template <typename F, int CIN>
+int launch_convolution(ocsc_ctx* c, const Geometry& g, const void* rho, F* phi, hipStream_t st,
+                       hipEvent_t* ev = nullptr) {
+  // ev[0] is recorded by the caller before the z pass; ev[1..4] after z, y, x, y
+  const MeshDims& d = g.d;
+  cpx<F>* A = (cpx<F>*)c->A.p;
+  cpx<F>* B = (cpx<F>*)c->B.p;
+  const int64_t R = (int64_t)d.nx * d.ny;
+  OCSC_SWITCH_M(d.mz, {
+    using TL = Tile<F, MM>;
+    const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
+    k_zfwd<F, MM, CIN><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, c->sc, tw_for<F>(c, MM),
+                                                        c->deposit_xcd ? kNumXcd : 1);
+  });
+  LAUNCHCHK("k_zfwd");
+  if (ev) (void)hipEventRecord(ev[1], st);
+  OCSC_SWITCH_M(d.my, {
+    using TL = Tile<F, MM>;
+    dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
+    k_yfwd<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, c->sc, tw_for<F>(c, MM));
+  });
+  LAUNCHCHK("k_yfwd");
+  if (ev) (void)hipEventRecord(ev[2], st);
+  OCSC_SWITCH_M(d.mx, {
+    using TL = Tile<F, MM>;
+    dim3 grid((d.hz + TL::T - 1) / TL::T, d.my);
+    k_xconv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, (const F*)c->G.p, d, c->sc, tw_for<F>(c, MM));
+  });
+  LAUNCHCHK("k_xconv");
+  if (ev) (void)hipEventRecord(ev[3], st);
+  OCSC_SWITCH_M(d.my, {
+    using TL = Tile<F, MM>;
+    dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
+    k_yinv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, A, d, c->sc, tw_for<F>(c, MM));
+  });
+  LAUNCHCHK("k_yinv");
+  if (ev) (void)hipEventRecord(ev[4], st);
+  OCSC_SWITCH_M(d.mz, {
+    using TL = Tile<F, MM>;
+    const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
+    k_zinv<F, MM><<<nb, kNT, TL::lds_bytes, st>>>(A, phi, d, c->sc, tw_for<F>(c, MM));
+  });
+  LAUNCHCHK("k_zinv");
+  return OCSC_OK;
+}
+
+// Mixed form: the three field components on the mesh, E = -grad phi as the
+// forward differences of the reference (sc.py:195-200) taken in Fourier
+// space, written as three FP32 grids one after the other at E.
+int launch_fields(ocsc_ctx* c, const Geometry& g, const void* rho, float* E, hipStream_t st,
+                  hipEvent_t* ev = nullptr) {
+  const MeshDims& d = g.d;
+  const int64_t nyz = (int64_t)d.nx * d.ny * d.hz;
+  cpx<float>* A = (cpx<float>*)c->A.p;
+  cpx<float>* B = (cpx<float>*)c->B.p;
+  cpx<float>* B2 = B + (int64_t)d.nx * d.my * d.hz;
+  const int64_t R = (int64_t)d.nx * d.ny;
+  OCSC_SWITCH_M(d.mz, {
+    using TL = Tile<float, MM>;
+    const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
+    k_zfwd<float, MM, kChargeFix1><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, c->sc, c->twf + MM,
+                                                                    c->deposit_xcd ? kNumXcd : 1);
+  });
+  LAUNCHCHK("k_zfwd");
+  if (ev) (void)hipEventRecord(ev[1], st);
+  OCSC_SWITCH_M(d.my, {
+    using TL = Tile<float, MM>;
+    dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
+    k_yfwd<float, MM><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, c->sc, c->twf + MM);
+  });
+  LAUNCHCHK("k_yfwd");
+  if (ev) (void)hipEventRecord(ev[2], st);
+  OCSC_SWITCH_M(d.mx, {
+    using FT = FieldTile<MM>;
+    dim3 grid((d.hz + FT::T2 - 1) / FT::T2, d.my);
+    k_xconv_field<MM><<<grid, kNT, FT::lds2, st>>>(B, B, B2, (const float*)c->G.p, d, c->sc, c->twf + MM,
+                                                   c->twd + MM);
+  });
+  LAUNCHCHK("k_xconv_field");
+  if (ev) (void)hipEventRecord(ev[3], st);
+  OCSC_SWITCH_M(d.my, {
+    using FT = FieldTile<MM>;
+    dim3 grid((d.hz + FT::T3 - 1) / FT::T3, d.nx);
+    k_yinv_field<MM><<<grid, kNT, FT::lds3, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, c->sc, c->twf + MM,
+                                                  c->twd + MM);
+  });
+  LAUNCHCHK("k_yinv_field");
+  if (ev) (void)hipEventRecord(ev[4], st);
+  OCSC_SWITCH_M(d.mz, {
+    using TL = Tile<float, MM>;
+    const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
+    k_zinv<float, MM, false><<<nb, kNT, TL::lds_bytes, st>>>(A, E, d, c->sc, c->twf + MM);
+    k_zinv<float, MM, false><<<nb, kNT, TL::lds_bytes, st>>>(A + nyz, E + g.ncell, d, c->sc, c->twf + MM);
+    k_zinv<float, MM, true><<<nb, kNT, TL::lds_bytes, st>>>(A + 2 * nyz, E + 2 * g.ncell, d, c->sc, c->twf + MM,
+                                                             c->twd + MM);
+  });
+  LAUNCHCHK("k_zinv(fields)");
+  return OCSC_OK;
+}
+
+KickInputs kick_inputs(const int32_t* nmesh, int64_t n, double E_GeV, double zstep, int random_mesh,
+                       double mesh_scale, double mesh_shift) {
+  KickInputs in{};
+  // sc.py:214-216 and coord_transform.py:19-20, 61 (same libm pow as numpy)
+  in.gamref = E_GeV / kMeGeV;
+  in.betaref = std::sqrt(1 - std::pow(in.gamref, -2.0));
+  in.pref = kMeEV * std::sqrt(std::pow(in.gamref, 2.0) - 1);
+  in.gbref = in.gamref * in.betaref;
+  in.cdT = zstep / in.betaref;
+  in.random_mesh = random_mesh ? 1 : 0;
+  in.mesh_scale = random_mesh ? mesh_scale : 1.0;
+  in.mesh_shift = random_mesh ? mesh_shift : 0.0;
+  for (int k = 0; k < 3; ++k) in.nmesh[k] = nmesh[k];
+  in.n = n;
+  in.mode = 0;
+  return in;
+}
+
+int nred_blocks(int64_t n) {
+  int64_t b = (n + kPT - 1) / kPT;
+  return (int)(b < kMaxRedBlocks ? (b < 1 ? 1 : b) : kMaxRedBlocks);
+}
+
+// the launch sequence of one kick (no host synchronisation, graph-capturable)
+template <typename F, int NWORD>
+int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const double* q, const KickInputs& in,
+                 int prec, hipStream_t st) {
+  const int64_t n = in.n;
+  const int nblk = nred_blocks(n);
+  const int npart = (int)((n + kPT - 1) / kPT);
+  Rows R{r, ld};
+  const bool prof = c->profiling != 0;
+  auto mark = [&](int i) {
+    if (prof) (void)hipEventRecord(c->ev[i], st);
+  };
+  mark(0);
+  k_setup<<<1, 64, 0, st>>>(c->sc, in);
+  LAUNCHCHK("k_setup");
+  k_passA<<<nblk, kPT, 0, st>>>(R, c->sc, c->part);
+  k_reduceA<<<1, kPT, 0, st>>>(c->sc, c->part, nblk);
+  mark(1);
+  k_passB<<<nblk, kPT, 0, st>>>(R, q, c->sc, c->part);
+  k_reduceB<<<1, kPT, 0, st>>>(c->sc, c->part, nblk, prec);
+  LAUNCHCHK("frame/extent passes");
+  mark(2);
+  const int64_t nz64 = (c->deposit_xcd ? kNumXcd : 1) * NWORD * g.ncell;
+  k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, st>>>((unsigned long long*)c->grid.p, nz64);
+  if (c->deposit_xcd)
+    k_deposit<NWORD, true><<<npart, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
+  else
+    k_deposit<NWORD, false><<<npart, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
+  LAUNCHCHK("deposit");
+  mark(3);
+  int rc;
+  if ((rc = launch_kernel_spectrum<F>(c, g, st, 0))) return rc;
+  mark(4);
+  if constexpr (sizeof(F) == 4) {
+    if ((rc = launch_fields(c, g, c->grid.p, (float*)c->phi.p, st, prof ? c->ev + 4 : nullptr))) return rc;
+  } else {
+    if ((rc = launch_convolution<F, NWORD == 2 ? kChargeFix2 : kChargeFix1>(c, g, c->grid.p, (F*)c->phi.p, st,
+                                                                              prof ? c->ev + 4 : nullptr)))
+      return rc;
+  }
+  mark(9);
+  k_kick<F><<<npart, kPT, 0, st>>>(r, ld, (const F*)c->phi.p, c->sc);
+  LAUNCHCHK("k_kick");
+  mark(10);
+  if (prof) {
+    HIPCHK(hipEventSynchronize(c->ev[10]));
+    for (int i = 0; i < kNumStages; ++i) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+      c->stage_ms[i] += ms;
+    }
+    c->stage_calls += 1;
+  }
+  return OCSC_OK;
+}
+
+int enqueue_kick_prec(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const double* q, const KickInputs& in,
+                      int prec, hipStream_t st) {
+  if (prec == OCSC_FP64) return enqueue_kick<double, 2>(c, g, r, ld, q, in, prec, st);
+  return enqueue_kick<float, 1>(c, g, r, ld, q, in, prec, st);
+}
+
+int ensure_prec(ocsc_ctx* c, const Geometry& g, int prec) {
+  if (prec == OCSC_FP64) return ensure_mesh<double>(c, g, 2);
+  return ensure_mesh<float>(c, g, 1);
+}
+
+__global__ void k_clear_err(KickScalars* __restrict__ s) {
+  if (threadIdx.x == 0) {
+    s->err = 0;
+    s->err_sticky = 0;
+  }
+}
+
+// error bits of the kicks since the last report (current kick and earlier
+// deferred ones); reported bits are cleared on the device
+int read_errors(ocsc_ctx* c, hipStream_t st, int32_t* err) {
+  HIPCHK(hipMemcpyAsync(c->sc_host, c->sc, sizeof(KickScalars), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *err = c->sc_host->err | c->sc_host->err_sticky;
+  if (*err) {
+    k_clear_err<<<1, 64, 0, st>>>(c->sc);
+    LAUNCHCHK("k_clear_err");
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return OCSC_OK;
+}
+
+int value_error(int32_t err) {
+  std::string m;
+  if (err & kErrNonFinite) m += "non-finite particle coordinates or charges; ";
+  if (err & kErrZeroCharge) m += "total charge is zero (charge centroid undefined); ";
+  if (err & kErrSteps) m += "zero or non-finite bunch extent on some axis; ";
+  if (err & kErrCell) m += "particle outside the mesh; ";
+  return fail(OCSC_EVALUE, m.empty() ? std::string("invalid bunch") : m);
+}
+
+int check_common(ocsc_ctx* c, const void* r, const double* q, int64_t n, int64_t ld) {
+  if (!c) return fail(OCSC_EINVAL, "null context");
+  if (n < 0) return fail(OCSC_EINVAL, "negative particle count");
+  if (n == 0) return fail(OCSC_EVALUE, "empty bunch (zero-size array to reduction operation)");
+  if (!r || !q) return fail(OCSC_EINVAL, "null particle or charge pointer");
+  if (ld < n) return fail(OCSC_EINVAL, "row stride ld must be >= n");
+  return OCSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ocsc_last_error(void) { return g_err.c_str(); }
+
+int ocsc_create(ocsc_ctx** out, int device) {
+  if (!out) return fail(OCSC_EINVAL, "null output pointer");
+  *out = nullptr;
+  HIPCHK(hipSetDevice(device));
+  ocsc_ctx* c = new ocsc_ctx();
+  c->device = device;
+  auto cleanup = [&](int rc) {
+    ocsc_destroy(c);
+    return rc;
+  };
+  if (hipMalloc(&c->sc, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "scalars"));
+  if (hipHostMalloc(&c->sc_host, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "pinned"));
+  if (hipMalloc(&c->part, sizeof(double) * kMaxRedBlocks * kNPartB) != hipSuccess)
+    return cleanup(fail(OCSC_ENOMEM, "partials"));
+  if (hipMalloc(&c->twf, sizeof(cpx<float>) * 2048) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "twiddles"));
+  if (hipMalloc(&c->twd, sizeof(cpx<double>) * 2048) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "twiddles"));
+  // W_M^k = exp(-2 pi i k / M), from the first octant by symmetry so every
+  // entry is a correctly rounded double of the exact value to within 1 ulp
+  cpx<double> hd[2048];
+  cpx<float> hf[2048];
+  for (int M = 8; M <= 1024; M *= 2) {
+    for (int k = 0; k < M; ++k) {
+      const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)M;
+      const double cr = (double)cosl(a), si = (double)-sinl(a);
+      hd[M + k] = cpx<double>{cr, si};
+      hf[M + k] = cpx<float>{(float)cosl(a), (float)-sinl(a)};
+    }
+  }
+  hd[0] = hd[1] = hd[2] = hd[3] = hd[4] = hd[5] = hd[6] = hd[7] = cpx<double>{1, 0};
+  for (int k = 0; k < 8; ++k) hf[k] = cpx<float>{1, 0};
+  if (hipMemcpy(c->twd, hd, sizeof(hd), hipMemcpyHostToDevice) != hipSuccess) return cleanup(fail(OCSC_EHIP, "copy"));
+  if (hipMemcpy(c->twf, hf, sizeof(hf), hipMemcpyHostToDevice) != hipSuccess) return cleanup(fail(OCSC_EHIP, "copy"));
+  KickScalars init{};
+  init.key_prec = -1;
+  if (hipMemcpy(c->sc, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(OCSC_EHIP, "copy"));
+  if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(OCSC_EHIP, "stream"));
+  for (int i = 0; i <= kNumStages; ++i)
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) return cleanup(fail(OCSC_EHIP, "event"));
+  *out = c;
+  return OCSC_OK;
+}
+
+int ocsc_destroy(ocsc_ctx* c) {
+  if (!c) return OCSC_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  for (Buf* b : {&c->grid, &c->A, &c->B, &c->phi, &c->G})
+    if (b->p) (void)hipFree(b->p);
+  if (c->sc) (void)hipFree(c->sc);
+  if (c->sc_host) (void)hipHostFree(c->sc_host);
+  if (c->part) (void)hipFree(c->part);
+  if (c->twf) (void)hipFree(c->twf);
+  if (c->twd) (void)hipFree(c->twd);
+  if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
+  for (int i = 0; i <= kNumStages; ++i)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  delete c;
+  return OCSC_OK;
+}
+
+int ocsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3], double E_GeV,
+               double zstep, int random_mesh, double mesh_scale, double mesh_shift, int prec, int sync,
+               void* stream) {
+  if (c && zstep == 0) return OCSC_OK;  // sc.py:210-212
+  int rc;
+  if ((rc = check_common(c, r, q, n, ld))) return rc;
+  if (!nmesh) return fail(OCSC_EINVAL, "null nmesh");
+  if (prec != OCSC_FP64 && prec != OCSC_MIXED) return fail(OCSC_EINVAL, "precision must be OCSC_FP64 or OCSC_MIXED");
+  if (!(E_GeV > 0) || !std::isfinite(zstep)) return fail(OCSC_EINVAL, "E_GeV must be > 0 and zstep finite");
+  HIPCHK(hipSetDevice(c->device));
+  Geometry g;
+  if ((rc = make_geometry(nmesh, g))) return rc;
+  if ((rc = ensure_prec(c, g, prec))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, random_mesh, mesh_scale, mesh_shift);
+  if ((rc = enqueue_kick_prec(c, g, r, ld, q, in, prec, st))) return rc;
+  if (sync) {
+    int32_t err = 0;
+    if ((rc = read_errors(c, st, &err))) return rc;
+    if (err) return value_error(err);
+  }
+  return OCSC_OK;
+}
+
+int ocsc_check(ocsc_ctx* c, void* stream) {
+  if (!c) return fail(OCSC_EINVAL, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  int32_t err = 0;
+  int rc;
+  if ((rc = read_errors(c, (hipStream_t)stream, &err))) return rc;
+  if (err) return value_error(err);
+  return OCSC_OK;
+}
+
+int ocsc_el_field(ocsc_ctx* c, const double* xyz, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
+                  double gamma, int random_mesh, double mesh_scale, double mesh_shift, int prec, double* E,
+                  int64_t lde, int32_t* cells, void* stream) {
+  int rc;
+  if ((rc = check_common(c, xyz, q, n, ld))) return rc;
+  if (!nmesh || !E || lde < n) return fail(OCSC_EINVAL, "bad nmesh / Exyz / lde");
+  if (prec != OCSC_FP64 && prec != OCSC_MIXED) return fail(OCSC_EINVAL, "bad precision");
+  HIPCHK(hipSetDevice(c->device));
+  Geometry g;
+  if ((rc = make_geometry(nmesh, g))) return rc;
+  if ((rc = ensure_prec(c, g, prec))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  KickInputs in = kick_inputs(nmesh, n, 1.0, 1.0, random_mesh, mesh_scale, mesh_shift);
+  in.mode = 1;
+  struct XcdOff {  // el_field deposits into a single grid copy
+    ocsc_ctx* c;
+    int saved;
+    ~XcdOff() { c->deposit_xcd = saved; }
+  } xcd_off{c, c->deposit_xcd};
+  c->deposit_xcd = 0;
+  in.gamma0 = gamma;
+  const int nblk = nred_blocks(n);
+  const int npart = (int)((n + kPT - 1) / kPT);
+  Rows R{xyz, ld};
+  k_setup<<<1, 64, 0, st>>>(c->sc, in);
+  k_passB_xyz<<<nblk, kPT, 0, st>>>(R, q, c->sc, c->part);
+  k_reduceB<<<1, kPT, 0, st>>>(c->sc, c->part, nblk, prec);
+  const int nword = prec == OCSC_FP64 ? 2 : 1;
+  const int64_t nz64 = nword * g.ncell;
+  k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 4096), 256, 0, st>>>((unsigned long long*)c->grid.p, nz64);
+  LAUNCHCHK("el_field setup");
+  if (prec == OCSC_FP64) {
+    k_deposit_xyz<2><<<npart, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
+    if ((rc = launch_kernel_spectrum<double>(c, g, st, 0))) return rc;
+    if ((rc = launch_convolution<double, kChargeFix2>(c, g, c->grid.p, (double*)c->phi.p, st))) return rc;
+    k_gather_xyz<double><<<npart, kPT, 0, st>>>(R, (const double*)c->phi.p, c->sc, E, lde, cells);
+  } else {
+    k_deposit_xyz<1><<<npart, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
+    if ((rc = launch_kernel_spectrum<float>(c, g, st, 0))) return rc;
+    if ((rc = launch_fields(c, g, c->grid.p, (float*)c->phi.p, st))) return rc;
+    k_gather_xyz<float><<<npart, kPT, 0, st>>>(R, (const float*)c->phi.p, c->sc, E, lde, cells);
+  }
+  LAUNCHCHK("el_field");
+  int32_t err = 0;
+  if ((rc = read_errors(c, st, &err))) return rc;
+  if (err) return value_error(err);
+  return OCSC_OK;
+}
+
+int ocsc_sym_kernel(ocsc_ctx* c, const int32_t nmesh[3], const double h[3], double* K, void* stream) {
+  if (!c || !nmesh || !h || !K) return fail(OCSC_EINVAL, "null argument");
+  for (int k = 0; k < 3; ++k)
+    if (nmesh[k] < 1 || nmesh[k] > 1024 || !(h[k] > 0)) return fail(OCSC_EINVAL, "bad shape or cell size");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  KickInputs in{};
+  in.mode = 2;
+  for (int k = 0; k < 3; ++k) {
+    in.nmesh[k] = nmesh[k];
+    in.steps[k] = h[k];
+  }
+  k_setup<<<1, 64, 0, st>>>(c->sc, in);
+  MeshDims d{};
+  d.nx = nmesh[0];
+  d.ny = nmesh[1];
+  d.nz = nmesh[2];
+  const int nb = ((d.nx + kIgfTx - 1) / kIgfTx) * ((d.ny + kIgfTy - 1) / kIgfTy) * ((d.nz + kIgfTz - 1) / kIgfTz);
+  k_igf<<<nb, 256, 0, st>>>(K, d, c->sc, 1);
+  LAUNCHCHK("k_igf");
+  HIPCHK(hipStreamSynchronize(st));
+  return OCSC_OK;
+}
+
+int ocsc_potential(ocsc_ctx* c, const double* rho, const int32_t nmesh[3], const double h[3], void* phi, int prec,
+                   void* stream) {
+  if (!c || !rho || !nmesh || !h || !phi) return fail(OCSC_EINVAL, "null argument");
+  if (prec != OCSC_FP64 && prec != OCSC_MIXED) return fail(OCSC_EINVAL, "bad precision");
+  for (int k = 0; k < 3; ++k)
+    if (!(h[k] > 0)) return fail(OCSC_EINVAL, "cell sizes must be > 0");
+  int rc;
+  HIPCHK(hipSetDevice(c->device));
+  Geometry g;
+  if ((rc = make_geometry(nmesh, g))) return rc;
+  if ((rc = ensure_prec(c, g, prec))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  KickInputs in{};
+  in.mode = 2;
+  for (int k = 0; k < 3; ++k) {
+    in.nmesh[k] = nmesh[k];
+    in.steps[k] = h[k];
+  }
+  k_setup<<<1, 64, 0, st>>>(c->sc, in);
+  if (prec == OCSC_FP64) {
+    if ((rc = launch_kernel_spectrum<double>(c, g, st, 1))) return rc;
+    if ((rc = launch_convolution<double, kChargeReal>(c, g, rho, (double*)phi, st))) return rc;
+  } else {
+    if ((rc = launch_kernel_spectrum<float>(c, g, st, 1))) return rc;
+    if ((rc = launch_convolution<float, kChargeReal>(c, g, rho, (float*)phi, st))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  return OCSC_OK;
+}
+
+int ocsc_drift(ocsc_ctx* c, double* r, int64_t ld, int64_t n, double L, double E_GeV, void* stream) {
+  if (!c || (!r && n > 0)) return fail(OCSC_EINVAL, "null argument");
+  if (n < 0 || ld < n) return fail(OCSC_EINVAL, "bad n / ld");
+  if (n == 0) return OCSC_OK;
+  HIPCHK(hipSetDevice(c->device));
+  // r_matrix.py:54-85 with k1 = hx = 0
+  const double gamma = E_GeV / kMeGeV;
+  const double igamma2 = gamma != 0 ? 1. / (gamma * gamma) : 0.;
+  const double beta = std::sqrt(1. - igamma2);
+  const double r56 = 0.0 - L / (beta * beta) * igamma2;
+  k_drift<<<(int)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(r, ld, n, L, r56);
+  LAUNCHCHK("k_drift");
+  return OCSC_OK;
+}
+
+int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
+                     double E_GeV, double zstep, double L, int nsteps, int prec, int use_graph, void* stream) {
+  int rc;
+  if ((rc = check_common(c, r, q, n, ld))) return rc;
+  if (!nmesh || nsteps < 0) return fail(OCSC_EINVAL, "bad nmesh / nsteps");
+  if (prec != OCSC_FP64 && prec != OCSC_MIXED) return fail(OCSC_EINVAL, "bad precision");
+  if (!(E_GeV > 0)) return fail(OCSC_EINVAL, "E_GeV must be > 0");
+  HIPCHK(hipSetDevice(c->device));
+  Geometry g;
+  if ((rc = make_geometry(nmesh, g))) return rc;
+  if ((rc = ensure_prec(c, g, prec))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, 0, 1.0, 0.0);
+  const double gamma = E_GeV / kMeGeV;
+  const double igamma2 = 1. / (gamma * gamma);
+  const double beta = std::sqrt(1. - igamma2);
+  const double r56 = 0.0 - L / (beta * beta) * igamma2;
+  const int nb = (int)((n + 255) / 256);
+  if (!use_graph) {
+    for (int s = 0; s < nsteps; ++s) {
+      k_drift<<<nb, 256, 0, st>>>(r, ld, n, L, r56);
+      if (zstep != 0 && (rc = enqueue_kick_prec(c, g, r, ld, q, in, prec, st))) return rc;
+    }
+  } else if (nsteps > 0) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const int prof = c->profiling;  // no event synchronisation inside a capture
+    c->profiling = 0;
+    HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+    k_drift<<<nb, 256, 0, c->cap_stream>>>(r, ld, n, L, r56);
+    int rc2 = zstep != 0 ? enqueue_kick_prec(c, g, r, ld, q, in, prec, c->cap_stream) : OCSC_OK;
+    hipError_t e = hipStreamEndCapture(c->cap_stream, &graph);
+    c->profiling = prof;
+    if (rc2) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc2;
+    }
+    if (e != hipSuccess) return fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(OCSC_EHIP, std::string("instantiate: ") + hipGetErrorString(e));
+    }
+    for (int s = 0; s < nsteps && e == hipSuccess; ++s) e = hipGraphLaunch(exec, st);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(OCSC_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
+  }
+  int32_t err = 0;
+  if ((rc = read_errors(c, st, &err))) return rc;
+  if (err) return value_error(err);
+  return OCSC_OK;
+}
+
+int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
+  if (!c || !name) return fail(OCSC_EINVAL, "null argument");
+  const std::string n(name);
+  if (n == "deposit_xcd") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "deposit_xcd must be 0 or 1");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipDeviceSynchronize());
+    c->deposit_xcd = (int)value;
+    return OCSC_OK;
+  }
+  return fail(OCSC_EINVAL, "unknown option " + n);
+}
+
+int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
+  if (!c || !name || !value) return fail(OCSC_EINVAL, "null argument");
+  const std::string n(name);
+  if (n == "deposit_xcd") {
+    *value = c->deposit_xcd;
+    return OCSC_OK;
+  }
+  return fail(OCSC_EINVAL, "unknown option " + n);
+}
+
+int ocsc_set_profiling(ocsc_ctx* c, int on) {
+  if (!c) return fail(OCSC_EINVAL, "null context");
+  c->profiling = on ? 1 : 0;
+  for (int i = 0; i < kNumStages; ++i) c->stage_ms[i] = 0;
+  c->stage_calls = 0;
+  return OCSC_OK;
+}
+
+int ocsc_stage_times(ocsc_ctx* c, double* ms, int nout, int64_t* calls) {
+  if (!c || (!ms && nout > 0)) return fail(OCSC_EINVAL, "null argument");
+  for (int i = 0; i < nout && i < kNumStages; ++i)
+    ms[i] = c->stage_calls ? c->stage_ms[i] / (double)c->stage_calls : 0.0;
+  if (calls) *calls = c->stage_calls;
+  return kNumStages;
+}
+
+const char* ocsc_stage_name(int i) { return (i >= 0 && i < kNumStages) ? kStageNames[i] : ""; }
+
+int ocsc_last_stats(ocsc_ctx* c, double* out, int nout, void* stream) {
+  if (!c || (!out && nout > 0)) return fail(OCSC_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(c->sc_host, c->sc, sizeof(KickScalars), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const KickScalars& s = *c->sc_host;
+  double v[20];
+  v[0] = s.gamma0;
+  v[1] = s.Pav;
+  for (int k = 0; k < 9; ++k) v[2 + k] = s.T[k];
+  for (int k = 0; k < 3; ++k) {
+    v[11 + k] = s.steps[k];
+    v[14 + k] = s.X_off[k];
+  }
+  v[17] = s.sumq;
+  v[18] = (double)(s.err | s.err_sticky);
+  v[19] = (double)s.rebuild;
+  for (int k = 0; k < nout && k < 20; ++k) out[k] = v[k];
+  return OCSC_OK;
+}
+
+}  // extern "C"

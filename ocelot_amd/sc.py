@@ -1,0 +1,273 @@
+"""Drop-in ``SpaceCharge`` physics process on the MI355X kernels.
+
+Same surface as the reference class (ocelot/cpbd/sc.py:76-258): constructor
+``SpaceCharge(step=1, nmesh_xyz=[63,63,63], low_order_kick=True,
+random_mesh=False)``, attributes mutable after construction, ``prepare(lat)``
+seeding the global numpy RNG with ``random_seed`` (10), ``apply(p_array,
+zstep)`` rewriting all six rows of ``p_array.rparticles`` in place, and the
+helper methods ``sym_kernel``, ``potential``, ``el_field``.  Two extra
+keyword arguments default to the reference behaviour: ``precision`` ("fp64"
+or "mixed") and ``device`` (GPU index; default: the current torch device or
+the device of a resident bunch).
+
+``p_array.rparticles`` may be a numpy array (it is copied to the GPU, kicked
+and copied back into the same ndarray, so views such as ``p_array.x()`` stay
+valid) or a CUDA float64 tensor (kicked in place in HBM, no host copy).
+
+Random mesh: with ``random_mesh`` on, each kick draws from the global numpy
+RNG exactly as the reference does, first U(1, 1.1) (extent factor) and then
+U(-0.5, 0.5) (one offset for all three axes); draws happen on the host and
+are passed to the kernels.
+
+Errors: the reference raises ValueError for an empty bunch, zero total
+charge and non-finite coordinates (numpy reductions / bincount); so does this
+class, leaving rparticles untouched.  ``nmesh`` below 4 on an axis (where the
+reference divides by n-3 <= 0) also raises ValueError here.
+
+Instances hold only plain data: device contexts live in a per-process
+registry, so deepcopy and pickle (Navigator, ParameterScanner) work.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .physproc import PhysProc
+
+_PREC = {"fp64": _lib.OCSC_FP64, "mixed": _lib.OCSC_MIXED}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_cuda_tensor(a):
+    try:
+        torch = _torch()
+    except ImportError:
+        return False
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+class SpaceCharge(PhysProc):
+    """3D space-charge kick (Poisson solve in the bunch rest frame by Hockney
+    FFT convolution with the integrated Green's function)."""
+
+    def __init__(self, step=1, **kwargs):
+        PhysProc.__init__(self)
+        self.step = step
+        self.nmesh_xyz = kwargs.get("nmesh_xyz", [63, 63, 63])
+        self.low_order_kick = kwargs.get("low_order_kick", True)
+        self.start_elem = None
+        self.end_elem = None
+        self.debug = False
+        self.random_mesh = kwargs.get("random_mesh", False)
+        self.random_seed = 10
+        self.precision = kwargs.get("precision", "fp64")
+        self.device = kwargs.get("device", None)
+
+    # ------------------------------------------------------------------ helpers
+    def _prec(self):
+        try:
+            return _PREC[self.precision]
+        except KeyError:
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {self.precision!r}")
+
+    def _device(self, tensor=None):
+        if tensor is not None:
+            return tensor.device.index
+        if self.device is not None:
+            return int(self.device)
+        torch = _torch()
+        return torch.cuda.current_device()
+
+    def _nmesh(self):
+        nm = np.array(self.nmesh_xyz).reshape(-1)
+        if nm.size != 3:
+            raise ValueError("nmesh_xyz must have three entries")
+        return _lib.i32x3(nm)
+
+    def _draws(self):
+        # sc.py:174-175 then sc.py:184-185, global numpy RNG
+        if not self.random_mesh:
+            return 0, 1.0, 0.0
+        scale = np.random.uniform(low=1, high=1.1)
+        shift = np.random.uniform(low=-0.5, high=0.5)
+        return 1, float(scale), float(shift)
+
+    # ------------------------------------------------------------------ PhysProc
+    def prepare(self, lat):
+        self.check_step()
+        if self.random_seed is not None:
+            np.random.seed(self.random_seed)
+
+    def kick_args(self, p_array, zstep):
+        """Host-side arguments of one kick, in the order the reference
+        consumes them: mesh, precision, energy, then the random-mesh draws.
+        Raises ValueError for an empty bunch before drawing (the reference
+        fails in the extent reduction, sc.py:173, before its first draw)."""
+        n = int(p_array.rparticles.shape[1])
+        if n == 0:
+            raise ValueError("zero-size array to reduction operation maximum which has no identity")
+        nmesh = self._nmesh()
+        prec = self._prec()
+        E = float(p_array.E)
+        rm, scale, shift = self._draws()
+        return dict(n=n, nmesh=nmesh, prec=prec, E=E, zstep=float(zstep), random_mesh=rm, mesh_scale=scale,
+                    mesh_shift=shift)
+
+    def apply(self, p_array, zstep):
+        if zstep == 0:
+            return
+        a = self.kick_args(p_array, zstep)
+        torch = _torch()
+        lib = _lib.load()
+        r = p_array.rparticles
+        n, nmesh, prec, E = a["n"], a["nmesh"], a["prec"], a["E"]
+        rm, scale, shift = a["random_mesh"], a["mesh_scale"], a["mesh_shift"]
+        if _is_cuda_tensor(r):
+            dev = self._device(r)
+            if r.dtype != torch.float64 or r.stride(1) != 1 or r.shape[0] != 6:
+                raise TypeError("device rparticles must be a (6, n) float64 tensor with contiguous rows")
+            q = p_array.q_array
+            if not _is_cuda_tensor(q):
+                q = torch.as_tensor(np.asarray(q, dtype=np.float64), device=r.device)
+            q = q.to(dtype=torch.float64).contiguous()
+            ctx = _lib.context(dev)
+            with torch.cuda.device(dev):
+                stream = torch.cuda.current_stream(dev).cuda_stream
+                _lib.check(lib.ocsc_apply(ctx.handle, r.data_ptr(), r.stride(0), q.data_ptr(), n, nmesh, E,
+                                          float(zstep), rm, scale, shift, prec, 1, stream))
+            return
+        dev = self._device()
+        ctx = _lib.context(dev)
+        with torch.cuda.device(dev):
+            rt = torch.from_numpy(np.ascontiguousarray(r, dtype=np.float64)).to(f"cuda:{dev}")
+            qt = torch.from_numpy(np.ascontiguousarray(p_array.q_array, dtype=np.float64)).to(f"cuda:{dev}")
+            stream = torch.cuda.current_stream(dev).cuda_stream
+            _lib.check(lib.ocsc_apply(ctx.handle, rt.data_ptr(), n, qt.data_ptr(), n, nmesh, E, float(zstep),
+                                      rm, scale, shift, prec, 1, stream))
+            r[...] = rt.cpu().numpy()
+
+    # ------------------------------------------------------------------ sub-steps
+    def sym_kernel(self, ijk2, hxyz):
+        """Integrated Green's function on shape ijk2 with cell sizes hxyz
+        (sc.py:109-133), FP64 numpy array."""
+        torch = _torch()
+        lib = _lib.load()
+        dev = self._device()
+        shape = tuple(int(v) for v in ijk2)
+        out = torch.empty(shape, dtype=torch.float64, device=f"cuda:{dev}")
+        ctx = _lib.context(dev)
+        with torch.cuda.device(dev):
+            stream = torch.cuda.current_stream(dev).cuda_stream
+            h = (ctypes.c_double * 3)(*[float(v) for v in hxyz])
+            _lib.check(lib.ocsc_sym_kernel(ctx.handle, _lib.i32x3(shape), h, out.data_ptr(), stream))
+        return out.cpu().numpy()
+
+    def potential(self, q, steps):
+        """Potential of the charge grid q (sc.py:135-168), FP64 numpy array
+        (FP32 values widened in mixed precision)."""
+        torch = _torch()
+        lib = _lib.load()
+        dev = self._device()
+        prec = self._prec()
+        rho = torch.from_numpy(np.ascontiguousarray(q, dtype=np.float64)).to(f"cuda:{dev}")
+        dt = torch.float64 if prec == _lib.OCSC_FP64 else torch.float32
+        phi = torch.empty(rho.shape, dtype=dt, device=rho.device)
+        ctx = _lib.context(dev)
+        with torch.cuda.device(dev):
+            stream = torch.cuda.current_stream(dev).cuda_stream
+            h = (ctypes.c_double * 3)(*[float(v) for v in steps])
+            _lib.check(lib.ocsc_potential(ctx.handle, rho.data_ptr(), _lib.i32x3(rho.shape), h, phi.data_ptr(),
+                                          prec, stream))
+        return phi.double().cpu().numpy()
+
+    def el_field(self, X, Q, gamma, nxyz, return_cells=False):
+        """Lab-frame field at rest-frame positions X (N, 3) (sc.py:170-205).
+        Like the reference, scales X[:, 2] by gamma in place and takes the
+        random-mesh draws when random_mesh is on."""
+        torch = _torch()
+        lib = _lib.load()
+        dev = self._device()
+        prec = self._prec()
+        n = X.shape[0]
+        if n == 0:
+            raise ValueError("zero-size array to reduction operation maximum which has no identity")
+        xyz = torch.from_numpy(np.ascontiguousarray(np.asarray(X, dtype=np.float64).T)).to(f"cuda:{dev}")
+        qt = torch.from_numpy(np.ascontiguousarray(Q, dtype=np.float64)).to(f"cuda:{dev}")
+        E = torch.empty((3, n), dtype=torch.float64, device=xyz.device)
+        cells = torch.empty(n, dtype=torch.int32, device=xyz.device) if return_cells else None
+        rm, scale, shift = self._draws()
+        nm = _lib.i32x3(np.array(nxyz).reshape(-1))
+        ctx = _lib.context(dev)
+        with torch.cuda.device(dev):
+            stream = torch.cuda.current_stream(dev).cuda_stream
+            _lib.check(lib.ocsc_el_field(ctx.handle, xyz.data_ptr(), n, qt.data_ptr(), n, nm, float(gamma), rm,
+                                         scale, shift, prec, E.data_ptr(), n,
+                                         cells.data_ptr() if cells is not None else None, stream))
+        X[:, 2] = X[:, 2] * gamma
+        out = np.ascontiguousarray(E.cpu().numpy().T)
+        if return_cells:
+            return out, cells.cpu().numpy()
+        return out
+
+    def last_stats(self, device=None):
+        """Scalars of the last kick on the device: gamma0, Pav, T, steps, X_off,
+        total charge, error bits, kernel-rebuild flag."""
+        lib = _lib.load()
+        dev = self._device() if device is None else int(device)
+        out = (ctypes.c_double * 20)()
+        _lib.check(lib.ocsc_last_stats(_lib.context(dev).handle, out, 20, None))
+        v = list(out)
+        return dict(gamma0=v[0], Pav=v[1], T=np.array(v[2:11]).reshape(3, 3), steps=np.array(v[11:14]),
+                    X_off=np.array(v[14:17]), sumq=v[17], err=int(v[18]), rebuilt=int(v[19]))
+
+    def __repr__(self) -> str:
+        cname = type(self).__name__
+        step = self.step
+        nmesh_xyz = self.nmesh_xyz
+        random_mesh = self.random_mesh
+        return f"<{cname}: {step=}, {nmesh_xyz=}, {random_mesh=}>"
+
+
+def drift(p_array, L, device=None):
+    """First-order drift of length L on a bunch (numpy or CUDA rows), the map
+    of a Drift element (ocelot/cpbd/r_matrix.py:41-91, k1 = hx = 0)."""
+    torch = _torch()
+    lib = _lib.load()
+    r = p_array.rparticles
+    n = int(r.shape[1])
+    if _is_cuda_tensor(r):
+        dev = r.device.index
+        with torch.cuda.device(dev):
+            _lib.check(lib.ocsc_drift(_lib.context(dev).handle, r.data_ptr(), r.stride(0), n, float(L),
+                                      float(p_array.E), torch.cuda.current_stream(dev).cuda_stream))
+        return
+    dev = torch.cuda.current_device() if device is None else int(device)
+    with torch.cuda.device(dev):
+        rt = torch.from_numpy(np.ascontiguousarray(r, dtype=np.float64)).to(f"cuda:{dev}")
+        _lib.check(lib.ocsc_drift(_lib.context(dev).handle, rt.data_ptr(), n, n, float(L), float(p_array.E),
+                                  torch.cuda.current_stream(dev).cuda_stream))
+        r[...] = rt.cpu().numpy()
+
+
+def track_drift(p_array, sc, L, dz, nsteps, use_graph=True):
+    """nsteps x [drift(L); sc.apply(p_array, dz)] on a device-resident bunch,
+    the step loop of ocelot/cpbd/track.py:470-477 for a drift lattice, with
+    one step captured in a HIP graph and replayed."""
+    torch = _torch()
+    lib = _lib.load()
+    r = p_array.rparticles
+    if not _is_cuda_tensor(r):
+        raise TypeError("track_drift needs a DeviceParticleArray")
+    if sc.random_mesh:
+        raise ValueError("track_drift does not take random-mesh draws; call apply per step instead")
+    dev = r.device.index
+    n = int(r.shape[1])
+    with torch.cuda.device(dev):
+        _lib.check(lib.ocsc_track_drift(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
+                                        p_array.q_array.data_ptr(), n, sc._nmesh(), float(p_array.E), float(dz),
+                                        float(L), int(nsteps), sc._prec(), 1 if use_graph else 0,
+                                        torch.cuda.current_stream(dev).cuda_stream))

@@ -1,0 +1,344 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle and the
+reference's golden vectors.  Tolerances, stated per test:
+  FP64 form:  fields and potentials within 1e-10 of the largest value (the
+              FFT length and the transcendental functions differ from numpy's
+              in the last bits), kicked particles within 1e-12 per row and the
+              kick itself within 1e-9; NGP cell indices bit-exact.
+  mixed form: E-field / kick RMS relative error below 1e-5 against FP64.
+"""
+import numpy as np
+import pytest
+
+from oracle import sc_oracle as orc
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+APPLY_CASES = ["apply_gauss_2k_15", "apply_aniso_5k_9_13_17", "apply_gauss_10k_31",
+               "apply_lowE_2k_7", "apply_flattop_4k_23_23_47"]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+    from ocelot_amd import _lib
+    _lib.load()
+
+
+def rel_max(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+def rel_rms(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+
+
+EPS = np.finfo(np.float64).eps
+
+
+def kick_err(r, r0, rref, row):
+    """Error of the kick (out - in) on one row relative to the kick size, with
+    the row's own FP64 rounding floor added: delta = (gamma/gamref - 1)/betaref
+    cancels against 1, so its absolute round-off is ~eps, not eps*|delta|."""
+    d = (r[row] - r0[row]) - (rref[row] - r0[row])
+    scale = np.max(np.abs(rref[row] - r0[row]))
+    floor = 16 * EPS * (1.0 + np.max(np.abs(rref[row]))) if row == 5 else 16 * EPS * np.max(np.abs(rref[row]))
+    return float(np.max(np.abs(d)) / (scale + floor / 1e-9))
+
+
+def case(golden, name):
+    return {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith(name + "/")}
+
+
+def parray(r, q, E):
+    from ocelot_amd import ParticleArray
+    p = ParticleArray()
+    p.rparticles = np.array(r, dtype=np.float64, copy=True)
+    p.q_array = np.array(q, dtype=np.float64, copy=True)
+    p.E = float(E)
+    return p
+
+
+def rest_frame(r, E):
+    """Rest-frame positions and gamma0 of a bunch, as apply computes them."""
+    gamref = E / orc.M_E_GEV
+    xp = orc.xxstg_2_xp_mad(r, gamref)
+    T, Pav = orc.frame(xp)
+    xyz = np.dot(xp[0:3].T, T)
+    gamma0 = np.sqrt((Pav / orc.M_E_EV) ** 2 + 1)
+    return xyz, gamma0
+
+
+# ------------------------------------------------------------------ IGF and potential
+
+@pytest.mark.parametrize("i", [0, 1, 2])
+def test_sym_kernel_golden(golden, i):
+    from ocelot_amd import SpaceCharge
+    K = SpaceCharge().sym_kernel(golden[f"kern_{i}/shape"], golden[f"kern_{i}/h"])
+    Kg = golden[f"kern_{i}/K"]
+    err = float(np.max(np.abs(K - Kg) / np.abs(Kg)))
+    print(f"sym_kernel[{i}] max elementwise rel err {err:.3e}")
+    assert err <= 1e-9
+
+
+def test_sym_kernel_c1_shape():
+    """63^3 cells of aspect 1:1:2600 (the 200k-particle bunch): the 8-corner
+    difference cancels F values up to ~1e12 times larger than K, so both the
+    reference's and this kernel carry round-off of a few eps * sum|F| per
+    element; that sum is the tolerance."""
+    from ocelot_amd import SpaceCharge
+    h = (1.55429071e-05, 1.49400660e-05, 3.90539787e-02)
+    K = SpaceCharge().sym_kernel((63, 63, 63), h)
+    Ko = orc.sym_kernel((63, 63, 63), h)
+    ax = [h[d] * np.arange(64) - h[d] / 2 for d in range(3)]
+    Fa = np.abs(orc.igf_primitive(*np.ix_(*ax)))
+    corner_sum = sum(Fa[a:a + 63, b:b + 63, c:c + 63] for a in (0, 1) for b in (0, 1) for c in (0, 1))
+    ratio = float(np.max(np.abs(K - Ko) / (EPS * corner_sum)))
+    print(f"sym_kernel 63^3: rel-to-max {rel_max(K, Ko):.3e}, max |dK| / (eps sum|F|) {ratio:.2f}")
+    assert ratio <= 32
+    assert rel_max(K, Ko) <= 1e-9
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("mixed", 1e-5)])
+def test_potential_golden(golden, prec, tol):
+    from ocelot_amd import SpaceCharge
+    phi = SpaceCharge(precision=prec).potential(golden["pot_0/rho"], golden["pot_0/steps"])
+    err = rel_max(phi, golden["pot_0/phi"])
+    print(f"potential[{prec}] rel-to-max {err:.3e}")
+    assert err <= tol
+
+
+@pytest.mark.parametrize("shape", [(31, 31, 31), (17, 40, 9), (63, 63, 63), (4, 4, 4)])
+def test_potential_vs_oracle(shape):
+    from ocelot_amd import SpaceCharge
+    rng = np.random.default_rng(sum(shape))
+    q = rng.uniform(0, 1e-12, shape) * (rng.uniform(0, 1, shape) < 0.3)
+    h = (2e-5, 3e-5, 0.04)
+    ref = orc.potential(q, h)
+    e64 = rel_max(SpaceCharge().potential(q, h), ref)
+    e32 = rel_max(SpaceCharge(precision="mixed").potential(q, h), ref)
+    print(f"potential {shape}: fp64 {e64:.3e} mixed {e32:.3e}")
+    # FFT lengths differ (power of two here, 2N-1 in numpy): round-off of
+    # order eps * log2(M^3) * (sum |rho| max K) / max phi
+    assert e64 <= 1e-11
+    assert e32 <= 1e-5
+
+
+# ------------------------------------------------------------------ el_field
+
+@pytest.mark.parametrize("name", APPLY_CASES)
+def test_el_field_golden_and_cells(golden, name):
+    from ocelot_amd import SpaceCharge
+    g = case(golden, name)
+    xyz, gamma0 = rest_frame(g["r_in"], float(g["E"]))
+    dbg = {}
+    E_or = orc.el_field(xyz.copy(), g["q"], gamma0, g["nmesh"], debug=dbg)
+    sc = SpaceCharge(nmesh_xyz=list(g["nmesh"]))
+    E64, cells = sc.el_field(xyz.copy(), g["q"], gamma0, g["nmesh"], return_cells=True)
+    e_gold = rel_max(E64, g["Exyz"])
+    mism = int(np.sum(cells != dbg["inds"]))
+    Em = SpaceCharge(nmesh_xyz=list(g["nmesh"]), precision="mixed").el_field(xyz.copy(), g["q"], gamma0, g["nmesh"])
+    e_mix = rel_rms(Em, E64)
+    print(f"el_field {name}: fp64 vs golden {e_gold:.3e}, vs oracle {rel_max(E64, E_or):.3e}, "
+          f"cell mismatches {mism}, mixed rms {e_mix:.3e}")
+    assert mism == 0
+    assert e_gold <= 1e-10
+    assert e_mix <= 1e-5
+
+
+# ------------------------------------------------------------------ apply
+
+@pytest.mark.parametrize("name", APPLY_CASES)
+def test_apply_fp64_golden(golden, name):
+    from ocelot_amd import SpaceCharge
+    g = case(golden, name)
+    p = parray(g["r_in"], g["q"], g["E"])
+    sc = SpaceCharge(nmesh_xyz=list(g["nmesh"]))
+    sc.apply(p, float(g["zstep"]))
+    r, r0, rg = p.rparticles, g["r_in"], g["r_out"]
+    row_err = max(rel_max(r[i], rg[i]) for i in range(6))
+    kerr = max(kick_err(r, r0, rg, i) for i in (1, 3, 5))
+    st = sc.last_stats()
+    print(f"apply {name}: row err {row_err:.3e}, kick err {kerr:.3e}, steps rel "
+          f"{rel_max(st['steps'], g['steps']):.3e}")
+    assert rel_max(st["steps"], g["steps"]) <= 1e-15
+    assert abs(st["gamma0"] - g["gamma0"]) <= 1e-15 * g["gamma0"]
+    assert row_err <= 1e-12
+    assert kerr <= 1e-9
+
+
+@pytest.mark.parametrize("name", APPLY_CASES)
+def test_apply_mixed_vs_fp64(golden, name):
+    from ocelot_amd import SpaceCharge
+    g = case(golden, name)
+    r0 = g["r_in"]
+    pm = parray(r0, g["q"], g["E"])
+    SpaceCharge(nmesh_xyz=list(g["nmesh"]), precision="mixed").apply(pm, float(g["zstep"]))
+    rg = g["r_out"]
+    kick = max(rel_rms(pm.rparticles[i] - r0[i], rg[i] - r0[i]) for i in (1, 3, 5))
+    print(f"apply mixed {name}: kick rms rel {kick:.3e}")
+    assert kick <= 1e-5
+
+
+def test_random_mesh_two_kicks(golden):
+    from ocelot_amd import SpaceCharge
+    sc = SpaceCharge(nmesh_xyz=[15, 15, 15], random_mesh=True)
+    sc.prepare(None)
+    p = parray(golden["rmesh/r_in"], golden["rmesh/q"], 0.130)
+    for k in (1, 2):
+        sc.apply(p, 0.1)
+        st = sc.last_stats()
+        assert rel_max(st["steps"], golden[f"rmesh/steps{k}"]) <= 1e-15
+        err = max(rel_max(p.rparticles[i], golden[f"rmesh/r_out{k}"][i]) for i in range(6))
+        print(f"random mesh kick {k}: row err {err:.3e}")
+        assert err <= 1e-12
+
+
+def test_c1_full_size_golden(golden):
+    """200k particles on 63^3, the reference's CPU configuration."""
+    from ocelot_amd import SpaceCharge
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("gauss_200k_63")
+    r0 = r.copy()
+    p = parray(r, q, E)
+    SpaceCharge(nmesh_xyz=list(nmesh)).apply(p, dz)
+    ro = p.rparticles
+    sub = golden["big_c1/r_out_sub"]
+    err = max(rel_max(ro[i, ::97], sub[i]) for i in range(6))
+    kick = max(kick_err(ro[:, ::97], r0[:, ::97], sub, i) for i in (1, 3, 5))
+    print(f"C1 200k/63^3: row err {err:.3e} kick err {kick:.3e}")
+    assert err <= 1e-12 and kick <= 1e-9
+    for i in range(6):
+        assert abs(ro[i].sum() - golden["big_c1/row_sum"][i]) <= 1e-10 * np.abs(ro[i]).sum()
+
+
+def test_c2_2m_fp64_and_mixed_vs_oracle():
+    """2M particles on 63^3: FP64 GPU vs the oracle, mixed vs FP64."""
+    from ocelot_amd import SpaceCharge
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("gauss_2m_63")
+    ro = r.copy()
+    orc.apply(ro, q, E, dz, nmesh)
+    p = parray(r, q, E)
+    SpaceCharge(nmesh_xyz=list(nmesh)).apply(p, dz)
+    pm = parray(r, q, E)
+    SpaceCharge(nmesh_xyz=list(nmesh), precision="mixed").apply(pm, dz)
+    kick = max(kick_err(p.rparticles, r, ro, i) for i in (1, 3, 5))
+    kick_m = max(rel_rms(pm.rparticles[i] - r[i], ro[i] - r[i]) for i in (1, 3, 5))
+    row = max(rel_max(p.rparticles[i], ro[i]) for i in range(6))
+    print(f"C2 2M/63^3: fp64 row {row:.3e} kick {kick:.3e}; mixed kick rms {kick_m:.3e}")
+    assert row <= 1e-12 and kick <= 1e-9 and kick_m <= 1e-5
+
+
+# ------------------------------------------------------------------ properties
+
+def test_deterministic_and_device_resident_equal():
+    from ocelot_amd import DeviceParticleArray, SpaceCharge
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=300_000, seed=3)
+    for prec in ("fp64", "mixed"):
+        sc = SpaceCharge(nmesh_xyz=[31, 31, 63], precision=prec)
+        a = parray(r, q, E)
+        sc.apply(a, dz)
+        b = parray(r, q, E)
+        sc.apply(b, dz)
+        assert np.array_equal(a.rparticles, b.rparticles), prec
+        d = DeviceParticleArray.from_host(parray(r, q, E))
+        sc.apply(d, dz)
+        assert np.array_equal(d.rparticles.cpu().numpy(), a.rparticles), prec
+
+
+def test_kernel_cache_hit_gives_identical_result():
+    from ocelot_amd import SpaceCharge
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("gauss_200k_63", n=50_000)
+    sc = SpaceCharge(nmesh_xyz=[31, 31, 31])
+    a = parray(r, q, E)
+    sc.apply(a, dz)
+    assert sc.last_stats()["rebuilt"] == 1
+    b = parray(r, q, E)
+    sc.apply(b, dz)
+    assert sc.last_stats()["rebuilt"] == 0
+    assert np.array_equal(a.rparticles, b.rparticles)
+
+
+@pytest.mark.parametrize("what", ["zero_charge", "nan", "inf_charge"])
+def test_invalid_bunch_raises_value_error_and_leaves_particles(what):
+    from ocelot_amd import SpaceCharge
+    from ocelot_amd.workloads import gaussian_bunch
+    r = gaussian_bunch(1000, seed=5)
+    q = np.full(1000, 1e-13)
+    if what == "zero_charge":
+        q[:] = 0.0
+    elif what == "nan":
+        r[0, 17] = np.nan
+    else:
+        q[3] = np.inf
+    p = parray(r, q, 0.13)
+    before = p.rparticles.copy()
+    with pytest.raises(ValueError):
+        SpaceCharge(nmesh_xyz=[15, 15, 15]).apply(p, 0.1)
+    assert np.array_equal(p.rparticles, before, equal_nan=True)
+    # the context stays usable
+    p2 = parray(gaussian_bunch(1000, seed=6), np.full(1000, 1e-13), 0.13)
+    SpaceCharge(nmesh_xyz=[15, 15, 15]).apply(p2, 0.1)
+
+
+def test_small_mesh_rejected():
+    from ocelot_amd import SpaceCharge
+    from ocelot_amd.workloads import gaussian_bunch
+    p = parray(gaussian_bunch(100, seed=1), np.full(100, 1e-13), 0.13)
+    with pytest.raises(ValueError):
+        SpaceCharge(nmesh_xyz=[3, 15, 15]).apply(p, 0.1)
+
+
+def test_drift_matches_oracle():
+    from ocelot_amd import drift
+    from ocelot_amd.workloads import gaussian_bunch
+    r = gaussian_bunch(10000, seed=9)
+    p = parray(r, np.zeros(10000), 0.13)
+    drift(p, 0.1)
+    ro = orc.drift(r.copy(), 0.1, 0.13)
+    assert max(rel_max(p.rparticles[i], ro[i]) for i in range(6)) <= 2e-16
+
+
+def test_track_drift_graph_eager_and_oracle():
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=40_000, seed=4)
+    sc = SpaceCharge(nmesh_xyz=[15, 15, 31])
+    ro = r.copy()
+    for _ in range(3):
+        orc.drift(ro, 0.1, E)
+        orc.apply(ro, q, E, dz, (15, 15, 31))
+    outs = []
+    for graph in (False, True):
+        d = DeviceParticleArray.from_host(parray(r, q, E))
+        track_drift(d, sc, 0.1, dz, 3, use_graph=graph)
+        outs.append(d.rparticles.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    err = max(rel_max(outs[0][i], ro[i]) for i in range(6))
+    kick = max(kick_err(outs[0], r, ro, i) for i in (1, 3, 5))
+    print(f"track 3 steps: row err {err:.3e} kick err {kick:.3e}")
+    assert err <= 1e-11 and kick <= 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_per_xcd_deposit_is_bit_identical(prec):
+    """Per-XCD grid copies with L2-local atomics sum the same integers as the
+    single device-scope grid, so the kicked bunch must be identical."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("gauss_2m_63", n=1_000_000, seed=8)
+    dev = torch.cuda.current_device()
+    outs = []
+    try:
+        for mode in (0, 1):
+            _lib.set_option(dev, "deposit_xcd", mode)
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            SpaceCharge(nmesh_xyz=list(nmesh), precision=prec).apply(d, dz)
+            outs.append(d.rparticles.cpu().numpy())
+    finally:
+        _lib.set_option(dev, "deposit_xcd", 0)
+    assert np.array_equal(outs[0], outs[1])

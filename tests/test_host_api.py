@@ -1,0 +1,123 @@
+"""Host-side contract of the drop-in SpaceCharge (no GPU): constructor,
+attributes, repr, prepare() seeding, random-mesh draw order, deepcopy and
+pickle, the zstep == 0 no-op and the empty-bunch error.  Mirrors the
+reference class (ocelot/cpbd/sc.py:92-107, 208-212, 253-258)."""
+import copy
+import pickle
+
+import numpy as np
+import pytest
+
+from ocelot_amd import ParticleArray, PhysProc, SpaceCharge
+
+
+def test_constructor_defaults_match_reference():
+    sc = SpaceCharge()
+    assert isinstance(sc, PhysProc)
+    assert sc.step == 1
+    assert sc.nmesh_xyz == [63, 63, 63]
+    assert sc.low_order_kick is True
+    assert sc.random_mesh is False
+    assert sc.random_seed == 10
+    assert sc.start_elem is None and sc.end_elem is None and sc.debug is False
+    assert sc.indx0 is None and sc.z0 is None and sc.energy is None
+    assert sc.precision == "fp64"
+
+
+def test_kwargs_and_mutation():
+    sc = SpaceCharge(step=5, nmesh_xyz=[31, 47, 63], random_mesh=True, precision="mixed")
+    assert sc.step == 5 and sc.nmesh_xyz == [31, 47, 63] and sc.random_mesh
+    sc.nmesh_xyz = [15, 15, 15]
+    sc.step = 1
+    assert repr(sc) == "<SpaceCharge: step=1, nmesh_xyz=[15, 15, 15], random_mesh=True>"
+
+
+def test_repr_matches_reference_format():
+    assert repr(SpaceCharge()) == "<SpaceCharge: step=1, nmesh_xyz=[63, 63, 63], random_mesh=False>"
+
+
+def test_prepare_seeds_global_rng():
+    sc = SpaceCharge()
+    np.random.seed(123)
+    sc.prepare(None)
+    a = np.random.uniform()
+    np.random.seed(10)
+    assert a == np.random.uniform()
+
+
+def test_random_mesh_draw_order():
+    """Two draws per kick from the global RNG: U(1,1.1) then U(-0.5,0.5)."""
+    sc = SpaceCharge(random_mesh=True)
+    p = ParticleArray(5)
+    p.E = 0.13
+    sc.prepare(None)
+    a1 = sc.kick_args(p, 0.1)
+    a2 = sc.kick_args(p, 0.1)
+    rs = np.random.RandomState(10)
+    for a in (a1, a2):
+        assert a["random_mesh"] == 1
+        assert a["mesh_scale"] == rs.uniform(low=1, high=1.1)
+        assert a["mesh_shift"] == rs.uniform(low=-0.5, high=0.5)
+
+
+def test_no_draws_without_random_mesh():
+    sc = SpaceCharge()
+    p = ParticleArray(5)
+    np.random.seed(7)
+    a = sc.kick_args(p, 0.1)
+    assert a["random_mesh"] == 0 and a["mesh_scale"] == 1.0 and a["mesh_shift"] == 0.0
+    b = np.random.uniform()
+    np.random.seed(7)
+    assert b == np.random.uniform()
+
+
+def test_empty_bunch_raises_before_drawing():
+    sc = SpaceCharge(random_mesh=True)
+    np.random.seed(3)
+    with pytest.raises(ValueError):
+        sc.apply(ParticleArray(0), 0.1)
+    b = np.random.uniform()
+    np.random.seed(3)
+    assert b == np.random.uniform()
+
+
+def test_zero_step_is_a_no_op_without_gpu():
+    sc = SpaceCharge()
+    p = ParticleArray(10)
+    p.rparticles[:] = np.arange(60).reshape(6, 10)
+    before = p.rparticles.copy()
+    sc.apply(p, 0)
+    sc.apply(p, 0.0)
+    assert np.array_equal(p.rparticles, before)
+
+
+def test_deepcopy_and_pickle_hold_plain_data():
+    sc = SpaceCharge(step=5, nmesh_xyz=[31, 31, 31], precision="mixed")
+    sc.indx0 = 3
+    c = copy.deepcopy(sc)
+    assert repr(c) == repr(sc) and c.indx0 == 3 and c.precision == "mixed"
+    d = pickle.loads(pickle.dumps(sc))
+    assert repr(d) == repr(sc) and d.__dict__ == sc.__dict__
+
+
+def test_bad_precision_and_mesh_shape():
+    p = ParticleArray(4)
+    with pytest.raises(ValueError):
+        SpaceCharge(precision="fp16").kick_args(p, 0.1)
+    with pytest.raises(ValueError):
+        SpaceCharge(nmesh_xyz=[31, 31]).kick_args(p, 0.1)
+
+
+def test_check_step_contract():
+    sc = SpaceCharge(step=2)
+    sc.prepare(None)
+    sc.step = 2.0
+    sc.prepare(None)
+
+
+def test_particle_array_layout():
+    p = ParticleArray(7)
+    assert p.rparticles.shape == (6, 7) and p.rparticles.flags.c_contiguous
+    assert p.q_array.shape == (7,) and p.n == 7
+    p.rparticles[4] = 1.0
+    assert np.all(p.tau() == 1.0)
