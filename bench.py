@@ -32,14 +32,19 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="flattop_10m_127")
     ap.add_argument("--particles", type=int, default=None, help="override the particle count")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000,
                     help="particles in the CPU-baseline kick (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--sort-every", type=int, default=10,
+                    help="re-sort the resident bunch by mesh cell every k tracking steps (0: never)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run one elementwise kernel reading and writing 1 GiB each, the reference "
+                         "for scaling profiler byte counters")
     ap.add_argument("--deposit-xcd", default="auto", choices=["0", "1", "auto"],
                     help="charge deposit into one grid (0), per-XCD copies (1), or pick the faster after "
                          "checking both give identical kicks (auto)")
@@ -105,6 +110,12 @@ def main():
     from ocelot_amd import _lib
     from ocelot_amd.workloads import make_bunch
 
+    if a.copy_calib:
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.add(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
     r, q, E, nmesh, dz = make_bunch(a.workload, n=a.particles)
     n = r.shape[1]
     host = ParticleArray()
@@ -122,18 +133,19 @@ def main():
             _lib.set_option(dev, "deposit_xcd", mode)
             sc = SpaceCharge(nmesh_xyz=list(nmesh), precision="mixed")
             d = DeviceParticleArray.from_host(host, dev)
-            sc.apply(d, dz)
+            track_drift(d, sc, L, dz, 2, use_graph=True)
             res.append(d.rparticles.clone())
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            for _ in range(3):
-                sc.apply(d, dz)
+            track_drift(d, sc, L, dz, 5, use_graph=True)
             torch.cuda.synchronize()
-            times[mode] = (time.perf_counter() - t0) / 3
+            times[mode] = (time.perf_counter() - t0) / 5
         same = bool(torch.equal(res[0], res[1]))
         mode = 1 if (same and times[1] < times[0]) else 0
         return mode, {"identical": same, "ms_per_kick": {str(k): round(v * 1e3, 4) for k, v in times.items()}}
 
+    _lib.set_option(dev, "sort_every", a.sort_every)
+    L = dz
     deposit_mode, deposit_tune = pick_deposit()
     _lib.set_option(dev, "deposit_xcd", deposit_mode)
     print(f"# deposit_xcd={deposit_mode} {deposit_tune}", file=sys.stderr, flush=True)
@@ -173,7 +185,9 @@ def main():
     field_rms = max(errs)
     del outs, r0
 
-    # per-stage HIP events over a few eager steps (same kick, same stream)
+    # per-stage HIP events over a few eager tracking steps (same kicks as the
+    # timed region, same stream; profiling switches the graph off)
+    import ctypes
     lib = _lib.load()
     ctx = _lib.context(dev)
     stages = {}
@@ -181,11 +195,8 @@ def main():
     for prec in ("mixed", "fp64"):
         sc = SpaceCharge(nmesh_xyz=list(nmesh), precision=prec)
         d = DeviceParticleArray.from_host(host, dev)
-        sc.apply(d, dz)
         _lib.check(lib.ocsc_set_profiling(ctx.handle, 1))
-        for _ in range(a.profile_steps):
-            sc.apply(d, dz)
-        import ctypes
+        track_drift(d, sc, L, dz, a.profile_steps, use_graph=True)
         ms = (ctypes.c_double * 16)()
         calls = ctypes.c_int64()
         ns = lib.ocsc_stage_times(ctx.handle, ms, 16, ctypes.byref(calls))
@@ -218,7 +229,8 @@ def main():
         "dtype": "fp32 mesh/FFT/gather + fp64 frame/Green/momentum",
         "data": "synthetic (seeded flat-top bunch, numpy default_rng)",
         "config": {"workload": a.workload, "particles": n, "nmesh": list(nmesh), "kick_dz_m": dz,
-                   "drift_m": L, "precision": "mixed", "hip_graph": True, "deposit_xcd": deposit_mode},
+                   "drift_m": L, "precision": "mixed", "hip_graph": True, "deposit_xcd": deposit_mode,
+                   "sort_every": a.sort_every},
         "deposit_autotune": deposit_tune,
         "fp64": None if t_64 is None else {"value": round(n / t_64, 1), "ms_per_step": round(t_64 * 1e3, 4)},
         "mixed_over_fp64": None if t_64 is None else round(t_64 / t_mix, 3),

@@ -2,7 +2,8 @@
 
     python -m ocelot_amd.build          # or __graft_entry__.build()
 
-hipcc compiles one translation unit, ocelot_amd/csrc/ocsc.hip, with
+hipcc compiles the translation units (ocsc.hip: kernels and C ABI; sort.hip:
+the rocPRIM radix sort) with
 floating-point contraction off (the particle arithmetic reproduces numpy's
 rounding order; fused multiply-adds are written explicitly where wanted) and
 packed-FP32 instructions disabled.
@@ -14,7 +15,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "ocsc.hip")
+SRCS = [os.path.join(HERE, "csrc", "ocsc.hip"), os.path.join(HERE, "csrc", "sort.hip")]
+OBJDIR = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libocsc.so")
 ARCH = os.environ.get("OCSC_OFFLOAD_ARCH", "gfx950")
 
@@ -26,17 +28,35 @@ def hipcc():
     raise RuntimeError("hipcc not found (set HIPCC)")
 
 
-def command(out=OUT, extra=()):
-    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-result",
-            "-I", os.path.join(REPO, "include"), "-I", os.path.join(HERE, "csrc"),
-            *extra, "-o", out, SRC]
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 
 
-def build(verbose=True):
+def compile_cmd(src, obj, extra=()):
+    return [hipcc(), f"--offload-arch={ARCH}", *FLAGS, "-I", os.path.join(REPO, "include"),
+            "-I", os.path.join(HERE, "csrc"), *extra, "-c", "-o", obj, src]
+
+
+def link_cmd(objs, out):
+    return [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs]
+
+
+def build(verbose=True, extra=()):
+    """Compile every translation unit (in parallel) and link libocsc.so."""
+    os.makedirs(OBJDIR, exist_ok=True)
+    procs, objs = [], []
+    for src in SRCS:
+        obj = os.path.join(OBJDIR, os.path.splitext(os.path.basename(src))[0] + ".o")
+        cmd = compile_cmd(src, obj, extra)
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+        objs.append(obj)
+    rcs = [p.wait() for p in procs]
+    if any(rcs):
+        raise subprocess.CalledProcessError(max(rcs), "hipcc")
     tmp = OUT + ".tmp"
-    cmd = command(tmp)
+    cmd = link_cmd(objs, tmp)
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

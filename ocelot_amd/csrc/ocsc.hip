@@ -12,6 +12,12 @@
 #include "mesh.hpp"
 #include "particles.hpp"
 
+namespace ocsc {
+// sort.hip
+hipError_t sort_pairs_u32(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                          const uint32_t* vals_in, uint32_t* vals_out, int64_t n, int end_bit, hipStream_t st);
+}  // namespace ocsc
+
 using namespace ocsc;
 
 namespace {
@@ -131,7 +137,7 @@ struct ocsc_ctx {
   int device = 0;
   KickScalars* sc = nullptr;       // device
   KickScalars* sc_host = nullptr;  // pinned mirror
-  double* part = nullptr;          // per-block partial sums
+  Buf part;                        // per-block partial sums
   cpx<float>* twf = nullptr;       // W_M^k for M at offset M (M = 8..1024)
   cpx<double>* twd = nullptr;
   hipStream_t cap_stream = nullptr;  // stream used for graph capture
@@ -140,6 +146,9 @@ struct ocsc_ctx {
   int profiling = 0;
   // tuning options (ocsc_set_option)
   int deposit_xcd = 0;  // 1: per-XCD grid copies with L2-local atomics
+  int sort_every = 20;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
+  // tracking workspace: two cell-ordered copies of the bunch, sort buffers
+  Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp;
   hipEvent_t ev[kNumStages + 1] = {};
   double stage_ms[kNumStages] = {};
   int64_t stage_calls = 0;
@@ -381,9 +390,23 @@ int nred_blocks(int64_t n) {
 }
 
 // the launch sequence of one kick (no host synchronisation, graph-capturable)
+// options of one kick inside a tracking loop
+struct StepOpts {
+  bool passA = true;   // false: the previous kick left its momentum sums in the partials
+  int prev_nblk = 0;   // number of those partials
+  bool drift = false;  // apply the next step's drift in the write-back
+  bool sump = false;   // leave this kick's momentum sums for the next kick
+  DriftArgs dr{0, 0};
+};
+
+template <typename F, bool DRIFT, bool SUMP>
+void launch_kick(ocsc_ctx* c, int npart, double* r, int64_t ld, const StepOpts& o, hipStream_t st) {
+  k_kick<F, DRIFT, SUMP><<<npart, kPT, 0, st>>>(r, ld, (const F*)c->phi.p, c->sc, o.dr, (double*)c->part.p);
+}
+
 template <typename F, int NWORD>
 int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const double* q, const KickInputs& in,
-                 int prec, hipStream_t st) {
+                 int prec, hipStream_t st, const StepOpts& o = StepOpts()) {
   const int64_t n = in.n;
   const int nblk = nred_blocks(n);
   const int npart = (int)((n + kPT - 1) / kPT);
@@ -395,19 +418,24 @@ int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const do
   mark(0);
   k_setup<<<1, 64, 0, st>>>(c->sc, in);
   LAUNCHCHK("k_setup");
-  k_passA<<<nblk, kPT, 0, st>>>(R, c->sc, c->part);
-  k_reduceA<<<1, kPT, 0, st>>>(c->sc, c->part, nblk);
+  if (o.passA) {
+    k_passA<<<nblk, kPT, 0, st>>>(R, c->sc, (double*)c->part.p);
+    k_reduceA<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk);
+  } else {
+    k_reduceA<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, o.prev_nblk);
+  }
   mark(1);
-  k_passB<<<nblk, kPT, 0, st>>>(R, q, c->sc, c->part);
-  k_reduceB<<<1, kPT, 0, st>>>(c->sc, c->part, nblk, prec);
+  k_passB<<<nblk, kPT, 0, st>>>(R, q, c->sc, (double*)c->part.p);
+  k_reduceB<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk, prec);
   LAUNCHCHK("frame/extent passes");
   mark(2);
   const int64_t nz64 = (c->deposit_xcd ? kNumXcd : 1) * NWORD * g.ncell;
   k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, st>>>((unsigned long long*)c->grid.p, nz64);
+  const int ndep = (int)((n + kPT * kDepItems - 1) / (kPT * kDepItems));
   if (c->deposit_xcd)
-    k_deposit<NWORD, true><<<npart, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
+    k_deposit<NWORD, true><<<ndep, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
   else
-    k_deposit<NWORD, false><<<npart, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
+    k_deposit<NWORD, false><<<ndep, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
   LAUNCHCHK("deposit");
   mark(3);
   int rc;
@@ -421,7 +449,14 @@ int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const do
       return rc;
   }
   mark(9);
-  k_kick<F><<<npart, kPT, 0, st>>>(r, ld, (const F*)c->phi.p, c->sc);
+  if (o.drift && o.sump)
+    launch_kick<F, true, true>(c, npart, r, ld, o, st);
+  else if (o.drift)
+    launch_kick<F, true, false>(c, npart, r, ld, o, st);
+  else if (o.sump)
+    launch_kick<F, false, true>(c, npart, r, ld, o, st);
+  else
+    launch_kick<F, false, false>(c, npart, r, ld, o, st);
   LAUNCHCHK("k_kick");
   mark(10);
   if (prof) {
@@ -437,9 +472,14 @@ int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const do
 }
 
 int enqueue_kick_prec(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const double* q, const KickInputs& in,
-                      int prec, hipStream_t st) {
-  if (prec == OCSC_FP64) return enqueue_kick<double, 2>(c, g, r, ld, q, in, prec, st);
-  return enqueue_kick<float, 1>(c, g, r, ld, q, in, prec, st);
+                      int prec, hipStream_t st, const StepOpts& o = StepOpts()) {
+  if (prec == OCSC_FP64) return enqueue_kick<double, 2>(c, g, r, ld, q, in, prec, st, o);
+  return enqueue_kick<float, 1>(c, g, r, ld, q, in, prec, st, o);
+}
+
+int ensure_part(ocsc_ctx* c, int64_t n) {
+  const size_t need = std::max<size_t>((size_t)kMaxRedBlocks * kNPartB, (size_t)3 * ((n + kPT - 1) / kPT));
+  return ensure(c->part, need * sizeof(double));
 }
 
 int ensure_prec(ocsc_ctx* c, const Geometry& g, int prec) {
@@ -504,8 +544,6 @@ int ocsc_create(ocsc_ctx** out, int device) {
   };
   if (hipMalloc(&c->sc, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "scalars"));
   if (hipHostMalloc(&c->sc_host, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "pinned"));
-  if (hipMalloc(&c->part, sizeof(double) * kMaxRedBlocks * kNPartB) != hipSuccess)
-    return cleanup(fail(OCSC_ENOMEM, "partials"));
   if (hipMalloc(&c->twf, sizeof(cpx<float>) * 2048) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "twiddles"));
   if (hipMalloc(&c->twd, sizeof(cpx<double>) * 2048) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "twiddles"));
   // W_M^k = exp(-2 pi i k / M), from the first octant by symmetry so every
@@ -540,11 +578,11 @@ int ocsc_destroy(ocsc_ctx* c) {
   if (!c) return OCSC_OK;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
-  for (Buf* b : {&c->grid, &c->A, &c->B, &c->phi, &c->G})
+  for (Buf* b : {&c->part, &c->grid, &c->A, &c->B, &c->phi, &c->G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
+                 &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp})
     if (b->p) (void)hipFree(b->p);
   if (c->sc) (void)hipFree(c->sc);
   if (c->sc_host) (void)hipHostFree(c->sc_host);
-  if (c->part) (void)hipFree(c->part);
   if (c->twf) (void)hipFree(c->twf);
   if (c->twd) (void)hipFree(c->twd);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
@@ -567,6 +605,7 @@ int ocsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   Geometry g;
   if ((rc = make_geometry(nmesh, g))) return rc;
   if ((rc = ensure_prec(c, g, prec))) return rc;
+  if ((rc = ensure_part(c, n))) return rc;
   hipStream_t st = (hipStream_t)stream;
   KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, random_mesh, mesh_scale, mesh_shift);
   if ((rc = enqueue_kick_prec(c, g, r, ld, q, in, prec, st))) return rc;
@@ -599,6 +638,7 @@ int ocsc_el_field(ocsc_ctx* c, const double* xyz, int64_t ld, const double* q, i
   Geometry g;
   if ((rc = make_geometry(nmesh, g))) return rc;
   if ((rc = ensure_prec(c, g, prec))) return rc;
+  if ((rc = ensure_part(c, n))) return rc;
   hipStream_t st = (hipStream_t)stream;
   KickInputs in = kick_inputs(nmesh, n, 1.0, 1.0, random_mesh, mesh_scale, mesh_shift);
   in.mode = 1;
@@ -613,8 +653,8 @@ int ocsc_el_field(ocsc_ctx* c, const double* xyz, int64_t ld, const double* q, i
   const int npart = (int)((n + kPT - 1) / kPT);
   Rows R{xyz, ld};
   k_setup<<<1, 64, 0, st>>>(c->sc, in);
-  k_passB_xyz<<<nblk, kPT, 0, st>>>(R, q, c->sc, c->part);
-  k_reduceB<<<1, kPT, 0, st>>>(c->sc, c->part, nblk, prec);
+  k_passB_xyz<<<nblk, kPT, 0, st>>>(R, q, c->sc, (double*)c->part.p);
+  k_reduceB<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk, prec);
   const int nword = prec == OCSC_FP64 ? 2 : 1;
   const int64_t nz64 = nword * g.ncell;
   k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 4096), 256, 0, st>>>((unsigned long long*)c->grid.p, nz64);
@@ -713,6 +753,7 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   if (!nmesh || nsteps < 0) return fail(OCSC_EINVAL, "bad nmesh / nsteps");
   if (prec != OCSC_FP64 && prec != OCSC_MIXED) return fail(OCSC_EINVAL, "bad precision");
   if (!(E_GeV > 0)) return fail(OCSC_EINVAL, "E_GeV must be > 0");
+  if (n > (int64_t)0xFFFFFFFF) return fail(OCSC_EINVAL, "tracking supports up to 2^32-1 particles");
   HIPCHK(hipSetDevice(c->device));
   Geometry g;
   if ((rc = make_geometry(nmesh, g))) return rc;
@@ -724,38 +765,136 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   const double beta = std::sqrt(1. - igamma2);
   const double r56 = 0.0 - L / (beta * beta) * igamma2;
   const int nb = (int)((n + 255) / 256);
-  if (!use_graph) {
-    for (int s = 0; s < nsteps; ++s) {
-      k_drift<<<nb, 256, 0, st>>>(r, ld, n, L, r56);
-      if (zstep != 0 && (rc = enqueue_kick_prec(c, g, r, ld, q, in, prec, st))) return rc;
+  if (nsteps == 0) return OCSC_OK;
+  if ((rc = ensure_part(c, n))) return rc;
+
+  // cell-ordered resident copy (sort_every > 0)
+  const bool sorted = c->sort_every > 0 && zstep != 0;
+  int end_bit = 1;
+  while ((int64_t(1) << end_bit) <= g.ncell) ++end_bit;  // keys 0..ncell (ncell: outside the mesh)
+  if (sorted) {
+    for (int k = 0; k < 2; ++k) {
+      if ((rc = ensure(c->rs[k], (size_t)6 * n * 8))) return rc;
+      if ((rc = ensure(c->qs[k], (size_t)n * 8))) return rc;
+      if ((rc = ensure(c->keys[k], (size_t)n * 4))) return rc;
+      if ((rc = ensure(c->perm[k], (size_t)n * 4))) return rc;
     }
-  } else if (nsteps > 0) {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    const int prof = c->profiling;  // no event synchronisation inside a capture
-    c->profiling = 0;
-    HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-    k_drift<<<nb, 256, 0, c->cap_stream>>>(r, ld, n, L, r56);
-    int rc2 = zstep != 0 ? enqueue_kick_prec(c, g, r, ld, q, in, prec, c->cap_stream) : OCSC_OK;
-    hipError_t e = hipStreamEndCapture(c->cap_stream, &graph);
-    c->profiling = prof;
-    if (rc2) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return rc2;
+    if ((rc = ensure(c->vals, (size_t)2 * n * 4))) return rc;
+    size_t tb = 0;
+    HIPCHK(sort_pairs_u32(nullptr, &tb, nullptr, nullptr, nullptr, nullptr, n, end_bit, st));
+    if ((rc = ensure(c->sort_tmp, tb + 256))) return rc;
+  }
+  // sort src (row stride lds) by NGP cell under the current kick scalars into
+  // the copy `to`; perm[to] = order relative to the caller's particles
+  auto resort = [&](const double* src, int64_t lds, const double* qsrc, int to, const uint32_t* perm_prev) -> int {
+    uint32_t* keys0 = (uint32_t*)c->keys[0].p;
+    uint32_t* keys1 = (uint32_t*)c->keys[1].p;
+    uint32_t* vin = (uint32_t*)c->vals.p;
+    uint32_t* vout = vin + n;  // order of src
+    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, c->sc, keys0, vin);
+    LAUNCHCHK("k_sort_keys");
+    size_t tb = c->sort_tmp.cap;
+    HIPCHK(sort_pairs_u32(c->sort_tmp.p, &tb, keys0, keys1, vin, vout, n, end_bit, st));
+    if (perm_prev)
+      k_compose<<<nb, 256, 0, st>>>(perm_prev, vout, (uint32_t*)c->perm[to].p, n);
+    else
+      HIPCHK(hipMemcpyAsync(c->perm[to].p, vout, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    k_permute_gather<<<nb, 256, 0, st>>>(src, lds, qsrc, vout, (double*)c->rs[to].p, n, (double*)c->qs[to].p, n);
+    LAUNCHCHK("k_permute_gather");
+    return OCSC_OK;
+  };
+
+  int cur = 0;
+  double* rr = r;
+  int64_t lr = ld;
+  const double* qq = q;
+  if (sorted) {
+    // kick scalars of the current bunch for the first sort keys
+    const int nblk = nred_blocks(n);
+    k_setup<<<1, 64, 0, st>>>(c->sc, in);
+    k_passA<<<nblk, kPT, 0, st>>>(Rows{r, ld}, c->sc, (double*)c->part.p);
+    k_reduceA<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk);
+    k_passB<<<nblk, kPT, 0, st>>>(Rows{r, ld}, q, c->sc, (double*)c->part.p);
+    k_reduceB<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk, prec);
+    LAUNCHCHK("initial frame");
+    if ((rc = resort(r, ld, q, 0, nullptr))) return rc;
+    rr = (double*)c->rs[0].p;
+    lr = n;
+    qq = (const double*)c->qs[0].p;
+  }
+  // Step loop.  The drift of step s+1 and the momentum sums of kick s+1 are
+  // fused into kick s, so the sequence is drift_0, then kicks 0..n-1 with
+  // kick s < n-1 drifting (first kick: own pass A; later kicks: partials).
+  const int npart = (int)((n + kPT - 1) / kPT);
+  auto opts_for = [&](int step) {
+    StepOpts o;
+    o.passA = step == 0;
+    o.prev_nblk = npart;
+    o.drift = o.sump = step < nsteps - 1;
+    o.dr = DriftArgs{L, r56};
+    return o;
+  };
+  if (zstep == 0) {
+    for (int step = 0; step < nsteps; ++step) k_drift<<<nb, 256, 0, st>>>(rr, lr, n, L, r56);
+  } else {
+    k_drift<<<nb, 256, 0, st>>>(rr, lr, n, L, r56);
+  }
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraph_t graphs[2] = {nullptr, nullptr};
+  auto cleanup = [&]() {
+    for (int k = 0; k < 2; ++k) {
+      if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+      if (graphs[k]) (void)hipGraphDestroy(graphs[k]);
     }
-    if (e != hipSuccess) return fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  };
+  for (int step = 0; step < nsteps && zstep != 0; ++step) {
+    if (sorted && step > 0 && step % c->sort_every == 0) {
+      if ((rc = resort(rr, n, qq, 1 - cur, (const uint32_t*)c->perm[cur].p))) {
+        cleanup();
+        return rc;
+      }
+      cur = 1 - cur;
+      rr = (double*)c->rs[cur].p;
+      qq = (const double*)c->qs[cur].p;
+    }
+    const StepOpts o = opts_for(step);
+    const bool middle = step > 0 && step < nsteps - 1;
+    if (!use_graph || c->profiling || !middle) {
+      if ((rc = enqueue_kick_prec(c, g, rr, lr, qq, in, prec, st, o))) {
+        cleanup();
+        return rc;
+      }
+      continue;
+    }
+    const int gi = sorted ? cur : 0;
+    if (!exec[gi]) {
+      HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+      int rc2 = enqueue_kick_prec(c, g, rr, lr, qq, in, prec, c->cap_stream, o);
+      hipError_t e = hipStreamEndCapture(c->cap_stream, &graphs[gi]);
+      if (rc2 || e != hipSuccess) {
+        cleanup();
+        return rc2 ? rc2 : fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
+      }
+      e = hipGraphInstantiate(&exec[gi], graphs[gi], nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        cleanup();
+        return fail(OCSC_EHIP, std::string("instantiate: ") + hipGetErrorString(e));
+      }
+    }
+    hipError_t e = hipGraphLaunch(exec[gi], st);
     if (e != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return fail(OCSC_EHIP, std::string("instantiate: ") + hipGetErrorString(e));
+      cleanup();
+      return fail(OCSC_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
     }
-    for (int s = 0; s < nsteps && e == hipSuccess; ++s) e = hipGraphLaunch(exec, st);
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(OCSC_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
+  }
+  if (sorted) {
+    k_permute_scatter<<<nb, 256, 0, st>>>(rr, n, (const uint32_t*)c->perm[cur].p, r, ld, n);
+    LAUNCHCHK("k_permute_scatter");
   }
   int32_t err = 0;
-  if ((rc = read_errors(c, st, &err))) return rc;
+  rc = read_errors(c, st, &err);
+  cleanup();
+  if (rc) return rc;
   if (err) return value_error(err);
   return OCSC_OK;
 }
@@ -770,6 +909,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->deposit_xcd = (int)value;
     return OCSC_OK;
   }
+  if (n == "sort_every") {
+    if (value < 0 || value > (1 << 30)) return fail(OCSC_EINVAL, "sort_every must be >= 0");
+    c->sort_every = (int)value;
+    return OCSC_OK;
+  }
   return fail(OCSC_EINVAL, "unknown option " + n);
 }
 
@@ -778,6 +922,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   const std::string n(name);
   if (n == "deposit_xcd") {
     *value = c->deposit_xcd;
+    return OCSC_OK;
+  }
+  if (n == "sort_every") {
+    *value = c->sort_every;
     return OCSC_OK;
   }
   return fail(OCSC_EINVAL, "unknown option " + n);

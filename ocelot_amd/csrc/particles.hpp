@@ -104,27 +104,52 @@ __device__ __forceinline__ void grid_coords(const double* xyz, const KickScalars
 }
 
 // ---------------------------------------------------------------------------
-// deterministic block reduction of NV doubles per thread (sum)
-template <int NV>
-__device__ __forceinline__ void block_sum(double* v, double* red) {
-  const int tid = threadIdx.x;
+// Deterministic block reductions: wave shuffles, then the waves' results in
+// a fixed order.  KIND 0: every value is a sum; KIND 1 (pass B layout):
+// values 0-2 minima, 3-5 maxima, the rest sums.
+template <int KIND>
+__device__ __forceinline__ double red_op(int c, double a, double b) {
+  if (KIND == 1 && c < 3) return b < a ? b : a;
+  if (KIND == 1 && c < 6) return b > a ? b : a;
+  return a + b;
+}
+
+// result valid in thread 0; lds holds (blockDim/64) * NV doubles
+template <int NV, int KIND>
+__device__ __forceinline__ void block_reduce(double* v, double* lds) {
 #pragma unroll
-  for (int c = 0; c < NV; ++c) red[c * kPT + tid] = v[c];
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) v[c] = red_op<KIND>(c, v[c], __shfl_down(v[c], off, 64));
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) lds[w * NV + c] = v[c];
   __syncthreads();
-  for (int w = kPT / 2; w > 0; w >>= 1) {
-    if (tid < w) {
+  if (threadIdx.x == 0)
+    for (int k = 1; k < nw; ++k)
 #pragma unroll
-      for (int c = 0; c < NV; ++c) red[c * kPT + tid] += red[c * kPT + tid + w];
-    }
-    __syncthreads();
-  }
+      for (int c = 0; c < NV; ++c) v[c] = red_op<KIND>(c, v[c], lds[k * NV + c]);
+}
+
+// partials are stored transposed, part[c * nblk + b], so the final reduction
+// reads them coalesced
+template <int NV, int KIND>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nblk, double* out, double* lds) {
+  double v[NV];
 #pragma unroll
-  for (int c = 0; c < NV; ++c) v[c] = red[c * kPT];
+  for (int c = 0; c < NV; ++c) v[c] = KIND == 1 && c < 3 ? __builtin_inf() : (KIND == 1 && c < 6 ? -__builtin_inf() : 0.0);
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) v[c] = red_op<KIND>(c, v[c], part[c * nblk + b]);
+  block_reduce<NV, KIND>(v, lds);
+#pragma unroll
+  for (int c = 0; c < NV; ++c) out[c] = v[c];
 }
 
 // pass A: per-block sums of the momenta (sc.py:224 numerator)
 __global__ void __launch_bounds__(kPT) k_passA(Rows R, KickScalars* __restrict__ scp, double* __restrict__ part) {
-  __shared__ double red[3 * kPT];
+  __shared__ double lds[(kPT / 64) * 3];
   const KickScalars s = *scp;
   double acc[3] = {0, 0, 0};
   const int64_t n = s.n;
@@ -135,19 +160,17 @@ __global__ void __launch_bounds__(kPT) k_passA(Rows R, KickScalars* __restrict__
     acc[1] += m[1];
     acc[2] += m[2];
   }
-  block_sum<3>(acc, red);
+  block_reduce<3, 0>(acc, lds);
   if (threadIdx.x == 0)
-    for (int c = 0; c < 3; ++c) part[blockIdx.x * 3 + c] = acc[c];
+    for (int c = 0; c < 3; ++c) part[c * gridDim.x + blockIdx.x] = acc[c];
 }
 
 // frame from the mean momentum (sc.py:224-239)
-__global__ void __launch_bounds__(kPT) k_reduceA(KickScalars* __restrict__ scp, const double* __restrict__ part,
-                                                 int nblk) {
-  __shared__ double red[3 * kPT];
-  double acc[3] = {0, 0, 0};
-  for (int b = threadIdx.x; b < nblk; b += kPT)
-    for (int c = 0; c < 3; ++c) acc[c] += part[b * 3 + c];
-  block_sum<3>(acc, red);
+__global__ void __launch_bounds__(1024) k_reduceA(KickScalars* __restrict__ scp, const double* __restrict__ part,
+                                                  int nblk) {
+  __shared__ double lds[16 * 3];
+  double acc[3];
+  reduce_partials<3, 0>(part, nblk, acc, lds);
   if (threadIdx.x != 0) return;
   KickScalars& s = *scp;
   const double n = (double)s.n;
@@ -175,7 +198,7 @@ __global__ void __launch_bounds__(kPT) k_reduceA(KickScalars* __restrict__ scp, 
 // pass B: rest-frame min/max, total charge, charge-weighted sums
 __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
                                                double* __restrict__ part) {
-  __shared__ double red[kPT];
+  __shared__ double lds[(kPT / 64) * kNPartB];
   const KickScalars s = *scp;
   const double inf = __builtin_inf();
   double mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
@@ -200,49 +223,20 @@ __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict_
     saq += __builtin_fabs(qq);
   }
   if (bad) atomicOr(&scp->err, (int)kErrNonFinite);
-  // min / max / sums reduced one quantity at a time through LDS
   double vals[kNPartB] = {mn[0], mn[1], mn[2], mx[0], mx[1], mx[2], sq, saq, sqx[0], sqx[1], sqx[2]};
-  const int tid = threadIdx.x;
-  for (int c = 0; c < kNPartB; ++c) {
-    red[tid] = vals[c];
-    __syncthreads();
-    for (int w = kPT / 2; w > 0; w >>= 1) {
-      if (tid < w) {
-        const double a = red[tid], b = red[tid + w];
-        red[tid] = c < 3 ? (b < a ? b : a) : (c < 6 ? (b > a ? b : a) : a + b);
-      }
-      __syncthreads();
-    }
-    if (tid == 0) part[blockIdx.x * kNPartB + c] = red[0];
-    __syncthreads();
-  }
+  block_reduce<kNPartB, 1>(vals, lds);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < kNPartB; ++c) part[c * gridDim.x + blockIdx.x] = vals[c];
 }
 
 // cell sizes, grid offset, charge scale and kernel-cache decision
 // (sc.py:172-186)
-__global__ void __launch_bounds__(kPT) k_reduceB(KickScalars* __restrict__ scp, const double* __restrict__ part,
-                                                 int nblk, int prec) {
-  __shared__ double red[kPT];
-  const int tid = threadIdx.x;
+__global__ void __launch_bounds__(1024) k_reduceB(KickScalars* __restrict__ scp, const double* __restrict__ part,
+                                                  int nblk, int prec) {
+  __shared__ double lds[16 * kNPartB];
   double out[kNPartB];
-  for (int c = 0; c < kNPartB; ++c) {
-    double a = c < 3 ? __builtin_inf() : (c < 6 ? -__builtin_inf() : 0.0);
-    for (int b = tid; b < nblk; b += kPT) {
-      const double v = part[b * kNPartB + c];
-      a = c < 3 ? (v < a ? v : a) : (c < 6 ? (v > a ? v : a) : a + v);
-    }
-    red[tid] = a;
-    __syncthreads();
-    for (int w = kPT / 2; w > 0; w >>= 1) {
-      if (tid < w) {
-        const double x = red[tid], y = red[tid + w];
-        red[tid] = c < 3 ? (y < x ? y : x) : (c < 6 ? (y > x ? y : x) : x + y);
-      }
-      __syncthreads();
-    }
-    out[c] = red[0];
-    __syncthreads();
-  }
+  reduce_partials<kNPartB, 1>(part, nblk, out, lds);
+  const int tid = threadIdx.x;
   if (tid != 0) return;
   KickScalars& s = *scp;
   int err = s.err;
@@ -308,36 +302,139 @@ __device__ __forceinline__ void grid_add(unsigned long long* g, int64_t idx, lon
     atomicAdd(g + idx, (unsigned long long)v);
 }
 
-template <int NWORD, bool XCD = false>
-__global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
-                                                 unsigned long long* __restrict__ grid) {
-  const KickScalars& s0 = *scp;
-  if (s0.err) return;
-  const KickScalars s = s0;
-  const int nx = s.nmesh[0], ny = s.nmesh[1], nz = s.nmesh[2];
-  const int64_t ncell = (int64_t)nx * ny * nz;
-  const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
-  if (p >= s.n) return;
+// NGP cell of a particle from its six rows, -1 if outside the mesh
+__device__ __forceinline__ int64_t ngp_cell(const double* R6, const KickScalars& s) {
   double xp[6], xyz[3], X[3];
-  to_xp(R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p], R.r[5 * R.ld + p], s, xp);
+  to_xp(R6[0], R6[1], R6[2], R6[3], R6[4], R6[5], s, xp);
   rot(xp, s.T, xyz);
   xyz[2] = xyz[2] * s.gamma0;
   grid_coords(xyz, s, X);
+  const int nx = s.nmesh[0], ny = s.nmesh[1], nz = s.nmesh[2];
   const double f0 = __builtin_floor(X[0]) + 1, f1 = __builtin_floor(X[1]) + 1, f2 = __builtin_floor(X[2]) + 1;
-  if (!(f0 >= 0 && f0 < nx && f1 >= 0 && f1 < ny && f2 >= 0 && f2 < nz)) {
-    atomicOr(&scp->err, (int)kErrCell);
-    return;
-  }
-  const int64_t idx = ((int64_t)f0 * ny + (int64_t)f1) * nz + (int64_t)f2;
-  const double v = q[p] * s.qscale;  // exact: qscale is a power of two
-  const double hi = __builtin_rint(v);
+  if (!(f0 >= 0 && f0 < nx && f1 >= 0 && f1 < ny && f2 >= 0 && f2 < nz)) return -1;
+  return ((int64_t)f0 * ny + (int64_t)f1) * nz + (int64_t)f2;
+}
+
+// Deposit, pass C.  A workgroup handles kDepItems consecutive particles per
+// thread and accumulates them in an LDS hash table keyed by cell (64-bit
+// integer LDS atomics), then adds each occupied slot to the global grid with
+// one atomic.  For a bunch kept in cell order (tracking) a workgroup's
+// particles occupy a few hundred neighbouring cells, so global atomics drop by
+// one to two orders of magnitude; in random order the table simply passes
+// every particle through.  Integer sums: the grid is identical either way.
+constexpr int kDepItems = 8;                 // particles per thread
+constexpr int kDepSlots = 2048;              // hash slots (power of two)
+
+template <bool XCD>
+__device__ __forceinline__ void flush_cell(unsigned long long* g, int64_t ncell, int64_t idx, long long hi,
+                                           long long lo, int nword) {
+  grid_add<XCD>(g, idx, hi);
+  if (nword == 2) grid_add<XCD>(g, ncell + idx, lo);
+}
+
+template <int NWORD, bool XCD = false>
+__global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
+                                                 unsigned long long* __restrict__ grid) {
+  __shared__ int hkey[kDepSlots];
+  __shared__ long long hval[NWORD][kDepSlots];
+  const KickScalars& s0 = *scp;
+  if (s0.err) return;  // uniform across the grid
+  const KickScalars s = s0;
+  const int64_t ncell = (int64_t)s.nmesh[0] * s.nmesh[1] * s.nmesh[2];
   unsigned long long* g = grid;
   if constexpr (XCD) g += (int64_t)xcc_id() * NWORD * ncell;
-  grid_add<XCD>(g, idx, (long long)hi);
-  if constexpr (NWORD == 2) {
-    const double lo = __builtin_rint((v - hi) * 0x1p30);
-    grid_add<XCD>(g, ncell + idx, (long long)lo);
+  for (int i = threadIdx.x; i < kDepSlots; i += kPT) {
+    hkey[i] = -1;
+    hval[0][i] = 0;
+    if constexpr (NWORD == 2) hval[1][i] = 0;
   }
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * (kPT * kDepItems);
+  int bad = 0;
+  for (int it = 0; it < kDepItems; ++it) {
+    const int64_t p = base + it * kPT + threadIdx.x;
+    if (p >= s.n) break;
+    const double R6[6] = {R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p],
+                          R.r[5 * R.ld + p]};
+    const int64_t idx = ngp_cell(R6, s);
+    if (idx < 0) {
+      bad = 1;
+      continue;
+    }
+    const double v = q[p] * s.qscale;  // exact: qscale is a power of two
+    const double h = __builtin_rint(v);
+    const long long hi = (long long)h;
+    long long lo = 0;
+    if constexpr (NWORD == 2) lo = (long long)__builtin_rint((v - h) * 0x1p30);
+    // linear probing; cell indices fit in 31 bits (mesh <= 1024^3)
+    unsigned slot = ((unsigned)idx * 2654435761u) >> (32 - ilog2(kDepSlots));
+    bool done = false;
+    for (int probe = 0; probe < 32; ++probe) {
+      const int k = atomicCAS(&hkey[slot], -1, (int)idx);
+      if (k == -1 || k == (int)idx) {
+        atomicAdd((unsigned long long*)&hval[0][slot], (unsigned long long)hi);
+        if constexpr (NWORD == 2) atomicAdd((unsigned long long*)&hval[1][slot], (unsigned long long)lo);
+        done = true;
+        break;
+      }
+      slot = (slot + 1) & (kDepSlots - 1);
+    }
+    if (!done) flush_cell<XCD>(g, ncell, idx, hi, lo, NWORD);  // crowded table: straight to the grid
+  }
+  if (bad) atomicOr(&scp->err, (int)kErrCell);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kDepSlots; i += kPT) {
+    const int k = hkey[i];
+    if (k >= 0) flush_cell<XCD>(g, ncell, k, hval[0][i], NWORD == 2 ? hval[NWORD - 1][i] : 0, NWORD);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cell ordering of a resident bunch (multi-kick tracking): sort keys are the
+// NGP cells under the scalars of the last kick; particles move little between
+// re-sorts, so consecutive particles keep sharing cells.
+__global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const KickScalars* __restrict__ scp,
+                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const KickScalars s = *scp;
+  const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
+  if (p >= s.n) return;
+  const double R6[6] = {R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p],
+                        R.r[5 * R.ld + p]};
+  const int64_t c = ngp_cell(R6, s);
+  const int64_t ncell = (int64_t)s.nmesh[0] * s.nmesh[1] * s.nmesh[2];
+  keys[p] = (uint32_t)(c >= 0 ? c : ncell);
+  vals[p] = (uint32_t)p;
+}
+
+// dst[:, j] = src[:, perm[j]] for the six rows and the charge
+__global__ void __launch_bounds__(256) k_permute_gather(const double* __restrict__ src, int64_t lds_,
+                                                        const double* __restrict__ qs, const uint32_t* __restrict__ perm,
+                                                        double* __restrict__ dst, int64_t ldd, double* __restrict__ qd,
+                                                        int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int64_t p = perm[j];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) dst[i * ldd + j] = src[i * lds_ + p];
+  qd[j] = qs[p];
+}
+
+// dst[:, perm[j]] = src[:, j] for the six rows
+__global__ void __launch_bounds__(256) k_permute_scatter(const double* __restrict__ src, int64_t lds_,
+                                                         const uint32_t* __restrict__ perm, double* __restrict__ dst,
+                                                         int64_t ldd, int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int64_t p = perm[j];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) dst[i * ldd + p] = src[i * lds_ + j];
+}
+
+// out[j] = a[b[j]]
+__global__ void __launch_bounds__(256) k_compose(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                                                 uint32_t* __restrict__ out, int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j < n) out[j] = a[b[j]];
 }
 
 // Trilinear interpolation at fractional grid coordinate c of the staggered
@@ -457,29 +554,52 @@ __device__ __forceinline__ void gather_field(const F* __restrict__ phi, const Ki
 }
 
 // pass D: gather, kick in the rotated frame, rotate back, write all six rows
-// in place (sc.py:241-251, coord_transform.py:16-54)
-template <typename F>
+// in place (sc.py:241-251, coord_transform.py:16-54).
+// Tracking fusions: DRIFT applies the next step's drift to the written rows
+// (x += L x', y += L y', tau += r56 delta, as numpy.dot(R, X) rounds it);
+// SUMP stores per-block sums of the momenta the next kick's pass A would
+// compute from the written rows (a drift leaves x', y', delta unchanged).
+struct DriftArgs {
+  double L, r56;
+};
+
+template <typename F, bool DRIFT = false, bool SUMP = false>
 __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
-                                              const KickScalars* __restrict__ scp) {
+                                              const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
+                                              double* __restrict__ part = nullptr) {
+  __shared__ double lds[SUMP ? (kPT / 64) * 3 : 1];
   const KickScalars& s0 = *scp;
-  if (s0.err) return;
+  if (s0.err) return;  // uniform across the grid
   const KickScalars s = s0;
   const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
-  if (p >= s.n) return;
-  double xp[6], xyz[3], X[3], pr[3], E[3], pb[3], o[6];
-  to_xp(r[p], r[ld + p], r[2 * ld + p], r[3 * ld + p], r[4 * ld + p], r[5 * ld + p], s, xp);
-  rot(xp, s.T, xyz);
-  xyz[2] = xyz[2] * s.gamma0;
-  grid_coords(xyz, s, X);
-  rot(xp + 3, s.T, pr);
-  gather_field<F>(phi, s, X, E);
-  pr[0] = pr[0] + s.kperp * E[0];
-  pr[1] = pr[1] + s.kperp * E[1];
-  pr[2] = pr[2] + s.cdT * E[2];
-  rot_back(pr, s.T, pb);
-  to_ocelot(xp, pb, s, o);
+  double m[3] = {0, 0, 0};
+  if (p < s.n) {
+    double xp[6], xyz[3], X[3], pr[3], E[3], pb[3], o[6];
+    to_xp(r[p], r[ld + p], r[2 * ld + p], r[3 * ld + p], r[4 * ld + p], r[5 * ld + p], s, xp);
+    rot(xp, s.T, xyz);
+    xyz[2] = xyz[2] * s.gamma0;
+    grid_coords(xyz, s, X);
+    rot(xp + 3, s.T, pr);
+    gather_field<F>(phi, s, X, E);
+    pr[0] = pr[0] + s.kperp * E[0];
+    pr[1] = pr[1] + s.kperp * E[1];
+    pr[2] = pr[2] + s.cdT * E[2];
+    rot_back(pr, s.T, pb);
+    to_ocelot(xp, pb, s, o);
+    if constexpr (DRIFT) {
+      o[0] = __builtin_fma(dr.L, o[1], o[0]);
+      o[2] = __builtin_fma(dr.L, o[3], o[2]);
+      o[4] = __builtin_fma(dr.r56, o[5], o[4]);
+    }
 #pragma unroll
-  for (int i = 0; i < 6; ++i) r[i * ld + p] = o[i];
+    for (int i = 0; i < 6; ++i) r[i * ld + p] = o[i];
+    if constexpr (SUMP) to_p(o[1], o[3], o[5], s, m);
+  }
+  if constexpr (SUMP) {
+    block_reduce<3, 0>(m, lds);
+    if (threadIdx.x == 0)
+      for (int c = 0; c < 3; ++c) part[c * gridDim.x + blockIdx.x] = m[c];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -488,7 +608,7 @@ __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld
 
 __global__ void __launch_bounds__(kPT) k_passB_xyz(Rows R, const double* __restrict__ q,
                                                    KickScalars* __restrict__ scp, double* __restrict__ part) {
-  __shared__ double red[kPT];
+  __shared__ double lds[(kPT / 64) * kNPartB];
   const KickScalars s = *scp;
   const double inf = __builtin_inf();
   double mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
@@ -510,20 +630,9 @@ __global__ void __launch_bounds__(kPT) k_passB_xyz(Rows R, const double* __restr
   }
   if (bad) atomicOr(&scp->err, (int)kErrNonFinite);
   double vals[kNPartB] = {mn[0], mn[1], mn[2], mx[0], mx[1], mx[2], sq, saq, sqx[0], sqx[1], sqx[2]};
-  const int tid = threadIdx.x;
-  for (int c = 0; c < kNPartB; ++c) {
-    red[tid] = vals[c];
-    __syncthreads();
-    for (int w = kPT / 2; w > 0; w >>= 1) {
-      if (tid < w) {
-        const double a = red[tid], b = red[tid + w];
-        red[tid] = c < 3 ? (b < a ? b : a) : (c < 6 ? (b > a ? b : a) : a + b);
-      }
-      __syncthreads();
-    }
-    if (tid == 0) part[blockIdx.x * kNPartB + c] = red[0];
-    __syncthreads();
-  }
+  block_reduce<kNPartB, 1>(vals, lds);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < kNPartB; ++c) part[c * gridDim.x + blockIdx.x] = vals[c];
 }
 
 template <int NWORD>
