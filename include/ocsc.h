@@ -104,7 +104,11 @@ int ocsc_track_drift(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double
 /* Tuning options of a context (the autotuner's knobs; no reference
  * counterpart).  "deposit_xcd": 0 = one charge grid, device-scope atomics;
  * 1 = one grid copy per XCD with atomics that complete in that XCD's L2,
- * summed exactly in the first FFT pass.  Both give bit-identical results. */
+ * summed exactly in the first FFT pass.  Both give bit-identical results.
+ * "sort_every": ocsc_track_drift keeps the bunch in mesh-cell order and
+ * re-sorts it every k steps (0: never; default 20).  "overlap": 1
+ * builds the Green's-function spectrum on a second stream concurrently with
+ * the charge deposit, 0 (default: they contend for the CUs) runs it in line. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 

@@ -40,6 +40,17 @@ __device__ __forceinline__ cpx<double> cmul(cpx<double> a, cpx<double> b) {
 template <typename R>
 __device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) { return {a.x * s, a.y * s}; }
 
+// reciprocal from the hardware estimate plus two Newton steps (error below
+// one ulp, not always correctly rounded): the mixed form's particle passes
+// use it where the FP32 field dominates the error anyway
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
 // 1 / (g*g) rounded once: double-double g^2, then one Newton correction of the
 // reciprocal; matches a correctly rounded pow(g, -2).
 __device__ __forceinline__ double inv_sq(double g) {
@@ -56,6 +67,7 @@ __device__ __forceinline__ double inv_sq(double g) {
 struct KickScalars {
   // inputs written by the host before the kick
   double gamref, betaref, pref, gbref;  // gbref = gamref*betaref
+  double inv_gbref, inv_gamref, inv_betaref, inv_pref;  // reciprocals for the fast (mixed) transforms
   double cdT;                           // zstep / betaref
   double mesh_scale;                    // 1 or the U(1,1.1) draw
   double mesh_shift;                    // 0 or the U(-0.5,0.5) draw
@@ -67,6 +79,7 @@ struct KickScalars {
   double Pav, gamma0, beta0, kperp;  // kperp = cdT*(1-beta0*beta0)
   // grid placement (ocelot/cpbd/sc.py:172-186)
   double steps[3];
+  double inv_steps[3];
   double X_off[3];
   double xmin[3], xmax[3];
   double sumq, sumabsq, sumqx[3];

@@ -83,6 +83,10 @@ __global__ void k_setup(KickScalars* __restrict__ s, KickInputs in) {
   s->betaref = in.betaref;
   s->pref = in.pref;
   s->gbref = in.gbref;
+  s->inv_gbref = 1.0 / in.gbref;
+  s->inv_gamref = 1.0 / in.gamref;
+  s->inv_betaref = 1.0 / in.betaref;
+  s->inv_pref = 1.0 / in.pref;
   s->cdT = in.cdT;
   s->mesh_scale = in.mesh_scale;
   s->mesh_shift = in.mesh_shift;
@@ -141,6 +145,9 @@ struct ocsc_ctx {
   cpx<float>* twf = nullptr;       // W_M^k for M at offset M (M = 8..1024)
   cpx<double>* twd = nullptr;
   hipStream_t cap_stream = nullptr;  // stream used for graph capture
+  hipStream_t side = nullptr;        // second stream: kernel spectrum next to the deposit
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int overlap = 0;                   // 1: build the kernel spectrum concurrently with the deposit
   Buf grid, A, B, phi, G;
   // stage timing (ocsc_set_profiling): events between the stages of a kick
   int profiling = 0;
@@ -425,10 +432,24 @@ int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const do
     k_reduceA<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, o.prev_nblk);
   }
   mark(1);
-  k_passB<<<nblk, kPT, 0, st>>>(R, q, c->sc, (double*)c->part.p);
+  if (prec == OCSC_MIXED)
+    k_passB<true><<<nblk, kPT, 0, st>>>(R, q, c->sc, (double*)c->part.p);
+  else
+    k_passB<false><<<nblk, kPT, 0, st>>>(R, q, c->sc, (double*)c->part.p);
   k_reduceB<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk, prec);
   LAUNCHCHK("frame/extent passes");
   mark(2);
+  int rc;
+  // kernel spectrum (needs only the cell sizes) on the side stream while the
+  // main stream deposits; joined before the first FFT pass.  Sequential when
+  // profiling, so the stage times stay separable.
+  const bool fork = c->overlap && !prof;
+  if (fork) {
+    HIPCHK(hipEventRecord(c->ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    if ((rc = launch_kernel_spectrum<F>(c, g, c->side, 0))) return rc;
+    HIPCHK(hipEventRecord(c->ev_join, c->side));
+  }
   const int64_t nz64 = (c->deposit_xcd ? kNumXcd : 1) * NWORD * g.ncell;
   k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, st>>>((unsigned long long*)c->grid.p, nz64);
   const int ndep = (int)((n + kPT * kDepItems - 1) / (kPT * kDepItems));
@@ -438,8 +459,10 @@ int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const do
     k_deposit<NWORD, false><<<ndep, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
   LAUNCHCHK("deposit");
   mark(3);
-  int rc;
-  if ((rc = launch_kernel_spectrum<F>(c, g, st, 0))) return rc;
+  if (fork)
+    HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
+  else if ((rc = launch_kernel_spectrum<F>(c, g, st, 0)))
+    return rc;
   mark(4);
   if constexpr (sizeof(F) == 4) {
     if ((rc = launch_fields(c, g, c->grid.p, (float*)c->phi.p, st, prof ? c->ev + 4 : nullptr))) return rc;
@@ -570,6 +593,10 @@ int ocsc_create(ocsc_ctx** out, int device) {
     return cleanup(fail(OCSC_EHIP, "stream"));
   for (int i = 0; i <= kNumStages; ++i)
     if (hipEventCreate(&c->ev[i]) != hipSuccess) return cleanup(fail(OCSC_EHIP, "event"));
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(OCSC_EHIP, "stream"));
+  if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+    return cleanup(fail(OCSC_EHIP, "event"));
   *out = c;
   return OCSC_OK;
 }
@@ -586,6 +613,9 @@ int ocsc_destroy(ocsc_ctx* c) {
   if (c->twf) (void)hipFree(c->twf);
   if (c->twd) (void)hipFree(c->twd);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
+  if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   for (int i = 0; i <= kNumStages; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   delete c;
@@ -770,8 +800,8 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
 
   // cell-ordered resident copy (sort_every > 0)
   const bool sorted = c->sort_every > 0 && zstep != 0;
-  int end_bit = 1;
-  while ((int64_t(1) << end_bit) <= g.ncell) ++end_bit;  // keys 0..ncell (ncell: outside the mesh)
+  int end_bit = 1;  // keys 0..nkeys (nkeys: outside the mesh)
+  while ((int64_t(1) << end_bit) <= brick_key_count(g.d.nx, g.d.ny, g.d.nz)) ++end_bit;
   if (sorted) {
     for (int k = 0; k < 2; ++k) {
       if ((rc = ensure(c->rs[k], (size_t)6 * n * 8))) return rc;
@@ -814,7 +844,7 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
     k_setup<<<1, 64, 0, st>>>(c->sc, in);
     k_passA<<<nblk, kPT, 0, st>>>(Rows{r, ld}, c->sc, (double*)c->part.p);
     k_reduceA<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk);
-    k_passB<<<nblk, kPT, 0, st>>>(Rows{r, ld}, q, c->sc, (double*)c->part.p);
+    k_passB<false><<<nblk, kPT, 0, st>>>(Rows{r, ld}, q, c->sc, (double*)c->part.p);
     k_reduceB<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk, prec);
     LAUNCHCHK("initial frame");
     if ((rc = resort(r, ld, q, 0, nullptr))) return rc;
@@ -914,6 +944,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->sort_every = (int)value;
     return OCSC_OK;
   }
+  if (n == "overlap") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "overlap must be 0 or 1");
+    c->overlap = (int)value;
+    return OCSC_OK;
+  }
   return fail(OCSC_EINVAL, "unknown option " + n);
 }
 
@@ -926,6 +961,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "sort_every") {
     *value = c->sort_every;
+    return OCSC_OK;
+  }
+  if (n == "overlap") {
+    *value = c->overlap;
     return OCSC_OK;
   }
   return fail(OCSC_EINVAL, "unknown option " + n);

@@ -52,6 +52,67 @@ __device__ __forceinline__ void to_xp(double r0, double r1, double r2, double r3
   xp[5] = u2 * gamma * beta * kMeEV;
 }
 
+// Fast variants for the mixed form (passes B and D, fused momentum sums):
+// one reciprocal per divisor, constant divisors replaced by their inverses.
+// Results differ from the exact sequence in the last bits only.
+__device__ __forceinline__ void to_xp_fast(double r0, double r1, double r2, double r3, double r4, double r5,
+                                           const KickScalars& s, double* xp) {
+  const double gamma = (s.betaref * r5 + 1) * s.gamref;
+  const double ig = rcp_nr(gamma);
+  const double beta = __builtin_sqrt(1 - ig * ig);
+  const double gb = gamma * beta;
+  const double t = gb * s.inv_gbref;
+  const double pz2pref = __builtin_sqrt(t * t - r1 * r1 - r3 * r3);
+  const double ipz = rcp_nr(pz2pref);
+  const double a0 = r1 * ipz, a1 = r3 * ipz;
+  const double inrm = rcp_nr(__builtin_sqrt(a0 * a0 + a1 * a1 + 1.0));
+  const double u0 = a0 * inrm, u1 = a1 * inrm, u2 = inrm;
+  xp[0] = r0 - u0 * beta * r4;
+  xp[1] = r2 - u1 * beta * r4;
+  xp[2] = -u2 * beta * r4;
+  const double gbm = gb * kMeEV;
+  xp[3] = u0 * gbm;
+  xp[4] = u1 * gbm;
+  xp[5] = u2 * gbm;
+}
+
+__device__ __forceinline__ void to_p_fast(double r1, double r3, double r5, const KickScalars& s, double* p) {
+  const double gamma = (s.betaref * r5 + 1) * s.gamref;
+  const double ig = rcp_nr(gamma);
+  const double beta = __builtin_sqrt(1 - ig * ig);
+  const double gb = gamma * beta;
+  const double t = gb * s.inv_gbref;
+  const double pz2pref = __builtin_sqrt(t * t - r1 * r1 - r3 * r3);
+  const double ipz = rcp_nr(pz2pref);
+  const double a0 = r1 * ipz, a1 = r3 * ipz;
+  const double inrm = rcp_nr(__builtin_sqrt(a0 * a0 + a1 * a1 + 1.0));
+  const double gbm = gb * kMeEV * inrm;
+  p[0] = a0 * gbm;
+  p[1] = a1 * gbm;
+  p[2] = gbm;
+}
+
+__device__ __forceinline__ void to_ocelot_fast(const double* xp, const double* p, const KickScalars& s, double* o) {
+  const double s2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+  const double gamma = __builtin_sqrt(1 + s2 * (1.0 / kMeEV2));
+  const double ig = rcp_nr(gamma);
+  const double beta = __builtin_sqrt(1 - ig * ig);
+  const double inrm = rcp_nr(__builtin_sqrt(s2));
+  const double u0 = p[0] * inrm, u1 = p[1] * inrm, u2 = p[2] * inrm;
+  const double cdt = -xp[2] * rcp_nr(beta * u2);
+  o[0] = xp[0] + beta * u0 * cdt;
+  o[2] = xp[1] + beta * u1 * cdt;
+  o[5] = (gamma * s.inv_gamref - 1) * s.inv_betaref;
+  o[4] = cdt;
+  o[1] = p[0] * s.inv_pref;
+  o[3] = p[1] * s.inv_pref;
+}
+
+__device__ __forceinline__ void grid_coords_fast(const double* xyz, const KickScalars& s, double* X) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) X[k] = xyz[k] * s.inv_steps[k] - s.X_off[k];
+}
+
 // the momentum part only (pass A)
 __device__ __forceinline__ void to_p(double r1, double r3, double r5, const KickScalars& s, double* p) {
   const double gamma = (s.betaref * r5 + 1) * s.gamref;
@@ -196,6 +257,7 @@ __global__ void __launch_bounds__(1024) k_reduceA(KickScalars* __restrict__ scp,
 }
 
 // pass B: rest-frame min/max, total charge, charge-weighted sums
+template <bool FAST>
 __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
                                                double* __restrict__ part) {
   __shared__ double lds[(kPT / 64) * kNPartB];
@@ -207,7 +269,11 @@ __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict_
   const int64_t n = s.n;
   for (int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x; p < n; p += (int64_t)gridDim.x * kPT) {
     double xp[6], xyz[3];
-    to_xp(R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p], R.r[5 * R.ld + p], s, xp);
+    if constexpr (FAST)
+      to_xp_fast(R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p], R.r[5 * R.ld + p], s,
+                 xp);
+    else
+      to_xp(R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p], R.r[5 * R.ld + p], s, xp);
     rot(xp, s.T, xyz);
     xyz[2] = xyz[2] * s.gamma0;
     const double qq = q[p];
@@ -253,6 +319,7 @@ __global__ void __launch_bounds__(1024) k_reduceB(KickScalars* __restrict__ scp,
     if (s.random_mesh) XX = XX * s.mesh_scale;
     const double h = XX / (double)(s.nmesh[k] - 3);
     s.steps[k] = h;
+    s.inv_steps[k] = 1.0 / h;
     if (!(h > 0) || !__builtin_isfinite(h)) err |= kErrSteps;
     const double Xmin = s.xmin[k] / h;
     const double Xmid = (s.sumqx[k] / h) / s.sumq;
@@ -393,16 +460,38 @@ __global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restric
 // Cell ordering of a resident bunch (multi-kick tracking): sort keys are the
 // NGP cells under the scalars of the last kick; particles move little between
 // re-sorts, so consecutive particles keep sharing cells.
+// Sort keys: bricks of kBrick cells (4 x 4 x 8, z fastest), bricks in C
+// order, cells in C order inside a brick, so a run of consecutive particles
+// fills a compact 3D box of the mesh.  Keys only steer locality, so the fast
+// transforms are used; out-of-mesh particles sort last.
+constexpr int kBx = 4, kBy = 4, kBz = 8;
+
+__host__ __device__ inline int64_t brick_key_count(int nx, int ny, int nz) {
+  const int64_t nbx = (nx + kBx - 1) / kBx, nby = (ny + kBy - 1) / kBy, nbz = (nz + kBz - 1) / kBz;
+  return nbx * nby * nbz * (kBx * kBy * kBz);
+}
+
 __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const KickScalars* __restrict__ scp,
                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const KickScalars s = *scp;
   const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
   if (p >= s.n) return;
-  const double R6[6] = {R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p],
-                        R.r[5 * R.ld + p]};
-  const int64_t c = ngp_cell(R6, s);
-  const int64_t ncell = (int64_t)s.nmesh[0] * s.nmesh[1] * s.nmesh[2];
-  keys[p] = (uint32_t)(c >= 0 ? c : ncell);
+  double xp[6], xyz[3], X[3];
+  to_xp_fast(R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p], R.r[5 * R.ld + p], s,
+             xp);
+  rot(xp, s.T, xyz);
+  xyz[2] = xyz[2] * s.gamma0;
+  grid_coords_fast(xyz, s, X);
+  const int nx = s.nmesh[0], ny = s.nmesh[1], nz = s.nmesh[2];
+  const double g0 = __builtin_floor(X[0]) + 1, g1 = __builtin_floor(X[1]) + 1, g2 = __builtin_floor(X[2]) + 1;
+  uint32_t key = (uint32_t)brick_key_count(nx, ny, nz);
+  if (g0 >= 0 && g0 < nx && g1 >= 0 && g1 < ny && g2 >= 0 && g2 < nz) {
+    const int f0 = (int)g0, f1 = (int)g1, f2 = (int)g2;
+    const int nby = (ny + kBy - 1) / kBy, nbz = (nz + kBz - 1) / kBz;
+    const int64_t brick = ((int64_t)(f0 / kBx) * nby + f1 / kBy) * nbz + f2 / kBz;
+    key = (uint32_t)(brick * (kBx * kBy * kBz) + ((f0 % kBx) * kBy + f1 % kBy) * kBz + f2 % kBz);
+  }
+  keys[p] = key;
   vals[p] = (uint32_t)p;
 }
 
@@ -563,29 +652,174 @@ struct DriftArgs {
   double L, r56;
 };
 
+// LDS tile of the three FP32 field grids over the box of mesh nodes that a
+// workgroup's particles touch (a bunch kept in cell order packs a workgroup
+// into a few cells); larger boxes fall back to gathers from global memory.
+constexpr int kTileNodes = 1024;
+
+struct TileBox {
+  int lo[3], dim[3], vol;
+};
+
+// block-wide box of the 3x3x3 stencils based at f (clamped to the mesh,
+// with one layer of margin below for the clamped corner at n-2)
+__device__ __forceinline__ TileBox block_box(const int* f, bool active, const int* n, int* lds6) {
+  int v[6];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    v[k] = active ? f[k] : 0x3fffffff;
+    v[3 + k] = active ? f[k] : -0x3fffffff;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      v[k] = min(v[k], __shfl_xor(v[k], off, 64));
+      v[3 + k] = max(v[3 + k], __shfl_xor(v[3 + k], off, 64));
+    }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) lds6[w * 6 + k] = v[k];
+  __syncthreads();
+  TileBox b;
+  int mn[3], mx[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = lds6[k];
+    mx[k] = lds6[3 + k];
+  }
+  for (int ww = 1; ww < (int)(blockDim.x >> 6); ++ww)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = min(mn[k], lds6[ww * 6 + k]);
+      mx[k] = max(mx[k], lds6[ww * 6 + 3 + k]);
+    }
+  b.vol = 1;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = max(mn[k] - 1, 0);
+    const int hi = min(mx[k] + 2, n[k] - 1);
+    b.dim[k] = hi >= b.lo[k] ? hi - b.lo[k] + 1 : 0;
+    b.vol *= b.dim[k];
+  }
+  return b;
+}
+
+// gather_grid reading the staged tile (same corners, same arithmetic)
+template <int AX>
+__device__ __forceinline__ float gather_tile(const float* __restrict__ T, const TileBox& b, const int* n,
+                                             const double* c) {
+  if (!(c[0] >= 0 && c[0] <= n[0] - 1 && c[1] >= 0 && c[1] <= n[1] - 1 && c[2] >= 0 && c[2] <= n[2] - 1))
+    return 0.f;
+  int i0[3];
+  float t[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int f = (int)__builtin_floor(c[k]);
+    f = f > n[k] - 2 ? n[k] - 2 : f;
+    i0[k] = f;
+    t[k] = (float)(c[k] - (double)f);
+  }
+  const int sx = b.dim[1] * b.dim[2], sy = b.dim[2];
+  const int o0 = (i0[0] - b.lo[0]) * sx + (i0[1] - b.lo[1]) * sy + (i0[2] - b.lo[2]);
+  const int nax = n[AX];
+  float v[8];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int iax = (AX == 0 ? i0[0] + a : (AX == 1 ? i0[1] + bb : i0[2] + e));
+        v[a * 4 + bb * 2 + e] = iax < nax - 1 ? T[o0 + a * sx + bb * sy + e] : 0.f;
+      }
+  float acc = 0.f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const float w = (a ? t[0] : 1 - t[0]) * (bb ? t[1] : 1 - t[1]) * (e ? t[2] : 1 - t[2]);
+        acc += w * v[a * 4 + bb * 2 + e];
+      }
+  return acc;
+}
+
 template <typename F, bool DRIFT = false, bool SUMP = false>
 __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
                                               const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
                                               double* __restrict__ part = nullptr) {
   __shared__ double lds[SUMP ? (kPT / 64) * 3 : 1];
+  // mixed form: field tile; FP64 form: unused
+  __shared__ float tile[sizeof(F) == 4 ? 3 * kTileNodes : 1];
+  __shared__ int lds6[(kPT / 64) * 6];
   const KickScalars& s0 = *scp;
   if (s0.err) return;  // uniform across the grid
   const KickScalars s = s0;
   const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
-  double m[3] = {0, 0, 0};
-  if (p < s.n) {
-    double xp[6], xyz[3], X[3], pr[3], E[3], pb[3], o[6];
-    to_xp(r[p], r[ld + p], r[2 * ld + p], r[3 * ld + p], r[4 * ld + p], r[5 * ld + p], s, xp);
+  const bool active = p < s.n;
+  // the mixed form (FP32 field) takes the fast transforms
+  constexpr bool FAST = sizeof(F) == 4;
+  double xp[6] = {0, 0, 0, 0, 0, 0}, xyz[3], X[3] = {0, 0, 0}, pr[3], E[3], pb[3], o[6];
+  if (active) {
+    if constexpr (FAST)
+      to_xp_fast(r[p], r[ld + p], r[2 * ld + p], r[3 * ld + p], r[4 * ld + p], r[5 * ld + p], s, xp);
+    else
+      to_xp(r[p], r[ld + p], r[2 * ld + p], r[3 * ld + p], r[4 * ld + p], r[5 * ld + p], s, xp);
     rot(xp, s.T, xyz);
     xyz[2] = xyz[2] * s.gamma0;
-    grid_coords(xyz, s, X);
+    if constexpr (FAST)
+      grid_coords_fast(xyz, s, X);
+    else
+      grid_coords(xyz, s, X);
+  }
+  bool staged = false;
+  TileBox box;
+  const int nn[3] = {s.nmesh[0], s.nmesh[1], s.nmesh[2]};
+  if constexpr (FAST) {
+    int f[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double fl = __builtin_floor(X[k]);
+      f[k] = fl < -1 ? -1 : (fl > nn[k] ? nn[k] : (int)fl);
+    }
+    box = block_box(f, active, nn, lds6);
+    if (box.vol > 0 && box.vol <= kTileNodes) {
+      staged = true;
+      const int64_t nc = (int64_t)nn[0] * nn[1] * nn[2];
+      const int dyz = box.dim[1] * box.dim[2];
+      for (int i = threadIdx.x; i < 3 * box.vol; i += kPT) {
+        const int comp = i / box.vol, node = i - comp * box.vol;
+        const int a = node / dyz, rem = node - a * dyz, b = rem / box.dim[2], c = rem - b * box.dim[2];
+        const int64_t gidx = ((int64_t)(box.lo[0] + a) * nn[1] + (box.lo[1] + b)) * nn[2] + (box.lo[2] + c);
+        tile[i] = ((const float*)phi)[comp * nc + gidx];
+      }
+      __syncthreads();
+    }
+  }
+  double m[3] = {0, 0, 0};
+  if (active) {
     rot(xp + 3, s.T, pr);
-    gather_field<F>(phi, s, X, E);
+    if (staged) {
+      const double cx[3] = {X[0], X[1] + 0.5, X[2] + 0.5};
+      const double cy[3] = {X[0] + 0.5, X[1], X[2] + 0.5};
+      const double cz[3] = {X[0] + 0.5, X[1] + 0.5, X[2]};
+      E[0] = (double)gather_tile<0>(tile, box, nn, cx) * s.gamma0;
+      E[1] = (double)gather_tile<1>(tile + box.vol, box, nn, cy) * s.gamma0;
+      E[2] = (double)gather_tile<2>(tile + 2 * box.vol, box, nn, cz);
+    } else {
+      gather_field<F>(phi, s, X, E);
+    }
     pr[0] = pr[0] + s.kperp * E[0];
     pr[1] = pr[1] + s.kperp * E[1];
     pr[2] = pr[2] + s.cdT * E[2];
     rot_back(pr, s.T, pb);
-    to_ocelot(xp, pb, s, o);
+    if constexpr (FAST)
+      to_ocelot_fast(xp, pb, s, o);
+    else
+      to_ocelot(xp, pb, s, o);
     if constexpr (DRIFT) {
       o[0] = __builtin_fma(dr.L, o[1], o[0]);
       o[2] = __builtin_fma(dr.L, o[3], o[2]);
@@ -593,7 +827,12 @@ __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) r[i * ld + p] = o[i];
-    if constexpr (SUMP) to_p(o[1], o[3], o[5], s, m);
+    if constexpr (SUMP) {
+      if constexpr (FAST)
+        to_p_fast(o[1], o[3], o[5], s, m);
+      else
+        to_p(o[1], o[3], o[5], s, m);
+    }
   }
   if constexpr (SUMP) {
     block_reduce<3, 0>(m, lds);
