@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--sort-every", type=int, default=10,
                     help="re-sort the resident bunch by mesh cell every k tracking steps (0: never)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the context measurements of the other configurations")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run one elementwise kernel reading and writing 1 GiB each, the reference "
                          "for scaling profiler byte counters")
@@ -96,6 +98,55 @@ def cpu_baseline(name, n_sample, seed=0):
     return dict(value=n_sample / dt, unit="macroparticle-kicks/s", cores=1, kind="port",
                 sample=f"{name}: one FP64 oracle kick of {n_sample} particles of the same distribution on "
                        f"{nmesh[0]}^3, {dt:.2f} s")
+
+
+def other_configs(torch, dev):
+    """Context lines (not the headline): the 64-bunch batched configuration
+    (uneven sizes, meshes 31/47/63, 20 kicks, mixed) and single drop-in kicks
+    of the 2M-particle 63^3 bunch (FP64, device-resident and host numpy)."""
+    from ocelot_amd import DeviceParticleArray, ParticleArray, SpaceCharge, apply_batched
+    from ocelot_amd.workloads import batch64, make_bunch
+    out = {}
+    bunches = batch64(seed=0)
+    ntot = sum(b["r"].shape[1] for b in bunches)
+    ds = []
+    for b in bunches:
+        p = ParticleArray()
+        p.rparticles, p.q_array, p.E = b["r"], b["q"], b["E"]
+        ds.append(DeviceParticleArray.from_host(p, dev))
+    items = [(d, b["nmesh"], 0.1) for d, b in zip(ds, bunches)]
+    apply_batched(items, precision="mixed", nkicks=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    apply_batched(items, precision="mixed", nkicks=20)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 20
+    out["batch64"] = {"particles": ntot, "bunches": len(bunches), "kicks": 20, "ms_per_round": round(dt * 1e3, 3),
+                      "value": round(ntot / dt, 1), "unit": "macroparticle-kicks/s", "precision": "mixed",
+                      "hip_graph": True}
+    del ds, items
+    r, q, E, nmesh, dz = make_bunch("gauss_2m_63")
+    host = ParticleArray()
+    host.rparticles, host.q_array, host.E = r, q, E
+    sc = SpaceCharge(nmesh_xyz=list(nmesh))
+    d = DeviceParticleArray.from_host(host, dev)
+    sc.apply(d, dz)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        sc.apply(d, dz)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 10
+    hp = host.copy()
+    t1 = time.perf_counter()
+    sc.apply(hp, dz)
+    dth = time.perf_counter() - t1
+    out["gauss_2m_63_fp64_apply"] = {"particles": r.shape[1], "ms_per_kick_device": round(dt * 1e3, 3),
+                                     "value_device": round(r.shape[1] / dt, 1),
+                                     "ms_per_kick_host_numpy": round(dth * 1e3, 3),
+                                     "value_host_numpy_incl_pcie": round(r.shape[1] / dth, 1),
+                                     "unit": "macroparticle-kicks/s", "precision": "fp64", "order": "caller's"}
+    return out
 
 
 def main():
@@ -217,6 +268,8 @@ def main():
                 "kick_total_bytes_fp64": total_b64,
                 "kick_total_GBs_fp64": round(total_b64 / (sum(prof["fp64"]) * 1e-3) / 1e9, 1)}
 
+    extra = None if a.no_extra else other_configs(torch, dev)
+
     cpu = None
     if a.cpu_sample > 0:
         cpu = cpu_baseline(a.workload, min(a.cpu_sample, n))
@@ -238,6 +291,7 @@ def main():
         "stages_ms": stages,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "other_configs": extra,
         "gpu_over_cpu": None if cpu is None else round(value / cpu["value"], 1),
     }
     print(json.dumps(out), flush=True)

@@ -101,6 +101,30 @@ int ocsc_track_drift(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double
                      const int32_t nmesh[3], double E_GeV, double zstep, double L, int nsteps, int precision,
                      int use_graph, void* stream);
 
+/* One bunch of a batched call (fields as in ocsc_apply). */
+typedef struct ocsc_bunch {
+  double* rparticles;
+  int64_t ld;
+  const double* q_array;
+  int64_t n;
+  int32_t nmesh[3];
+  double E_GeV;
+  double zstep;
+  int32_t random_mesh;
+  double mesh_scale;
+  double mesh_shift;
+} ocsc_bunch;
+
+/* nkicks rounds of SpaceCharge.apply over nb independent bunches of uneven
+ * size, mesh and energy (each round kicks every bunch once).  Bunches are
+ * spread over up to 8 workspaces with their own streams so small bunches run
+ * concurrently; with use_graph != 0 one round is captured in a HIP graph and
+ * replayed.  Per bunch the result equals ocsc_apply.  random_mesh requires
+ * nkicks == 1.  Synchronises at the end to report input errors.  No
+ * reference counterpart beyond calling apply per bunch. */
+int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int precision, int nkicks, int use_graph,
+                       void* stream);
+
 /* Tuning options of a context (the autotuner's knobs; no reference
  * counterpart).  "deposit_xcd": 0 = one charge grid, device-scope atomics;
  * 1 = one grid copy per XCD with atomics that complete in that XCD's L2,

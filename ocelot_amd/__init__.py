@@ -8,6 +8,6 @@ does, and raises if it is missing.
 """
 from .physproc import PhysProc
 from .beam import ParticleArray, DeviceParticleArray
-from .sc import SpaceCharge, drift, track_drift
+from .sc import SpaceCharge, apply_batched, drift, track_drift
 
-__all__ = ["PhysProc", "ParticleArray", "DeviceParticleArray", "SpaceCharge", "drift", "track_drift"]
+__all__ = ["PhysProc", "ParticleArray", "DeviceParticleArray", "SpaceCharge", "apply_batched", "drift", "track_drift"]

@@ -39,6 +39,7 @@ SIGNATURES = {
     "ocsc_drift": (_c_int, [_vp, _vp, _i64, _i64, _dbl, _dbl, _vp]),
     "ocsc_track_drift": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int, _c_int, _vp]),
     "ocsc_last_stats": (_c_int, [_vp, _dp, _c_int, _vp]),
+    "ocsc_apply_batched": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "ocsc_set_option": (_c_int, [_vp, ctypes.c_char_p, _i64]),
     "ocsc_get_option": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_i64)]),
     "ocsc_set_profiling": (_c_int, [_vp, _c_int]),
@@ -84,6 +85,13 @@ def check(rc):
     if rc == OCSC_EINVAL:
         raise ValueError("invalid argument: " + msg)
     raise OcscError(f"ocsc error {rc}: {msg}")
+
+
+class Bunch(ctypes.Structure):
+    """ocsc_bunch of include/ocsc.h"""
+    _fields_ = [("rparticles", _vp), ("ld", _i64), ("q_array", _vp), ("n", _i64),
+                ("nmesh", ctypes.c_int32 * 3), ("E_GeV", _dbl), ("zstep", _dbl),
+                ("random_mesh", ctypes.c_int32), ("mesh_scale", _dbl), ("mesh_shift", _dbl)]
 
 
 def i32x3(v):

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 #include "fft.hpp"
@@ -137,18 +138,29 @@ enum Stage : int {
 const char* const kStageNames[kNumStages] = {"frame", "extent", "deposit", "green", "fft_z", "fft_y",
                                              "conv_x", "ifft_y", "ifft_z", "kick"};
 
-struct ocsc_ctx {
-  int device = 0;
+// Workspace of one kick in flight: device scalars, partials, charge grid,
+// FFT scratch, potential / fields, kernel spectrum (with its cache key in
+// the scalars).  A context owns lane 0; batched calls add lanes so that
+// independent bunches can be kicked concurrently on their own streams.
+struct Lane {
   KickScalars* sc = nullptr;       // device
   KickScalars* sc_host = nullptr;  // pinned mirror
   Buf part;                        // per-block partial sums
+  Buf grid, A, B, phi, G;
+  hipStream_t stream = nullptr;    // extra lanes only
+  hipEvent_t done = nullptr;       // extra lanes only
+};
+
+struct ocsc_ctx {
+  int device = 0;
+  Lane lane0;
+  std::vector<Lane*> lanes;        // extra lanes for batched calls
   cpx<float>* twf = nullptr;       // W_M^k for M at offset M (M = 8..1024)
   cpx<double>* twd = nullptr;
   hipStream_t cap_stream = nullptr;  // stream used for graph capture
   hipStream_t side = nullptr;        // second stream: kernel spectrum next to the deposit
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int overlap = 0;                   // 1: build the kernel spectrum concurrently with the deposit
-  Buf grid, A, B, phi, G;
   // stage timing (ocsc_set_profiling): events between the stages of a kick
   int profiling = 0;
   // tuning options (ocsc_set_option)
@@ -188,10 +200,10 @@ int make_geometry(const int32_t* nmesh, Geometry& g) {
 }
 
 template <typename F>
-int ensure_mesh(ocsc_ctx* c, const Geometry& g, int nword) {
+int ensure_mesh(ocsc_ctx* c, Lane& L, const Geometry& g, int nword) {
   const MeshDims& d = g.d;
   int rc;
-  if ((rc = ensure(c->grid, (size_t)(c->deposit_xcd ? kNumXcd : 1) * nword * g.ncell * 8))) return rc;
+  if ((rc = ensure(L.grid, (size_t)(c->deposit_xcd ? kNumXcd : 1) * nword * g.ncell * 8))) return rc;
   size_t a = (size_t)d.nx * d.ny * d.hz * sizeof(cpx<F>);
   size_t b = (size_t)d.nx * d.my * d.hz * sizeof(cpx<F>);
   // the kernel-spectrum build borrows A and B as FP64 scratch
@@ -202,10 +214,10 @@ int ensure_mesh(ocsc_ctx* c, const Geometry& g, int nword) {
     a = std::max(a, 3 * (size_t)d.nx * d.ny * d.hz * sizeof(cpx<float>));
     b = std::max(b, 2 * (size_t)d.nx * d.my * d.hz * sizeof(cpx<float>));
   }
-  if ((rc = ensure(c->A, a))) return rc;
-  if ((rc = ensure(c->B, b))) return rc;
-  if ((rc = ensure(c->phi, (size_t)g.ncell * std::max(sizeof(double), 3 * sizeof(float))))) return rc;
-  if ((rc = ensure(c->G, (size_t)d.hx * d.hy * d.hz * sizeof(double)))) return rc;
+  if ((rc = ensure(L.A, a))) return rc;
+  if ((rc = ensure(L.B, b))) return rc;
+  if ((rc = ensure(L.phi, (size_t)g.ncell * std::max(sizeof(double), 3 * sizeof(float))))) return rc;
+  if ((rc = ensure(L.G, (size_t)d.hx * d.hy * d.hz * sizeof(double)))) return rc;
   return OCSC_OK;
 }
 
@@ -239,25 +251,25 @@ const cpx<F>* tw_for(ocsc_ctx* c, int M) {
 // kernel spectrum G from the current cell sizes (skipped on device when the
 // cache key matches, unless force)
 template <typename F>
-int launch_kernel_spectrum(ocsc_ctx* c, const Geometry& g, hipStream_t st, int force) {
+int launch_kernel_spectrum(ocsc_ctx* c, Lane& L, const Geometry& g, hipStream_t st, int force) {
   const MeshDims& d = g.d;
-  double* K = (double*)c->B.p;
-  double* Kz = (double*)c->A.p;
-  double* Kzy = (double*)c->B.p;
+  double* K = (double*)L.B.p;
+  double* Kz = (double*)L.A.p;
+  double* Kzy = (double*)L.B.p;
   const int nb_igf = ((d.nx + kIgfTx - 1) / kIgfTx) * ((d.ny + kIgfTy - 1) / kIgfTy) * ((d.nz + kIgfTz - 1) / kIgfTz);
-  k_igf<<<nb_igf, 256, 0, st>>>(K, d, c->sc, force);
+  k_igf<<<nb_igf, 256, 0, st>>>(K, d, L.sc, force);
   LAUNCHCHK("k_igf");
   const int64_t R = (int64_t)d.nx * d.ny;
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<double, MM>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zcos<MM><<<nb, kNT, TL::lds_bytes, st>>>(K, Kz, d, c->sc, c->twd + MM, force);
+    k_zcos<MM><<<nb, kNT, TL::lds_bytes, st>>>(K, Kz, d, L.sc, c->twd + MM, force);
   });
   LAUNCHCHK("k_zcos");
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<double, MM>;
     dim3 grid((d.hz + 2 * TL::T - 1) / (2 * TL::T), d.nx);
-    k_scos<MM, double><<<grid, kNT, TL::lds_bytes, st>>>(Kz, Kzy, d.ny, d.hz, d.hy, c->sc, c->twd + MM, force, 0, 1.0);
+    k_scos<MM, double><<<grid, kNT, TL::lds_bytes, st>>>(Kz, Kzy, d.ny, d.hz, d.hy, L.sc, c->twd + MM, force, 0, 1.0);
   });
   LAUNCHCHK("k_scos(y)");
   const double inv_m3 = 1.0 / ((double)d.mx * (double)d.my * (double)d.mz);
@@ -265,7 +277,7 @@ int launch_kernel_spectrum(ocsc_ctx* c, const Geometry& g, hipStream_t st, int f
     using TL = Tile<double, MM>;
     const int inner = d.hy * d.hz;
     dim3 grid((inner + 2 * TL::T - 1) / (2 * TL::T), 1);
-    k_scos<MM, F><<<grid, kNT, TL::lds_bytes, st>>>(Kzy, (F*)c->G.p, d.nx, inner, d.hx, c->sc, c->twd + MM, force, 1,
+    k_scos<MM, F><<<grid, kNT, TL::lds_bytes, st>>>(Kzy, (F*)L.G.p, d.nx, inner, d.hx, L.sc, c->twd + MM, force, 1,
                                                      inv_m3);
   });
   LAUNCHCHK("k_scos(x)");
@@ -274,17 +286,17 @@ int launch_kernel_spectrum(ocsc_ctx* c, const Geometry& g, hipStream_t st, int f
 
 // phi = G * rho on the doubled grid; rho read through charge_at<CIN>
 template <typename F, int CIN>
-int launch_convolution(ocsc_ctx* c, const Geometry& g, const void* rho, F* phi, hipStream_t st,
+int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, F* phi, hipStream_t st,
                        hipEvent_t* ev = nullptr) {
   // ev[0] is recorded by the caller before the z pass; ev[1..4] after z, y, x, y
   const MeshDims& d = g.d;
-  cpx<F>* A = (cpx<F>*)c->A.p;
-  cpx<F>* B = (cpx<F>*)c->B.p;
+  cpx<F>* A = (cpx<F>*)L.A.p;
+  cpx<F>* B = (cpx<F>*)L.B.p;
   const int64_t R = (int64_t)d.nx * d.ny;
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<F, MM>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zfwd<F, MM, CIN><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, c->sc, tw_for<F>(c, MM),
+    k_zfwd<F, MM, CIN><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, L.sc, tw_for<F>(c, MM),
                                                         c->deposit_xcd ? kNumXcd : 1);
   });
   LAUNCHCHK("k_zfwd");
@@ -292,28 +304,28 @@ int launch_convolution(ocsc_ctx* c, const Geometry& g, const void* rho, F* phi, 
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<F, MM>;
     dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
-    k_yfwd<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, c->sc, tw_for<F>(c, MM));
+    k_yfwd<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, L.sc, tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_yfwd");
   if (ev) (void)hipEventRecord(ev[2], st);
   OCSC_SWITCH_M(d.mx, {
     using TL = Tile<F, MM>;
     dim3 grid((d.hz + TL::T - 1) / TL::T, d.my);
-    k_xconv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, (const F*)c->G.p, d, c->sc, tw_for<F>(c, MM));
+    k_xconv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, (const F*)L.G.p, d, L.sc, tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_xconv");
   if (ev) (void)hipEventRecord(ev[3], st);
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<F, MM>;
     dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
-    k_yinv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, A, d, c->sc, tw_for<F>(c, MM));
+    k_yinv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, A, d, L.sc, tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_yinv");
   if (ev) (void)hipEventRecord(ev[4], st);
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<F, MM>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zinv<F, MM><<<nb, kNT, TL::lds_bytes, st>>>(A, phi, d, c->sc, tw_for<F>(c, MM));
+    k_zinv<F, MM><<<nb, kNT, TL::lds_bytes, st>>>(A, phi, d, L.sc, tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_zinv");
   return OCSC_OK;
@@ -322,18 +334,18 @@ int launch_convolution(ocsc_ctx* c, const Geometry& g, const void* rho, F* phi, 
 // Mixed form: the three field components on the mesh, E = -grad phi as the
 // forward differences of the reference (sc.py:195-200) taken in Fourier
 // space, written as three FP32 grids one after the other at E.
-int launch_fields(ocsc_ctx* c, const Geometry& g, const void* rho, float* E, hipStream_t st,
+int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, float* E, hipStream_t st,
                   hipEvent_t* ev = nullptr) {
   const MeshDims& d = g.d;
   const int64_t nyz = (int64_t)d.nx * d.ny * d.hz;
-  cpx<float>* A = (cpx<float>*)c->A.p;
-  cpx<float>* B = (cpx<float>*)c->B.p;
+  cpx<float>* A = (cpx<float>*)L.A.p;
+  cpx<float>* B = (cpx<float>*)L.B.p;
   cpx<float>* B2 = B + (int64_t)d.nx * d.my * d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<float, MM>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zfwd<float, MM, kChargeFix1><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, c->sc, c->twf + MM,
+    k_zfwd<float, MM, kChargeFix1><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, L.sc, c->twf + MM,
                                                                     c->deposit_xcd ? kNumXcd : 1);
   });
   LAUNCHCHK("k_zfwd");
@@ -341,14 +353,14 @@ int launch_fields(ocsc_ctx* c, const Geometry& g, const void* rho, float* E, hip
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<float, MM>;
     dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
-    k_yfwd<float, MM><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, c->sc, c->twf + MM);
+    k_yfwd<float, MM><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, L.sc, c->twf + MM);
   });
   LAUNCHCHK("k_yfwd");
   if (ev) (void)hipEventRecord(ev[2], st);
   OCSC_SWITCH_M(d.mx, {
     using FT = FieldTile<MM>;
     dim3 grid((d.hz + FT::T2 - 1) / FT::T2, d.my);
-    k_xconv_field<MM><<<grid, kNT, FT::lds2, st>>>(B, B, B2, (const float*)c->G.p, d, c->sc, c->twf + MM,
+    k_xconv_field<MM><<<grid, kNT, FT::lds2, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
                                                    c->twd + MM);
   });
   LAUNCHCHK("k_xconv_field");
@@ -356,7 +368,7 @@ int launch_fields(ocsc_ctx* c, const Geometry& g, const void* rho, float* E, hip
   OCSC_SWITCH_M(d.my, {
     using FT = FieldTile<MM>;
     dim3 grid((d.hz + FT::T3 - 1) / FT::T3, d.nx);
-    k_yinv_field<MM><<<grid, kNT, FT::lds3, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, c->sc, c->twf + MM,
+    k_yinv_field<MM><<<grid, kNT, FT::lds3, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
                                                   c->twd + MM);
   });
   LAUNCHCHK("k_yinv_field");
@@ -364,9 +376,9 @@ int launch_fields(ocsc_ctx* c, const Geometry& g, const void* rho, float* E, hip
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<float, MM>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zinv<float, MM, false><<<nb, kNT, TL::lds_bytes, st>>>(A, E, d, c->sc, c->twf + MM);
-    k_zinv<float, MM, false><<<nb, kNT, TL::lds_bytes, st>>>(A + nyz, E + g.ncell, d, c->sc, c->twf + MM);
-    k_zinv<float, MM, true><<<nb, kNT, TL::lds_bytes, st>>>(A + 2 * nyz, E + 2 * g.ncell, d, c->sc, c->twf + MM,
+    k_zinv<float, MM, false><<<nb, kNT, TL::lds_bytes, st>>>(A, E, d, L.sc, c->twf + MM);
+    k_zinv<float, MM, false><<<nb, kNT, TL::lds_bytes, st>>>(A + nyz, E + g.ncell, d, L.sc, c->twf + MM);
+    k_zinv<float, MM, true><<<nb, kNT, TL::lds_bytes, st>>>(A + 2 * nyz, E + 2 * g.ncell, d, L.sc, c->twf + MM,
                                                              c->twd + MM);
   });
   LAUNCHCHK("k_zinv(fields)");
@@ -407,12 +419,12 @@ struct StepOpts {
 };
 
 template <typename F, bool DRIFT, bool SUMP>
-void launch_kick(ocsc_ctx* c, int npart, double* r, int64_t ld, const StepOpts& o, hipStream_t st) {
-  k_kick<F, DRIFT, SUMP><<<npart, kPT, 0, st>>>(r, ld, (const F*)c->phi.p, c->sc, o.dr, (double*)c->part.p);
+void launch_kick(ocsc_ctx* c, Lane& L, int npart, double* r, int64_t ld, const StepOpts& o, hipStream_t st) {
+  k_kick<F, DRIFT, SUMP><<<npart, kPT, 0, st>>>(r, ld, (const F*)L.phi.p, L.sc, o.dr, (double*)L.part.p);
 }
 
 template <typename F, int NWORD>
-int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const double* q, const KickInputs& in,
+int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld, const double* q, const KickInputs& in,
                  int prec, hipStream_t st, const StepOpts& o = StepOpts()) {
   const int64_t n = in.n;
   const int nblk = nred_blocks(n);
@@ -423,20 +435,20 @@ int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const do
     if (prof) (void)hipEventRecord(c->ev[i], st);
   };
   mark(0);
-  k_setup<<<1, 64, 0, st>>>(c->sc, in);
+  k_setup<<<1, 64, 0, st>>>(L.sc, in);
   LAUNCHCHK("k_setup");
   if (o.passA) {
-    k_passA<<<nblk, kPT, 0, st>>>(R, c->sc, (double*)c->part.p);
-    k_reduceA<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk);
+    k_passA<<<nblk, kPT, 0, st>>>(R, L.sc, (double*)L.part.p);
+    k_reduceA<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk);
   } else {
-    k_reduceA<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, o.prev_nblk);
+    k_reduceA<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, o.prev_nblk);
   }
   mark(1);
   if (prec == OCSC_MIXED)
-    k_passB<true><<<nblk, kPT, 0, st>>>(R, q, c->sc, (double*)c->part.p);
+    k_passB<true><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p);
   else
-    k_passB<false><<<nblk, kPT, 0, st>>>(R, q, c->sc, (double*)c->part.p);
-  k_reduceB<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk, prec);
+    k_passB<false><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p);
+  k_reduceB<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk, prec);
   LAUNCHCHK("frame/extent passes");
   mark(2);
   int rc;
@@ -447,39 +459,39 @@ int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const do
   if (fork) {
     HIPCHK(hipEventRecord(c->ev_fork, st));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-    if ((rc = launch_kernel_spectrum<F>(c, g, c->side, 0))) return rc;
+    if ((rc = launch_kernel_spectrum<F>(c, L, g, c->side, 0))) return rc;
     HIPCHK(hipEventRecord(c->ev_join, c->side));
   }
   const int64_t nz64 = (c->deposit_xcd ? kNumXcd : 1) * NWORD * g.ncell;
-  k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, st>>>((unsigned long long*)c->grid.p, nz64);
+  k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, st>>>((unsigned long long*)L.grid.p, nz64);
   const int ndep = (int)((n + kPT * kDepItems - 1) / (kPT * kDepItems));
   if (c->deposit_xcd)
-    k_deposit<NWORD, true><<<ndep, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
+    k_deposit<NWORD, true><<<ndep, kPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
   else
-    k_deposit<NWORD, false><<<ndep, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
+    k_deposit<NWORD, false><<<ndep, kPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
   LAUNCHCHK("deposit");
   mark(3);
   if (fork)
     HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
-  else if ((rc = launch_kernel_spectrum<F>(c, g, st, 0)))
+  else if ((rc = launch_kernel_spectrum<F>(c, L, g, st, 0)))
     return rc;
   mark(4);
   if constexpr (sizeof(F) == 4) {
-    if ((rc = launch_fields(c, g, c->grid.p, (float*)c->phi.p, st, prof ? c->ev + 4 : nullptr))) return rc;
+    if ((rc = launch_fields(c, L, g, L.grid.p, (float*)L.phi.p, st, prof ? c->ev + 4 : nullptr))) return rc;
   } else {
-    if ((rc = launch_convolution<F, NWORD == 2 ? kChargeFix2 : kChargeFix1>(c, g, c->grid.p, (F*)c->phi.p, st,
+    if ((rc = launch_convolution<F, NWORD == 2 ? kChargeFix2 : kChargeFix1>(c, L, g, L.grid.p, (F*)L.phi.p, st,
                                                                               prof ? c->ev + 4 : nullptr)))
       return rc;
   }
   mark(9);
   if (o.drift && o.sump)
-    launch_kick<F, true, true>(c, npart, r, ld, o, st);
+    launch_kick<F, true, true>(c, L, npart, r, ld, o, st);
   else if (o.drift)
-    launch_kick<F, true, false>(c, npart, r, ld, o, st);
+    launch_kick<F, true, false>(c, L, npart, r, ld, o, st);
   else if (o.sump)
-    launch_kick<F, false, true>(c, npart, r, ld, o, st);
+    launch_kick<F, false, true>(c, L, npart, r, ld, o, st);
   else
-    launch_kick<F, false, false>(c, npart, r, ld, o, st);
+    launch_kick<F, false, false>(c, L, npart, r, ld, o, st);
   LAUNCHCHK("k_kick");
   mark(10);
   if (prof) {
@@ -494,20 +506,20 @@ int enqueue_kick(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const do
   return OCSC_OK;
 }
 
-int enqueue_kick_prec(ocsc_ctx* c, const Geometry& g, double* r, int64_t ld, const double* q, const KickInputs& in,
+int enqueue_kick_prec(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld, const double* q, const KickInputs& in,
                       int prec, hipStream_t st, const StepOpts& o = StepOpts()) {
-  if (prec == OCSC_FP64) return enqueue_kick<double, 2>(c, g, r, ld, q, in, prec, st, o);
-  return enqueue_kick<float, 1>(c, g, r, ld, q, in, prec, st, o);
+  if (prec == OCSC_FP64) return enqueue_kick<double, 2>(c, L, g, r, ld, q, in, prec, st, o);
+  return enqueue_kick<float, 1>(c, L, g, r, ld, q, in, prec, st, o);
 }
 
-int ensure_part(ocsc_ctx* c, int64_t n) {
+int ensure_part(ocsc_ctx* c, Lane& L, int64_t n) {
   const size_t need = std::max<size_t>((size_t)kMaxRedBlocks * kNPartB, (size_t)3 * ((n + kPT - 1) / kPT));
-  return ensure(c->part, need * sizeof(double));
+  return ensure(L.part, need * sizeof(double));
 }
 
-int ensure_prec(ocsc_ctx* c, const Geometry& g, int prec) {
-  if (prec == OCSC_FP64) return ensure_mesh<double>(c, g, 2);
-  return ensure_mesh<float>(c, g, 1);
+int ensure_prec(ocsc_ctx* c, Lane& L, const Geometry& g, int prec) {
+  if (prec == OCSC_FP64) return ensure_mesh<double>(c, L, g, 2);
+  return ensure_mesh<float>(c, L, g, 1);
 }
 
 __global__ void k_clear_err(KickScalars* __restrict__ s) {
@@ -519,12 +531,12 @@ __global__ void k_clear_err(KickScalars* __restrict__ s) {
 
 // error bits of the kicks since the last report (current kick and earlier
 // deferred ones); reported bits are cleared on the device
-int read_errors(ocsc_ctx* c, hipStream_t st, int32_t* err) {
-  HIPCHK(hipMemcpyAsync(c->sc_host, c->sc, sizeof(KickScalars), hipMemcpyDeviceToHost, st));
+int read_errors(ocsc_ctx* c, Lane& L, hipStream_t st, int32_t* err) {
+  HIPCHK(hipMemcpyAsync(L.sc_host, L.sc, sizeof(KickScalars), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  *err = c->sc_host->err | c->sc_host->err_sticky;
+  *err = L.sc_host->err | L.sc_host->err_sticky;
   if (*err) {
-    k_clear_err<<<1, 64, 0, st>>>(c->sc);
+    k_clear_err<<<1, 64, 0, st>>>(L.sc);
     LAUNCHCHK("k_clear_err");
     HIPCHK(hipStreamSynchronize(st));
   }
@@ -551,6 +563,34 @@ int check_common(ocsc_ctx* c, const void* r, const double* q, int64_t n, int64_t
 
 }  // namespace
 
+namespace {
+
+constexpr int kMaxLanes = 8;
+
+int init_lane(Lane& L, bool own_stream) {
+  if (hipMalloc(&L.sc, sizeof(KickScalars)) != hipSuccess) return fail(OCSC_ENOMEM, "scalars");
+  if (hipHostMalloc(&L.sc_host, sizeof(KickScalars)) != hipSuccess) return fail(OCSC_ENOMEM, "pinned");
+  KickScalars init{};
+  init.key_prec = -1;
+  HIPCHK(hipMemcpy(L.sc, &init, sizeof(init), hipMemcpyHostToDevice));
+  if (own_stream) {
+    HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+  }
+  return OCSC_OK;
+}
+
+void free_lane(Lane& L) {
+  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G})
+    if (b->p) (void)hipFree(b->p);
+  if (L.sc) (void)hipFree(L.sc);
+  if (L.sc_host) (void)hipHostFree(L.sc_host);
+  if (L.stream) (void)hipStreamDestroy(L.stream);
+  if (L.done) (void)hipEventDestroy(L.done);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* ocsc_last_error(void) { return g_err.c_str(); }
@@ -561,12 +601,13 @@ int ocsc_create(ocsc_ctx** out, int device) {
   HIPCHK(hipSetDevice(device));
   ocsc_ctx* c = new ocsc_ctx();
   c->device = device;
+  Lane& L = c->lane0;
   auto cleanup = [&](int rc) {
     ocsc_destroy(c);
     return rc;
   };
-  if (hipMalloc(&c->sc, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "scalars"));
-  if (hipHostMalloc(&c->sc_host, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "pinned"));
+  if (hipMalloc(&L.sc, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "scalars"));
+  if (hipHostMalloc(&L.sc_host, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "pinned"));
   if (hipMalloc(&c->twf, sizeof(cpx<float>) * 2048) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "twiddles"));
   if (hipMalloc(&c->twd, sizeof(cpx<double>) * 2048) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "twiddles"));
   // W_M^k = exp(-2 pi i k / M), from the first octant by symmetry so every
@@ -587,7 +628,7 @@ int ocsc_create(ocsc_ctx** out, int device) {
   if (hipMemcpy(c->twf, hf, sizeof(hf), hipMemcpyHostToDevice) != hipSuccess) return cleanup(fail(OCSC_EHIP, "copy"));
   KickScalars init{};
   init.key_prec = -1;
-  if (hipMemcpy(c->sc, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpy(L.sc, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(OCSC_EHIP, "copy"));
   if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(OCSC_EHIP, "stream"));
@@ -603,13 +644,14 @@ int ocsc_create(ocsc_ctx** out, int device) {
 
 int ocsc_destroy(ocsc_ctx* c) {
   if (!c) return OCSC_OK;
+  Lane& L = c->lane0;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
-  for (Buf* b : {&c->part, &c->grid, &c->A, &c->B, &c->phi, &c->G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
+  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
                  &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp})
     if (b->p) (void)hipFree(b->p);
-  if (c->sc) (void)hipFree(c->sc);
-  if (c->sc_host) (void)hipHostFree(c->sc_host);
+  if (L.sc) (void)hipFree(L.sc);
+  if (L.sc_host) (void)hipHostFree(L.sc_host);
   if (c->twf) (void)hipFree(c->twf);
   if (c->twd) (void)hipFree(c->twd);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
@@ -618,6 +660,10 @@ int ocsc_destroy(ocsc_ctx* c) {
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   for (int i = 0; i <= kNumStages; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  for (Lane* l : c->lanes) {
+    free_lane(*l);
+    delete l;
+  }
   delete c;
   return OCSC_OK;
 }
@@ -632,16 +678,17 @@ int ocsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   if (prec != OCSC_FP64 && prec != OCSC_MIXED) return fail(OCSC_EINVAL, "precision must be OCSC_FP64 or OCSC_MIXED");
   if (!(E_GeV > 0) || !std::isfinite(zstep)) return fail(OCSC_EINVAL, "E_GeV must be > 0 and zstep finite");
   HIPCHK(hipSetDevice(c->device));
+  Lane& L = c->lane0;
   Geometry g;
   if ((rc = make_geometry(nmesh, g))) return rc;
-  if ((rc = ensure_prec(c, g, prec))) return rc;
-  if ((rc = ensure_part(c, n))) return rc;
+  if ((rc = ensure_prec(c, L, g, prec))) return rc;
+  if ((rc = ensure_part(c, L, n))) return rc;
   hipStream_t st = (hipStream_t)stream;
   KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, random_mesh, mesh_scale, mesh_shift);
-  if ((rc = enqueue_kick_prec(c, g, r, ld, q, in, prec, st))) return rc;
+  if ((rc = enqueue_kick_prec(c, L, g, r, ld, q, in, prec, st))) return rc;
   if (sync) {
     int32_t err = 0;
-    if ((rc = read_errors(c, st, &err))) return rc;
+    if ((rc = read_errors(c, L, st, &err))) return rc;
     if (err) return value_error(err);
   }
   return OCSC_OK;
@@ -650,9 +697,10 @@ int ocsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
 int ocsc_check(ocsc_ctx* c, void* stream) {
   if (!c) return fail(OCSC_EINVAL, "null context");
   HIPCHK(hipSetDevice(c->device));
+  Lane& L = c->lane0;
   int32_t err = 0;
   int rc;
-  if ((rc = read_errors(c, (hipStream_t)stream, &err))) return rc;
+  if ((rc = read_errors(c, L, (hipStream_t)stream, &err))) return rc;
   if (err) return value_error(err);
   return OCSC_OK;
 }
@@ -665,10 +713,11 @@ int ocsc_el_field(ocsc_ctx* c, const double* xyz, int64_t ld, const double* q, i
   if (!nmesh || !E || lde < n) return fail(OCSC_EINVAL, "bad nmesh / Exyz / lde");
   if (prec != OCSC_FP64 && prec != OCSC_MIXED) return fail(OCSC_EINVAL, "bad precision");
   HIPCHK(hipSetDevice(c->device));
+  Lane& L = c->lane0;
   Geometry g;
   if ((rc = make_geometry(nmesh, g))) return rc;
-  if ((rc = ensure_prec(c, g, prec))) return rc;
-  if ((rc = ensure_part(c, n))) return rc;
+  if ((rc = ensure_prec(c, L, g, prec))) return rc;
+  if ((rc = ensure_part(c, L, n))) return rc;
   hipStream_t st = (hipStream_t)stream;
   KickInputs in = kick_inputs(nmesh, n, 1.0, 1.0, random_mesh, mesh_scale, mesh_shift);
   in.mode = 1;
@@ -682,27 +731,27 @@ int ocsc_el_field(ocsc_ctx* c, const double* xyz, int64_t ld, const double* q, i
   const int nblk = nred_blocks(n);
   const int npart = (int)((n + kPT - 1) / kPT);
   Rows R{xyz, ld};
-  k_setup<<<1, 64, 0, st>>>(c->sc, in);
-  k_passB_xyz<<<nblk, kPT, 0, st>>>(R, q, c->sc, (double*)c->part.p);
-  k_reduceB<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk, prec);
+  k_setup<<<1, 64, 0, st>>>(L.sc, in);
+  k_passB_xyz<<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p);
+  k_reduceB<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk, prec);
   const int nword = prec == OCSC_FP64 ? 2 : 1;
   const int64_t nz64 = nword * g.ncell;
-  k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 4096), 256, 0, st>>>((unsigned long long*)c->grid.p, nz64);
+  k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 4096), 256, 0, st>>>((unsigned long long*)L.grid.p, nz64);
   LAUNCHCHK("el_field setup");
   if (prec == OCSC_FP64) {
-    k_deposit_xyz<2><<<npart, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
-    if ((rc = launch_kernel_spectrum<double>(c, g, st, 0))) return rc;
-    if ((rc = launch_convolution<double, kChargeFix2>(c, g, c->grid.p, (double*)c->phi.p, st))) return rc;
-    k_gather_xyz<double><<<npart, kPT, 0, st>>>(R, (const double*)c->phi.p, c->sc, E, lde, cells);
+    k_deposit_xyz<2><<<npart, kPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
+    if ((rc = launch_kernel_spectrum<double>(c, L, g, st, 0))) return rc;
+    if ((rc = launch_convolution<double, kChargeFix2>(c, L, g, L.grid.p, (double*)L.phi.p, st))) return rc;
+    k_gather_xyz<double><<<npart, kPT, 0, st>>>(R, (const double*)L.phi.p, L.sc, E, lde, cells);
   } else {
-    k_deposit_xyz<1><<<npart, kPT, 0, st>>>(R, q, c->sc, (unsigned long long*)c->grid.p);
-    if ((rc = launch_kernel_spectrum<float>(c, g, st, 0))) return rc;
-    if ((rc = launch_fields(c, g, c->grid.p, (float*)c->phi.p, st))) return rc;
-    k_gather_xyz<float><<<npart, kPT, 0, st>>>(R, (const float*)c->phi.p, c->sc, E, lde, cells);
+    k_deposit_xyz<1><<<npart, kPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
+    if ((rc = launch_kernel_spectrum<float>(c, L, g, st, 0))) return rc;
+    if ((rc = launch_fields(c, L, g, L.grid.p, (float*)L.phi.p, st))) return rc;
+    k_gather_xyz<float><<<npart, kPT, 0, st>>>(R, (const float*)L.phi.p, L.sc, E, lde, cells);
   }
   LAUNCHCHK("el_field");
   int32_t err = 0;
-  if ((rc = read_errors(c, st, &err))) return rc;
+  if ((rc = read_errors(c, L, st, &err))) return rc;
   if (err) return value_error(err);
   return OCSC_OK;
 }
@@ -712,6 +761,7 @@ int ocsc_sym_kernel(ocsc_ctx* c, const int32_t nmesh[3], const double h[3], doub
   for (int k = 0; k < 3; ++k)
     if (nmesh[k] < 1 || nmesh[k] > 1024 || !(h[k] > 0)) return fail(OCSC_EINVAL, "bad shape or cell size");
   HIPCHK(hipSetDevice(c->device));
+  Lane& L = c->lane0;
   hipStream_t st = (hipStream_t)stream;
   KickInputs in{};
   in.mode = 2;
@@ -719,13 +769,13 @@ int ocsc_sym_kernel(ocsc_ctx* c, const int32_t nmesh[3], const double h[3], doub
     in.nmesh[k] = nmesh[k];
     in.steps[k] = h[k];
   }
-  k_setup<<<1, 64, 0, st>>>(c->sc, in);
+  k_setup<<<1, 64, 0, st>>>(L.sc, in);
   MeshDims d{};
   d.nx = nmesh[0];
   d.ny = nmesh[1];
   d.nz = nmesh[2];
   const int nb = ((d.nx + kIgfTx - 1) / kIgfTx) * ((d.ny + kIgfTy - 1) / kIgfTy) * ((d.nz + kIgfTz - 1) / kIgfTz);
-  k_igf<<<nb, 256, 0, st>>>(K, d, c->sc, 1);
+  k_igf<<<nb, 256, 0, st>>>(K, d, L.sc, 1);
   LAUNCHCHK("k_igf");
   HIPCHK(hipStreamSynchronize(st));
   return OCSC_OK;
@@ -739,9 +789,10 @@ int ocsc_potential(ocsc_ctx* c, const double* rho, const int32_t nmesh[3], const
     if (!(h[k] > 0)) return fail(OCSC_EINVAL, "cell sizes must be > 0");
   int rc;
   HIPCHK(hipSetDevice(c->device));
+  Lane& L = c->lane0;
   Geometry g;
   if ((rc = make_geometry(nmesh, g))) return rc;
-  if ((rc = ensure_prec(c, g, prec))) return rc;
+  if ((rc = ensure_prec(c, L, g, prec))) return rc;
   hipStream_t st = (hipStream_t)stream;
   KickInputs in{};
   in.mode = 2;
@@ -749,13 +800,13 @@ int ocsc_potential(ocsc_ctx* c, const double* rho, const int32_t nmesh[3], const
     in.nmesh[k] = nmesh[k];
     in.steps[k] = h[k];
   }
-  k_setup<<<1, 64, 0, st>>>(c->sc, in);
+  k_setup<<<1, 64, 0, st>>>(L.sc, in);
   if (prec == OCSC_FP64) {
-    if ((rc = launch_kernel_spectrum<double>(c, g, st, 1))) return rc;
-    if ((rc = launch_convolution<double, kChargeReal>(c, g, rho, (double*)phi, st))) return rc;
+    if ((rc = launch_kernel_spectrum<double>(c, L, g, st, 1))) return rc;
+    if ((rc = launch_convolution<double, kChargeReal>(c, L, g, rho, (double*)phi, st))) return rc;
   } else {
-    if ((rc = launch_kernel_spectrum<float>(c, g, st, 1))) return rc;
-    if ((rc = launch_convolution<float, kChargeReal>(c, g, rho, (float*)phi, st))) return rc;
+    if ((rc = launch_kernel_spectrum<float>(c, L, g, st, 1))) return rc;
+    if ((rc = launch_convolution<float, kChargeReal>(c, L, g, rho, (float*)phi, st))) return rc;
   }
   HIPCHK(hipStreamSynchronize(st));
   return OCSC_OK;
@@ -777,7 +828,7 @@ int ocsc_drift(ocsc_ctx* c, double* r, int64_t ld, int64_t n, double L, double E
 }
 
 int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
-                     double E_GeV, double zstep, double L, int nsteps, int prec, int use_graph, void* stream) {
+                     double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, void* stream) {
   int rc;
   if ((rc = check_common(c, r, q, n, ld))) return rc;
   if (!nmesh || nsteps < 0) return fail(OCSC_EINVAL, "bad nmesh / nsteps");
@@ -785,18 +836,19 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   if (!(E_GeV > 0)) return fail(OCSC_EINVAL, "E_GeV must be > 0");
   if (n > (int64_t)0xFFFFFFFF) return fail(OCSC_EINVAL, "tracking supports up to 2^32-1 particles");
   HIPCHK(hipSetDevice(c->device));
+  Lane& L = c->lane0;
   Geometry g;
   if ((rc = make_geometry(nmesh, g))) return rc;
-  if ((rc = ensure_prec(c, g, prec))) return rc;
+  if ((rc = ensure_prec(c, L, g, prec))) return rc;
   hipStream_t st = (hipStream_t)stream;
   KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, 0, 1.0, 0.0);
   const double gamma = E_GeV / kMeGeV;
   const double igamma2 = 1. / (gamma * gamma);
   const double beta = std::sqrt(1. - igamma2);
-  const double r56 = 0.0 - L / (beta * beta) * igamma2;
+  const double r56 = 0.0 - Ld / (beta * beta) * igamma2;
   const int nb = (int)((n + 255) / 256);
   if (nsteps == 0) return OCSC_OK;
-  if ((rc = ensure_part(c, n))) return rc;
+  if ((rc = ensure_part(c, L, n))) return rc;
 
   // cell-ordered resident copy (sort_every > 0)
   const bool sorted = c->sort_every > 0 && zstep != 0;
@@ -821,7 +873,7 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
     uint32_t* keys1 = (uint32_t*)c->keys[1].p;
     uint32_t* vin = (uint32_t*)c->vals.p;
     uint32_t* vout = vin + n;  // order of src
-    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, c->sc, keys0, vin);
+    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, L.sc, keys0, vin);
     LAUNCHCHK("k_sort_keys");
     size_t tb = c->sort_tmp.cap;
     HIPCHK(sort_pairs_u32(c->sort_tmp.p, &tb, keys0, keys1, vin, vout, n, end_bit, st));
@@ -841,11 +893,11 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   if (sorted) {
     // kick scalars of the current bunch for the first sort keys
     const int nblk = nred_blocks(n);
-    k_setup<<<1, 64, 0, st>>>(c->sc, in);
-    k_passA<<<nblk, kPT, 0, st>>>(Rows{r, ld}, c->sc, (double*)c->part.p);
-    k_reduceA<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk);
-    k_passB<false><<<nblk, kPT, 0, st>>>(Rows{r, ld}, q, c->sc, (double*)c->part.p);
-    k_reduceB<<<1, 1024, 0, st>>>(c->sc, (double*)c->part.p, nblk, prec);
+    k_setup<<<1, 64, 0, st>>>(L.sc, in);
+    k_passA<<<nblk, kPT, 0, st>>>(Rows{r, ld}, L.sc, (double*)L.part.p);
+    k_reduceA<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk);
+    k_passB<false><<<nblk, kPT, 0, st>>>(Rows{r, ld}, q, L.sc, (double*)L.part.p);
+    k_reduceB<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk, prec);
     LAUNCHCHK("initial frame");
     if ((rc = resort(r, ld, q, 0, nullptr))) return rc;
     rr = (double*)c->rs[0].p;
@@ -861,13 +913,13 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
     o.passA = step == 0;
     o.prev_nblk = npart;
     o.drift = o.sump = step < nsteps - 1;
-    o.dr = DriftArgs{L, r56};
+    o.dr = DriftArgs{Ld, r56};
     return o;
   };
   if (zstep == 0) {
-    for (int step = 0; step < nsteps; ++step) k_drift<<<nb, 256, 0, st>>>(rr, lr, n, L, r56);
+    for (int step = 0; step < nsteps; ++step) k_drift<<<nb, 256, 0, st>>>(rr, lr, n, Ld, r56);
   } else {
-    k_drift<<<nb, 256, 0, st>>>(rr, lr, n, L, r56);
+    k_drift<<<nb, 256, 0, st>>>(rr, lr, n, Ld, r56);
   }
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   hipGraph_t graphs[2] = {nullptr, nullptr};
@@ -890,7 +942,7 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
     const StepOpts o = opts_for(step);
     const bool middle = step > 0 && step < nsteps - 1;
     if (!use_graph || c->profiling || !middle) {
-      if ((rc = enqueue_kick_prec(c, g, rr, lr, qq, in, prec, st, o))) {
+      if ((rc = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, st, o))) {
         cleanup();
         return rc;
       }
@@ -899,7 +951,7 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
     const int gi = sorted ? cur : 0;
     if (!exec[gi]) {
       HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-      int rc2 = enqueue_kick_prec(c, g, rr, lr, qq, in, prec, c->cap_stream, o);
+      int rc2 = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, c->cap_stream, o);
       hipError_t e = hipStreamEndCapture(c->cap_stream, &graphs[gi]);
       if (rc2 || e != hipSuccess) {
         cleanup();
@@ -922,10 +974,107 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
     LAUNCHCHK("k_permute_scatter");
   }
   int32_t err = 0;
-  rc = read_errors(c, st, &err);
+  rc = read_errors(c, L, st, &err);
   cleanup();
   if (rc) return rc;
   if (err) return value_error(err);
+  return OCSC_OK;
+}
+
+int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int nkicks, int use_graph,
+                       void* stream) {
+  if (!c || (!b && nb > 0) || nb < 0 || nkicks < 0) return fail(OCSC_EINVAL, "bad arguments");
+  if (prec != OCSC_FP64 && prec != OCSC_MIXED) return fail(OCSC_EINVAL, "bad precision");
+  if (nb == 0 || nkicks == 0) return OCSC_OK;
+  int rc;
+  std::vector<Geometry> geo(nb);
+  std::vector<KickInputs> ins(nb);
+  for (int i = 0; i < nb; ++i) {
+    const ocsc_bunch& u = b[i];
+    if ((rc = check_common(c, u.rparticles, u.q_array, u.n, u.ld))) return rc;
+    if (!(u.E_GeV > 0) || !std::isfinite(u.zstep)) return fail(OCSC_EINVAL, "E_GeV must be > 0 and zstep finite");
+    if (u.random_mesh && nkicks > 1) return fail(OCSC_EINVAL, "random_mesh draws differ per kick: use nkicks = 1");
+    if ((rc = make_geometry(u.nmesh, geo[i]))) return rc;
+    ins[i] = kick_inputs(u.nmesh, u.n, u.E_GeV, u.zstep, u.random_mesh, u.mesh_scale, u.mesh_shift);
+  }
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  const int nl = nb < kMaxLanes ? nb : kMaxLanes;
+  while ((int)c->lanes.size() < nl) {
+    Lane* l = new Lane();
+    if ((rc = init_lane(*l, true))) {
+      free_lane(*l);
+      delete l;
+      return rc;
+    }
+    c->lanes.push_back(l);
+  }
+  for (int i = 0; i < nb; ++i) {
+    Lane& L = *c->lanes[i % nl];
+    if ((rc = ensure_prec(c, L, geo[i], prec))) return rc;
+    if ((rc = ensure_part(c, L, b[i].n))) return rc;
+  }
+  // one round: every bunch kicked once, bunch i on lane i % nl
+  const int prof = c->profiling, ovl = c->overlap;
+  c->profiling = 0;  // stage events and the side stream are per context
+  c->overlap = 0;
+  auto round = [&](hipStream_t origin) -> int {
+    HIPCHK(hipEventRecord(c->ev_fork, origin));
+    for (int l = 0; l < nl; ++l) HIPCHK(hipStreamWaitEvent(c->lanes[l]->stream, c->ev_fork, 0));
+    for (int i = 0; i < nb; ++i) {
+      Lane& L = *c->lanes[i % nl];
+      if (b[i].zstep == 0) continue;
+      int r2 = enqueue_kick_prec(c, L, geo[i], b[i].rparticles, b[i].ld, b[i].q_array, ins[i], prec, L.stream);
+      if (r2) return r2;
+    }
+    for (int l = 0; l < nl; ++l) {
+      HIPCHK(hipEventRecord(c->lanes[l]->done, c->lanes[l]->stream));
+      HIPCHK(hipStreamWaitEvent(origin, c->lanes[l]->done, 0));
+    }
+    return OCSC_OK;
+  };
+  auto restore = [&]() {
+    c->profiling = prof;
+    c->overlap = ovl;
+  };
+  if (!use_graph) {
+    for (int k = 0; k < nkicks; ++k)
+      if ((rc = round(st))) {
+        restore();
+        return rc;
+      }
+  } else {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      restore();
+      return fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
+    }
+    int r2 = round(c->cap_stream);
+    e = hipStreamEndCapture(c->cap_stream, &graph);
+    if (r2 || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      restore();
+      return r2 ? r2 : fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    for (int k = 0; k < nkicks && e == hipSuccess; ++k) e = hipGraphLaunch(exec, st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      restore();
+      return fail(OCSC_EHIP, std::string("batched graph: ") + hipGetErrorString(e));
+    }
+  }
+  restore();
+  int32_t err_all = 0;
+  for (int l = 0; l < nl; ++l) {
+    int32_t err = 0;
+    if ((rc = read_errors(c, *c->lanes[l], st, &err))) return rc;
+    err_all |= err;
+  }
+  if (err_all) return value_error(err_all);
   return OCSC_OK;
 }
 
@@ -991,10 +1140,11 @@ const char* ocsc_stage_name(int i) { return (i >= 0 && i < kNumStages) ? kStageN
 int ocsc_last_stats(ocsc_ctx* c, double* out, int nout, void* stream) {
   if (!c || (!out && nout > 0)) return fail(OCSC_EINVAL, "null argument");
   HIPCHK(hipSetDevice(c->device));
+  Lane& L = c->lane0;
   hipStream_t st = (hipStream_t)stream;
-  HIPCHK(hipMemcpyAsync(c->sc_host, c->sc, sizeof(KickScalars), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(L.sc_host, L.sc, sizeof(KickScalars), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  const KickScalars& s = *c->sc_host;
+  const KickScalars& s = *L.sc_host;
   double v[20];
   v[0] = s.gamma0;
   v[1] = s.Pav;

@@ -271,3 +271,42 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True):
                                         p_array.q_array.data_ptr(), n, sc._nmesh(), float(p_array.E), float(dz),
                                         float(L), int(nsteps), sc._prec(), 1 if use_graph else 0,
                                         torch.cuda.current_stream(dev).cuda_stream))
+
+
+def apply_batched(items, precision="fp64", nkicks=1, use_graph=True, random_mesh=False):
+    """Kick many independent device-resident bunches (uneven sizes, meshes and
+    energies) together, ``nkicks`` times each: ``items`` is a list of
+    ``(p_array, nmesh_xyz, zstep)`` with ``p_array`` a DeviceParticleArray on
+    one device.  Per bunch the result equals ``SpaceCharge(nmesh_xyz).apply``.
+    ``random_mesh`` takes the reference's two global draws per bunch (one
+    round only)."""
+    torch = _torch()
+    lib = _lib.load()
+    if not items:
+        return
+    prec = _PREC[precision]
+    dev = items[0][0].rparticles.device.index
+    arr = (_lib.Bunch * len(items))()
+    keep = []
+    for b, (p, nmesh, dz) in zip(arr, items):
+        r = p.rparticles
+        if not _is_cuda_tensor(r) or r.device.index != dev:
+            raise TypeError("apply_batched needs DeviceParticleArrays on one device")
+        if r.shape[1] == 0:
+            raise ValueError("zero-size array to reduction operation maximum which has no identity")
+        q = p.q_array.to(dtype=torch.float64).contiguous()
+        keep.append(q)
+        b.rparticles, b.ld, b.q_array, b.n = r.data_ptr(), r.stride(0), q.data_ptr(), r.shape[1]
+        for k, v in enumerate(np.array(nmesh).reshape(-1)):
+            b.nmesh[k] = int(v)
+        b.E_GeV, b.zstep = float(p.E), float(dz)
+        if random_mesh:
+            b.random_mesh = 1
+            b.mesh_scale = float(np.random.uniform(low=1, high=1.1))
+            b.mesh_shift = float(np.random.uniform(low=-0.5, high=0.5))
+        else:
+            b.random_mesh, b.mesh_scale, b.mesh_shift = 0, 1.0, 0.0
+    with torch.cuda.device(dev):
+        _lib.check(lib.ocsc_apply_batched(_lib.context(dev).handle, ctypes.cast(arr, ctypes.c_void_p), len(items),
+                                          prec, int(nkicks), 1 if use_graph else 0,
+                                          torch.cuda.current_stream(dev).cuda_stream))

@@ -17,7 +17,8 @@ Pinning (how each function is checked, see tests/test_oracle_*.py):
   xxstg_2_xp_mad / xp_2_xxstg_mad -- golden vectors written by the reference,
       round-trip identity and closed-form single-particle kinematics.
   sym_kernel   -- golden vectors; brute-force cubature of the cell integral of
-      1/r; scaling K(l*h) = l^2 K(h); far field ~ hx*hy*hz/r.
+      1/r; scaling K(l*h) = l^2 K(h); far field ~ hx*hy*hz/r; the
+      extended-precision form against cubature on elongated cells.
   potential    -- golden vectors; direct O(N^6) convolution sum on small grids.
   deposit      -- golden charge grids; brute-force per-particle NGP loop.
   el_field / apply -- golden vectors written by the reference on seeded bunches
@@ -88,24 +89,31 @@ def igf_primitive(x, y, z):
             - z * z * 0.5 * np.arctan(x * y / (z * r)) + x * y * np.log(z + r))
 
 
-def sym_kernel(ijk2, hxyz):
+def sym_kernel(ijk2, hxyz, dtype=np.float64):
     """Integrated Green's function on the first octant, shape ijk2.
     K[i,j,k] = integral of 1/r over the cell centred at (i*hx, j*hy, k*hz),
     evaluated as the 8-corner alternating difference of F at the cell faces
-    (i -/+ 1/2) h.  Follows ocelot/cpbd/sc.py:109-133."""
+    (i -/+ 1/2) h.  Follows ocelot/cpbd/sc.py:109-133.
+
+    ``dtype=np.longdouble`` evaluates F and the difference in x87 extended
+    precision (64-bit mantissa) and rounds K to float64 at the end: the
+    accuracy reference for meshes whose cells are far from cubic, where the
+    8-corner difference cancels most of F's digits (the reference's float64
+    result then carries errors of 1e-9 relative in the kick)."""
     nx, ny, nz = (int(v) for v in ijk2)
-    hx, hy, hz = (float(v) for v in hxyz)
-    xs = hx * np.arange(nx + 1) - hx / 2
-    ys = hy * np.arange(ny + 1) - hy / 2
-    zs = hz * np.arange(nz + 1) - hz / 2
+    hx, hy, hz = (dtype(float(v)) for v in hxyz)
+    xs = hx * np.arange(nx + 1, dtype=dtype) - hx / 2
+    ys = hy * np.arange(ny + 1, dtype=dtype) - hy / 2
+    zs = hz * np.arange(nz + 1, dtype=dtype) - hz / 2
     F = igf_primitive(*np.ix_(xs, ys, zs))
     hi_x, lo_x = slice(1, nx + 1), slice(0, nx)
     hi_y, lo_y = slice(1, ny + 1), slice(0, ny)
     hi_z, lo_z = slice(1, nz + 1), slice(0, nz)
-    return (F[hi_x, hi_y, hi_z] - F[lo_x, hi_y, hi_z]
-            - F[hi_x, lo_y, hi_z] + F[lo_x, lo_y, hi_z]
-            - F[hi_x, hi_y, lo_z] + F[lo_x, hi_y, lo_z]
-            + F[hi_x, lo_y, lo_z] - F[lo_x, lo_y, lo_z])
+    K = (F[hi_x, hi_y, hi_z] - F[lo_x, hi_y, hi_z]
+         - F[hi_x, lo_y, hi_z] + F[lo_x, lo_y, hi_z]
+         - F[hi_x, hi_y, lo_z] + F[lo_x, hi_y, lo_z]
+         + F[hi_x, lo_y, lo_z] - F[lo_x, lo_y, lo_z])
+    return K.astype(np.float64)
 
 
 def mirror_kernel(K):
@@ -121,15 +129,16 @@ def mirror_kernel(K):
     return K2
 
 
-def potential(q, steps):
+def potential(q, steps, kernel_dtype=np.float64):
     """Open-boundary potential of the charge grid q by Hockney's doubled-grid
     FFT convolution with the integrated Green's function, scaled by
-    1/(4 pi eps0 hx hy hz).  ocelot/cpbd/sc.py:135-168 (numpy.fft branch)."""
+    1/(4 pi eps0 hx hy hz).  ocelot/cpbd/sc.py:135-168 (numpy.fft branch).
+    ``kernel_dtype`` as in sym_kernel (float64 = the reference)."""
     hx, hy, hz = (float(v) for v in steps)
     nx, ny, nz = q.shape
     rho2 = np.zeros((2 * nx - 1, 2 * ny - 1, 2 * nz - 1))
     rho2[:nx, :ny, :nz] = q
-    K2 = mirror_kernel(sym_kernel(q.shape, steps))
+    K2 = mirror_kernel(sym_kernel(q.shape, steps, kernel_dtype))
     out = np.real(np.fft.ifftn(np.fft.fftn(rho2) * np.fft.fftn(K2)))
     return out[:nx, :ny, :nz] / (4 * PI * EPSILON_0 * hx * hy * hz)
 
@@ -195,7 +204,8 @@ def gather(Ex, Ey, Ez, X, gamma):
     return out
 
 
-def el_field(xyz, Q, gamma, nxyz, mesh_scale=None, mesh_shift=None, debug=None):
+def el_field(xyz, Q, gamma, nxyz, mesh_scale=None, mesh_shift=None, debug=None,
+             kernel_dtype=np.float64):
     """Lab-frame field (V/m) at the particles from rest-frame positions xyz
     (N,3; modified in place like the reference).  ocelot/cpbd/sc.py:170-205.
     If ``debug`` is a dict it receives steps, X_off, X, inds, rho, phi."""
@@ -203,7 +213,7 @@ def el_field(xyz, Q, gamma, nxyz, mesh_scale=None, mesh_shift=None, debug=None):
     X, steps, X_off = mesh_setup(xyz, Q, gamma, nxyz, mesh_scale, mesh_shift)
     inds = ngp_indices(X, nxyz)
     rho = deposit(inds, Q, nxyz)
-    phi = potential(rho, steps)
+    phi = potential(rho, steps, kernel_dtype)
     Ex, Ey, Ez = field_from_potential(phi, steps)
     E = gather(Ex, Ey, Ez, X, gamma)
     if debug is not None:
@@ -224,10 +234,12 @@ def frame(xp):
     return np.c_[t1, t2, t3], Pav
 
 
-def apply(rparticles, q_array, E, zstep, nmesh_xyz, mesh_scale=None, mesh_shift=None, debug=None):
+def apply(rparticles, q_array, E, zstep, nmesh_xyz, mesh_scale=None, mesh_shift=None, debug=None,
+          kernel_dtype=np.float64):
     """One space-charge kick; rparticles (6,N) is overwritten in place, all
     six rows.  ocelot/cpbd/sc.py:208-251.  ``mesh_scale``/``mesh_shift`` as in
-    mesh_setup (None = random_mesh off).  Returns rparticles."""
+    mesh_setup (None = random_mesh off); ``kernel_dtype`` as in sym_kernel.
+    Returns rparticles."""
     if zstep == 0:
         return rparticles
     nmesh_xyz = np.array(nmesh_xyz)
@@ -239,7 +251,7 @@ def apply(rparticles, q_array, E, zstep, nmesh_xyz, mesh_scale=None, mesh_shift=
     xp[3:6] = np.dot(xp[3:6].T, T).T
     gamma0 = np.sqrt((Pav / M_E_EV) ** 2 + 1)
     beta0 = np.sqrt(1 - gamma0 ** -2)
-    Exyz = el_field(xyz, q_array, gamma0, nmesh_xyz, mesh_scale, mesh_shift, debug)
+    Exyz = el_field(xyz, q_array, gamma0, nmesh_xyz, mesh_scale, mesh_shift, debug, kernel_dtype)
     cdT = zstep / betref
     xp[3] = xp[3] + cdT * (1 - beta0 * beta0) * Exyz[:, 0]
     xp[4] = xp[4] + cdT * (1 - beta0 * beta0) * Exyz[:, 1]

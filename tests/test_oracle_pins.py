@@ -39,6 +39,23 @@ def test_igf_matches_cubature(h):
         assert abs(K[idx] - ref) <= 1e-7 * abs(ref), (idx, K[idx], ref)
 
 
+@pytest.mark.parametrize("h", [(1.6e-6, 1.6e-6, 1e-4), (2e-5, 2e-5, 4e-6)])
+def test_igf_extended_precision_matches_cubature(h):
+    """On cells far from cubic the float64 8-corner difference loses digits
+    (1e-12 relative here); the extended-precision evaluation, the accuracy
+    reference for such meshes, matches the cubature to a few ulps of float64."""
+    K64 = orc.sym_kernel((6, 6, 6), h)
+    Kx = orc.sym_kernel((6, 6, 6), h, np.longdouble)
+    assert Kx.dtype == np.float64
+    worst64 = 0.0
+    for idx in [(1, 0, 0), (5, 5, 1), (0, 3, 2), (4, 1, 0)]:
+        c = [idx[d] * h[d] for d in range(3)]
+        ref = _cell_integral(c, h, nsub=2, npts=12)
+        assert abs(Kx[idx] - ref) <= 4e-15 * abs(ref), (idx, Kx[idx], ref)
+        worst64 = max(worst64, abs(K64[idx] - ref) / abs(ref))
+    assert worst64 > 1e-13          # the case is ill-conditioned for float64
+
+
 def test_igf_origin_cell_closed_form():
     # integral of 1/r over the unit cube centred at 0 = 8 * (1/2)^2 * I1,
     # with I1 = -pi/4 + 3/2 ln(2 + sqrt 3) the corner-cube integral
