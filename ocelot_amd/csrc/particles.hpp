@@ -358,6 +358,17 @@ __device__ __forceinline__ int xcc_id() {
   return (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) & (kNumXcd - 1));
 }
 
+// Workgroups are dispatched to the XCDs round-robin (workgroup b runs on XCD
+// b mod 8).  xcd_block renumbers them so that each XCD takes one contiguous
+// run of blocks: for a bunch kept in cell order that run covers one slab of
+// the mesh, whose field and charge cells then stay in that XCD's own L2.
+// A bijection of [0, nblk) for any nblk; only the order of work changes.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblk) {
+  const int64_t q = nblk / kNumXcd, rem = nblk - q * kNumXcd;
+  const int64_t x = b % kNumXcd, j = b / kNumXcd;
+  return x * q + (x < rem ? x : rem) + j;
+}
+
 // one fixed-point atomic add; XCD mode adds into the copy of the grid owned
 // by the executing XCD with workgroup-scope atomics, which complete in that
 // XCD's L2 instead of at the memory side
@@ -416,7 +427,7 @@ __global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restric
     if constexpr (NWORD == 2) hval[1][i] = 0;
   }
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * (kPT * kDepItems);
+  const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (kPT * kDepItems);
   int bad = 0;
   for (int it = 0; it < kDepItems; ++it) {
     const int64_t p = base + it * kPT + threadIdx.x;
@@ -652,117 +663,21 @@ struct DriftArgs {
   double L, r56;
 };
 
-// LDS tile of the three FP32 field grids over the box of mesh nodes that a
-// workgroup's particles touch (a bunch kept in cell order packs a workgroup
-// into a few cells); larger boxes fall back to gathers from global memory.
-constexpr int kTileNodes = 1024;
-
-struct TileBox {
-  int lo[3], dim[3], vol;
-};
-
-// block-wide box of the 3x3x3 stencils based at f (clamped to the mesh,
-// with one layer of margin below for the clamped corner at n-2)
-__device__ __forceinline__ TileBox block_box(const int* f, bool active, const int* n, int* lds6) {
-  int v[6];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    v[k] = active ? f[k] : 0x3fffffff;
-    v[3 + k] = active ? f[k] : -0x3fffffff;
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      v[k] = min(v[k], __shfl_xor(v[k], off, 64));
-      v[3 + k] = max(v[3 + k], __shfl_xor(v[3 + k], off, 64));
-    }
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int k = 0; k < 6; ++k) lds6[w * 6 + k] = v[k];
-  __syncthreads();
-  TileBox b;
-  int mn[3], mx[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    mn[k] = lds6[k];
-    mx[k] = lds6[3 + k];
-  }
-  for (int ww = 1; ww < (int)(blockDim.x >> 6); ++ww)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      mn[k] = min(mn[k], lds6[ww * 6 + k]);
-      mx[k] = max(mx[k], lds6[ww * 6 + 3 + k]);
-    }
-  b.vol = 1;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    b.lo[k] = max(mn[k] - 1, 0);
-    const int hi = min(mx[k] + 2, n[k] - 1);
-    b.dim[k] = hi >= b.lo[k] ? hi - b.lo[k] + 1 : 0;
-    b.vol *= b.dim[k];
-  }
-  return b;
-}
-
-// gather_grid reading the staged tile (same corners, same arithmetic)
-template <int AX>
-__device__ __forceinline__ float gather_tile(const float* __restrict__ T, const TileBox& b, const int* n,
-                                             const double* c) {
-  if (!(c[0] >= 0 && c[0] <= n[0] - 1 && c[1] >= 0 && c[1] <= n[1] - 1 && c[2] >= 0 && c[2] <= n[2] - 1))
-    return 0.f;
-  int i0[3];
-  float t[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    int f = (int)__builtin_floor(c[k]);
-    f = f > n[k] - 2 ? n[k] - 2 : f;
-    i0[k] = f;
-    t[k] = (float)(c[k] - (double)f);
-  }
-  const int sx = b.dim[1] * b.dim[2], sy = b.dim[2];
-  const int o0 = (i0[0] - b.lo[0]) * sx + (i0[1] - b.lo[1]) * sy + (i0[2] - b.lo[2]);
-  const int nax = n[AX];
-  float v[8];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const int iax = (AX == 0 ? i0[0] + a : (AX == 1 ? i0[1] + bb : i0[2] + e));
-        v[a * 4 + bb * 2 + e] = iax < nax - 1 ? T[o0 + a * sx + bb * sy + e] : 0.f;
-      }
-  float acc = 0.f;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const float w = (a ? t[0] : 1 - t[0]) * (bb ? t[1] : 1 - t[1]) * (e ? t[2] : 1 - t[2]);
-        acc += w * v[a * 4 + bb * 2 + e];
-      }
-  return acc;
-}
-
 template <typename F, bool DRIFT = false, bool SUMP = false>
 __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
                                               const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
                                               double* __restrict__ part = nullptr) {
   __shared__ double lds[SUMP ? (kPT / 64) * 3 : 1];
-  // mixed form: field tile; FP64 form: unused
-  __shared__ float tile[sizeof(F) == 4 ? 3 * kTileNodes : 1];
-  __shared__ int lds6[(kPT / 64) * 6];
   const KickScalars& s0 = *scp;
   if (s0.err) return;  // uniform across the grid
   const KickScalars s = s0;
-  const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t p = blk * kPT + threadIdx.x;
   const bool active = p < s.n;
   // the mixed form (FP32 field) takes the fast transforms
   constexpr bool FAST = sizeof(F) == 4;
-  double xp[6] = {0, 0, 0, 0, 0, 0}, xyz[3], X[3] = {0, 0, 0}, pr[3], E[3], pb[3], o[6];
+  double xp[6], xyz[3], X[3], pr[3], E[3], pb[3], o[6];
+  double m[3] = {0, 0, 0};
   if (active) {
     if constexpr (FAST)
       to_xp_fast(r[p], r[ld + p], r[2 * ld + p], r[3 * ld + p], r[4 * ld + p], r[5 * ld + p], s, xp);
@@ -774,44 +689,8 @@ __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld
       grid_coords_fast(xyz, s, X);
     else
       grid_coords(xyz, s, X);
-  }
-  bool staged = false;
-  TileBox box;
-  const int nn[3] = {s.nmesh[0], s.nmesh[1], s.nmesh[2]};
-  if constexpr (FAST) {
-    int f[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double fl = __builtin_floor(X[k]);
-      f[k] = fl < -1 ? -1 : (fl > nn[k] ? nn[k] : (int)fl);
-    }
-    box = block_box(f, active, nn, lds6);
-    if (box.vol > 0 && box.vol <= kTileNodes) {
-      staged = true;
-      const int64_t nc = (int64_t)nn[0] * nn[1] * nn[2];
-      const int dyz = box.dim[1] * box.dim[2];
-      for (int i = threadIdx.x; i < 3 * box.vol; i += kPT) {
-        const int comp = i / box.vol, node = i - comp * box.vol;
-        const int a = node / dyz, rem = node - a * dyz, b = rem / box.dim[2], c = rem - b * box.dim[2];
-        const int64_t gidx = ((int64_t)(box.lo[0] + a) * nn[1] + (box.lo[1] + b)) * nn[2] + (box.lo[2] + c);
-        tile[i] = ((const float*)phi)[comp * nc + gidx];
-      }
-      __syncthreads();
-    }
-  }
-  double m[3] = {0, 0, 0};
-  if (active) {
     rot(xp + 3, s.T, pr);
-    if (staged) {
-      const double cx[3] = {X[0], X[1] + 0.5, X[2] + 0.5};
-      const double cy[3] = {X[0] + 0.5, X[1], X[2] + 0.5};
-      const double cz[3] = {X[0] + 0.5, X[1] + 0.5, X[2]};
-      E[0] = (double)gather_tile<0>(tile, box, nn, cx) * s.gamma0;
-      E[1] = (double)gather_tile<1>(tile + box.vol, box, nn, cy) * s.gamma0;
-      E[2] = (double)gather_tile<2>(tile + 2 * box.vol, box, nn, cz);
-    } else {
-      gather_field<F>(phi, s, X, E);
-    }
+    gather_field<F>(phi, s, X, E);
     pr[0] = pr[0] + s.kperp * E[0];
     pr[1] = pr[1] + s.kperp * E[1];
     pr[2] = pr[2] + s.cdT * E[2];
@@ -837,7 +716,7 @@ __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld
   if constexpr (SUMP) {
     block_reduce<3, 0>(m, lds);
     if (threadIdx.x == 0)
-      for (int c = 0; c < 3; ++c) part[c * gridDim.x + blockIdx.x] = m[c];
+      for (int c = 0; c < 3; ++c) part[c * gridDim.x + blk] = m[c];
   }
 }
 
