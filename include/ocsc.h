@@ -101,6 +101,25 @@ int ocsc_track_drift(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double
                      const int32_t nmesh[3], double E_GeV, double zstep, double L, int nsteps, int precision,
                      int use_graph, void* stream);
 
+/* Beam moments behind get_envelope (ocelot/cpbd/beam/analysis.py:27-224),
+ * computed on the device over the six rows and q_array.  flags:
+ * OCSC_MOM_BOUNDS selects the particles with
+ * mean(tau) + b0 std(tau) <= tau <= mean(tau) + b1 std(tau) (analysis.py:59-67,
+ * slice=None); OCSC_MOM_AUTO_DISP estimates the dispersion from the selected
+ * particles (analysis.py:90-98); OCSC_MOM_DISP_GIVEN takes disp = {Dx, Dxp,
+ * Dy, Dyp} (analysis.py:99-103); otherwise D = 0.  out (host) receives
+ * min(nout, OCSC_MOM_NOUT) values: [0] selected count, [1] sum of all q,
+ * [2] <p>, [3..6] Dx Dxp Dy Dyp, [7..11] <x> <y> <px> <py> <tau> of the
+ * corrected coordinates, [12..23] xx xpx pxpx yy ypy pypy tautau xy pxpy xpy
+ * ypx pp, [24] mean(tau) and [25] std(tau) over all particles.  With fewer
+ * than 3 selected particles only [0..2] are meaningful.  Synchronous. */
+#define OCSC_MOM_BOUNDS 1
+#define OCSC_MOM_AUTO_DISP 2
+#define OCSC_MOM_DISP_GIVEN 4
+#define OCSC_MOM_NOUT 26
+int ocsc_moments(ocsc_ctx* ctx, const double* rparticles, int64_t ld, const double* q_array, int64_t n, int flags,
+                 const double disp[4], double b0, double b1, double* out, int nout, void* stream);
+
 /* One bunch of a batched call (fields as in ocsc_apply). */
 typedef struct ocsc_bunch {
   double* rparticles;
@@ -130,7 +149,7 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * 1 = one grid copy per XCD with atomics that complete in that XCD's L2,
  * summed exactly in the first FFT pass.  Both give bit-identical results.
  * "sort_every": ocsc_track_drift keeps the bunch in mesh-cell order and
- * re-sorts it every k steps (0: never; default 20).  "overlap": 1
+ * re-sorts it every k steps (0: never; default 10).  "overlap": 1
  * builds the Green's-function spectrum on a second stream concurrently with
  * the charge deposit, 0 (default: they contend for the CUs) runs it in line. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);

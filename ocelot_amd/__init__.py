@@ -9,5 +9,7 @@ does, and raises if it is missing.
 from .physproc import PhysProc
 from .beam import ParticleArray, DeviceParticleArray
 from .sc import SpaceCharge, apply_batched, drift, track_drift
+from .envelope import Twiss, get_envelope
 
-__all__ = ["PhysProc", "ParticleArray", "DeviceParticleArray", "SpaceCharge", "apply_batched", "drift", "track_drift"]
+__all__ = ["PhysProc", "ParticleArray", "DeviceParticleArray", "SpaceCharge", "apply_batched", "drift", "track_drift",
+           "Twiss", "get_envelope"]

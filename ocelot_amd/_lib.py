@@ -40,6 +40,7 @@ SIGNATURES = {
     "ocsc_track_drift": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int, _c_int, _vp]),
     "ocsc_last_stats": (_c_int, [_vp, _dp, _c_int, _vp]),
     "ocsc_apply_batched": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "ocsc_moments": (_c_int, [_vp, _vp, _i64, _vp, _i64, _c_int, _dp, _dbl, _dbl, _dp, _c_int, _vp]),
     "ocsc_set_option": (_c_int, [_vp, ctypes.c_char_p, _i64]),
     "ocsc_get_option": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_i64)]),
     "ocsc_set_profiling": (_c_int, [_vp, _c_int]),

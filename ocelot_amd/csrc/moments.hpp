@@ -1,0 +1,152 @@
+// Beam moments of a bunch on the device: the statistics behind Ocelot's
+// get_envelope (ocelot/cpbd/beam/analysis.py:27-224).  Every quantity is a
+// mean over the particles, taken in passes that each read the six rows once;
+// the per-block sums go through the same fixed reduction trees as the kick's
+// frame sums, so results are reproducible run to run.
+//
+//   pass 0 (all particles)   sum tau, sum q
+//   pass 1 (all, bounds)     sum (tau - mean)^2            -> std(tau), window
+//   pass 2 (selected)        count, sums of p, x, px, y, py, tau
+//   pass 3 (sel., auto_disp) sums of (p - <p>)^2 and (p - <p>)(u - <u>)
+//   pass 4 (selected)        sums of the dispersion-corrected x, y, px, py
+//   pass 5 (selected)        the twelve centred second moments
+#pragma once
+#include "particles.hpp"
+
+namespace ocsc {
+
+// rows of rparticles (beam/particle.py:17-22): x, px, y, py, tau, p
+struct MomState {
+  int64_t n;          // particles
+  int32_t bounds;     // select lo <= tau <= hi
+  int32_t disp_mode;  // 0: D = 0, 1: D given, 2: D estimated (auto_disp)
+  double b0, b1;      // window in units of std(tau) around mean(tau)
+  double lo, hi;
+  double tau_all, std_all, sumq;
+  double cnt;
+  double m[6];        // means of the selected particles: p, x, px, y, py, tau
+  double D[4];        // Dx, Dxp, Dy, Dyp
+  double var_p;
+  double c[4];        // means of the corrected x, y, px, py
+  double mom[12];     // xx, xpx, pxpx, yy, ypy, pypy, tautau, xy, pxpy, xpy, ypx, pp
+};
+
+constexpr int kMomNV[6] = {2, 1, 7, 5, 4, 12};
+
+// dispersion-corrected coordinates (analysis.py:112-130): u -= D * p, then
+// px, py scaled by 1 - p - p^2/2 + px^2/2 + py^2/2 of the corrected px, py
+__device__ __forceinline__ void mom_corrected(const double* R6, const MomState& s, double* x, double* y,
+                                              double* px, double* py) {
+  const double p = R6[5];
+  *x = R6[0] - s.D[0] * p;
+  const double px0 = R6[1] - s.D[1] * p;
+  *y = R6[2] - s.D[2] * p;
+  const double py0 = R6[3] - s.D[3] * p;
+  const double f = 1. - p - 0.5 * p * p + 0.5 * px0 * px0 + 0.5 * py0 * py0;
+  *px = px0 * f;
+  *py = py0 * f;
+}
+
+template <int PASS>
+__global__ void __launch_bounds__(kPT) k_mom(Rows R, const double* __restrict__ q, const MomState* __restrict__ msp,
+                                             double* __restrict__ part) {
+  constexpr int NV = kMomNV[PASS];
+  __shared__ double lds[(kPT / 64) * NV];
+  const MomState s = *msp;
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) acc[c] = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < s.n; i += (int64_t)gridDim.x * kPT) {
+    const double tau = R.r[4 * R.ld + i];
+    if constexpr (PASS == 0) {
+      acc[0] += tau;
+      acc[1] += q[i];
+      continue;
+    }
+    if constexpr (PASS == 1) {
+      const double d = tau - s.tau_all;
+      acc[0] += d * d;
+      continue;
+    }
+    if (s.bounds && !(s.lo <= tau && tau <= s.hi)) continue;
+    double R6[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) R6[k] = k == 4 ? tau : R.r[k * R.ld + i];
+    if constexpr (PASS == 2) {
+      acc[0] += 1.0;
+      acc[1] += R6[5];
+      acc[2] += R6[0];
+      acc[3] += R6[1];
+      acc[4] += R6[2];
+      acc[5] += R6[3];
+      acc[6] += tau;
+    } else if constexpr (PASS == 3) {
+      const double dp = R6[5] - s.m[0];
+      acc[0] += dp * dp;
+      acc[1] += dp * (R6[0] - s.m[1]);
+      acc[2] += dp * (R6[1] - s.m[2]);
+      acc[3] += dp * (R6[2] - s.m[3]);
+      acc[4] += dp * (R6[3] - s.m[4]);
+    } else {
+      double x, y, px, py;
+      mom_corrected(R6, s, &x, &y, &px, &py);
+      if constexpr (PASS == 4) {
+        acc[0] += x;
+        acc[1] += y;
+        acc[2] += px;
+        acc[3] += py;
+      } else {
+        const double dx = x - s.c[0], dy = y - s.c[1], dpx = px - s.c[2], dpy = py - s.c[3];
+        const double dt = tau - s.m[5], dp = R6[5] - s.m[0];
+        acc[0] += dx * dx;
+        acc[1] += dx * dpx;
+        acc[2] += dpx * dpx;
+        acc[3] += dy * dy;
+        acc[4] += dy * dpy;
+        acc[5] += dpy * dpy;
+        acc[6] += dt * dt;
+        acc[7] += dx * dy;
+        acc[8] += dpx * dpy;
+        acc[9] += dx * dpy;
+        acc[10] += dy * dpx;
+        acc[11] += dp * dp;
+      }
+    }
+  }
+  block_reduce<NV, 0>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < NV; ++c) part[c * gridDim.x + blockIdx.x] = acc[c];
+}
+
+// one block: totals of a pass, then the scalars the next pass needs
+template <int PASS>
+__global__ void __launch_bounds__(1024) k_mom_reduce(MomState* __restrict__ msp, const double* __restrict__ part,
+                                                     int nblk) {
+  constexpr int NV = kMomNV[PASS];
+  __shared__ double lds[16 * NV];
+  double t[NV];
+  reduce_partials<NV, 0>(part, nblk, t, lds);
+  if (threadIdx.x != 0) return;
+  MomState& s = *msp;
+  const double n = (double)s.n;
+  if constexpr (PASS == 0) {
+    s.tau_all = t[0] / n;
+    s.sumq = t[1];
+  } else if constexpr (PASS == 1) {
+    s.std_all = __builtin_sqrt(t[0] / n);
+    s.lo = s.tau_all + s.std_all * s.b0;
+    s.hi = s.tau_all + s.std_all * s.b1;
+  } else if constexpr (PASS == 2) {
+    s.cnt = t[0];
+    for (int k = 0; k < 6; ++k) s.m[k] = t[k + 1] / t[0];
+  } else if constexpr (PASS == 3) {
+    s.var_p = t[0] / s.cnt;
+    for (int k = 0; k < 4; ++k) s.D[k] = t[k + 1] / s.cnt / s.var_p;
+  } else if constexpr (PASS == 4) {
+    for (int k = 0; k < 4; ++k) s.c[k] = t[k] / s.cnt;
+  } else {
+    for (int k = 0; k < 12; ++k) s.mom[k] = t[k] / s.cnt;
+  }
+}
+
+}  // namespace ocsc

@@ -12,6 +12,7 @@
 #include "fft.hpp"
 #include "mesh.hpp"
 #include "particles.hpp"
+#include "moments.hpp"
 
 namespace ocsc {
 // sort.hip
@@ -165,9 +166,12 @@ struct ocsc_ctx {
   int profiling = 0;
   // tuning options (ocsc_set_option)
   int deposit_xcd = 0;  // 1: per-XCD grid copies with L2-local atomics
-  int sort_every = 20;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
+  int sort_every = 10;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
   // tracking workspace: two cell-ordered copies of the bunch, sort buffers
   Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp;
+  // beam moments: device state and its pinned host mirror
+  MomState* mom = nullptr;
+  MomState* mom_host = nullptr;
   hipEvent_t ev[kNumStages + 1] = {};
   double stage_ms[kNumStages] = {};
   int64_t stage_calls = 0;
@@ -652,6 +656,8 @@ int ocsc_destroy(ocsc_ctx* c) {
     if (b->p) (void)hipFree(b->p);
   if (L.sc) (void)hipFree(L.sc);
   if (L.sc_host) (void)hipHostFree(L.sc_host);
+  if (c->mom) (void)hipFree(c->mom);
+  if (c->mom_host) (void)hipHostFree(c->mom_host);
   if (c->twf) (void)hipFree(c->twf);
   if (c->twd) (void)hipFree(c->twd);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
@@ -824,6 +830,69 @@ int ocsc_drift(ocsc_ctx* c, double* r, int64_t ld, int64_t n, double L, double E
   const double r56 = 0.0 - L / (beta * beta) * igamma2;
   k_drift<<<(int)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(r, ld, n, L, r56);
   LAUNCHCHK("k_drift");
+  return OCSC_OK;
+}
+
+int ocsc_moments(ocsc_ctx* c, const double* r, int64_t ld, const double* q, int64_t n, int flags,
+                 const double disp[4], double b0, double b1, double* out, int nout, void* stream) {
+  if (!c || !out || ((!r || !q) && n > 0)) return fail(OCSC_EINVAL, "null argument");
+  if (n < 0 || ld < n) return fail(OCSC_EINVAL, "bad n / ld");
+  if ((flags & OCSC_MOM_DISP_GIVEN) && !disp) return fail(OCSC_EINVAL, "OCSC_MOM_DISP_GIVEN needs disp");
+  if (n == 0) return fail(OCSC_EVALUE, "empty bunch");
+  HIPCHK(hipSetDevice(c->device));
+  Lane& L = c->lane0;
+  hipStream_t st = (hipStream_t)stream;
+  if (!c->mom) {
+    HIPCHK(hipMalloc(&c->mom, sizeof(MomState)));
+    HIPCHK(hipHostMalloc(&c->mom_host, sizeof(MomState), hipHostMallocDefault));
+  }
+  const int nblk = nred_blocks(n);
+  int rc;
+  if ((rc = ensure(L.part, (size_t)12 * nblk * sizeof(double)))) return rc;
+  // the pinned mirror may still be read by an earlier copy on another stream
+  HIPCHK(hipStreamSynchronize(st));
+  MomState& h = *c->mom_host;
+  std::memset(&h, 0, sizeof(h));
+  h.n = n;
+  h.bounds = (flags & OCSC_MOM_BOUNDS) ? 1 : 0;
+  h.b0 = b0;
+  h.b1 = b1;
+  h.disp_mode = (flags & OCSC_MOM_DISP_GIVEN) ? 1 : ((flags & OCSC_MOM_AUTO_DISP) ? 2 : 0);
+  if (h.disp_mode == 1)
+    for (int k = 0; k < 4; ++k) h.D[k] = disp[k];
+  HIPCHK(hipMemcpyAsync(c->mom, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  double* part = (double*)L.part.p;
+  const Rows R{r, ld};
+  k_mom<0><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+  k_mom_reduce<0><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+  if (h.bounds) {
+    k_mom<1><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+    k_mom_reduce<1><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+  }
+  k_mom<2><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+  k_mom_reduce<2><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+  if (h.disp_mode == 2) {
+    k_mom<3><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+    k_mom_reduce<3><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+  }
+  k_mom<4><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+  k_mom_reduce<4><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+  k_mom<5><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+  k_mom_reduce<5><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+  LAUNCHCHK("k_mom");
+  HIPCHK(hipMemcpyAsync(&h, c->mom, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double v[OCSC_MOM_NOUT];
+  v[0] = h.cnt;
+  v[1] = h.sumq;
+  v[2] = h.m[0];
+  for (int k = 0; k < 4; ++k) v[3 + k] = h.D[k];
+  for (int k = 0; k < 4; ++k) v[7 + k] = h.c[k];
+  v[11] = h.m[5];
+  for (int k = 0; k < 12; ++k) v[12 + k] = h.mom[k];
+  v[24] = h.tau_all;
+  v[25] = h.std_all;
+  for (int k = 0; k < nout && k < OCSC_MOM_NOUT; ++k) out[k] = v[k];
   return OCSC_OK;
 }
 

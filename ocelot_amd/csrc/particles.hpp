@@ -52,58 +52,37 @@ __device__ __forceinline__ void to_xp(double r0, double r1, double r2, double r3
   xp[5] = u2 * gamma * beta * kMeEV;
 }
 
-// Fast variants for the mixed form (passes B and D, fused momentum sums):
-// one reciprocal per divisor, constant divisors replaced by their inverses.
+// Fast variants for the mixed form (passes B and D, sort keys), from the same
+// kinematics with the common factors cancelled.  With t = gamma beta /
+// (gamma_ref beta_ref) and pz = sqrt(t^2 - x'^2 - y'^2), the direction
+// (a0, a1, 1)/|.| of coord_transform.py:83-91 is (x', y', pz)/t, so the
+// momentum is (x', y', pz) p_ref and the common-time shift beta u tau is
+// (x', y', pz) tau gamma_ref beta_ref / gamma: one square root and one
+// reciprocal per particle instead of three of each.  Going back
+// (coord_transform.py:31-53), beta u0 cdt = -z p0/p2 and cdt = -z m gamma/p2.
 // Results differ from the exact sequence in the last bits only.
 __device__ __forceinline__ void to_xp_fast(double r0, double r1, double r2, double r3, double r4, double r5,
                                            const KickScalars& s, double* xp) {
   const double gamma = (s.betaref * r5 + 1) * s.gamref;
-  const double ig = rcp_nr(gamma);
-  const double beta = __builtin_sqrt(1 - ig * ig);
-  const double gb = gamma * beta;
-  const double t = gb * s.inv_gbref;
-  const double pz2pref = __builtin_sqrt(t * t - r1 * r1 - r3 * r3);
-  const double ipz = rcp_nr(pz2pref);
-  const double a0 = r1 * ipz, a1 = r3 * ipz;
-  const double inrm = rcp_nr(__builtin_sqrt(a0 * a0 + a1 * a1 + 1.0));
-  const double u0 = a0 * inrm, u1 = a1 * inrm, u2 = inrm;
-  xp[0] = r0 - u0 * beta * r4;
-  xp[1] = r2 - u1 * beta * r4;
-  xp[2] = -u2 * beta * r4;
-  const double gbm = gb * kMeEV;
-  xp[3] = u0 * gbm;
-  xp[4] = u1 * gbm;
-  xp[5] = u2 * gbm;
-}
-
-__device__ __forceinline__ void to_p_fast(double r1, double r3, double r5, const KickScalars& s, double* p) {
-  const double gamma = (s.betaref * r5 + 1) * s.gamref;
-  const double ig = rcp_nr(gamma);
-  const double beta = __builtin_sqrt(1 - ig * ig);
-  const double gb = gamma * beta;
-  const double t = gb * s.inv_gbref;
-  const double pz2pref = __builtin_sqrt(t * t - r1 * r1 - r3 * r3);
-  const double ipz = rcp_nr(pz2pref);
-  const double a0 = r1 * ipz, a1 = r3 * ipz;
-  const double inrm = rcp_nr(__builtin_sqrt(a0 * a0 + a1 * a1 + 1.0));
-  const double gbm = gb * kMeEV * inrm;
-  p[0] = a0 * gbm;
-  p[1] = a1 * gbm;
-  p[2] = gbm;
+  const double t2 = (gamma * gamma - 1) * (s.inv_gbref * s.inv_gbref);
+  const double pz = __builtin_sqrt(t2 - r1 * r1 - r3 * r3);
+  const double w = r4 * (s.gbref * rcp_nr(gamma));
+  xp[0] = r0 - r1 * w;
+  xp[1] = r2 - r3 * w;
+  xp[2] = -(pz * w);
+  xp[3] = r1 * s.pref;
+  xp[4] = r3 * s.pref;
+  xp[5] = pz * s.pref;
 }
 
 __device__ __forceinline__ void to_ocelot_fast(const double* xp, const double* p, const KickScalars& s, double* o) {
   const double s2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
   const double gamma = __builtin_sqrt(1 + s2 * (1.0 / kMeEV2));
-  const double ig = rcp_nr(gamma);
-  const double beta = __builtin_sqrt(1 - ig * ig);
-  const double inrm = rcp_nr(__builtin_sqrt(s2));
-  const double u0 = p[0] * inrm, u1 = p[1] * inrm, u2 = p[2] * inrm;
-  const double cdt = -xp[2] * rcp_nr(beta * u2);
-  o[0] = xp[0] + beta * u0 * cdt;
-  o[2] = xp[1] + beta * u1 * cdt;
+  const double a = -xp[2] * rcp_nr(p[2]);
+  o[0] = xp[0] + p[0] * a;
+  o[2] = xp[1] + p[1] * a;
+  o[4] = a * (kMeEV * gamma);
   o[5] = (gamma * s.inv_gamref - 1) * s.inv_betaref;
-  o[4] = cdt;
   o[1] = p[0] * s.inv_pref;
   o[3] = p[1] * s.inv_pref;
 }
@@ -200,9 +179,24 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
   double v[NV];
 #pragma unroll
   for (int c = 0; c < NV; ++c) v[c] = KIND == 1 && c < 3 ? __builtin_inf() : (KIND == 1 && c < 6 ? -__builtin_inf() : 0.0);
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+  // U partials per thread are loaded before any is combined, so U loads are
+  // in flight instead of one; each thread still combines its partials in
+  // increasing b (out-of-range slots are skipped), so the tree is unchanged
+  constexpr int U = NV > 8 ? 2 : (NV > 4 ? 4 : 8);
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * blockDim.x) {
+    double t[U][NV];
 #pragma unroll
-    for (int c = 0; c < NV; ++c) v[c] = red_op<KIND>(c, v[c], part[c * nblk + b]);
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * blockDim.x;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) t[u][c] = b < nblk ? part[c * nblk + b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (b0 + u * (int)blockDim.x < nblk)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) v[c] = red_op<KIND>(c, v[c], t[u][c]);
+  }
   block_reduce<NV, KIND>(v, lds);
 #pragma unroll
   for (int c = 0; c < NV; ++c) out[c] = v[c];
@@ -380,13 +374,22 @@ __device__ __forceinline__ void grid_add(unsigned long long* g, int64_t idx, lon
     atomicAdd(g + idx, (unsigned long long)v);
 }
 
-// NGP cell of a particle from its six rows, -1 if outside the mesh
+// NGP cell of a particle from its six rows, -1 if outside the mesh; FAST
+// (mixed form) takes the reciprocal-multiply transforms, the FP64 form the
+// exact sequence of the reference
+template <bool FAST>
 __device__ __forceinline__ int64_t ngp_cell(const double* R6, const KickScalars& s) {
   double xp[6], xyz[3], X[3];
-  to_xp(R6[0], R6[1], R6[2], R6[3], R6[4], R6[5], s, xp);
+  if constexpr (FAST)
+    to_xp_fast(R6[0], R6[1], R6[2], R6[3], R6[4], R6[5], s, xp);
+  else
+    to_xp(R6[0], R6[1], R6[2], R6[3], R6[4], R6[5], s, xp);
   rot(xp, s.T, xyz);
   xyz[2] = xyz[2] * s.gamma0;
-  grid_coords(xyz, s, X);
+  if constexpr (FAST)
+    grid_coords_fast(xyz, s, X);
+  else
+    grid_coords(xyz, s, X);
   const int nx = s.nmesh[0], ny = s.nmesh[1], nz = s.nmesh[2];
   const double f0 = __builtin_floor(X[0]) + 1, f1 = __builtin_floor(X[1]) + 1, f2 = __builtin_floor(X[2]) + 1;
   if (!(f0 >= 0 && f0 < nx && f1 >= 0 && f1 < ny && f2 >= 0 && f2 < nz)) return -1;
@@ -434,7 +437,7 @@ __global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restric
     if (p >= s.n) break;
     const double R6[6] = {R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p],
                           R.r[5 * R.ld + p]};
-    const int64_t idx = ngp_cell(R6, s);
+    const int64_t idx = ngp_cell<NWORD == 1>(R6, s);  // one word: the mixed form
     if (idx < 0) {
       bad = 1;
       continue;
@@ -584,7 +587,10 @@ __device__ __forceinline__ F gather_diff(const F* __restrict__ phi, int nx, int 
 
 // Mixed form: trilinear interpolation at c of a precomputed FP32 field
 // component E_AX (spectrally differenced, so no cancellation here), with the
-// same zero-plane and edge rules as gather_diff.
+// same zero-plane and edge rules as gather_diff.  The two z-neighbours of a
+// corner pair are adjacent in memory and are read with one 8-byte load (the
+// address is only 4-byte aligned, which global loads allow): 12 loads per
+// particle for the three components instead of 24.
 template <int AX>
 __device__ __forceinline__ float gather_grid(const float* __restrict__ E, int nx, int ny, int nz, const double* c) {
   const int n[3] = {nx, ny, nz};
@@ -599,18 +605,24 @@ __device__ __forceinline__ float gather_grid(const float* __restrict__ E, int nx
     t[k] = (float)(c[k] - (double)f);
   }
   const int64_t sx = (int64_t)ny * nz, sy = nz;
-  const int nax = n[AX];
   const int64_t o0 = i0[0] * sx + i0[1] * sy + i0[2];
   float v[8];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const int iax = (AX == 0 ? i0[0] + a : (AX == 1 ? i0[1] + b : i0[2] + e));
-        v[a * 4 + b * 2 + e] = iax < nax - 1 ? E[o0 + a * sx + b * sy + e] : 0.f;
+    for (int b = 0; b < 2; ++b) {
+      float2 pr;
+      __builtin_memcpy(&pr, E + o0 + a * sx + b * sy, sizeof(pr));
+      bool z0 = false, z1 = false;  // corners on the component's last plane read 0
+      if (AX == 0) z0 = z1 = i0[0] + a >= nx - 1;
+      if (AX == 1) z0 = z1 = i0[1] + b >= ny - 1;
+      if (AX == 2) {
+        z0 = i0[2] >= nz - 1;
+        z1 = i0[2] + 1 >= nz - 1;
       }
+      v[a * 4 + b * 2 + 0] = z0 ? 0.f : pr.x;
+      v[a * 4 + b * 2 + 1] = z1 ? 0.f : pr.y;
+    }
   float acc = 0.f;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -622,6 +634,65 @@ __device__ __forceinline__ float gather_grid(const float* __restrict__ E, int nx
         acc += w * v[a * 4 + b * 2 + e];
       }
   return acc;
+}
+
+// Mixed form, the three components at once.  Component k samples its grid at
+// X on axis k and at X + 1/2 on the other two (sc.py:202-204), so the six
+// floors and fractions are computed once; offsets are 32-bit (grids hold at
+// most 512^3 nodes), so each pair load is one scalar-based global load.
+// Interpolation by nested linear blends in FP32; zero-plane and edge rules as
+// in gather_grid.
+__device__ __forceinline__ void gather_field3(const float* __restrict__ E, const int* n, const double* X,
+                                              float* out) {
+  int f[2][3];
+  float t[2][3];
+  bool ok[2][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const double c = h ? X[k] + 0.5 : X[k];
+      ok[h][k] = c >= 0 && c <= n[k] - 1;
+      int fl = ok[h][k] ? (int)__builtin_floor(c) : 0;
+      fl = fl > n[k] - 2 ? n[k] - 2 : fl;
+      f[h][k] = fl;
+      t[h][k] = (float)(c - (double)fl);
+    }
+  const unsigned sy = (unsigned)n[2], sx = (unsigned)(n[1] * n[2]), nc = sx * (unsigned)n[0];
+  const char* base = (const char*)E;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int h0 = ax == 0 ? 0 : 1, h1 = ax == 1 ? 0 : 1, h2 = ax == 2 ? 0 : 1;
+    const int i0 = f[h0][0], i1 = f[h1][1], i2 = f[h2][2];
+    const unsigned o = ax * nc + (unsigned)i0 * sx + (unsigned)i1 * sy + (unsigned)i2;
+    float v[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        float2 pr;
+        __builtin_memcpy(&pr, base + (size_t)((o + a * sx + b * sy) * 4u), sizeof(pr));
+        bool z0 = false, z1 = false;  // corners on the component's last plane read 0
+        if (ax == 0) z0 = z1 = i0 + a >= n[0] - 1;
+        if (ax == 1) z0 = z1 = i1 + b >= n[1] - 1;
+        if (ax == 2) {
+          z0 = i2 >= n[2] - 1;
+          z1 = i2 + 1 >= n[2] - 1;
+        }
+        v[a][b][0] = z0 ? 0.f : pr.x;
+        v[a][b][1] = z1 ? 0.f : pr.y;
+      }
+    const float tx = t[h0][0], ty = t[h1][1], tz = t[h2][2];
+    float r[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const float q0 = __builtin_fmaf(tz, v[a][0][1] - v[a][0][0], v[a][0][0]);
+      const float q1 = __builtin_fmaf(tz, v[a][1][1] - v[a][1][0], v[a][1][0]);
+      r[a] = __builtin_fmaf(ty, q1 - q0, q0);
+    }
+    const float res = __builtin_fmaf(tx, r[1] - r[0], r[0]);
+    out[ax] = (ok[h0][0] && ok[h1][1] && ok[h2][2]) ? res : 0.f;
+  }
 }
 
 // field at a particle from the mesh result: FP64 potential (differenced
@@ -678,6 +749,7 @@ __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld
   constexpr bool FAST = sizeof(F) == 4;
   double xp[6], xyz[3], X[3], pr[3], E[3], pb[3], o[6];
   double m[3] = {0, 0, 0};
+  const int nn[3] = {s.nmesh[0], s.nmesh[1], s.nmesh[2]};
   if (active) {
     if constexpr (FAST)
       to_xp_fast(r[p], r[ld + p], r[2 * ld + p], r[3 * ld + p], r[4 * ld + p], r[5 * ld + p], s, xp);
@@ -690,7 +762,15 @@ __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld
     else
       grid_coords(xyz, s, X);
     rot(xp + 3, s.T, pr);
-    gather_field<F>(phi, s, X, E);
+    if constexpr (FAST) {
+      float g[3];
+      gather_field3((const float*)phi, nn, X, g);
+      E[0] = (double)g[0] * s.gamma0;
+      E[1] = (double)g[1] * s.gamma0;
+      E[2] = (double)g[2];
+    } else {
+      gather_field<F>(phi, s, X, E);
+    }
     pr[0] = pr[0] + s.kperp * E[0];
     pr[1] = pr[1] + s.kperp * E[1];
     pr[2] = pr[2] + s.cdT * E[2];
@@ -707,10 +787,16 @@ __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld
 #pragma unroll
     for (int i = 0; i < 6; ++i) r[i * ld + p] = o[i];
     if constexpr (SUMP) {
-      if constexpr (FAST)
-        to_p_fast(o[1], o[3], o[5], s, m);
-      else
+      // next kick's pass A: the momentum of the written rows (a drift keeps
+      // x', y', delta); the mixed form takes the lab momentum it just wrote
+      // them from, equal up to rounding
+      if constexpr (FAST) {
+        m[0] = pb[0];
+        m[1] = pb[1];
+        m[2] = pb[2];
+      } else {
         to_p(o[1], o[3], o[5], s, m);
+      }
     }
   }
   if constexpr (SUMP) {

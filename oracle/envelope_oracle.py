@@ -1,0 +1,100 @@
+"""FP64 CPU oracle for the beam moments of Ocelot's get_envelope.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for ``ocsc_moments`` /
+``ocelot_amd.get_envelope``: a numpy restatement of
+ocelot/cpbd/beam/analysis.py:27-224 (bounds with slice=None, dispersion from
+tws_i or estimated with auto_disp, the px/py correction factor, the twelve
+centred moments, emittances, Twiss beta/alpha).  Only ``tests/`` may import
+it; the product path never does.
+
+Pinning (tests/test_envelope_oracle.py): a point set with exactly known mean
+and covariance (every moment, emittance and beta/alpha in closed form), an
+uncoupled beam (eigen-emittances = the two plane emittances), a beam built
+with known dispersion (auto_disp recovers it), brute-force window selection,
+and the eigen-emittances against numpy.linalg.eigvals of Sigma*S.
+"""
+import numpy as np
+
+M_E_GEV = 9.10938215e-31 * 299792458.0 ** 2 / 1.6021766208e-19 / 1e+9
+
+
+def moments(r, q, bounds=None, auto_disp=False, disp=None):
+    """Dictionary of the quantities get_envelope computes from the particles
+    (analysis.py:58-158): count, q, p, D*, corrected means, second moments."""
+    tau = r[4]
+    out = {"q": float(np.sum(q))}
+    sel = np.ones(r.shape[1], dtype=bool)
+    if bounds is not None:                      # analysis.py:59-67, slice=None
+        sig0 = np.std(tau)
+        z0 = np.mean(tau)
+        sel = (z0 + sig0 * bounds[0] <= tau) & (tau <= z0 + sig0 * bounds[1])
+    x, px, y, py, tau, p = r[0][sel], r[1][sel], r[2][sel], r[3][sel], tau[sel], r[5][sel]
+    out["count"] = int(sel.sum())
+    out["p"] = float(np.mean(p)) if p.size else float("nan")
+    if p.size < 3:
+        return out
+    D = (0.0, 0.0, 0.0, 0.0)
+    if disp is not None:                        # analysis.py:99-103
+        D = tuple(float(v) for v in disp)
+    elif auto_disp:                             # analysis.py:90-98
+        mp, var_p = np.mean(p), np.var(p)
+        D = tuple(float(np.mean((p - mp) * (u - np.mean(u))) / var_p) for u in (x, px, y, py))
+    out["Dx"], out["Dxp"], out["Dy"], out["Dyp"] = D
+    x = x - D[0] * p                            # analysis.py:105-115
+    px = px - D[1] * p
+    y = y - D[2] * p
+    py = py - D[3] * p
+    f = 1. - p - 0.5 * p * p + 0.5 * px * px + 0.5 * py * py   # analysis.py:117-123
+    px = px * f
+    py = py * f
+    m = {k: float(np.mean(v)) for k, v in (("x", x), ("y", y), ("px", px), ("py", py), ("tau", tau))}
+    out.update(m)
+    c = {"x": x - m["x"], "y": y - m["y"], "px": px - m["px"], "py": py - m["py"], "tau": tau - m["tau"],
+         "p": p - out["p"]}
+    for name, (a, b) in {"xx": ("x", "x"), "xpx": ("x", "px"), "pxpx": ("px", "px"), "yy": ("y", "y"),
+                         "ypy": ("y", "py"), "pypy": ("py", "py"), "tautau": ("tau", "tau"), "xy": ("x", "y"),
+                         "pxpy": ("px", "py"), "xpy": ("x", "py"), "ypx": ("y", "px"), "pp": ("p", "p")}.items():
+        out[name] = np.mean(c[a] * c[b])             # analysis.py:134-158 (numpy scalars, as there)
+    return out
+
+
+def twiss(mo, E):
+    """Derived Twiss scalars from the moments (analysis.py:160-224).  The
+    eigen-emittances are the roots e1 >= e2 of t^2 - s t + det(Sigma) with
+    s = -tr((Sigma S)^2)/2 (eigenvalues of Sigma*S are +-i e1, +-i e2)."""
+    t = dict(mo)
+    t["emit_x"] = np.sqrt(mo["xx"] * mo["pxpx"] - mo["xpx"] ** 2)
+    t["emit_y"] = np.sqrt(mo["yy"] * mo["pypy"] - mo["ypy"] ** 2)
+    g = E / M_E_GEV
+    b = np.sqrt(1 - g ** -2) if g != 0 else 1.
+    t["emit_xn"] = t["emit_x"] * g * b
+    t["emit_yn"] = t["emit_y"] * g * b
+    Sig = sigma_matrix(mo)
+    S = np.array([[0, 0, 1, 0], [0, 0, 0, 1], [-1, 0, 0, 0], [0, -1, 0, 0]], dtype=float)
+    M = Sig @ S
+    s = -0.5 * np.trace(M @ M)
+    d = np.linalg.det(Sig)
+    disc = np.sqrt(max(s * s - 4 * d, 0.0))
+    t["eigemit_1"] = float(np.sqrt(max((s + disc) / 2, 0.0)))
+    t["eigemit_2"] = float(np.sqrt(max((s - disc) / 2, 0.0)))
+    t["beta_x"] = mo["xx"] / t["emit_x"]
+    t["beta_y"] = mo["yy"] / t["emit_y"]
+    t["alpha_x"] = -mo["xpx"] / t["emit_x"]
+    t["alpha_y"] = -mo["ypy"] / t["emit_y"]
+    return t
+
+
+def sigma_matrix(mo):
+    """4x4 transverse moment matrix in (x, y, px, py) order (analysis.py:160-163)."""
+    return np.array([[mo["xx"], mo["xy"], mo["xpx"], mo["xpy"]],
+                     [mo["xy"], mo["yy"], mo["ypx"], mo["ypy"]],
+                     [mo["xpx"], mo["ypx"], mo["pxpx"], mo["pxpy"]],
+                     [mo["xpy"], mo["ypy"], mo["pxpy"], mo["pypy"]]])
+
+
+def get_envelope(r, q, E, bounds=None, auto_disp=False, disp=None):
+    """All of the above: the dictionary of Twiss attributes get_envelope sets."""
+    mo = moments(r, q, bounds, auto_disp, disp)
+    if mo["count"] < 3:
+        return mo
+    return twiss(mo, E)

@@ -1,0 +1,112 @@
+"""get_envelope on the GPU (ocsc_moments through the C ABI) against the
+reference's own output (golden vectors) and the oracle.  Tolerance: 1e-9
+relative to each quantity's natural scale (sums are taken in a different
+order than numpy's pairwise mean, so results differ in the last bits)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import envelope_oracle as eo
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+
+
+@pytest.fixture(scope="module")
+def env_golden():
+    with np.load(os.path.join(HERE, "golden", "envelope_golden.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def parray(r, q, E):
+    from ocelot_amd import ParticleArray
+    p = ParticleArray()
+    p.rparticles = np.array(r, dtype=np.float64, copy=True)
+    p.q_array = np.array(q, dtype=np.float64, copy=True)
+    p.E = float(E)
+    return p
+
+
+SCALE = {"x": "xx", "y": "yy", "px": "pxpx", "py": "pypy", "tau": "tautau", "p": "pp",
+         "xy": ("xx", "yy"), "xpx": ("xx", "pxpx"), "ypy": ("yy", "pypy"), "pxpy": ("pxpx", "pypy"),
+         "xpy": ("xx", "pypy"), "ypx": ("yy", "pxpx")}
+
+
+def close(a, got, ref_d):
+    """|got - ref| within 1e-9 of the quantity's scale."""
+    ref = ref_d[a]
+    sc = abs(ref)
+    s = SCALE.get(a)
+    if isinstance(s, str):
+        sc = max(sc, np.sqrt(abs(ref_d[s])))
+    elif isinstance(s, tuple):
+        sc = max(sc, np.sqrt(abs(ref_d[s[0]] * ref_d[s[1]])))
+    return abs(got - ref) <= 1e-9 * sc + 1e-300
+
+
+def kwargs_for(g, setting):
+    from ocelot_amd import Twiss
+    if setting == "bounds":
+        return {"bounds": list(g["bounds/b"])}
+    if setting == "autodisp":
+        return {"auto_disp": True}
+    if setting == "twsi":
+        t = Twiss()
+        t.Dx, t.Dxp, t.Dy, t.Dyp = (float(v) for v in g["disp/tws_i"])
+        return {"tws_i": t}
+    return {}
+
+
+@pytest.mark.parametrize("bunch", ["gauss", "flat"])
+@pytest.mark.parametrize("setting", ["plain", "bounds", "autodisp", "twsi"])
+def test_envelope_golden(env_golden, bunch, setting):
+    from ocelot_amd import get_envelope
+    g = env_golden
+    attrs = [str(a) for a in g["meta/attrs"]]
+    ref = dict(zip(attrs, g[f"{bunch}/{setting}"]))
+    p = parray(g[f"{bunch}/r"], g[f"{bunch}/q"], g[f"{bunch}/E"])
+    tws = get_envelope(p, **kwargs_for(g, setting))
+    bad = [(a, getattr(tws, a), ref[a]) for a in attrs if not close(a, float(getattr(tws, a)), ref)]
+    assert not bad, bad
+
+
+def test_envelope_two_particles(env_golden):
+    from ocelot_amd import get_envelope
+    g = env_golden
+    ref = dict(zip([str(a) for a in g["meta/attrs"]], g["two/plain"]))
+    tws = get_envelope(parray(g["two/r"], g["two/q"], g["two/E"]))
+    assert tws.q == pytest.approx(ref["q"], rel=1e-15) and tws.p == pytest.approx(ref["p"], rel=1e-14)
+    assert tws.xx == 0.0 and tws.emit_x == 0.0
+
+
+def test_envelope_10m_device_resident_vs_oracle():
+    """The 10M-particle flat-top bunch of the tracking benchmark, resident in
+    HBM: moments vs the oracle, and the device path equals the host path."""
+    from ocelot_amd import DeviceParticleArray, get_envelope
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, _, _ = make_bunch("flattop_10m_127")
+    ref = eo.get_envelope(r, q, E, auto_disp=True)
+    host = parray(r, q, E)
+    d = DeviceParticleArray.from_host(host, torch.cuda.current_device())
+    torch.cuda.synchronize()
+    import time
+    t0 = time.perf_counter()
+    tws = get_envelope(d, auto_disp=True)
+    dt = time.perf_counter() - t0
+    print(f"get_envelope 10M on device: {dt * 1e3:.2f} ms")
+    bad = [a for a in ("p", "Dx", "Dxp", "Dy", "Dyp", "x", "y", "px", "py", "tau", "xx", "xpx", "pxpx", "yy",
+                       "ypy", "pypy", "tautau", "xy", "pxpy", "xpy", "ypx", "pp", "emit_x", "emit_y",
+                       "eigemit_1", "eigemit_2", "beta_x", "alpha_x")
+           if not close(a, float(getattr(tws, a)), ref)]
+    assert not bad, [(a, getattr(tws, a), ref[a]) for a in bad]
+    th = get_envelope(host, auto_disp=True)
+    for a in ("xx", "pxpx", "emit_x", "Dx", "tau"):
+        assert getattr(th, a) == getattr(tws, a)
