@@ -252,17 +252,17 @@ __global__ void __launch_bounds__(kNT) k_zinv(const cpx<F>* __restrict__ A, F* _
 // columns per workgroup of the kernels that carry two or three fields
 template <int M>
 struct FieldTile {
-  static constexpr int T2 = Tile<float, M>::T;  // x pass: T forward, 2T inverse
-  static constexpr int S2 = 2 * T2 + 1;
-  static constexpr size_t lds2 = (size_t)M * S2 * sizeof(cpx<float>);
-  static constexpr int T3 = Tile<float, M>::T / 2 < 1 ? 1 : Tile<float, M>::T / 2;  // y inverse: 3 fields
-  static constexpr int S3 = 3 * T3 + 1;
-  static constexpr size_t lds3 = (size_t)M * S3 * sizeof(cpx<float>);
+  static constexpr int T = Tile<float, M>::T;  // columns per workgroup
+  static constexpr int S = T + 1;
+  static constexpr int PER = (M * T + kNT - 1) / kNT;  // spectrum values held per thread
+  static constexpr size_t lds = (size_t)M * S * sizeof(cpx<float>);
 };
 
 // Mixed form, x pass: forward FFT along x, times the kernel spectrum, then
 // two inverse FFTs: Qx = IFFT(S * Dx) (the x field) and P = IFFT(S) (the
 // potential, from which the y and z fields follow by their own factors).
+// The product S*G stays in registers between the two inverse transforms, so
+// LDS holds one tile of T columns (several workgroups per CU).
 template <int M>
 __global__ void __launch_bounds__(kNT) k_xconv_field(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
                                                      cpx<float>* __restrict__ P, const float* __restrict__ G,
@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(kNT) k_xconv_field(const cpx<float>* __restric
                                                      const cpx<float>* __restrict__ tw,
                                                      const cpx<double>* __restrict__ twd) {
   using F = float;
-  constexpr int T = FieldTile<M>::T2, S = FieldTile<M>::S2;
+  constexpr int T = FieldTile<M>::T, S = FieldTile<M>::S, PER = FieldTile<M>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
@@ -289,30 +289,53 @@ __global__ void __launch_bounds__(kNT) k_xconv_field(const cpx<float>* __restric
   lds_fft<F, M, T, kNT, -1, S>(s, tw);
   const int fy = ky <= My - ky ? ky : My - ky;
   const double inv_hx = 1.0 / sc->steps[0];
-  for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int t = i % T, kx = i / T;
-    const int kz = kz0 + t;
-    const int fx = kx <= M - kx ? kx : M - kx;
-    const F g = kz < Hz ? G[((int64_t)fx * Hy + fy) * Hz + kz] : (F)0;
-    const cpx<F> v = cscale(s[kx * S + t], g);
-    s[kx * S + t] = cmul(v, dfac<F>(twd, kx, inv_hx));
-    s[kx * S + T + t] = v;
+  cpx<F> v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * kNT;
+    v[k] = cpx<F>{0, 0};
+    if (M * T % kNT == 0 || i < M * T) {
+      const int t = i % T, kx = i / T;
+      const int kz = kz0 + t;
+      const int fx = kx <= M - kx ? kx : M - kx;
+      const F g = kz < Hz ? G[((int64_t)fx * Hy + fy) * Hz + kz] : (F)0;
+      v[k] = cscale(s[kx * S + t], g);
+    }
   }
   __syncthreads();
-  lds_fft<F, M, 2 * T, kNT, +1, S>(s, tw);
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * kNT;
+    if (M * T % kNT == 0 || i < M * T) {
+      const int t = i % T, kx = i / T;
+      s[kx * S + t] = cmul(v[k], dfac<F>(twd, kx, inv_hx));
+    }
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, +1, S>(s, tw);
   for (int i = threadIdx.x; i < nx * T; i += kNT) {
     const int t = i % T, x = i / T;
     const int kz = kz0 + t;
-    if (kz < Hz) {
-      const int64_t o = x * cs + (int64_t)ky * Hz + kz;
-      Qx[o] = s[x * S + t];
-      P[o] = s[x * S + T + t];
-    }
+    if (kz < Hz) Qx[x * cs + (int64_t)ky * Hz + kz] = s[x * S + t];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * kNT;
+    if (M * T % kNT == 0 || i < M * T) s[(i / T) * S + i % T] = v[k];
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, +1, S>(s, tw);
+  for (int i = threadIdx.x; i < nx * T; i += kNT) {
+    const int t = i % T, x = i / T;
+    const int kz = kz0 + t;
+    if (kz < Hz) P[x * cs + (int64_t)ky * Hz + kz] = s[x * S + t];
   }
 }
 
 // Mixed form, y inverse: Ex' = IFFT_y(Qx), Ey' = IFFT_y(P * Dy), Phi' = IFFT_y(P)
 // (all [nx][ny][Hz]); the z field is Dz * Phi', applied in the z inverse.
+// Three transforms of one LDS tile in turn, P held in registers.
 template <int M>
 __global__ void __launch_bounds__(kNT) k_yinv_field(const cpx<float>* __restrict__ Qx,
                                                     const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax,
@@ -321,7 +344,7 @@ __global__ void __launch_bounds__(kNT) k_yinv_field(const cpx<float>* __restrict
                                                     const cpx<float>* __restrict__ tw,
                                                     const cpx<double>* __restrict__ twd) {
   using F = float;
-  constexpr int T = FieldTile<M>::T3, S = FieldTile<M>::S3;
+  constexpr int T = FieldTile<M>::T, S = FieldTile<M>::S, PER = FieldTile<M>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
@@ -329,31 +352,51 @@ __global__ void __launch_bounds__(kNT) k_yinv_field(const cpx<float>* __restrict
   const int kz0 = blockIdx.x * T;
   const int x = blockIdx.y;
   const double inv_hy = 1.0 / sc->steps[1];
-  for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int t = i % T, ky = i / T;
-    const int kz = kz0 + t;
-    cpx<F> q{0, 0}, p{0, 0};
-    if (kz < Hz) {
-      const int64_t o = ((int64_t)x * M + ky) * Hz + kz;
-      q = Qx[o];
-      p = P[o];
+  cpx<F> p[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * kNT;
+    p[k] = cpx<F>{0, 0};
+    if (M * T % kNT == 0 || i < M * T) {
+      const int t = i % T, ky = i / T;
+      const int kz = kz0 + t;
+      cpx<F> q{0, 0};
+      if (kz < Hz) {
+        const int64_t o = ((int64_t)x * M + ky) * Hz + kz;
+        q = Qx[o];
+        p[k] = P[o];
+      }
+      s[ky * S + t] = q;
     }
-    s[ky * S + t] = q;
-    s[ky * S + T + t] = cmul(p, dfac<F>(twd, ky, inv_hy));
-    s[ky * S + 2 * T + t] = p;
   }
   __syncthreads();
-  lds_fft<F, M, 3 * T, kNT, +1, S>(s, tw);
-  for (int i = threadIdx.x; i < ny * T; i += kNT) {
-    const int t = i % T, y = i / T;
-    const int kz = kz0 + t;
-    if (kz < Hz) {
-      const int64_t o = ((int64_t)x * ny + y) * Hz + kz;
-      Ax[o] = s[y * S + t];
-      Ay[o] = s[y * S + T + t];
-      Ap[o] = s[y * S + 2 * T + t];
+  lds_fft<F, M, T, kNT, +1, S>(s, tw);
+  auto store = [&](cpx<F>* dst) {
+    for (int i = threadIdx.x; i < ny * T; i += kNT) {
+      const int t = i % T, y = i / T;
+      const int kz = kz0 + t;
+      if (kz < Hz) dst[((int64_t)x * ny + y) * Hz + kz] = s[y * S + t];
     }
+  };
+  store(Ax);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * kNT;
+    if (M * T % kNT == 0 || i < M * T) s[(i / T) * S + i % T] = cmul(p[k], dfac<F>(twd, i / T, inv_hy));
   }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, +1, S>(s, tw);
+  store(Ay);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * kNT;
+    if (M * T % kNT == 0 || i < M * T) s[(i / T) * S + i % T] = p[k];
+  }
+  __syncthreads();
+  lds_fft<F, M, T, kNT, +1, S>(s, tw);
+  store(Ap);
 }
 
 // ---------------------------------------------------------------------------

@@ -61,55 +61,53 @@ __global__ void __launch_bounds__(kPT) k_mom(Rows R, const double* __restrict__ 
     if constexpr (PASS == 0) {
       acc[0] += tau;
       acc[1] += q[i];
-      continue;
-    }
-    if constexpr (PASS == 1) {
+    } else if constexpr (PASS == 1) {
       const double d = tau - s.tau_all;
       acc[0] += d * d;
-      continue;
-    }
-    if (s.bounds && !(s.lo <= tau && tau <= s.hi)) continue;
-    double R6[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) R6[k] = k == 4 ? tau : R.r[k * R.ld + i];
-    if constexpr (PASS == 2) {
-      acc[0] += 1.0;
-      acc[1] += R6[5];
-      acc[2] += R6[0];
-      acc[3] += R6[1];
-      acc[4] += R6[2];
-      acc[5] += R6[3];
-      acc[6] += tau;
-    } else if constexpr (PASS == 3) {
-      const double dp = R6[5] - s.m[0];
-      acc[0] += dp * dp;
-      acc[1] += dp * (R6[0] - s.m[1]);
-      acc[2] += dp * (R6[1] - s.m[2]);
-      acc[3] += dp * (R6[2] - s.m[3]);
-      acc[4] += dp * (R6[3] - s.m[4]);
     } else {
-      double x, y, px, py;
-      mom_corrected(R6, s, &x, &y, &px, &py);
-      if constexpr (PASS == 4) {
-        acc[0] += x;
-        acc[1] += y;
-        acc[2] += px;
-        acc[3] += py;
+      if (s.bounds && !(s.lo <= tau && tau <= s.hi)) continue;
+      double R6[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) R6[k] = k == 4 ? tau : R.r[k * R.ld + i];
+      if constexpr (PASS == 2) {
+        acc[0] += 1.0;
+        acc[1] += R6[5];
+        acc[2] += R6[0];
+        acc[3] += R6[1];
+        acc[4] += R6[2];
+        acc[5] += R6[3];
+        acc[6] += tau;
+      } else if constexpr (PASS == 3) {
+        const double dp = R6[5] - s.m[0];
+        acc[0] += dp * dp;
+        acc[1] += dp * (R6[0] - s.m[1]);
+        acc[2] += dp * (R6[1] - s.m[2]);
+        acc[3] += dp * (R6[2] - s.m[3]);
+        acc[4] += dp * (R6[3] - s.m[4]);
       } else {
-        const double dx = x - s.c[0], dy = y - s.c[1], dpx = px - s.c[2], dpy = py - s.c[3];
-        const double dt = tau - s.m[5], dp = R6[5] - s.m[0];
-        acc[0] += dx * dx;
-        acc[1] += dx * dpx;
-        acc[2] += dpx * dpx;
-        acc[3] += dy * dy;
-        acc[4] += dy * dpy;
-        acc[5] += dpy * dpy;
-        acc[6] += dt * dt;
-        acc[7] += dx * dy;
-        acc[8] += dpx * dpy;
-        acc[9] += dx * dpy;
-        acc[10] += dy * dpx;
-        acc[11] += dp * dp;
+        double x, y, px, py;
+        mom_corrected(R6, s, &x, &y, &px, &py);
+        if constexpr (PASS == 4) {
+          acc[0] += x;
+          acc[1] += y;
+          acc[2] += px;
+          acc[3] += py;
+        } else {
+          const double dx = x - s.c[0], dy = y - s.c[1], dpx = px - s.c[2], dpy = py - s.c[3];
+          const double dt = tau - s.m[5], dp = R6[5] - s.m[0];
+          acc[0] += dx * dx;
+          acc[1] += dx * dpx;
+          acc[2] += dpx * dpx;
+          acc[3] += dy * dy;
+          acc[4] += dy * dpy;
+          acc[5] += dpy * dpy;
+          acc[6] += dt * dt;
+          acc[7] += dx * dy;
+          acc[8] += dpx * dpy;
+          acc[9] += dx * dpy;
+          acc[10] += dy * dpx;
+          acc[11] += dp * dp;
+        }
       }
     }
   }

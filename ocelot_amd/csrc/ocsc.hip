@@ -166,6 +166,7 @@ struct ocsc_ctx {
   int profiling = 0;
   // tuning options (ocsc_set_option)
   int deposit_xcd = 0;  // 1: per-XCD grid copies with L2-local atomics
+  int red_blocks = kMaxRedBlocks;  // blocks of the strided reduction passes
   int sort_every = 10;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
   // tracking workspace: two cell-ordered copies of the bunch, sort buffers
   Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp;
@@ -363,16 +364,16 @@ int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, floa
   if (ev) (void)hipEventRecord(ev[2], st);
   OCSC_SWITCH_M(d.mx, {
     using FT = FieldTile<MM>;
-    dim3 grid((d.hz + FT::T2 - 1) / FT::T2, d.my);
-    k_xconv_field<MM><<<grid, kNT, FT::lds2, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
+    dim3 grid((d.hz + FT::T - 1) / FT::T, d.my);
+    k_xconv_field<MM><<<grid, kNT, FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
                                                    c->twd + MM);
   });
   LAUNCHCHK("k_xconv_field");
   if (ev) (void)hipEventRecord(ev[3], st);
   OCSC_SWITCH_M(d.my, {
     using FT = FieldTile<MM>;
-    dim3 grid((d.hz + FT::T3 - 1) / FT::T3, d.nx);
-    k_yinv_field<MM><<<grid, kNT, FT::lds3, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
+    dim3 grid((d.hz + FT::T - 1) / FT::T, d.nx);
+    k_yinv_field<MM><<<grid, kNT, FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
                                                   c->twd + MM);
   });
   LAUNCHCHK("k_yinv_field");
@@ -407,9 +408,12 @@ KickInputs kick_inputs(const int32_t* nmesh, int64_t n, double E_GeV, double zst
   return in;
 }
 
-int nred_blocks(int64_t n) {
+// grid of the strided reduction passes (A, B, moments): one full round of
+// resident workgroups (no tail of half-empty rounds), at most one per 256
+// particles
+int nred_blocks(const ocsc_ctx* c, int64_t n) {
   int64_t b = (n + kPT - 1) / kPT;
-  return (int)(b < kMaxRedBlocks ? (b < 1 ? 1 : b) : kMaxRedBlocks);
+  return (int)(b < c->red_blocks ? (b < 1 ? 1 : b) : c->red_blocks);
 }
 
 // the launch sequence of one kick (no host synchronisation, graph-capturable)
@@ -431,7 +435,7 @@ template <typename F, int NWORD>
 int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld, const double* q, const KickInputs& in,
                  int prec, hipStream_t st, const StepOpts& o = StepOpts()) {
   const int64_t n = in.n;
-  const int nblk = nred_blocks(n);
+  const int nblk = nred_blocks(c, n);
   const int npart = (int)((n + kPT - 1) / kPT);
   Rows R{r, ld};
   const bool prof = c->profiling != 0;
@@ -444,6 +448,11 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   if (o.passA) {
     k_passA<<<nblk, kPT, 0, st>>>(R, L.sc, (double*)L.part.p);
     k_reduceA<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk);
+  } else if (o.prev_nblk > kMaxRedBlocks) {
+    // one partial per kick block: fold them on many CUs first
+    double* folded = (double*)L.part.p + (size_t)3 * o.prev_nblk;
+    k_fold<3><<<kFoldBlocks, 256, 0, st>>>((const double*)L.part.p, o.prev_nblk, folded);
+    k_reduceA<<<1, 1024, 0, st>>>(L.sc, folded, kFoldBlocks);
   } else {
     k_reduceA<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, o.prev_nblk);
   }
@@ -517,7 +526,8 @@ int enqueue_kick_prec(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_
 }
 
 int ensure_part(ocsc_ctx* c, Lane& L, int64_t n) {
-  const size_t need = std::max<size_t>((size_t)kMaxRedBlocks * kNPartB, (size_t)3 * ((n + kPT - 1) / kPT));
+  const size_t need =
+      std::max<size_t>((size_t)kMaxRedBlocks * kNPartB, (size_t)3 * ((n + kPT - 1) / kPT) + 3 * kFoldBlocks);
   return ensure(L.part, need * sizeof(double));
 }
 
@@ -630,6 +640,13 @@ int ocsc_create(ocsc_ctx** out, int device) {
   for (int k = 0; k < 8; ++k) hf[k] = cpx<float>{1, 0};
   if (hipMemcpy(c->twd, hd, sizeof(hd), hipMemcpyHostToDevice) != hipSuccess) return cleanup(fail(OCSC_EHIP, "copy"));
   if (hipMemcpy(c->twf, hf, sizeof(hf), hipMemcpyHostToDevice) != hipSuccess) return cleanup(fail(OCSC_EHIP, "copy"));
+  {
+    int ncu = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_passB<true>, kPT, 0) == hipSuccess && ncu > 0 &&
+        per_cu > 0)
+      c->red_blocks = std::min(kMaxRedBlocks, ncu * per_cu);
+  }
   KickScalars init{};
   init.key_prec = -1;
   if (hipMemcpy(L.sc, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess)
@@ -734,7 +751,7 @@ int ocsc_el_field(ocsc_ctx* c, const double* xyz, int64_t ld, const double* q, i
   } xcd_off{c, c->deposit_xcd};
   c->deposit_xcd = 0;
   in.gamma0 = gamma;
-  const int nblk = nred_blocks(n);
+  const int nblk = nred_blocks(c, n);
   const int npart = (int)((n + kPT - 1) / kPT);
   Rows R{xyz, ld};
   k_setup<<<1, 64, 0, st>>>(L.sc, in);
@@ -846,7 +863,7 @@ int ocsc_moments(ocsc_ctx* c, const double* r, int64_t ld, const double* q, int6
     HIPCHK(hipMalloc(&c->mom, sizeof(MomState)));
     HIPCHK(hipHostMalloc(&c->mom_host, sizeof(MomState), hipHostMallocDefault));
   }
-  const int nblk = nred_blocks(n);
+  const int nblk = nred_blocks(c, n);
   int rc;
   if ((rc = ensure(L.part, (size_t)12 * nblk * sizeof(double)))) return rc;
   // the pinned mirror may still be read by an earlier copy on another stream
@@ -961,7 +978,7 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   const double* qq = q;
   if (sorted) {
     // kick scalars of the current bunch for the first sort keys
-    const int nblk = nred_blocks(n);
+    const int nblk = nred_blocks(c, n);
     k_setup<<<1, 64, 0, st>>>(L.sc, in);
     k_passA<<<nblk, kPT, 0, st>>>(Rows{r, ld}, L.sc, (double*)L.part.p);
     k_reduceA<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk);

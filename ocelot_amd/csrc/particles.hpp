@@ -22,7 +22,8 @@
 namespace ocsc {
 
 constexpr int kPT = 256;          // threads per workgroup, particle passes
-constexpr int kMaxRedBlocks = 8192;  // partial-sum blocks of passes A and B
+constexpr int kMaxRedBlocks = 2048;  // partial-sum blocks of passes A and B
+constexpr int kFoldBlocks = 128;     // first-level blocks folding the kick's per-block momentum sums
 constexpr int kNPartB = 11;          // partials per block in pass B
 constexpr int kNumXcd = 8;           // grid copies of the per-XCD deposit
 
@@ -161,15 +162,23 @@ __device__ __forceinline__ void block_reduce(double* v, double* lds) {
   for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
     for (int c = 0; c < NV; ++c) v[c] = red_op<KIND>(c, v[c], __shfl_down(v[c], off, 64));
-  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if ((threadIdx.x & 63) == 0)
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
 #pragma unroll
     for (int c = 0; c < NV; ++c) lds[w * NV + c] = v[c];
   __syncthreads();
-  if (threadIdx.x == 0)
-    for (int k = 1; k < nw; ++k)
+  // the waves' results combined by the same shuffle tree in wave 0
+  if (w == 0) {
 #pragma unroll
-      for (int c = 0; c < NV; ++c) v[c] = red_op<KIND>(c, v[c], lds[k * NV + c]);
+    for (int c = 0; c < NV; ++c) {
+      const double id = (KIND == 1 && c < 3) ? __builtin_inf() : ((KIND == 1 && c < 6) ? -__builtin_inf() : 0.0);
+      v[c] = lane < nw ? lds[lane * NV + c] : id;
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1)
+#pragma unroll
+      for (int c = 0; c < NV; ++c) v[c] = red_op<KIND>(c, v[c], __shfl_down(v[c], off, 64));
+  }
 }
 
 // partials are stored transposed, part[c * nblk + b], so the final reduction
@@ -200,6 +209,26 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
   block_reduce<NV, KIND>(v, lds);
 #pragma unroll
   for (int c = 0; c < NV; ++c) out[c] = v[c];
+}
+
+// First level of a two-level sum over many block partials: block f of the
+// grid adds its contiguous chunk of partials (in increasing order per thread,
+// then the fixed block tree), so a single-block reduction only reads gridDim
+// values per component.  Deterministic for a given (nblk, gridDim).
+template <int NV>
+__global__ void __launch_bounds__(256) k_fold(const double* __restrict__ part, int nblk, double* __restrict__ out) {
+  __shared__ double lds[4 * NV];
+  const int nf = gridDim.x, chunk = (nblk + nf - 1) / nf;
+  const int b0 = blockIdx.x * chunk, b1 = b0 + chunk < nblk ? b0 + chunk : nblk;
+  double v[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) v[c] = 0;
+  for (int b = b0 + threadIdx.x; b < b1; b += 256)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) v[c] += part[c * nblk + b];
+  block_reduce<NV, 0>(v, lds);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < NV; ++c) out[c * nf + blockIdx.x] = v[c];
 }
 
 // pass A: per-block sums of the momenta (sc.py:224 numerator)
@@ -261,16 +290,17 @@ __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict_
   double sq = 0, saq = 0, sqx[3] = {0, 0, 0};
   int bad = 0;
   const int64_t n = s.n;
-  for (int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x; p < n; p += (int64_t)gridDim.x * kPT) {
+  // two particles per iteration, both loaded before either is used (more
+  // loads in flight); accumulated in the order of the plain strided loop
+  const int64_t stride = (int64_t)gridDim.x * kPT;
+  auto body = [&](const double* v, double qq) {
     double xp[6], xyz[3];
     if constexpr (FAST)
-      to_xp_fast(R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p], R.r[5 * R.ld + p], s,
-                 xp);
+      to_xp_fast(v[0], v[1], v[2], v[3], v[4], v[5], s, xp);
     else
-      to_xp(R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p], R.r[5 * R.ld + p], s, xp);
+      to_xp(v[0], v[1], v[2], v[3], v[4], v[5], s, xp);
     rot(xp, s.T, xyz);
     xyz[2] = xyz[2] * s.gamma0;
-    const double qq = q[p];
     bad |= !(__builtin_isfinite(xyz[0]) && __builtin_isfinite(xyz[1]) && __builtin_isfinite(xyz[2]) &&
              __builtin_isfinite(qq));
 #pragma unroll
@@ -281,6 +311,21 @@ __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict_
     }
     sq += qq;
     saq += __builtin_fabs(qq);
+  };
+  for (int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x; p < n; p += 2 * stride) {
+    const int64_t p2 = p + stride;
+    const bool two = p2 < n;
+    double a[6], b[6], qa, qb = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a[k] = R.r[k * R.ld + p];
+    qa = q[p];
+    if (two) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) b[k] = R.r[k * R.ld + p2];
+      qb = q[p2];
+    }
+    body(a, qa);
+    if (two) body(b, qb);
   }
   if (bad) atomicOr(&scp->err, (int)kErrNonFinite);
   double vals[kNPartB] = {mn[0], mn[1], mn[2], mx[0], mx[1], mx[2], sq, saq, sqx[0], sqx[1], sqx[2]};
@@ -432,17 +477,28 @@ __global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restric
   __syncthreads();
   const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (kPT * kDepItems);
   int bad = 0;
-  for (int it = 0; it < kDepItems; ++it) {
-    const int64_t p = base + it * kPT + threadIdx.x;
+  // the rows of item it + 1 are loaded before item it goes through the table
+  double cur[7], nxt[7] = {0, 0, 0, 0, 0, 0, 0};
+  auto load = [&](int64_t p, double* v) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = R.r[k * R.ld + p];
+    v[6] = q[p];
+  };
+  int64_t p = base + threadIdx.x;
+  if (p < s.n) load(p, cur);
+  for (int it = 0; it < kDepItems; ++it, p += kPT) {
     if (p >= s.n) break;
-    const double R6[6] = {R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p],
-                          R.r[5 * R.ld + p]};
-    const int64_t idx = ngp_cell<NWORD == 1>(R6, s);  // one word: the mixed form
+    const int64_t pn = p + kPT;
+    if (it + 1 < kDepItems && pn < s.n) load(pn, nxt);
+    const int64_t idx = ngp_cell<NWORD == 1>(cur, s);  // one word: the mixed form
+    const double qv = cur[6];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) cur[k] = nxt[k];
     if (idx < 0) {
       bad = 1;
       continue;
     }
-    const double v = q[p] * s.qscale;  // exact: qscale is a power of two
+    const double v = qv * s.qscale;  // exact: qscale is a power of two
     const double h = __builtin_rint(v);
     const long long hi = (long long)h;
     long long lo = 0;
