@@ -570,7 +570,10 @@ __global__ void __launch_bounds__(256) k_permute_gather(const double* __restrict
                                                         const double* __restrict__ qs, const uint32_t* __restrict__ perm,
                                                         double* __restrict__ dst, int64_t ldd, double* __restrict__ qd,
                                                         int64_t n) {
-  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  // one contiguous run of destinations per XCD: after a re-sort the sources
+  // of a run come from the same region of the mesh, so the lines they share
+  // are fetched into one L2
+  const int64_t j = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (j >= n) return;
   const int64_t p = perm[j];
 #pragma unroll
