@@ -150,8 +150,10 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * summed exactly in the first FFT pass.  Both give bit-identical results.
  * "sort_every": ocsc_track_drift keeps the bunch in mesh-cell order and
  * re-sorts it every k steps (0: never; default 10).  "overlap": 1
- * builds the Green's-function spectrum on a second stream concurrently with
- * the charge deposit, 0 (default: they contend for the CUs) runs it in line. */
+ * (default) builds the Green's-function spectrum on a second stream
+ * concurrently with the charge deposit (the FP64 Green's function is
+ * arithmetic-bound, the deposit memory-bound), 0 runs it in line.  Both give
+ * bit-identical results. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 

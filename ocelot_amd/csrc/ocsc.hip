@@ -161,7 +161,7 @@ struct ocsc_ctx {
   hipStream_t cap_stream = nullptr;  // stream used for graph capture
   hipStream_t side = nullptr;        // second stream: kernel spectrum next to the deposit
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int overlap = 0;                   // 1: build the kernel spectrum concurrently with the deposit
+  int overlap = 1;                   // 1: build the kernel spectrum concurrently with the deposit
   // stage timing (ocsc_set_profiling): events between the stages of a kick
   int profiling = 0;
   // tuning options (ocsc_set_option)

@@ -364,6 +364,30 @@ def test_per_xcd_deposit_is_bit_identical(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_overlapped_kernel_spectrum_is_bit_identical(prec):
+    """The Green's-function spectrum built on a second stream next to the
+    deposit (overlap on, joined before the first FFT pass) gives the same
+    kicks as the in-line build, eagerly and in captured tracking graphs."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=300_000, seed=12)
+    dev = torch.cuda.current_device()
+    old = _lib.get_option(dev, "overlap")
+    outs = {}
+    try:
+        for ov in (0, 1):
+            _lib.set_option(dev, "overlap", ov)
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            sc = SpaceCharge(nmesh_xyz=[31, 31, 63], precision=prec)
+            sc.apply(d, dz)
+            track_drift(d, sc, dz, dz, 5, use_graph=True)
+            outs[ov] = d.rparticles.cpu().numpy()
+    finally:
+        _lib.set_option(dev, "overlap", old)
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_sorted_tracking_matches_unsorted(prec):
     """Tracking keeps a cell-ordered copy of the bunch (re-sorted every k
     steps) and restores the caller's order at the end; only reduction orders
