@@ -36,9 +36,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="flattop_10m_127")
     ap.add_argument("--particles", type=int, default=None, help="override the particle count")
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000,
+    ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="particles in the CPU-baseline kick (0: skip)")
-    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--profile-steps", type=int, default=None,
+                    help="eager steps timed per stage with HIP events (default: one re-sort cycle)")
     ap.add_argument("--sort-every", type=int, default=10,
                     help="re-sort the resident bunch by mesh cell every k tracking steps (0: never)")
     ap.add_argument("--no-fp64", action="store_true")
@@ -75,28 +76,33 @@ def algorithmic_bytes(n, nmesh, prec):
     return part + mesh, kick
 
 
-def cpu_baseline(name, n_sample, seed=0):
-    """One FP64 oracle kick (numpy/scipy, the reference's fallback library
-    calls) on a particle sample of the workload, BLAS limited to one thread."""
+def cpu_baseline(name, n_sample, steps=3, seed=0):
+    """FP64 oracle tracking steps (numpy/scipy, the reference's fallback
+    library calls): ``steps`` x [drift, kick] on a particle sample of the
+    workload (the full bunch by default), BLAS limited to one thread."""
     from oracle import sc_oracle as orc
     from ocelot_amd.workloads import make_bunch
     try:
         from threadpoolctl import threadpool_limits
     except ImportError:  # pragma: no cover
         threadpool_limits = None
-    # same distribution and total charge, fewer macroparticles
+    # same distribution and total charge
     r, q, E, nmesh, dz = make_bunch(name, n=n_sample, seed=seed)
+
+    def run():
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            orc.drift(r, dz, E)
+            orc.apply(r, q, E, dz, nmesh)
+        return time.perf_counter() - t0
+
     if threadpool_limits is not None:
         with threadpool_limits(limits=1):
-            t0 = time.perf_counter()
-            orc.apply(r, q, E, dz, nmesh)
-            dt = time.perf_counter() - t0
+            dt = run()
     else:
-        t0 = time.perf_counter()
-        orc.apply(r, q, E, dz, nmesh)
-        dt = time.perf_counter() - t0
-    return dict(value=n_sample / dt, unit="macroparticle-kicks/s", cores=1, kind="port",
-                sample=f"{name}: one FP64 oracle kick of {n_sample} particles of the same distribution on "
+        dt = run()
+    return dict(value=n_sample * steps / dt, unit="macroparticle-kicks/s", cores=1, kind="port",
+                sample=f"{name}: {steps} FP64 oracle steps (drift {dz} m + kick) of {n_sample} particles on "
                        f"{nmesh[0]}^3, {dt:.2f} s")
 
 
@@ -181,6 +187,10 @@ def main():
             return int(a.deposit_xcd), {}
         res, times = [], {}
         nt = 40  # enough steps that the per-call sort and scatter do not dominate
+        # first use of the tracking buffers (allocation, graph capture) outside the timing
+        warm = DeviceParticleArray.from_host(host, dev)
+        track_drift(warm, SpaceCharge(nmesh_xyz=list(nmesh), precision="mixed"), L, dz, 12, use_graph=True)
+        del warm
         for mode in (0, 1):
             _lib.set_option(dev, "deposit_xcd", mode)
             sc = SpaceCharge(nmesh_xyz=list(nmesh), precision="mixed")
@@ -203,23 +213,37 @@ def main():
     _lib.set_option(dev, "deposit_xcd", deposit_mode)
     print(f"# deposit_xcd={deposit_mode} {deposit_tune}", file=sys.stderr, flush=True)
 
+    import ctypes
+    lib = _lib.load()
+    ctx = _lib.context(dev)
+
     def run(prec, steps, warmup):
+        """Timed loop; the kick kernel of every step is also timed with HIP
+        events captured into the step graphs (kick_timing), on its stream."""
         sc = SpaceCharge(nmesh_xyz=list(nmesh), precision=prec)
         d = DeviceParticleArray.from_host(host, dev)
         track_drift(d, sc, L, dz, warmup, use_graph=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        track_drift(d, sc, L, dz, steps, use_graph=True)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        return (t1 - t0) / steps, d
+        _lib.set_option(dev, "kick_timing", 1)
+        try:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            track_drift(d, sc, L, dz, steps, use_graph=True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+        finally:
+            _lib.set_option(dev, "kick_timing", 0)
+        ms = (ctypes.c_double * steps)()
+        cnt = ctypes.c_int64()
+        _lib.check(lib.ocsc_kick_times(ctx.handle, ms, steps, ctypes.byref(cnt)))
+        kick_ms = float(np.mean(ms[:cnt.value])) if cnt.value else None
+        return (t1 - t0) / steps, kick_ms
 
     print(f"# workload {a.workload}: {n} particles, mesh {nmesh}", file=sys.stderr, flush=True)
-    t_mix, _ = run("mixed", a.steps, a.warmup)
+    t_mix, kick_ms_live = run("mixed", a.steps, a.warmup)
     print(f"# mixed: {t_mix * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
     t_64 = None
     if not a.no_fp64:
-        t_64, _ = run("fp64", a.steps, a.warmup)
+        t_64, _k64 = run("fp64", a.steps, a.warmup)
         print(f"# fp64: {t_64 * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
 
     # field error of one kick, mixed vs FP64, at full size (kick deltas are the
@@ -238,18 +262,16 @@ def main():
     field_rms = max(errs)
     del outs, r0
 
-    # per-stage HIP events over a few eager tracking steps (same kicks as the
-    # timed region, same stream; profiling switches the graph off)
-    import ctypes
-    lib = _lib.load()
-    ctx = _lib.context(dev)
+    # per-stage HIP events over one re-sort cycle of eager tracking steps (same
+    # kicks as the timed region, same stream; profiling switches the graph off)
     stages = {}
     prof = {}
     for prec in ("mixed", "fp64"):
         sc = SpaceCharge(nmesh_xyz=list(nmesh), precision=prec)
         d = DeviceParticleArray.from_host(host, dev)
         _lib.check(lib.ocsc_set_profiling(ctx.handle, 1))
-        track_drift(d, sc, L, dz, a.profile_steps, use_graph=True)
+        # a whole re-sort cycle, so the per-stage means match the timed loop's mix
+        track_drift(d, sc, L, dz, a.profile_steps or max(a.sort_every, 1) + 1, use_graph=True)
         ms = (ctypes.c_double * 16)()
         calls = ctypes.c_int64()
         ns = lib.ocsc_stage_times(ctx.handle, ms, 16, ctypes.byref(calls))
@@ -258,13 +280,25 @@ def main():
         prof[prec] = [ms[i] for i in range(ns)]
     total_b, kick_b = algorithmic_bytes(n, nmesh, "mixed")
     total_b64, _ = algorithmic_bytes(n, nmesh, "fp64")
-    kick_ms = stages["mixed"]["kick"]
+    kick_ms = kick_ms_live
     dom = max(stages["mixed"], key=lambda k: stages["mixed"][k])
     achieved = kick_b / (kick_ms * 1e-3) / 1e9
+    traffic, traffic_src = None, None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         f"k_kick_{a.workload}_traffic.json")
+    if os.path.exists(tpath) and a.particles is None:
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        traffic = tr["traffic_bytes"]
+        traffic_src = (f"profiles/{os.path.basename(tpath)}: rocprofv3 FETCH_SIZE x{tr['calibration']['fetch_scale']}"
+                       f" + WRITE_SIZE x{tr['calibration']['write_scale']} per dispatch (copy-kernel calibration)")
     roofline = {"kernel": "k_kick<float> (gather + kick + write-back)", "bound": "hbm",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel_ms": kick_ms, "algorithmic_bytes": kick_b, "largest_stage": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                "kernel_ms": round(kick_ms, 4),
+                "kernel_ms_source": "HIP events around k_kick captured into the step graphs of the timed loop "
+                                    "(mean over its steps, on the kernel's stream)",
+                "algorithmic_bytes": kick_b, "largest_stage": dom,
                 "kick_total_bytes": total_b,
                 "kick_total_GBs": round(total_b / (sum(prof["mixed"]) * 1e-3) / 1e9, 1),
                 "kick_total_bytes_fp64": total_b64,

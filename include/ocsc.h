@@ -153,7 +153,8 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * (default) builds the Green's-function spectrum on a second stream
  * concurrently with the charge deposit (the FP64 Green's function is
  * arithmetic-bound, the deposit memory-bound), 0 runs it in line.  Both give
- * bit-identical results. */
+ * bit-identical results.  "kick_timing": 1 times the kick kernel of every
+ * tracking step (see ocsc_kick_times), 0 (default) does not. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 
@@ -166,6 +167,14 @@ int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 int ocsc_set_profiling(ocsc_ctx* ctx, int on);
 int ocsc_stage_times(ocsc_ctx* ctx, double* ms, int nout, int64_t* calls);
 const char* ocsc_stage_name(int i);
+
+/* Kick-kernel durations of the last ocsc_track_drift run with option
+ * "kick_timing" = 1: a pair of HIP events per step is recorded around the
+ * kick kernel on the caller's stream, as event-record nodes inside the
+ * captured graphs (the timed loop itself, not a separate profiling run).
+ * Writes min(nout, nsteps) milliseconds and the step count.  No reference
+ * counterpart. */
+int ocsc_kick_times(ocsc_ctx* ctx, double* ms, int nout, int64_t* count);
 
 /* Scalars of the last kick, for diagnostics and parity tests (synchronises):
  * out[0] gamma0, [1] Pav, [2..10] T (row-major, T[j][k]), [11..13] steps,

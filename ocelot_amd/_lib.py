@@ -46,6 +46,7 @@ SIGNATURES = {
     "ocsc_set_profiling": (_c_int, [_vp, _c_int]),
     "ocsc_stage_times": (_c_int, [_vp, _dp, _c_int, ctypes.POINTER(_i64)]),
     "ocsc_stage_name": (ctypes.c_char_p, [_c_int]),
+    "ocsc_kick_times": (_c_int, [_vp, _dp, _c_int, ctypes.POINTER(_i64)]),
 }
 
 _lib = None

@@ -173,9 +173,21 @@ struct ocsc_ctx {
   // beam moments: device state and its pinned host mirror
   MomState* mom = nullptr;
   MomState* mom_host = nullptr;
-  hipEvent_t ev[kNumStages + 1] = {};
+  // A ring of event sets, one per kick in flight: kicks are timed without a
+  // host wait after each one (which would idle the GPU between kicks); a set
+  // is read back when its slot comes round again or when the times are asked.
+  static constexpr int kEvRing = 64;
+  hipEvent_t ev[kEvRing][kNumStages + 1] = {};
+  bool ev_pending[kEvRing] = {};
+  int64_t ev_seq = 0;
   double stage_ms[kNumStages] = {};
   int64_t stage_calls = 0;
+  // kick timing (option "kick_timing"): a pair of events per tracking step,
+  // recorded around the kick kernel inside the captured graphs, and the
+  // per-step kick durations of the last ocsc_track_drift
+  int kick_timing = 0;
+  std::vector<hipEvent_t> kick_ev;
+  std::vector<double> kick_ms;
 };
 
 namespace {
@@ -416,6 +428,27 @@ int nred_blocks(const ocsc_ctx* c, int64_t n) {
   return (int)(b < c->red_blocks ? (b < 1 ? 1 : b) : c->red_blocks);
 }
 
+// read back one timed kick of the profiling ring
+int flush_slot(ocsc_ctx* c, int k) {
+  if (!c->ev_pending[k]) return OCSC_OK;
+  HIPCHK(hipEventSynchronize(c->ev[k][kNumStages]));
+  for (int i = 0; i < kNumStages; ++i) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[k][i], c->ev[k][i + 1]));
+    c->stage_ms[i] += ms;
+  }
+  c->stage_calls += 1;
+  c->ev_pending[k] = false;
+  return OCSC_OK;
+}
+
+int flush_all(ocsc_ctx* c) {
+  int rc;
+  for (int64_t j = c->ev_seq - ocsc_ctx::kEvRing; j < c->ev_seq; ++j)
+    if (j >= 0 && (rc = flush_slot(c, (int)(j % ocsc_ctx::kEvRing)))) return rc;
+  return OCSC_OK;
+}
+
 // the launch sequence of one kick (no host synchronisation, graph-capturable)
 // options of one kick inside a tracking loop
 struct StepOpts {
@@ -424,6 +457,8 @@ struct StepOpts {
   bool drift = false;  // apply the next step's drift in the write-back
   bool sump = false;   // leave this kick's momentum sums for the next kick
   DriftArgs dr{0, 0};
+  hipEvent_t kev0 = nullptr, kev1 = nullptr;  // recorded around the kick kernel (kick timing)
+  bool kev_external = false;                  // capturing: the graph gets event-record nodes instead
 };
 
 template <typename F, bool DRIFT, bool SUMP>
@@ -439,8 +474,14 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   const int npart = (int)((n + kPT - 1) / kPT);
   Rows R{r, ld};
   const bool prof = c->profiling != 0;
+  const int slot = (int)(c->ev_seq % ocsc_ctx::kEvRing);
+  hipEvent_t* ev = c->ev[slot];
+  if (prof) {
+    int rc0;
+    if ((rc0 = flush_slot(c, slot))) return rc0;
+  }
   auto mark = [&](int i) {
-    if (prof) (void)hipEventRecord(c->ev[i], st);
+    if (prof) (void)hipEventRecord(ev[i], st);
   };
   mark(0);
   k_setup<<<1, 64, 0, st>>>(L.sc, in);
@@ -490,13 +531,14 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
     return rc;
   mark(4);
   if constexpr (sizeof(F) == 4) {
-    if ((rc = launch_fields(c, L, g, L.grid.p, (float*)L.phi.p, st, prof ? c->ev + 4 : nullptr))) return rc;
+    if ((rc = launch_fields(c, L, g, L.grid.p, (float*)L.phi.p, st, prof ? ev + 4 : nullptr))) return rc;
   } else {
     if ((rc = launch_convolution<F, NWORD == 2 ? kChargeFix2 : kChargeFix1>(c, L, g, L.grid.p, (F*)L.phi.p, st,
-                                                                              prof ? c->ev + 4 : nullptr)))
+                                                                              prof ? ev + 4 : nullptr)))
       return rc;
   }
   mark(9);
+  if (o.kev0 && !o.kev_external) HIPCHK(hipEventRecord(o.kev0, st));
   if (o.drift && o.sump)
     launch_kick<F, true, true>(c, L, npart, r, ld, o, st);
   else if (o.drift)
@@ -506,15 +548,11 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   else
     launch_kick<F, false, false>(c, L, npart, r, ld, o, st);
   LAUNCHCHK("k_kick");
+  if (o.kev1 && !o.kev_external) HIPCHK(hipEventRecord(o.kev1, st));
   mark(10);
   if (prof) {
-    HIPCHK(hipEventSynchronize(c->ev[10]));
-    for (int i = 0; i < kNumStages; ++i) {
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
-      c->stage_ms[i] += ms;
-    }
-    c->stage_calls += 1;
+    c->ev_pending[slot] = true;
+    c->ev_seq += 1;
   }
   return OCSC_OK;
 }
@@ -653,8 +691,6 @@ int ocsc_create(ocsc_ctx** out, int device) {
     return cleanup(fail(OCSC_EHIP, "copy"));
   if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(OCSC_EHIP, "stream"));
-  for (int i = 0; i <= kNumStages; ++i)
-    if (hipEventCreate(&c->ev[i]) != hipSuccess) return cleanup(fail(OCSC_EHIP, "event"));
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(OCSC_EHIP, "stream"));
   if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
@@ -681,8 +717,10 @@ int ocsc_destroy(ocsc_ctx* c) {
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-  for (int i = 0; i <= kNumStages; ++i)
-    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  for (auto& set : c->ev)
+    for (hipEvent_t e : set)
+      if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->kick_ev) (void)hipEventDestroy(e);
   for (Lane* l : c->lanes) {
     free_lane(*l);
     delete l;
@@ -994,14 +1032,29 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   // fused into kick s, so the sequence is drift_0, then kicks 0..n-1 with
   // kick s < n-1 drifting (first kick: own pass A; later kicks: partials).
   const int npart = (int)((n + kPT - 1) / kPT);
+  const bool ktime = c->kick_timing != 0 && zstep != 0;
+  if (ktime) {
+    while (c->kick_ev.size() < (size_t)2 * nsteps) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c->kick_ev.push_back(e);
+    }
+  }
+  c->kick_ms.clear();
   auto opts_for = [&](int step) {
     StepOpts o;
     o.passA = step == 0;
     o.prev_nblk = npart;
     o.drift = o.sump = step < nsteps - 1;
     o.dr = DriftArgs{Ld, r56};
+    if (ktime) {
+      o.kev0 = c->kick_ev[2 * step];
+      o.kev1 = c->kick_ev[2 * step + 1];
+    }
     return o;
   };
+  // graph event-record nodes of the kick events, per captured graph
+  hipGraphNode_t knode[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   if (zstep == 0) {
     for (int step = 0; step < nsteps; ++step) k_drift<<<nb, 256, 0, st>>>(rr, lr, n, Ld, r56);
   } else {
@@ -1036,18 +1089,56 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
     }
     const int gi = sorted ? cur : 0;
     if (!exec[gi]) {
+      StepOpts oc = o;
+      oc.kev_external = true;
       HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-      int rc2 = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, c->cap_stream, o);
+      int rc2 = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, c->cap_stream, oc);
       hipError_t e = hipStreamEndCapture(c->cap_stream, &graphs[gi]);
       if (rc2 || e != hipSuccess) {
         cleanup();
         return rc2 ? rc2 : fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
+      }
+      if (ktime) {
+        // the kick is the graph's only sink (every other node feeds it);
+        // bracket it with event-record nodes: deps -> start -> kick -> end
+        size_t nn = 0;
+        HIPCHK(hipGraphGetNodes(graphs[gi], nullptr, &nn));
+        std::vector<hipGraphNode_t> nodes(nn);
+        HIPCHK(hipGraphGetNodes(graphs[gi], nodes.data(), &nn));
+        hipGraphNode_t kick = nullptr;
+        int nsink = 0;
+        for (hipGraphNode_t nd : nodes) {
+          size_t nout = 0;
+          HIPCHK(hipGraphNodeGetDependentNodes(nd, nullptr, &nout));
+          if (nout == 0) {
+            kick = nd;
+            ++nsink;
+          }
+        }
+        hipGraphNodeType t;
+        if (nsink != 1 || hipGraphNodeGetType(kick, &t) != hipSuccess || t != hipGraphNodeTypeKernel) {
+          cleanup();
+          return fail(OCSC_EHIP, "kick timing: the captured step graph has no single kernel sink");
+        }
+        size_t nd = 0;
+        HIPCHK(hipGraphNodeGetDependencies(kick, nullptr, &nd));
+        std::vector<hipGraphNode_t> deps(nd);
+        HIPCHK(hipGraphNodeGetDependencies(kick, deps.data(), &nd));
+        std::vector<hipGraphNode_t> to(nd, kick);
+        if (nd) HIPCHK(hipGraphRemoveDependencies(graphs[gi], deps.data(), to.data(), nd));
+        HIPCHK(hipGraphAddEventRecordNode(&knode[gi][0], graphs[gi], deps.data(), nd, o.kev0));
+        HIPCHK(hipGraphAddDependencies(graphs[gi], &knode[gi][0], &kick, 1));
+        HIPCHK(hipGraphAddEventRecordNode(&knode[gi][1], graphs[gi], &kick, 1, o.kev1));
       }
       e = hipGraphInstantiate(&exec[gi], graphs[gi], nullptr, nullptr, 0);
       if (e != hipSuccess) {
         cleanup();
         return fail(OCSC_EHIP, std::string("instantiate: ") + hipGetErrorString(e));
       }
+    }
+    if (ktime) {
+      HIPCHK(hipGraphExecEventRecordNodeSetEvent(exec[gi], knode[gi][0], o.kev0));
+      HIPCHK(hipGraphExecEventRecordNodeSetEvent(exec[gi], knode[gi][1], o.kev1));
     }
     hipError_t e = hipGraphLaunch(exec[gi], st);
     if (e != hipSuccess) {
@@ -1063,7 +1154,23 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   rc = read_errors(c, L, st, &err);
   cleanup();
   if (rc) return rc;
+  if (ktime) {
+    HIPCHK(hipEventSynchronize(c->kick_ev[2 * nsteps - 1]));
+    c->kick_ms.resize(nsteps);
+    for (int step = 0; step < nsteps; ++step) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->kick_ev[2 * step], c->kick_ev[2 * step + 1]));
+      c->kick_ms[step] = ms;
+    }
+  }
   if (err) return value_error(err);
+  return OCSC_OK;
+}
+
+int ocsc_kick_times(ocsc_ctx* c, double* ms, int nout, int64_t* count) {
+  if (!c || (!ms && nout > 0)) return fail(OCSC_EINVAL, "null argument");
+  for (int i = 0; i < nout && i < (int)c->kick_ms.size(); ++i) ms[i] = c->kick_ms[i];
+  if (count) *count = (int64_t)c->kick_ms.size();
   return OCSC_OK;
 }
 
@@ -1179,6 +1286,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->sort_every = (int)value;
     return OCSC_OK;
   }
+  if (n == "kick_timing") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "kick_timing must be 0 or 1");
+    c->kick_timing = (int)value;
+    return OCSC_OK;
+  }
   if (n == "overlap") {
     if (value != 0 && value != 1) return fail(OCSC_EINVAL, "overlap must be 0 or 1");
     c->overlap = (int)value;
@@ -1202,11 +1314,22 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
     *value = c->overlap;
     return OCSC_OK;
   }
+  if (n == "kick_timing") {
+    *value = c->kick_timing;
+    return OCSC_OK;
+  }
   return fail(OCSC_EINVAL, "unknown option " + n);
 }
 
 int ocsc_set_profiling(ocsc_ctx* c, int on) {
   if (!c) return fail(OCSC_EINVAL, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  if (on && !c->ev[0][0])
+    for (auto& set : c->ev)
+      for (hipEvent_t& e : set)
+        if (hipEventCreate(&e) != hipSuccess) return fail(OCSC_EHIP, "event");
+  int rc;
+  if ((rc = flush_all(c))) return rc;  // settle kicks still in flight, then start afresh
   c->profiling = on ? 1 : 0;
   for (int i = 0; i < kNumStages; ++i) c->stage_ms[i] = 0;
   c->stage_calls = 0;
@@ -1215,6 +1338,8 @@ int ocsc_set_profiling(ocsc_ctx* c, int on) {
 
 int ocsc_stage_times(ocsc_ctx* c, double* ms, int nout, int64_t* calls) {
   if (!c || (!ms && nout > 0)) return fail(OCSC_EINVAL, "null argument");
+  int rc;
+  if ((rc = flush_all(c))) return rc;
   for (int i = 0; i < nout && i < kNumStages; ++i)
     ms[i] = c->stage_calls ? c->stage_ms[i] / (double)c->stage_calls : 0.0;
   if (calls) *calls = c->stage_calls;

@@ -387,6 +387,34 @@ def test_overlapped_kernel_spectrum_is_bit_identical(prec):
     assert np.array_equal(outs[0], outs[1])
 
 
+def test_kick_timing_inside_graphs():
+    """With kick timing on, event-record nodes around the kick are captured
+    into the step graphs and re-pointed at a fresh event pair on every replay:
+    one positive duration per step, and the same kicks as without timing."""
+    import ctypes
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=300_000, seed=14)
+    dev = torch.cuda.current_device()
+    lib = _lib.load()
+    outs = {}
+    try:
+        for kt in (0, 1):
+            _lib.set_option(dev, "kick_timing", kt)
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            track_drift(d, SpaceCharge(nmesh_xyz=[31, 31, 63], precision="mixed"), dz, dz, 25, use_graph=True)
+            outs[kt] = d.rparticles.cpu().numpy()
+    finally:
+        _lib.set_option(dev, "kick_timing", 0)
+    ms = (ctypes.c_double * 64)()
+    cnt = ctypes.c_int64()
+    _lib.check(lib.ocsc_kick_times(_lib.context(dev).handle, ms, 64, ctypes.byref(cnt)))
+    t = np.array(ms[:cnt.value])
+    print(f"kick times over 25 steps: mean {t.mean() * 1e3:.1f} us, min {t.min() * 1e3:.1f}, max {t.max() * 1e3:.1f}")
+    assert cnt.value == 25 and np.all(t > 0) and np.all(t < 1000)
+    assert np.array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_sorted_tracking_matches_unsorted(prec):
     """Tracking keeps a cell-ordered copy of the bunch (re-sorted every k
