@@ -104,19 +104,23 @@ int ocsc_track_drift(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double
 /* Beam moments behind get_envelope (ocelot/cpbd/beam/analysis.py:27-224),
  * computed on the device over the six rows and q_array.  flags:
  * OCSC_MOM_BOUNDS selects the particles with
- * mean(tau) + b0 std(tau) <= tau <= mean(tau) + b1 std(tau) (analysis.py:59-67,
- * slice=None); OCSC_MOM_AUTO_DISP estimates the dispersion from the selected
+ * z0 + b0 std(tau) <= tau <= z0 + b1 std(tau) (analysis.py:59-67), z0 =
+ * mean(tau) (slice=None) or, with OCSC_MOM_IMAX as well, the peak of the
+ * smoothed current profile (slice="Imax", analysis.py:61-64, 266-340);
+ * OCSC_MOM_AUTO_DISP estimates the dispersion from the selected
  * particles (analysis.py:90-98); OCSC_MOM_DISP_GIVEN takes disp = {Dx, Dxp,
  * Dy, Dyp} (analysis.py:99-103); otherwise D = 0.  out (host) receives
  * min(nout, OCSC_MOM_NOUT) values: [0] selected count, [1] sum of all q,
  * [2] <p>, [3..6] Dx Dxp Dy Dyp, [7..11] <x> <y> <px> <py> <tau> of the
  * corrected coordinates, [12..23] xx xpx pxpx yy ypy pypy tautau xy pxpy xpy
- * ypx pp, [24] mean(tau) and [25] std(tau) over all particles.  With fewer
- * than 3 selected particles only [0..2] are meaningful.  Synchronous. */
+ * ypx pp, [24] mean(tau) and [25] std(tau) over all particles, [26] the
+ * window centre z0.  With fewer than 3 selected particles only [0..2] are
+ * meaningful.  Synchronous. */
 #define OCSC_MOM_BOUNDS 1
 #define OCSC_MOM_AUTO_DISP 2
 #define OCSC_MOM_DISP_GIVEN 4
-#define OCSC_MOM_NOUT 26
+#define OCSC_MOM_IMAX 8
+#define OCSC_MOM_NOUT 27
 int ocsc_moments(ocsc_ctx* ctx, const double* rparticles, int64_t ld, const double* q_array, int64_t n, int flags,
                  const double disp[4], double b0, double b1, double* out, int nout, void* stream);
 

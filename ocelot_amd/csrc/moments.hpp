@@ -29,6 +29,7 @@ struct MomState {
   double var_p;
   double c[4];        // means of the corrected x, y, px, py
   double mom[12];     // xx, xpx, pxpx, yy, ypy, pypy, tautau, xy, pxpy, xpy, ypx, pp
+  double tau_min, tau_max;  // extent of tau (slice = "Imax")
 };
 
 constexpr int kMomNV[6] = {2, 1, 7, 5, 4, 12};
@@ -145,6 +146,82 @@ __global__ void __launch_bounds__(1024) k_mom_reduce(MomState* __restrict__ msp,
   } else {
     for (int k = 0; k < 12; ++k) s.mom[k] = t[k] / s.cnt;
   }
+}
+
+// ---------------------------------------------------------------------------
+// slice = "Imax" (analysis.py:61-64): the window is centred on the peak of the
+// current profile s_to_cur(tau, 0.01 std(tau), ...) (analysis.py:266-340):
+// first-order (cloud-in-cell) weights of every particle on a uniform grid,
+// then a Gaussian smoothing (on the host, a few thousand bins).
+
+// extent of tau over all particles
+__global__ void __launch_bounds__(kPT) k_tau_range(Rows R, int64_t n, double* __restrict__ part) {
+  __shared__ double lds[(kPT / 64) * 6];
+  const double inf = __builtin_inf();
+  double v[6] = {inf, inf, inf, -inf, -inf, -inf};
+  for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
+    const double t = R.r[4 * R.ld + i];
+    v[0] = t < v[0] ? t : v[0];
+    v[3] = t > v[3] ? t : v[3];
+  }
+  block_reduce<6, 1>(v, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = v[0];
+    part[gridDim.x + blockIdx.x] = v[3];
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_tau_range_reduce(MomState* __restrict__ msp, const double* __restrict__ part,
+                                                           int nblk) {
+  __shared__ double lds[16 * 6];
+  const double inf = __builtin_inf();
+  double v[6] = {inf, inf, inf, -inf, -inf, -inf};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    v[0] = part[b] < v[0] ? part[b] : v[0];
+    v[3] = part[nblk + b] > v[3] ? part[nblk + b] : v[3];
+  }
+  block_reduce<6, 1>(v, lds);
+  if (threadIdx.x == 0) {
+    msp->tau_min = v[0];
+    msp->tau_max = v[3];
+  }
+}
+
+// Cloud-in-cell weights (analysis.py:254-260, 316-321) as exact fixed-point
+// integers: bin I = floor((tau - a)/ds) gets xi = 1 + I - c, bin I + 1 gets
+// 1 - xi, each scaled by wscale (a power of two) and rounded.  One LDS
+// histogram per workgroup (nbins <= kCicLds), written out whole; k_cic_sum adds
+// the workgroups' histograms bin by bin.
+constexpr int kCicLds = 8192;
+
+// bins [j0, j0 + nb) of the nbins-bin profile (windows keep the LDS bounded)
+__global__ void __launch_bounds__(kPT) k_cic(Rows R, int64_t n, double a, double ds, int nbins, int j0, int nb,
+                                             double wscale, long long* __restrict__ hist) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  long long* h = (long long*)smem_raw;
+  for (int j = threadIdx.x; j < nb; j += kPT) h[j] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
+    const double c = (R.r[4 * R.ld + i] - a) / ds;
+    const double fl = __builtin_floor(c);
+    const double xi = 1 + fl - c;
+    int I = (int)(fl < 0 ? 0 : (fl > nbins - 2 ? nbins - 2 : fl));
+    if (I >= j0 && I < j0 + nb)
+      atomicAdd((unsigned long long*)&h[I - j0], (unsigned long long)(long long)__builtin_rint(xi * wscale));
+    if (I + 1 >= j0 && I + 1 < j0 + nb)
+      atomicAdd((unsigned long long*)&h[I + 1 - j0], (unsigned long long)(long long)__builtin_rint((1 - xi) * wscale));
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nb; j += kPT) hist[(int64_t)blockIdx.x * nb + j] = h[j];
+}
+
+__global__ void __launch_bounds__(256) k_cic_sum(const long long* __restrict__ hist, int nblk, int nbins,
+                                                 long long* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= nbins) return;
+  long long acc = 0;
+  for (int b = 0; b < nblk; ++b) acc += hist[(int64_t)b * nbins + j];
+  out[j] = acc;
 }
 
 }  // namespace ocsc

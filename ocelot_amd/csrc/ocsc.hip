@@ -173,6 +173,7 @@ struct ocsc_ctx {
   // beam moments: device state and its pinned host mirror
   MomState* mom = nullptr;
   MomState* mom_host = nullptr;
+  Buf cic;  // current-profile histograms (slice = "Imax")
   // A ring of event sets, one per kick in flight: kicks are timed without a
   // host wait after each one (which would idle the GPU between kicks); a set
   // is read back when its slot comes round again or when the times are asked.
@@ -705,7 +706,7 @@ int ocsc_destroy(ocsc_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
-                 &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp})
+                 &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic})
     if (b->p) (void)hipFree(b->p);
   if (L.sc) (void)hipFree(L.sc);
   if (L.sc_host) (void)hipHostFree(L.sc_host);
@@ -888,11 +889,96 @@ int ocsc_drift(ocsc_ctx* c, double* r, int64_t ld, int64_t n, double L, double E
   return OCSC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Centre of the peak-current window (analysis.py:61-64): the current profile
+// s_to_cur(tau, 0.01 std(tau), ...) (analysis.py:266-340) on its own grid,
+// a = min - 3 sigma, b = max + 3 sigma, ds = (b - a)/ceil((b - a)/(sigma/4)),
+// cloud-in-cell weights summed on the device as fixed-point integers, then the
+// Gaussian smoothing (2K+1 taps, K = floor(3 sigma/ds + 1/2)) and the first
+// maximum on the host; returns z0 = i ds + a and stores the window in the
+// device state.
+int imax_window(ocsc_ctx* c, const Rows& R, int64_t n, hipStream_t st, double* z0_out) {
+  const int nblk = nred_blocks(c, n);
+  double* part = (double*)c->lane0.part.p;
+  k_tau_range<<<nblk, kPT, 0, st>>>(R, n, part);
+  k_tau_range_reduce<<<1, 1024, 0, st>>>(c->mom, part, nblk);
+  LAUNCHCHK("k_tau_range");
+  MomState& h = *c->mom_host;
+  HIPCHK(hipMemcpyAsync(&h, c->mom, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const double sig0 = h.std_all;
+  const double sigma = 0.01 * sig0;
+  if (!(sigma > 0) || !std::isfinite(sigma)) return fail(OCSC_EVALUE, "slice='Imax' needs a spread in tau");
+  const double a = h.tau_min - 3 * sigma, b = h.tau_max + 3 * sigma;
+  const double ds0 = 0.25 * sigma;
+  const double nf = std::ceil((b - a) / ds0);
+  if (!(nf >= 1) || nf > 1e8) return fail(OCSC_EVALUE, "slice='Imax': current profile grid out of range");
+  const int N = (int)nf;
+  const double ds = (b - a) / N;
+  const int nbins = N + 1;
+  int e = 0;
+  (void)std::frexp((double)n, &e);
+  const double wscale = std::ldexp(1.0, 62 - e);  // n * wscale < 2^62
+  const int ncic = 256;
+  const int win = std::min(nbins, kCicLds);
+  int rc;
+  if ((rc = ensure(c->cic, ((size_t)ncic * win + nbins) * sizeof(long long)))) return rc;
+  long long* hist = (long long*)c->cic.p;
+  long long* tot = hist + (size_t)ncic * win;
+  // windows of at most kCicLds bins (one for any ordinary bunch)
+  for (int j0 = 0; j0 < nbins; j0 += win) {
+    const int nb = std::min(win, nbins - j0);
+    k_cic<<<ncic, kPT, (size_t)nb * sizeof(long long), st>>>(R, n, a, ds, nbins, j0, nb, wscale, hist);
+    k_cic_sum<<<(nb + 255) / 256, 256, 0, st>>>(hist, ncic, nb, tot + j0);
+  }
+  LAUNCHCHK("k_cic");
+  std::vector<long long> C(nbins);
+  HIPCHK(hipMemcpyAsync(C.data(), tot, (size_t)nbins * sizeof(long long), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const int K = (int)std::floor(3 * sigma / ds + 0.5);
+  std::vector<double> G(2 * K + 1);
+  double gs = 0;
+  for (int m = 0; m <= 2 * K; ++m) {
+    const double x = (double)(m - K) * ds / sigma;
+    G[m] = std::exp(-0.5 * (x * x));
+    gs += G[m];
+  }
+  int imax = 0;
+  double best = -1;
+  for (int j = 0; j < nbins; ++j) {
+    double acc = 0;
+    for (int m = 0; m <= 2 * K; ++m) {
+      const int i = j + K - m;
+      if (i >= 0 && i < nbins) acc += (double)C[i] * G[m];
+    }
+    acc = acc / gs;
+    if (acc > best) {
+      best = acc;
+      imax = j;
+    }
+  }
+  const double z0 = (double)imax * ds + a;
+  const double lohi[2] = {z0 + sig0 * h.b0, z0 + sig0 * h.b1};
+  HIPCHK(hipMemcpyAsync((char*)c->mom + offsetof(MomState, lo), lohi, sizeof(lohi), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *z0_out = z0;
+  return OCSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ocsc_moments(ocsc_ctx* c, const double* r, int64_t ld, const double* q, int64_t n, int flags,
                  const double disp[4], double b0, double b1, double* out, int nout, void* stream) {
   if (!c || !out || ((!r || !q) && n > 0)) return fail(OCSC_EINVAL, "null argument");
   if (n < 0 || ld < n) return fail(OCSC_EINVAL, "bad n / ld");
   if ((flags & OCSC_MOM_DISP_GIVEN) && !disp) return fail(OCSC_EINVAL, "OCSC_MOM_DISP_GIVEN needs disp");
+  if ((flags & OCSC_MOM_IMAX) && !(flags & OCSC_MOM_BOUNDS))
+    return fail(OCSC_EINVAL, "OCSC_MOM_IMAX selects the window centre and needs OCSC_MOM_BOUNDS");
   if (n == 0) return fail(OCSC_EVALUE, "empty bunch");
   HIPCHK(hipSetDevice(c->device));
   Lane& L = c->lane0;
@@ -920,9 +1006,15 @@ int ocsc_moments(ocsc_ctx* c, const double* r, int64_t ld, const double* q, int6
   const Rows R{r, ld};
   k_mom<0><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
   k_mom_reduce<0><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+  double z0 = 0;
   if (h.bounds) {
     k_mom<1><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
     k_mom_reduce<1><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+    if (flags & OCSC_MOM_IMAX) {
+      if ((rc = imax_window(c, R, n, st, &z0))) return rc;
+    } else {
+      z0 = -1;  // mean(tau), set below from the device state
+    }
   }
   k_mom<2><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
   k_mom_reduce<2><<<1, 1024, 0, st>>>(c->mom, part, nblk);
@@ -947,6 +1039,7 @@ int ocsc_moments(ocsc_ctx* c, const double* r, int64_t ld, const double* q, int6
   for (int k = 0; k < 12; ++k) v[12 + k] = h.mom[k];
   v[24] = h.tau_all;
   v[25] = h.std_all;
+  v[26] = (flags & OCSC_MOM_IMAX) && h.bounds ? z0 : h.tau_all;
   for (int k = 0; k < nout && k < OCSC_MOM_NOUT; ++k) out[k] = v[k];
   return OCSC_OK;
 }

@@ -536,11 +536,11 @@ __global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restric
 // Cell ordering of a resident bunch (multi-kick tracking): sort keys are the
 // NGP cells under the scalars of the last kick; particles move little between
 // re-sorts, so consecutive particles keep sharing cells.
-// Sort keys: bricks of kBrick cells (4 x 4 x 8, z fastest), bricks in C
+// Sort keys: bricks of kBrick cells (8 x 8 x 4, z fastest), bricks in C
 // order, cells in C order inside a brick, so a run of consecutive particles
 // fills a compact 3D box of the mesh.  Keys only steer locality, so the fast
 // transforms are used; out-of-mesh particles sort last.
-constexpr int kBx = 4, kBy = 4, kBz = 8;
+constexpr int kBx = 8, kBy = 8, kBz = 4;
 
 __host__ __device__ inline int64_t brick_key_count(int nx, int ny, int nz) {
   const int64_t nbx = (nx + kBx - 1) / kBx, nby = (ny + kBy - 1) / kBy, nbz = (nz + kBz - 1) / kBz;

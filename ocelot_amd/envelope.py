@@ -13,8 +13,10 @@ matrix Sigma and the symplectic form S are +-i e1, +-i e2, so e1^2 + e2^2 =
 -tr((Sigma S)^2)/2 and e1^2 e2^2 = det(Sigma); e1 >= e2 are the roots of that
 quadratic (the reference evaluates the same two roots in closed form).
 
-``slice="Imax"`` (window centred on the peak-current slice, which needs the
-reference's current profile ``s_to_cur``) is not supported and raises.
+With ``bounds``, ``slice="Imax"`` centres the window on the peak of the
+smoothed current profile (the reference's ``s_to_cur`` with width
+0.01 std(tau)): the cloud-in-cell profile is summed on the device, the
+smoothing and the peak search run on the host over its few thousand bins.
 """
 import numpy as np
 
@@ -56,7 +58,7 @@ def eigen_emittances(Sigma):
     return float(abs(r1.real)), float(abs(r2.real))
 
 
-def moments(p_array, bounds=None, auto_disp=False, disp=None, device=None):
+def moments(p_array, bounds=None, auto_disp=False, disp=None, device=None, imax=False):
     """Raw output of ocsc_moments (OCSC_MOM_NOUT values) for a bunch."""
     import ctypes
     import torch
@@ -67,12 +69,14 @@ def moments(p_array, bounds=None, auto_disp=False, disp=None, device=None):
     if bounds is not None:
         flags |= 1
         b0, b1 = float(bounds[0]), float(bounds[1])
+        if imax:
+            flags |= 8
     if disp is not None:
         flags |= 4
     elif auto_disp:
         flags |= 2
     dbuf = (ctypes.c_double * 4)(*(disp if disp is not None else (0.0, 0.0, 0.0, 0.0)))
-    out = (ctypes.c_double * 26)()
+    out = (ctypes.c_double * 27)()
     if isinstance(r, torch.Tensor) and r.is_cuda:
         if r.dtype != torch.float64 or r.stride(1) != 1 or r.shape[0] != 6:
             raise TypeError("device rparticles must be a (6, n) float64 tensor with contiguous rows")
@@ -91,7 +95,7 @@ def moments(p_array, bounds=None, auto_disp=False, disp=None, device=None):
     with torch.cuda.device(dev):
         stream = torch.cuda.current_stream(dev).cuda_stream
         _lib.check(lib.ocsc_moments(_lib.context(dev).handle, rt.data_ptr(), ld, q.data_ptr(), n, flags, dbuf,
-                                    b0, b1, out, 26, stream))
+                                    b0, b1, out, 27, stream))
     return np.array(out[:])
 
 
@@ -99,12 +103,11 @@ def get_envelope(p_array, tws_i=None, bounds=None, slice=None, auto_disp=False, 
     """Twiss parameters of a bunch (analysis.py:27-224) from moments taken on
     the GPU.  Returns a ``Twiss``; with fewer than 3 selected particles only
     E, q and p are set, as in the reference."""
-    if bounds is not None and slice == "Imax":
-        raise NotImplementedError("slice='Imax' (peak-current window) is not supported on the GPU path")
     disp = None
     if tws_i is not None:
         disp = (tws_i.Dx, tws_i.Dxp, tws_i.Dy, tws_i.Dyp)
-    v = moments(p_array, bounds=bounds, auto_disp=auto_disp and tws_i is None, disp=disp, device=device)
+    v = moments(p_array, bounds=bounds, auto_disp=auto_disp and tws_i is None, disp=disp, device=device,
+                imax=slice == "Imax")
     tws = Twiss()
     tws.E = float(p_array.E)
     tws.q = float(v[1])

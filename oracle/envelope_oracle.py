@@ -7,7 +7,9 @@ tws_i or estimated with auto_disp, the px/py correction factor, the twelve
 centred moments, emittances, Twiss beta/alpha).  Only ``tests/`` may import
 it; the product path never does.
 
-Pinning (tests/test_envelope_oracle.py): a point set with exactly known mean
+Pinning (tests/test_envelope_oracle.py): golden vectors written by the
+reference (incl. slice="Imax"), the current profile against a brute-force
+per-particle loop and its total charge, a point set with exactly known mean
 and covariance (every moment, emittance and beta/alpha in closed form), an
 uncoupled beam (eigen-emittances = the two plane emittances), a beam built
 with known dispersion (auto_disp recovers it), brute-force window selection,
@@ -18,15 +20,44 @@ import numpy as np
 M_E_GEV = 9.10938215e-31 * 299792458.0 ** 2 / 1.6021766208e-19 / 1e+9
 
 
-def moments(r, q, bounds=None, auto_disp=False, disp=None):
+def s_to_cur(A, sigma, q0, v=299792458.0):
+    """Current profile (analysis.py:266-340, default grid): cloud-in-cell
+    weights of the positions A on a grid from min - 3 sigma to max + 3 sigma
+    with step sigma/4 (rounded to a whole number of steps), smoothed by a
+    normalised Gaussian of 2K+1 taps (K = floor(3 sigma/ds + 1/2)), the
+    'same'-size centred convolution, scaled to amperes.  Returns (N+1, 2)."""
+    a = np.min(A) - 3 * sigma
+    b = np.max(A) + 3 * sigma
+    N = int(np.ceil((b - a) / (0.25 * sigma)))
+    ds = (b - a) / N
+    B = np.zeros((N + 1, 2))
+    B[:, 0] = np.arange(0, (N + 0.5) * ds, ds) + a
+    cA = (A - a) / ds
+    I = np.int_(np.floor(cA))
+    xi = 1 + I - cA
+    C = np.bincount(I, xi, N + 1) + np.bincount(I + 1, 1 - xi, N + 1)
+    K = int(np.floor(3 * sigma / ds + 0.5))
+    G = np.exp(-0.5 * (np.arange(-K, K + 1) * ds / sigma) ** 2)
+    G = G / np.sum(G)
+    B[:, 1] = np.convolve(C, G)[K:N + 1 + K]
+    B[:, 1] = q0 * v / (ds * np.sum(B[:, 1])) * B[:, 1]
+    return B
+
+
+def moments(r, q, bounds=None, auto_disp=False, disp=None, slice=None):
     """Dictionary of the quantities get_envelope computes from the particles
     (analysis.py:58-158): count, q, p, D*, corrected means, second moments."""
     tau = r[4]
     out = {"q": float(np.sum(q))}
     sel = np.ones(r.shape[1], dtype=bool)
-    if bounds is not None:                      # analysis.py:59-67, slice=None
+    if bounds is not None:                      # analysis.py:59-67
         sig0 = np.std(tau)
-        z0 = np.mean(tau)
+        if slice == "Imax":
+            B = s_to_cur(tau, 0.01 * sig0, np.sum(q))
+            z0 = B[np.argmax(B[:, 1]), 0]
+        else:
+            z0 = np.mean(tau)
+        out["z0"] = float(z0)
         sel = (z0 + sig0 * bounds[0] <= tau) & (tau <= z0 + sig0 * bounds[1])
     x, px, y, py, tau, p = r[0][sel], r[1][sel], r[2][sel], r[3][sel], tau[sel], r[5][sel]
     out["count"] = int(sel.sum())
@@ -92,9 +123,9 @@ def sigma_matrix(mo):
                      [mo["xpy"], mo["ypy"], mo["pxpy"], mo["pypy"]]])
 
 
-def get_envelope(r, q, E, bounds=None, auto_disp=False, disp=None):
+def get_envelope(r, q, E, bounds=None, auto_disp=False, disp=None, slice=None):
     """All of the above: the dictionary of Twiss attributes get_envelope sets."""
-    mo = moments(r, q, bounds, auto_disp, disp)
+    mo = moments(r, q, bounds, auto_disp, disp, slice)
     if mo["count"] < 3:
         return mo
     return twiss(mo, E)

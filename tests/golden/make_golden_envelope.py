@@ -8,7 +8,8 @@ Usage:  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden_envelope.py
 
 Bunches (seeded, FP64): a correlated Gaussian with dispersion and x-y
 coupling (5000 particles) and a flat-top (3000).  Settings per bunch: plain,
-bounds=[-2, 1] (slice=None), auto_disp=True, tws_i with given dispersion;
+bounds=[-2, 1] (slice=None), auto_disp=True, tws_i with given dispersion,
+bounds=[-0.5, 0.5] with slice="Imax" (window on the current peak);
 plus a 2-particle bunch (fewer than 3: only E, q, p are set).  numexpr is
 absent here, so the reference's numpy branch ran (recorded in meta).
 """
@@ -73,7 +74,7 @@ def main():
     tws_i = Twiss()
     tws_i.Dx, tws_i.Dxp, tws_i.Dy, tws_i.Dyp = 0.3, 0.01, -0.2, 0.005
     settings = {"plain": {}, "bounds": {"bounds": [-2.0, 1.0]}, "autodisp": {"auto_disp": True},
-                "twsi": {"tws_i": tws_i}}
+                "twsi": {"tws_i": tws_i}, "imax": {"bounds": [-0.5, 0.5], "slice": "Imax"}}
     for name, (r, q, E) in {"gauss": (*bunch_gauss(5000, 1), 0.13), "flat": (*bunch_flat(3000, 2), 0.7)}.items():
         out[f"{name}/r"] = r
         out[f"{name}/q"] = q
@@ -85,6 +86,12 @@ def main():
     out["two/plain"] = run(r, q, 0.13)
     out["disp/tws_i"] = np.array([0.3, 0.01, -0.2, 0.005])
     out["bounds/b"] = np.array([-2.0, 1.0])
+    out["imax/b"] = np.array([-0.5, 0.5])
+    for name in ("gauss", "flat"):
+        from ocelot.cpbd.beam.analysis import s_to_cur
+        tau = out[f"{name}/r"][4]
+        B = s_to_cur(tau, 0.01 * np.std(tau), np.sum(out[f"{name}/q"]))
+        out[f"{name}/cur"] = B
     path = os.path.join(HERE, "envelope_golden.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

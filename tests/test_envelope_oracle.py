@@ -18,11 +18,45 @@ def env_golden():
 
 def settings(g):
     return {"plain": {}, "bounds": {"bounds": list(g["bounds/b"])}, "autodisp": {"auto_disp": True},
-            "twsi": {"disp": tuple(g["disp/tws_i"])}}
+            "twsi": {"disp": tuple(g["disp/tws_i"])}, "imax": {"bounds": list(g["imax/b"]), "slice": "Imax"}}
 
 
 @pytest.mark.parametrize("bunch", ["gauss", "flat"])
-@pytest.mark.parametrize("setting", ["plain", "bounds", "autodisp", "twsi"])
+def test_current_profile_matches_reference(env_golden, bunch):
+    g = env_golden
+    tau, q = g[f"{bunch}/r"][4], g[f"{bunch}/q"]
+    B = eo.s_to_cur(tau, 0.01 * np.std(tau), np.sum(q))
+    ref = g[f"{bunch}/cur"]
+    assert B.shape == ref.shape
+    assert np.max(np.abs(B[:, 0] - ref[:, 0])) <= 1e-15 * np.max(np.abs(ref[:, 0]))
+    assert np.max(np.abs(B[:, 1] - ref[:, 1])) <= 1e-12 * np.max(np.abs(ref[:, 1]))
+
+
+def test_current_profile_brute_force():
+    """Cloud-in-cell weights by a per-particle loop, the smoothing by a direct
+    sum; the integral of the current over s is the total charge times v."""
+    rng = np.random.default_rng(3)
+    A = rng.normal(0, 1e-4, 300)
+    sigma = 2e-6
+    B = eo.s_to_cur(A, sigma, 1e-9, v=1.0)
+    a, ds, n = B[0, 0], B[1, 0] - B[0, 0], B.shape[0]
+    C = np.zeros(n)
+    for x in A:
+        c = (x - a) / ds
+        i = int(np.floor(c))
+        C[i] += 1 + i - c
+        C[i + 1] += c - i
+    K = int(np.floor(3 * sigma / ds + 0.5))
+    G = np.exp(-0.5 * (np.arange(-K, K + 1) * ds / sigma) ** 2)
+    G /= G.sum()
+    S = np.array([sum(C[j + K - m] * G[m] for m in range(2 * K + 1) if 0 <= j + K - m < n) for j in range(n)])
+    S *= 1e-9 / (ds * S.sum())
+    assert np.max(np.abs(B[:, 1] - S)) <= 1e-9 * np.max(S)
+    assert np.sum(B[:, 1]) * ds == pytest.approx(1e-9, rel=1e-12)
+
+
+@pytest.mark.parametrize("bunch", ["gauss", "flat"])
+@pytest.mark.parametrize("setting", ["plain", "bounds", "autodisp", "twsi", "imax"])
 def test_oracle_matches_reference(env_golden, bunch, setting):
     g = env_golden
     attrs = [str(a) for a in g["meta/attrs"]]

@@ -62,11 +62,13 @@ def kwargs_for(g, setting):
         t = Twiss()
         t.Dx, t.Dxp, t.Dy, t.Dyp = (float(v) for v in g["disp/tws_i"])
         return {"tws_i": t}
+    if setting == "imax":
+        return {"bounds": list(g["imax/b"]), "slice": "Imax"}
     return {}
 
 
 @pytest.mark.parametrize("bunch", ["gauss", "flat"])
-@pytest.mark.parametrize("setting", ["plain", "bounds", "autodisp", "twsi"])
+@pytest.mark.parametrize("setting", ["plain", "bounds", "autodisp", "twsi", "imax"])
 def test_envelope_golden(env_golden, bunch, setting):
     from ocelot_amd import get_envelope
     g = env_golden
@@ -85,6 +87,24 @@ def test_envelope_two_particles(env_golden):
     tws = get_envelope(parray(g["two/r"], g["two/q"], g["two/E"]))
     assert tws.q == pytest.approx(ref["q"], rel=1e-15) and tws.p == pytest.approx(ref["p"], rel=1e-14)
     assert tws.xx == 0.0 and tws.emit_x == 0.0
+
+
+def test_envelope_imax_10m_vs_oracle():
+    """slice='Imax' on the 10M flat-top bunch (two current horns): the window
+    centre is the oracle's current peak and the selected moments agree."""
+    from ocelot_amd import DeviceParticleArray, get_envelope
+    from ocelot_amd.envelope import moments
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, _, _ = make_bunch("hollow_50m_255", n=2_000_000, seed=3)
+    ref = eo.get_envelope(r, q, E, bounds=[-0.3, 0.3], slice="Imax")
+    d = DeviceParticleArray.from_host(parray(r, q, E), torch.cuda.current_device())
+    v = moments(d, bounds=[-0.3, 0.3], imax=True)
+    print(f"Imax window centre: device {v[26]:.9e}, oracle {ref['z0']:.9e}; selected {int(v[0])} vs {ref['count']}")
+    assert v[26] == pytest.approx(ref["z0"], abs=1e-15) and int(v[0]) == ref["count"]
+    tws = get_envelope(d, bounds=[-0.3, 0.3], slice="Imax")
+    bad = [a for a in ("p", "x", "y", "px", "py", "tau", "xx", "pxpx", "yy", "pypy", "tautau", "emit_x", "beta_x")
+           if not close(a, float(getattr(tws, a)), ref)]
+    assert not bad, [(a, getattr(tws, a), ref[a]) for a in bad]
 
 
 def test_envelope_10m_device_resident_vs_oracle():

@@ -123,11 +123,8 @@ def test_particle_array_layout():
     assert np.all(p.tau() == 1.0)
 
 
-def test_twiss_defaults_and_imax_rejected():
-    """Twiss starts at zero like the reference's (beam/core.py:16-75); the
-    peak-current window (slice='Imax') is refused before any GPU work."""
-    from ocelot_amd import ParticleArray, Twiss, get_envelope
+def test_twiss_defaults():
+    """Twiss starts at zero like the reference's (beam/core.py:16-75)."""
+    from ocelot_amd import Twiss
     t = Twiss()
     assert t.emit_x == 0.0 and t.beta_x == 0.0 and t.Dx == 0.0 and t.xx == 0.0 and t.q == 0.0
-    with pytest.raises(NotImplementedError):
-        get_envelope(ParticleArray(10), bounds=[-1, 1], slice="Imax")
