@@ -108,8 +108,9 @@ def cpu_baseline(name, n_sample, steps=3, seed=0):
 
 def other_configs(torch, dev):
     """Context lines (not the headline): the 64-bunch batched configuration
-    (uneven sizes, meshes 31/47/63, 20 kicks, mixed) and single drop-in kicks
-    of the 2M-particle 63^3 bunch (FP64, device-resident and host numpy)."""
+    (uneven sizes, meshes 31/47/63, 20 kicks, mixed), single drop-in kicks
+    of the 2M-particle 63^3 bunch (FP64, device-resident and host numpy), and
+    the headline tracking loop with per-step Twiss (track()'s tws_track)."""
     from ocelot_amd import DeviceParticleArray, ParticleArray, SpaceCharge, apply_batched
     from ocelot_amd.workloads import batch64, make_bunch
     out = {}
@@ -152,6 +153,23 @@ def other_configs(torch, dev):
                                      "ms_per_kick_host_numpy": round(dth * 1e3, 3),
                                      "value_host_numpy_incl_pcie": round(r.shape[1] / dth, 1),
                                      "unit": "macroparticle-kicks/s", "precision": "fp64", "order": "caller's"}
+    del d, hp
+    # the headline loop with track()'s per-step Twiss (moments on the device)
+    from ocelot_amd import track_drift
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127")
+    host = ParticleArray()
+    host.rparticles, host.q_array, host.E = r, q, E
+    d = DeviceParticleArray.from_host(host, dev)
+    sc = SpaceCharge(nmesh_xyz=list(nmesh), precision="mixed")
+    track_drift(d, sc, dz, dz, 3, calc_tws=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tws = track_drift(d, sc, dz, dz, 20, calc_tws=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 20
+    out["flattop_10m_127_track_with_twiss"] = {"particles": r.shape[1], "steps": 20, "ms_per_step": round(dt * 1e3, 4),
+                                               "value": round(r.shape[1] / dt, 1), "unit": "macroparticle-kicks/s",
+                                               "twiss_rows": len(tws), "precision": "mixed"}
     return out
 
 

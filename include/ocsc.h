@@ -124,6 +124,17 @@ int ocsc_track_drift(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double
 int ocsc_moments(ocsc_ctx* ctx, const double* rparticles, int64_t ld, const double* q_array, int64_t n, int flags,
                  const double disp[4], double b0, double b1, double* out, int nout, void* stream);
 
+/* ocsc_track_drift with the beam moments of ocsc_moments taken before the
+ * first step and after every step (after the kick, before the next drift),
+ * as track() does with get_envelope (ocelot/cpbd/track.py:462-485).  mom_flags,
+ * disp, b0, b1 as in ocsc_moments (OCSC_MOM_IMAX is not available here).
+ * tws_out (host) receives (nsteps + 1) rows of OCSC_MOM_NOUT values.  The
+ * particles end up exactly as with ocsc_track_drift. */
+int ocsc_track_drift_tws(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,
+                         const int32_t nmesh[3], double E_GeV, double zstep, double L, int nsteps, int precision,
+                         int use_graph, int mom_flags, const double disp[4], double b0, double b1, double* tws_out,
+                         void* stream);
+
 /* One bunch of a batched call (fields as in ocsc_apply). */
 typedef struct ocsc_bunch {
   double* rparticles;

@@ -38,6 +38,8 @@ SIGNATURES = {
     "ocsc_potential": (_c_int, [_vp, _vp, _i32p, _dp, _vp, _c_int, _vp]),
     "ocsc_drift": (_c_int, [_vp, _vp, _i64, _i64, _dbl, _dbl, _vp]),
     "ocsc_track_drift": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int, _c_int, _vp]),
+    "ocsc_track_drift_tws": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int, _c_int,
+                                      _c_int, _dp, _dbl, _dbl, _dp, _vp]),
     "ocsc_last_stats": (_c_int, [_vp, _dp, _c_int, _vp]),
     "ocsc_apply_batched": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "ocsc_moments": (_c_int, [_vp, _vp, _i64, _vp, _i64, _c_int, _dp, _dbl, _dbl, _dp, _c_int, _vp]),

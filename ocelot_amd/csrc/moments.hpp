@@ -148,6 +148,39 @@ __global__ void __launch_bounds__(1024) k_mom_reduce(MomState* __restrict__ msp,
   }
 }
 
+// Per-step moments inside a tracking loop (graph-capturable: no host copies).
+// k_mom_init fills the working state from scalars; k_mom_store appends it to
+// the step table at a device-side counter.
+__global__ void k_mom_init(MomState* __restrict__ s, int64_t n, int bounds, int disp_mode, double b0, double b1,
+                           double d0, double d1, double d2, double d3) {
+  if (threadIdx.x != 0) return;
+  MomState m{};
+  m.n = n;
+  m.bounds = bounds;
+  m.disp_mode = disp_mode;
+  m.b0 = b0;
+  m.b1 = b1;
+  if (disp_mode == 1) {
+    m.D[0] = d0;
+    m.D[1] = d1;
+    m.D[2] = d2;
+    m.D[3] = d3;
+  }
+  *s = m;
+}
+
+__global__ void k_mom_store(const MomState* __restrict__ s, MomState* __restrict__ table, int* __restrict__ counter,
+                            int cap) {
+  if (threadIdx.x != 0) return;
+  const int k = *counter;
+  if (k < cap) table[k] = *s;
+  *counter = k + 1;
+}
+
+__global__ void k_counter_reset(int* __restrict__ counter) {
+  if (threadIdx.x == 0) *counter = 0;
+}
+
 // ---------------------------------------------------------------------------
 // slice = "Imax" (analysis.py:61-64): the window is centred on the peak of the
 // current profile s_to_cur(tau, 0.01 std(tau), ...) (analysis.py:266-340):

@@ -174,6 +174,7 @@ struct ocsc_ctx {
   MomState* mom = nullptr;
   MomState* mom_host = nullptr;
   Buf cic;  // current-profile histograms (slice = "Imax")
+  Buf mom_part, mom_tab, mom_ctr;  // per-step moments in tracking: partials, table, counter
   // A ring of event sets, one per kick in flight: kicks are timed without a
   // host wait after each one (which would idle the GPU between kicks); a set
   // is read back when its slot comes round again or when the times are asked.
@@ -706,7 +707,8 @@ int ocsc_destroy(ocsc_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
-                 &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic})
+                 &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
+                 &c->mom_part, &c->mom_tab, &c->mom_ctr})
     if (b->p) (void)hipFree(b->p);
   if (L.sc) (void)hipFree(L.sc);
   if (L.sc_host) (void)hipHostFree(L.sc_host);
@@ -1044,8 +1046,64 @@ int ocsc_moments(ocsc_ctx* c, const double* r, int64_t ld, const double* q, int6
   return OCSC_OK;
 }
 
-int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
-                     double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// moments requested per tracking step (the tws_track list of track())
+struct TwsReq {
+  int bounds = 0, disp_mode = 0;  // as in MomState
+  double b0 = 0, b1 = 0;
+  double disp[4] = {0, 0, 0, 0};
+  int cap = 0;  // rows of the step table
+};
+
+// the moments passes of ocsc_moments on the device only, appended to the
+// step table (graph-capturable; own partial buffer, so the kick's fused
+// momentum sums are left alone)
+int enqueue_moments(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, const TwsReq& t, hipStream_t st) {
+  const int nblk = nred_blocks(c, n);
+  double* part = (double*)c->mom_part.p;
+  MomState* cur = c->mom;
+  k_mom_init<<<1, 64, 0, st>>>(cur, n, t.bounds, t.disp_mode, t.b0, t.b1, t.disp[0], t.disp[1], t.disp[2],
+                               t.disp[3]);
+  k_mom<0><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+  k_mom_reduce<0><<<1, 1024, 0, st>>>(cur, part, nblk);
+  if (t.bounds) {
+    k_mom<1><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+    k_mom_reduce<1><<<1, 1024, 0, st>>>(cur, part, nblk);
+  }
+  k_mom<2><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+  k_mom_reduce<2><<<1, 1024, 0, st>>>(cur, part, nblk);
+  if (t.disp_mode == 2) {
+    k_mom<3><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+    k_mom_reduce<3><<<1, 1024, 0, st>>>(cur, part, nblk);
+  }
+  k_mom<4><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+  k_mom_reduce<4><<<1, 1024, 0, st>>>(cur, part, nblk);
+  k_mom<5><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+  k_mom_reduce<5><<<1, 1024, 0, st>>>(cur, part, nblk);
+  k_mom_store<<<1, 64, 0, st>>>(cur, (MomState*)c->mom_tab.p, (int*)c->mom_ctr.p, t.cap);
+  LAUNCHCHK("tracking moments");
+  return OCSC_OK;
+}
+
+void moments_row(const MomState& h, double* v) {
+  v[0] = h.cnt;
+  v[1] = h.sumq;
+  v[2] = h.m[0];
+  for (int k = 0; k < 4; ++k) v[3 + k] = h.D[k];
+  for (int k = 0; k < 4; ++k) v[7 + k] = h.c[k];
+  v[11] = h.m[5];
+  for (int k = 0; k < 12; ++k) v[12 + k] = h.mom[k];
+  v[24] = h.tau_all;
+  v[25] = h.std_all;
+  v[26] = h.tau_all;
+}
+
+int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
+               double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, void* stream,
+               const TwsReq* tw, double* tws_out) {
   int rc;
   if ((rc = check_common(c, r, q, n, ld))) return rc;
   if (!nmesh || nsteps < 0) return fail(OCSC_EINVAL, "bad nmesh / nsteps");
@@ -1064,8 +1122,21 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   const double beta = std::sqrt(1. - igamma2);
   const double r56 = 0.0 - Ld / (beta * beta) * igamma2;
   const int nb = (int)((n + 255) / 256);
-  if (nsteps == 0) return OCSC_OK;
+  if (nsteps == 0 && !tw) return OCSC_OK;
   if ((rc = ensure_part(c, L, n))) return rc;
+  if (tw) {
+    if (c->kick_timing) return fail(OCSC_EINVAL, "kick timing and per-step moments cannot be combined");
+    if (!c->mom) {
+      HIPCHK(hipMalloc(&c->mom, sizeof(MomState)));
+      HIPCHK(hipHostMalloc(&c->mom_host, sizeof(MomState), hipHostMallocDefault));
+    }
+    if ((rc = ensure(c->mom_part, (size_t)12 * c->red_blocks * sizeof(double)))) return rc;
+    if ((rc = ensure(c->mom_tab, (size_t)(nsteps + 1) * sizeof(MomState)))) return rc;
+    if ((rc = ensure(c->mom_ctr, sizeof(int)))) return rc;
+    k_counter_reset<<<1, 64, 0, st>>>((int*)c->mom_ctr.p);
+    // the bunch before the first step (tw0 of track())
+    if ((rc = enqueue_moments(c, Rows{r, ld}, q, n, *tw, st))) return rc;
+  }
 
   // cell-ordered resident copy (sort_every > 0)
   const bool sorted = c->sort_every > 0 && zstep != 0;
@@ -1139,6 +1210,7 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
     o.passA = step == 0;
     o.prev_nblk = npart;
     o.drift = o.sump = step < nsteps - 1;
+    if (tw) o.drift = false;  // moments are taken between the kick and the next drift
     o.dr = DriftArgs{Ld, r56};
     if (ktime) {
       o.kev0 = c->kick_ev[2 * step];
@@ -1149,10 +1221,24 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   // graph event-record nodes of the kick events, per captured graph
   hipGraphNode_t knode[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   if (zstep == 0) {
-    for (int step = 0; step < nsteps; ++step) k_drift<<<nb, 256, 0, st>>>(rr, lr, n, Ld, r56);
-  } else {
+    for (int step = 0; step < nsteps; ++step) {
+      k_drift<<<nb, 256, 0, st>>>(rr, lr, n, Ld, r56);
+      if (tw && (rc = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, st))) return rc;
+    }
+  } else if (nsteps > 0) {
     k_drift<<<nb, 256, 0, st>>>(rr, lr, n, Ld, r56);
   }
+  // one tracking step after the drift: the kick and, with per-step moments,
+  // the moments and the next drift
+  auto step_body = [&](int step, const StepOpts& o, hipStream_t s2) -> int {
+    int rc3;
+    if ((rc3 = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, s2, o))) return rc3;
+    if (tw) {
+      if ((rc3 = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, s2))) return rc3;
+      if (step < nsteps - 1) k_drift<<<nb, 256, 0, s2>>>(rr, lr, n, Ld, r56);
+    }
+    return OCSC_OK;
+  };
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   hipGraph_t graphs[2] = {nullptr, nullptr};
   auto cleanup = [&]() {
@@ -1174,7 +1260,7 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
     const StepOpts o = opts_for(step);
     const bool middle = step > 0 && step < nsteps - 1;
     if (!use_graph || c->profiling || !middle) {
-      if ((rc = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, st, o))) {
+      if ((rc = step_body(step, o, st))) {
         cleanup();
         return rc;
       }
@@ -1185,7 +1271,7 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
       StepOpts oc = o;
       oc.kev_external = true;
       HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-      int rc2 = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, c->cap_stream, oc);
+      int rc2 = step_body(step, oc, c->cap_stream);
       hipError_t e = hipStreamEndCapture(c->cap_stream, &graphs[gi]);
       if (rc2 || e != hipSuccess) {
         cleanup();
@@ -1256,8 +1342,41 @@ int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
       c->kick_ms[step] = ms;
     }
   }
+  if (tw && tws_out) {
+    std::vector<MomState> tab(nsteps + 1);
+    HIPCHK(hipMemcpyAsync(tab.data(), c->mom_tab.p, (size_t)(nsteps + 1) * sizeof(MomState), hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int k = 0; k <= nsteps; ++k) moments_row(tab[k], tws_out + (size_t)k * OCSC_MOM_NOUT);
+  }
   if (err) return value_error(err);
   return OCSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
+                     double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, void* stream) {
+  return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph, stream, nullptr, nullptr);
+}
+
+int ocsc_track_drift_tws(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
+                         double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, int mom_flags,
+                         const double disp[4], double b0, double b1, double* tws_out, void* stream) {
+  if (!tws_out) return fail(OCSC_EINVAL, "null tws_out");
+  if (mom_flags & OCSC_MOM_IMAX) return fail(OCSC_EINVAL, "slice='Imax' is not available inside tracking");
+  if ((mom_flags & OCSC_MOM_DISP_GIVEN) && !disp) return fail(OCSC_EINVAL, "OCSC_MOM_DISP_GIVEN needs disp");
+  TwsReq t;
+  t.bounds = (mom_flags & OCSC_MOM_BOUNDS) ? 1 : 0;
+  t.disp_mode = (mom_flags & OCSC_MOM_DISP_GIVEN) ? 1 : ((mom_flags & OCSC_MOM_AUTO_DISP) ? 2 : 0);
+  t.b0 = b0;
+  t.b1 = b1;
+  if (t.disp_mode == 1)
+    for (int k = 0; k < 4; ++k) t.disp[k] = disp[k];
+  t.cap = nsteps + 1;
+  return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph, stream, &t, tws_out);
 }
 
 int ocsc_kick_times(ocsc_ctx* c, double* ms, int nout, int64_t* count) {

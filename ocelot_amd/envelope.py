@@ -108,13 +108,19 @@ def get_envelope(p_array, tws_i=None, bounds=None, slice=None, auto_disp=False, 
         disp = (tws_i.Dx, tws_i.Dxp, tws_i.Dy, tws_i.Dyp)
     v = moments(p_array, bounds=bounds, auto_disp=auto_disp and tws_i is None, disp=disp, device=device,
                 imax=slice == "Imax")
+    return twiss_from_moments(v, float(p_array.E), disp is not None or auto_disp)
+
+
+def twiss_from_moments(v, E, with_disp):
+    """A Twiss from one row of moments (OCSC_MOM_NOUT values, the layout of
+    ocsc_moments), with the derived quantities of analysis.py:160-224."""
     tws = Twiss()
-    tws.E = float(p_array.E)
+    tws.E = float(E)
     tws.q = float(v[1])
     tws.p = float(v[2])
     if v[0] < 3:
         return tws
-    if disp is not None or auto_disp:
+    if with_disp:
         tws.Dx, tws.Dxp, tws.Dy, tws.Dyp = (float(a) for a in v[3:7])
     # numpy scalars, as in the reference: a zero emittance gives inf/nan with a
     # RuntimeWarning rather than an exception

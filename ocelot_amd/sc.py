@@ -253,10 +253,16 @@ def drift(p_array, L, device=None):
         r[...] = rt.cpu().numpy()
 
 
-def track_drift(p_array, sc, L, dz, nsteps, use_graph=True):
+def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, bounds=None, auto_disp=False,
+                tws_i=None):
     """nsteps x [drift(L); sc.apply(p_array, dz)] on a device-resident bunch,
     the step loop of ocelot/cpbd/track.py:470-477 for a drift lattice, with
-    one step captured in a HIP graph and replayed."""
+    one step captured in a HIP graph and replayed.
+
+    With ``calc_tws`` the beam moments are taken on the device before the
+    first step and after every step, and the list of Twiss (``tws_track`` of
+    track(), ``s`` = cumulative length) is returned; ``bounds``, ``auto_disp``
+    and ``tws_i`` as in get_envelope (slice='Imax' is not available here)."""
     torch = _torch()
     lib = _lib.load()
     r = p_array.rparticles
@@ -267,10 +273,40 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True):
     dev = r.device.index
     n = int(r.shape[1])
     with torch.cuda.device(dev):
-        _lib.check(lib.ocsc_track_drift(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
-                                        p_array.q_array.data_ptr(), n, sc._nmesh(), float(p_array.E), float(dz),
-                                        float(L), int(nsteps), sc._prec(), 1 if use_graph else 0,
-                                        torch.cuda.current_stream(dev).cuda_stream))
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        if not calc_tws:
+            _lib.check(lib.ocsc_track_drift(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
+                                            p_array.q_array.data_ptr(), n, sc._nmesh(), float(p_array.E),
+                                            float(dz), float(L), int(nsteps), sc._prec(), 1 if use_graph else 0,
+                                            stream))
+            return None
+        from .envelope import twiss_from_moments
+        flags, b0, b1 = 0, 0.0, 0.0
+        disp = (0.0, 0.0, 0.0, 0.0)
+        if bounds is not None:
+            flags |= 1
+            b0, b1 = float(bounds[0]), float(bounds[1])
+        if tws_i is not None:
+            flags |= 4
+            disp = (tws_i.Dx, tws_i.Dxp, tws_i.Dy, tws_i.Dyp)
+        elif auto_disp:
+            flags |= 2
+        nout = 27
+        out = (ctypes.c_double * ((int(nsteps) + 1) * nout))()
+        dbuf = (ctypes.c_double * 4)(*disp)
+        _lib.check(lib.ocsc_track_drift_tws(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
+                                            p_array.q_array.data_ptr(), n, sc._nmesh(), float(p_array.E),
+                                            float(dz), float(L), int(nsteps), sc._prec(), 1 if use_graph else 0,
+                                            flags, dbuf, b0, b1, out, stream))
+    rows = np.frombuffer(out, dtype=np.float64).reshape(int(nsteps) + 1, nout)
+    tws, s = [], 0.0
+    for k in range(int(nsteps) + 1):
+        t = twiss_from_moments(rows[k], p_array.E, tws_i is not None or auto_disp)
+        if k:
+            s += float(L)  # track.py:481-483 accumulates the step lengths
+        t.s = s
+        tws.append(t)
+    return tws
 
 
 def apply_batched(items, precision="fp64", nkicks=1, use_graph=True, random_mesh=False):

@@ -130,3 +130,38 @@ def test_envelope_10m_device_resident_vs_oracle():
     th = get_envelope(host, auto_disp=True)
     for a in ("xx", "pxpx", "emit_x", "Dx", "tau"):
         assert getattr(th, a) == getattr(tws, a)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_track_drift_with_twiss_list(graph):
+    """track_drift(calc_tws=True) gives the tws_track list of track() on a
+    drift lattice: the moments before the first step and after each step
+    (kick done, next drift not yet), against the oracle's own loop; the
+    particles end exactly as without the moments."""
+    from oracle import sc_oracle as orc
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, _, dz = make_bunch("flattop_10m_127", n=200_000, seed=21)
+    nmesh, nsteps = [31, 31, 63], 12
+    dev = torch.cuda.current_device()
+    sc = SpaceCharge(nmesh_xyz=nmesh, precision="fp64")
+    d0 = DeviceParticleArray.from_host(parray(r, q, E), dev)
+    track_drift(d0, sc, dz, dz, nsteps, use_graph=graph)
+    d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+    tws = track_drift(d, sc, dz, dz, nsteps, use_graph=graph, calc_tws=True, auto_disp=True)
+    assert torch.equal(d.rparticles, d0.rparticles)
+    assert len(tws) == nsteps + 1
+    ro = r.copy()
+    ref = [eo.get_envelope(ro, q, E, auto_disp=True)]
+    for _ in range(nsteps):
+        orc.drift(ro, dz, E)
+        orc.apply(ro, q, E, dz, nmesh)
+        ref.append(eo.get_envelope(ro, q, E, auto_disp=True))
+    worst = 0.0
+    for k, (t, rf) in enumerate(zip(tws, ref)):
+        for a in ("emit_x", "emit_y", "beta_x", "alpha_x", "xx", "pxpx", "tautau", "pp", "Dx", "x", "px"):
+            scale = max(abs(rf[a]), np.sqrt(abs(rf[SCALE[a]])) if isinstance(SCALE.get(a), str) else 0)
+            worst = max(worst, abs(float(getattr(t, a)) - rf[a]) / scale)
+        assert t.s == pytest.approx(k * dz, rel=1e-12)
+    print(f"track_drift tws ({'graph' if graph else 'eager'}): worst relative deviation {worst:.2e} over {nsteps} steps")
+    assert worst <= 1e-8
