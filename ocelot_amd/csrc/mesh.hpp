@@ -212,15 +212,11 @@ __global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<
 
 // z pass inverse: two Hermitian half spectra -> one complex inverse FFT ->
 // two real rows of the potential, z < nz
-template <typename F, int M, bool DERIV = false>
-__global__ void __launch_bounds__(kNT) k_zinv(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d,
-                                              const KickScalars* __restrict__ sc,
-                                              const cpx<F>* __restrict__ tw,
-                                              const cpx<double>* __restrict__ twd = nullptr) {
+template <typename F, int M, bool DERIV>
+__device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __restrict__ phi, const MeshDims& d,
+                                          const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
+                                          const cpx<double>* __restrict__ twd, cpx<F>* s) {
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<F>* s = (cpx<F>*)smem_raw;
-  if (sc->err) return;
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
@@ -247,6 +243,33 @@ __global__ void __launch_bounds__(kNT) k_zinv(const cpx<F>* __restrict__ A, F* _
     const cpx<F> v = s[z * S + (rl >> 1)];
     phi[row * nz + z] = (rl & 1) ? v.y : v.x;
   }
+}
+
+template <typename F, int M, bool DERIV = false>
+__global__ void __launch_bounds__(kNT) k_zinv(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d,
+                                              const KickScalars* __restrict__ sc,
+                                              const cpx<F>* __restrict__ tw,
+                                              const cpx<double>* __restrict__ twd = nullptr) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (sc->err) return;
+  zinv_rows<F, M, DERIV>(A, phi, d, sc, tw, twd, (cpx<F>*)smem_raw);
+}
+
+// Mixed form: the three field components in one launch (blockIdx.y), the
+// third (z) with the spectral difference factor
+template <int M>
+__global__ void __launch_bounds__(kNT) k_zinv3(const cpx<float>* __restrict__ A, int64_t nyz, float* __restrict__ E,
+                                               int64_t ncell, MeshDims d, const KickScalars* __restrict__ sc,
+                                               const cpx<float>* __restrict__ tw,
+                                               const cpx<double>* __restrict__ twd) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (sc->err) return;
+  const int comp = blockIdx.y;
+  cpx<float>* s = (cpx<float>*)smem_raw;
+  if (comp == 2)
+    zinv_rows<float, M, true>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s);
+  else
+    zinv_rows<float, M, false>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s);
 }
 
 // columns per workgroup of the kernels that carry two or three fields

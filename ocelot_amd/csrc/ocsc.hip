@@ -395,10 +395,7 @@ int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, floa
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<float, MM>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zinv<float, MM, false><<<nb, kNT, TL::lds_bytes, st>>>(A, E, d, L.sc, c->twf + MM);
-    k_zinv<float, MM, false><<<nb, kNT, TL::lds_bytes, st>>>(A + nyz, E + g.ncell, d, L.sc, c->twf + MM);
-    k_zinv<float, MM, true><<<nb, kNT, TL::lds_bytes, st>>>(A + 2 * nyz, E + 2 * g.ncell, d, L.sc, c->twf + MM,
-                                                             c->twd + MM);
+    k_zinv3<MM><<<dim3(nb, 3), kNT, TL::lds_bytes, st>>>(A, nyz, E, g.ncell, d, L.sc, c->twf + MM, c->twd + MM);
   });
   LAUNCHCHK("k_zinv(fields)");
   return OCSC_OK;
@@ -520,11 +517,11 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   }
   const int64_t nz64 = (c->deposit_xcd ? kNumXcd : 1) * NWORD * g.ncell;
   k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, st>>>((unsigned long long*)L.grid.p, nz64);
-  const int ndep = (int)((n + kPT * kDepItems - 1) / (kPT * kDepItems));
+  const int ndep = (int)((n + kDepPT * kDepItems - 1) / (kDepPT * kDepItems));
   if (c->deposit_xcd)
-    k_deposit<NWORD, true><<<ndep, kPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
+    k_deposit<NWORD, true><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
   else
-    k_deposit<NWORD, false><<<ndep, kPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
+    k_deposit<NWORD, false><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
   LAUNCHCHK("deposit");
   mark(3);
   if (fork)

@@ -454,6 +454,7 @@ __device__ __forceinline__ int64_t ngp_cell(const double* R6, const KickScalars&
 // particles occupy a few hundred neighbouring cells, so global atomics drop by
 // one to two orders of magnitude; in random order the table simply passes
 // every particle through.  Integer sums: the grid is identical either way.
+constexpr int kDepPT = 256;                  // threads per workgroup
 constexpr int kDepItems = 8;                 // particles per thread
 constexpr int kDepSlots = 2048;              // hash slots (power of two)
 
@@ -465,7 +466,7 @@ __device__ __forceinline__ void flush_cell(unsigned long long* g, int64_t ncell,
 }
 
 template <int NWORD, bool XCD = false>
-__global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
+__global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
                                                  unsigned long long* __restrict__ grid) {
   __shared__ int hkey[kDepSlots];
   __shared__ long long hval[NWORD][kDepSlots];
@@ -475,13 +476,13 @@ __global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restric
   const int64_t ncell = (int64_t)s.nmesh[0] * s.nmesh[1] * s.nmesh[2];
   unsigned long long* g = grid;
   if constexpr (XCD) g += (int64_t)xcc_id() * NWORD * ncell;
-  for (int i = threadIdx.x; i < kDepSlots; i += kPT) {
+  for (int i = threadIdx.x; i < kDepSlots; i += kDepPT) {
     hkey[i] = -1;
     hval[0][i] = 0;
     if constexpr (NWORD == 2) hval[1][i] = 0;
   }
   __syncthreads();
-  const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (kPT * kDepItems);
+  const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (kDepPT * kDepItems);
   int bad = 0;
   // the rows of item it + 1 are loaded before item it goes through the table
   double cur[7], nxt[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -492,9 +493,9 @@ __global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restric
   };
   int64_t p = base + threadIdx.x;
   if (p < s.n) load(p, cur);
-  for (int it = 0; it < kDepItems; ++it, p += kPT) {
+  for (int it = 0; it < kDepItems; ++it, p += kDepPT) {
     if (p >= s.n) break;
-    const int64_t pn = p + kPT;
+    const int64_t pn = p + kDepPT;
     if (it + 1 < kDepItems && pn < s.n) load(pn, nxt);
     const int64_t idx = ngp_cell<NWORD == 1>(cur, s);  // one word: the mixed form
     const double qv = cur[6];
@@ -526,7 +527,7 @@ __global__ void __launch_bounds__(kPT) k_deposit(Rows R, const double* __restric
   }
   if (bad) atomicOr(&scp->err, (int)kErrCell);
   __syncthreads();
-  for (int i = threadIdx.x; i < kDepSlots; i += kPT) {
+  for (int i = threadIdx.x; i < kDepSlots; i += kDepPT) {
     const int k = hkey[i];
     if (k >= 0) flush_cell<XCD>(g, ncell, k, hval[0][i], NWORD == 2 ? hval[NWORD - 1][i] : 0, NWORD);
   }
