@@ -71,18 +71,26 @@ struct Dft {
   }
 };
 
-// Radix plan of a length: passes of radix 16/8/4/2, largest first.
-template <int M>
+// Radix plan of a length: passes of radix RMAX (16 or 8), then one
+// remainder pass of radix 2/4/8.
+template <int M, int RMAX = 16>
 struct Plan {
   static constexpr int L = ilog2(M);
-  // number of radix-16 passes, then one remainder pass
-  static constexpr int n16 = L / 4;
-  static constexpr int rem = L % 4;  // 0, 1 (radix 2), 2 (radix 4), 3 (radix 8)
-  static constexpr int npass = n16 + (rem ? 1 : 0);
-  static constexpr int radix(int p) { return p < n16 ? 16 : (1 << rem); }
+  static constexpr int lr = ilog2(RMAX);
+  static constexpr int nbig = L / lr;
+  static constexpr int rem = L % lr;
+  static constexpr int npass = nbig + (rem ? 1 : 0);
+  static constexpr int radix(int p) { return p < nbig ? RMAX : (1 << rem); }
   // smallest radix decides the butterfly count per column
-  static constexpr int rmin = rem ? (1 << rem) : 16;
+  static constexpr int rmin = rem ? (1 << rem) : RMAX;
 };
+
+// largest radix that still gives every thread a butterfly: M/R butterflies
+// per column times T columns should cover NT threads
+template <int M, int T, int NT>
+constexpr int radix_for() {
+  return (M / 16) * T >= NT ? 16 : 8;
+}
 
 // One Stockham pass of radix R over T columns (LDS row stride S), NT threads.
 template <typename F, int M, int R, int T, int NT, int DIR, int S>
@@ -130,8 +138,9 @@ __device__ __forceinline__ void stockham_pass(cpx<F>* s, int Ns, const cpx<F>* _
 
 template <typename F, int M, int T, int NT, int DIR, int S, int P = 0>
 __device__ __forceinline__ void lds_fft_from(cpx<F>* s, int Ns, const cpx<F>* __restrict__ tw) {
-  if constexpr (P < Plan<M>::npass) {
-    constexpr int R = Plan<M>::radix(P);
+  using PL = Plan<M, radix_for<M, T, NT>()>;
+  if constexpr (P < PL::npass) {
+    constexpr int R = PL::radix(P);
     stockham_pass<F, M, R, T, NT, DIR, S>(s, Ns, tw);
     lds_fft_from<F, M, T, NT, DIR, S, P + 1>(s, Ns * R, tw);
   }

@@ -377,17 +377,17 @@ int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, floa
   LAUNCHCHK("k_yfwd");
   if (ev) (void)hipEventRecord(ev[2], st);
   OCSC_SWITCH_M(d.mx, {
-    using FT = FieldTile<MM>;
+    using FT = FieldTile<MM, kNTF>;
     dim3 grid((d.hz + FT::T - 1) / FT::T, d.my);
-    k_xconv_field<MM><<<grid, kNT, FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
+    k_xconv_field<MM><<<grid, kNTF, FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
                                                    c->twd + MM);
   });
   LAUNCHCHK("k_xconv_field");
   if (ev) (void)hipEventRecord(ev[3], st);
   OCSC_SWITCH_M(d.my, {
-    using FT = FieldTile<MM>;
+    using FT = FieldTile<MM, kNTF>;
     dim3 grid((d.hz + FT::T - 1) / FT::T, d.nx);
-    k_yinv_field<MM><<<grid, kNT, FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
+    k_yinv_field<MM><<<grid, kNTF, FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
                                                   c->twd + MM);
   });
   LAUNCHCHK("k_yinv_field");
