@@ -10,6 +10,7 @@ from .physproc import PhysProc
 from .beam import ParticleArray, DeviceParticleArray
 from .sc import SpaceCharge, apply_batched, drift, track_drift
 from .envelope import Twiss, get_envelope
+from .autotune import tune_tracking
 
 __all__ = ["PhysProc", "ParticleArray", "DeviceParticleArray", "SpaceCharge", "apply_batched", "drift", "track_drift",
-           "Twiss", "get_envelope"]
+           "Twiss", "get_envelope", "tune_tracking"]

@@ -1,0 +1,98 @@
+"""Autotuner for the tracking loop: picks, per bunch size, mesh and precision,
+the context options that only change speed.
+
+    from ocelot_amd.autotune import tune_tracking
+    best = tune_tracking(d_array, sc, L=0.1, dz=0.1)     # sets the options
+
+Knobs (``ocsc_set_option``): ``sort_every`` (re-sort interval of the
+cell-ordered copy: sorting costs a radix sort and a gather of the bunch, and
+buys deposit/gather locality that decays as particles drift), ``deposit_xcd``
+(one charge grid or one per XCD) and ``overlap`` (Green's-function spectrum on
+a second stream next to the deposit).  The search is coordinate-wise: each
+knob in turn over its candidates with the others fixed, timing ``steps``
+graph-captured steps on a copy of the bunch.  ``deposit_xcd`` and ``overlap``
+must leave the particles bit-identical (checked); ``sort_every`` changes only
+reduction orders.  Results are cached per (n, nmesh, precision, device), so a
+second call for the same shape only re-applies the options.
+"""
+import time
+
+from . import _lib
+
+DEFAULT_SPACE = {"sort_every": (5, 7, 10, 14, 20), "deposit_xcd": (0, 1), "overlap": (0, 1)}
+BIT_IDENTICAL = ("deposit_xcd", "overlap")
+
+_cache = {}
+
+
+def search(space, measure, start):
+    """Coordinate search.  ``measure(options) -> (seconds, result)``; returns
+    (best options, {knob: {value: seconds}}, results by options) — results are
+    whatever ``measure`` returns beside the time (used for identity checks)."""
+    best = dict(start)
+    t_best, _ = measure(best)
+    timings, results = {}, {}
+    for knob, values in space.items():
+        timings[knob] = {}
+        for v in values:
+            opts = dict(best, **{knob: v})
+            t, res = measure(opts)
+            timings[knob][v] = t
+            results[tuple(sorted(opts.items()))] = res
+            if t < t_best:
+                best, t_best = opts, t
+    return best, timings, results
+
+
+def tune_tracking(p_array, sc, L, dz, steps=30, space=None, check_identical=True, force=False):
+    """Tune and apply the tracking options for this bunch and mesh; returns
+    {"options": ..., "ms_per_step": ..., "timings_ms": ..., "cached": bool}."""
+    import torch
+    from .beam import DeviceParticleArray
+    from .sc import track_drift
+    r = p_array.rparticles
+    if not (isinstance(r, torch.Tensor) and r.is_cuda):
+        raise TypeError("tune_tracking needs a DeviceParticleArray")
+    dev = r.device.index
+    key = (int(r.shape[1]), tuple(sc._nmesh()), sc._prec(), dev)
+    if key in _cache and not force:
+        out = dict(_cache[key], cached=True)
+        for k, v in out["options"].items():
+            _lib.set_option(dev, k, v)
+        return out
+    space = dict(DEFAULT_SPACE if space is None else space)
+    start = {k: _lib.get_option(dev, k) for k in space}
+
+    def measure(opts):
+        for k, v in opts.items():
+            _lib.set_option(dev, k, v)
+        d = DeviceParticleArray(r.clone(), p_array.q_array, p_array.E, getattr(p_array, "s", 0.0))
+        track_drift(d, sc, L, dz, 3, use_graph=True)  # first use of buffers and graphs
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        track_drift(d, sc, L, dz, steps, use_graph=True)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        bits = d.rparticles.contiguous().view(torch.int64)
+        w = torch.arange(1, bits.shape[1] + 1, device=bits.device, dtype=torch.int64)
+        # fingerprint of the exact bit patterns (wrapping integer sums)
+        return t, (int(bits.sum()), int((bits * w).sum()))
+
+    best, timings, results = search(space, measure, start)
+    if check_identical:
+        for knob in BIT_IDENTICAL:
+            if knob not in space:
+                continue
+            outs = [res for opts, res in results.items()
+                    if all(v == best[k] for k, v in opts if k != knob) and len(opts) == len(best)]
+            for o in outs[1:]:
+                if o != outs[0]:
+                    raise RuntimeError(f"option {knob} changed the tracked particles")
+    for k, v in best.items():
+        _lib.set_option(dev, k, v)
+    out = {"options": best,
+           "ms_per_step": round(min(t for tm in timings.values() for t in tm.values()) * 1e3, 4),
+           "timings_ms": {k: {str(v): round(t * 1e3, 4) for v, t in tm.items()} for k, tm in timings.items()},
+           "steps": steps}
+    _cache[key] = out
+    return dict(out, cached=False)
