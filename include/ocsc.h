@@ -135,6 +135,48 @@ int ocsc_track_drift_tws(ocsc_ctx* ctx, double* rparticles, int64_t ld, const do
                          int use_graph, int mom_flags, const double disp[4], double b0, double b1, double* tws_out,
                          void* stream);
 
+/* LSC.apply (ocelot/cpbd/sc.py:538-600): one longitudinal space-charge kick
+ * of length dz on the device-resident bunch, delta += W(tau) 1e-9 / pc_ref.
+ * The transverse size comes from the slice mean + b0 std <= tau < mean + b1 std
+ * of tau (sc.py:566-577): the mean of std(x) and std(y) there, or with
+ * step_profile != 0 the smaller half-extent of x and y.  The current profile
+ * is s_to_cur(tau, std(tau) smooth_param, sum q, v) (beam/analysis.py:266-340)
+ * and the wake that of the round Gaussian (step_profile == 0, sc.py:302-341)
+ * or uniform (sc.py:343-369) beam impedance times 1 + K_max^2 fill_factor/2
+ * (sc.py:436-441; K_max and fill_factor as LSC.apply derives them from the
+ * undulators of the line, 0 without).  dz < 1e-10 does nothing.  info (host,
+ * may be NULL) receives min(ninfo, OCSC_LSC_NINFO) values: [0] transverse
+ * size, [1] std(tau), [2] profile bins, [3] bin width, [4] first bin
+ * position, [5] sum of q, [6] particles in the slice, [7] mean(tau).
+ * OCSC_EVALUE where the reference raises (empty bunch, no spread in tau,
+ * non-finite tau, empty slice with the step profile).  Synchronises once
+ * (after the slice statistics, to size the profile grid); the kick itself is
+ * left queued on stream. */
+#define OCSC_LSC_NINFO 8
+int ocsc_lsc_apply(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n, double E_GeV,
+                   double dz, int step_profile, double smooth_param, double b0, double b1, double K_max,
+                   double fill_factor, double* info, int ninfo, void* stream);
+
+/* Profile grid, current (A) and wake W (V, the kick before scaling) of the
+ * last ocsc_lsc_apply on this context (the quantities B[:, 0], B[:, 1] and W
+ * of sc.py:582-587), copied to host arrays of at least *nbins entries (any
+ * may be NULL).  *nbins is set in every case; OCSC_EINVAL if cap < *nbins.
+ * Synchronises the device. */
+int ocsc_lsc_profile(ocsc_ctx* ctx, double* s, double* current, double* wake, int64_t cap, int64_t* nbins);
+
+/* LSC.wake_lsc (ocelot/cpbd/sc.py:411-465): wake (V/C) of the normalised
+ * profile bunch[nb] on the uniform grid s[nb] (host arrays), for the
+ * Gaussian (step_profile == 0) or step transverse profile of size sigma.
+ * Synchronous. */
+int ocsc_lsc_wake(ocsc_ctx* ctx, const double* s, const double* bunch, int64_t nb, double gamma, double sigma,
+                  double dz, int step_profile, double K_max, double fill_factor, double* wake, void* stream);
+
+/* LSC.imp_lsc / LSC.imp_step_lsc (ocelot/cpbd/sc.py:302-369): imaginary part
+ * of the impedance (ohm; the real part is zero) at the n angular frequencies
+ * w (host arrays).  Synchronous. */
+int ocsc_lsc_impedance(ocsc_ctx* ctx, const double* w, int64_t n, double gamma, double sigma, double dz,
+                       int step_profile, double* z_imag, void* stream);
+
 /* One bunch of a batched call (fields as in ocsc_apply). */
 typedef struct ocsc_bunch {
   double* rparticles;

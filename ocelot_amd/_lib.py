@@ -49,6 +49,11 @@ SIGNATURES = {
     "ocsc_stage_times": (_c_int, [_vp, _dp, _c_int, ctypes.POINTER(_i64)]),
     "ocsc_stage_name": (ctypes.c_char_p, [_c_int]),
     "ocsc_kick_times": (_c_int, [_vp, _dp, _c_int, ctypes.POINTER(_i64)]),
+    "ocsc_lsc_apply": (_c_int, [_vp, _vp, _i64, _vp, _i64, _dbl, _dbl, _c_int, _dbl, _dbl, _dbl, _dbl, _dbl, _dp,
+                                _c_int, _vp]),
+    "ocsc_lsc_profile": (_c_int, [_vp, _dp, _dp, _dp, _i64, ctypes.POINTER(_i64)]),
+    "ocsc_lsc_wake": (_c_int, [_vp, _dp, _dp, _i64, _dbl, _dbl, _dbl, _c_int, _dbl, _dbl, _dp, _vp]),
+    "ocsc_lsc_impedance": (_c_int, [_vp, _dp, _i64, _dbl, _dbl, _dbl, _c_int, _dp, _vp]),
 }
 
 _lib = None

@@ -13,6 +13,7 @@
 #include "mesh.hpp"
 #include "particles.hpp"
 #include "moments.hpp"
+#include "lsc.hpp"
 
 namespace ocsc {
 // sort.hip
@@ -190,6 +191,13 @@ struct ocsc_ctx {
   int kick_timing = 0;
   std::vector<hipEvent_t> kick_ev;
   std::vector<double> kick_ms;
+  // longitudinal space charge: device state, pinned mirror, per-bin arrays
+  // and the profile grid of the last kick
+  LscState* lsc = nullptr;
+  LscState* lsc_host = nullptr;
+  Buf lsc_bins, lsc_cic;
+  int lsc_nbins = 0;
+  double lsc_a = 0, lsc_ds = 0;
 };
 
 namespace {
@@ -705,8 +713,10 @@ int ocsc_destroy(ocsc_ctx* c) {
   (void)hipDeviceSynchronize();
   for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
                  &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
-                 &c->mom_part, &c->mom_tab, &c->mom_ctr})
+                 &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic})
     if (b->p) (void)hipFree(b->p);
+  if (c->lsc) (void)hipFree(c->lsc);
+  if (c->lsc_host) (void)hipHostFree(c->lsc_host);
   if (L.sc) (void)hipFree(L.sc);
   if (L.sc_host) (void)hipHostFree(L.sc_host);
   if (c->mom) (void)hipFree(c->mom);
@@ -892,6 +902,28 @@ int ocsc_drift(ocsc_ctx* c, double* r, int64_t ld, int64_t n, double L, double E
 
 namespace {
 
+// Cloud-in-cell weights of tau on the grid a + j ds, j < nbins, as exact
+// fixed-point sums (moments.hpp k_cic); *tot points at the nbins totals in buf.
+int cic_profile(Buf& buf, const Rows& R, int64_t n, double a, double ds, int nbins, hipStream_t st, long long** tot) {
+  int e = 0;
+  (void)std::frexp((double)n, &e);
+  const double wscale = std::ldexp(1.0, 62 - e);  // n * wscale < 2^62
+  const int ncic = 256;
+  const int win = std::min(nbins, kCicLds);
+  int rc;
+  if ((rc = ensure(buf, ((size_t)ncic * win + nbins) * sizeof(long long)))) return rc;
+  long long* hist = (long long*)buf.p;
+  *tot = hist + (size_t)ncic * win;
+  // windows of at most kCicLds bins (one for any ordinary bunch)
+  for (int j0 = 0; j0 < nbins; j0 += win) {
+    const int nb = std::min(win, nbins - j0);
+    k_cic<<<ncic, kPT, (size_t)nb * sizeof(long long), st>>>(R, n, a, ds, nbins, j0, nb, wscale, hist);
+    k_cic_sum<<<(nb + 255) / 256, 256, 0, st>>>(hist, ncic, nb, *tot + j0);
+  }
+  LAUNCHCHK("k_cic");
+  return OCSC_OK;
+}
+
 // Centre of the peak-current window (analysis.py:61-64): the current profile
 // s_to_cur(tau, 0.01 std(tau), ...) (analysis.py:266-340) on its own grid,
 // a = min - 3 sigma, b = max + 3 sigma, ds = (b - a)/ceil((b - a)/(sigma/4)),
@@ -918,22 +950,9 @@ int imax_window(ocsc_ctx* c, const Rows& R, int64_t n, hipStream_t st, double* z
   const int N = (int)nf;
   const double ds = (b - a) / N;
   const int nbins = N + 1;
-  int e = 0;
-  (void)std::frexp((double)n, &e);
-  const double wscale = std::ldexp(1.0, 62 - e);  // n * wscale < 2^62
-  const int ncic = 256;
-  const int win = std::min(nbins, kCicLds);
+  long long* tot = nullptr;
   int rc;
-  if ((rc = ensure(c->cic, ((size_t)ncic * win + nbins) * sizeof(long long)))) return rc;
-  long long* hist = (long long*)c->cic.p;
-  long long* tot = hist + (size_t)ncic * win;
-  // windows of at most kCicLds bins (one for any ordinary bunch)
-  for (int j0 = 0; j0 < nbins; j0 += win) {
-    const int nb = std::min(win, nbins - j0);
-    k_cic<<<ncic, kPT, (size_t)nb * sizeof(long long), st>>>(R, n, a, ds, nbins, j0, nb, wscale, hist);
-    k_cic_sum<<<(nb + 255) / 256, 256, 0, st>>>(hist, ncic, nb, tot + j0);
-  }
-  LAUNCHCHK("k_cic");
+  if ((rc = cic_profile(c->cic, R, n, a, ds, nbins, st, &tot))) return rc;
   std::vector<long long> C(nbins);
   HIPCHK(hipMemcpyAsync(C.data(), tot, (size_t)nbins * sizeof(long long), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -1577,6 +1596,238 @@ int ocsc_last_stats(ocsc_ctx* c, double* out, int nout, void* stream) {
   v[18] = (double)(s.err | s.err_sticky);
   v[19] = (double)s.rebuild;
   for (int k = 0; k < nout && k < 20; ++k) out[k] = v[k];
+  return OCSC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Longitudinal space charge (LSC, ocelot/cpbd/sc.py:261-600)
+
+namespace {
+
+constexpr double kZ0 = 1. / (kC * kEps0);  // globals.py: Z0 = 1/(c eps0)
+
+// impedance constants (sc.py:302-369, 446-449), formed as numpy forms them
+// (no fused multiply-adds in these host expressions)
+LscImp lsc_imp(int step_profile, double gamma, double sigma, double dz, double K_max, double fill_factor) {
+#pragma clang fp contract(off)
+  LscImp p{};
+  p.step_profile = step_profile;
+  p.sigma = sigma;
+  p.dz = dz;
+  p.factor = 1 + 0.5 * K_max * K_max * fill_factor;
+  if (!step_profile) {
+    p.coef = kZ0 / (4 * kPi * kC * (gamma * gamma));
+    p.gamma_c = gamma * kC;
+  } else {
+    p.coef = kZ0 * kC;
+    p.gamma_c = kC * gamma;
+  }
+  return p;
+}
+
+// step and frequency constants of wake_lsc for a profile grid starting
+// s0, s0 + ds (sc.py:442-465, analysis.py:376-383 and sc.py:403-409)
+struct WakeGrid {
+  double inv_dt0;  // 1/dt of the profile grid (impedance frequencies)
+  double dt;       // dt of the padded grid s0 + (k + 1) ds (spectrum)
+  double ndf;      // n df of the padded grid (inverse transform)
+};
+
+WakeGrid wake_grid(double s0, double s1, int64_t nb) {
+#pragma clang fp contract(off)
+  WakeGrid g{};
+  const double ds = s1 - s0;
+  const double dt0 = ds / kC;
+  g.inv_dt0 = 1 / dt0;
+  const double n = (double)(2 * nb);
+  const double sb0 = s0 + 1.0 * ds, sb1 = s0 + 2.0 * ds;
+  g.dt = (sb1 - sb0) / kC;
+  const double df = 1 / g.dt * 1.0 / n;
+  g.ndf = n * df;
+  return g;
+}
+
+// spectrum, impedance and inverse transform on the device; out_m = scale w_m
+int lsc_wake_device(ocsc_ctx* c, const double* bunch, int nb, const WakeGrid& g, const LscImp& imp, double scale,
+                    double* Zr, double* Zi, double* out, hipStream_t st) {
+  k_lsc_spectrum<<<nb, 256, 0, st>>>(bunch, nb, g.dt, g.inv_dt0, imp, Zr, Zi);
+  k_lsc_wake<<<nb, 256, 0, st>>>(Zr, Zi, nb, g.ndf, scale, out);
+  LAUNCHCHK("k_lsc_wake");
+  (void)c;
+  return OCSC_OK;
+}
+
+constexpr int kLscMaxBins = 1 << 22;
+
+}  // namespace
+
+extern "C" {
+
+int ocsc_lsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, double E_GeV, double dz,
+                   int step_profile, double smooth_param, double b0, double b1, double K_max, double fill_factor,
+                   double* info, int ninfo, void* stream) {
+#pragma clang fp contract(off)
+  if (!c || ((!r || !q) && n > 0) || (ninfo > 0 && !info)) return fail(OCSC_EINVAL, "null argument");
+  if (n < 0 || ld < n) return fail(OCSC_EINVAL, "bad n / ld");
+  if (dz < 1e-10) return OCSC_OK;  // sc.py:556-558
+  if (n == 0) return fail(OCSC_EVALUE, "zero-size array to reduction operation minimum which has no identity");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  if (!c->lsc) {
+    HIPCHK(hipMalloc(&c->lsc, sizeof(LscState)));
+    HIPCHK(hipHostMalloc(&c->lsc_host, sizeof(LscState), hipHostMallocDefault));
+  }
+  Lane& L = c->lane0;
+  const int nblk = nred_blocks(c, n);
+  int rc;
+  if ((rc = ensure(L.part, (size_t)12 * nblk * sizeof(double)))) return rc;
+  HIPCHK(hipStreamSynchronize(st));  // the pinned mirror may still be read by an earlier copy
+  LscState& h = *c->lsc_host;
+  std::memset(&h, 0, sizeof(h));
+  h.n = n;
+  h.b0 = b0;
+  h.b1 = b1;
+  HIPCHK(hipMemcpyAsync(c->lsc, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  double* part = (double*)L.part.p;
+  const Rows R{r, ld};
+  k_lsc_stat<0><<<nblk, kPT, 0, st>>>(R, q, c->lsc, part);
+  k_lsc_stat_reduce<0><<<1, 1024, 0, st>>>(c->lsc, part, nblk);
+  k_lsc_stat<1><<<nblk, kPT, 0, st>>>(R, q, c->lsc, part);
+  k_lsc_stat_reduce<1><<<1, 1024, 0, st>>>(c->lsc, part, nblk);
+  k_lsc_stat<2><<<nblk, kPT, 0, st>>>(R, q, c->lsc, part);
+  k_lsc_stat_reduce<2><<<1, 1024, 0, st>>>(c->lsc, part, nblk);
+  if (!step_profile) {
+    k_lsc_stat<3><<<nblk, kPT, 0, st>>>(R, q, c->lsc, part);
+    k_lsc_stat_reduce<3><<<1, 1024, 0, st>>>(c->lsc, part, nblk);
+  }
+  LAUNCHCHK("k_lsc_stat");
+  HIPCHK(hipMemcpyAsync(&h, c->lsc, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // transverse size of the slice (sc.py:572-577); an empty slice gives NaN
+  // for the Gaussian profile (numpy's std of nothing), which zeroes the
+  // impedance (sc.py:322-323), and an error for the step profile
+  double sigma;
+  if (step_profile) {
+    if (!(h.cnt > 0)) return fail(OCSC_EVALUE, "zero-size array to reduction operation maximum which has no identity");
+    const double ex = h.xmax - h.xmin, ey = h.ymax - h.ymin;
+    sigma = (ey < ex ? ey : ex) / 2;
+  } else {
+    sigma = h.cnt > 0 ? (std::sqrt(h.xvar) + std::sqrt(h.yvar)) / 2. : std::nan("");
+  }
+  // profile grid of s_to_cur(tau, std(tau) smooth_param) (analysis.py:306-315)
+  const double sig = h.tau_std * smooth_param;
+  const double a = h.tau_min - 3 * sig, b = h.tau_max + 3 * sig;
+  const double nf = std::ceil((b - a) / (0.25 * sig));
+  if (!std::isfinite(nf) || !(nf >= 1))
+    return fail(OCSC_EVALUE, "current profile: tau has no spread or is not finite (cannot size the grid)");
+  if (nf >= kLscMaxBins) return fail(OCSC_EVALUE, "current profile: more than 2^22 bins");
+  const int N = (int)nf;
+  const double ds = (b - a) / N;
+  const int nbins = N + 1;
+  const int K = (int)std::floor(3 * sig / ds + 0.5);
+  if (2 * K + 1 > kLscMaxTaps) return fail(OCSC_EVALUE, "current profile: smoothing kernel too wide");
+  // beam constants (sc.py:579-581, 593)
+  const double gamma = E_GeV / kMeGeV;
+  const double v = std::sqrt(1 - 1 / (gamma * gamma)) * kC;
+  const double pc_ref = std::sqrt(E_GeV * E_GeV / (kMeGeV * kMeGeV) - 1) * kMeGeV;
+  // per-bin arrays: current, bunch, Z (re, im), W
+  if ((rc = ensure(c->lsc_bins, (size_t)5 * nbins * sizeof(double)))) return rc;
+  double* cur = (double*)c->lsc_bins.p;
+  double* bunch = cur + nbins;
+  double* Zr = bunch + nbins;
+  double* Zi = Zr + nbins;
+  double* W = Zi + nbins;
+  long long* tot = nullptr;
+  if ((rc = cic_profile(c->lsc_cic, R, n, a, ds, nbins, st, &tot))) return rc;
+  k_lsc_profile<<<1, 1024, 0, st>>>(tot, nbins, K, ds, sig, h.sumq, v, cur, bunch);
+  LAUNCHCHK("k_lsc_profile");
+  // wake on the grid x_j = j ds + a; W = -wake q = w q (sc.py:587)
+  const double x1 = 1.0 * ds + a;
+  const WakeGrid g = wake_grid(a, x1, nbins);
+  const LscImp imp = lsc_imp(step_profile, gamma, sigma, dz, K_max, fill_factor);
+  if ((rc = lsc_wake_device(c, bunch, nbins, g, imp, h.sumq, Zr, Zi, W, st))) return rc;
+  k_lsc_kick<<<nred_blocks(c, n), kPT, 0, st>>>(r, ld, n, W, nbins, a, ds, pc_ref);
+  LAUNCHCHK("k_lsc_kick");
+  c->lsc_nbins = nbins;
+  c->lsc_a = a;
+  c->lsc_ds = ds;
+  const double v_info[OCSC_LSC_NINFO] = {sigma, h.tau_std, (double)nbins, ds, a, h.sumq, h.cnt, h.tau_mean};
+  for (int k = 0; k < ninfo && k < OCSC_LSC_NINFO; ++k) info[k] = v_info[k];
+  return OCSC_OK;
+}
+
+int ocsc_lsc_profile(ocsc_ctx* c, double* s, double* current, double* wake, int64_t cap, int64_t* nbins) {
+  if (!c || !nbins) return fail(OCSC_EINVAL, "null argument");
+  *nbins = c->lsc_nbins;
+  if (c->lsc_nbins == 0) return fail(OCSC_EINVAL, "no LSC kick on this context yet");
+  if (cap < c->lsc_nbins) return fail(OCSC_EINVAL, "cap below the number of profile bins");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipDeviceSynchronize());
+  const int nb = c->lsc_nbins;
+  const double* dcur = (const double*)c->lsc_bins.p;
+  if (current) HIPCHK(hipMemcpy(current, dcur, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost));
+  if (wake) HIPCHK(hipMemcpy(wake, dcur + (size_t)4 * nb, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost));
+  if (s)
+    for (int j = 0; j < nb; ++j) {
+#pragma clang fp contract(off)
+      s[j] = (double)j * c->lsc_ds + c->lsc_a;
+    }
+  return OCSC_OK;
+}
+
+int ocsc_lsc_wake(ocsc_ctx* c, const double* s, const double* bunch, int64_t nb, double gamma, double sigma, double dz,
+                  int step_profile, double K_max, double fill_factor, double* wake, void* stream) {
+  if (!c || !s || !bunch || !wake) return fail(OCSC_EINVAL, "null argument");
+  if (nb < 2 || nb >= kLscMaxBins) return fail(OCSC_EINVAL, "profile length must be in [2, 2^22)");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = ensure(c->lsc_bins, (size_t)5 * nb * sizeof(double)))) return rc;
+  double* d = (double*)c->lsc_bins.p;
+  double* db = d + nb;
+  double* Zr = db + nb;
+  double* Zi = Zr + nb;
+  double* W = Zi + nb;
+  c->lsc_nbins = 0;  // the last kick's profile is overwritten
+  HIPCHK(hipMemcpyAsync(db, bunch, (size_t)nb * sizeof(double), hipMemcpyHostToDevice, st));
+  const WakeGrid g = wake_grid(s[0], s[1], nb);
+  const LscImp imp = lsc_imp(step_profile, gamma, sigma, dz, K_max, fill_factor);
+  if ((rc = lsc_wake_device(c, db, (int)nb, g, imp, -1.0, Zr, Zi, W, st))) return rc;
+  HIPCHK(hipMemcpyAsync(wake, W, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return OCSC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void k_lsc_impedance(const double* __restrict__ w, int64_t n, LscImp imp, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = lsc_impedance(w[i], imp);
+}
+}  // namespace
+
+extern "C" {
+
+int ocsc_lsc_impedance(ocsc_ctx* c, const double* w, int64_t n, double gamma, double sigma, double dz,
+                       int step_profile, double* z_imag, void* stream) {
+  if (!c || ((!w || !z_imag) && n > 0)) return fail(OCSC_EINVAL, "null argument");
+  if (n < 0 || n > ((int64_t)1 << 30)) return fail(OCSC_EINVAL, "bad n");
+  if (n == 0) return OCSC_OK;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = ensure(c->lsc_cic, (size_t)2 * n * sizeof(double)))) return rc;
+  double* dw = (double*)c->lsc_cic.p;
+  double* dz_out = dw + n;
+  HIPCHK(hipMemcpyAsync(dw, w, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+  const LscImp imp = lsc_imp(step_profile, gamma, sigma, dz, 0.0, 0.0);
+  k_lsc_impedance<<<(int)((n + 255) / 256), 256, 0, st>>>(dw, n, imp, dz_out);
+  LAUNCHCHK("k_lsc_impedance");
+  HIPCHK(hipMemcpyAsync(z_imag, dz_out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return OCSC_OK;
 }
 

@@ -3,6 +3,7 @@ attributes, repr, prepare() seeding, random-mesh draw order, deepcopy and
 pickle, the zstep == 0 no-op and the empty-bunch error.  Mirrors the
 reference class (ocelot/cpbd/sc.py:92-107, 208-212, 253-258)."""
 import copy
+import os
 import pickle
 
 import numpy as np
@@ -128,3 +129,69 @@ def test_twiss_defaults():
     from ocelot_amd import Twiss
     t = Twiss()
     assert t.emit_x == 0.0 and t.beta_x == 0.0 and t.Dx == 0.0 and t.xx == 0.0 and t.q == 0.0
+
+
+# ---------------------------------------------------------------------------
+# LSC host logic (no GPU): constructor defaults, the undulator profile of the
+# line and the filling factor against the values the reference computed for
+# the same line (tests/golden/lsc_golden.npz, written by make_golden_lsc.py).
+
+class _Elem:
+    def __init__(self, l):
+        self.l = l
+
+
+class Undulator(_Elem):
+    def __init__(self, lperiod, nperiods, Kx=0.0, Ky=0.0):
+        super().__init__(lperiod * nperiods)
+        self.Kx, self.Ky = Kx, Ky
+
+
+class _Line:
+    def __init__(self, seq):
+        self.sequence = seq
+
+    def get_sequence_part(self, start, stop):
+        i0 = self.sequence.index(start) if start is not None else 0
+        i1 = self.sequence.index(stop) if stop is not None else len(self.sequence) - 1
+        return self.sequence[i0:i1 + 1]
+
+
+def _lsc_line():
+    m1, m2 = _Elem(0.0), _Elem(0.0)
+    return m1, m2, _Line([m1, _Elem(0.7), Undulator(0.04, 25, Kx=1.8), _Elem(0.45), m2])
+
+
+def test_lsc_defaults_match_reference():
+    from ocelot_amd import LSC
+    lsc = LSC()
+    assert (lsc.step, lsc.step_profile, lsc.smooth_param, lsc.bounds, lsc.slice) == (1, False, 0.1, [-0.4, 0.4], None)
+    assert lsc.K_s_func is None and lsc._is_undul_in_beam_line is False
+    lsc2 = LSC(step=3, step_profile=True, smooth_param=0.2, bounds=[-1, 1])
+    assert (lsc2.step, lsc2.step_profile, lsc2.smooth_param, lsc2.bounds) == (3, True, 0.2, [-1, 1])
+    assert lsc.kick_args(0.1) == (0.0, 0.0)
+
+
+def test_lsc_undulator_profile_and_filling_factor():
+    from ocelot_amd import LSC
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "lsc_golden.npz"))
+    m1, m2, line = _lsc_line()
+    lsc = LSC()
+    lsc.start_elem, lsc.end_elem, lsc.s_start = m1, m2, 0.0
+    lsc.prepare(line)
+    assert lsc._is_undul_in_beam_line
+    np.testing.assert_array_equal(lsc.K_s_func(g["und/s_pts"]), g["und/K_pts"])
+    for (z0, dz), ff, K in zip(g["und/steps"], g["und/fill"], g["und/Kmax"]):
+        lsc.z0 = z0
+        assert lsc.compute_filling_factor(z0 - dz, z0, num_points=200) == ff
+        assert lsc.kick_args(dz) == (K, ff)
+
+
+def test_lsc_two_plane_undulator_is_ignored():
+    from ocelot_amd import LSC
+    m1, m2 = _Elem(0.0), _Elem(0.0)
+    line = _Line([m1, Undulator(0.04, 10, Kx=1.0, Ky=1.0), m2])
+    lsc = LSC()
+    lsc.start_elem, lsc.end_elem, lsc.s_start = m1, m2, 0.0
+    lsc.prepare(line)
+    assert not lsc._is_undul_in_beam_line
