@@ -111,8 +111,9 @@ def cpu_baseline(name, n_sample, steps=3, seed=0):
 def other_configs(torch, dev):
     """Context lines (not the headline): the 64-bunch batched configuration
     (uneven sizes, meshes 31/47/63, 20 kicks, mixed), single drop-in kicks
-    of the 2M-particle 63^3 bunch (FP64, device-resident and host numpy), and
-    the headline tracking loop with per-step Twiss (track()'s tws_track)."""
+    of the 2M-particle 63^3 bunch (FP64, device-resident and host numpy), the
+    headline tracking loop with per-step Twiss (track()'s tws_track), and the
+    LSC kick of the same resident bunch."""
     from ocelot_amd import DeviceParticleArray, ParticleArray, SpaceCharge, apply_batched
     from ocelot_amd.workloads import batch64, make_bunch
     out = {}
@@ -172,6 +173,19 @@ def other_configs(torch, dev):
     out["flattop_10m_127_track_with_twiss"] = {"particles": r.shape[1], "steps": 20, "ms_per_step": round(dt * 1e3, 4),
                                                "value": round(r.shape[1] / dt, 1), "unit": "macroparticle-kicks/s",
                                                "twiss_rows": len(tws), "precision": "mixed"}
+    # longitudinal space charge (LSC.apply) on the same resident bunch
+    from ocelot_amd import LSC
+    lsc = LSC()
+    lsc.apply(d, dz)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        lsc.apply(d, dz)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 20
+    out["flattop_10m_lsc_apply"] = {"particles": r.shape[1], "kicks": 20, "ms_per_kick": round(dt * 1e3, 4),
+                                    "value": round(r.shape[1] / dt, 1), "unit": "macroparticle-kicks/s",
+                                    "profile_bins": lsc.last_info["nbins"], "precision": "fp64"}
     return out
 
 
