@@ -11,8 +11,9 @@ from .physproc import PhysProc
 from .beam import ParticleArray, DeviceParticleArray
 from .sc import SpaceCharge, apply_batched, drift, track_drift
 from .lsc import LSC
+from .io import load_particle_array, save_particle_array
 from .envelope import Twiss, get_envelope
 from .autotune import tune_tracking
 
 __all__ = ["PhysProc", "ParticleArray", "DeviceParticleArray", "SpaceCharge", "LSC", "apply_batched", "drift",
-           "track_drift", "Twiss", "get_envelope", "tune_tracking"]
+           "track_drift", "Twiss", "get_envelope", "tune_tracking", "load_particle_array", "save_particle_array"]

@@ -165,3 +165,20 @@ def test_track_drift_with_twiss_list(graph):
         assert t.s == pytest.approx(k * dz, rel=1e-12)
     print(f"track_drift tws ({'graph' if graph else 'eager'}): worst relative deviation {worst:.2e} over {nsteps} steps")
     assert worst <= 1e-8
+
+
+def test_npz_bunch_loads_into_hbm(tmp_path):
+    """A bunch file the reference wrote, loaded straight into HBM, equals the
+    host load; saving the device bunch gives the same file contents."""
+    from ocelot_amd import io as oio
+    ref = os.path.join(HERE, "golden", "ref_bunch_track_wo_lsc.npz")
+    h = oio.load_particle_array(ref)
+    d = oio.load_particle_array(ref, device=0)
+    assert d.rparticles.is_cuda
+    np.testing.assert_array_equal(d.rparticles.cpu().numpy(), h.rparticles)
+    np.testing.assert_array_equal(d.q_array.cpu().numpy(), h.q_array)
+    assert d.E == float(h.E) and d.s == float(h.s)
+    f = str(tmp_path / "d.npz")
+    oio.save_particle_array(f, d)
+    back = oio.load_particle_array(f)
+    np.testing.assert_array_equal(back.rparticles, h.rparticles)
