@@ -177,6 +177,24 @@ int ocsc_lsc_wake(ocsc_ctx* ctx, const double* s, const double* bunch, int64_t n
 int ocsc_lsc_impedance(ocsc_ctx* ctx, const double* w, int64_t n, double gamma, double sigma, double dz,
                        int step_profile, double* z_imag, void* stream);
 
+/* csr_convolution (ocelot/cpbd/csr.py:69-79): the first na + nb - 2 terms of
+ * the linear convolution of a[na] and b[nb] (host arrays) into out.
+ * Synchronous. */
+int ocsc_csr_convolution(ocsc_ctx* ctx, const double* a, int64_t na, const double* b, int64_t nb, double* out,
+                         void* stream);
+
+/* The kick at the end of CSR.apply (ocelot/cpbd/csr.py:1105-1143) on the
+ * device-resident bunch: lam_K1 = csr_convolution(lam_ds, K1 reversed) / st *
+ * delta_s with st = (s2 - s1) / Ns, then delta += interp(-tau / st - sa / st,
+ * 0..len-1, lam_K1) 1e-9 / pc_ref with sa = s1 + st / 2 (numpy.interp's
+ * rules).  lam_ds[nlam] is the equidistant line charge and (s1, s2, Ns) its
+ * mesh, as the reference's binning returns them; K1[nk] the step-averaged CSR
+ * kernel (host arrays).  lam_K1 (host, may be NULL) receives the nlam + nk - 2
+ * wake values.  Synchronous. */
+int ocsc_csr_kick(ocsc_ctx* ctx, double* rparticles, int64_t ld, int64_t n, double E_GeV, const double* lam_ds,
+                  int64_t nlam, const double* K1, int64_t nk, double s1, double s2, double Ns, double delta_s,
+                  double* lam_K1, void* stream);
+
 /* One bunch of a batched call (fields as in ocsc_apply). */
 typedef struct ocsc_bunch {
   double* rparticles;

@@ -14,6 +14,7 @@
 #include "particles.hpp"
 #include "moments.hpp"
 #include "lsc.hpp"
+#include "csr.hpp"
 
 namespace ocsc {
 // sort.hip
@@ -196,6 +197,7 @@ struct ocsc_ctx {
   LscState* lsc = nullptr;
   LscState* lsc_host = nullptr;
   Buf lsc_bins, lsc_cic;
+  Buf csr_bins;  // CSR: line charge, kernel, convolution
   int lsc_nbins = 0;
   double lsc_a = 0, lsc_ds = 0;
 };
@@ -713,7 +715,7 @@ int ocsc_destroy(ocsc_ctx* c) {
   (void)hipDeviceSynchronize();
   for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
                  &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
-                 &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic})
+                 &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->csr_bins})
     if (b->p) (void)hipFree(b->p);
   if (c->lsc) (void)hipFree(c->lsc);
   if (c->lsc_host) (void)hipHostFree(c->lsc_host);
@@ -1828,6 +1830,78 @@ int ocsc_lsc_impedance(ocsc_ctx* c, const double* w, int64_t n, double gamma, do
   LAUNCHCHK("k_lsc_impedance");
   HIPCHK(hipMemcpyAsync(z_imag, dz_out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  return OCSC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// CSR kick tail (ocelot/cpbd/csr.py:69-79, 1105-1143)
+
+namespace {
+
+// c = conv(a, b)[:P+Q-2] / div * mul on the device; a, b host arrays
+int csr_conv_device(ocsc_ctx* c, const double* a, int64_t P, const double* b, int64_t Q, double div, double mul,
+                    hipStream_t st, double** out) {
+  if (P < 1 || Q < 1 || P + Q > (1 << 24)) return fail(OCSC_EINVAL, "convolution lengths must be >= 1, sum < 2^24");
+  const int64_t L = P + Q - 2;
+  int rc;
+  if ((rc = ensure(c->csr_bins, (size_t)(P + Q + std::max<int64_t>(L, 1)) * sizeof(double)))) return rc;
+  double* da = (double*)c->csr_bins.p;
+  double* db = da + P;
+  double* dc = db + Q;
+  HIPCHK(hipMemcpyAsync(da, a, (size_t)P * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(db, b, (size_t)Q * sizeof(double), hipMemcpyHostToDevice, st));
+  if (L > 0) k_conv<<<(int)L, 256, 0, st>>>(da, (int)P, db, (int)Q, div, mul, dc);
+  LAUNCHCHK("k_conv");
+  *out = dc;
+  return OCSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocsc_csr_convolution(ocsc_ctx* c, const double* a, int64_t na, const double* b, int64_t nb, double* out,
+                         void* stream) {
+  if (!c || !a || !b || (!out && na + nb > 2)) return fail(OCSC_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  double* dc = nullptr;
+  int rc;
+  if ((rc = csr_conv_device(c, a, na, b, nb, 1.0, 1.0, st, &dc))) return rc;
+  if (na + nb > 2) HIPCHK(hipMemcpyAsync(out, dc, (size_t)(na + nb - 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return OCSC_OK;
+}
+
+int ocsc_csr_kick(ocsc_ctx* c, double* r, int64_t ld, int64_t n, double E_GeV, const double* lam_ds, int64_t nlam,
+                  const double* K1, int64_t nk, double s1, double s2, double Ns, double delta_s, double* lam_K1,
+                  void* stream) {
+#pragma clang fp contract(off)
+  if (!c || !lam_ds || !K1 || (!r && n > 0)) return fail(OCSC_EINVAL, "null argument");
+  if (n < 0 || ld < n) return fail(OCSC_EINVAL, "bad n / ld");
+  if (nlam + nk - 2 < 1) return fail(OCSC_EVALUE, "array of sample points is empty");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  // csr.py:1105-1106 and 1138-1143, in the reference's operation order
+  const double stp = (s2 - s1) / Ns;
+  const double sa = s1 + stp / 2.;
+  const double inv_st = 1. / stp;
+  const double off = 0. - sa / stp;
+  const double pc_ref = std::sqrt(E_GeV * E_GeV / (kMeGeV * kMeGeV) - 1) * kMeGeV;
+  std::vector<double> krev(nk);
+  for (int64_t k = 0; k < nk; ++k) krev[k] = K1[nk - 1 - k];
+  double* dlk = nullptr;
+  int rc;
+  if ((rc = csr_conv_device(c, lam_ds, nlam, krev.data(), nk, stp, delta_s, st, &dlk))) return rc;
+  const int L = (int)(nlam + nk - 2);
+  if (n > 0) {
+    k_csr_kick<<<nred_blocks(c, n), kPT, 0, st>>>(r, ld, n, dlk, L, inv_st, off, pc_ref);
+    LAUNCHCHK("k_csr_kick");
+  }
+  if (lam_K1) HIPCHK(hipMemcpyAsync(lam_K1, dlk, (size_t)L * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host copy of the reversed kernel is read by the upload
   return OCSC_OK;
 }
 
