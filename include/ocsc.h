@@ -157,6 +157,20 @@ int ocsc_lsc_apply(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* 
                    double dz, int step_profile, double smooth_param, double b0, double b1, double K_max,
                    double fill_factor, double* info, int ninfo, void* stream);
 
+/* ocsc_track_drift with an LSC kick (as ocsc_lsc_apply, step length lsc_dz,
+ * no undulator factor) after every space-charge kick: per step drift(L),
+ * SpaceCharge.apply(zstep), LSC.apply(lsc_dz), the order track() applies two
+ * processes added in that order (ocelot/cpbd/track.py:470-477).  zstep == 0
+ * leaves the space-charge kick out; lsc_dz < 1e-10 the LSC kick.  Graph-
+ * capturable: the profile grid is planned on the device, up to 8192 bins;
+ * a step whose profile would need more, or whose bunch has no spread in tau
+ * or an empty step-profile slice, gets no LSC kick and the call returns
+ * OCSC_EVALUE at the end.  Cannot be combined with the option "kick_timing". */
+int ocsc_track_drift_lsc(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,
+                         const int32_t nmesh[3], double E_GeV, double zstep, double L, int nsteps, int precision,
+                         int use_graph, double lsc_dz, int step_profile, double smooth_param, double b0, double b1,
+                         void* stream);
+
 /* Profile grid, current (A) and wake W (V, the kick before scaling) of the
  * last ocsc_lsc_apply on this context (the quantities B[:, 0], B[:, 1] and W
  * of sc.py:582-587), copied to host arrays of at least *nbins entries (any

@@ -120,44 +120,6 @@ __global__ void __launch_bounds__(1024) k_lsc_stat_reduce(LscState* __restrict__
 // B1 (A) and bunch = current/(q0 c) (1/m, sc.py:585).  One workgroup.
 constexpr int kLscMaxTaps = 64;
 
-__global__ void __launch_bounds__(1024) k_lsc_profile(const long long* __restrict__ C, int nbins, int K, double ds,
-                                                      double sigma, double q0, double v, double* __restrict__ cur,
-                                                      double* __restrict__ bunch) {
-  __shared__ double G[kLscMaxTaps];
-  __shared__ double lds[16];
-  __shared__ double tot;
-  if (threadIdx.x == 0) {
-    double gs = 0;
-    for (int m = 0; m <= 2 * K; ++m) {
-      const double x = (double)(m - K) * ds / sigma;
-      G[m] = exp(-0.5 * (x * x));
-      gs += G[m];
-    }
-    for (int m = 0; m <= 2 * K; ++m) G[m] = G[m] / gs;
-  }
-  __syncthreads();
-  double part = 0;
-  for (int j = threadIdx.x; j < nbins; j += blockDim.x) {
-    double acc = 0;
-    for (int m = 0; m <= 2 * K; ++m) {
-      const int i = j + K - m;
-      if (i >= 0 && i < nbins) acc += (double)C[i] * G[m];
-    }
-    cur[j] = acc;
-    part += acc;
-  }
-  block_reduce<1, 0>(&part, lds);
-  if (threadIdx.x == 0) tot = part;
-  __syncthreads();
-  const double scale = q0 * v / (ds * tot);
-  const double qc = q0 * kC;
-  for (int j = threadIdx.x; j < nbins; j += blockDim.x) {
-    const double b = scale * cur[j];
-    cur[j] = b;
-    bunch[j] = b / qc;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Impedances (imaginary parts; both are purely reactive).
 constexpr double kEulerGamma = 0.57721566490153286061;
@@ -234,6 +196,112 @@ struct LscImp {
   double factor;  // 1 + K_max^2 fill_factor / 2
 };
 
+// Everything the per-bin and kick kernels need for one kick, formed from the
+// slice statistics either on the host (ocsc_lsc_apply) or on the device
+// (tracking graphs).  nbins == 0 means "no kick" (err says why).
+struct LscPlan {
+  int nbins, K, err;
+  double a, ds, sig;          // profile grid a + j ds, smoothing width
+  double sigma;               // transverse size (Gaussian rms or step radius)
+  double q0, v, pc_ref;       // total charge, beam speed, reference momentum (GeV)
+  double scale;               // W = scale w: q0 for the kick, -1 for wake_lsc
+  double inv_dt0, dt, ndf;    // wake_lsc's frequency and step constants
+  LscImp imp;
+};
+
+// inputs of a kick that do not come from the particles
+struct LscConst {
+  int step_profile;
+  double smooth, gamma, v, pc_ref, dz, K_max, fill_factor;
+  int cap;  // most profile bins allowed
+};
+
+enum { kLscOk = 0, kLscNoSpread = 1, kLscTooManyBins = 2, kLscEmptySlice = 3, kLscWideSmoothing = 4 };
+
+constexpr double kZ0 = 1. / (kC * kEps0);  // globals.py: Z0 = 1/(c eps0)
+
+// impedance constants (sc.py:302-369, 446-449), formed as numpy forms them
+__host__ __device__ inline LscImp lsc_imp(int step_profile, double gamma, double sigma, double dz, double K_max,
+                                          double fill_factor) {
+#pragma clang fp contract(off)
+  LscImp p{};
+  p.step_profile = step_profile;
+  p.sigma = sigma;
+  p.dz = dz;
+  p.factor = 1 + 0.5 * K_max * K_max * fill_factor;
+  if (!step_profile) {
+    p.coef = kZ0 / (4 * kPi * kC * (gamma * gamma));
+    p.gamma_c = gamma * kC;
+  } else {
+    p.coef = kZ0 * kC;
+    p.gamma_c = kC * gamma;
+  }
+  return p;
+}
+
+// step and frequency constants of wake_lsc for a profile grid starting s0,
+// s1 (sc.py:442-465, analysis.py:376-383 and sc.py:403-409)
+__host__ __device__ inline void lsc_wake_grid(double s0, double s1, int64_t nb, LscPlan* p) {
+#pragma clang fp contract(off)
+  const double ds = s1 - s0;
+  const double dt0 = ds / kC;
+  p->inv_dt0 = 1 / dt0;
+  const double n = (double)(2 * nb);
+  const double sb0 = s0 + 1.0 * ds, sb1 = s0 + 2.0 * ds;
+  p->dt = (sb1 - sb0) / kC;
+  const double df = 1 / p->dt * 1.0 / n;
+  p->ndf = n * df;
+}
+
+// the plan of one kick (sc.py:566-587 with analysis.py:306-315), in the
+// reference's operation order
+__host__ __device__ inline void lsc_make_plan(const LscState& h, const LscConst& k, LscPlan* p) {
+#pragma clang fp contract(off)
+  *p = LscPlan{};
+  double sigma;
+  if (k.step_profile) {
+    if (!(h.cnt > 0)) {
+      p->err = kLscEmptySlice;
+      return;
+    }
+    const double ex = h.xmax - h.xmin, ey = h.ymax - h.ymin;
+    sigma = (ey < ex ? ey : ex) / 2;
+  } else {
+    // numpy's std of an empty slice is NaN, which zeroes the impedance
+    sigma = h.cnt > 0 ? (sqrt(h.xvar) + sqrt(h.yvar)) / 2. : __builtin_nan("");
+  }
+  const double sig = h.tau_std * k.smooth;
+  const double a = h.tau_min - 3 * sig, b = h.tau_max + 3 * sig;
+  const double nf = ceil((b - a) / (0.25 * sig));
+  if (!(nf >= 1) || !(nf <= 1e300)) {
+    p->err = kLscNoSpread;
+    return;
+  }
+  if (nf + 1 > (double)k.cap) {
+    p->err = kLscTooManyBins;
+    return;
+  }
+  const int N = (int)nf;
+  const double ds = (b - a) / N;
+  const int K = (int)floor(3 * sig / ds + 0.5);
+  if (2 * K + 1 > kLscMaxTaps) {
+    p->err = kLscWideSmoothing;
+    return;
+  }
+  p->nbins = N + 1;
+  p->K = K;
+  p->a = a;
+  p->ds = ds;
+  p->sig = sig;
+  p->sigma = sigma;
+  p->q0 = h.sumq;
+  p->v = k.v;
+  p->pc_ref = k.pc_ref;
+  p->scale = h.sumq;  // W = -wake q (sc.py:587)
+  lsc_wake_grid(a, 1.0 * ds + a, p->nbins, p);
+  p->imp = lsc_imp(k.step_profile, k.gamma, sigma, k.dz, k.K_max, k.fill_factor);
+}
+
 // imaginary part of the impedance at angular frequency w (sc.py:302-369)
 __device__ inline double lsc_impedance(double w, const LscImp& p) {
 #pragma clang fp contract(off)
@@ -271,12 +339,14 @@ __device__ inline void unit_root(int64_t m, int64_t n, double* c, double* s) {
 
 // spectrum on the doubled grid (n = 2 nb): Zb_k = dt sum_j c bunch_j e^{-2 pi i jk/n}
 // (the zero padding adds nothing), times the impedance at w_k = 2 pi f_k,
-// f_k = (1/dt0) k / n; stored as Z = i Zim_a Zb for k < nb.  One workgroup per k.
-__global__ void __launch_bounds__(256) k_lsc_spectrum(const double* __restrict__ bunch, int nb, double dt,
-                                                      double inv_dt0, LscImp imp, double* __restrict__ Zr,
-                                                      double* __restrict__ Zi) {
+// f_k = (1/dt0) k / n; stored as Z = i Zim_a Zb for k < nb.  One workgroup
+// per k; workgroups beyond the plan's bins return at once.
+__global__ void __launch_bounds__(256) k_lsc_spectrum(const double* __restrict__ bunch, const LscPlan* __restrict__ p,
+                                                      double* __restrict__ Zr, double* __restrict__ Zi) {
   __shared__ double lds[4 * 2];
   const int k = blockIdx.x;
+  const int nb = p->nbins;
+  if (k >= nb) return;
   const int64_t n = 2 * (int64_t)nb;
   double v[2] = {0.0, 0.0};
   for (int j = threadIdx.x; j < nb; j += 256) {
@@ -288,9 +358,9 @@ __global__ void __launch_bounds__(256) k_lsc_spectrum(const double* __restrict__
   }
   block_reduce<2, 0>(v, lds);
   if (threadIdx.x == 0) {
-    const double br = dt * v[0], bi = dt * v[1];
-    const double f = inv_dt0 * (double)k / (double)n;
-    const double za = lsc_impedance(f * 2 * kPi, imp);
+    const double br = p->dt * v[0], bi = p->dt * v[1];
+    const double f = p->inv_dt0 * (double)k / (double)n;
+    const double za = lsc_impedance(f * 2 * kPi, p->imp);
     Zr[k] = -za * bi;
     Zi[k] = za * br;
   }
@@ -300,10 +370,12 @@ __global__ void __launch_bounds__(256) k_lsc_spectrum(const double* __restrict__
 // are Z_0..Z_{nb-1}, conj(Z_{nb-1}) (the reference's completion), scaled by
 // n df: w_m = df [Re Z_0 + 2 sum_{k=1}^{nb-1} Re(Z_k e^{2 pi i km/n}) + Re Z_{nb-1} (-1)^m];
 // out_m = scale w_m.  One workgroup per m.
-__global__ void __launch_bounds__(256) k_lsc_wake(const double* __restrict__ Zr, const double* __restrict__ Zi, int nb,
-                                                  double ndf, double scale, double* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_lsc_wake(const double* __restrict__ Zr, const double* __restrict__ Zi,
+                                                  const LscPlan* __restrict__ p, double* __restrict__ out) {
   __shared__ double lds[4];
   const int m = blockIdx.x;
+  const int nb = p->nbins;
+  if (m >= nb) return;
   const int64_t n = 2 * (int64_t)nb;
   double acc = 0.0;
   for (int k = 1 + threadIdx.x; k < nb; k += 256) {
@@ -314,7 +386,7 @@ __global__ void __launch_bounds__(256) k_lsc_wake(const double* __restrict__ Zr,
   block_reduce<1, 0>(&acc, lds);
   if (threadIdx.x == 0) {
     const double tot = Zr[0] + 2.0 * acc + ((m & 1) ? -Zr[nb - 1] : Zr[nb - 1]);
-    out[m] = ndf * (tot / (double)n) * scale;
+    out[m] = p->ndf * (tot / (double)n) * p->scale;
   }
 }
 
@@ -326,9 +398,11 @@ __device__ inline double lsc_grid(int j, double a, double ds) {
 
 // delta += interp(tau, grid, W) 1e-9 / pc_ref, numpy.interp's rules
 __global__ void __launch_bounds__(kPT) k_lsc_kick(double* __restrict__ r, int64_t ld, int64_t n,
-                                                  const double* __restrict__ W, int nbins, double a, double ds,
-                                                  double pc_ref) {
+                                                  const double* __restrict__ W, const LscPlan* __restrict__ p) {
 #pragma clang fp contract(off)
+  const int nbins = p->nbins;
+  if (nbins == 0) return;
+  const double a = p->a, ds = p->ds, pc_ref = p->pc_ref;
   const double x0 = lsc_grid(0, a, ds), xl = lsc_grid(nbins - 1, a, ds);
   for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
     const double tau = r[4 * ld + i];
@@ -360,6 +434,99 @@ __global__ void __launch_bounds__(kPT) k_lsc_kick(double* __restrict__ r, int64_
     }
     r[5 * ld + i] = r[5 * ld + i] + dE * 1e-9 / pc_ref;
   }
+}
+
+// smoothed, normalised profile (analysis.py:322-339, sc.py:585), one workgroup
+__global__ void __launch_bounds__(1024) k_lsc_profile(const long long* __restrict__ C, const LscPlan* __restrict__ p,
+                                                      double* __restrict__ cur, double* __restrict__ bunch) {
+  __shared__ double G[kLscMaxTaps];
+  __shared__ double lds[16];
+  __shared__ double tot;
+  const int nbins = p->nbins, K = p->K;
+  if (nbins == 0) return;
+  const double ds = p->ds, sigma = p->sig;
+  if (threadIdx.x == 0) {
+    double gs = 0;
+    for (int m = 0; m <= 2 * K; ++m) {
+      const double x = (double)(m - K) * ds / sigma;
+      G[m] = exp(-0.5 * (x * x));
+      gs += G[m];
+    }
+    for (int m = 0; m <= 2 * K; ++m) G[m] = G[m] / gs;
+  }
+  __syncthreads();
+  double part = 0;
+  for (int j = threadIdx.x; j < nbins; j += blockDim.x) {
+    double acc = 0;
+    for (int m = 0; m <= 2 * K; ++m) {
+      const int i = j + K - m;
+      if (i >= 0 && i < nbins) acc += (double)C[i] * G[m];
+    }
+    cur[j] = acc;
+    part += acc;
+  }
+  block_reduce<1, 0>(&part, lds);
+  if (threadIdx.x == 0) tot = part;
+  __syncthreads();
+  const double scale = p->q0 * p->v / (ds * tot);
+  const double qc = p->q0 * kC;
+  for (int j = threadIdx.x; j < nbins; j += blockDim.x) {
+    const double b = scale * cur[j];
+    cur[j] = b;
+    bunch[j] = b / qc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Inside tracking graphs: state set-up, the plan on the device, and the
+// current histogram on a grid read from the plan (one LDS window of at most
+// kCicLds bins).
+__global__ void k_lsc_init(LscState* __restrict__ s, int64_t n, double b0, double b1) {
+  if (threadIdx.x != 0) return;
+  LscState h{};
+  h.n = n;
+  h.b0 = b0;
+  h.b1 = b1;
+  *s = h;
+}
+
+__global__ void k_lsc_plan(const LscState* __restrict__ s, LscConst k, LscPlan* __restrict__ p, int* __restrict__ err) {
+  if (threadIdx.x != 0) return;
+  LscPlan q;
+  lsc_make_plan(*s, k, &q);
+  *p = q;
+  if (q.err && *err == 0) *err = q.err;
+}
+
+__global__ void __launch_bounds__(kPT) k_cic_plan(Rows R, int64_t n, const LscPlan* __restrict__ p, double wscale,
+                                                  long long* __restrict__ hist) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  long long* h = (long long*)smem_raw;
+  const int nbins = p->nbins;
+  if (nbins == 0) return;
+  const double a = p->a, ds = p->ds;
+  for (int j = threadIdx.x; j < nbins; j += kPT) h[j] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
+    const double c = (R.r[4 * R.ld + i] - a) / ds;
+    const double fl = __builtin_floor(c);
+    const double xi = 1 + fl - c;
+    const int I = (int)(fl < 0 ? 0 : (fl > nbins - 2 ? nbins - 2 : fl));
+    atomicAdd((unsigned long long*)&h[I], (unsigned long long)(long long)__builtin_rint(xi * wscale));
+    atomicAdd((unsigned long long*)&h[I + 1], (unsigned long long)(long long)__builtin_rint((1 - xi) * wscale));
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nbins; j += kPT) hist[(int64_t)blockIdx.x * nbins + j] = h[j];
+}
+
+__global__ void __launch_bounds__(256) k_cic_sum_plan(const long long* __restrict__ hist, int nblk,
+                                                      const LscPlan* __restrict__ p, long long* __restrict__ out) {
+  const int nbins = p->nbins;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= nbins) return;
+  long long acc = 0;
+  for (int b = 0; b < nblk; ++b) acc += hist[(int64_t)b * nbins + j];
+  out[j] = acc;
 }
 
 }  // namespace ocsc

@@ -154,6 +154,12 @@ struct Lane {
   hipEvent_t done = nullptr;       // extra lanes only
 };
 
+// pinned mirror of the LSC state and plan
+struct LscHost {
+  LscState s;
+  LscPlan p;
+};
+
 struct ocsc_ctx {
   int device = 0;
   Lane lane0;
@@ -195,7 +201,9 @@ struct ocsc_ctx {
   // longitudinal space charge: device state, pinned mirror, per-bin arrays
   // and the profile grid of the last kick
   LscState* lsc = nullptr;
-  LscState* lsc_host = nullptr;
+  LscPlan* lsc_plan = nullptr;
+  int* lsc_err = nullptr;  // first plan error inside tracking graphs
+  LscHost* lsc_host = nullptr;
   Buf lsc_bins, lsc_cic;
   Buf csr_bins;  // CSR: line charge, kernel, convolution
   int lsc_nbins = 0;
@@ -718,6 +726,8 @@ int ocsc_destroy(ocsc_ctx* c) {
                  &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->csr_bins})
     if (b->p) (void)hipFree(b->p);
   if (c->lsc) (void)hipFree(c->lsc);
+  if (c->lsc_plan) (void)hipFree(c->lsc_plan);
+  if (c->lsc_err) (void)hipFree(c->lsc_err);
   if (c->lsc_host) (void)hipHostFree(c->lsc_host);
   if (L.sc) (void)hipFree(L.sc);
   if (L.sc_host) (void)hipHostFree(L.sc_host);
@@ -1119,9 +1129,22 @@ void moments_row(const MomState& h, double* v) {
   v[26] = h.tau_all;
 }
 
+// an LSC kick after every space-charge kick of a tracking loop
+struct LscReq {
+  LscConst k;
+  double b0, b1;
+};
+int lsc_buffers(ocsc_ctx* c, int64_t bins);
+LscConst lsc_const(int step_profile, double smooth, double E_GeV, double dz, double K_max, double fill_factor,
+                   int cap);
+constexpr int kLscGraphBins = kCicLds;  // kicks inside tracking graphs (device-planned grid)
+int enqueue_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const LscConst& k, double b0,
+                double b1, hipStream_t st);
+const char* lsc_err_text(int e);
+
 int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
                double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, void* stream,
-               const TwsReq* tw, double* tws_out) {
+               const TwsReq* tw, double* tws_out, const LscReq* lq = nullptr) {
   int rc;
   if ((rc = check_common(c, r, q, n, ld))) return rc;
   if (!nmesh || nsteps < 0) return fail(OCSC_EINVAL, "bad nmesh / nsteps");
@@ -1142,6 +1165,13 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   const int nb = (int)((n + 255) / 256);
   if (nsteps == 0 && !tw) return OCSC_OK;
   if ((rc = ensure_part(c, L, n))) return rc;
+  if (lq) {
+    if (c->kick_timing) return fail(OCSC_EINVAL, "kick timing and LSC kicks cannot be combined");
+    if ((rc = lsc_buffers(c, lq->k.cap))) return rc;
+    if ((rc = ensure(c->lsc_cic, ((size_t)256 * lq->k.cap + lq->k.cap) * sizeof(long long)))) return rc;
+    if ((rc = ensure(L.part, (size_t)12 * nred_blocks(c, n) * sizeof(double)))) return rc;
+    k_counter_reset<<<1, 64, 0, st>>>(c->lsc_err);
+  }
   if (tw) {
     if (c->kick_timing) return fail(OCSC_EINVAL, "kick timing and per-step moments cannot be combined");
     if (!c->mom) {
@@ -1229,6 +1259,12 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     o.prev_nblk = npart;
     o.drift = o.sump = step < nsteps - 1;
     if (tw) o.drift = false;  // moments are taken between the kick and the next drift
+    if (lq) {
+      // the LSC kick changes delta after the space-charge kick: no fused
+      // drift, and every kick forms its own momentum sums
+      o.drift = o.sump = false;
+      o.passA = true;
+    }
     o.dr = DriftArgs{Ld, r56};
     if (ktime) {
       o.kev0 = c->kick_ev[2 * step];
@@ -1241,6 +1277,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   if (zstep == 0) {
     for (int step = 0; step < nsteps; ++step) {
       k_drift<<<nb, 256, 0, st>>>(rr, lr, n, Ld, r56);
+      if (lq && (rc = enqueue_lsc(c, rr, lr, qq, n, lq->k, lq->b0, lq->b1, st))) return rc;
       if (tw && (rc = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, st))) return rc;
     }
   } else if (nsteps > 0) {
@@ -1251,10 +1288,9 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   auto step_body = [&](int step, const StepOpts& o, hipStream_t s2) -> int {
     int rc3;
     if ((rc3 = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, s2, o))) return rc3;
-    if (tw) {
-      if ((rc3 = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, s2))) return rc3;
-      if (step < nsteps - 1) k_drift<<<nb, 256, 0, s2>>>(rr, lr, n, Ld, r56);
-    }
+    if (lq && (rc3 = enqueue_lsc(c, rr, lr, qq, n, lq->k, lq->b0, lq->b1, s2))) return rc3;
+    if (tw && (rc3 = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, s2))) return rc3;
+    if ((tw || lq) && step < nsteps - 1) k_drift<<<nb, 256, 0, s2>>>(rr, lr, n, Ld, r56);
     return OCSC_OK;
   };
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -1368,12 +1404,30 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     for (int k = 0; k <= nsteps; ++k) moments_row(tab[k], tws_out + (size_t)k * OCSC_MOM_NOUT);
   }
   if (err) return value_error(err);
+  if (lq) {
+    int lerr = 0;
+    HIPCHK(hipMemcpyAsync(&lerr, c->lsc_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (lerr) return fail(OCSC_EVALUE, std::string("LSC kick skipped: ") + lsc_err_text(lerr));
+  }
   return OCSC_OK;
 }
 
 }  // namespace
 
 extern "C" {
+
+int ocsc_track_drift_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
+                         double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, double lsc_dz,
+                         int step_profile, double smooth_param, double b0, double b1, void* stream) {
+  if (!(lsc_dz >= 1e-10)) return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph,
+                                            stream, nullptr, nullptr);  // sc.py:556-558
+  LscReq lq;
+  lq.k = lsc_const(step_profile, smooth_param, E_GeV, lsc_dz, 0.0, 0.0, kLscGraphBins);
+  lq.b0 = b0;
+  lq.b1 = b1;
+  return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph, stream, nullptr, nullptr, &lq);
+}
 
 int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
                      double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, void* stream) {
@@ -1608,92 +1662,39 @@ int ocsc_last_stats(ocsc_ctx* c, double* out, int nout, void* stream) {
 
 namespace {
 
-constexpr double kZ0 = 1. / (kC * kEps0);  // globals.py: Z0 = 1/(c eps0)
+constexpr int kLscMaxBins = 1 << 22;  // one-shot kicks (host-planned grid)
 
-// impedance constants (sc.py:302-369, 446-449), formed as numpy forms them
-// (no fused multiply-adds in these host expressions)
-LscImp lsc_imp(int step_profile, double gamma, double sigma, double dz, double K_max, double fill_factor) {
+LscConst lsc_const(int step_profile, double smooth, double E_GeV, double dz, double K_max, double fill_factor,
+                   int cap) {
 #pragma clang fp contract(off)
-  LscImp p{};
-  p.step_profile = step_profile;
-  p.sigma = sigma;
-  p.dz = dz;
-  p.factor = 1 + 0.5 * K_max * K_max * fill_factor;
-  if (!step_profile) {
-    p.coef = kZ0 / (4 * kPi * kC * (gamma * gamma));
-    p.gamma_c = gamma * kC;
-  } else {
-    p.coef = kZ0 * kC;
-    p.gamma_c = kC * gamma;
-  }
-  return p;
+  LscConst k{};
+  k.step_profile = step_profile;
+  k.smooth = smooth;
+  k.gamma = E_GeV / kMeGeV;                                               // sc.py:579
+  k.v = std::sqrt(1 - 1 / (k.gamma * k.gamma)) * kC;                      // sc.py:580
+  k.pc_ref = std::sqrt(E_GeV * E_GeV / (kMeGeV * kMeGeV) - 1) * kMeGeV;  // sc.py:593
+  k.dz = dz;
+  k.K_max = K_max;
+  k.fill_factor = fill_factor;
+  k.cap = cap;
+  return k;
 }
 
-// step and frequency constants of wake_lsc for a profile grid starting
-// s0, s0 + ds (sc.py:442-465, analysis.py:376-383 and sc.py:403-409)
-struct WakeGrid {
-  double inv_dt0;  // 1/dt of the profile grid (impedance frequencies)
-  double dt;       // dt of the padded grid s0 + (k + 1) ds (spectrum)
-  double ndf;      // n df of the padded grid (inverse transform)
-};
-
-WakeGrid wake_grid(double s0, double s1, int64_t nb) {
-#pragma clang fp contract(off)
-  WakeGrid g{};
-  const double ds = s1 - s0;
-  const double dt0 = ds / kC;
-  g.inv_dt0 = 1 / dt0;
-  const double n = (double)(2 * nb);
-  const double sb0 = s0 + 1.0 * ds, sb1 = s0 + 2.0 * ds;
-  g.dt = (sb1 - sb0) / kC;
-  const double df = 1 / g.dt * 1.0 / n;
-  g.ndf = n * df;
-  return g;
-}
-
-// spectrum, impedance and inverse transform on the device; out_m = scale w_m
-int lsc_wake_device(ocsc_ctx* c, const double* bunch, int nb, const WakeGrid& g, const LscImp& imp, double scale,
-                    double* Zr, double* Zi, double* out, hipStream_t st) {
-  k_lsc_spectrum<<<nb, 256, 0, st>>>(bunch, nb, g.dt, g.inv_dt0, imp, Zr, Zi);
-  k_lsc_wake<<<nb, 256, 0, st>>>(Zr, Zi, nb, g.ndf, scale, out);
-  LAUNCHCHK("k_lsc_wake");
-  (void)c;
-  return OCSC_OK;
-}
-
-constexpr int kLscMaxBins = 1 << 22;
-
-}  // namespace
-
-extern "C" {
-
-int ocsc_lsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, double E_GeV, double dz,
-                   int step_profile, double smooth_param, double b0, double b1, double K_max, double fill_factor,
-                   double* info, int ninfo, void* stream) {
-#pragma clang fp contract(off)
-  if (!c || ((!r || !q) && n > 0) || (ninfo > 0 && !info)) return fail(OCSC_EINVAL, "null argument");
-  if (n < 0 || ld < n) return fail(OCSC_EINVAL, "bad n / ld");
-  if (dz < 1e-10) return OCSC_OK;  // sc.py:556-558
-  if (n == 0) return fail(OCSC_EVALUE, "zero-size array to reduction operation minimum which has no identity");
-  HIPCHK(hipSetDevice(c->device));
-  hipStream_t st = (hipStream_t)stream;
+int lsc_buffers(ocsc_ctx* c, int64_t bins) {
   if (!c->lsc) {
     HIPCHK(hipMalloc(&c->lsc, sizeof(LscState)));
-    HIPCHK(hipHostMalloc(&c->lsc_host, sizeof(LscState), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&c->lsc_plan, sizeof(LscPlan)));
+    HIPCHK(hipMalloc(&c->lsc_err, sizeof(int)));
+    HIPCHK(hipHostMalloc(&c->lsc_host, sizeof(LscHost), hipHostMallocDefault));
   }
-  Lane& L = c->lane0;
+  return ensure(c->lsc_bins, (size_t)5 * bins * sizeof(double));
+}
+
+// slice statistics (sc.py:566-577) into c->lsc; the state is initialised by
+// the caller
+void lsc_stats(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, int step_profile, hipStream_t st) {
+  double* part = (double*)c->lane0.part.p;
   const int nblk = nred_blocks(c, n);
-  int rc;
-  if ((rc = ensure(L.part, (size_t)12 * nblk * sizeof(double)))) return rc;
-  HIPCHK(hipStreamSynchronize(st));  // the pinned mirror may still be read by an earlier copy
-  LscState& h = *c->lsc_host;
-  std::memset(&h, 0, sizeof(h));
-  h.n = n;
-  h.b0 = b0;
-  h.b1 = b1;
-  HIPCHK(hipMemcpyAsync(c->lsc, &h, sizeof(h), hipMemcpyHostToDevice, st));
-  double* part = (double*)L.part.p;
-  const Rows R{r, ld};
   k_lsc_stat<0><<<nblk, kPT, 0, st>>>(R, q, c->lsc, part);
   k_lsc_stat_reduce<0><<<1, 1024, 0, st>>>(c->lsc, part, nblk);
   k_lsc_stat<1><<<nblk, kPT, 0, st>>>(R, q, c->lsc, part);
@@ -1704,59 +1705,102 @@ int ocsc_lsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t 
     k_lsc_stat<3><<<nblk, kPT, 0, st>>>(R, q, c->lsc, part);
     k_lsc_stat_reduce<3><<<1, 1024, 0, st>>>(c->lsc, part, nblk);
   }
-  LAUNCHCHK("k_lsc_stat");
-  HIPCHK(hipMemcpyAsync(&h, c->lsc, sizeof(h), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  // transverse size of the slice (sc.py:572-577); an empty slice gives NaN
-  // for the Gaussian profile (numpy's std of nothing), which zeroes the
-  // impedance (sc.py:322-323), and an error for the step profile
-  double sigma;
-  if (step_profile) {
-    if (!(h.cnt > 0)) return fail(OCSC_EVALUE, "zero-size array to reduction operation maximum which has no identity");
-    const double ex = h.xmax - h.xmin, ey = h.ymax - h.ymin;
-    sigma = (ey < ex ? ey : ex) / 2;
-  } else {
-    sigma = h.cnt > 0 ? (std::sqrt(h.xvar) + std::sqrt(h.yvar)) / 2. : std::nan("");
-  }
-  // profile grid of s_to_cur(tau, std(tau) smooth_param) (analysis.py:306-315)
-  const double sig = h.tau_std * smooth_param;
-  const double a = h.tau_min - 3 * sig, b = h.tau_max + 3 * sig;
-  const double nf = std::ceil((b - a) / (0.25 * sig));
-  if (!std::isfinite(nf) || !(nf >= 1))
-    return fail(OCSC_EVALUE, "current profile: tau has no spread or is not finite (cannot size the grid)");
-  if (nf >= kLscMaxBins) return fail(OCSC_EVALUE, "current profile: more than 2^22 bins");
-  const int N = (int)nf;
-  const double ds = (b - a) / N;
-  const int nbins = N + 1;
-  const int K = (int)std::floor(3 * sig / ds + 0.5);
-  if (2 * K + 1 > kLscMaxTaps) return fail(OCSC_EVALUE, "current profile: smoothing kernel too wide");
-  // beam constants (sc.py:579-581, 593)
-  const double gamma = E_GeV / kMeGeV;
-  const double v = std::sqrt(1 - 1 / (gamma * gamma)) * kC;
-  const double pc_ref = std::sqrt(E_GeV * E_GeV / (kMeGeV * kMeGeV) - 1) * kMeGeV;
-  // per-bin arrays: current, bunch, Z (re, im), W
-  if ((rc = ensure(c->lsc_bins, (size_t)5 * nbins * sizeof(double)))) return rc;
+}
+
+// profile, wake and kick from the histogram totals and the plan on the
+// device; grids of `grid` workgroups (the plan's bins, or the cap)
+int lsc_from_profile(ocsc_ctx* c, double* r, int64_t ld, int64_t n, const long long* tot, int grid, hipStream_t st) {
   double* cur = (double*)c->lsc_bins.p;
-  double* bunch = cur + nbins;
-  double* Zr = bunch + nbins;
-  double* Zi = Zr + nbins;
-  double* W = Zi + nbins;
+  const int64_t cap = (int64_t)(c->lsc_bins.cap / (5 * sizeof(double)));
+  double* bunch = cur + cap;
+  double* Zr = bunch + cap;
+  double* Zi = Zr + cap;
+  double* W = Zi + cap;
+  k_lsc_profile<<<1, 1024, 0, st>>>(tot, c->lsc_plan, cur, bunch);
+  k_lsc_spectrum<<<grid, 256, 0, st>>>(bunch, c->lsc_plan, Zr, Zi);
+  k_lsc_wake<<<grid, 256, 0, st>>>(Zr, Zi, c->lsc_plan, W);
+  if (n > 0) k_lsc_kick<<<nred_blocks(c, n), kPT, 0, st>>>(r, ld, n, W, c->lsc_plan);
+  LAUNCHCHK("lsc kernels");
+  return OCSC_OK;
+}
+
+double cic_wscale(int64_t n) {
+  int e = 0;
+  (void)std::frexp((double)n, &e);
+  return std::ldexp(1.0, 62 - e);  // n * wscale < 2^62
+}
+
+// one LSC kick with no host synchronisation (graph-capturable): state,
+// statistics, plan, histogram, profile, wake, kick
+int enqueue_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const LscConst& k, double b0,
+                double b1, hipStream_t st) {
+  const int cap = k.cap;
+  const int ncic = 256;
+  k_lsc_init<<<1, 64, 0, st>>>(c->lsc, n, b0, b1);
+  lsc_stats(c, Rows{r, ld}, q, n, k.step_profile, st);
+  k_lsc_plan<<<1, 64, 0, st>>>(c->lsc, k, c->lsc_plan, c->lsc_err);
+  long long* hist = (long long*)c->lsc_cic.p;
+  long long* tot = hist + (size_t)ncic * cap;
+  k_cic_plan<<<ncic, kPT, (size_t)cap * sizeof(long long), st>>>(Rows{r, ld}, n, c->lsc_plan, cic_wscale(n), hist);
+  k_cic_sum_plan<<<(cap + 255) / 256, 256, 0, st>>>(hist, ncic, c->lsc_plan, tot);
+  LAUNCHCHK("k_cic_plan");
+  return lsc_from_profile(c, r, ld, n, tot, cap, st);
+}
+
+const char* lsc_err_text(int e) {
+  switch (e) {
+    case kLscNoSpread: return "current profile: tau has no spread or is not finite (cannot size the grid)";
+    case kLscTooManyBins: return "current profile: more bins than allowed";
+    case kLscEmptySlice: return "zero-size array to reduction operation maximum which has no identity";
+    case kLscWideSmoothing: return "current profile: smoothing kernel too wide";
+    default: return "LSC error";
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocsc_lsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, double E_GeV, double dz,
+                   int step_profile, double smooth_param, double b0, double b1, double K_max, double fill_factor,
+                   double* info, int ninfo, void* stream) {
+  if (!c || ((!r || !q) && n > 0) || (ninfo > 0 && !info)) return fail(OCSC_EINVAL, "null argument");
+  if (n < 0 || ld < n) return fail(OCSC_EINVAL, "bad n / ld");
+  if (dz < 1e-10) return OCSC_OK;  // sc.py:556-558
+  if (n == 0) return fail(OCSC_EVALUE, "zero-size array to reduction operation minimum which has no identity");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = lsc_buffers(c, 0))) return rc;
+  if ((rc = ensure(c->lane0.part, (size_t)12 * nred_blocks(c, n) * sizeof(double)))) return rc;
+  HIPCHK(hipStreamSynchronize(st));  // the pinned mirror may still be read by an earlier copy
+  LscHost& H = *c->lsc_host;
+  std::memset(&H, 0, sizeof(H));
+  H.s.n = n;
+  H.s.b0 = b0;
+  H.s.b1 = b1;
+  HIPCHK(hipMemcpyAsync(c->lsc, &H.s, sizeof(H.s), hipMemcpyHostToDevice, st));
+  const Rows R{r, ld};
+  lsc_stats(c, R, q, n, step_profile, st);
+  LAUNCHCHK("k_lsc_stat");
+  HIPCHK(hipMemcpyAsync(&H.s, c->lsc, sizeof(H.s), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // the grid size depends on the data: plan on the host
+  const LscConst k = lsc_const(step_profile, smooth_param, E_GeV, dz, K_max, fill_factor, kLscMaxBins);
+  lsc_make_plan(H.s, k, &H.p);
+  if (H.p.err) return fail(OCSC_EVALUE, lsc_err_text(H.p.err));
+  const int nbins = H.p.nbins;
+  if ((rc = lsc_buffers(c, nbins))) return rc;
+  HIPCHK(hipMemcpyAsync(c->lsc_plan, &H.p, sizeof(H.p), hipMemcpyHostToDevice, st));
   long long* tot = nullptr;
-  if ((rc = cic_profile(c->lsc_cic, R, n, a, ds, nbins, st, &tot))) return rc;
-  k_lsc_profile<<<1, 1024, 0, st>>>(tot, nbins, K, ds, sig, h.sumq, v, cur, bunch);
-  LAUNCHCHK("k_lsc_profile");
-  // wake on the grid x_j = j ds + a; W = -wake q = w q (sc.py:587)
-  const double x1 = 1.0 * ds + a;
-  const WakeGrid g = wake_grid(a, x1, nbins);
-  const LscImp imp = lsc_imp(step_profile, gamma, sigma, dz, K_max, fill_factor);
-  if ((rc = lsc_wake_device(c, bunch, nbins, g, imp, h.sumq, Zr, Zi, W, st))) return rc;
-  k_lsc_kick<<<nred_blocks(c, n), kPT, 0, st>>>(r, ld, n, W, nbins, a, ds, pc_ref);
-  LAUNCHCHK("k_lsc_kick");
+  if ((rc = cic_profile(c->lsc_cic, R, n, H.p.a, H.p.ds, nbins, st, &tot))) return rc;
+  if ((rc = lsc_from_profile(c, r, ld, n, tot, nbins, st))) return rc;
   c->lsc_nbins = nbins;
-  c->lsc_a = a;
-  c->lsc_ds = ds;
-  const double v_info[OCSC_LSC_NINFO] = {sigma, h.tau_std, (double)nbins, ds, a, h.sumq, h.cnt, h.tau_mean};
-  for (int k = 0; k < ninfo && k < OCSC_LSC_NINFO; ++k) info[k] = v_info[k];
+  c->lsc_a = H.p.a;
+  c->lsc_ds = H.p.ds;
+  const double v_info[OCSC_LSC_NINFO] = {H.p.sigma, H.s.tau_std, (double)nbins, H.p.ds, H.p.a, H.s.sumq, H.s.cnt,
+                                         H.s.tau_mean};
+  for (int j = 0; j < ninfo && j < OCSC_LSC_NINFO; ++j) info[j] = v_info[j];
   return OCSC_OK;
 }
 
@@ -1768,9 +1812,10 @@ int ocsc_lsc_profile(ocsc_ctx* c, double* s, double* current, double* wake, int6
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipDeviceSynchronize());
   const int nb = c->lsc_nbins;
+  const int64_t bcap = (int64_t)(c->lsc_bins.cap / (5 * sizeof(double)));
   const double* dcur = (const double*)c->lsc_bins.p;
   if (current) HIPCHK(hipMemcpy(current, dcur, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost));
-  if (wake) HIPCHK(hipMemcpy(wake, dcur + (size_t)4 * nb, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost));
+  if (wake) HIPCHK(hipMemcpy(wake, dcur + 4 * bcap, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost));
   if (s)
     for (int j = 0; j < nb; ++j) {
 #pragma clang fp contract(off)
@@ -1786,17 +1831,25 @@ int ocsc_lsc_wake(ocsc_ctx* c, const double* s, const double* bunch, int64_t nb,
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
   int rc;
-  if ((rc = ensure(c->lsc_bins, (size_t)5 * nb * sizeof(double)))) return rc;
-  double* d = (double*)c->lsc_bins.p;
-  double* db = d + nb;
-  double* Zr = db + nb;
-  double* Zi = Zr + nb;
-  double* W = Zi + nb;
+  if ((rc = lsc_buffers(c, nb))) return rc;
   c->lsc_nbins = 0;  // the last kick's profile is overwritten
+  HIPCHK(hipStreamSynchronize(st));
+  LscPlan& P = c->lsc_host->p;
+  P = LscPlan{};
+  P.nbins = (int)nb;
+  P.scale = -1.0;  // wake_lsc returns minus the transform
+  lsc_wake_grid(s[0], s[1], nb, &P);
+  P.imp = lsc_imp(step_profile, gamma, sigma, dz, K_max, fill_factor);
+  HIPCHK(hipMemcpyAsync(c->lsc_plan, &P, sizeof(P), hipMemcpyHostToDevice, st));
+  const int64_t bcap = (int64_t)(c->lsc_bins.cap / (5 * sizeof(double)));
+  double* db = (double*)c->lsc_bins.p + bcap;
+  double* Zr = db + bcap;
+  double* Zi = Zr + bcap;
+  double* W = Zi + bcap;
   HIPCHK(hipMemcpyAsync(db, bunch, (size_t)nb * sizeof(double), hipMemcpyHostToDevice, st));
-  const WakeGrid g = wake_grid(s[0], s[1], nb);
-  const LscImp imp = lsc_imp(step_profile, gamma, sigma, dz, K_max, fill_factor);
-  if ((rc = lsc_wake_device(c, db, (int)nb, g, imp, -1.0, Zr, Zi, W, st))) return rc;
+  k_lsc_spectrum<<<(int)nb, 256, 0, st>>>(db, c->lsc_plan, Zr, Zi);
+  k_lsc_wake<<<(int)nb, 256, 0, st>>>(Zr, Zi, c->lsc_plan, W);
+  LAUNCHCHK("k_lsc_wake");
   HIPCHK(hipMemcpyAsync(wake, W, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return OCSC_OK;

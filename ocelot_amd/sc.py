@@ -254,7 +254,7 @@ def drift(p_array, L, device=None):
 
 
 def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, bounds=None, auto_disp=False,
-                tws_i=None):
+                tws_i=None, lsc=None):
     """nsteps x [drift(L); sc.apply(p_array, dz)] on a device-resident bunch,
     the step loop of ocelot/cpbd/track.py:470-477 for a drift lattice, with
     one step captured in a HIP graph and replayed.
@@ -262,18 +262,40 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
     With ``calc_tws`` the beam moments are taken on the device before the
     first step and after every step, and the list of Twiss (``tws_track`` of
     track(), ``s`` = cumulative length) is returned; ``bounds``, ``auto_disp``
-    and ``tws_i`` as in get_envelope (slice='Imax' is not available here)."""
+    and ``tws_i`` as in get_envelope (slice='Imax' is not available here).
+
+    With ``lsc`` (an LSC instance) every step also applies ``lsc.apply(p_array,
+    dz)`` after the space-charge kick, inside the same graph (the order in
+    which track() applies two processes added SpaceCharge first); ``sc`` may
+    then be None for LSC-only tracking.  Not combined with ``calc_tws``; a line
+    with undulators needs the per-step filling factor, so call apply instead."""
     torch = _torch()
     lib = _lib.load()
     r = p_array.rparticles
     if not _is_cuda_tensor(r):
         raise TypeError("track_drift needs a DeviceParticleArray")
-    if sc.random_mesh:
+    if sc is not None and sc.random_mesh:
         raise ValueError("track_drift does not take random-mesh draws; call apply per step instead")
+    if sc is None and lsc is None:
+        raise ValueError("track_drift needs a SpaceCharge or an LSC process")
     dev = r.device.index
     n = int(r.shape[1])
     with torch.cuda.device(dev):
         stream = torch.cuda.current_stream(dev).cuda_stream
+        if lsc is not None:
+            if calc_tws:
+                raise ValueError("track_drift: calc_tws with lsc is not available; take get_envelope per step")
+            if lsc._is_undul_in_beam_line:
+                raise ValueError("track_drift: an LSC process with undulators in the line needs apply per step")
+            nmesh = sc._nmesh() if sc is not None else _lib.i32x3(np.array([4, 4, 4]))
+            prec = sc._prec() if sc is not None else _lib.OCSC_FP64
+            b0, b1 = (float(v) for v in lsc.bounds)
+            _lib.check(lib.ocsc_track_drift_lsc(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
+                                                p_array.q_array.data_ptr(), n, nmesh, float(p_array.E),
+                                                float(dz) if sc is not None else 0.0, float(L), int(nsteps), prec,
+                                                1 if use_graph else 0, float(dz), int(bool(lsc.step_profile)),
+                                                float(lsc.smooth_param), b0, b1, stream))
+            return None
         if not calc_tws:
             _lib.check(lib.ocsc_track_drift(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
                                             p_array.q_array.data_ptr(), n, sc._nmesh(), float(p_array.E),

@@ -238,3 +238,74 @@ def test_lsc_empty_gaussian_slice_gives_no_kick(lg):
     p = parray(lg["gauss/r"], lg["gauss/q"], 1.0)
     LSC(bounds=[50, 60]).apply(p, 0.1)
     np.testing.assert_array_equal(p.rparticles, lg["gauss/r"] + 0.0)
+
+
+def _resident(r, q, E):
+    from ocelot_amd import DeviceParticleArray
+    return DeviceParticleArray(torch.from_numpy(r.copy()).cuda(), torch.from_numpy(q.copy()).cuda(), E)
+
+
+@pytest.mark.parametrize("step", [False, True])
+def test_tracking_with_lsc_equals_step_by_step(step):
+    """Graph-captured SC + LSC tracking (LSC grid planned on the device) is
+    bit-identical to the eager loop drift, SpaceCharge.apply, LSC.apply (grid
+    planned on the host), with the resident copy unsorted."""
+    from ocelot_amd import LSC, SpaceCharge, drift, track_drift
+    from ocelot_amd import _lib
+    print(f"[lsc tracking] step_profile={step}", flush=True)
+    r, q = _bunch(300_000, "flat", 9)
+    E, L, dz, nsteps = 0.13, 0.05, 0.05, 6
+    sc = SpaceCharge(nmesh_xyz=[31, 31, 31])
+    lsc = LSC(step_profile=step)
+    a = _resident(r, q, E)
+    b = _resident(r, q, E)
+    dev = a.rparticles.device.index
+    old = _lib.get_option(dev, "sort_every")
+    _lib.set_option(dev, "sort_every", 0)
+    try:
+        track_drift(a, sc, L, dz, nsteps, lsc=lsc)
+        for _ in range(nsteps):
+            drift(b, L)
+            sc.apply(b, dz)
+            lsc.apply(b, dz)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a.rparticles.cpu().numpy(), b.rparticles.cpu().numpy())
+        # sorted resident copy: same physics, sums in another order
+        c = _resident(r, q, E)
+        _lib.set_option(dev, "sort_every", 3)
+        track_drift(c, sc, L, dz, nsteps, lsc=lsc)
+        got, ref = c.rparticles.cpu().numpy(), b.rparticles.cpu().numpy()
+        assert rel(got[5] - r[5], ref[5] - r[5]) < 1e-9
+    finally:
+        _lib.set_option(dev, "sort_every", old)
+
+
+def test_lsc_only_tracking_against_oracle():
+    """sc=None: drift + LSC per step, against the oracle's own loop."""
+    from ocelot_amd import LSC, track_drift
+    from oracle import sc_oracle as so
+    print("[lsc only tracking]", flush=True)
+    r, q = _bunch(200_000, "gauss", 4)
+    E, L, dz, nsteps = 0.13, 0.2, 0.2, 5
+    a = _resident(r, q, E)
+    track_drift(a, None, L, dz, nsteps, lsc=LSC())
+    ref = r.copy()
+    for _ in range(nsteps):
+        so.drift(ref, L, E)
+        lo.apply(ref, q, E, dz)
+    got = a.rparticles.cpu().numpy()
+    assert rel(got[5] - r[5], ref[5] - r[5]) < 1e-9
+    assert rel(got[4], ref[4]) < 1e-12
+
+
+def test_tracking_lsc_reports_impossible_grid():
+    """No spread in tau: the device plan refuses the kick, tracking reports
+    it at the end (ValueError), and no LSC kick is applied."""
+    from ocelot_amd import LSC, track_drift
+    r, q = _bunch(10_000, "gauss", 1)
+    r[4] = 0.0
+    r[5] = 0.0   # no energy spread: the drifts keep tau at 0
+    a = _resident(r, q, 0.13)
+    with pytest.raises(ValueError):
+        track_drift(a, None, 0.1, 0.1, 2, lsc=LSC())
+    np.testing.assert_array_equal(a.rparticles.cpu().numpy()[5], r[5])

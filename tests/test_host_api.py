@@ -195,3 +195,20 @@ def test_lsc_two_plane_undulator_is_ignored():
     lsc.start_elem, lsc.end_elem, lsc.s_start = m1, m2, 0.0
     lsc.prepare(line)
     assert not lsc._is_undul_in_beam_line
+
+
+def test_lsc_survives_navigator_copies():
+    """The Navigator deep-copies its process table on add_physics_proc and on
+    every reset_position (ref navi.py:142-189), after prepare(lat); pickling
+    carries processes into parameter scans.  The prepared LSC keeps its K(s)
+    profile and settings through both."""
+    from ocelot_amd import LSC
+    m1, m2, line = _lsc_line()
+    lsc = LSC(step=2, step_profile=True, smooth_param=0.2)
+    lsc.start_elem, lsc.end_elem, lsc.s_start = m1, m2, 0.0
+    lsc.prepare(line)
+    lsc.z0 = 0.75
+    for other in (copy.deepcopy(lsc), pickle.loads(pickle.dumps(lsc))):
+        assert (other.step, other.step_profile, other.smooth_param, other.bounds) == (2, True, 0.2, [-0.4, 0.4])
+        assert other._is_undul_in_beam_line
+        assert other.kick_args(0.1) == lsc.kick_args(0.1)
