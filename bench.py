@@ -112,8 +112,9 @@ def other_configs(torch, dev):
     """Context lines (not the headline): the 64-bunch batched configuration
     (uneven sizes, meshes 31/47/63, 20 kicks, mixed), single drop-in kicks
     of the 2M-particle 63^3 bunch (FP64, device-resident and host numpy), the
-    headline tracking loop with per-step Twiss (track()'s tws_track), and the
-    LSC kick of the same resident bunch."""
+    headline tracking loop with per-step Twiss (track()'s tws_track), the
+    LSC kick of the same resident bunch, and the tracking loop with an LSC kick
+    after every space-charge kick."""
     from ocelot_amd import DeviceParticleArray, ParticleArray, SpaceCharge, apply_batched
     from ocelot_amd.workloads import batch64, make_bunch
     out = {}
@@ -186,6 +187,16 @@ def other_configs(torch, dev):
     out["flattop_10m_lsc_apply"] = {"particles": r.shape[1], "kicks": 20, "ms_per_kick": round(dt * 1e3, 4),
                                     "value": round(r.shape[1] / dt, 1), "unit": "macroparticle-kicks/s",
                                     "profile_bins": lsc.last_info["nbins"], "precision": "fp64"}
+    # the headline loop with an LSC kick after every space-charge kick, one graph per step
+    track_drift(d, sc, dz, dz, 3, lsc=lsc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    track_drift(d, sc, dz, dz, 20, lsc=lsc)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 20
+    out["flattop_10m_127_track_sc_lsc"] = {"particles": r.shape[1], "steps": 20, "ms_per_step": round(dt * 1e3, 4),
+                                           "value": round(r.shape[1] / dt, 1), "unit": "macroparticle-steps/s",
+                                           "precision": "mixed SC + fp64 LSC", "hip_graph": True}
     return out
 
 
