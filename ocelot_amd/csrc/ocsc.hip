@@ -1217,7 +1217,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
       k_compose<<<nb, 256, 0, st>>>(perm_prev, vout, (uint32_t*)c->perm[to].p, n);
     else
       HIPCHK(hipMemcpyAsync(c->perm[to].p, vout, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
-    k_permute_gather<<<nb, 256, 0, st>>>(src, lds, qsrc, vout, (double*)c->rs[to].p, n, (double*)c->qs[to].p, n);
+    k_permute_gather<<<dim3(nb, 7), 256, 0, st>>>(src, lds, qsrc, vout, (double*)c->rs[to].p, n, (double*)c->qs[to].p, n);
     LAUNCHCHK("k_permute_gather");
     return OCSC_OK;
   };
@@ -1380,7 +1380,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     }
   }
   if (sorted) {
-    k_permute_scatter<<<nb, 256, 0, st>>>(rr, n, (const uint32_t*)c->perm[cur].p, r, ld, n);
+    k_permute_scatter<<<dim3(nb, 6), 256, 0, st>>>(rr, n, (const uint32_t*)c->perm[cur].p, r, ld, n);
     LAUNCHCHK("k_permute_scatter");
   }
   int32_t err = 0;

@@ -573,6 +573,10 @@ __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const KickScalars* __
 }
 
 // dst[:, j] = src[:, perm[j]] for the six rows and the charge
+// One row per blockIdx.y (rows 0-5, then q as row 6): workgroups are
+// dispatched in x-major order, so a pass's random reads (gather) or writes
+// (scatter) stay within one row of n doubles, which the memory-side cache
+// holds, instead of spreading over all seven rows at once.
 __global__ void __launch_bounds__(256) k_permute_gather(const double* __restrict__ src, int64_t lds_,
                                                         const double* __restrict__ qs, const uint32_t* __restrict__ perm,
                                                         double* __restrict__ dst, int64_t ldd, double* __restrict__ qd,
@@ -582,21 +586,22 @@ __global__ void __launch_bounds__(256) k_permute_gather(const double* __restrict
   // are fetched into one L2
   const int64_t j = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (j >= n) return;
+  const int row = blockIdx.y;
   const int64_t p = perm[j];
-#pragma unroll
-  for (int i = 0; i < 6; ++i) dst[i * ldd + j] = src[i * lds_ + p];
-  qd[j] = qs[p];
+  if (row < 6)
+    dst[row * ldd + j] = src[row * lds_ + p];
+  else
+    qd[j] = qs[p];
 }
 
-// dst[:, perm[j]] = src[:, j] for the six rows
+// dst[:, perm[j]] = src[:, j] for the six rows (one row per blockIdx.y)
 __global__ void __launch_bounds__(256) k_permute_scatter(const double* __restrict__ src, int64_t lds_,
                                                          const uint32_t* __restrict__ perm, double* __restrict__ dst,
                                                          int64_t ldd, int64_t n) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (j >= n) return;
-  const int64_t p = perm[j];
-#pragma unroll
-  for (int i = 0; i < 6; ++i) dst[i * ldd + p] = src[i * lds_ + j];
+  const int row = blockIdx.y;
+  dst[row * ldd + perm[j]] = src[row * lds_ + j];
 }
 
 // out[j] = a[b[j]]
