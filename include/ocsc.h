@@ -162,7 +162,7 @@ int ocsc_lsc_apply(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* 
  * SpaceCharge.apply(zstep), LSC.apply(lsc_dz), the order track() applies two
  * processes added in that order (ocelot/cpbd/track.py:470-477).  zstep == 0
  * leaves the space-charge kick out; lsc_dz < 1e-10 the LSC kick.  Graph-
- * capturable: the profile grid is planned on the device, up to 8192 bins;
+ * capturable: the profile grid is planned on the device, up to 4096 bins;
  * a step whose profile would need more, or whose bunch has no spread in tau
  * or an empty step-profile slice, gets no LSC kick and the call returns
  * OCSC_EVALUE at the end.  Cannot be combined with the option "kick_timing". */

@@ -396,12 +396,25 @@ __device__ inline double lsc_grid(int j, double a, double ds) {
   return (double)j * ds + a;
 }
 
-// delta += interp(tau, grid, W) 1e-9 / pc_ref, numpy.interp's rules
+// delta += interp(tau, grid, W) 1e-9 / pc_ref, numpy.interp's rules; with
+// DRIFT, then the next step's drift x += L x', y += L y', tau += r56 delta
+// (the same fused multiply-adds as the drift kernel).  A refused plan
+// (nbins == 0) leaves delta alone but still drifts.
+template <bool DRIFT>
 __global__ void __launch_bounds__(kPT) k_lsc_kick(double* __restrict__ r, int64_t ld, int64_t n,
-                                                  const double* __restrict__ W, const LscPlan* __restrict__ p) {
+                                                  const double* __restrict__ W, const LscPlan* __restrict__ p,
+                                                  double L, double r56) {
 #pragma clang fp contract(off)
   const int nbins = p->nbins;
-  if (nbins == 0) return;
+  if (nbins == 0) {
+    if (DRIFT)
+      for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
+        r[i] = __builtin_fma(L, r[ld + i], r[i]);
+        r[2 * ld + i] = __builtin_fma(L, r[3 * ld + i], r[2 * ld + i]);
+        r[4 * ld + i] = __builtin_fma(r56, r[5 * ld + i], r[4 * ld + i]);
+      }
+    return;
+  }
   const double a = p->a, ds = p->ds, pc_ref = p->pc_ref;
   const double x0 = lsc_grid(0, a, ds), xl = lsc_grid(nbins - 1, a, ds);
   for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
@@ -432,7 +445,13 @@ __global__ void __launch_bounds__(kPT) k_lsc_kick(double* __restrict__ r, int64_
         }
       }
     }
-    r[5 * ld + i] = r[5 * ld + i] + dE * 1e-9 / pc_ref;
+    const double d = r[5 * ld + i] + dE * 1e-9 / pc_ref;
+    r[5 * ld + i] = d;
+    if (DRIFT) {
+      r[i] = __builtin_fma(L, r[ld + i], r[i]);
+      r[2 * ld + i] = __builtin_fma(L, r[3 * ld + i], r[2 * ld + i]);
+      r[4 * ld + i] = __builtin_fma(r56, d, tau);
+    }
   }
 }
 
@@ -517,16 +536,6 @@ __global__ void __launch_bounds__(kPT) k_cic_plan(Rows R, int64_t n, const LscPl
   }
   __syncthreads();
   for (int j = threadIdx.x; j < nbins; j += kPT) hist[(int64_t)blockIdx.x * nbins + j] = h[j];
-}
-
-__global__ void __launch_bounds__(256) k_cic_sum_plan(const long long* __restrict__ hist, int nblk,
-                                                      const LscPlan* __restrict__ p, long long* __restrict__ out) {
-  const int nbins = p->nbins;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= nbins) return;
-  long long acc = 0;
-  for (int b = 0; b < nblk; ++b) acc += hist[(int64_t)b * nbins + j];
-  out[j] = acc;
 }
 
 }  // namespace ocsc

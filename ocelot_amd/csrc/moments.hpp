@@ -248,13 +248,30 @@ __global__ void __launch_bounds__(kPT) k_cic(Rows R, int64_t n, double a, double
   for (int j = threadIdx.x; j < nb; j += kPT) hist[(int64_t)blockIdx.x * nb + j] = h[j];
 }
 
-__global__ void __launch_bounds__(256) k_cic_sum(const long long* __restrict__ hist, int nblk, int nbins,
-                                                 long long* __restrict__ out) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= nbins) return;
+// Totals of the per-workgroup histograms: a workgroup takes 64 bins, its 16
+// waves add every 16th histogram, then wave 0 adds the 16 wave sums.  The
+// sums are integers, so the result is exact in any order.  nbins_dev, when
+// given, overrides nbins (a grid planned on the device); workgroups beyond
+// the bins return at once.
+constexpr int kCicBlocks = 1024;  // workgroups of the histogram pass
+
+__global__ void __launch_bounds__(1024) k_cic_sum(const long long* __restrict__ hist, int nblk, int nbins,
+                                                  const int* __restrict__ nbins_dev, long long* __restrict__ out) {
+  __shared__ long long part[16][64];
+  const int nb = nbins_dev ? *nbins_dev : nbins;
+  if ((int)blockIdx.x * 64 >= nb) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
   long long acc = 0;
-  for (int b = 0; b < nblk; ++b) acc += hist[(int64_t)b * nbins + j];
-  out[j] = acc;
+  if (j < nb)
+    for (int b = w; b < nblk; b += 16) acc += hist[(int64_t)b * nb + j];
+  part[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && j < nb) {
+    long long t = 0;
+    for (int k = 0; k < 16; ++k) t += part[k][lane];
+    out[j] = t;
+  }
 }
 
 }  // namespace ocsc

@@ -920,7 +920,7 @@ int cic_profile(Buf& buf, const Rows& R, int64_t n, double a, double ds, int nbi
   int e = 0;
   (void)std::frexp((double)n, &e);
   const double wscale = std::ldexp(1.0, 62 - e);  // n * wscale < 2^62
-  const int ncic = 256;
+  const int ncic = kCicBlocks;
   const int win = std::min(nbins, kCicLds);
   int rc;
   if ((rc = ensure(buf, ((size_t)ncic * win + nbins) * sizeof(long long)))) return rc;
@@ -930,7 +930,7 @@ int cic_profile(Buf& buf, const Rows& R, int64_t n, double a, double ds, int nbi
   for (int j0 = 0; j0 < nbins; j0 += win) {
     const int nb = std::min(win, nbins - j0);
     k_cic<<<ncic, kPT, (size_t)nb * sizeof(long long), st>>>(R, n, a, ds, nbins, j0, nb, wscale, hist);
-    k_cic_sum<<<(nb + 255) / 256, 256, 0, st>>>(hist, ncic, nb, *tot + j0);
+    k_cic_sum<<<(nb + 63) / 64, 1024, 0, st>>>(hist, ncic, nb, nullptr, *tot + j0);
   }
   LAUNCHCHK("k_cic");
   return OCSC_OK;
@@ -1137,9 +1137,9 @@ struct LscReq {
 int lsc_buffers(ocsc_ctx* c, int64_t bins);
 LscConst lsc_const(int step_profile, double smooth, double E_GeV, double dz, double K_max, double fill_factor,
                    int cap);
-constexpr int kLscGraphBins = kCicLds;  // kicks inside tracking graphs (device-planned grid)
+constexpr int kLscGraphBins = 4096;  // kicks inside tracking graphs (device-planned grid; 32 KB of LDS)
 int enqueue_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const LscConst& k, double b0,
-                double b1, hipStream_t st);
+                double b1, hipStream_t st, const DriftArgs* dr = nullptr);
 const char* lsc_err_text(int e);
 
 int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
@@ -1168,7 +1168,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   if (lq) {
     if (c->kick_timing) return fail(OCSC_EINVAL, "kick timing and LSC kicks cannot be combined");
     if ((rc = lsc_buffers(c, lq->k.cap))) return rc;
-    if ((rc = ensure(c->lsc_cic, ((size_t)256 * lq->k.cap + lq->k.cap) * sizeof(long long)))) return rc;
+    if ((rc = ensure(c->lsc_cic, ((size_t)kCicBlocks * lq->k.cap + lq->k.cap) * sizeof(long long)))) return rc;
     if ((rc = ensure(L.part, (size_t)12 * nred_blocks(c, n) * sizeof(double)))) return rc;
     k_counter_reset<<<1, 64, 0, st>>>(c->lsc_err);
   }
@@ -1288,9 +1288,12 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   auto step_body = [&](int step, const StepOpts& o, hipStream_t s2) -> int {
     int rc3;
     if ((rc3 = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, s2, o))) return rc3;
-    if (lq && (rc3 = enqueue_lsc(c, rr, lr, qq, n, lq->k, lq->b0, lq->b1, s2))) return rc3;
+    // with an LSC kick and no moments, the next drift is fused into the LSC kick
+    const DriftArgs next{Ld, r56};
+    const bool fuse = lq && !tw && step < nsteps - 1;
+    if (lq && (rc3 = enqueue_lsc(c, rr, lr, qq, n, lq->k, lq->b0, lq->b1, s2, fuse ? &next : nullptr))) return rc3;
     if (tw && (rc3 = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, s2))) return rc3;
-    if ((tw || lq) && step < nsteps - 1) k_drift<<<nb, 256, 0, s2>>>(rr, lr, n, Ld, r56);
+    if ((tw || lq) && !fuse && step < nsteps - 1) k_drift<<<nb, 256, 0, s2>>>(rr, lr, n, Ld, r56);
     return OCSC_OK;
   };
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -1709,7 +1712,8 @@ void lsc_stats(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, int step_
 
 // profile, wake and kick from the histogram totals and the plan on the
 // device; grids of `grid` workgroups (the plan's bins, or the cap)
-int lsc_from_profile(ocsc_ctx* c, double* r, int64_t ld, int64_t n, const long long* tot, int grid, hipStream_t st) {
+int lsc_from_profile(ocsc_ctx* c, double* r, int64_t ld, int64_t n, const long long* tot, int grid, hipStream_t st,
+                     const DriftArgs* dr = nullptr) {
   double* cur = (double*)c->lsc_bins.p;
   const int64_t cap = (int64_t)(c->lsc_bins.cap / (5 * sizeof(double)));
   double* bunch = cur + cap;
@@ -1719,7 +1723,10 @@ int lsc_from_profile(ocsc_ctx* c, double* r, int64_t ld, int64_t n, const long l
   k_lsc_profile<<<1, 1024, 0, st>>>(tot, c->lsc_plan, cur, bunch);
   k_lsc_spectrum<<<grid, 256, 0, st>>>(bunch, c->lsc_plan, Zr, Zi);
   k_lsc_wake<<<grid, 256, 0, st>>>(Zr, Zi, c->lsc_plan, W);
-  if (n > 0) k_lsc_kick<<<nred_blocks(c, n), kPT, 0, st>>>(r, ld, n, W, c->lsc_plan);
+  if (n > 0 && dr)
+    k_lsc_kick<true><<<nred_blocks(c, n), kPT, 0, st>>>(r, ld, n, W, c->lsc_plan, dr->L, dr->r56);
+  else if (n > 0)
+    k_lsc_kick<false><<<nred_blocks(c, n), kPT, 0, st>>>(r, ld, n, W, c->lsc_plan, 0.0, 0.0);
   LAUNCHCHK("lsc kernels");
   return OCSC_OK;
 }
@@ -1733,18 +1740,18 @@ double cic_wscale(int64_t n) {
 // one LSC kick with no host synchronisation (graph-capturable): state,
 // statistics, plan, histogram, profile, wake, kick
 int enqueue_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const LscConst& k, double b0,
-                double b1, hipStream_t st) {
+                double b1, hipStream_t st, const DriftArgs* dr) {
   const int cap = k.cap;
-  const int ncic = 256;
+  const int ncic = kCicBlocks;
   k_lsc_init<<<1, 64, 0, st>>>(c->lsc, n, b0, b1);
   lsc_stats(c, Rows{r, ld}, q, n, k.step_profile, st);
   k_lsc_plan<<<1, 64, 0, st>>>(c->lsc, k, c->lsc_plan, c->lsc_err);
   long long* hist = (long long*)c->lsc_cic.p;
   long long* tot = hist + (size_t)ncic * cap;
   k_cic_plan<<<ncic, kPT, (size_t)cap * sizeof(long long), st>>>(Rows{r, ld}, n, c->lsc_plan, cic_wscale(n), hist);
-  k_cic_sum_plan<<<(cap + 255) / 256, 256, 0, st>>>(hist, ncic, c->lsc_plan, tot);
+  k_cic_sum<<<(cap + 63) / 64, 1024, 0, st>>>(hist, ncic, cap, &c->lsc_plan->nbins, tot);
   LAUNCHCHK("k_cic_plan");
-  return lsc_from_profile(c, r, ld, n, tot, cap, st);
+  return lsc_from_profile(c, r, ld, n, tot, cap, st, dr);
 }
 
 const char* lsc_err_text(int e) {
