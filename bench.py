@@ -53,7 +53,33 @@ def parse():
                     help="charge deposit into one grid (0), per-XCD copies (1), or let the autotuner pick "
                          "after checking both give identical kicks (auto)")
     ap.add_argument("--tune-steps", type=int, default=30, help="timed steps per autotuner candidate")
+    ap.add_argument("--lsc-only", action="store_true",
+                    help="only time tracking with an LSC kick after every space-charge kick (--warmup, then "
+                         "--steps graph-captured steps) and two one-shot LSC kicks; one JSON line")
     return ap.parse_args()
+
+
+def lsc_only(a, torch, host, nmesh, dz, dev):
+    """Context run for profiling the LSC kernels (not the headline)."""
+    from ocelot_amd import LSC, DeviceParticleArray, SpaceCharge, track_drift
+    d = DeviceParticleArray.from_host(host, dev)
+    sc, lsc = SpaceCharge(nmesh_xyz=list(nmesh), precision="mixed"), LSC()
+    track_drift(d, sc, dz, dz, a.warmup, lsc=lsc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    track_drift(d, sc, dz, dz, a.steps, lsc=lsc)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(a.steps, 1)
+    t1 = time.perf_counter()
+    lsc.apply(d, dz)
+    lsc.apply(d, dz)
+    torch.cuda.synchronize()
+    da = (time.perf_counter() - t1) / 2
+    n = d.n
+    print(json.dumps({"metric": "SC + LSC tracking step", "workload": a.workload, "particles": n,
+                      "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 4),
+                      "value": round(n / dt, 1), "unit": "macroparticle-steps/s",
+                      "lsc_apply_ms": round(da * 1e3, 4), "profile_bins": lsc.last_info["nbins"]}))
 
 
 def algorithmic_bytes(n, nmesh, prec):
@@ -224,6 +250,8 @@ def main():
     host.rparticles, host.q_array, host.E = r, q, E
     dev = torch.cuda.current_device()
     L = dz
+    if a.lsc_only:
+        return lsc_only(a, torch, host, nmesh, dz, dev)
 
     # the library's autotuner picks the speed-only options (re-sort interval,
     # deposit mode, Green's-function overlap) for this bunch and mesh; options
