@@ -150,6 +150,7 @@ struct Lane {
   KickScalars* sc_host = nullptr;  // pinned mirror
   Buf part;                        // per-block partial sums
   Buf grid, A, B, phi, G;
+  Buf E64;                         // FP64 form: the three difference fields of phi
   hipStream_t stream = nullptr;    // extra lanes only
   hipEvent_t done = nullptr;       // extra lanes only
 };
@@ -255,6 +256,8 @@ int ensure_mesh(ocsc_ctx* c, Lane& L, const Geometry& g, int nword) {
   if ((rc = ensure(L.B, b))) return rc;
   if ((rc = ensure(L.phi, (size_t)g.ncell * std::max(sizeof(double), 3 * sizeof(float))))) return rc;
   if ((rc = ensure(L.G, (size_t)d.hx * d.hy * d.hz * sizeof(double)))) return rc;
+  if constexpr (sizeof(F) == 8)
+    if ((rc = ensure(L.E64, (size_t)3 * g.ncell * sizeof(double)))) return rc;
   return OCSC_OK;
 }
 
@@ -480,7 +483,9 @@ struct StepOpts {
 
 template <typename F, bool DRIFT, bool SUMP>
 void launch_kick(ocsc_ctx* c, Lane& L, int npart, double* r, int64_t ld, const StepOpts& o, hipStream_t st) {
-  k_kick<F, DRIFT, SUMP><<<npart, kPT, 0, st>>>(r, ld, (const F*)L.phi.p, L.sc, o.dr, (double*)L.part.p);
+  // mixed form: the three FP32 field grids; FP64 form: the k_diff64 fields
+  const F* field = sizeof(F) == 8 ? (const F*)L.E64.p : (const F*)L.phi.p;
+  k_kick<F, DRIFT, SUMP><<<npart, kPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p);
 }
 
 template <typename F, int NWORD>
@@ -553,6 +558,8 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
     if ((rc = launch_convolution<F, NWORD == 2 ? kChargeFix2 : kChargeFix1>(c, L, g, L.grid.p, (F*)L.phi.p, st,
                                                                               prof ? ev + 4 : nullptr)))
       return rc;
+    k_diff64<<<(int)((g.ncell + 255) / 256), 256, 0, st>>>((const double*)L.phi.p, L.sc, (double*)L.E64.p);
+    LAUNCHCHK("k_diff64");
   }
   mark(9);
   if (o.kev0 && !o.kev_external) HIPCHK(hipEventRecord(o.kev0, st));
@@ -650,7 +657,7 @@ int init_lane(Lane& L, bool own_stream) {
 }
 
 void free_lane(Lane& L) {
-  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G})
+  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &L.E64})
     if (b->p) (void)hipFree(b->p);
   if (L.sc) (void)hipFree(L.sc);
   if (L.sc_host) (void)hipHostFree(L.sc_host);
@@ -722,7 +729,7 @@ int ocsc_destroy(ocsc_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
-                 &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
+                 &L.E64, &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
                  &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->csr_bins})
     if (b->p) (void)hipFree(b->p);
   if (c->lsc) (void)hipFree(c->lsc);

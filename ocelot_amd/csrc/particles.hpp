@@ -795,6 +795,72 @@ __device__ __forceinline__ void gather_field(const F* __restrict__ phi, const Ki
   E[2] = (double)gather_diff<F, 2>(phi, nx, ny, nz, cz, h[2]);
 }
 
+// FP64 form: the three staggered difference fields E_AX = (phi[i] -
+// phi[i+1]) / h_AX (zero on each component's last plane) formed once per kick
+// from the potential, the arrays the reference interpolates (sc.py:198-204);
+// the kick then reads 8 corner values per component instead of 16 potential
+// values and divides nothing.  Stored one after the other, C order.
+__global__ void __launch_bounds__(256) k_diff64(const double* __restrict__ phi, const KickScalars* __restrict__ scp,
+                                                double* __restrict__ E) {
+  const int nx = scp->nmesh[0], ny = scp->nmesh[1], nz = scp->nmesh[2];
+  const int64_t nc = (int64_t)nx * ny * nz;
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= nc) return;
+  const int64_t sx = (int64_t)ny * nz;
+  const int i = (int)(o / sx), j = (int)((o / nz) % ny), k = (int)(o % nz);
+  const double p0 = phi[o];
+  E[o] = i < nx - 1 ? (p0 - phi[o + sx]) / scp->steps[0] : 0.0;
+  E[nc + o] = j < ny - 1 ? (p0 - phi[o + nz]) / scp->steps[1] : 0.0;
+  E[2 * nc + o] = k < nz - 1 ? (p0 - phi[o + 1]) / scp->steps[2] : 0.0;
+}
+
+// trilinear interpolation of a k_diff64 component, in gather_diff's order of
+// corners and operations (the same result bit for bit)
+template <int AX>
+__device__ __forceinline__ double gather_grid64(const double* __restrict__ E, int nx, int ny, int nz, const double* c) {
+  const int n[3] = {nx, ny, nz};
+  if (!(c[0] >= 0 && c[0] <= nx - 1 && c[1] >= 0 && c[1] <= ny - 1 && c[2] >= 0 && c[2] <= nz - 1)) return 0.0;
+  int i0[3];
+  double t[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int f = (int)__builtin_floor(c[k]);
+    f = f > n[k] - 2 ? n[k] - 2 : f;
+    i0[k] = f;
+    t[k] = c[k] - (double)f;
+  }
+  const int64_t sx = (int64_t)ny * nz, sy = nz;
+  double acc = 0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      // the two z-neighbours of a corner pair in one 16-byte load
+      double2 pr;
+      __builtin_memcpy(&pr, E + (i0[0] + a) * sx + (i0[1] + b) * sy + i0[2], sizeof(pr));
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const double ev = e ? pr.y : pr.x;
+        const double w = (a ? t[0] : 1 - t[0]) * (b ? t[1] : 1 - t[1]) * (e ? t[2] : 1 - t[2]);
+        acc += w * ev;
+      }
+    }
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void gather_field64(const double* __restrict__ E3, const KickScalars& s, const double* X,
+                                               double* E) {
+  const int nx = s.nmesh[0], ny = s.nmesh[1], nz = s.nmesh[2];
+  const int64_t nc = (int64_t)nx * ny * nz;
+  const double cx[3] = {X[0], X[1] + 0.5, X[2] + 0.5};
+  const double cy[3] = {X[0] + 0.5, X[1], X[2] + 0.5};
+  const double cz[3] = {X[0] + 0.5, X[1] + 0.5, X[2]};
+  E[0] = gather_grid64<0>(E3, nx, ny, nz, cx) * s.gamma0;
+  E[1] = gather_grid64<1>(E3 + nc, nx, ny, nz, cy) * s.gamma0;
+  E[2] = gather_grid64<2>(E3 + 2 * nc, nx, ny, nz, cz);
+}
+
 // pass D: gather, kick in the rotated frame, rotate back, write all six rows
 // in place (sc.py:241-251, coord_transform.py:16-54).
 // Tracking fusions: DRIFT applies the next step's drift to the written rows
@@ -842,7 +908,7 @@ __global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld
       E[1] = (double)g[1] * s.gamma0;
       E[2] = (double)g[2];
     } else {
-      gather_field<F>(phi, s, X, E);
+      gather_field64((const double*)phi, s, X, E);  // phi: the k_diff64 fields
     }
     pr[0] = pr[0] + s.kperp * E[0];
     pr[1] = pr[1] + s.kperp * E[1];
