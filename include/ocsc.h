@@ -157,6 +157,15 @@ int ocsc_lsc_apply(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* 
                    double dz, int step_profile, double smooth_param, double b0, double b1, double K_max,
                    double fill_factor, double* info, int ninfo, void* stream);
 
+/* ocsc_track_drift with SpaceCharge(random_mesh=True) (ocelot/cpbd/sc.py:
+ * 174-175, 184-185): draws (host) holds nsteps pairs (mesh_scale, mesh_shift),
+ * the two global numpy draws of each kick in the reference's order; kick k
+ * takes pair k through a device-side counter, so the captured step graphs
+ * replay with a new pair each time. */
+int ocsc_track_drift_random_mesh(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,
+                                 const int32_t nmesh[3], double E_GeV, double zstep, double L, int nsteps,
+                                 int precision, int use_graph, const double* draws, void* stream);
+
 /* ocsc_track_drift with an LSC kick (as ocsc_lsc_apply, step length lsc_dz,
  * no undulator factor) after every space-charge kick: per step drift(L),
  * SpaceCharge.apply(zstep), LSC.apply(lsc_dz), the order track() applies two

@@ -54,6 +54,8 @@ SIGNATURES = {
     "ocsc_lsc_profile": (_c_int, [_vp, _dp, _dp, _dp, _i64, ctypes.POINTER(_i64)]),
     "ocsc_lsc_wake": (_c_int, [_vp, _dp, _dp, _i64, _dbl, _dbl, _dbl, _c_int, _dbl, _dbl, _dp, _vp]),
     "ocsc_lsc_impedance": (_c_int, [_vp, _dp, _i64, _dbl, _dbl, _dbl, _c_int, _dp, _vp]),
+    "ocsc_track_drift_random_mesh": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int,
+                                              _c_int, _dp, _vp]),
     "ocsc_track_drift_lsc": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int, _c_int,
                                       _dbl, _c_int, _dbl, _dbl, _dbl, _vp]),
     "ocsc_csr_convolution": (_c_int, [_vp, _dp, _i64, _dp, _i64, _dp, _vp]),

@@ -79,6 +79,11 @@ struct KickInputs {
   int32_t random_mesh, mode;  // mode 0: kick; 1: el_field (gamma0 given); 2: steps given
   int32_t nmesh[3];
   int64_t n;
+  // tracking with random_mesh: the (scale, shift) pair of kick k is draws[2k],
+  // draws[2k + 1], k read from and advanced in draw_ctr (device), k < ndraws
+  const double* draws;
+  int* draw_ctr;
+  int32_t ndraws;
 };
 
 __global__ void k_setup(KickScalars* __restrict__ s, KickInputs in) {
@@ -94,6 +99,14 @@ __global__ void k_setup(KickScalars* __restrict__ s, KickInputs in) {
   s->cdT = in.cdT;
   s->mesh_scale = in.mesh_scale;
   s->mesh_shift = in.mesh_shift;
+  if (in.draws) {
+    const int k = *in.draw_ctr;
+    if (k < in.ndraws) {
+      s->mesh_scale = in.draws[2 * k];
+      s->mesh_shift = in.draws[2 * k + 1];
+    }
+    *in.draw_ctr = k + 1;
+  }
   s->random_mesh = in.random_mesh;
   for (int k = 0; k < 3; ++k) s->nmesh[k] = in.nmesh[k];
   s->n = in.n;
@@ -179,6 +192,7 @@ struct ocsc_ctx {
   int sort_every = 10;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
   // tracking workspace: two cell-ordered copies of the bunch, sort buffers
   Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp;
+  Buf draws;  // random-mesh draws of a tracking call and their counter
   // beam moments: device state and its pinned host mirror
   MomState* mom = nullptr;
   MomState* mom_host = nullptr;
@@ -729,7 +743,7 @@ int ocsc_destroy(ocsc_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
-                 &L.E64, &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
+                 &L.E64, &c->draws, &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
                  &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->csr_bins})
     if (b->p) (void)hipFree(b->p);
   if (c->lsc) (void)hipFree(c->lsc);
@@ -1151,7 +1165,7 @@ const char* lsc_err_text(int e);
 
 int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
                double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, void* stream,
-               const TwsReq* tw, double* tws_out, const LscReq* lq = nullptr) {
+               const TwsReq* tw, double* tws_out, const LscReq* lq = nullptr, const double* draws = nullptr) {
   int rc;
   if ((rc = check_common(c, r, q, n, ld))) return rc;
   if (!nmesh || nsteps < 0) return fail(OCSC_EINVAL, "bad nmesh / nsteps");
@@ -1164,7 +1178,18 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   if ((rc = make_geometry(nmesh, g))) return rc;
   if ((rc = ensure_prec(c, L, g, prec))) return rc;
   hipStream_t st = (hipStream_t)stream;
-  KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, 0, 1.0, 0.0);
+  KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, draws ? 1 : 0, 1.0, 0.0);
+  const KickInputs in_keys = in;  // the first sort keys take no draw
+  if (draws && nsteps > 0 && zstep != 0) {
+    if ((rc = ensure(c->draws, (size_t)2 * nsteps * sizeof(double) + 256))) return rc;
+    HIPCHK(hipMemcpyAsync(c->draws.p, draws, (size_t)2 * nsteps * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // the caller's (pageable) array may go away after the call
+    int* ctr = (int*)((char*)c->draws.p + (size_t)2 * nsteps * sizeof(double));
+    k_counter_reset<<<1, 64, 0, st>>>(ctr);
+    in.draws = (const double*)c->draws.p;
+    in.draw_ctr = ctr;
+    in.ndraws = nsteps;
+  }
   const double gamma = E_GeV / kMeGeV;
   const double igamma2 = 1. / (gamma * gamma);
   const double beta = std::sqrt(1. - igamma2);
@@ -1236,7 +1261,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   if (sorted) {
     // kick scalars of the current bunch for the first sort keys
     const int nblk = nred_blocks(c, n);
-    k_setup<<<1, 64, 0, st>>>(L.sc, in);
+    k_setup<<<1, 64, 0, st>>>(L.sc, in_keys);
     k_passA<<<nblk, kPT, 0, st>>>(Rows{r, ld}, L.sc, (double*)L.part.p);
     k_reduceA<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk);
     k_passB<false><<<nblk, kPT, 0, st>>>(Rows{r, ld}, q, L.sc, (double*)L.part.p);
@@ -1437,6 +1462,14 @@ int ocsc_track_drift_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, in
   lq.b0 = b0;
   lq.b1 = b1;
   return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph, stream, nullptr, nullptr, &lq);
+}
+
+int ocsc_track_drift_random_mesh(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n,
+                                 const int32_t nmesh[3], double E_GeV, double zstep, double Ld, int nsteps, int prec,
+                                 int use_graph, const double* draws, void* stream) {
+  if (!draws && nsteps > 0 && zstep != 0) return fail(OCSC_EINVAL, "null draws");
+  return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph, stream, nullptr, nullptr,
+                    nullptr, draws);
 }
 
 int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],

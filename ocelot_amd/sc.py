@@ -274,8 +274,8 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
     r = p_array.rparticles
     if not _is_cuda_tensor(r):
         raise TypeError("track_drift needs a DeviceParticleArray")
-    if sc is not None and sc.random_mesh:
-        raise ValueError("track_drift does not take random-mesh draws; call apply per step instead")
+    if sc is not None and sc.random_mesh and (lsc is not None or calc_tws):
+        raise ValueError("track_drift: random_mesh is not combined with lsc or calc_tws; call apply per step")
     if sc is None and lsc is None:
         raise ValueError("track_drift needs a SpaceCharge or an LSC process")
     dev = r.device.index
@@ -295,6 +295,17 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
                                                 float(dz) if sc is not None else 0.0, float(L), int(nsteps), prec,
                                                 1 if use_graph else 0, float(dz), int(bool(lsc.step_profile)),
                                                 float(lsc.smooth_param), b0, b1, stream))
+            return None
+        if sc.random_mesh:
+            # the reference draws U(1, 1.1) then U(-0.5, 0.5) from the global
+            # numpy RNG in every kick (sc.py:174-175, 184-185); no kick, no draw
+            k = int(nsteps) if float(dz) != 0 else 0
+            draws = np.array([sc._draws()[1:] for _ in range(k)], dtype=np.float64).reshape(-1)
+            buf = (ctypes.c_double * max(draws.size, 1))(*draws)
+            _lib.check(lib.ocsc_track_drift_random_mesh(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
+                                                        p_array.q_array.data_ptr(), n, sc._nmesh(),
+                                                        float(p_array.E), float(dz), float(L), int(nsteps),
+                                                        sc._prec(), 1 if use_graph else 0, buf, stream))
             return None
         if not calc_tws:
             _lib.check(lib.ocsc_track_drift(_lib.context(dev).handle, r.data_ptr(), r.stride(0),

@@ -343,6 +343,38 @@ def test_track_drift_graph_eager_and_oracle():
     assert err <= 1e-11 and kick <= 1e-8
 
 
+def test_track_drift_random_mesh_against_oracle_and_rng_order():
+    """random_mesh inside graph-captured tracking: every kick takes the two
+    global numpy draws the reference takes (scale, then shift), passed to the
+    device up front; the result follows the oracle's own loop with the same
+    draws, graph and eager agree bit for bit, and the global RNG ends where
+    per-step apply calls leave it."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=40_000, seed=6)
+    nsteps = 4
+    sc = SpaceCharge(nmesh_xyz=[15, 15, 31], random_mesh=True)
+    np.random.seed(21)
+    draws = [(np.random.uniform(low=1, high=1.1), np.random.uniform(low=-0.5, high=0.5)) for _ in range(nsteps)]
+    after = np.random.uniform()
+    ro = r.copy()
+    for k in range(nsteps):
+        orc.drift(ro, 0.1, E)
+        orc.apply(ro, q, E, dz, (15, 15, 31), mesh_scale=draws[k][0], mesh_shift=draws[k][1])
+    outs = []
+    for graph in (False, True):
+        np.random.seed(21)
+        d = DeviceParticleArray.from_host(parray(r, q, E))
+        track_drift(d, sc, 0.1, dz, nsteps, use_graph=graph)
+        assert np.random.uniform() == after
+        outs.append(d.rparticles.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    err = max(rel_max(outs[0][i], ro[i]) for i in range(6))
+    kick = max(kick_err(outs[0], r, ro, i) for i in (1, 3, 5))
+    print(f"random-mesh track {nsteps} steps: row err {err:.3e} kick err {kick:.3e}")
+    assert err <= 1e-11 and kick <= 1e-8
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_per_xcd_deposit_is_bit_identical(prec):
     """Per-XCD grid copies with L2-local atomics sum the same integers as the
