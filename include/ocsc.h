@@ -181,7 +181,8 @@ int ocsc_track_drift_lsc(ocsc_ctx* ctx, double* rparticles, int64_t ld, const do
                          void* stream);
 
 /* Profile grid, current (A) and wake W (V, the kick before scaling) of the
- * last ocsc_lsc_apply on this context (the quantities B[:, 0], B[:, 1] and W
+ * last ocsc_lsc_apply on this context (OCSC_EINVAL after an LSC tracking call
+ * or ocsc_lsc_wake, which reuse the arrays) (the quantities B[:, 0], B[:, 1] and W
  * of sc.py:582-587), copied to host arrays of at least *nbins entries (any
  * may be NULL).  *nbins is set in every case; OCSC_EINVAL if cap < *nbins.
  * Synchronises the device. */

@@ -1203,6 +1203,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     if ((rc = ensure(c->lsc_cic, ((size_t)kCicBlocks * lq->k.cap + lq->k.cap) * sizeof(long long)))) return rc;
     if ((rc = ensure(L.part, (size_t)12 * nred_blocks(c, n) * sizeof(double)))) return rc;
     k_counter_reset<<<1, 64, 0, st>>>(c->lsc_err);
+    c->lsc_nbins = 0;  // the per-bin arrays now belong to the tracking kicks
   }
   if (tw) {
     if (c->kick_timing) return fail(OCSC_EINVAL, "kick timing and per-step moments cannot be combined");
