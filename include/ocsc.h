@@ -166,19 +166,22 @@ int ocsc_track_drift_random_mesh(ocsc_ctx* ctx, double* rparticles, int64_t ld, 
                                  const int32_t nmesh[3], double E_GeV, double zstep, double L, int nsteps,
                                  int precision, int use_graph, const double* draws, void* stream);
 
-/* ocsc_track_drift with an LSC kick (as ocsc_lsc_apply, step length lsc_dz,
- * no undulator factor) after every space-charge kick: per step drift(L),
+/* ocsc_track_drift with an LSC kick (as ocsc_lsc_apply, step length lsc_dz)
+ * after every space-charge kick: per step drift(L),
  * SpaceCharge.apply(zstep), LSC.apply(lsc_dz), the order track() applies two
  * processes added in that order (ocelot/cpbd/track.py:470-477).  zstep == 0
  * leaves the space-charge kick out; lsc_dz < 1e-10 the LSC kick.  Graph-
  * capturable: the profile grid is planned on the device, up to 4096 bins;
  * a step whose profile would need more, or whose bunch has no spread in tau
  * or an empty step-profile slice, gets no LSC kick and the call returns
- * OCSC_EVALUE at the end.  Cannot be combined with the option "kick_timing". */
+ * OCSC_EVALUE at the end.  und_factors (host, may be NULL: no undulator)
+ * holds the impedance factor 1 + K_max^2 fill_factor/2 of each step, as
+ * LSC.apply derives it from the line at that step (sc.py:559-564, 436-441).
+ * Cannot be combined with the option "kick_timing". */
 int ocsc_track_drift_lsc(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,
                          const int32_t nmesh[3], double E_GeV, double zstep, double L, int nsteps, int precision,
                          int use_graph, double lsc_dz, int step_profile, double smooth_param, double b0, double b1,
-                         void* stream);
+                         const double* und_factors, void* stream);
 
 /* Profile grid, current (A) and wake W (V, the kick before scaling) of the
  * last ocsc_lsc_apply on this context (OCSC_EINVAL after an LSC tracking call

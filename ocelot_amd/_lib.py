@@ -57,7 +57,7 @@ SIGNATURES = {
     "ocsc_track_drift_random_mesh": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int,
                                               _c_int, _dp, _vp]),
     "ocsc_track_drift_lsc": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int, _c_int,
-                                      _dbl, _c_int, _dbl, _dbl, _dbl, _vp]),
+                                      _dbl, _c_int, _dbl, _dbl, _dbl, _dp, _vp]),
     "ocsc_csr_convolution": (_c_int, [_vp, _dp, _i64, _dp, _i64, _dp, _vp]),
     "ocsc_csr_kick": (_c_int, [_vp, _vp, _i64, _i64, _dbl, _dp, _i64, _dp, _i64, _dbl, _dbl, _dbl, _dbl, _dp, _vp]),
 }

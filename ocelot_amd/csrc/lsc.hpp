@@ -214,6 +214,12 @@ struct LscConst {
   int step_profile;
   double smooth, gamma, v, pc_ref, dz, K_max, fill_factor;
   int cap;  // most profile bins allowed
+  // tracking through undulators: the impedance factor 1 + K_max^2 fill/2 of
+  // kick k is factors[k] (formed on the host as lsc_imp forms it), k read
+  // from and advanced in fctr (device), k < nfactors
+  const double* factors;
+  int* fctr;
+  int nfactors;
 };
 
 enum { kLscOk = 0, kLscNoSpread = 1, kLscTooManyBins = 2, kLscEmptySlice = 3, kLscWideSmoothing = 4 };
@@ -513,6 +519,11 @@ __global__ void k_lsc_plan(const LscState* __restrict__ s, LscConst k, LscPlan* 
   if (threadIdx.x != 0) return;
   LscPlan q;
   lsc_make_plan(*s, k, &q);
+  if (k.factors) {
+    const int i = *k.fctr;
+    if (i < k.nfactors) q.imp.factor = k.factors[i];
+    *k.fctr = i + 1;
+  }
   *p = q;
   if (q.err && *err == 0) *err = q.err;
 }

@@ -220,6 +220,7 @@ struct ocsc_ctx {
   int* lsc_err = nullptr;  // first plan error inside tracking graphs
   LscHost* lsc_host = nullptr;
   Buf lsc_bins, lsc_cic;
+  Buf lsc_fac;  // per-step undulator factors of an LSC tracking call and their counter
   Buf csr_bins;  // CSR: line charge, kernel, convolution
   int lsc_nbins = 0;
   double lsc_a = 0, lsc_ds = 0;
@@ -744,7 +745,7 @@ int ocsc_destroy(ocsc_ctx* c) {
   (void)hipDeviceSynchronize();
   for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
                  &L.E64, &c->draws, &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
-                 &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->csr_bins})
+                 &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->lsc_fac, &c->csr_bins})
     if (b->p) (void)hipFree(b->p);
   if (c->lsc) (void)hipFree(c->lsc);
   if (c->lsc_plan) (void)hipFree(c->lsc_plan);
@@ -1455,13 +1456,28 @@ extern "C" {
 
 int ocsc_track_drift_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
                          double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, double lsc_dz,
-                         int step_profile, double smooth_param, double b0, double b1, void* stream) {
+                         int step_profile, double smooth_param, double b0, double b1, const double* und_factors,
+                         void* stream) {
   if (!(lsc_dz >= 1e-10)) return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph,
                                             stream, nullptr, nullptr);  // sc.py:556-558
+  if (!c) return fail(OCSC_EINVAL, "null argument");
   LscReq lq;
   lq.k = lsc_const(step_profile, smooth_param, E_GeV, lsc_dz, 0.0, 0.0, kLscGraphBins);
   lq.b0 = b0;
   lq.b1 = b1;
+  if (und_factors && nsteps > 0) {
+    int rc;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = ensure(c->lsc_fac, (size_t)nsteps * sizeof(double) + 256))) return rc;
+    HIPCHK(hipMemcpyAsync(c->lsc_fac.p, und_factors, (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // the caller's (pageable) array may go away after the call
+    int* ctr = (int*)((char*)c->lsc_fac.p + (size_t)nsteps * sizeof(double));
+    k_counter_reset<<<1, 64, 0, st>>>(ctr);
+    lq.k.factors = (const double*)c->lsc_fac.p;
+    lq.k.fctr = ctr;
+    lq.k.nfactors = nsteps;
+  }
   return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph, stream, nullptr, nullptr, &lq);
 }
 

@@ -254,7 +254,7 @@ def drift(p_array, L, device=None):
 
 
 def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, bounds=None, auto_disp=False,
-                tws_i=None, lsc=None):
+                tws_i=None, lsc=None, z0=None):
     """nsteps x [drift(L); sc.apply(p_array, dz)] on a device-resident bunch,
     the step loop of ocelot/cpbd/track.py:470-477 for a drift lattice, with
     one step captured in a HIP graph and replayed.
@@ -267,8 +267,11 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
     With ``lsc`` (an LSC instance) every step also applies ``lsc.apply(p_array,
     dz)`` after the space-charge kick, inside the same graph (the order in
     which track() applies two processes added SpaceCharge first); ``sc`` may
-    then be None for LSC-only tracking.  Not combined with ``calc_tws``; a line
-    with undulators needs the per-step filling factor, so call apply instead."""
+    then be None for LSC-only tracking.  Not combined with ``calc_tws``.  For a
+    prepared LSC with undulators in its line, step k ends at ``z0 + (k + 1) L``
+    (``z0``: the lattice position at the start, default ``p_array.s``) and takes
+    the K_max and filling factor of that step, as LSC.apply would with the
+    Navigator's z0; ``lsc.z0`` is left at the end position."""
     torch = _torch()
     lib = _lib.load()
     r = p_array.rparticles
@@ -285,8 +288,15 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
         if lsc is not None:
             if calc_tws:
                 raise ValueError("track_drift: calc_tws with lsc is not available; take get_envelope per step")
-            if lsc._is_undul_in_beam_line:
-                raise ValueError("track_drift: an LSC process with undulators in the line needs apply per step")
+            fac = None
+            if lsc._is_undul_in_beam_line and float(dz) >= 1e-10:
+                start = float(p_array.s if z0 is None else z0)
+                vals = []
+                for k in range(int(nsteps)):
+                    lsc.z0 = start + (k + 1) * float(L)
+                    K_max, fill_factor = lsc.kick_args(float(dz))
+                    vals.append(1 + 0.5 * K_max * K_max * fill_factor)   # as the impedance factor is formed
+                fac = (ctypes.c_double * max(len(vals), 1))(*vals)
             nmesh = sc._nmesh() if sc is not None else _lib.i32x3(np.array([4, 4, 4]))
             prec = sc._prec() if sc is not None else _lib.OCSC_FP64
             b0, b1 = (float(v) for v in lsc.bounds)
@@ -294,7 +304,7 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
                                                 p_array.q_array.data_ptr(), n, nmesh, float(p_array.E),
                                                 float(dz) if sc is not None else 0.0, float(L), int(nsteps), prec,
                                                 1 if use_graph else 0, float(dz), int(bool(lsc.step_profile)),
-                                                float(lsc.smooth_param), b0, b1, stream))
+                                                float(lsc.smooth_param), b0, b1, fac, stream))
             return None
         if sc.random_mesh:
             # the reference draws U(1, 1.1) then U(-0.5, 0.5) from the global
