@@ -127,7 +127,9 @@ int ocsc_moments(ocsc_ctx* ctx, const double* rparticles, int64_t ld, const doub
 /* ocsc_track_drift with the beam moments of ocsc_moments taken before the
  * first step and after every step (after the kick, before the next drift),
  * as track() does with get_envelope (ocelot/cpbd/track.py:462-485).  mom_flags,
- * disp, b0, b1 as in ocsc_moments (OCSC_MOM_IMAX is not available here).
+ * disp, b0, b1 as in ocsc_moments; with OCSC_MOM_IMAX the current profile of
+ * each step is planned on the device, up to 8192 bins (a step that would need
+ * more keeps the window at mean(tau) and the call returns OCSC_EVALUE).
  * tws_out (host) receives (nsteps + 1) rows of OCSC_MOM_NOUT values.  The
  * particles end up exactly as with ocsc_track_drift. */
 int ocsc_track_drift_tws(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,

@@ -528,25 +528,5 @@ __global__ void k_lsc_plan(const LscState* __restrict__ s, LscConst k, LscPlan* 
   if (q.err && *err == 0) *err = q.err;
 }
 
-__global__ void __launch_bounds__(kPT) k_cic_plan(Rows R, int64_t n, const LscPlan* __restrict__ p, double wscale,
-                                                  long long* __restrict__ hist) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  long long* h = (long long*)smem_raw;
-  const int nbins = p->nbins;
-  if (nbins == 0) return;
-  const double a = p->a, ds = p->ds;
-  for (int j = threadIdx.x; j < nbins; j += kPT) h[j] = 0;
-  __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
-    const double c = (R.r[4 * R.ld + i] - a) / ds;
-    const double fl = __builtin_floor(c);
-    const double xi = 1 + fl - c;
-    const int I = (int)(fl < 0 ? 0 : (fl > nbins - 2 ? nbins - 2 : fl));
-    atomicAdd((unsigned long long*)&h[I], (unsigned long long)(long long)__builtin_rint(xi * wscale));
-    atomicAdd((unsigned long long*)&h[I + 1], (unsigned long long)(long long)__builtin_rint((1 - xi) * wscale));
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j < nbins; j += kPT) hist[(int64_t)blockIdx.x * nbins + j] = h[j];
-}
 
 }  // namespace ocsc

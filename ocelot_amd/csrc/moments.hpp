@@ -30,6 +30,7 @@ struct MomState {
   double c[4];        // means of the corrected x, y, px, py
   double mom[12];     // xx, xpx, pxpx, yy, ypy, pypy, tautau, xy, pxpy, xpy, ypx, pp
   double tau_min, tau_max;  // extent of tau (slice = "Imax")
+  double z0;                // window centre: mean(tau), or the peak-current position (slice = "Imax")
 };
 
 constexpr int kMomNV[6] = {2, 1, 7, 5, 4, 12};
@@ -131,6 +132,7 @@ __global__ void __launch_bounds__(1024) k_mom_reduce(MomState* __restrict__ msp,
   if constexpr (PASS == 0) {
     s.tau_all = t[0] / n;
     s.sumq = t[1];
+    s.z0 = s.tau_all;
   } else if constexpr (PASS == 1) {
     s.std_all = __builtin_sqrt(t[0] / n);
     s.lo = s.tau_all + s.std_all * s.b0;
@@ -271,6 +273,128 @@ __global__ void __launch_bounds__(1024) k_cic_sum(const long long* __restrict__ 
     long long t = 0;
     for (int k = 0; k < 16; ++k) t += part[k][lane];
     out[j] = t;
+  }
+}
+
+// Histogram on a grid read from a plan P (members nbins, a, ds) planned on
+// the device: one LDS window of at most the launch's dynamic LDS (bins).
+template <class P>
+__global__ void __launch_bounds__(kPT) k_cic_grid(Rows R, int64_t n, const P* __restrict__ p, double wscale,
+                                                  long long* __restrict__ hist) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  long long* h = (long long*)smem_raw;
+  const int nbins = p->nbins;
+  if (nbins == 0) return;
+  const double a = p->a, ds = p->ds;
+  for (int j = threadIdx.x; j < nbins; j += kPT) h[j] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
+    const double c = (R.r[4 * R.ld + i] - a) / ds;
+    const double fl = __builtin_floor(c);
+    const double xi = 1 + fl - c;
+    const int I = (int)(fl < 0 ? 0 : (fl > nbins - 2 ? nbins - 2 : fl));
+    atomicAdd((unsigned long long*)&h[I], (unsigned long long)(long long)__builtin_rint(xi * wscale));
+    atomicAdd((unsigned long long*)&h[I + 1], (unsigned long long)(long long)__builtin_rint((1 - xi) * wscale));
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nbins; j += kPT) hist[(int64_t)blockIdx.x * nbins + j] = h[j];
+}
+
+// slice = "Imax" inside tracking graphs: the grid of s_to_cur(tau, 0.01
+// std(tau), ...) planned on the device from the moments state (the one-shot
+// path plans it on the host the same way), then the smoothing and the first
+// maximum in one workgroup; the window is centred there.
+struct ImaxPlan {
+  int nbins, K, err;  // err: 1 no spread / not finite, 2 more bins than the cap
+  double a, ds, sig;
+};
+
+__global__ void k_imax_plan(const MomState* __restrict__ s, ImaxPlan* __restrict__ p, int cap,
+                            int* __restrict__ err) {
+  if (threadIdx.x != 0) return;
+  ImaxPlan q{};
+  const double sigma = 0.01 * s->std_all;
+  const double a = s->tau_min - 3 * sigma, b = s->tau_max + 3 * sigma;
+  const double ds0 = 0.25 * sigma;
+  const double nf = ceil((b - a) / ds0);
+  if (!(nf >= 1) || !(nf <= 1e300)) {
+    q.err = 1;
+  } else if (nf + 1 > (double)cap) {
+    q.err = 2;
+  } else {
+    const int N = (int)nf;
+    q.nbins = N + 1;
+    q.ds = (b - a) / N;
+    q.a = a;
+    q.sig = sigma;
+    q.K = (int)floor(3 * sigma / q.ds + 0.5);
+  }
+  *p = q;
+  if (q.err && *err == 0) *err = q.err;
+}
+
+// smoothed profile (2K+1 Gaussian taps, then divided by their sum) and its
+// first maximum; sets the window [z0 + b0 std, z0 + b1 std] in the state
+__global__ void __launch_bounds__(1024) k_imax_peak(const long long* __restrict__ C, const ImaxPlan* __restrict__ p,
+                                                    MomState* __restrict__ msp) {
+  __shared__ double G[128];
+  __shared__ double gsum;
+  __shared__ double bv[16];
+  __shared__ int bj[16];
+  const int nbins = p->nbins, K = p->K;
+  if (nbins == 0 || 2 * K + 1 > 128) return;
+  const double ds = p->ds, sigma = p->sig;
+  if (threadIdx.x == 0) {
+    double gs = 0;
+    for (int m = 0; m <= 2 * K; ++m) {
+      const double x = (double)(m - K) * ds / sigma;
+      G[m] = exp(-0.5 * (x * x));
+      gs += G[m];
+    }
+    gsum = gs;
+  }
+  __syncthreads();
+  double best = -1;
+  int jbest = 0;
+  for (int j = threadIdx.x; j < nbins; j += blockDim.x) {
+    double acc = 0;
+    for (int m = 0; m <= 2 * K; ++m) {
+      const int i = j + K - m;
+      if (i >= 0 && i < nbins) acc += (double)C[i] * G[m];
+    }
+    acc = acc / gsum;
+    if (acc > best) {
+      best = acc;
+      jbest = j;
+    }
+  }
+  // largest value, first index among equals
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_down(best, off, 64);
+    const int oj = __shfl_down(jbest, off, 64);
+    if (ov > best || (ov == best && oj < jbest)) {
+      best = ov;
+      jbest = oj;
+    }
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    bv[w] = best;
+    bj[w] = jbest;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
+      if (bv[k] > best || (bv[k] == best && bj[k] < jbest)) {
+        best = bv[k];
+        jbest = bj[k];
+      }
+    MomState& s = *msp;
+    const double z0 = (double)jbest * ds + p->a;
+    s.z0 = z0;
+    s.lo = z0 + s.std_all * s.b0;
+    s.hi = z0 + s.std_all * s.b1;
   }
 }
 

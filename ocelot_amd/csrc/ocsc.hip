@@ -223,6 +223,8 @@ struct ocsc_ctx {
   Buf lsc_fac;  // per-step undulator factors of an LSC tracking call and their counter
   Buf csr_bins;  // CSR: line charge, kernel, convolution
   int lsc_nbins = 0;
+  ImaxPlan* imax_plan = nullptr;  // slice = "Imax" inside tracking graphs
+  int* imax_err = nullptr;
   double lsc_a = 0, lsc_ds = 0;
 };
 
@@ -749,6 +751,8 @@ int ocsc_destroy(ocsc_ctx* c) {
     if (b->p) (void)hipFree(b->p);
   if (c->lsc) (void)hipFree(c->lsc);
   if (c->lsc_plan) (void)hipFree(c->lsc_plan);
+  if (c->imax_plan) (void)hipFree(c->imax_plan);
+  if (c->imax_err) (void)hipFree(c->imax_err);
   if (c->lsc_err) (void)hipFree(c->lsc_err);
   if (c->lsc_host) (void)hipHostFree(c->lsc_host);
   if (L.sc) (void)hipFree(L.sc);
@@ -1103,10 +1107,14 @@ namespace {
 // moments requested per tracking step (the tws_track list of track())
 struct TwsReq {
   int bounds = 0, disp_mode = 0;  // as in MomState
+  int imax = 0;                   // window centred on the peak current (slice = "Imax")
   double b0 = 0, b1 = 0;
   double disp[4] = {0, 0, 0, 0};
   int cap = 0;  // rows of the step table
 };
+
+constexpr int kImaxGraphBins = kCicLds;  // slice = "Imax" inside tracking graphs (64 KB of LDS)
+double cic_wscale(int64_t n);
 
 // the moments passes of ocsc_moments on the device only, appended to the
 // step table (graph-capturable; own partial buffer, so the kick's fused
@@ -1122,6 +1130,19 @@ int enqueue_moments(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, cons
   if (t.bounds) {
     k_mom<1><<<nblk, kPT, 0, st>>>(R, q, cur, part);
     k_mom_reduce<1><<<1, 1024, 0, st>>>(cur, part, nblk);
+    if (t.imax) {
+      // centre on the peak of the current profile (analysis.py:61-64)
+      const int cap = kImaxGraphBins;
+      long long* hist = (long long*)c->cic.p;
+      long long* tot = hist + (size_t)kCicBlocks * cap;
+      k_tau_range<<<nblk, kPT, 0, st>>>(R, n, part);
+      k_tau_range_reduce<<<1, 1024, 0, st>>>(cur, part, nblk);
+      k_imax_plan<<<1, 64, 0, st>>>(cur, c->imax_plan, cap, c->imax_err);
+      k_cic_grid<ImaxPlan><<<kCicBlocks, kPT, (size_t)cap * sizeof(long long), st>>>(R, n, c->imax_plan,
+                                                                                     cic_wscale(n), hist);
+      k_cic_sum<<<(cap + 63) / 64, 1024, 0, st>>>(hist, kCicBlocks, cap, &c->imax_plan->nbins, tot);
+      k_imax_peak<<<1, 1024, 0, st>>>(tot, c->imax_plan, cur);
+    }
   }
   k_mom<2><<<nblk, kPT, 0, st>>>(R, q, cur, part);
   k_mom_reduce<2><<<1, 1024, 0, st>>>(cur, part, nblk);
@@ -1148,7 +1169,7 @@ void moments_row(const MomState& h, double* v) {
   for (int k = 0; k < 12; ++k) v[12 + k] = h.mom[k];
   v[24] = h.tau_all;
   v[25] = h.std_all;
-  v[26] = h.tau_all;
+  v[26] = h.z0;
 }
 
 // an LSC kick after every space-charge kick of a tracking loop
@@ -1216,6 +1237,15 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     if ((rc = ensure(c->mom_tab, (size_t)(nsteps + 1) * sizeof(MomState)))) return rc;
     if ((rc = ensure(c->mom_ctr, sizeof(int)))) return rc;
     k_counter_reset<<<1, 64, 0, st>>>((int*)c->mom_ctr.p);
+    if (tw->imax) {
+      if (!c->imax_plan) {
+        HIPCHK(hipMalloc(&c->imax_plan, sizeof(ImaxPlan)));
+        HIPCHK(hipMalloc(&c->imax_err, sizeof(int)));
+      }
+      if ((rc = ensure(c->cic, ((size_t)kCicBlocks * kImaxGraphBins + kImaxGraphBins) * sizeof(long long))))
+        return rc;
+      k_counter_reset<<<1, 64, 0, st>>>(c->imax_err);
+    }
     // the bunch before the first step (tw0 of track())
     if ((rc = enqueue_moments(c, Rows{r, ld}, q, n, *tw, st))) return rc;
   }
@@ -1441,6 +1471,14 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     for (int k = 0; k <= nsteps; ++k) moments_row(tab[k], tws_out + (size_t)k * OCSC_MOM_NOUT);
   }
   if (err) return value_error(err);
+  if (tw && tw->imax) {
+    int ierr = 0;
+    HIPCHK(hipMemcpyAsync(&ierr, c->imax_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (ierr == 1) return fail(OCSC_EVALUE, "slice='Imax' needs a spread in tau (a step's window stayed at mean(tau))");
+    if (ierr) return fail(OCSC_EVALUE, "slice='Imax' inside tracking: current profile over 8192 bins (a step's window "
+                                       "stayed at mean(tau)); take get_envelope per step instead");
+  }
   if (lq) {
     int lerr = 0;
     HIPCHK(hipMemcpyAsync(&lerr, c->lsc_err, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1498,10 +1536,12 @@ int ocsc_track_drift_tws(ocsc_ctx* c, double* r, int64_t ld, const double* q, in
                          double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, int mom_flags,
                          const double disp[4], double b0, double b1, double* tws_out, void* stream) {
   if (!tws_out) return fail(OCSC_EINVAL, "null tws_out");
-  if (mom_flags & OCSC_MOM_IMAX) return fail(OCSC_EINVAL, "slice='Imax' is not available inside tracking");
+  if ((mom_flags & OCSC_MOM_IMAX) && !(mom_flags & OCSC_MOM_BOUNDS))
+    return fail(OCSC_EINVAL, "OCSC_MOM_IMAX selects the window centre and needs OCSC_MOM_BOUNDS");
   if ((mom_flags & OCSC_MOM_DISP_GIVEN) && !disp) return fail(OCSC_EINVAL, "OCSC_MOM_DISP_GIVEN needs disp");
   TwsReq t;
   t.bounds = (mom_flags & OCSC_MOM_BOUNDS) ? 1 : 0;
+  t.imax = (mom_flags & OCSC_MOM_IMAX) ? 1 : 0;
   t.disp_mode = (mom_flags & OCSC_MOM_DISP_GIVEN) ? 1 : ((mom_flags & OCSC_MOM_AUTO_DISP) ? 2 : 0);
   t.b0 = b0;
   t.b1 = b1;
@@ -1805,9 +1845,10 @@ int enqueue_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, 
   k_lsc_plan<<<1, 64, 0, st>>>(c->lsc, k, c->lsc_plan, c->lsc_err);
   long long* hist = (long long*)c->lsc_cic.p;
   long long* tot = hist + (size_t)ncic * cap;
-  k_cic_plan<<<ncic, kPT, (size_t)cap * sizeof(long long), st>>>(Rows{r, ld}, n, c->lsc_plan, cic_wscale(n), hist);
+  k_cic_grid<LscPlan><<<ncic, kPT, (size_t)cap * sizeof(long long), st>>>(Rows{r, ld}, n, c->lsc_plan, cic_wscale(n),
+                                                                         hist);
   k_cic_sum<<<(cap + 63) / 64, 1024, 0, st>>>(hist, ncic, cap, &c->lsc_plan->nbins, tot);
-  LAUNCHCHK("k_cic_plan");
+  LAUNCHCHK("k_cic_grid");
   return lsc_from_profile(c, r, ld, n, tot, cap, st, dr);
 }
 

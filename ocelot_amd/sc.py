@@ -254,15 +254,15 @@ def drift(p_array, L, device=None):
 
 
 def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, bounds=None, auto_disp=False,
-                tws_i=None, lsc=None, z0=None):
+                tws_i=None, lsc=None, z0=None, slice=None):
     """nsteps x [drift(L); sc.apply(p_array, dz)] on a device-resident bunch,
     the step loop of ocelot/cpbd/track.py:470-477 for a drift lattice, with
     one step captured in a HIP graph and replayed.
 
     With ``calc_tws`` the beam moments are taken on the device before the
     first step and after every step, and the list of Twiss (``tws_track`` of
-    track(), ``s`` = cumulative length) is returned; ``bounds``, ``auto_disp``
-    and ``tws_i`` as in get_envelope (slice='Imax' is not available here).
+    track(), ``s`` = cumulative length) is returned; ``bounds``, ``slice``
+    (None or "Imax"), ``auto_disp`` and ``tws_i`` as in get_envelope.
 
     With ``lsc`` (an LSC instance) every step also applies ``lsc.apply(p_array,
     dz)`` after the space-charge kick, inside the same graph (the order in
@@ -329,6 +329,8 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
         if bounds is not None:
             flags |= 1
             b0, b1 = float(bounds[0]), float(bounds[1])
+            if slice == "Imax":
+                flags |= 8
         if tws_i is not None:
             flags |= 4
             disp = (tws_i.Dx, tws_i.Dxp, tws_i.Dy, tws_i.Dyp)

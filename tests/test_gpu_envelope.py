@@ -167,6 +167,40 @@ def test_track_drift_with_twiss_list(graph):
     assert worst <= 1e-8
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_track_drift_twiss_with_imax_window(graph):
+    """track()'s Twiss list with bounds and slice='Imax': each step's window is
+    centred on the peak of its current profile, planned and found on the
+    device inside the step graphs; against the oracle's own loop, and the last
+    row against the one-shot get_envelope of the final bunch."""
+    from oracle import sc_oracle as orc
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, get_envelope, track_drift
+    from ocelot_amd.workloads import make_bunch
+    print(f"[imax tracking] graph={graph}", flush=True)
+    r, q, E, _, dz = make_bunch("flattop_10m_127", n=200_000, seed=23)
+    nmesh, nsteps, bounds = [31, 31, 63], 6, [-0.5, 0.5]
+    dev = torch.cuda.current_device()
+    sc = SpaceCharge(nmesh_xyz=nmesh, precision="fp64")
+    d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+    tws = track_drift(d, sc, dz, dz, nsteps, use_graph=graph, calc_tws=True, bounds=bounds, slice="Imax")
+    ro = r.copy()
+    ref = [eo.get_envelope(ro, q, E, bounds=bounds, slice="Imax")]
+    for _ in range(nsteps):
+        orc.drift(ro, dz, E)
+        orc.apply(ro, q, E, dz, nmesh)
+        ref.append(eo.get_envelope(ro, q, E, bounds=bounds, slice="Imax"))
+    worst = 0.0
+    for t, rf in zip(tws, ref):
+        for a in ("emit_x", "emit_y", "beta_x", "alpha_x", "xx", "pxpx", "tautau", "pp", "x", "px"):
+            scale = max(abs(rf[a]), np.sqrt(abs(rf[SCALE[a]])) if isinstance(SCALE.get(a), str) else 0)
+            worst = max(worst, abs(float(getattr(t, a)) - rf[a]) / scale)
+    print(f"imax tws ({'graph' if graph else 'eager'}): worst relative deviation {worst:.2e}")
+    assert worst <= 1e-8
+    last = get_envelope(d, bounds=bounds, slice="Imax")
+    for a in ("emit_x", "beta_x", "xx", "tautau", "pp"):
+        assert float(getattr(tws[-1], a)) == pytest.approx(float(getattr(last, a)), rel=1e-12)
+
+
 def test_npz_bunch_loads_into_hbm(tmp_path):
     """A bunch file the reference wrote, loaded straight into HBM, equals the
     host load; saving the device bunch gives the same file contents."""
