@@ -159,6 +159,34 @@ int ocsc_lsc_apply(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* 
                    double dz, int step_profile, double smooth_param, double b0, double b1, double K_max,
                    double fill_factor, double* info, int ninfo, void* stream);
 
+/* Options of ocsc_track: everything that can run inside the captured step of
+ * a drift-line tracking loop.  Zero-initialise, then set what is wanted. */
+typedef struct ocsc_track_opts {
+  int use_graph;             /* capture one step in a HIP graph and replay it */
+  /* per-step moments (as ocsc_track_drift_tws) when tws_out != NULL */
+  double* tws_out;           /* (nsteps + 1) rows of OCSC_MOM_NOUT values */
+  int mom_flags;             /* OCSC_MOM_* */
+  double mom_b0, mom_b1;
+  double disp[4];
+  /* an LSC kick after every space-charge kick (as ocsc_track_drift_lsc) when lsc != 0 */
+  int lsc;
+  double lsc_dz;
+  int lsc_step_profile;
+  double lsc_smooth_param, lsc_b0, lsc_b1;
+  const double* lsc_und_factors; /* per step, or NULL */
+  /* random_mesh draws (as ocsc_track_drift_random_mesh), or NULL */
+  const double* mesh_draws;
+} ocsc_track_opts;
+
+/* nsteps x [drift(L); SpaceCharge kick(zstep) (none if zstep == 0); LSC kick;
+ * moments] on a device-resident bunch, the tracking loop of
+ * ocelot/cpbd/track.py:462-485 on a drift line with the processes and Twiss
+ * that opts asks for.  The other ocsc_track_drift* entry points are this call
+ * with one option set. */
+int ocsc_track(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,
+               const int32_t nmesh[3], double E_GeV, double zstep, double L, int nsteps, int precision,
+               const ocsc_track_opts* opts, void* stream);
+
 /* ocsc_track_drift with SpaceCharge(random_mesh=True) (ocelot/cpbd/sc.py:
  * 174-175, 184-185): draws (host) holds nsteps pairs (mesh_scale, mesh_shift),
  * the two global numpy draws of each kick in the reference's order; kick k

@@ -54,6 +54,7 @@ SIGNATURES = {
     "ocsc_lsc_profile": (_c_int, [_vp, _dp, _dp, _dp, _i64, ctypes.POINTER(_i64)]),
     "ocsc_lsc_wake": (_c_int, [_vp, _dp, _dp, _i64, _dbl, _dbl, _dbl, _c_int, _dbl, _dbl, _dp, _vp]),
     "ocsc_lsc_impedance": (_c_int, [_vp, _dp, _i64, _dbl, _dbl, _dbl, _c_int, _dp, _vp]),
+    "ocsc_track": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int, _vp, _vp]),
     "ocsc_track_drift_random_mesh": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int,
                                               _c_int, _dp, _vp]),
     "ocsc_track_drift_lsc": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _dbl, _c_int, _c_int, _c_int,
@@ -100,6 +101,15 @@ def check(rc):
     if rc == OCSC_EINVAL:
         raise ValueError("invalid argument: " + msg)
     raise OcscError(f"ocsc error {rc}: {msg}")
+
+
+class TrackOpts(ctypes.Structure):
+    """ocsc_track_opts of include/ocsc.h"""
+    _fields_ = [("use_graph", ctypes.c_int), ("tws_out", _dp), ("mom_flags", ctypes.c_int),
+                ("mom_b0", ctypes.c_double), ("mom_b1", ctypes.c_double), ("disp", ctypes.c_double * 4),
+                ("lsc", ctypes.c_int), ("lsc_dz", ctypes.c_double), ("lsc_step_profile", ctypes.c_int),
+                ("lsc_smooth_param", ctypes.c_double), ("lsc_b0", ctypes.c_double), ("lsc_b1", ctypes.c_double),
+                ("lsc_und_factors", _dp), ("mesh_draws", _dp)]
 
 
 class Bunch(ctypes.Structure):

@@ -1496,35 +1496,26 @@ int ocsc_track_drift_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, in
                          double E_GeV, double zstep, double Ld, int nsteps, int prec, int use_graph, double lsc_dz,
                          int step_profile, double smooth_param, double b0, double b1, const double* und_factors,
                          void* stream) {
-  if (!(lsc_dz >= 1e-10)) return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph,
-                                            stream, nullptr, nullptr);  // sc.py:556-558
-  if (!c) return fail(OCSC_EINVAL, "null argument");
-  LscReq lq;
-  lq.k = lsc_const(step_profile, smooth_param, E_GeV, lsc_dz, 0.0, 0.0, kLscGraphBins);
-  lq.b0 = b0;
-  lq.b1 = b1;
-  if (und_factors && nsteps > 0) {
-    int rc;
-    HIPCHK(hipSetDevice(c->device));
-    hipStream_t st = (hipStream_t)stream;
-    if ((rc = ensure(c->lsc_fac, (size_t)nsteps * sizeof(double) + 256))) return rc;
-    HIPCHK(hipMemcpyAsync(c->lsc_fac.p, und_factors, (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));  // the caller's (pageable) array may go away after the call
-    int* ctr = (int*)((char*)c->lsc_fac.p + (size_t)nsteps * sizeof(double));
-    k_counter_reset<<<1, 64, 0, st>>>(ctr);
-    lq.k.factors = (const double*)c->lsc_fac.p;
-    lq.k.fctr = ctr;
-    lq.k.nfactors = nsteps;
-  }
-  return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph, stream, nullptr, nullptr, &lq);
+  ocsc_track_opts o{};
+  o.use_graph = use_graph;
+  o.lsc = 1;
+  o.lsc_dz = lsc_dz;
+  o.lsc_step_profile = step_profile;
+  o.lsc_smooth_param = smooth_param;
+  o.lsc_b0 = b0;
+  o.lsc_b1 = b1;
+  o.lsc_und_factors = und_factors;
+  return ocsc_track(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, &o, stream);
 }
 
 int ocsc_track_drift_random_mesh(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n,
                                  const int32_t nmesh[3], double E_GeV, double zstep, double Ld, int nsteps, int prec,
                                  int use_graph, const double* draws, void* stream) {
   if (!draws && nsteps > 0 && zstep != 0) return fail(OCSC_EINVAL, "null draws");
-  return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph, stream, nullptr, nullptr,
-                    nullptr, draws);
+  ocsc_track_opts o{};
+  o.use_graph = use_graph;
+  o.mesh_draws = draws;
+  return ocsc_track(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, &o, stream);
 }
 
 int ocsc_track_drift(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
@@ -1549,6 +1540,52 @@ int ocsc_track_drift_tws(ocsc_ctx* c, double* r, int64_t ld, const double* q, in
     for (int k = 0; k < 4; ++k) t.disp[k] = disp[k];
   t.cap = nsteps + 1;
   return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, use_graph, stream, &t, tws_out);
+}
+
+int ocsc_track(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3], double E_GeV,
+               double zstep, double Ld, int nsteps, int prec, const ocsc_track_opts* o, void* stream) {
+  if (!o) return fail(OCSC_EINVAL, "null opts");
+  TwsReq t;
+  const TwsReq* tw = nullptr;
+  if (o->tws_out) {
+    const int f = o->mom_flags;
+    if ((f & OCSC_MOM_IMAX) && !(f & OCSC_MOM_BOUNDS))
+      return fail(OCSC_EINVAL, "OCSC_MOM_IMAX selects the window centre and needs OCSC_MOM_BOUNDS");
+    t.bounds = (f & OCSC_MOM_BOUNDS) ? 1 : 0;
+    t.imax = (f & OCSC_MOM_IMAX) ? 1 : 0;
+    t.disp_mode = (f & OCSC_MOM_DISP_GIVEN) ? 1 : ((f & OCSC_MOM_AUTO_DISP) ? 2 : 0);
+    t.b0 = o->mom_b0;
+    t.b1 = o->mom_b1;
+    if (t.disp_mode == 1)
+      for (int k = 0; k < 4; ++k) t.disp[k] = o->disp[k];
+    t.cap = nsteps + 1;
+    tw = &t;
+  }
+  if (o->lsc && o->lsc_dz >= 1e-10) {
+    if (!c) return fail(OCSC_EINVAL, "null argument");
+    LscReq lq;
+    lq.k = lsc_const(o->lsc_step_profile, o->lsc_smooth_param, E_GeV, o->lsc_dz, 0.0, 0.0, kLscGraphBins);
+    lq.b0 = o->lsc_b0;
+    lq.b1 = o->lsc_b1;
+    if (o->lsc_und_factors && nsteps > 0) {
+      int rc;
+      HIPCHK(hipSetDevice(c->device));
+      hipStream_t st = (hipStream_t)stream;
+      if ((rc = ensure(c->lsc_fac, (size_t)nsteps * sizeof(double) + 256))) return rc;
+      HIPCHK(hipMemcpyAsync(c->lsc_fac.p, o->lsc_und_factors, (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice,
+                            st));
+      HIPCHK(hipStreamSynchronize(st));  // the caller's (pageable) array may go away after the call
+      int* ctr = (int*)((char*)c->lsc_fac.p + (size_t)nsteps * sizeof(double));
+      k_counter_reset<<<1, 64, 0, st>>>(ctr);
+      lq.k.factors = (const double*)c->lsc_fac.p;
+      lq.k.fctr = ctr;
+      lq.k.nfactors = nsteps;
+    }
+    return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, o->use_graph, stream, tw, o->tws_out,
+                      &lq, o->mesh_draws);
+  }
+  return track_impl(c, r, ld, q, n, nmesh, E_GeV, zstep, Ld, nsteps, prec, o->use_graph, stream, tw, o->tws_out,
+                    nullptr, o->mesh_draws);
 }
 
 int ocsc_kick_times(ocsc_ctx* c, double* ms, int nout, int64_t* count) {

@@ -256,7 +256,7 @@ def drift(p_array, L, device=None):
 def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, bounds=None, auto_disp=False,
                 tws_i=None, lsc=None, z0=None, slice=None):
     """nsteps x [drift(L); sc.apply(p_array, dz)] on a device-resident bunch,
-    the step loop of ocelot/cpbd/track.py:470-477 for a drift lattice, with
+    the step loop of ocelot/cpbd/track.py:462-485 for a drift lattice, with
     one step captured in a HIP graph and replayed.
 
     With ``calc_tws`` the beam moments are taken on the device before the
@@ -267,68 +267,59 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
     With ``lsc`` (an LSC instance) every step also applies ``lsc.apply(p_array,
     dz)`` after the space-charge kick, inside the same graph (the order in
     which track() applies two processes added SpaceCharge first); ``sc`` may
-    then be None for LSC-only tracking.  Not combined with ``calc_tws``.  For a
-    prepared LSC with undulators in its line, step k ends at ``z0 + (k + 1) L``
-    (``z0``: the lattice position at the start, default ``p_array.s``) and takes
-    the K_max and filling factor of that step, as LSC.apply would with the
-    Navigator's z0; ``lsc.z0`` is left at the end position."""
+    then be None for LSC-only tracking.  For a prepared LSC with undulators in
+    its line, step k ends at ``z0 + (k + 1) L`` (``z0``: the lattice position at
+    the start, default ``p_array.s``) and takes the K_max and filling factor of
+    that step, as LSC.apply would with the Navigator's z0; ``lsc.z0`` is left at
+    the end position.
+
+    With ``sc.random_mesh`` each kick takes the reference's two global numpy
+    draws (scale, then shift), all taken before the call in kick order."""
     torch = _torch()
     lib = _lib.load()
     r = p_array.rparticles
     if not _is_cuda_tensor(r):
         raise TypeError("track_drift needs a DeviceParticleArray")
-    if sc is not None and sc.random_mesh and (lsc is not None or calc_tws):
-        raise ValueError("track_drift: random_mesh is not combined with lsc or calc_tws; call apply per step")
     if sc is None and lsc is None:
         raise ValueError("track_drift needs a SpaceCharge or an LSC process")
     dev = r.device.index
     n = int(r.shape[1])
-    with torch.cuda.device(dev):
-        stream = torch.cuda.current_stream(dev).cuda_stream
-        if lsc is not None:
-            if calc_tws:
-                raise ValueError("track_drift: calc_tws with lsc is not available; take get_envelope per step")
-            fac = None
-            if lsc._is_undul_in_beam_line and float(dz) >= 1e-10:
-                start = float(p_array.s if z0 is None else z0)
-                vals = []
-                for k in range(int(nsteps)):
-                    lsc.z0 = start + (k + 1) * float(L)
-                    K_max, fill_factor = lsc.kick_args(float(dz))
-                    vals.append(1 + 0.5 * K_max * K_max * fill_factor)   # as the impedance factor is formed
-                fac = (ctypes.c_double * max(len(vals), 1))(*vals)
-            nmesh = sc._nmesh() if sc is not None else _lib.i32x3(np.array([4, 4, 4]))
-            prec = sc._prec() if sc is not None else _lib.OCSC_FP64
-            b0, b1 = (float(v) for v in lsc.bounds)
-            _lib.check(lib.ocsc_track_drift_lsc(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
-                                                p_array.q_array.data_ptr(), n, nmesh, float(p_array.E),
-                                                float(dz) if sc is not None else 0.0, float(L), int(nsteps), prec,
-                                                1 if use_graph else 0, float(dz), int(bool(lsc.step_profile)),
-                                                float(lsc.smooth_param), b0, b1, fac, stream))
-            return None
-        if sc.random_mesh:
-            # the reference draws U(1, 1.1) then U(-0.5, 0.5) from the global
-            # numpy RNG in every kick (sc.py:174-175, 184-185); no kick, no draw
-            k = int(nsteps) if float(dz) != 0 else 0
-            draws = np.array([sc._draws()[1:] for _ in range(k)], dtype=np.float64).reshape(-1)
-            buf = (ctypes.c_double * max(draws.size, 1))(*draws)
-            _lib.check(lib.ocsc_track_drift_random_mesh(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
-                                                        p_array.q_array.data_ptr(), n, sc._nmesh(),
-                                                        float(p_array.E), float(dz), float(L), int(nsteps),
-                                                        sc._prec(), 1 if use_graph else 0, buf, stream))
-            return None
-        if not calc_tws:
-            _lib.check(lib.ocsc_track_drift(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
-                                            p_array.q_array.data_ptr(), n, sc._nmesh(), float(p_array.E),
-                                            float(dz), float(L), int(nsteps), sc._prec(), 1 if use_graph else 0,
-                                            stream))
-            return None
-        from .envelope import twiss_from_moments
-        flags, b0, b1 = 0, 0.0, 0.0
-        disp = (0.0, 0.0, 0.0, 0.0)
+    nsteps = int(nsteps)
+    opts = _lib.TrackOpts()
+    opts.use_graph = 1 if use_graph else 0
+    zstep = float(dz) if sc is not None else 0.0
+    nmesh = sc._nmesh() if sc is not None else _lib.i32x3(np.array([4, 4, 4]))
+    prec = sc._prec() if sc is not None else _lib.OCSC_FP64
+    keep = []
+    if sc is not None and sc.random_mesh and zstep != 0:
+        # the reference draws U(1, 1.1) then U(-0.5, 0.5) from the global numpy
+        # RNG in every kick (sc.py:174-175, 184-185); no kick, no draw
+        draws = np.array([sc._draws()[1:] for _ in range(nsteps)], dtype=np.float64).reshape(-1)
+        buf = (ctypes.c_double * max(draws.size, 1))(*draws)
+        keep.append(buf)
+        opts.mesh_draws = ctypes.cast(buf, _lib._dp)
+    if lsc is not None:
+        opts.lsc = 1
+        opts.lsc_dz = float(dz)
+        opts.lsc_step_profile = int(bool(lsc.step_profile))
+        opts.lsc_smooth_param = float(lsc.smooth_param)
+        opts.lsc_b0, opts.lsc_b1 = (float(v) for v in lsc.bounds)
+        if lsc._is_undul_in_beam_line and float(dz) >= 1e-10:
+            start = float(p_array.s if z0 is None else z0)
+            vals = []
+            for k in range(nsteps):
+                lsc.z0 = start + (k + 1) * float(L)
+                K_max, fill_factor = lsc.kick_args(float(dz))
+                vals.append(1 + 0.5 * K_max * K_max * fill_factor)   # as the impedance factor is formed
+            fac = (ctypes.c_double * max(len(vals), 1))(*vals)
+            keep.append(fac)
+            opts.lsc_und_factors = ctypes.cast(fac, _lib._dp)
+    nout = 27
+    if calc_tws:
+        flags, disp = 0, (0.0, 0.0, 0.0, 0.0)
         if bounds is not None:
             flags |= 1
-            b0, b1 = float(bounds[0]), float(bounds[1])
+            opts.mom_b0, opts.mom_b1 = float(bounds[0]), float(bounds[1])
             if slice == "Imax":
                 flags |= 8
         if tws_i is not None:
@@ -336,16 +327,22 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
             disp = (tws_i.Dx, tws_i.Dxp, tws_i.Dy, tws_i.Dyp)
         elif auto_disp:
             flags |= 2
-        nout = 27
-        out = (ctypes.c_double * ((int(nsteps) + 1) * nout))()
-        dbuf = (ctypes.c_double * 4)(*disp)
-        _lib.check(lib.ocsc_track_drift_tws(_lib.context(dev).handle, r.data_ptr(), r.stride(0),
-                                            p_array.q_array.data_ptr(), n, sc._nmesh(), float(p_array.E),
-                                            float(dz), float(L), int(nsteps), sc._prec(), 1 if use_graph else 0,
-                                            flags, dbuf, b0, b1, out, stream))
-    rows = np.frombuffer(out, dtype=np.float64).reshape(int(nsteps) + 1, nout)
+        opts.mom_flags = flags
+        for k in range(4):
+            opts.disp[k] = float(disp[k])
+        out = (ctypes.c_double * ((nsteps + 1) * nout))()
+        opts.tws_out = ctypes.cast(out, _lib._dp)
+    with torch.cuda.device(dev):
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        _lib.check(lib.ocsc_track(_lib.context(dev).handle, r.data_ptr(), r.stride(0), p_array.q_array.data_ptr(), n,
+                                  nmesh, float(p_array.E), zstep, float(L), nsteps, prec, ctypes.byref(opts),
+                                  stream))
+    if not calc_tws:
+        return None
+    from .envelope import twiss_from_moments
+    rows = np.frombuffer(out, dtype=np.float64).reshape(nsteps + 1, nout)
     tws, s = [], 0.0
-    for k in range(int(nsteps) + 1):
+    for k in range(nsteps + 1):
         t = twiss_from_moments(rows[k], p_array.E, tws_i is not None or auto_disp)
         if k:
             s += float(L)  # track.py:481-483 accumulates the step lengths

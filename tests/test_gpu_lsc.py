@@ -342,3 +342,39 @@ def test_lsc_tracking_through_undulator_equals_step_by_step(step):
     assert max(factors) > 0 and min(factors) == 0    # the steps cross into the undulator
     assert la.z0 == pytest.approx(s0 + nsteps * L)
     np.testing.assert_array_equal(a.rparticles.cpu().numpy(), b.rparticles.cpu().numpy())
+
+
+def test_tracking_sc_lsc_and_twiss_in_one_loop():
+    """track()'s usual combination, SpaceCharge + LSC with calc_tws, in one
+    captured loop (ocsc_track): the particles equal the same loop without
+    moments bit for bit, and equal the eager sequence drift, SpaceCharge.apply,
+    LSC.apply (unsorted copy); each Twiss row equals get_envelope of the eager
+    bunch after that step."""
+    from ocelot_amd import LSC, SpaceCharge, drift, get_envelope, track_drift
+    from ocelot_amd import _lib
+    print("[sc+lsc+tws tracking]", flush=True)
+    r, q = _bunch(300_000, "flat", 13)
+    E, L, dz, nsteps = 0.13, 0.05, 0.05, 5
+    sc, lsc = SpaceCharge(nmesh_xyz=[31, 31, 31]), LSC()
+    dev = torch.cuda.current_device()
+    old = _lib.get_option(dev, "sort_every")
+    _lib.set_option(dev, "sort_every", 0)
+    try:
+        a, b, e = _resident(r, q, E), _resident(r, q, E), _resident(r, q, E)
+        tws = track_drift(a, sc, L, dz, nsteps, calc_tws=True, lsc=lsc, bounds=[-0.5, 0.5])
+        track_drift(b, sc, L, dz, nsteps, lsc=lsc)
+        ref = [get_envelope(e, bounds=[-0.5, 0.5])]
+        for _ in range(nsteps):
+            drift(e, L)
+            sc.apply(e, dz)
+            lsc.apply(e, dz)
+            ref.append(get_envelope(e, bounds=[-0.5, 0.5]))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a.rparticles.cpu().numpy(), b.rparticles.cpu().numpy())
+        np.testing.assert_array_equal(a.rparticles.cpu().numpy(), e.rparticles.cpu().numpy())
+        assert len(tws) == nsteps + 1
+        for t, rf in zip(tws, ref):
+            for k in ("emit_x", "emit_y", "beta_x", "alpha_y", "xx", "pp", "tautau"):
+                assert float(getattr(t, k)) == pytest.approx(float(getattr(rf, k)), rel=1e-12, abs=0)
+    finally:
+        _lib.set_option(dev, "sort_every", old)
