@@ -15,7 +15,9 @@
 //      rules (ends held, exact grid hits) (sc.py:589-600).
 //
 // Every sum goes through fixed reduction trees: results do not depend on
-// launch timing.
+// launch timing.  The kernels after the statistics read one LscPlan, formed
+// either on the host (one-shot kicks: any grid size, one synchronisation) or
+// by k_lsc_plan on the device (kicks inside captured tracking graphs).
 #pragma once
 #include "particles.hpp"
 
@@ -503,9 +505,8 @@ __global__ void __launch_bounds__(1024) k_lsc_profile(const long long* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Inside tracking graphs: state set-up, the plan on the device, and the
-// current histogram on a grid read from the plan (one LDS window of at most
-// kCicLds bins).
+// Inside tracking graphs: state set-up and the plan on the device (the
+// histogram on the plan's grid is moments.hpp k_cic_grid).
 __global__ void k_lsc_init(LscState* __restrict__ s, int64_t n, double b0, double b1) {
   if (threadIdx.x != 0) return;
   LscState h{};
