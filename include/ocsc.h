@@ -151,9 +151,10 @@ int ocsc_track_drift_tws(ocsc_ctx* ctx, double* rparticles, int64_t ld, const do
  * size, [1] std(tau), [2] profile bins, [3] bin width, [4] first bin
  * position, [5] sum of q, [6] particles in the slice, [7] mean(tau).
  * OCSC_EVALUE where the reference raises (empty bunch, no spread in tau,
- * non-finite tau, empty slice with the step profile).  Synchronises once
- * (after the slice statistics, to size the profile grid); the kick itself is
- * left queued on stream. */
+ * non-finite tau, empty slice with the step profile); the bunch is then not
+ * kicked.  The profile grid is planned on the device (up to 4096 bins;
+ * larger grids are planned again on the host from the same statistics);
+ * synchronises once, at the end. */
 #define OCSC_LSC_NINFO 8
 int ocsc_lsc_apply(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n, double E_GeV,
                    double dz, int step_profile, double smooth_param, double b0, double b1, double K_max,

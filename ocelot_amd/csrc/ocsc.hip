@@ -1913,30 +1913,37 @@ int ocsc_lsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t 
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
   int rc;
-  if ((rc = lsc_buffers(c, 0))) return rc;
+  if ((rc = lsc_buffers(c, kLscGraphBins))) return rc;
+  if ((rc = ensure(c->lsc_cic, ((size_t)kCicBlocks * kLscGraphBins + kLscGraphBins) * sizeof(long long)))) return rc;
   if ((rc = ensure(c->lane0.part, (size_t)12 * nred_blocks(c, n) * sizeof(double)))) return rc;
-  HIPCHK(hipStreamSynchronize(st));  // the pinned mirror may still be read by an earlier copy
+  // the whole kick with its grid planned on the device (no synchronisation
+  // in the middle); a profile that needs more bins than that path holds is
+  // planned again on the host from the same statistics (the bunch is not
+  // kicked by a refused plan)
+  k_counter_reset<<<1, 64, 0, st>>>(c->lsc_err);
+  if ((rc = enqueue_lsc(c, r, ld, q, n,
+                        lsc_const(step_profile, smooth_param, E_GeV, dz, K_max, fill_factor, kLscGraphBins), b0, b1,
+                        st)))
+    return rc;
   LscHost& H = *c->lsc_host;
-  std::memset(&H, 0, sizeof(H));
-  H.s.n = n;
-  H.s.b0 = b0;
-  H.s.b1 = b1;
-  HIPCHK(hipMemcpyAsync(c->lsc, &H.s, sizeof(H.s), hipMemcpyHostToDevice, st));
-  const Rows R{r, ld};
-  lsc_stats(c, R, q, n, step_profile, st);
-  LAUNCHCHK("k_lsc_stat");
   HIPCHK(hipMemcpyAsync(&H.s, c->lsc, sizeof(H.s), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&H.p, c->lsc_plan, sizeof(H.p), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  // the grid size depends on the data: plan on the host
-  const LscConst k = lsc_const(step_profile, smooth_param, E_GeV, dz, K_max, fill_factor, kLscMaxBins);
-  lsc_make_plan(H.s, k, &H.p);
-  if (H.p.err) return fail(OCSC_EVALUE, lsc_err_text(H.p.err));
+  if (H.p.err == kLscTooManyBins) {
+    const LscConst k = lsc_const(step_profile, smooth_param, E_GeV, dz, K_max, fill_factor, kLscMaxBins);
+    lsc_make_plan(H.s, k, &H.p);
+    if (H.p.err) return fail(OCSC_EVALUE, lsc_err_text(H.p.err));
+    const int nb = H.p.nbins;
+    if ((rc = lsc_buffers(c, nb))) return rc;
+    HIPCHK(hipMemcpyAsync(c->lsc_plan, &H.p, sizeof(H.p), hipMemcpyHostToDevice, st));
+    long long* tot = nullptr;
+    if ((rc = cic_profile(c->lsc_cic, Rows{r, ld}, n, H.p.a, H.p.ds, nb, st, &tot))) return rc;
+    if ((rc = lsc_from_profile(c, r, ld, n, tot, nb, st))) return rc;
+    HIPCHK(hipStreamSynchronize(st));  // the plan is read from pinned memory
+  } else if (H.p.err) {
+    return fail(OCSC_EVALUE, lsc_err_text(H.p.err));
+  }
   const int nbins = H.p.nbins;
-  if ((rc = lsc_buffers(c, nbins))) return rc;
-  HIPCHK(hipMemcpyAsync(c->lsc_plan, &H.p, sizeof(H.p), hipMemcpyHostToDevice, st));
-  long long* tot = nullptr;
-  if ((rc = cic_profile(c->lsc_cic, R, n, H.p.a, H.p.ds, nbins, st, &tot))) return rc;
-  if ((rc = lsc_from_profile(c, r, ld, n, tot, nbins, st))) return rc;
   c->lsc_nbins = nbins;
   c->lsc_a = H.p.a;
   c->lsc_ds = H.p.ds;

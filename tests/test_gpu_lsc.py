@@ -378,3 +378,21 @@ def test_tracking_sc_lsc_and_twiss_in_one_loop():
                 assert float(getattr(t, k)) == pytest.approx(float(getattr(rf, k)), rel=1e-12, abs=0)
     finally:
         _lib.set_option(dev, "sort_every", old)
+
+
+def test_lsc_apply_long_tail_takes_host_planned_grid():
+    """A far outlier stretches the profile beyond the device-planned capacity
+    (4096 bins): the kick is planned again on the host from the same
+    statistics and still matches the oracle."""
+    from ocelot_amd import LSC, DeviceParticleArray
+    r, q = _bunch(500_000, "gauss", 17)
+    r[4, 0] = 3e-3          # 150 std away
+    ref, dbg = r.copy(), {}
+    lo.apply(ref, q, 1.5, 0.4, debug=dbg)
+    assert len(dbg["B"]) > 4096
+    d = DeviceParticleArray(torch.from_numpy(r.copy()).cuda(), torch.from_numpy(q).cuda(), 1.5)
+    lsc = LSC()
+    lsc.apply(d, 0.4)
+    assert lsc.last_info["nbins"] == len(dbg["B"])
+    got = d.rparticles.cpu().numpy()
+    assert rel(got[5] - r[5], ref[5] - r[5]) < 1e-10
