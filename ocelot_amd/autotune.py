@@ -14,7 +14,10 @@ graph-captured steps on a copy of the bunch.  ``deposit_xcd`` and ``overlap``
 must leave the particles bit-identical (checked); ``sort_every`` changes only
 reduction orders.  Results are cached per (n, nmesh, precision, device), so a
 second call for the same shape only re-applies the options.
+
+``SpaceCharge(autotune=True)`` runs this at the start of ``track_drift``.
 """
+import copy
 import time
 
 from . import _lib
@@ -62,6 +65,11 @@ def tune_tracking(p_array, sc, L, dz, steps=30, space=None, check_identical=True
         return out
     space = dict(DEFAULT_SPACE if space is None else space)
     start = {k: _lib.get_option(dev, k) for k in space}
+    # the trial runs take no random-mesh draws from the global RNG (the
+    # caller's kicks keep their draws) and do not tune again
+    sc = copy.copy(sc)
+    sc.random_mesh = False
+    sc.autotune = False
 
     def measure(opts):
         for k, v in opts.items():

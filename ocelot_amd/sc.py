@@ -5,10 +5,11 @@ Same surface as the reference class (ocelot/cpbd/sc.py:76-258): constructor
 random_mesh=False)``, attributes mutable after construction, ``prepare(lat)``
 seeding the global numpy RNG with ``random_seed`` (10), ``apply(p_array,
 zstep)`` rewriting all six rows of ``p_array.rparticles`` in place, and the
-helper methods ``sym_kernel``, ``potential``, ``el_field``.  Two extra
+helper methods ``sym_kernel``, ``potential``, ``el_field``.  Three extra
 keyword arguments default to the reference behaviour: ``precision`` ("fp64"
-or "mixed") and ``device`` (GPU index; default: the current torch device or
-the device of a resident bunch).
+or "mixed"), ``autotune`` (False; True: ``track_drift`` tunes the speed-only
+tracking options for the bunch shape first) and ``device`` (GPU index;
+default: the current torch device or the device of a resident bunch).
 
 ``p_array.rparticles`` may be a numpy array (it is copied to the GPU, kicked
 and copied back into the same ndarray, so views such as ``p_array.x()`` stay
@@ -65,6 +66,7 @@ class SpaceCharge(PhysProc):
         self.random_mesh = kwargs.get("random_mesh", False)
         self.random_seed = 10
         self.precision = kwargs.get("precision", "fp64")
+        self.autotune = kwargs.get("autotune", False)
         self.device = kwargs.get("device", None)
 
     # ------------------------------------------------------------------ helpers
@@ -274,7 +276,11 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
     the end position.
 
     With ``sc.random_mesh`` each kick takes the reference's two global numpy
-    draws (scale, then shift), all taken before the call in kick order."""
+    draws (scale, then shift), all taken before the call in kick order.
+
+    With ``sc.autotune`` the speed-only options of the tracking loop are first
+    tuned for this bunch size, mesh and precision (ocelot_amd.autotune, on a
+    copy of the bunch, once per shape; no random draws are taken)."""
     torch = _torch()
     lib = _lib.load()
     r = p_array.rparticles
@@ -285,6 +291,9 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
     dev = r.device.index
     n = int(r.shape[1])
     nsteps = int(nsteps)
+    if sc is not None and getattr(sc, "autotune", False) and nsteps > 0 and float(dz) != 0:
+        from .autotune import tune_tracking
+        tune_tracking(p_array, sc, L, dz)
     opts = _lib.TrackOpts()
     opts.use_graph = 1 if use_graph else 0
     zstep = float(dz) if sc is not None else 0.0

@@ -52,3 +52,41 @@ def test_tune_tracking_on_gpu():
     finally:
         for k, v in old.items():
             _lib.set_option(dev, k, v)
+
+
+@pytest.mark.gpu
+def test_spacecharge_autotune_keyword_in_tracking():
+    """SpaceCharge(autotune=True): track_drift tunes first (options set, result
+    cached), takes exactly the random-mesh draws of its own kicks from the
+    global RNG, and tracks like the untuned process up to reduction order."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+    import numpy as np
+    from ocelot_amd import DeviceParticleArray, ParticleArray, SpaceCharge, _lib, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, _, dz = make_bunch("flattop_10m_127", n=200_000, seed=11)
+    p = ParticleArray()
+    p.rparticles, p.q_array, p.E = r, q, E
+    dev = torch.cuda.current_device()
+    old = {k: _lib.get_option(dev, k) for k in autotune.DEFAULT_SPACE}
+    try:
+        outs, states = [], []
+        for tune in (False, True):
+            sc = SpaceCharge(nmesh_xyz=[31, 31, 47], precision="mixed", random_mesh=True, autotune=tune)
+            np.random.seed(3)
+            d = DeviceParticleArray.from_host(p, dev)
+            track_drift(d, sc, dz, dz, 6)
+            outs.append(d.rparticles.cpu().numpy())
+            states.append(np.random.get_state()[1].copy())
+        np.testing.assert_array_equal(states[0], states[1])        # same 12 draws consumed
+        key = (200_000, (31, 31, 47), _lib.OCSC_MIXED, dev)
+        assert key in autotune._cache
+        for k, v in autotune._cache[key]["options"].items():
+            assert _lib.get_option(dev, k) == v
+        a, b = outs
+        for i in range(6):
+            assert np.abs(a[i] - b[i]).max() <= 1e-8 * np.abs(a[i]).max()
+    finally:
+        for k, v in old.items():
+            _lib.set_option(dev, k, v)

@@ -22,7 +22,8 @@ def test_constructor_defaults_match_reference():
     assert sc.random_seed == 10
     assert sc.start_elem is None and sc.end_elem is None and sc.debug is False
     assert sc.indx0 is None and sc.z0 is None and sc.energy is None
-    assert sc.precision == "fp64"
+    assert sc.precision == "fp64" and sc.autotune is False
+    assert repr(SpaceCharge(autotune=True)) == repr(sc)   # the extra keywords leave the repr alone
 
 
 def test_kwargs_and_mutation():
