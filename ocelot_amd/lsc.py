@@ -170,13 +170,19 @@ class LSC(PhysProc):
                                               *args, stream))
         else:
             dev = self._device()
+            direct = (isinstance(r, np.ndarray) and r.dtype == np.float64 and r.flags.c_contiguous
+                      and r.flags.writeable and r.size > 0)
             with torch.cuda.device(dev):
-                rt = torch.from_numpy(np.ascontiguousarray(r, dtype=np.float64)).to(f"cuda:{dev}")
+                host = torch.from_numpy(r if direct else np.ascontiguousarray(r, dtype=np.float64))
+                rt = host.to(f"cuda:{dev}")
                 qt = torch.from_numpy(np.ascontiguousarray(p_array.q_array, dtype=np.float64)).to(f"cuda:{dev}")
                 stream = torch.cuda.current_stream(dev).cuda_stream
                 _lib.check(lib.ocsc_lsc_apply(_lib.context(dev).handle, rt.data_ptr(), n, qt.data_ptr(), n, *args,
                                               stream))
-                r[5] = rt[5].cpu().numpy()
+                if direct:
+                    host[5].copy_(rt[5])   # only delta changes: one row back, into the caller's ndarray
+                else:
+                    r[5] = rt[5].cpu().numpy()
         self.last_info = dict(sigma=info[0], sigma_tau=info[1], nbins=int(info[2]), ds=info[3], s0=info[4],
                               q=info[5], count=int(info[6]), mean_tau=info[7], K_max=K_max, fill_factor=fill_factor)
 

@@ -43,6 +43,12 @@ def _torch():
     return torch
 
 
+def _host_direct(a):
+    """True for an ndarray the GPU copies can read and write in place."""
+    return (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+            and a.flags.writeable and a.size > 0)
+
+
 def _is_cuda_tensor(a):
     try:
         torch = _torch()
@@ -144,13 +150,21 @@ class SpaceCharge(PhysProc):
             return
         dev = self._device()
         ctx = _lib.context(dev)
+        # a C-ordered, writable float64 ndarray is copied to and from the GPU
+        # as it lies (no host-side temporaries); anything else goes through a
+        # contiguous copy and is assigned back element-wise
+        direct = _host_direct(r)
         with torch.cuda.device(dev):
-            rt = torch.from_numpy(np.ascontiguousarray(r, dtype=np.float64)).to(f"cuda:{dev}")
+            host = torch.from_numpy(r if direct else np.ascontiguousarray(r, dtype=np.float64))
+            rt = host.to(f"cuda:{dev}")
             qt = torch.from_numpy(np.ascontiguousarray(p_array.q_array, dtype=np.float64)).to(f"cuda:{dev}")
             stream = torch.cuda.current_stream(dev).cuda_stream
             _lib.check(lib.ocsc_apply(ctx.handle, rt.data_ptr(), n, qt.data_ptr(), n, nmesh, E, float(zstep),
                                       rm, scale, shift, prec, 1, stream))
-            r[...] = rt.cpu().numpy()
+            if direct:
+                host.copy_(rt)
+            else:
+                r[...] = rt.cpu().numpy()
 
     # ------------------------------------------------------------------ sub-steps
     def sym_kernel(self, ijk2, hxyz):

@@ -268,6 +268,32 @@ def test_deterministic_and_device_resident_equal():
         assert np.array_equal(d.rparticles.cpu().numpy(), a.rparticles), prec
 
 
+def test_host_layouts_write_back_in_place():
+    """Host rparticles in C order (copied to and from the GPU as they lie),
+    Fortran order and a strided view (through a contiguous copy) give the
+    same bits, and views taken before the kick see the new values, as after
+    the reference's row assignments (coord_transform.py:43-53)."""
+    from ocelot_amd import SpaceCharge
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=100_000, seed=8)
+    sc = SpaceCharge(nmesh_xyz=[31, 31, 47])
+    a = parray(r, q, E)
+    x_view = a.rparticles[0]
+    sc.apply(a, dz)
+    assert not np.array_equal(r[0], a.rparticles[0])
+    assert np.shares_memory(x_view, a.rparticles) and np.array_equal(x_view, a.rparticles[0])
+    f = parray(r, q, E)
+    f.rparticles = np.asfortranarray(r)
+    sc.apply(f, dz)
+    assert f.rparticles.flags.f_contiguous and np.array_equal(f.rparticles, a.rparticles)
+    wide = np.zeros((6, 2 * r.shape[1]))
+    wide[:, ::2] = r
+    v = parray(r, q, E)
+    v.rparticles = wide[:, ::2]
+    sc.apply(v, dz)
+    assert np.array_equal(wide[:, ::2], a.rparticles) and not wide[:, 1::2].any()
+
+
 def test_kernel_cache_hit_gives_identical_result():
     from ocelot_amd import SpaceCharge
     from ocelot_amd.workloads import make_bunch
