@@ -129,9 +129,15 @@ def cpu_baseline(name, n_sample, steps=3, seed=0):
             dt = run()
     else:
         dt = run()
+    import importlib.util
+    import scipy
+    host = dict(cpu_count=os.cpu_count(), blas_threads=1 if threadpool_limits is not None else None,
+                fft="numpy.fft (the reference's fallback branch)",
+                pyfftw_installed=importlib.util.find_spec("pyfftw") is not None,
+                numpy=np.__version__, scipy=scipy.__version__)
     return dict(value=n_sample * steps / dt, unit="macroparticle-kicks/s", cores=1, kind="port",
                 sample=f"{name}: {steps} FP64 oracle steps (drift {dz} m + kick) of {n_sample} particles on "
-                       f"{nmesh[0]}^3, {dt:.2f} s")
+                       f"{nmesh[0]}^3, {dt:.2f} s", host=host)
 
 
 def other_configs(torch, dev):
