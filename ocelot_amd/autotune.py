@@ -52,12 +52,12 @@ def tune_tracking(p_array, sc, L, dz, steps=30, space=None, check_identical=True
     {"options": ..., "ms_per_step": ..., "timings_ms": ..., "cached": bool}."""
     import torch
     from .beam import DeviceParticleArray
-    from .sc import track_drift
+    from .sc import _nmesh_of, _prec_of, track_drift
     r = p_array.rparticles
     if not (isinstance(r, torch.Tensor) and r.is_cuda):
         raise TypeError("tune_tracking needs a DeviceParticleArray")
     dev = r.device.index
-    key = (int(r.shape[1]), tuple(sc._nmesh()), sc._prec(), dev)
+    key = (int(r.shape[1]), tuple(_nmesh_of(sc)), _prec_of(sc), dev)
     if key in _cache and not force:
         out = dict(_cache[key], cached=True)
         for k, v in out["options"].items():

@@ -57,6 +57,55 @@ def _is_cuda_tensor(a):
     return isinstance(a, torch.Tensor) and a.is_cuda
 
 
+# ---------------------------------------------------------------------------
+# Host-side kick arguments from the process attributes the reference defines
+# (nmesh_xyz, random_mesh) and the optional precision / device.
+
+def _prec_of(proc):
+    prec = getattr(proc, "precision", "fp64")
+    try:
+        return _PREC[prec]
+    except KeyError:
+        raise ValueError(f"precision must be 'fp64' or 'mixed', not {prec!r}")
+
+
+def _device_of(proc, tensor=None):
+    if tensor is not None:
+        return tensor.device.index
+    dev = getattr(proc, "device", None)
+    if dev is not None:
+        return int(dev)
+    return _torch().cuda.current_device()
+
+
+def _nmesh_of(proc):
+    nm = np.array(proc.nmesh_xyz).reshape(-1)
+    if nm.size != 3:
+        raise ValueError("nmesh_xyz must have three entries")
+    return _lib.i32x3(nm)
+
+
+def _draws_of(proc):
+    # sc.py:174-175 then sc.py:184-185, global numpy RNG
+    if not proc.random_mesh:
+        return 0, 1.0, 0.0
+    scale = np.random.uniform(low=1, high=1.1)
+    shift = np.random.uniform(low=-0.5, high=0.5)
+    return 1, float(scale), float(shift)
+
+
+def _kick_args(proc, p_array, zstep):
+    n = int(p_array.rparticles.shape[1])
+    if n == 0:
+        raise ValueError("zero-size array to reduction operation maximum which has no identity")
+    nmesh = _nmesh_of(proc)
+    prec = _prec_of(proc)
+    E = float(p_array.E)
+    rm, scale, shift = _draws_of(proc)
+    return dict(n=n, nmesh=nmesh, prec=prec, E=E, zstep=float(zstep), random_mesh=rm, mesh_scale=scale,
+                mesh_shift=shift)
+
+
 class SpaceCharge(PhysProc):
     """3D space-charge kick (Poisson solve in the bunch rest frame by Hockney
     FFT convolution with the integrated Green's function)."""
@@ -76,33 +125,19 @@ class SpaceCharge(PhysProc):
         self.device = kwargs.get("device", None)
 
     # ------------------------------------------------------------------ helpers
+    # (module functions below, so the kick also runs on any object with the
+    # reference's attributes, e.g. ocelot's own SpaceCharge instance)
     def _prec(self):
-        try:
-            return _PREC[self.precision]
-        except KeyError:
-            raise ValueError(f"precision must be 'fp64' or 'mixed', not {self.precision!r}")
+        return _prec_of(self)
 
     def _device(self, tensor=None):
-        if tensor is not None:
-            return tensor.device.index
-        if self.device is not None:
-            return int(self.device)
-        torch = _torch()
-        return torch.cuda.current_device()
+        return _device_of(self, tensor)
 
     def _nmesh(self):
-        nm = np.array(self.nmesh_xyz).reshape(-1)
-        if nm.size != 3:
-            raise ValueError("nmesh_xyz must have three entries")
-        return _lib.i32x3(nm)
+        return _nmesh_of(self)
 
     def _draws(self):
-        # sc.py:174-175 then sc.py:184-185, global numpy RNG
-        if not self.random_mesh:
-            return 0, 1.0, 0.0
-        scale = np.random.uniform(low=1, high=1.1)
-        shift = np.random.uniform(low=-0.5, high=0.5)
-        return 1, float(scale), float(shift)
+        return _draws_of(self)
 
     # ------------------------------------------------------------------ PhysProc
     def prepare(self, lat):
@@ -115,27 +150,19 @@ class SpaceCharge(PhysProc):
         consumes them: mesh, precision, energy, then the random-mesh draws.
         Raises ValueError for an empty bunch before drawing (the reference
         fails in the extent reduction, sc.py:173, before its first draw)."""
-        n = int(p_array.rparticles.shape[1])
-        if n == 0:
-            raise ValueError("zero-size array to reduction operation maximum which has no identity")
-        nmesh = self._nmesh()
-        prec = self._prec()
-        E = float(p_array.E)
-        rm, scale, shift = self._draws()
-        return dict(n=n, nmesh=nmesh, prec=prec, E=E, zstep=float(zstep), random_mesh=rm, mesh_scale=scale,
-                    mesh_shift=shift)
+        return _kick_args(self, p_array, zstep)
 
     def apply(self, p_array, zstep):
         if zstep == 0:
             return
-        a = self.kick_args(p_array, zstep)
+        a = _kick_args(self, p_array, zstep)
         torch = _torch()
         lib = _lib.load()
         r = p_array.rparticles
         n, nmesh, prec, E = a["n"], a["nmesh"], a["prec"], a["E"]
         rm, scale, shift = a["random_mesh"], a["mesh_scale"], a["mesh_shift"]
         if _is_cuda_tensor(r):
-            dev = self._device(r)
+            dev = _device_of(self, r)
             if r.dtype != torch.float64 or r.stride(1) != 1 or r.shape[0] != 6:
                 raise TypeError("device rparticles must be a (6, n) float64 tensor with contiguous rows")
             q = p_array.q_array
@@ -148,7 +175,7 @@ class SpaceCharge(PhysProc):
                 _lib.check(lib.ocsc_apply(ctx.handle, r.data_ptr(), r.stride(0), q.data_ptr(), n, nmesh, E,
                                           float(zstep), rm, scale, shift, prec, 1, stream))
             return
-        dev = self._device()
+        dev = _device_of(self)
         ctx = _lib.context(dev)
         # a C-ordered, writable float64 ndarray is copied to and from the GPU
         # as it lies (no host-side temporaries); anything else goes through a
@@ -172,7 +199,7 @@ class SpaceCharge(PhysProc):
         (sc.py:109-133), FP64 numpy array."""
         torch = _torch()
         lib = _lib.load()
-        dev = self._device()
+        dev = _device_of(self)
         shape = tuple(int(v) for v in ijk2)
         out = torch.empty(shape, dtype=torch.float64, device=f"cuda:{dev}")
         ctx = _lib.context(dev)
@@ -187,8 +214,8 @@ class SpaceCharge(PhysProc):
         (FP32 values widened in mixed precision)."""
         torch = _torch()
         lib = _lib.load()
-        dev = self._device()
-        prec = self._prec()
+        dev = _device_of(self)
+        prec = _prec_of(self)
         rho = torch.from_numpy(np.ascontiguousarray(q, dtype=np.float64)).to(f"cuda:{dev}")
         dt = torch.float64 if prec == _lib.OCSC_FP64 else torch.float32
         phi = torch.empty(rho.shape, dtype=dt, device=rho.device)
@@ -206,8 +233,8 @@ class SpaceCharge(PhysProc):
         random-mesh draws when random_mesh is on."""
         torch = _torch()
         lib = _lib.load()
-        dev = self._device()
-        prec = self._prec()
+        dev = _device_of(self)
+        prec = _prec_of(self)
         n = X.shape[0]
         if n == 0:
             raise ValueError("zero-size array to reduction operation maximum which has no identity")
@@ -215,7 +242,7 @@ class SpaceCharge(PhysProc):
         qt = torch.from_numpy(np.ascontiguousarray(Q, dtype=np.float64)).to(f"cuda:{dev}")
         E = torch.empty((3, n), dtype=torch.float64, device=xyz.device)
         cells = torch.empty(n, dtype=torch.int32, device=xyz.device) if return_cells else None
-        rm, scale, shift = self._draws()
+        rm, scale, shift = _draws_of(self)
         nm = _lib.i32x3(np.array(nxyz).reshape(-1))
         ctx = _lib.context(dev)
         with torch.cuda.device(dev):
@@ -233,7 +260,7 @@ class SpaceCharge(PhysProc):
         """Scalars of the last kick on the device: gamma0, Pav, T, steps, X_off,
         total charge, error bits, kernel-rebuild flag."""
         lib = _lib.load()
-        dev = self._device() if device is None else int(device)
+        dev = _device_of(self) if device is None else int(device)
         out = (ctypes.c_double * 20)()
         _lib.check(lib.ocsc_last_stats(_lib.context(dev).handle, out, 20, None))
         v = list(out)
@@ -311,13 +338,13 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
     opts = _lib.TrackOpts()
     opts.use_graph = 1 if use_graph else 0
     zstep = float(dz) if sc is not None else 0.0
-    nmesh = sc._nmesh() if sc is not None else _lib.i32x3(np.array([4, 4, 4]))
-    prec = sc._prec() if sc is not None else _lib.OCSC_FP64
+    nmesh = _nmesh_of(sc) if sc is not None else _lib.i32x3(np.array([4, 4, 4]))
+    prec = _prec_of(sc) if sc is not None else _lib.OCSC_FP64
     keep = []
     if sc is not None and sc.random_mesh and zstep != 0:
         # the reference draws U(1, 1.1) then U(-0.5, 0.5) from the global numpy
         # RNG in every kick (sc.py:174-175, 184-185); no kick, no draw
-        draws = np.array([sc._draws()[1:] for _ in range(nsteps)], dtype=np.float64).reshape(-1)
+        draws = np.array([_draws_of(sc)[1:] for _ in range(nsteps)], dtype=np.float64).reshape(-1)
         buf = (ctypes.c_double * max(draws.size, 1))(*draws)
         keep.append(buf)
         opts.mesh_draws = ctypes.cast(buf, _lib._dp)

@@ -294,6 +294,23 @@ def test_host_layouts_write_back_in_place():
     assert np.array_equal(wide[:, ::2], a.rparticles) and not wide[:, 1::2].any()
 
 
+def test_apply_on_a_foreign_process_object():
+    """SpaceCharge.apply called on an object that has only the reference
+    class's attributes (the maintainer's delegation from ocelot's own
+    SpaceCharge) kicks exactly as the drop-in instance, random draws included."""
+    import types
+    from ocelot_amd import SpaceCharge
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("gauss_200k_63", n=50_000, seed=2)
+    for rm in (False, True):
+        a, b = parray(r, q, E), parray(r, q, E)
+        np.random.seed(9)
+        SpaceCharge(nmesh_xyz=[31, 31, 31], random_mesh=rm).apply(a, dz)
+        np.random.seed(9)
+        SpaceCharge.apply(types.SimpleNamespace(nmesh_xyz=[31, 31, 31], random_mesh=rm), b, dz)
+        assert np.array_equal(a.rparticles, b.rparticles), rm
+
+
 def test_kernel_cache_hit_gives_identical_result():
     from ocelot_amd import SpaceCharge
     from ocelot_amd.workloads import make_bunch

@@ -213,3 +213,20 @@ def test_lsc_survives_navigator_copies():
         assert (other.step, other.step_profile, other.smooth_param, other.bounds) == (2, True, 0.2, [-0.4, 0.4])
         assert other._is_undul_in_beam_line
         assert other.kick_args(0.1) == lsc.kick_args(0.1)
+
+
+def test_kick_args_on_a_foreign_process_object():
+    """The drop-in's apply reads only the reference's attributes (nmesh_xyz,
+    random_mesh; precision and device when present), so a maintainer can call
+    it on Ocelot's own SpaceCharge instance: kick_args on a plain namespace
+    gives the instance's arguments and takes the same global draws."""
+    import types
+    ns = types.SimpleNamespace(nmesh_xyz=[31, 47, 63], random_mesh=True)
+    sc = SpaceCharge(nmesh_xyz=[31, 47, 63], random_mesh=True)
+    p = types.SimpleNamespace(rparticles=np.zeros((6, 10)), E=0.13)
+    np.random.seed(4)
+    a = SpaceCharge.kick_args(ns, p, 0.1)
+    np.random.seed(4)
+    b = sc.kick_args(p, 0.1)
+    assert list(a.pop("nmesh")) == list(b.pop("nmesh")) == [31, 47, 63]
+    assert a == b and a["prec"] == 0 and a["random_mesh"] == 1
