@@ -40,6 +40,31 @@ def _is_cuda_tensor(a):
     return isinstance(a, torch.Tensor) and a.is_cuda
 
 
+def _device_of(proc, tensor=None):
+    if tensor is not None:
+        return tensor.device.index
+    dev = getattr(proc, "device", None)
+    if dev is not None:
+        return int(dev)
+    return _torch().cuda.current_device()
+
+
+def _filling_factor(K_s_func, x0, x1, num_points):
+    x = np.linspace(x0, x1, num_points)
+    k_vals = K_s_func(x)
+    nonzero_intervals = (k_vals[:-1] != 0) | (k_vals[1:] != 0)
+    dx = (x1 - x0) / (num_points - 1)
+    return np.sum(nonzero_intervals) * dx / (x1 - x0)
+
+
+def _kick_args(proc, dz):
+    if getattr(proc, "_is_undul_in_beam_line", False):
+        fill_factor = _filling_factor(proc.K_s_func, proc.z0 - dz, proc.z0, 200)
+        K_max = np.max(proc.K_s_func(np.linspace(proc.z0 - dz, proc.z0, num=200)))
+        return float(K_max), float(fill_factor)
+    return 0.0, 0.0
+
+
 def _is_undulator(elem):
     return any(k.__name__ == "Undulator" for k in type(elem).__mro__)
 
@@ -60,11 +85,7 @@ class LSC(PhysProc):
 
     # ------------------------------------------------------------------ helpers
     def _device(self, tensor=None):
-        if tensor is not None:
-            return tensor.device.index
-        if self.device is not None:
-            return int(self.device)
-        return _torch().cuda.current_device()
+        return _device_of(self, tensor)
 
     def imp_lsc(self, gamma, sigma, w, dz):
         """Impedance of a round Gaussian beam of rms size sigma over dz
@@ -80,7 +101,7 @@ class LSC(PhysProc):
         lib = _lib.load()
         w = np.ascontiguousarray(np.asarray(w, dtype=np.float64).reshape(-1))
         out = np.empty_like(w)
-        dev = self._device()
+        dev = _device_of(self)
         torch = _torch()
         with torch.cuda.device(dev):
             _lib.check(lib.ocsc_lsc_impedance(_lib.context(dev).handle, w.ctypes.data_as(_lib._dp), w.size,
@@ -97,7 +118,7 @@ class LSC(PhysProc):
         if s.shape != b.shape or s.ndim != 1:
             raise ValueError("s and bunch must be 1D arrays of one length")
         out = np.empty_like(b)
-        dev = self._device()
+        dev = _device_of(self)
         torch = _torch()
         with torch.cuda.device(dev):
             _lib.check(lib.ocsc_lsc_wake(_lib.context(dev).handle, s.ctypes.data_as(_lib._dp),
@@ -130,24 +151,19 @@ class LSC(PhysProc):
     def compute_filling_factor(self, x0, x1, num_points=100):
         """Fraction of [x0, x1] on num_points samples where K is nonzero at
         either end of a sample interval (sc.py:505-536)."""
-        x = np.linspace(x0, x1, num_points)
-        k_vals = self.K_s_func(x)
-        nonzero_intervals = (k_vals[:-1] != 0) | (k_vals[1:] != 0)
-        dx = (x1 - x0) / (num_points - 1)
-        return np.sum(nonzero_intervals) * dx / (x1 - x0)
+        return _filling_factor(self.K_s_func, x0, x1, num_points)
 
     def kick_args(self, dz):
         """(K_max, fill_factor) of the step ending at z0 (sc.py:559-564)."""
-        if self._is_undul_in_beam_line:
-            fill_factor = self.compute_filling_factor(self.z0 - dz, self.z0, num_points=200)
-            K_max = np.max(self.K_s_func(np.linspace(self.z0 - dz, self.z0, num=200)))
-            return float(K_max), float(fill_factor)
-        return 0.0, 0.0
+        return _kick_args(self, dz)
 
     def apply(self, p_array, dz):
+        """LSC energy kick on rparticles[5] in place (sc.py:538-600).  Reads
+        only attributes the reference class defines, so it also runs on
+        Ocelot's own LSC instance."""
         if dz < 1e-10:
             return
-        K_max, fill_factor = self.kick_args(dz)
+        K_max, fill_factor = _kick_args(self, dz)
         torch = _torch()
         lib = _lib.load()
         r = p_array.rparticles
@@ -157,7 +173,7 @@ class LSC(PhysProc):
         args = (float(p_array.E), float(dz), int(bool(self.step_profile)), float(self.smooth_param), b0, b1,
                 K_max, fill_factor, info, 8)
         if _is_cuda_tensor(r):
-            dev = self._device(r)
+            dev = _device_of(self, r)
             if r.dtype != torch.float64 or r.stride(1) != 1 or r.shape[0] != 6:
                 raise TypeError("device rparticles must be a (6, n) float64 tensor with contiguous rows")
             q = p_array.q_array
@@ -169,7 +185,7 @@ class LSC(PhysProc):
                 _lib.check(lib.ocsc_lsc_apply(_lib.context(dev).handle, r.data_ptr(), r.stride(0), q.data_ptr(), n,
                                               *args, stream))
         else:
-            dev = self._device()
+            dev = _device_of(self)
             direct = (isinstance(r, np.ndarray) and r.dtype == np.float64 and r.flags.c_contiguous
                       and r.flags.writeable and r.size > 0)
             with torch.cuda.device(dev):
@@ -190,7 +206,7 @@ class LSC(PhysProc):
         """Grid (m), current (A) and wake W (V) of the last kick on this
         device: the arrays x, B[:, 1] and W of the reference's apply."""
         lib = _lib.load()
-        dev = self._device() if device is None else int(device)
+        dev = _device_of(self) if device is None else int(device)
         h = _lib.context(dev).handle
         nb = ctypes.c_int64(0)
         rc = lib.ocsc_lsc_profile(h, None, None, None, 0, ctypes.byref(nb))

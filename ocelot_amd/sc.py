@@ -354,12 +354,13 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
         opts.lsc_step_profile = int(bool(lsc.step_profile))
         opts.lsc_smooth_param = float(lsc.smooth_param)
         opts.lsc_b0, opts.lsc_b1 = (float(v) for v in lsc.bounds)
-        if lsc._is_undul_in_beam_line and float(dz) >= 1e-10:
+        if getattr(lsc, "_is_undul_in_beam_line", False) and float(dz) >= 1e-10:
+            from .lsc import _kick_args as _lsc_kick_args
             start = float(p_array.s if z0 is None else z0)
             vals = []
             for k in range(nsteps):
                 lsc.z0 = start + (k + 1) * float(L)
-                K_max, fill_factor = lsc.kick_args(float(dz))
+                K_max, fill_factor = _lsc_kick_args(lsc, float(dz))
                 vals.append(1 + 0.5 * K_max * K_max * fill_factor)   # as the impedance factor is formed
             fac = (ctypes.c_double * max(len(vals), 1))(*vals)
             keep.append(fac)

@@ -67,6 +67,20 @@ def test_lsc_matches_reference_golden(lg, case, step):
     np.testing.assert_array_equal(p.rparticles[:5], r[:5])
 
 
+def test_lsc_apply_on_a_foreign_process_object(lg):
+    """LSC.apply called on an object with only the reference class's
+    attributes (Ocelot's own LSC delegating to the GPU) kicks as the drop-in
+    instance does, bit for bit."""
+    import types
+    from ocelot_amd import LSC
+    r, q, E = lg["gauss/r"], lg["gauss/q"], float(lg["gauss/E"])
+    a, b = parray(r, q, E), parray(r, q, E)
+    LSC(step_profile=True, smooth_param=0.2).apply(a, 0.5)
+    ns = types.SimpleNamespace(step_profile=True, smooth_param=0.2, bounds=[-0.4, 0.4], _is_undul_in_beam_line=False)
+    LSC.apply(ns, b, 0.5)
+    assert np.array_equal(a.rparticles, b.rparticles) and ns.last_info["nbins"] > 0
+
+
 class _Elem:
     def __init__(self, l):
         self.l = l

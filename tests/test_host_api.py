@@ -188,6 +188,24 @@ def test_lsc_undulator_profile_and_filling_factor():
         assert lsc.kick_args(dz) == (K, ff)
 
 
+def test_lsc_kick_args_on_a_foreign_process_object():
+    """LSC.kick_args reads only the reference class's attributes (K_s_func,
+    _is_undul_in_beam_line, z0): on a plain namespace holding a prepared
+    instance's values it gives the golden K_max and filling factor."""
+    import types
+    from ocelot_amd import LSC
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "lsc_golden.npz"))
+    m1, m2, line = _lsc_line()
+    lsc = LSC()
+    lsc.start_elem, lsc.end_elem, lsc.s_start = m1, m2, 0.0
+    lsc.prepare(line)
+    ns = types.SimpleNamespace(K_s_func=lsc.K_s_func, _is_undul_in_beam_line=True, z0=None)
+    for (z0, dz), ff, K in zip(g["und/steps"], g["und/fill"], g["und/Kmax"]):
+        ns.z0 = z0
+        assert LSC.kick_args(ns, dz) == (K, ff)
+    assert LSC.kick_args(types.SimpleNamespace(_is_undul_in_beam_line=False), 0.1) == (0.0, 0.0)
+
+
 def test_lsc_two_plane_undulator_is_ignored():
     from ocelot_amd import LSC
     m1, m2 = _Elem(0.0), _Elem(0.0)
