@@ -407,59 +407,85 @@ __device__ inline double lsc_grid(int j, double a, double ds) {
 // delta += interp(tau, grid, W) 1e-9 / pc_ref, numpy.interp's rules; with
 // DRIFT, then the next step's drift x += L x', y += L y', tau += r56 delta
 // (the same fused multiply-adds as the drift kernel).  A refused plan
-// (nbins == 0) leaves delta alone but still drifts.
-template <bool DRIFT>
+// (nbins == 0) leaves delta alone but still drifts.  With SUMP, per-block
+// sums of the momenta of the written rows for the next space-charge kick,
+// exactly as k_passA would form them (the drift keeps x', y', delta).
+template <bool DRIFT, bool SUMP = false>
 __global__ void __launch_bounds__(kPT) k_lsc_kick(double* __restrict__ r, int64_t ld, int64_t n,
                                                   const double* __restrict__ W, const LscPlan* __restrict__ p,
-                                                  double L, double r56) {
+                                                  double L, double r56, const KickScalars* __restrict__ scp = nullptr,
+                                                  double* __restrict__ part = nullptr) {
 #pragma clang fp contract(off)
+  __shared__ double lds[SUMP ? (kPT / 64) * 3 : 1];
   const int nbins = p->nbins;
+  double acc[3] = {0, 0, 0};
+  KickScalars ks{};
+  if constexpr (SUMP) ks = *scp;
+  // SUMP: the next space-charge kick's momentum sums over the written rows,
+  // in k_passA's grid, order and partial layout (same result bit for bit)
+  auto sump = [&](int64_t i, double d) {
+    if constexpr (SUMP) {
+      double m[3];
+      to_p(r[ld + i], r[3 * ld + i], d, ks, m);
+      acc[0] += m[0];
+      acc[1] += m[1];
+      acc[2] += m[2];
+    }
+  };
   if (nbins == 0) {
-    if (DRIFT)
-      for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
+    for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
+      if (DRIFT) {
         r[i] = __builtin_fma(L, r[ld + i], r[i]);
         r[2 * ld + i] = __builtin_fma(L, r[3 * ld + i], r[2 * ld + i]);
         r[4 * ld + i] = __builtin_fma(r56, r[5 * ld + i], r[4 * ld + i]);
       }
-    return;
-  }
-  const double a = p->a, ds = p->ds, pc_ref = p->pc_ref;
-  const double x0 = lsc_grid(0, a, ds), xl = lsc_grid(nbins - 1, a, ds);
-  for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
-    const double tau = r[4 * ld + i];
-    double dE;
-    if (tau != tau) {
-      dE = tau;
-    } else if (tau < x0) {
-      dE = W[0];
-    } else if (tau >= xl) {
-      dE = W[nbins - 1];
-    } else {
-      // x_j <= tau < x_{j+1}, 0 <= j <= nbins - 2
-      double fj = __builtin_floor((tau - a) / ds);
-      int j = fj < 0 ? 0 : (fj > nbins - 2 ? nbins - 2 : (int)fj);
-      while (j > 0 && lsc_grid(j, a, ds) > tau) --j;
-      while (j < nbins - 2 && lsc_grid(j + 1, a, ds) <= tau) ++j;
-      const double xj = lsc_grid(j, a, ds);
-      if (xj == tau) {
-        dE = W[j];
+      sump(i, r[5 * ld + i]);
+    }
+  } else {
+    const double a = p->a, ds = p->ds, pc_ref = p->pc_ref;
+    const double x0 = lsc_grid(0, a, ds), xl = lsc_grid(nbins - 1, a, ds);
+    for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
+      const double tau = r[4 * ld + i];
+      double dE;
+      if (tau != tau) {
+        dE = tau;
+      } else if (tau < x0) {
+        dE = W[0];
+      } else if (tau >= xl) {
+        dE = W[nbins - 1];
       } else {
-        const double xj1 = lsc_grid(j + 1, a, ds);
-        const double slope = (W[j + 1] - W[j]) / (xj1 - xj);
-        dE = slope * (tau - xj) + W[j];
-        if (dE != dE) {
-          dE = slope * (tau - xj1) + W[j + 1];
-          if (dE != dE && W[j] == W[j + 1]) dE = W[j];
+        // x_j <= tau < x_{j+1}, 0 <= j <= nbins - 2
+        double fj = __builtin_floor((tau - a) / ds);
+        int j = fj < 0 ? 0 : (fj > nbins - 2 ? nbins - 2 : (int)fj);
+        while (j > 0 && lsc_grid(j, a, ds) > tau) --j;
+        while (j < nbins - 2 && lsc_grid(j + 1, a, ds) <= tau) ++j;
+        const double xj = lsc_grid(j, a, ds);
+        if (xj == tau) {
+          dE = W[j];
+        } else {
+          const double xj1 = lsc_grid(j + 1, a, ds);
+          const double slope = (W[j + 1] - W[j]) / (xj1 - xj);
+          dE = slope * (tau - xj) + W[j];
+          if (dE != dE) {
+            dE = slope * (tau - xj1) + W[j + 1];
+            if (dE != dE && W[j] == W[j + 1]) dE = W[j];
+          }
         }
       }
+      const double d = r[5 * ld + i] + dE * 1e-9 / pc_ref;
+      r[5 * ld + i] = d;
+      if (DRIFT) {
+        r[i] = __builtin_fma(L, r[ld + i], r[i]);
+        r[2 * ld + i] = __builtin_fma(L, r[3 * ld + i], r[2 * ld + i]);
+        r[4 * ld + i] = __builtin_fma(r56, d, tau);
+      }
+      sump(i, d);
     }
-    const double d = r[5 * ld + i] + dE * 1e-9 / pc_ref;
-    r[5 * ld + i] = d;
-    if (DRIFT) {
-      r[i] = __builtin_fma(L, r[ld + i], r[i]);
-      r[2 * ld + i] = __builtin_fma(L, r[3 * ld + i], r[2 * ld + i]);
-      r[4 * ld + i] = __builtin_fma(r56, d, tau);
-    }
+  }
+  if constexpr (SUMP) {
+    block_reduce<3, 0>(acc, lds);
+    if (threadIdx.x == 0)
+      for (int c = 0; c < 3; ++c) part[c * gridDim.x + blockIdx.x] = acc[c];
   }
 }
 

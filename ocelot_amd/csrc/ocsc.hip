@@ -1181,8 +1181,14 @@ int lsc_buffers(ocsc_ctx* c, int64_t bins);
 LscConst lsc_const(int step_profile, double smooth, double E_GeV, double dz, double K_max, double fill_factor,
                    int cap);
 constexpr int kLscGraphBins = 4096;  // kicks inside tracking graphs (device-planned grid; 32 KB of LDS)
+// the LSC kick's momentum sums for the next space-charge kick (the pass-A
+// partials of the kick scalars sc, nred_blocks of them)
+struct LscSump {
+  const KickScalars* sc;
+  double* part;
+};
 int enqueue_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const LscConst& k, double b0,
-                double b1, hipStream_t st, const DriftArgs* dr = nullptr);
+                double b1, hipStream_t st, const DriftArgs* dr = nullptr, const LscSump* sm = nullptr);
 const char* lsc_err_text(int e);
 
 int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
@@ -1325,9 +1331,10 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     if (tw) o.drift = false;  // moments are taken between the kick and the next drift
     if (lq) {
       // the LSC kick changes delta after the space-charge kick: no fused
-      // drift, and every kick forms its own momentum sums
+      // drift here; the LSC kick leaves the next kick's momentum sums, in
+      // pass A's partial layout
       o.drift = o.sump = false;
-      o.passA = true;
+      o.prev_nblk = nred_blocks(c, n);
     }
     o.dr = DriftArgs{Ld, r56};
     if (ktime) {
@@ -1355,7 +1362,10 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     // with an LSC kick and no moments, the next drift is fused into the LSC kick
     const DriftArgs next{Ld, r56};
     const bool fuse = lq && !tw && step < nsteps - 1;
-    if (lq && (rc3 = enqueue_lsc(c, rr, lr, qq, n, lq->k, lq->b0, lq->b1, s2, fuse ? &next : nullptr))) return rc3;
+    const LscSump sm{L.sc, (double*)L.part.p};  // read by the next step's kick
+    if (lq && (rc3 = enqueue_lsc(c, rr, lr, qq, n, lq->k, lq->b0, lq->b1, s2, fuse ? &next : nullptr,
+                                 step < nsteps - 1 ? &sm : nullptr)))
+      return rc3;
     if (tw && (rc3 = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, s2))) return rc3;
     if ((tw || lq) && !fuse && step < nsteps - 1) k_drift<<<nb, 256, 0, s2>>>(rr, lr, n, Ld, r56);
     return OCSC_OK;
@@ -1847,7 +1857,7 @@ void lsc_stats(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, int step_
 // profile, wake and kick from the histogram totals and the plan on the
 // device; grids of `grid` workgroups (the plan's bins, or the cap)
 int lsc_from_profile(ocsc_ctx* c, double* r, int64_t ld, int64_t n, const long long* tot, int grid, hipStream_t st,
-                     const DriftArgs* dr = nullptr) {
+                     const DriftArgs* dr = nullptr, const LscSump* sm = nullptr) {
   double* cur = (double*)c->lsc_bins.p;
   const int64_t cap = (int64_t)(c->lsc_bins.cap / (5 * sizeof(double)));
   double* bunch = cur + cap;
@@ -1857,10 +1867,15 @@ int lsc_from_profile(ocsc_ctx* c, double* r, int64_t ld, int64_t n, const long l
   k_lsc_profile<<<1, 1024, 0, st>>>(tot, c->lsc_plan, cur, bunch);
   k_lsc_spectrum<<<grid, 256, 0, st>>>(bunch, c->lsc_plan, Zr, Zi);
   k_lsc_wake<<<grid, 256, 0, st>>>(Zr, Zi, c->lsc_plan, W);
-  if (n > 0 && dr)
-    k_lsc_kick<true><<<nred_blocks(c, n), kPT, 0, st>>>(r, ld, n, W, c->lsc_plan, dr->L, dr->r56);
+  const int nblk = nred_blocks(c, n);
+  if (n > 0 && dr && sm)
+    k_lsc_kick<true, true><<<nblk, kPT, 0, st>>>(r, ld, n, W, c->lsc_plan, dr->L, dr->r56, sm->sc, sm->part);
+  else if (n > 0 && dr)
+    k_lsc_kick<true><<<nblk, kPT, 0, st>>>(r, ld, n, W, c->lsc_plan, dr->L, dr->r56);
+  else if (n > 0 && sm)
+    k_lsc_kick<false, true><<<nblk, kPT, 0, st>>>(r, ld, n, W, c->lsc_plan, 0.0, 0.0, sm->sc, sm->part);
   else if (n > 0)
-    k_lsc_kick<false><<<nred_blocks(c, n), kPT, 0, st>>>(r, ld, n, W, c->lsc_plan, 0.0, 0.0);
+    k_lsc_kick<false><<<nblk, kPT, 0, st>>>(r, ld, n, W, c->lsc_plan, 0.0, 0.0);
   LAUNCHCHK("lsc kernels");
   return OCSC_OK;
 }
@@ -1874,7 +1889,7 @@ double cic_wscale(int64_t n) {
 // one LSC kick with no host synchronisation (graph-capturable): state,
 // statistics, plan, histogram, profile, wake, kick
 int enqueue_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const LscConst& k, double b0,
-                double b1, hipStream_t st, const DriftArgs* dr) {
+                double b1, hipStream_t st, const DriftArgs* dr, const LscSump* sm) {
   const int cap = k.cap;
   const int ncic = kCicBlocks;
   k_lsc_init<<<1, 64, 0, st>>>(c->lsc, n, b0, b1);
@@ -1886,7 +1901,7 @@ int enqueue_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, 
                                                                          hist);
   k_cic_sum<<<(cap + 63) / 64, 1024, 0, st>>>(hist, ncic, cap, &c->lsc_plan->nbins, tot);
   LAUNCHCHK("k_cic_grid");
-  return lsc_from_profile(c, r, ld, n, tot, cap, st, dr);
+  return lsc_from_profile(c, r, ld, n, tot, cap, st, dr, sm);
 }
 
 const char* lsc_err_text(int e) {
