@@ -444,42 +444,74 @@ __global__ void __launch_bounds__(kPT) k_lsc_kick(double* __restrict__ r, int64_
   } else {
     const double a = p->a, ds = p->ds, pc_ref = p->pc_ref;
     const double x0 = lsc_grid(0, a, ds), xl = lsc_grid(nbins - 1, a, ds);
-    for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPT) {
-      const double tau = r[4 * ld + i];
-      double dE;
-      if (tau != tau) {
-        dE = tau;
-      } else if (tau < x0) {
-        dE = W[0];
-      } else if (tau >= xl) {
-        dE = W[nbins - 1];
-      } else {
-        // x_j <= tau < x_{j+1}, 0 <= j <= nbins - 2
-        double fj = __builtin_floor((tau - a) / ds);
-        int j = fj < 0 ? 0 : (fj > nbins - 2 ? nbins - 2 : (int)fj);
-        while (j > 0 && lsc_grid(j, a, ds) > tau) --j;
-        while (j < nbins - 2 && lsc_grid(j + 1, a, ds) <= tau) ++j;
-        const double xj = lsc_grid(j, a, ds);
-        if (xj == tau) {
-          dE = W[j];
-        } else {
-          const double xj1 = lsc_grid(j + 1, a, ds);
-          const double slope = (W[j + 1] - W[j]) / (xj1 - xj);
-          dE = slope * (tau - xj) + W[j];
-          if (dE != dE) {
-            dE = slope * (tau - xj1) + W[j + 1];
-            if (dE != dE && W[j] == W[j + 1]) dE = W[j];
-          }
-        }
+    // interpolated wake at tau (numpy.interp)
+    auto wake_at = [&](double tau) -> double {
+      if (tau != tau) return tau;
+      if (tau < x0) return W[0];
+      if (tau >= xl) return W[nbins - 1];
+      // x_j <= tau < x_{j+1}, 0 <= j <= nbins - 2
+      double fj = __builtin_floor((tau - a) / ds);
+      int j = fj < 0 ? 0 : (fj > nbins - 2 ? nbins - 2 : (int)fj);
+      while (j > 0 && lsc_grid(j, a, ds) > tau) --j;
+      while (j < nbins - 2 && lsc_grid(j + 1, a, ds) <= tau) ++j;
+      const double xj = lsc_grid(j, a, ds);
+      if (xj == tau) return W[j];
+      const double xj1 = lsc_grid(j + 1, a, ds);
+      const double slope = (W[j + 1] - W[j]) / (xj1 - xj);
+      double dE = slope * (tau - xj) + W[j];
+      if (dE != dE) {
+        dE = slope * (tau - xj1) + W[j + 1];
+        if (dE != dE && W[j] == W[j + 1]) dE = W[j];
       }
-      const double d = r[5 * ld + i] + dE * 1e-9 / pc_ref;
+      return dE;
+    };
+    struct Part {
+      double x, xp, y, yp, tau, d;
+    };
+    auto load = [&](int64_t i, Part& v) {
+      v.tau = r[4 * ld + i];
+      v.d = r[5 * ld + i];
+      if (DRIFT || SUMP) {
+        v.xp = r[ld + i];
+        v.yp = r[3 * ld + i];
+      }
+      if (DRIFT) {
+        v.x = r[i];
+        v.y = r[2 * ld + i];
+      }
+    };
+    auto kick = [&](int64_t i, const Part& v) {
+      const double d = v.d + wake_at(v.tau) * 1e-9 / pc_ref;
       r[5 * ld + i] = d;
       if (DRIFT) {
-        r[i] = __builtin_fma(L, r[ld + i], r[i]);
-        r[2 * ld + i] = __builtin_fma(L, r[3 * ld + i], r[2 * ld + i]);
-        r[4 * ld + i] = __builtin_fma(r56, d, tau);
+        r[i] = __builtin_fma(L, v.xp, v.x);
+        r[2 * ld + i] = __builtin_fma(L, v.yp, v.y);
+        r[4 * ld + i] = __builtin_fma(r56, d, v.tau);
       }
-      sump(i, d);
+      if constexpr (SUMP) {
+        double m[3];
+        to_p(v.xp, v.yp, d, ks, m);
+        acc[0] += m[0];
+        acc[1] += m[1];
+        acc[2] += m[2];
+      }
+    };
+    // two particles per iteration, both loaded before either is written (the
+    // compiler cannot move loads above stores to the same rows); the thread's
+    // particles keep their order, so the sums do too
+    const int64_t stride = (int64_t)gridDim.x * kPT;
+    int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x;
+    for (; i + stride < n; i += 2 * stride) {
+      Part u, w;
+      load(i, u);
+      load(i + stride, w);
+      kick(i, u);
+      kick(i + stride, w);
+    }
+    if (i < n) {
+      Part u;
+      load(i, u);
+      kick(i, u);
     }
   }
   if constexpr (SUMP) {
