@@ -10,6 +10,11 @@
 //   pass 3 (sel., auto_disp) sums of (p - <p>)^2 and (p - <p>)(u - <u>)
 //   pass 4 (selected)        sums of the dispersion-corrected x, y, px, py
 //   pass 5 (selected)        the twelve centred second moments
+// With the dispersion known before the passes (given, or zero), pass 4 needs
+// nothing from pass 2 and runs in the same read of the rows: pass 7 = 2 + 4;
+// without a tau window pass 0's sums are pass 2's too: pass 6 = 0 + 2 + 4.
+// Every value is summed in the same order and tree as in its own pass, so the
+// results are the same bit for bit.
 #pragma once
 #include "particles.hpp"
 
@@ -33,7 +38,7 @@ struct MomState {
   double z0;                // window centre: mean(tau), or the peak-current position (slice = "Imax")
 };
 
-constexpr int kMomNV[6] = {2, 1, 7, 5, 4, 12};
+constexpr int kMomNV[8] = {2, 1, 7, 5, 4, 12, 12, 11};
 
 // dispersion-corrected coordinates (analysis.py:112-130): u -= D * p, then
 // px, py scaled by 1 - p - p^2/2 + px^2/2 + py^2/2 of the corrected px, py
@@ -71,7 +76,7 @@ __global__ void __launch_bounds__(kPT) k_mom(Rows R, const double* __restrict__ 
       double R6[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) R6[k] = k == 4 ? tau : R.r[k * R.ld + i];
-      if constexpr (PASS == 2) {
+      if constexpr (PASS == 2 || PASS == 6 || PASS == 7) {
         acc[0] += 1.0;
         acc[1] += R6[5];
         acc[2] += R6[0];
@@ -79,6 +84,16 @@ __global__ void __launch_bounds__(kPT) k_mom(Rows R, const double* __restrict__ 
         acc[4] += R6[2];
         acc[5] += R6[3];
         acc[6] += tau;
+        if constexpr (PASS == 6) acc[7] += q[i];
+        if constexpr (PASS != 2) {
+          constexpr int o = PASS == 6 ? 8 : 7;
+          double x, y, px, py;
+          mom_corrected(R6, s, &x, &y, &px, &py);
+          acc[o] += x;
+          acc[o + 1] += y;
+          acc[o + 2] += px;
+          acc[o + 3] += py;
+        }
       } else if constexpr (PASS == 3) {
         const double dp = R6[5] - s.m[0];
         acc[0] += dp * dp;
@@ -137,9 +152,18 @@ __global__ void __launch_bounds__(1024) k_mom_reduce(MomState* __restrict__ msp,
     s.std_all = __builtin_sqrt(t[0] / n);
     s.lo = s.tau_all + s.std_all * s.b0;
     s.hi = s.tau_all + s.std_all * s.b1;
-  } else if constexpr (PASS == 2) {
+  } else if constexpr (PASS == 2 || PASS == 6 || PASS == 7) {
+    if constexpr (PASS == 6) {
+      s.tau_all = t[6] / n;
+      s.sumq = t[7];
+      s.z0 = s.tau_all;
+    }
     s.cnt = t[0];
     for (int k = 0; k < 6; ++k) s.m[k] = t[k + 1] / t[0];
+    if constexpr (PASS != 2) {
+      constexpr int o = PASS == 6 ? 8 : 7;
+      for (int k = 0; k < 4; ++k) s.c[k] = t[o + k] / s.cnt;
+    }
   } else if constexpr (PASS == 3) {
     s.var_p = t[0] / s.cnt;
     for (int k = 0; k < 4; ++k) s.D[k] = t[k + 1] / s.cnt / s.var_p;

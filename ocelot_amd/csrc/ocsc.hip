@@ -1060,26 +1060,34 @@ int ocsc_moments(ocsc_ctx* c, const double* r, int64_t ld, const double* q, int6
   HIPCHK(hipMemcpyAsync(c->mom, &h, sizeof(h), hipMemcpyHostToDevice, st));
   double* part = (double*)L.part.p;
   const Rows R{r, ld};
-  k_mom<0><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
-  k_mom_reduce<0><<<1, 1024, 0, st>>>(c->mom, part, nblk);
   double z0 = 0;
-  if (h.bounds) {
-    k_mom<1><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
-    k_mom_reduce<1><<<1, 1024, 0, st>>>(c->mom, part, nblk);
-    if (flags & OCSC_MOM_IMAX) {
-      if ((rc = imax_window(c, R, n, st, &z0))) return rc;
+  if (!h.bounds && h.disp_mode != 2) {
+    k_mom<6><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);  // passes 0, 2, 4 in one read
+    k_mom_reduce<6><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+  } else {
+    k_mom<0><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+    k_mom_reduce<0><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+    if (h.bounds) {
+      k_mom<1><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+      k_mom_reduce<1><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+      if (flags & OCSC_MOM_IMAX) {
+        if ((rc = imax_window(c, R, n, st, &z0))) return rc;
+      } else {
+        z0 = -1;  // mean(tau), set below from the device state
+      }
+    }
+    if (h.disp_mode != 2) {
+      k_mom<7><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);  // passes 2 and 4 in one read
+      k_mom_reduce<7><<<1, 1024, 0, st>>>(c->mom, part, nblk);
     } else {
-      z0 = -1;  // mean(tau), set below from the device state
+      k_mom<2><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+      k_mom_reduce<2><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+      k_mom<3><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+      k_mom_reduce<3><<<1, 1024, 0, st>>>(c->mom, part, nblk);
+      k_mom<4><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
+      k_mom_reduce<4><<<1, 1024, 0, st>>>(c->mom, part, nblk);
     }
   }
-  k_mom<2><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
-  k_mom_reduce<2><<<1, 1024, 0, st>>>(c->mom, part, nblk);
-  if (h.disp_mode == 2) {
-    k_mom<3><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
-    k_mom_reduce<3><<<1, 1024, 0, st>>>(c->mom, part, nblk);
-  }
-  k_mom<4><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
-  k_mom_reduce<4><<<1, 1024, 0, st>>>(c->mom, part, nblk);
   k_mom<5><<<nblk, kPT, 0, st>>>(R, q, c->mom, part);
   k_mom_reduce<5><<<1, 1024, 0, st>>>(c->mom, part, nblk);
   LAUNCHCHK("k_mom");
@@ -1125,8 +1133,14 @@ int enqueue_moments(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, cons
   MomState* cur = c->mom;
   k_mom_init<<<1, 64, 0, st>>>(cur, n, t.bounds, t.disp_mode, t.b0, t.b1, t.disp[0], t.disp[1], t.disp[2],
                                t.disp[3]);
-  k_mom<0><<<nblk, kPT, 0, st>>>(R, q, cur, part);
-  k_mom_reduce<0><<<1, 1024, 0, st>>>(cur, part, nblk);
+  const bool one_read = !t.bounds && t.disp_mode != 2;  // passes 0, 2, 4 together
+  if (one_read) {
+    k_mom<6><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+    k_mom_reduce<6><<<1, 1024, 0, st>>>(cur, part, nblk);
+  } else {
+    k_mom<0><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+    k_mom_reduce<0><<<1, 1024, 0, st>>>(cur, part, nblk);
+  }
   if (t.bounds) {
     k_mom<1><<<nblk, kPT, 0, st>>>(R, q, cur, part);
     k_mom_reduce<1><<<1, 1024, 0, st>>>(cur, part, nblk);
@@ -1144,14 +1158,17 @@ int enqueue_moments(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, cons
       k_imax_peak<<<1, 1024, 0, st>>>(tot, c->imax_plan, cur);
     }
   }
-  k_mom<2><<<nblk, kPT, 0, st>>>(R, q, cur, part);
-  k_mom_reduce<2><<<1, 1024, 0, st>>>(cur, part, nblk);
   if (t.disp_mode == 2) {
+    k_mom<2><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+    k_mom_reduce<2><<<1, 1024, 0, st>>>(cur, part, nblk);
     k_mom<3><<<nblk, kPT, 0, st>>>(R, q, cur, part);
     k_mom_reduce<3><<<1, 1024, 0, st>>>(cur, part, nblk);
+    k_mom<4><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+    k_mom_reduce<4><<<1, 1024, 0, st>>>(cur, part, nblk);
+  } else if (!one_read) {
+    k_mom<7><<<nblk, kPT, 0, st>>>(R, q, cur, part);  // passes 2 and 4 in one read
+    k_mom_reduce<7><<<1, 1024, 0, st>>>(cur, part, nblk);
   }
-  k_mom<4><<<nblk, kPT, 0, st>>>(R, q, cur, part);
-  k_mom_reduce<4><<<1, 1024, 0, st>>>(cur, part, nblk);
   k_mom<5><<<nblk, kPT, 0, st>>>(R, q, cur, part);
   k_mom_reduce<5><<<1, 1024, 0, st>>>(cur, part, nblk);
   k_mom_store<<<1, 64, 0, st>>>(cur, (MomState*)c->mom_tab.p, (int*)c->mom_ctr.p, t.cap);
