@@ -145,8 +145,8 @@ def other_configs(torch, dev):
     (uneven sizes, meshes 31/47/63, 20 kicks, mixed), single drop-in kicks
     of the 2M-particle 63^3 bunch (FP64, device-resident and host numpy), the
     headline tracking loop with per-step Twiss (track()'s tws_track), the
-    LSC kick of the same resident bunch, and the tracking loop with an LSC kick
-    after every space-charge kick."""
+    LSC kick of the same resident bunch, the tracking loop with an LSC kick
+    after every space-charge kick, and the CSR kick tail on the same bunch."""
     from ocelot_amd import DeviceParticleArray, ParticleArray, SpaceCharge, apply_batched
     from ocelot_amd.workloads import batch64, make_bunch
     out = {}
@@ -230,6 +230,26 @@ def other_configs(torch, dev):
     out["flattop_10m_127_track_sc_lsc"] = {"particles": r.shape[1], "steps": 20, "ms_per_step": round(dt * 1e3, 4),
                                            "value": round(r.shape[1] / dt, 1), "unit": "macroparticle-steps/s",
                                            "precision": "mixed SC + fp64 LSC", "hip_graph": True}
+    # the CSR kick tail (convolution of a 1001-point line charge with a
+    # 1001-point kernel, interpolated energy kick of every particle) on the
+    # same resident bunch; per call, including the small host copies
+    from ocelot_amd.csr import csr_kick
+    ns = 1000
+    s1, s2 = -1.6e-3, 1.6e-3
+    zg = np.linspace(s1, s2, ns + 1)
+    lam = np.exp(-0.5 * (zg / 6e-4) ** 2)
+    lam /= lam.sum() * (s2 - s1) / ns
+    K1 = -1e3 / (1.0 + np.arange(ns + 1))
+    csr_kick(d, lam, K1, s1, s2, ns, dz)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        csr_kick(d, lam, K1, s1, s2, ns, dz)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 20
+    out["flattop_10m_csr_kick"] = {"particles": r.shape[1], "kicks": 20, "mesh": ns + 1, "ms_per_kick": round(dt * 1e3, 4),
+                                   "value": round(r.shape[1] / dt, 1), "unit": "macroparticle-kicks/s",
+                                   "kick_GBs": round(24 * r.shape[1] / dt / 1e9, 1), "precision": "fp64"}
     return out
 
 
