@@ -84,7 +84,9 @@ def lsc_only(a, torch, host, nmesh, dz, dev):
 
 def algorithmic_bytes(n, nmesh, prec):
     """Compulsory HBM bytes of one kick (particle passes A-D + pruned mesh
-    passes), and of the kick kernel alone.  See DESIGN.md for the derivation."""
+    passes), and of the kick kernel alone: the kick reads and writes the six
+    FP64 rows (96 B per particle) and reads the three FP32 field grids (12 B
+    per cell; 24 B of FP64 difference fields in the FP64 form)."""
     nx, ny, nz = nmesh
     mx, my, mz = (1 << int(np.ceil(np.log2(2 * v - 1))) for v in nmesh)
     hz = mz // 2 + 1
@@ -100,7 +102,9 @@ def algorithmic_bytes(n, nmesh, prec):
             + (2 if nf == 3 else 1) * c * nx * my * hz           # x pass write (Qx, P)
             + (2 if nf == 3 else 1) * c * nx * my * hz + nf * c * nx * ny * hz  # inverse y
             + nf * c * nx * ny * hz + nf * f * nc)               # inverse z -> fields
-    kick = 96 * n + nf * f * nc
+    if prec != "mixed":
+        mesh += 32 * nc                                        # k_diff64: potential in, three fields out
+    kick = 96 * n + (12 if prec == "mixed" else 24) * nc
     return part + mesh, kick
 
 
@@ -256,9 +260,9 @@ def other_configs(torch, dev):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on exactly one GPU; the multi-GPU design "
-                                   "(particle sharding + one charge-grid all-reduce) is described in "
-                                   "DESIGN.md and not built", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; running on several "
+                                   "(particles sharded by index, one all-reduce of the charge grid per "
+                                   "kick) is designed but not built", "n_gpus": a.gpus}))
         sys.exit(2)
     import torch
     from ocelot_amd import DeviceParticleArray, ParticleArray, SpaceCharge, track_drift
