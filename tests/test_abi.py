@@ -69,3 +69,22 @@ def test_missing_library_fails_loudly(tmp_path):
             _lib.load(str(tmp_path / "nope.so"))
     finally:
         _lib._lib = saved
+
+
+@pytest.mark.parametrize("compiler,std", [("gcc", "-std=c11"), ("g++", "-std=c++17")])
+def test_documented_c_usage_compiles_against_the_header(tmp_path, compiler, std):
+    """The C caller example of the integration notes, wrapped in a function,
+    compiles warning-free against include/ocsc.h as C and as C++ (argument
+    counts and types of every entry point it calls)."""
+    txt = open(os.path.join(REPO, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```c\n(.*?)```", txt, re.S)
+    assert blocks, "no C example found"
+    body = "".join(b.replace('#include "ocsc.h"\n', "") for b in blocks)
+    src = ('#include <stdio.h>\n#include <stdint.h>\n#include "ocsc.h"\n'
+           "void example(double* d_r, double* d_q, int64_t n, void* stream, const double* draws, double* tws) {\n"
+           + body + "}\n")
+    f = tmp_path / ("ex.c" if compiler == "gcc" else "ex.cpp")
+    f.write_text(src)
+    r = subprocess.run([compiler, std, "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(REPO, "include"),
+                        str(f)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
