@@ -502,7 +502,7 @@ template <typename F, bool DRIFT, bool SUMP>
 void launch_kick(ocsc_ctx* c, Lane& L, int npart, double* r, int64_t ld, const StepOpts& o, hipStream_t st) {
   // mixed form: the three FP32 field grids; FP64 form: the k_diff64 fields
   const F* field = sizeof(F) == 8 ? (const F*)L.E64.p : (const F*)L.phi.p;
-  k_kick<F, DRIFT, SUMP><<<npart, kPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p);
+  k_kick<F, DRIFT, SUMP><<<npart, kKickPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p);
 }
 
 template <typename F, int NWORD>
@@ -510,7 +510,7 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
                  int prec, hipStream_t st, const StepOpts& o = StepOpts()) {
   const int64_t n = in.n;
   const int nblk = nred_blocks(c, n);
-  const int npart = (int)((n + kPT - 1) / kPT);
+  const int npart = (int)((n + kKickPT - 1) / kKickPT);  // kick workgroups
   Rows R{r, ld};
   const bool prof = c->profiling != 0;
   const int slot = (int)(c->ev_seq % ocsc_ctx::kEvRing);
@@ -606,7 +606,7 @@ int enqueue_kick_prec(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_
 
 int ensure_part(ocsc_ctx* c, Lane& L, int64_t n) {
   const size_t need =
-      std::max<size_t>((size_t)kMaxRedBlocks * kNPartB, (size_t)3 * ((n + kPT - 1) / kPT) + 3 * kFoldBlocks);
+      std::max<size_t>((size_t)kMaxRedBlocks * kNPartB, (size_t)3 * ((n + kKickPT - 1) / kKickPT) + 3 * kFoldBlocks);
   return ensure(L.part, need * sizeof(double));
 }
 
@@ -1330,7 +1330,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   // Step loop.  The drift of step s+1 and the momentum sums of kick s+1 are
   // fused into kick s, so the sequence is drift_0, then kicks 0..n-1 with
   // kick s < n-1 drifting (first kick: own pass A; later kicks: partials).
-  const int npart = (int)((n + kPT - 1) / kPT);
+  const int npart = (int)((n + kKickPT - 1) / kKickPT);  // kick workgroups: its momentum partials
   const bool ktime = c->kick_timing != 0 && zstep != 0;
   if (ktime) {
     while (c->kick_ev.size() < (size_t)2 * nsteps) {

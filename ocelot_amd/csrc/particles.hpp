@@ -860,16 +860,20 @@ struct DriftArgs {
   double L, r56;
 };
 
+// threads per workgroup of the kick, one particle each (and one momentum
+// partial per workgroup); 128 and 512 measured no faster
+constexpr int kKickPT = 256;
+
 template <typename F, bool DRIFT = false, bool SUMP = false>
-__global__ void __launch_bounds__(kPT) k_kick(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
-                                              const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
-                                              double* __restrict__ part = nullptr) {
-  __shared__ double lds[SUMP ? (kPT / 64) * 3 : 1];
+__global__ void __launch_bounds__(kKickPT) k_kick(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
+                                                  const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
+                                                  double* __restrict__ part = nullptr) {
+  __shared__ double lds[SUMP ? (kKickPT / 64) * 3 : 1];
   const KickScalars& s0 = *scp;
   if (s0.err) return;  // uniform across the grid
   const KickScalars s = s0;
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t p = blk * kPT + threadIdx.x;
+  const int64_t p = blk * kKickPT + threadIdx.x;
   const bool active = p < s.n;
   // both forms take the fast transforms (to_xp_fast, to_ocelot_fast: the
   // rows differ from the reference's operation sequence in the last bits;
