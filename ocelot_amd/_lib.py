@@ -167,5 +167,7 @@ def context(device):
     if ctx is None:
         ctx = Context(device)
         with _lock:
-            _contexts[device] = ctx
+            # two threads may race to create it: the first one stored wins and
+            # the other context is released when it goes out of scope
+            ctx = _contexts.setdefault(device, ctx)
     return ctx
