@@ -347,6 +347,33 @@ def test_invalid_bunch_raises_value_error_and_leaves_particles(what):
     SpaceCharge(nmesh_xyz=[15, 15, 15]).apply(p2, 0.1)
 
 
+@pytest.mark.parametrize("what", ["one_particle", "identical_particles"])
+def test_zero_extent_raises_value_error_and_leaves_particles(what):
+    """No spread on any axis: the reference divides the zero extent by n-3,
+    gets NaN positions and fails in bincount with ValueError (as the oracle
+    does); the drop-in raises ValueError and leaves the particles as they were."""
+    from ocelot_amd import SpaceCharge
+    n = 1 if what == "one_particle" else 5
+    r = np.zeros((6, n))
+    r[0], r[2], r[5] = 1e-4, -2e-4, 1e-3
+    q = np.full(n, 1e-13)
+    with pytest.raises(ValueError), np.errstate(all="ignore"):
+        orc.apply(r.copy(), q, 0.13, 0.1, (15, 15, 15))
+    p = parray(r, q, 0.13)
+    with pytest.raises(ValueError):
+        SpaceCharge(nmesh_xyz=[15, 15, 15]).apply(p, 0.1)
+    assert np.array_equal(p.rparticles, r)
+    # the same inside tracking: the error is reported at the end of the call
+    from ocelot_amd import DeviceParticleArray, track_drift
+    d = DeviceParticleArray.from_host(parray(r, q, 0.13))
+    with pytest.raises(ValueError):
+        track_drift(d, SpaceCharge(nmesh_xyz=[15, 15, 15]), 0.1, 0.1, 3)
+    from ocelot_amd.workloads import gaussian_bunch
+    p2 = parray(gaussian_bunch(1000, seed=6), np.full(1000, 1e-13), 0.13)
+    SpaceCharge(nmesh_xyz=[15, 15, 15]).apply(p2, 0.1)   # the context stays usable
+    assert np.isfinite(p2.rparticles).all()
+
+
 def test_small_mesh_rejected():
     from ocelot_amd import SpaceCharge
     from ocelot_amd.workloads import gaussian_bunch
