@@ -311,6 +311,31 @@ def test_apply_on_a_foreign_process_object():
         assert np.array_equal(a.rparticles, b.rparticles), rm
 
 
+def test_device_rows_with_a_larger_row_stride():
+    """Device rparticles as a (6, n) view of wider rows (row stride ld > n),
+    one-shot and in tracking: the same bits as the contiguous bunch, and the
+    columns past n untouched."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=60_000, seed=12)
+    n = r.shape[1]
+    sc = SpaceCharge(nmesh_xyz=[31, 31, 31], precision="mixed")
+    for track in (False, True):
+        a = DeviceParticleArray.from_host(parray(r, q, E))
+        wide = torch.full((6, n + 777), 7.0, dtype=torch.float64, device="cuda")
+        wide[:, :n] = torch.from_numpy(r).cuda()
+        b = DeviceParticleArray(wide[:, :n], torch.from_numpy(q.copy()).cuda(), E)
+        assert b.rparticles.stride(0) == n + 777
+        if track:
+            track_drift(a, sc, dz, dz, 4)
+            track_drift(b, sc, dz, dz, 4)
+        else:
+            sc.apply(a, dz)
+            sc.apply(b, dz)
+        assert torch.equal(a.rparticles, wide[:, :n]), track
+        assert bool((wide[:, n:] == 7.0).all()), track
+
+
 def test_kernel_cache_hit_gives_identical_result():
     from ocelot_amd import SpaceCharge
     from ocelot_amd.workloads import make_bunch
@@ -374,12 +399,18 @@ def test_zero_extent_raises_value_error_and_leaves_particles(what):
     assert np.isfinite(p2.rparticles).all()
 
 
-def test_small_mesh_rejected():
+@pytest.mark.parametrize("nmesh", [[3, 15, 15], [15, 15, 513]])
+def test_mesh_outside_limits_rejected(nmesh):
+    """nmesh < 4 (the reference divides by n-3 <= 0) and nmesh > 512 (past the
+    largest transform, 1024 points) raise ValueError before any work; the
+    particles are untouched."""
     from ocelot_amd import SpaceCharge
     from ocelot_amd.workloads import gaussian_bunch
-    p = parray(gaussian_bunch(100, seed=1), np.full(100, 1e-13), 0.13)
+    r = gaussian_bunch(100, seed=1)
+    p = parray(r, np.full(100, 1e-13), 0.13)
     with pytest.raises(ValueError):
-        SpaceCharge(nmesh_xyz=[3, 15, 15]).apply(p, 0.1)
+        SpaceCharge(nmesh_xyz=nmesh).apply(p, 0.1)
+    assert np.array_equal(p.rparticles, r)
 
 
 def test_drift_matches_oracle():
