@@ -163,6 +163,7 @@ struct Lane {
   KickScalars* sc_host = nullptr;  // pinned mirror
   Buf part;                        // per-block partial sums
   Buf grid, A, B, phi, G;
+  Buf KA, KB;                      // kernel-spectrum build scratch (its own, so it can overlap the FFTs)
   Buf E64;                         // FP64 form: the three difference fields of phi
   hipStream_t stream = nullptr;    // extra lanes only
   hipEvent_t done = nullptr;       // extra lanes only
@@ -261,9 +262,6 @@ int ensure_mesh(ocsc_ctx* c, Lane& L, const Geometry& g, int nword) {
   if ((rc = ensure(L.grid, (size_t)(c->deposit_xcd ? kNumXcd : 1) * nword * g.ncell * 8))) return rc;
   size_t a = (size_t)d.nx * d.ny * d.hz * sizeof(cpx<F>);
   size_t b = (size_t)d.nx * d.my * d.hz * sizeof(cpx<F>);
-  // the kernel-spectrum build borrows A and B as FP64 scratch
-  a = std::max(a, (size_t)d.nx * d.ny * d.hz * 8);
-  b = std::max(b, std::max((size_t)g.ncell * 8, (size_t)d.nx * d.hy * d.hz * 8));
   if constexpr (sizeof(F) == 4) {
     // mixed form carries three fields through the inverse passes
     a = std::max(a, 3 * (size_t)d.nx * d.ny * d.hz * sizeof(cpx<float>));
@@ -271,6 +269,9 @@ int ensure_mesh(ocsc_ctx* c, Lane& L, const Geometry& g, int nword) {
   }
   if ((rc = ensure(L.A, a))) return rc;
   if ((rc = ensure(L.B, b))) return rc;
+  // kernel-spectrum scratch: the IGF grid / its y pass, and its z pass
+  if ((rc = ensure(L.KB, std::max((size_t)g.ncell, (size_t)d.nx * d.hy * d.hz) * 8))) return rc;
+  if ((rc = ensure(L.KA, (size_t)d.nx * d.ny * d.hz * 8))) return rc;
   if ((rc = ensure(L.phi, (size_t)g.ncell * std::max(sizeof(double), 3 * sizeof(float))))) return rc;
   if ((rc = ensure(L.G, (size_t)d.hx * d.hy * d.hz * sizeof(double)))) return rc;
   if constexpr (sizeof(F) == 8)
@@ -310,9 +311,9 @@ const cpx<F>* tw_for(ocsc_ctx* c, int M) {
 template <typename F>
 int launch_kernel_spectrum(ocsc_ctx* c, Lane& L, const Geometry& g, hipStream_t st, int force) {
   const MeshDims& d = g.d;
-  double* K = (double*)L.B.p;
-  double* Kz = (double*)L.A.p;
-  double* Kzy = (double*)L.B.p;
+  double* K = (double*)L.KB.p;
+  double* Kz = (double*)L.KA.p;
+  double* Kzy = (double*)L.KB.p;
   const int nb_igf = ((d.nx + kIgfTx - 1) / kIgfTx) * ((d.ny + kIgfTy - 1) / kIgfTy) * ((d.nz + kIgfTz - 1) / kIgfTz);
   k_igf<<<nb_igf, 256, 0, st>>>(K, d, L.sc, force);
   LAUNCHCHK("k_igf");
@@ -344,8 +345,9 @@ int launch_kernel_spectrum(ocsc_ctx* c, Lane& L, const Geometry& g, hipStream_t 
 // phi = G * rho on the doubled grid; rho read through charge_at<CIN>
 template <typename F, int CIN>
 int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, F* phi, hipStream_t st,
-                       hipEvent_t* ev = nullptr) {
-  // ev[0] is recorded by the caller before the z pass; ev[1..4] after z, y, x, y
+                       hipEvent_t* ev = nullptr, hipEvent_t join = nullptr) {
+  // ev[0] is recorded by the caller before the z pass; ev[1..4] after z, y, x, y.
+  // join: the kernel spectrum built on another stream, awaited before the x pass
   const MeshDims& d = g.d;
   cpx<F>* A = (cpx<F>*)L.A.p;
   cpx<F>* B = (cpx<F>*)L.B.p;
@@ -365,6 +367,7 @@ int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho,
   });
   LAUNCHCHK("k_yfwd");
   if (ev) (void)hipEventRecord(ev[2], st);
+  if (join) HIPCHK(hipStreamWaitEvent(st, join, 0));
   OCSC_SWITCH_M(d.mx, {
     using TL = Tile<F, MM>;
     dim3 grid((d.hz + TL::T - 1) / TL::T, d.my);
@@ -392,7 +395,7 @@ int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho,
 // forward differences of the reference (sc.py:195-200) taken in Fourier
 // space, written as three FP32 grids one after the other at E.
 int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, float* E, hipStream_t st,
-                  hipEvent_t* ev = nullptr) {
+                  hipEvent_t* ev = nullptr, hipEvent_t join = nullptr) {
   const MeshDims& d = g.d;
   const int64_t nyz = (int64_t)d.nx * d.ny * d.hz;
   cpx<float>* A = (cpx<float>*)L.A.p;
@@ -414,6 +417,7 @@ int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, floa
   });
   LAUNCHCHK("k_yfwd");
   if (ev) (void)hipEventRecord(ev[2], st);
+  if (join) HIPCHK(hipStreamWaitEvent(st, join, 0));  // the kernel spectrum, built on another stream
   OCSC_SWITCH_M(d.mx, {
     using FT = FieldTile<MM, kNTF>;
     dim3 grid((d.hz + FT::T - 1) / FT::T, d.my);
@@ -546,8 +550,9 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   mark(2);
   int rc;
   // kernel spectrum (needs only the cell sizes) on the side stream while the
-  // main stream deposits; joined before the first FFT pass.  Sequential when
-  // profiling, so the stage times stay separable.
+  // main stream deposits and runs the z and y passes; joined before the x
+  // pass, the first to read it.  Sequential when profiling, so the stage times
+  // stay separable.
   const bool fork = c->overlap && !prof;
   if (fork) {
     HIPCHK(hipEventRecord(c->ev_fork, st));
@@ -564,16 +569,16 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
     k_deposit<NWORD, false><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
   LAUNCHCHK("deposit");
   mark(3);
-  if (fork)
-    HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
-  else if ((rc = launch_kernel_spectrum<F>(c, L, g, st, 0)))
-    return rc;
+  if (!fork && (rc = launch_kernel_spectrum<F>(c, L, g, st, 0))) return rc;
   mark(4);
   if constexpr (sizeof(F) == 4) {
-    if ((rc = launch_fields(c, L, g, L.grid.p, (float*)L.phi.p, st, prof ? ev + 4 : nullptr))) return rc;
+    if ((rc = launch_fields(c, L, g, L.grid.p, (float*)L.phi.p, st, prof ? ev + 4 : nullptr,
+                            fork ? c->ev_join : nullptr)))
+      return rc;
   } else {
     if ((rc = launch_convolution<F, NWORD == 2 ? kChargeFix2 : kChargeFix1>(c, L, g, L.grid.p, (F*)L.phi.p, st,
-                                                                              prof ? ev + 4 : nullptr)))
+                                                                              prof ? ev + 4 : nullptr,
+                                                                              fork ? c->ev_join : nullptr)))
       return rc;
     k_diff64<<<(int)((g.ncell + 255) / 256), 256, 0, st>>>((const double*)L.phi.p, L.sc, (double*)L.E64.p);
     LAUNCHCHK("k_diff64");
@@ -674,7 +679,7 @@ int init_lane(Lane& L, bool own_stream) {
 }
 
 void free_lane(Lane& L) {
-  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &L.E64})
+  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.KA, &L.KB, &L.phi, &L.G, &L.E64})
     if (b->p) (void)hipFree(b->p);
   if (L.sc) (void)hipFree(L.sc);
   if (L.sc_host) (void)hipHostFree(L.sc_host);
@@ -745,7 +750,8 @@ int ocsc_destroy(ocsc_ctx* c) {
   Lane& L = c->lane0;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
-  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1], &c->keys[0],
+  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.KA, &L.KB, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1],
+                 &c->keys[0],
                  &L.E64, &c->draws, &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
                  &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->lsc_fac, &c->csr_bins})
     if (b->p) (void)hipFree(b->p);
