@@ -184,6 +184,7 @@ struct ocsc_ctx {
   hipStream_t cap_stream = nullptr;  // stream used for graph capture
   hipStream_t side = nullptr;        // second stream: kernel spectrum next to the deposit
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork0 = nullptr, ev_zero = nullptr;  // charge-grid clear on the side stream
   int overlap = 1;                   // 1: build the kernel spectrum concurrently with the deposit
   // stage timing (ocsc_set_profiling): events between the stages of a kick
   int profiling = 0;
@@ -526,6 +527,23 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   auto mark = [&](int i) {
     if (prof) (void)hipEventRecord(ev[i], st);
   };
+  // Side-stream work (not when profiling, so the stage times stay separable):
+  // the charge grid is cleared while the frame and extent passes run (the
+  // previous kick's z pass, earlier on this stream, was its last reader), and
+  // the kernel spectrum, which needs only the cell sizes, is built while the
+  // deposit and the z and y passes run; the main stream waits for each only
+  // before its first reader.
+  const bool fork = c->overlap && !prof;
+  const int64_t nz64 = (c->deposit_xcd ? kNumXcd : 1) * NWORD * g.ncell;
+  auto zero_grid = [&](hipStream_t s) {
+    k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, s>>>((unsigned long long*)L.grid.p, nz64);
+  };
+  if (fork) {
+    HIPCHK(hipEventRecord(c->ev_fork0, st));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork0, 0));
+    zero_grid(c->side);
+    HIPCHK(hipEventRecord(c->ev_zero, c->side));
+  }
   mark(0);
   k_setup<<<1, 64, 0, st>>>(L.sc, in);
   LAUNCHCHK("k_setup");
@@ -549,19 +567,15 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   LAUNCHCHK("frame/extent passes");
   mark(2);
   int rc;
-  // kernel spectrum (needs only the cell sizes) on the side stream while the
-  // main stream deposits and runs the z and y passes; joined before the x
-  // pass, the first to read it.  Sequential when profiling, so the stage times
-  // stay separable.
-  const bool fork = c->overlap && !prof;
   if (fork) {
     HIPCHK(hipEventRecord(c->ev_fork, st));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     if ((rc = launch_kernel_spectrum<F>(c, L, g, c->side, 0))) return rc;
     HIPCHK(hipEventRecord(c->ev_join, c->side));
+    HIPCHK(hipStreamWaitEvent(st, c->ev_zero, 0));
+  } else {
+    zero_grid(st);
   }
-  const int64_t nz64 = (c->deposit_xcd ? kNumXcd : 1) * NWORD * g.ncell;
-  k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, st>>>((unsigned long long*)L.grid.p, nz64);
   const int ndep = (int)((n + kDepPT * kDepItems - 1) / (kDepPT * kDepItems));
   if (c->deposit_xcd)
     k_deposit<NWORD, true><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
@@ -739,7 +753,9 @@ int ocsc_create(ocsc_ctx** out, int device) {
     return cleanup(fail(OCSC_EHIP, "stream"));
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(OCSC_EHIP, "stream"));
   if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork0, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_zero, hipEventDisableTiming) != hipSuccess)
     return cleanup(fail(OCSC_EHIP, "event"));
   *out = c;
   return OCSC_OK;
@@ -771,6 +787,8 @@ int ocsc_destroy(ocsc_ctx* c) {
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_fork0) (void)hipEventDestroy(c->ev_fork0);
+  if (c->ev_zero) (void)hipEventDestroy(c->ev_zero);
   for (auto& set : c->ev)
     for (hipEvent_t e : set)
       if (e) (void)hipEventDestroy(e);
