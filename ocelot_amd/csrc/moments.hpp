@@ -54,9 +54,15 @@ __device__ __forceinline__ void mom_corrected(const double* R6, const MomState& 
   *py = py0 * f;
 }
 
-template <int PASS>
+// DRIFT (pass 5 inside tracking, the last read of a step): the next step's
+// drift is applied to the rows after their moments are added, to every
+// particle (selected or not), with the drift kernel's fused multiply-adds:
+// x += L x', y += L y', tau += r56 delta.  rw is R.r, writable.
+template <int PASS, bool DRIFT = false>
 __global__ void __launch_bounds__(kPT) k_mom(Rows R, const double* __restrict__ q, const MomState* __restrict__ msp,
-                                             double* __restrict__ part) {
+                                             double* __restrict__ part, double* rw = nullptr, double L = 0,
+                                             double r56 = 0) {
+  static_assert(!DRIFT || PASS == 5, "the drift rides on the last pass");
   constexpr int NV = kMomNV[PASS];
   __shared__ double lds[(kPT / 64) * NV];
   const MomState s = *msp;
@@ -65,6 +71,33 @@ __global__ void __launch_bounds__(kPT) k_mom(Rows R, const double* __restrict__ 
   for (int c = 0; c < NV; ++c) acc[c] = 0;
   for (int64_t i = (int64_t)blockIdx.x * kPT + threadIdx.x; i < s.n; i += (int64_t)gridDim.x * kPT) {
     const double tau = R.r[4 * R.ld + i];
+    if constexpr (DRIFT) {
+      const int64_t ld = R.ld;
+      const double x = rw[i], xp = rw[ld + i], y = rw[2 * ld + i], yp = rw[3 * ld + i], d = rw[5 * ld + i];
+      if (!s.bounds || (s.lo <= tau && tau <= s.hi)) {
+        const double R6[6] = {x, xp, y, yp, tau, d};
+        double cx, cy, cpx, cpy;
+        mom_corrected(R6, s, &cx, &cy, &cpx, &cpy);
+        const double dx = cx - s.c[0], dy = cy - s.c[1], dpx = cpx - s.c[2], dpy = cpy - s.c[3];
+        const double dt = tau - s.m[5], dp = d - s.m[0];
+        acc[0] += dx * dx;
+        acc[1] += dx * dpx;
+        acc[2] += dpx * dpx;
+        acc[3] += dy * dy;
+        acc[4] += dy * dpy;
+        acc[5] += dpy * dpy;
+        acc[6] += dt * dt;
+        acc[7] += dx * dy;
+        acc[8] += dpx * dpy;
+        acc[9] += dx * dpy;
+        acc[10] += dy * dpx;
+        acc[11] += dp * dp;
+      }
+      rw[i] = __builtin_fma(L, xp, x);
+      rw[2 * ld + i] = __builtin_fma(L, yp, y);
+      rw[4 * ld + i] = __builtin_fma(r56, d, tau);
+      continue;
+    }
     if constexpr (PASS == 0) {
       acc[0] += tau;
       acc[1] += q[i];

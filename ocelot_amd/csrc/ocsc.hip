@@ -1151,7 +1151,9 @@ double cic_wscale(int64_t n);
 // the moments passes of ocsc_moments on the device only, appended to the
 // step table (graph-capturable; own partial buffer, so the kick's fused
 // momentum sums are left alone)
-int enqueue_moments(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, const TwsReq& t, hipStream_t st) {
+// dr: apply the next step's drift in the last pass (after the moments)
+int enqueue_moments(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, const TwsReq& t, hipStream_t st,
+                    const DriftArgs* dr = nullptr) {
   const int nblk = nred_blocks(c, n);
   double* part = (double*)c->mom_part.p;
   MomState* cur = c->mom;
@@ -1193,7 +1195,10 @@ int enqueue_moments(ocsc_ctx* c, const Rows& R, const double* q, int64_t n, cons
     k_mom<7><<<nblk, kPT, 0, st>>>(R, q, cur, part);  // passes 2 and 4 in one read
     k_mom_reduce<7><<<1, 1024, 0, st>>>(cur, part, nblk);
   }
-  k_mom<5><<<nblk, kPT, 0, st>>>(R, q, cur, part);
+  if (dr)
+    k_mom<5, true><<<nblk, kPT, 0, st>>>(R, q, cur, part, const_cast<double*>(R.r), dr->L, dr->r56);
+  else
+    k_mom<5><<<nblk, kPT, 0, st>>>(R, q, cur, part);
   k_mom_reduce<5><<<1, 1024, 0, st>>>(cur, part, nblk);
   k_mom_store<<<1, 64, 0, st>>>(cur, (MomState*)c->mom_tab.p, (int*)c->mom_ctr.p, t.cap);
   LAUNCHCHK("tracking moments");
@@ -1400,15 +1405,17 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   auto step_body = [&](int step, const StepOpts& o, hipStream_t s2) -> int {
     int rc3;
     if ((rc3 = enqueue_kick_prec(c, L, g, rr, lr, qq, in, prec, s2, o))) return rc3;
-    // with an LSC kick and no moments, the next drift is fused into the LSC kick
+    // the next drift: fused into the LSC kick when there are no moments
     const DriftArgs next{Ld, r56};
     const bool fuse = lq && !tw && step < nsteps - 1;
     const LscSump sm{L.sc, (double*)L.part.p};  // read by the next step's kick
     if (lq && (rc3 = enqueue_lsc(c, rr, lr, qq, n, lq->k, lq->b0, lq->b1, s2, fuse ? &next : nullptr,
                                  step < nsteps - 1 ? &sm : nullptr)))
       return rc3;
-    if (tw && (rc3 = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, s2))) return rc3;
-    if ((tw || lq) && !fuse && step < nsteps - 1) k_drift<<<nb, 256, 0, s2>>>(rr, lr, n, Ld, r56);
+    // with moments, the next drift rides on their last pass (after the kick's
+    // and the LSC kick's writes); otherwise the kick or the LSC kick carries it
+    if (tw && (rc3 = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, s2, step < nsteps - 1 ? &next : nullptr)))
+      return rc3;
     return OCSC_OK;
   };
   hipGraphExec_t exec[2] = {nullptr, nullptr};
