@@ -374,6 +374,14 @@ def main():
         prof[prec] = [ms[i] for i in range(ns)]
     total_b, kick_b = algorithmic_bytes(n, nmesh, "mixed")
     total_b64, _ = algorithmic_bytes(n, nmesh, "fp64")
+    # algorithmic GB/s of the particle stages in the same eager cycle: extent
+    # (pass B) and deposit read six rows and q (56 B per particle), the kick
+    # reads and writes the rows and reads the field grids
+    nc = int(np.prod(nmesh))
+    stage_bytes = {"mixed": {"extent": 56 * n, "deposit": 56 * n, "kick": 96 * n + 12 * nc},
+                   "fp64": {"extent": 56 * n, "deposit": 56 * n, "kick": 96 * n + 24 * nc}}
+    stages_gbs = {p: {k: round(b / (stages[p][k] * 1e-3) / 1e9, 1) for k, b in stage_bytes[p].items()
+                      if stages[p].get(k)} for p in stages}
     kick_ms = kick_ms_live
     dom = max(stages["mixed"], key=lambda k: stages["mixed"][k])
     achieved = kick_b / (kick_ms * 1e-3) / 1e9
@@ -420,6 +428,7 @@ def main():
         "field_rms_rel_err": field_rms,
         "field_max_rel_err": field_max,
         "stages_ms": stages,
+        "stages_GBs": stages_gbs,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "other_configs": extra,
