@@ -263,10 +263,10 @@ def apply(rparticles, q_array, E, zstep, nmesh_xyz, mesh_scale=None, mesh_shift=
     return rparticles
 
 
-def drift(rparticles, L, E):
-    """First-order drift of length L (ocelot/cpbd/r_matrix.py:41-91 with
-    k1 = hx = 0: R01 = R23 = L, R45 = -L/(beta^2 gamma^2)); applied as
-    R.X (ocelot/cpbd/transformations/transfer_map.py:40-50)."""
+def drift_matrix(L, E):
+    """First-order R matrix of a drift of length L at energy E [GeV]
+    (ocelot/cpbd/r_matrix.py:41-91 with k1 = hx = 0): the identity with
+    R01 = R23 = L and R45 = -L/(beta^2 gamma^2)."""
     gamma = E / M_E_GEV
     igamma2 = 1. / (gamma * gamma) if gamma != 0 else 0.
     beta = np.sqrt(1. - igamma2)
@@ -274,5 +274,13 @@ def drift(rparticles, L, E):
     R[0, 1] = L
     R[2, 3] = L
     R[4, 5] = -L / (beta * beta) * igamma2
-    rparticles[:] = np.dot(R, rparticles)
+    return R
+
+
+def drift(rparticles, L, E):
+    """First-order drift of length L, applied as R.X
+    (ocelot/cpbd/transformations/transfer_map.py:40-50); pinned by the
+    reference's own Drift element R matrices and tracked particles
+    (tests/golden/track_golden.npz)."""
+    rparticles[:] = np.dot(drift_matrix(L, E), rparticles)
     return rparticles
