@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the context measurements of the other configurations")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the per-configuration errors against the FP64 oracle")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run one elementwise kernel reading and writing 1 GiB each, the reference "
                          "for scaling profiler byte counters")
@@ -111,7 +113,9 @@ def algorithmic_bytes(n, nmesh, prec):
 def cpu_baseline(name, n_sample, steps=3, seed=0):
     """FP64 oracle tracking steps (numpy/scipy, the reference's fallback
     library calls): ``steps`` x [drift, kick] on a particle sample of the
-    workload (the full bunch by default), BLAS limited to one thread."""
+    workload (the full bunch by default), BLAS limited to one thread.
+    Returns the baseline line and the oracle's result (final rows and the
+    cell indices of the first kick) for the accuracy check."""
     from oracle import sc_oracle as orc
     from ocelot_amd.workloads import make_bunch
     try:
@@ -120,12 +124,14 @@ def cpu_baseline(name, n_sample, steps=3, seed=0):
         threadpool_limits = None
     # same distribution and total charge
     r, q, E, nmesh, dz = make_bunch(name, n=n_sample, seed=seed)
+    r0 = r.copy()
+    dbg = {}
 
     def run():
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for k in range(steps):
             orc.drift(r, dz, E)
-            orc.apply(r, q, E, dz, nmesh)
+            orc.apply(r, q, E, dz, nmesh, debug=dbg if k == 0 else None)
         return time.perf_counter() - t0
 
     if threadpool_limits is not None:
@@ -139,9 +145,114 @@ def cpu_baseline(name, n_sample, steps=3, seed=0):
                 fft="numpy.fft (the reference's fallback branch)",
                 pyfftw_installed=importlib.util.find_spec("pyfftw") is not None,
                 numpy=np.__version__, scipy=scipy.__version__)
-    return dict(value=n_sample * steps / dt, unit="macroparticle-kicks/s", cores=1, kind="port",
+    line = dict(value=n_sample * steps / dt, unit="macroparticle-kicks/s", cores=1, kind="port",
                 sample=f"{name}: {steps} FP64 oracle steps (drift {dz} m + kick) of {n_sample} particles on "
                        f"{nmesh[0]}^3, {dt:.2f} s", host=host)
+    return line, dict(r0=r0, q=q, E=E, nmesh=nmesh, dz=dz, steps=steps, r=r, inds=dbg["inds"])
+
+
+def kick_errors(r0, got, ref):
+    """Errors of a kicked bunch against the oracle's: rows (max relative to
+    each row's largest value), the kick = momentum change of rows x', y',
+    delta (max relative to its largest value, with the row's FP64 floor, and
+    RMS relative); the kick is the lab-frame field at the particles times
+    fixed factors, so its RMS error is the field's."""
+    eps = np.finfo(np.float64).eps
+    row = max(float(np.max(np.abs(got[i] - ref[i])) / np.max(np.abs(ref[i]))) for i in range(6))
+    kmax, krms = 0.0, 0.0
+    for i in (1, 3, 5):
+        dg, dr = got[i] - r0[i], ref[i] - r0[i]
+        floor = 16 * eps * ((1.0 if i == 5 else 0.0) + np.max(np.abs(ref[i])))
+        kmax = max(kmax, float(np.max(np.abs(dg - dr)) / (np.max(np.abs(dr)) + floor / 1e-9)))
+        krms = max(krms, float(np.linalg.norm(dg - dr) / np.linalg.norm(dr)))
+    return {"row_max_rel": row, "kick_max_rel": kmax, "kick_rms_rel": krms}
+
+
+def errors_vs_oracle(torch, dev, c3):
+    """Accuracy of both forms against the FP64 oracle, per configuration:
+    the 200k and 2M Gaussian bunches on 63^3 (one kick, full size), the
+    10M flat-top on 127^3 (the CPU baseline's oracle steps, full size), four
+    bunches of the 64-bunch set (the smallest, the largest, the smallest per
+    mesh size; one batched kick) and the 50M hollow bunch on 255^3 (at full
+    size mixed against FP64; against the oracle on 100k particles on meshes
+    255 long on one axis, the oracle's 509^3 transform being minutes long).
+    Cell mismatches: the deposit kernel's NGP cells against the oracle's."""
+    from oracle import sc_oracle as orc
+    from ocelot_amd import DeviceParticleArray, ParticleArray, SpaceCharge, apply_batched, drift, track_drift
+    from ocelot_amd.workloads import batch64, make_bunch
+
+    def pa(r, q, E):
+        p = ParticleArray()
+        p.rparticles, p.q_array, p.E = r.copy(), q, E
+        return p
+
+    def one_kick(name, n=None, nmesh=None):
+        r, q, E, nm, dz = make_bunch(name, n=n)
+        nm = nm if nmesh is None else nmesh
+        ro = r.copy()
+        dbg = {}
+        orc.apply(ro, q, E, dz, nm, debug=dbg)
+        out = {"particles": r.shape[1], "nmesh": list(nm)}
+        for prec in ("fp64", "mixed"):
+            d = DeviceParticleArray.from_host(pa(r, q, E), dev)
+            cells = SpaceCharge(nmesh_xyz=list(nm), precision=prec).apply_cells(d, dz).cpu().numpy()
+            out[prec] = kick_errors(r, d.rparticles.cpu().numpy(), ro)
+            out[prec]["cell_mismatches"] = int(np.sum(cells != dbg["inds"]))
+        return out
+
+    res = {"C1_gauss_200k_63": one_kick("gauss_200k_63"), "C2_gauss_2m_63": one_kick("gauss_2m_63")}
+    if c3 is not None:
+        out = {"particles": c3["r0"].shape[1], "nmesh": list(c3["nmesh"]), "steps": c3["steps"]}
+        for prec in ("fp64", "mixed"):
+            sc = SpaceCharge(nmesh_xyz=list(c3["nmesh"]), precision=prec)
+            d = DeviceParticleArray.from_host(pa(c3["r0"], c3["q"], c3["E"]), dev)
+            track_drift(d, sc, c3["dz"], c3["dz"], c3["steps"], use_graph=True)
+            out[prec] = kick_errors(c3["r0"], d.rparticles.cpu().numpy(), c3["r"])
+            d = DeviceParticleArray.from_host(pa(c3["r0"], c3["q"], c3["E"]), dev)
+            drift(d, c3["dz"])
+            cells = sc.apply_cells(d, c3["dz"]).cpu().numpy()
+            out[prec]["cell_mismatches_first_kick"] = int(np.sum(cells != c3["inds"]))
+        res["C3_flattop_10m_127"] = out
+    bunches = batch64(seed=0)
+    sizes = [b["r"].shape[1] for b in bunches]
+    picks = {int(np.argmin(sizes)), int(np.argmax(sizes))}
+    for m in (31, 47, 63):
+        cand = [i for i, b in enumerate(bunches) if b["nmesh"][0] == m]
+        if cand:
+            picks.add(min(cand, key=lambda i: sizes[i]))
+    picks = sorted(picks)
+    worst = {"fp64": {}, "mixed": {}}
+    for prec in ("fp64", "mixed"):
+        ds = [DeviceParticleArray.from_host(pa(bunches[i]["r"], bunches[i]["q"], bunches[i]["E"]), dev) for i in picks]
+        apply_batched([(d, bunches[i]["nmesh"], 0.1) for d, i in zip(ds, picks)], precision=prec, nkicks=1)
+        for d, i in zip(ds, picks):
+            b = bunches[i]
+            if prec == "fp64":
+                b["ro"] = b["r"].copy()
+                orc.apply(b["ro"], b["q"], b["E"], 0.1, b["nmesh"])
+            e = kick_errors(b["r"], d.rparticles.cpu().numpy(), b["ro"])
+            for k, v in e.items():
+                worst[prec][k] = max(worst[prec].get(k, 0.0), v)
+    res["C4_batch64_4_bunches"] = {"bunches": picks, "particles": [sizes[i] for i in picks], **worst}
+    # C5: full size, mixed against FP64 (device), then the 255-long axes vs the oracle
+    r, q, E, nm, dz = make_bunch("hollow_50m_255")
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        d = DeviceParticleArray.from_host(pa(r, q, E), dev)
+        SpaceCharge(nmesh_xyz=list(nm), precision=prec).apply(d, dz)
+        outs[prec] = d.rparticles
+        del d
+    r0 = torch.as_tensor(r, device=outs["fp64"].device)
+    rms = max(float(torch.linalg.norm((outs["mixed"][i] - r0[i]) - (outs["fp64"][i] - r0[i]))
+                    / torch.linalg.norm(outs["fp64"][i] - r0[i])) for i in (1, 3, 5))
+    del outs, r0, r, q
+    axes = {}
+    for nmesh in ((255, 15, 15), (15, 255, 15), (15, 15, 255)):
+        e = one_kick("hollow_50m_255", n=100_000, nmesh=nmesh)
+        axes["x".join(map(str, nmesh))] = {p: e[p] for p in ("fp64", "mixed")}
+    res["C5_hollow_50m_255"] = {"particles": 50_000_000, "mixed_vs_fp64_kick_rms_rel": rms,
+                                "axes_255_vs_oracle_100k": axes}
+    return res
 
 
 def other_configs(torch, dev):
@@ -408,9 +519,10 @@ def main():
 
     extra = None if a.no_extra else other_configs(torch, dev)
 
-    cpu = None
+    cpu, c3 = None, None
     if a.cpu_sample > 0:
-        cpu = cpu_baseline(a.workload, min(a.cpu_sample, n))
+        cpu, c3 = cpu_baseline(a.workload, min(a.cpu_sample, n))
+    acc = None if a.no_accuracy else errors_vs_oracle(torch, dev, c3)
 
     value = n / t_mix
     out = {
@@ -431,6 +543,7 @@ def main():
         "stages_GBs": stages_gbs,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "errors_vs_oracle": acc,
         "other_configs": extra,
         "gpu_over_cpu": None if cpu is None else round(value / cpu["value"], 1),
     }

@@ -64,6 +64,15 @@ int ocsc_apply(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_ar
                const int32_t nmesh[3], double E_GeV, double zstep, int random_mesh, double mesh_scale,
                double mesh_shift, int precision, int sync, void* stream);
 
+/* ocsc_apply that also writes, for every particle, the flat C-order NGP cell
+ * index its charge was deposited to (the inds of ocelot/cpbd/sc.py:187-192:
+ * (floor(X0)+1) ny nz + (floor(X1)+1) nz + floor(X2)+1), as computed by the
+ * deposit kernel of this kick, into the device array cells[n] (int32).  With
+ * cells == NULL it is ocsc_apply. */
+int ocsc_apply_cells(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,
+                     const int32_t nmesh[3], double E_GeV, double zstep, int random_mesh, double mesh_scale,
+                     double mesh_shift, int precision, int sync, int32_t* cells, void* stream);
+
 /* Error flags left by kicks issued with sync == 0 (synchronises stream);
  * returns OCSC_EVALUE if any kick saw an invalid bunch and clears the flag. */
 int ocsc_check(ocsc_ctx* ctx, void* stream);

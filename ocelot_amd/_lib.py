@@ -3,9 +3,12 @@
 The library is the product path: there is no CPU fallback.  If it is missing
 or fails to load, every entry point raises.
 """
+import contextlib
 import ctypes
 import os
 import threading
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("OCSC_LIB", os.path.join(HERE, "libocsc.so"))
@@ -31,6 +34,8 @@ SIGNATURES = {
     "ocsc_destroy": (_c_int, [_vp]),
     "ocsc_last_error": (ctypes.c_char_p, []),
     "ocsc_apply": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _c_int, _dbl, _dbl, _c_int, _c_int, _vp]),
+    "ocsc_apply_cells": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _c_int, _dbl, _dbl, _c_int, _c_int,
+                                  _vp, _vp]),
     "ocsc_check": (_c_int, [_vp, _vp]),
     "ocsc_el_field": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _c_int, _dbl, _dbl, _c_int, _vp, _i64, _vp,
                                _vp]),
@@ -129,7 +134,10 @@ def d3(v):
 
 
 class Context:
-    """One library context (device workspaces and caches) per device."""
+    """One library context (device workspaces and caches) per device.  A
+    context must not be used by two host threads at once (ctypes releases the
+    GIL during a call), so every use holds ``lock`` from the first library
+    call to the last host copy that depends on it (see ``using``)."""
 
     def __init__(self, device):
         lib = load()
@@ -137,6 +145,7 @@ class Context:
         check(lib.ocsc_create(ctypes.byref(h), int(device)))
         self.handle = h
         self.device = int(device)
+        self.lock = threading.RLock()
 
     def __del__(self):
         try:
@@ -147,12 +156,14 @@ class Context:
 
 
 def set_option(device, name, value):
-    check(load().ocsc_set_option(context(device).handle, name.encode(), int(value)))
+    with using(device) as h:
+        check(load().ocsc_set_option(h, name.encode(), int(value)))
 
 
 def get_option(device, name):
     v = _i64()
-    check(load().ocsc_get_option(context(device).handle, name.encode(), ctypes.byref(v)))
+    with using(device) as h:
+        check(load().ocsc_get_option(h, name.encode(), ctypes.byref(v)))
     return v.value
 
 
@@ -171,3 +182,21 @@ def context(device):
             # the other context is released when it goes out of scope
             ctx = _contexts.setdefault(device, ctx)
     return ctx
+
+
+@contextlib.contextmanager
+def using(device):
+    """The handle of the device's context, used by this thread alone for the
+    duration of the block (calls from other threads on the same device wait)."""
+    ctx = context(device)
+    with ctx.lock:
+        yield ctx.handle
+
+
+def check_charges(q, n):
+    """The charges must be one value per particle: the reference fails with
+    ValueError in np.dot(Q, X) otherwise (ocelot/cpbd/sc.py:182); the kernels
+    would read past a short array."""
+    shape = tuple(q.shape) if hasattr(q, "shape") else np.shape(q)
+    if shape != (int(n),):
+        raise ValueError(f"q_array must hold one charge per particle: shape {shape}, {int(n)} particles")

@@ -65,6 +65,9 @@ class DeviceParticleArray:
             raise TypeError("rparticles must be a (6, n) float64 tensor")
         if rparticles.stride(1) != 1:
             raise TypeError("rparticles rows must be contiguous")
+        if tuple(q_array.shape) != (rparticles.shape[1],):
+            raise ValueError(f"q_array must hold one charge per particle: shape {tuple(q_array.shape)}, "
+                             f"{rparticles.shape[1]} particles")
         self.rparticles = rparticles
         self.q_array = q_array.to(device=rparticles.device, dtype=torch.float64).contiguous()
         self.E = float(E)

@@ -31,8 +31,8 @@ def csr_convolution(a, b, device=None):
     out = np.empty(max(a.size + b.size - 2, 0))
     dev = _dev(device)
     torch = _torch()
-    with torch.cuda.device(dev):
-        _lib.check(lib.ocsc_csr_convolution(_lib.context(dev).handle, a.ctypes.data_as(_lib._dp), a.size,
+    with torch.cuda.device(dev), _lib.using(dev) as h:
+        _lib.check(lib.ocsc_csr_convolution(h, a.ctypes.data_as(_lib._dp), a.size,
                                             b.ctypes.data_as(_lib._dp), b.size, out.ctypes.data_as(_lib._dp),
                                             torch.cuda.current_stream(dev).cuda_stream))
     return out
@@ -53,14 +53,14 @@ def csr_kick(p_array, lam_ds, K1, s1, s2, Ns, delta_s, device=None):
         if r.dtype != torch.float64 or r.stride(1) != 1 or r.shape[0] != 6:
             raise TypeError("device rparticles must be a (6, n) float64 tensor with contiguous rows")
         dev = r.device.index
-        with torch.cuda.device(dev):
-            _lib.check(lib.ocsc_csr_kick(_lib.context(dev).handle, r.data_ptr(), r.stride(0), n, float(p_array.E),
+        with torch.cuda.device(dev), _lib.using(dev) as h:
+            _lib.check(lib.ocsc_csr_kick(h, r.data_ptr(), r.stride(0), n, float(p_array.E),
                                          *args, torch.cuda.current_stream(dev).cuda_stream))
         return out
     dev = _dev(device)
-    with torch.cuda.device(dev):
+    with torch.cuda.device(dev), _lib.using(dev) as h:
         rt = torch.from_numpy(np.ascontiguousarray(r, dtype=np.float64)).to(f"cuda:{dev}")
-        _lib.check(lib.ocsc_csr_kick(_lib.context(dev).handle, rt.data_ptr(), n, n, float(p_array.E), *args,
+        _lib.check(lib.ocsc_csr_kick(h, rt.data_ptr(), n, n, float(p_array.E), *args,
                                      torch.cuda.current_stream(dev).cuda_stream))
         r[5] = rt[5].cpu().numpy()
     return out

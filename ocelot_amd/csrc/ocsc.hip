@@ -501,6 +501,7 @@ struct StepOpts {
   DriftArgs dr{0, 0};
   hipEvent_t kev0 = nullptr, kev1 = nullptr;  // recorded around the kick kernel (kick timing)
   bool kev_external = false;                  // capturing: the graph gets event-record nodes instead
+  int32_t* cells = nullptr;                   // deposit writes each particle's NGP cell here (or null)
 };
 
 template <typename F, bool DRIFT, bool SUMP>
@@ -578,9 +579,9 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   }
   const int ndep = (int)((n + kDepPT * kDepItems - 1) / (kDepPT * kDepItems));
   if (c->deposit_xcd)
-    k_deposit<NWORD, true><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
+    k_deposit<NWORD, true><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p, o.cells);
   else
-    k_deposit<NWORD, false><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p);
+    k_deposit<NWORD, false><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p, o.cells);
   LAUNCHCHK("deposit");
   mark(3);
   if (!fork && (rc = launch_kernel_spectrum<F>(c, L, g, st, 0))) return rc;
@@ -804,6 +805,13 @@ int ocsc_destroy(ocsc_ctx* c) {
 int ocsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3], double E_GeV,
                double zstep, int random_mesh, double mesh_scale, double mesh_shift, int prec, int sync,
                void* stream) {
+  return ocsc_apply_cells(c, r, ld, q, n, nmesh, E_GeV, zstep, random_mesh, mesh_scale, mesh_shift, prec, sync,
+                          nullptr, stream);
+}
+
+int ocsc_apply_cells(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
+                     double E_GeV, double zstep, int random_mesh, double mesh_scale, double mesh_shift, int prec,
+                     int sync, int32_t* cells, void* stream) {
   if (c && zstep == 0) return OCSC_OK;  // sc.py:210-212
   int rc;
   if ((rc = check_common(c, r, q, n, ld))) return rc;
@@ -818,7 +826,9 @@ int ocsc_apply(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   if ((rc = ensure_part(c, L, n))) return rc;
   hipStream_t st = (hipStream_t)stream;
   KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, random_mesh, mesh_scale, mesh_shift);
-  if ((rc = enqueue_kick_prec(c, L, g, r, ld, q, in, prec, st))) return rc;
+  StepOpts o;
+  o.cells = cells;
+  if ((rc = enqueue_kick_prec(c, L, g, r, ld, q, in, prec, st, o))) return rc;
   if (sync) {
     int32_t err = 0;
     if ((rc = read_errors(c, L, st, &err))) return rc;

@@ -453,9 +453,11 @@ __device__ __forceinline__ void flush_cell(unsigned long long* g, int64_t ncell,
   if (nword == 2) grid_add<XCD>(g, ncell + idx, lo);
 }
 
+// cells (may be null): receives the flat cell index each particle's charge
+// went to (-1 outside the mesh), in the order of the rows
 template <int NWORD, bool XCD = false>
 __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
-                                                 unsigned long long* __restrict__ grid) {
+                                                 unsigned long long* __restrict__ grid, int32_t* __restrict__ cells) {
   constexpr int kSlots = NWORD == 2 ? kDepSlots2 : kDepSlots;
   __shared__ int hkey[kSlots];
   __shared__ long long hval[NWORD][kSlots];
@@ -487,6 +489,7 @@ __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __rest
     const int64_t pn = p + kDepPT;
     if (it + 1 < kDepItems && pn < s.n) load(pn, nxt);
     const int64_t idx = ngp_cell<NWORD == 1>(cur, s);  // one word: the mixed form
+    if (cells) cells[p] = (int32_t)idx;
     const double qv = cur[6];
 #pragma unroll
     for (int k = 0; k < 7; ++k) cur[k] = nxt[k];

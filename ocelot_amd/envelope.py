@@ -92,9 +92,10 @@ def moments(p_array, bounds=None, auto_disp=False, disp=None, device=None, imax=
         q = torch.from_numpy(np.ascontiguousarray(p_array.q_array, dtype=np.float64)).to(f"cuda:{dev}")
         ld = rt.shape[1]
     n = int(rt.shape[1])
-    with torch.cuda.device(dev):
+    _lib.check_charges(q, n)
+    with torch.cuda.device(dev), _lib.using(dev) as h:
         stream = torch.cuda.current_stream(dev).cuda_stream
-        _lib.check(lib.ocsc_moments(_lib.context(dev).handle, rt.data_ptr(), ld, q.data_ptr(), n, flags, dbuf,
+        _lib.check(lib.ocsc_moments(h, rt.data_ptr(), ld, q.data_ptr(), n, flags, dbuf,
                                     b0, b1, out, 27, stream))
     return np.array(out[:])
 

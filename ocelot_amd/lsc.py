@@ -103,8 +103,8 @@ class LSC(PhysProc):
         out = np.empty_like(w)
         dev = _device_of(self)
         torch = _torch()
-        with torch.cuda.device(dev):
-            _lib.check(lib.ocsc_lsc_impedance(_lib.context(dev).handle, w.ctypes.data_as(_lib._dp), w.size,
+        with torch.cuda.device(dev), _lib.using(dev) as h:
+            _lib.check(lib.ocsc_lsc_impedance(h, w.ctypes.data_as(_lib._dp), w.size,
                                               float(gamma), float(sigma), float(dz), int(step),
                                               out.ctypes.data_as(_lib._dp), torch.cuda.current_stream(dev).cuda_stream))
         return out
@@ -120,8 +120,8 @@ class LSC(PhysProc):
         out = np.empty_like(b)
         dev = _device_of(self)
         torch = _torch()
-        with torch.cuda.device(dev):
-            _lib.check(lib.ocsc_lsc_wake(_lib.context(dev).handle, s.ctypes.data_as(_lib._dp),
+        with torch.cuda.device(dev), _lib.using(dev) as h:
+            _lib.check(lib.ocsc_lsc_wake(h, s.ctypes.data_as(_lib._dp),
                                          b.ctypes.data_as(_lib._dp), b.size, float(gamma), float(sigma), float(dz),
                                          int(bool(self.step_profile)), float(K_max), float(fill_factor),
                                          out.ctypes.data_as(_lib._dp), torch.cuda.current_stream(dev).cuda_stream))
@@ -168,6 +168,7 @@ class LSC(PhysProc):
         lib = _lib.load()
         r = p_array.rparticles
         n = int(r.shape[1])
+        _lib.check_charges(p_array.q_array, n)
         b0, b1 = (float(v) for v in self.bounds)
         info = (ctypes.c_double * 8)()
         args = (float(p_array.E), float(dz), int(bool(self.step_profile)), float(self.smooth_param), b0, b1,
@@ -180,20 +181,20 @@ class LSC(PhysProc):
             if not _is_cuda_tensor(q):
                 q = torch.as_tensor(np.asarray(q, dtype=np.float64), device=r.device)
             q = q.to(dtype=torch.float64).contiguous()
-            with torch.cuda.device(dev):
+            with torch.cuda.device(dev), _lib.using(dev) as h:
                 stream = torch.cuda.current_stream(dev).cuda_stream
-                _lib.check(lib.ocsc_lsc_apply(_lib.context(dev).handle, r.data_ptr(), r.stride(0), q.data_ptr(), n,
+                _lib.check(lib.ocsc_lsc_apply(h, r.data_ptr(), r.stride(0), q.data_ptr(), n,
                                               *args, stream))
         else:
             dev = _device_of(self)
             direct = (isinstance(r, np.ndarray) and r.dtype == np.float64 and r.flags.c_contiguous
                       and r.flags.writeable and r.size > 0)
-            with torch.cuda.device(dev):
+            with torch.cuda.device(dev), _lib.using(dev) as h:
                 host = torch.from_numpy(r if direct else np.ascontiguousarray(r, dtype=np.float64))
                 rt = host.to(f"cuda:{dev}")
                 qt = torch.from_numpy(np.ascontiguousarray(p_array.q_array, dtype=np.float64)).to(f"cuda:{dev}")
                 stream = torch.cuda.current_stream(dev).cuda_stream
-                _lib.check(lib.ocsc_lsc_apply(_lib.context(dev).handle, rt.data_ptr(), n, qt.data_ptr(), n, *args,
+                _lib.check(lib.ocsc_lsc_apply(h, rt.data_ptr(), n, qt.data_ptr(), n, *args,
                                               stream))
                 if direct:
                     host[5].copy_(rt[5])   # only delta changes: one row back, into the caller's ndarray
@@ -207,14 +208,14 @@ class LSC(PhysProc):
         device: the arrays x, B[:, 1] and W of the reference's apply."""
         lib = _lib.load()
         dev = _device_of(self) if device is None else int(device)
-        h = _lib.context(dev).handle
-        nb = ctypes.c_int64(0)
-        rc = lib.ocsc_lsc_profile(h, None, None, None, 0, ctypes.byref(nb))
-        if nb.value == 0:
-            _lib.check(rc)
-        s, cur, W = (np.empty(nb.value) for _ in range(3))
-        _lib.check(lib.ocsc_lsc_profile(h, s.ctypes.data_as(_lib._dp), cur.ctypes.data_as(_lib._dp),
-                                        W.ctypes.data_as(_lib._dp), nb.value, ctypes.byref(nb)))
+        with _lib.using(dev) as h:
+            nb = ctypes.c_int64(0)
+            rc = lib.ocsc_lsc_profile(h, None, None, None, 0, ctypes.byref(nb))
+            if nb.value == 0:
+                _lib.check(rc)
+            s, cur, W = (np.empty(nb.value) for _ in range(3))
+            _lib.check(lib.ocsc_lsc_profile(h, s.ctypes.data_as(_lib._dp), cur.ctypes.data_as(_lib._dp),
+                                            W.ctypes.data_as(_lib._dp), nb.value, ctypes.byref(nb)))
         return s, cur, W
 
     def __repr__(self) -> str:

@@ -85,11 +85,18 @@ def _nmesh_of(proc):
     return _lib.i32x3(nm)
 
 
-def _draws_of(proc):
-    # sc.py:174-175 then sc.py:184-185, global numpy RNG
+def _draws_of(proc, q=None, n=None):
+    """The two random-mesh draws of one kick, sc.py:174-175 then sc.py:184-185,
+    global numpy RNG.  With q, the charges are checked between the draws,
+    where the reference's np.dot(Q, X) (sc.py:182) raises for a q_array of
+    the wrong length."""
     if not proc.random_mesh:
+        if q is not None:
+            _lib.check_charges(q, n)
         return 0, 1.0, 0.0
     scale = np.random.uniform(low=1, high=1.1)
+    if q is not None:
+        _lib.check_charges(q, n)
     shift = np.random.uniform(low=-0.5, high=0.5)
     return 1, float(scale), float(shift)
 
@@ -101,9 +108,53 @@ def _kick_args(proc, p_array, zstep):
     nmesh = _nmesh_of(proc)
     prec = _prec_of(proc)
     E = float(p_array.E)
-    rm, scale, shift = _draws_of(proc)
+    rm, scale, shift = _draws_of(proc, p_array.q_array, n)
     return dict(n=n, nmesh=nmesh, prec=prec, E=E, zstep=float(zstep), random_mesh=rm, mesh_scale=scale,
                 mesh_shift=shift)
+
+
+def _kick(proc, p_array, zstep, want_cells):
+    """One kick of SpaceCharge.apply (sc.py:208-251) on host or device rows;
+    with ``want_cells`` the deposit's cell indices are returned."""
+    a = _kick_args(proc, p_array, zstep)
+    torch = _torch()
+    lib = _lib.load()
+    r = p_array.rparticles
+    n, nmesh, prec, E = a["n"], a["nmesh"], a["prec"], a["E"]
+    rm, scale, shift = a["random_mesh"], a["mesh_scale"], a["mesh_shift"]
+    if _is_cuda_tensor(r):
+        dev = _device_of(proc, r)
+        if r.dtype != torch.float64 or r.stride(1) != 1 or r.shape[0] != 6:
+            raise TypeError("device rparticles must be a (6, n) float64 tensor with contiguous rows")
+        q = p_array.q_array
+        if not _is_cuda_tensor(q):
+            q = torch.as_tensor(np.asarray(q, dtype=np.float64), device=r.device)
+        q = q.to(device=r.device, dtype=torch.float64).contiguous()
+        cells = torch.empty(n, dtype=torch.int32, device=r.device) if want_cells else None
+        with torch.cuda.device(dev), _lib.using(dev) as h:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+            _lib.check(lib.ocsc_apply_cells(h, r.data_ptr(), r.stride(0), q.data_ptr(), n, nmesh, E, float(zstep),
+                                            rm, scale, shift, prec, 1,
+                                            cells.data_ptr() if cells is not None else None, stream))
+        return cells
+    dev = _device_of(proc)
+    # a C-ordered, writable float64 ndarray is copied to and from the GPU
+    # as it lies (no host-side temporaries); anything else goes through a
+    # contiguous copy and is assigned back element-wise
+    direct = _host_direct(r)
+    with torch.cuda.device(dev), _lib.using(dev) as h:
+        host = torch.from_numpy(r if direct else np.ascontiguousarray(r, dtype=np.float64))
+        rt = host.to(f"cuda:{dev}")
+        qt = torch.from_numpy(np.ascontiguousarray(p_array.q_array, dtype=np.float64)).to(f"cuda:{dev}")
+        cells = torch.empty(n, dtype=torch.int32, device=rt.device) if want_cells else None
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        _lib.check(lib.ocsc_apply_cells(h, rt.data_ptr(), n, qt.data_ptr(), n, nmesh, E, float(zstep), rm, scale,
+                                        shift, prec, 1, cells.data_ptr() if cells is not None else None, stream))
+        if direct:
+            host.copy_(rt)
+        else:
+            r[...] = rt.cpu().numpy()
+    return cells.cpu().numpy() if cells is not None else None
 
 
 class SpaceCharge(PhysProc):
@@ -149,49 +200,24 @@ class SpaceCharge(PhysProc):
         """Host-side arguments of one kick, in the order the reference
         consumes them: mesh, precision, energy, then the random-mesh draws.
         Raises ValueError for an empty bunch before drawing (the reference
-        fails in the extent reduction, sc.py:173, before its first draw)."""
+        fails in the extent reduction, sc.py:173, before its first draw) and
+        for a q_array of the wrong length between the two draws (np.dot(Q, X),
+        sc.py:182)."""
         return _kick_args(self, p_array, zstep)
 
     def apply(self, p_array, zstep):
         if zstep == 0:
             return
-        a = _kick_args(self, p_array, zstep)
-        torch = _torch()
-        lib = _lib.load()
-        r = p_array.rparticles
-        n, nmesh, prec, E = a["n"], a["nmesh"], a["prec"], a["E"]
-        rm, scale, shift = a["random_mesh"], a["mesh_scale"], a["mesh_shift"]
-        if _is_cuda_tensor(r):
-            dev = _device_of(self, r)
-            if r.dtype != torch.float64 or r.stride(1) != 1 or r.shape[0] != 6:
-                raise TypeError("device rparticles must be a (6, n) float64 tensor with contiguous rows")
-            q = p_array.q_array
-            if not _is_cuda_tensor(q):
-                q = torch.as_tensor(np.asarray(q, dtype=np.float64), device=r.device)
-            q = q.to(dtype=torch.float64).contiguous()
-            ctx = _lib.context(dev)
-            with torch.cuda.device(dev):
-                stream = torch.cuda.current_stream(dev).cuda_stream
-                _lib.check(lib.ocsc_apply(ctx.handle, r.data_ptr(), r.stride(0), q.data_ptr(), n, nmesh, E,
-                                          float(zstep), rm, scale, shift, prec, 1, stream))
-            return
-        dev = _device_of(self)
-        ctx = _lib.context(dev)
-        # a C-ordered, writable float64 ndarray is copied to and from the GPU
-        # as it lies (no host-side temporaries); anything else goes through a
-        # contiguous copy and is assigned back element-wise
-        direct = _host_direct(r)
-        with torch.cuda.device(dev):
-            host = torch.from_numpy(r if direct else np.ascontiguousarray(r, dtype=np.float64))
-            rt = host.to(f"cuda:{dev}")
-            qt = torch.from_numpy(np.ascontiguousarray(p_array.q_array, dtype=np.float64)).to(f"cuda:{dev}")
-            stream = torch.cuda.current_stream(dev).cuda_stream
-            _lib.check(lib.ocsc_apply(ctx.handle, rt.data_ptr(), n, qt.data_ptr(), n, nmesh, E, float(zstep),
-                                      rm, scale, shift, prec, 1, stream))
-            if direct:
-                host.copy_(rt)
-            else:
-                r[...] = rt.cpu().numpy()
+        _kick(self, p_array, zstep, want_cells=False)
+
+    def apply_cells(self, p_array, zstep):
+        """``apply``, returning for every particle the flat C-order NGP cell
+        index (int32; the ``inds`` of sc.py:187-192) its charge was deposited
+        to by this kick's deposit kernel: a CUDA tensor for device rows, a
+        numpy array for host rows; None for zstep == 0 (no kick)."""
+        if zstep == 0:
+            return None
+        return _kick(self, p_array, zstep, want_cells=True)
 
     # ------------------------------------------------------------------ sub-steps
     def sym_kernel(self, ijk2, hxyz):
@@ -202,11 +228,10 @@ class SpaceCharge(PhysProc):
         dev = _device_of(self)
         shape = tuple(int(v) for v in ijk2)
         out = torch.empty(shape, dtype=torch.float64, device=f"cuda:{dev}")
-        ctx = _lib.context(dev)
-        with torch.cuda.device(dev):
+        with torch.cuda.device(dev), _lib.using(dev) as ch:
             stream = torch.cuda.current_stream(dev).cuda_stream
             h = (ctypes.c_double * 3)(*[float(v) for v in hxyz])
-            _lib.check(lib.ocsc_sym_kernel(ctx.handle, _lib.i32x3(shape), h, out.data_ptr(), stream))
+            _lib.check(lib.ocsc_sym_kernel(ch, _lib.i32x3(shape), h, out.data_ptr(), stream))
         return out.cpu().numpy()
 
     def potential(self, q, steps):
@@ -219,11 +244,10 @@ class SpaceCharge(PhysProc):
         rho = torch.from_numpy(np.ascontiguousarray(q, dtype=np.float64)).to(f"cuda:{dev}")
         dt = torch.float64 if prec == _lib.OCSC_FP64 else torch.float32
         phi = torch.empty(rho.shape, dtype=dt, device=rho.device)
-        ctx = _lib.context(dev)
-        with torch.cuda.device(dev):
+        with torch.cuda.device(dev), _lib.using(dev) as ch:
             stream = torch.cuda.current_stream(dev).cuda_stream
             h = (ctypes.c_double * 3)(*[float(v) for v in steps])
-            _lib.check(lib.ocsc_potential(ctx.handle, rho.data_ptr(), _lib.i32x3(rho.shape), h, phi.data_ptr(),
+            _lib.check(lib.ocsc_potential(ch, rho.data_ptr(), _lib.i32x3(rho.shape), h, phi.data_ptr(),
                                           prec, stream))
         return phi.double().cpu().numpy()
 
@@ -242,12 +266,11 @@ class SpaceCharge(PhysProc):
         qt = torch.from_numpy(np.ascontiguousarray(Q, dtype=np.float64)).to(f"cuda:{dev}")
         E = torch.empty((3, n), dtype=torch.float64, device=xyz.device)
         cells = torch.empty(n, dtype=torch.int32, device=xyz.device) if return_cells else None
-        rm, scale, shift = _draws_of(self)
+        rm, scale, shift = _draws_of(self, np.asarray(Q), n)
         nm = _lib.i32x3(np.array(nxyz).reshape(-1))
-        ctx = _lib.context(dev)
-        with torch.cuda.device(dev):
+        with torch.cuda.device(dev), _lib.using(dev) as ch:
             stream = torch.cuda.current_stream(dev).cuda_stream
-            _lib.check(lib.ocsc_el_field(ctx.handle, xyz.data_ptr(), n, qt.data_ptr(), n, nm, float(gamma), rm,
+            _lib.check(lib.ocsc_el_field(ch, xyz.data_ptr(), n, qt.data_ptr(), n, nm, float(gamma), rm,
                                          scale, shift, prec, E.data_ptr(), n,
                                          cells.data_ptr() if cells is not None else None, stream))
         X[:, 2] = X[:, 2] * gamma
@@ -262,7 +285,8 @@ class SpaceCharge(PhysProc):
         lib = _lib.load()
         dev = _device_of(self) if device is None else int(device)
         out = (ctypes.c_double * 20)()
-        _lib.check(lib.ocsc_last_stats(_lib.context(dev).handle, out, 20, None))
+        with _lib.using(dev) as ch:
+            _lib.check(lib.ocsc_last_stats(ch, out, 20, None))
         v = list(out)
         return dict(gamma0=v[0], Pav=v[1], T=np.array(v[2:11]).reshape(3, 3), steps=np.array(v[11:14]),
                     X_off=np.array(v[14:17]), sumq=v[17], err=int(v[18]), rebuilt=int(v[19]))
@@ -284,14 +308,14 @@ def drift(p_array, L, device=None):
     n = int(r.shape[1])
     if _is_cuda_tensor(r):
         dev = r.device.index
-        with torch.cuda.device(dev):
-            _lib.check(lib.ocsc_drift(_lib.context(dev).handle, r.data_ptr(), r.stride(0), n, float(L),
+        with torch.cuda.device(dev), _lib.using(dev) as h:
+            _lib.check(lib.ocsc_drift(h, r.data_ptr(), r.stride(0), n, float(L),
                                       float(p_array.E), torch.cuda.current_stream(dev).cuda_stream))
         return
     dev = torch.cuda.current_device() if device is None else int(device)
-    with torch.cuda.device(dev):
+    with torch.cuda.device(dev), _lib.using(dev) as h:
         rt = torch.from_numpy(np.ascontiguousarray(r, dtype=np.float64)).to(f"cuda:{dev}")
-        _lib.check(lib.ocsc_drift(_lib.context(dev).handle, rt.data_ptr(), n, n, float(L), float(p_array.E),
+        _lib.check(lib.ocsc_drift(h, rt.data_ptr(), n, n, float(L), float(p_array.E),
                                   torch.cuda.current_stream(dev).cuda_stream))
         r[...] = rt.cpu().numpy()
 
@@ -332,6 +356,7 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
     dev = r.device.index
     n = int(r.shape[1])
     nsteps = int(nsteps)
+    _lib.check_charges(p_array.q_array, n)
     if sc is not None and getattr(sc, "autotune", False) and nsteps > 0 and float(dz) != 0:
         from .autotune import tune_tracking
         tune_tracking(p_array, sc, L, dz)
@@ -383,11 +408,14 @@ def track_drift(p_array, sc, L, dz, nsteps, use_graph=True, calc_tws=False, boun
             opts.disp[k] = float(disp[k])
         out = (ctypes.c_double * ((nsteps + 1) * nout))()
         opts.tws_out = ctypes.cast(out, _lib._dp)
-    with torch.cuda.device(dev):
+    with torch.cuda.device(dev), _lib.using(dev) as h:
         stream = torch.cuda.current_stream(dev).cuda_stream
-        _lib.check(lib.ocsc_track(_lib.context(dev).handle, r.data_ptr(), r.stride(0), p_array.q_array.data_ptr(), n,
+        _lib.check(lib.ocsc_track(h, r.data_ptr(), r.stride(0), p_array.q_array.data_ptr(), n,
                                   nmesh, float(p_array.E), zstep, float(L), nsteps, prec, ctypes.byref(opts),
                                   stream))
+    # each step's drift advances the position, as TransferMap.apply does
+    # (ocelot/cpbd/transformations/transformation.py:131)
+    p_array.s = float(p_array.s) + nsteps * float(L)
     if not calc_tws:
         return None
     from .envelope import twiss_from_moments
@@ -423,6 +451,7 @@ def apply_batched(items, precision="fp64", nkicks=1, use_graph=True, random_mesh
             raise TypeError("apply_batched needs DeviceParticleArrays on one device")
         if r.shape[1] == 0:
             raise ValueError("zero-size array to reduction operation maximum which has no identity")
+        _lib.check_charges(p.q_array, r.shape[1])
         q = p.q_array.to(dtype=torch.float64).contiguous()
         keep.append(q)
         b.rparticles, b.ld, b.q_array, b.n = r.data_ptr(), r.stride(0), q.data_ptr(), r.shape[1]
@@ -435,7 +464,7 @@ def apply_batched(items, precision="fp64", nkicks=1, use_graph=True, random_mesh
             b.mesh_shift = float(np.random.uniform(low=-0.5, high=0.5))
         else:
             b.random_mesh, b.mesh_scale, b.mesh_shift = 0, 1.0, 0.0
-    with torch.cuda.device(dev):
-        _lib.check(lib.ocsc_apply_batched(_lib.context(dev).handle, ctypes.cast(arr, ctypes.c_void_p), len(items),
+    with torch.cuda.device(dev), _lib.using(dev) as h:
+        _lib.check(lib.ocsc_apply_batched(h, ctypes.cast(arr, ctypes.c_void_p), len(items),
                                           prec, int(nkicks), 1 if use_graph else 0,
                                           torch.cuda.current_stream(dev).cuda_stream))

@@ -241,10 +241,47 @@ def test_kick_args_on_a_foreign_process_object():
     import types
     ns = types.SimpleNamespace(nmesh_xyz=[31, 47, 63], random_mesh=True)
     sc = SpaceCharge(nmesh_xyz=[31, 47, 63], random_mesh=True)
-    p = types.SimpleNamespace(rparticles=np.zeros((6, 10)), E=0.13)
+    p = types.SimpleNamespace(rparticles=np.zeros((6, 10)), q_array=np.zeros(10), E=0.13)
     np.random.seed(4)
     a = SpaceCharge.kick_args(ns, p, 0.1)
     np.random.seed(4)
     b = sc.kick_args(p, 0.1)
     assert list(a.pop("nmesh")) == list(b.pop("nmesh")) == [31, 47, 63]
     assert a == b and a["prec"] == 0 and a["random_mesh"] == 1
+
+
+@pytest.mark.parametrize("q_len", [4, 6])
+def test_wrong_length_q_array_raises_between_the_draws(q_len):
+    """A q_array that is not one charge per particle raises ValueError before
+    anything reaches the device (the kernels would read past a short array).
+    The reference fails in np.dot(Q, X) (sc.py:182), after its first
+    random-mesh draw (sc.py:174-175) and before the second (sc.py:184-185):
+    exactly one draw is taken, and the particles are untouched."""
+    sc = SpaceCharge(random_mesh=True)
+    p = ParticleArray(5)
+    p.E = 0.13
+    p.rparticles[:] = np.arange(30.0).reshape(6, 5) * 1e-6
+    before = p.rparticles.copy()
+    p.q_array = np.full(q_len, 1e-13)
+    np.random.seed(4)
+    with pytest.raises(ValueError):
+        sc.apply(p, 0.1)
+    after = np.random.uniform()
+    rs = np.random.RandomState(4)
+    rs.uniform(low=1, high=1.1)
+    assert after == rs.uniform()
+    assert np.array_equal(p.rparticles, before)
+    with pytest.raises(ValueError):
+        SpaceCharge().apply(p, 0.1)
+
+
+def test_wrong_length_q_array_raises_in_lsc():
+    from ocelot_amd import LSC
+    p = ParticleArray(5)
+    p.E = 0.13
+    p.rparticles[4] = np.linspace(-1e-3, 1e-3, 5)
+    p.q_array = np.full(3, 1e-13)
+    before = p.rparticles.copy()
+    with pytest.raises(ValueError):
+        LSC().apply(p, 0.1)
+    assert np.array_equal(p.rparticles, before)
