@@ -4,7 +4,13 @@ axis) against the FP64 oracle, and the NGP cell indices of the deposit kernel
 
 Tolerances as in test_gpu_parity.py: FP64 rows within 1e-12 of each row's
 largest value and the kick within 1e-9 of its size (plus the row's FP64
-floor); mixed kicks within 1e-5 RMS relative; cell indices bit-exact.
+floor); mixed kicks within 1e-5 RMS relative; cell indices bit-exact.  The
+flat-top bunch at 500 MeV has rest-frame cells about a thousand times longer
+in z than across, where the FP64 Green's function cancels: the reference's
+own FP64 kick is then 5e-9 to 1.3e-8 away from the same algorithm with the
+kernel in extended precision.  There the FP64 result must lie within three
+times the reference's own deviation from that extended-precision answer
+(as for the 255-long axes in test_gpu_parity.py).
 """
 import numpy as np
 import pytest
@@ -15,7 +21,8 @@ from oracle import sc_oracle as orc
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
-from test_gpu_parity import APPLY_CASES, case, kick_err, parray, rel_max, rel_rms  # noqa: E402
+from test_gpu_parity import (APPLY_CASES, accuracy_vs_extended, case, kick_err, parray, rel_max,  # noqa: E402
+                             rel_rms, within_reference_accuracy)
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -66,9 +73,11 @@ def test_m256_axes_vs_oracle(nmesh):
     row = max(rel_max(p.rparticles[i], ro[i]) for i in range(6))
     kick = max(kick_err(p.rparticles, r, ro, i) for i in (1, 3, 5))
     kick_m = max(rel_rms(pm.rparticles[i] - r[i], ro[i] - r[i]) for i in (1, 3, 5))
-    print(f"mesh {nmesh}: fp64 row {row:.3e} kick {kick:.3e}; mixed kick rms {kick_m:.3e}; cell mismatches "
-          f"{int(np.sum(c64 != inds))} / {int(np.sum(cm != inds))}")
-    assert row <= 1e-12 and kick <= 1e-9 and kick_m <= 1e-5
+    g, o = accuracy_vs_extended(p.rparticles, r, q, E, dz, nmesh, ro)
+    print(f"mesh {nmesh}: fp64 vs oracle row {row:.3e} kick {kick:.3e}; vs extended-kernel oracle: hip row "
+          f"{g[0]:.3e} kick {g[1]:.3e}, reference row {o[0]:.3e} kick {o[1]:.3e}; mixed kick rms {kick_m:.3e}; "
+          f"cell mismatches {int(np.sum(c64 != inds))} / {int(np.sum(cm != inds))}")
+    assert within_reference_accuracy(g, o) and kick_m <= 1e-5
     assert np.array_equal(c64, inds) and np.array_equal(cm, inds)
 
 
@@ -89,8 +98,11 @@ def test_c3_127_one_kick_vs_oracle():
     kick = max(kick_err(out["fp64"], r, ro, i) for i in (1, 3, 5))
     kick_m = max(rel_rms(out["mixed"][i] - r[i], ro[i] - r[i]) for i in (1, 3, 5))
     mism = {p: int(np.sum(cells[p] != inds)) for p in cells}
-    print(f"1M/127^3: fp64 row {row:.3e} kick {kick:.3e}; mixed kick rms {kick_m:.3e}; cell mismatches {mism}")
-    assert row <= 1e-12 and kick <= 1e-9 and kick_m <= 1e-5
+    g, o = accuracy_vs_extended(out["fp64"], r, q, E, dz, nmesh, ro)
+    print(f"1M/127^3: fp64 vs oracle row {row:.3e} kick {kick:.3e}; vs extended-kernel oracle: hip row {g[0]:.3e} "
+          f"kick {g[1]:.3e}, reference row {o[0]:.3e} kick {o[1]:.3e}; mixed kick rms {kick_m:.3e}; "
+          f"cell mismatches {mism}")
+    assert within_reference_accuracy(g, o) and kick_m <= 1e-5
     assert mism == {"fp64": 0, "mixed": 0}
 
 
