@@ -73,6 +73,24 @@ int ocsc_apply_cells(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double
                      const int32_t nmesh[3], double E_GeV, double zstep, int random_mesh, double mesh_scale,
                      double mesh_shift, int precision, int sync, int32_t* cells, void* stream);
 
+/* ocsc_apply that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * workspace of the kick lives in one static device array of
+ * OCSC_NOALLOC_SCRATCH_BYTES (40 MiB) per GPU, held until the process ends;
+ * meshes whose workspace does not fit (above 64^3 in either precision, or
+ * with option "deposit_xcd" = 1 on large meshes) return OCSC_EINVAL before
+ * any work.  The caller passes rparticles and q_array as for ocsc_apply; the
+ * result is the same bit for bit.  All contexts on one GPU share its array:
+ * calls that overlap in time on one GPU (other streams, other contexts,
+ * other host threads) race and must be serialised by the caller.  The
+ * kernel-spectrum cache lives in the array too.  With sync == 0, input errors
+ * are reported by the next call with sync != 0 on that GPU. */
+#define OCSC_NOALLOC_SCRATCH_BYTES (40u << 20)
+int ocsc_apply_noalloc(ocsc_ctx* ctx, double* rparticles, int64_t ld, const double* q_array, int64_t n,
+                       const int32_t nmesh[3], double E_GeV, double zstep, int random_mesh, double mesh_scale,
+                       double mesh_shift, int precision, int sync, void* stream);
+
 /* Error flags left by kicks issued with sync == 0 (synchronises stream);
  * returns OCSC_EVALUE if any kick saw an invalid bunch and clears the flag. */
 int ocsc_check(ocsc_ctx* ctx, void* stream);

@@ -36,6 +36,8 @@ SIGNATURES = {
     "ocsc_apply": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _c_int, _dbl, _dbl, _c_int, _c_int, _vp]),
     "ocsc_apply_cells": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _c_int, _dbl, _dbl, _c_int, _c_int,
                                   _vp, _vp]),
+    "ocsc_apply_noalloc": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _dbl, _c_int, _dbl, _dbl, _c_int, _c_int,
+                                    _vp]),
     "ocsc_check": (_c_int, [_vp, _vp]),
     "ocsc_el_field": (_c_int, [_vp, _vp, _i64, _vp, _i64, _i32p, _dbl, _c_int, _dbl, _dbl, _c_int, _vp, _i64, _vp,
                                _vp]),

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -256,27 +257,44 @@ int make_geometry(const int32_t* nmesh, Geometry& g) {
   return OCSC_OK;
 }
 
+// bytes of each mesh workspace of one kick (E64: FP64 form only)
+struct MeshSizes {
+  size_t grid, A, B, KB, KA, phi, G, E64;
+};
+
 template <typename F>
-int ensure_mesh(ocsc_ctx* c, Lane& L, const Geometry& g, int nword) {
+MeshSizes mesh_sizes(const Geometry& g, int nword, int deposit_xcd) {
   const MeshDims& d = g.d;
-  int rc;
-  if ((rc = ensure(L.grid, (size_t)(c->deposit_xcd ? kNumXcd : 1) * nword * g.ncell * 8))) return rc;
-  size_t a = (size_t)d.nx * d.ny * d.hz * sizeof(cpx<F>);
-  size_t b = (size_t)d.nx * d.my * d.hz * sizeof(cpx<F>);
+  MeshSizes m;
+  m.grid = (size_t)(deposit_xcd ? kNumXcd : 1) * nword * g.ncell * 8;
+  m.A = (size_t)d.nx * d.ny * d.hz * sizeof(cpx<F>);
+  m.B = (size_t)d.nx * d.my * d.hz * sizeof(cpx<F>);
   if constexpr (sizeof(F) == 4) {
     // mixed form carries three fields through the inverse passes
-    a = std::max(a, 3 * (size_t)d.nx * d.ny * d.hz * sizeof(cpx<float>));
-    b = std::max(b, 2 * (size_t)d.nx * d.my * d.hz * sizeof(cpx<float>));
+    m.A = std::max(m.A, 3 * (size_t)d.nx * d.ny * d.hz * sizeof(cpx<float>));
+    m.B = std::max(m.B, 2 * (size_t)d.nx * d.my * d.hz * sizeof(cpx<float>));
   }
-  if ((rc = ensure(L.A, a))) return rc;
-  if ((rc = ensure(L.B, b))) return rc;
   // kernel-spectrum scratch: the IGF grid / its y pass, and its z pass
-  if ((rc = ensure(L.KB, std::max((size_t)g.ncell, (size_t)d.nx * d.hy * d.hz) * 8))) return rc;
-  if ((rc = ensure(L.KA, (size_t)d.nx * d.ny * d.hz * 8))) return rc;
-  if ((rc = ensure(L.phi, (size_t)g.ncell * std::max(sizeof(double), 3 * sizeof(float))))) return rc;
-  if ((rc = ensure(L.G, (size_t)d.hx * d.hy * d.hz * sizeof(double)))) return rc;
-  if constexpr (sizeof(F) == 8)
-    if ((rc = ensure(L.E64, (size_t)3 * g.ncell * sizeof(double)))) return rc;
+  m.KB = std::max((size_t)g.ncell, (size_t)d.nx * d.hy * d.hz) * 8;
+  m.KA = (size_t)d.nx * d.ny * d.hz * 8;
+  m.phi = (size_t)g.ncell * std::max(sizeof(double), 3 * sizeof(float));
+  m.G = (size_t)d.hx * d.hy * d.hz * sizeof(double);
+  m.E64 = sizeof(F) == 8 ? (size_t)3 * g.ncell * sizeof(double) : 0;
+  return m;
+}
+
+template <typename F>
+int ensure_mesh(ocsc_ctx* c, Lane& L, const Geometry& g, int nword) {
+  const MeshSizes m = mesh_sizes<F>(g, nword, c->deposit_xcd);
+  int rc;
+  if ((rc = ensure(L.grid, m.grid))) return rc;
+  if ((rc = ensure(L.A, m.A))) return rc;
+  if ((rc = ensure(L.B, m.B))) return rc;
+  if ((rc = ensure(L.KB, m.KB))) return rc;
+  if ((rc = ensure(L.KA, m.KA))) return rc;
+  if ((rc = ensure(L.phi, m.phi))) return rc;
+  if ((rc = ensure(L.G, m.G))) return rc;
+  if (m.E64 && (rc = ensure(L.E64, m.E64))) return rc;
   return OCSC_OK;
 }
 
@@ -680,6 +698,9 @@ namespace {
 
 constexpr int kMaxLanes = 8;
 
+// device address of the no-allocation workspace on `device` (current)
+int scratch_for_device(int device, void** out);
+
 int init_lane(Lane& L, bool own_stream) {
   if (hipMalloc(&L.sc, sizeof(KickScalars)) != hipSuccess) return fail(OCSC_ENOMEM, "scalars");
   if (hipHostMalloc(&L.sc_host, sizeof(KickScalars)) != hipSuccess) return fail(OCSC_ENOMEM, "pinned");
@@ -719,6 +740,12 @@ int ocsc_create(ocsc_ctx** out, int device) {
     ocsc_destroy(c);
     return rc;
   };
+  {
+    // resolve this device's no-allocation workspace once, at set-up
+    void* sp = nullptr;
+    const int rc = scratch_for_device(device, &sp);
+    if (rc) return cleanup(rc);
+  }
   if (hipMalloc(&L.sc, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "scalars"));
   if (hipHostMalloc(&L.sc_host, sizeof(KickScalars)) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "pinned"));
   if (hipMalloc(&c->twf, sizeof(cpx<float>) * 2048) != hipSuccess) return cleanup(fail(OCSC_ENOMEM, "twiddles"));
@@ -832,6 +859,106 @@ int ocsc_apply_cells(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   if (sync) {
     int32_t err = 0;
     if ((rc = read_errors(c, L, st, &err))) return rc;
+    if (err) return value_error(err);
+  }
+  return OCSC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Kicks without device allocation (ocsc_apply_noalloc): every workspace of
+// the kick (scalars, partials, charge grid, FFT scratch, kernel spectrum,
+// fields) lives in one statically sized device array.  In host code the
+// array's name is only a stand-in; its device address comes from
+// hipGetSymbolAddress, taken once per device with that device current (in
+// ocsc_create) and kept in a per-device table.
+namespace {
+
+constexpr size_t kScratchBytes = OCSC_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device `device`
+int scratch_for_device(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(OCSC_EINVAL, "device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(OCSC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return OCSC_OK;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// lay the workspaces of one kick out in the device's scratch array
+int scratch_lane(ocsc_ctx* c, const Geometry& g, int prec, Lane* S) {
+  void* base = nullptr;
+  int rc;
+  if ((rc = scratch_for_device(c->device, &base))) return rc;
+  const MeshSizes m = prec == OCSC_FP64 ? mesh_sizes<double>(g, 2, c->deposit_xcd)
+                                        : mesh_sizes<float>(g, 1, c->deposit_xcd);
+  size_t off = 0;
+  auto take = [&](Buf& b, size_t bytes) {
+    b.p = bytes ? (void*)((char*)base + off) : nullptr;
+    b.cap = bytes;
+    off += align256(bytes);
+  };
+  S->sc = (KickScalars*)base;
+  off = align256(sizeof(KickScalars));
+  // partials of the frame and extent passes (the one-shot kick writes none)
+  take(S->part, (size_t)kMaxRedBlocks * kNPartB * sizeof(double));
+  take(S->grid, m.grid);
+  take(S->A, m.A);
+  take(S->B, m.B);
+  take(S->KB, m.KB);
+  take(S->KA, m.KA);
+  take(S->phi, m.phi);
+  take(S->G, m.G);
+  take(S->E64, m.E64);
+  if (off > kScratchBytes)
+    return fail(OCSC_EINVAL, "mesh " + std::to_string(g.d.nx) + "x" + std::to_string(g.d.ny) + "x" +
+                                 std::to_string(g.d.nz) + " needs " + std::to_string(off) +
+                                 " bytes of workspace; ocsc_apply_noalloc holds at most " +
+                                 std::to_string(kScratchBytes) +
+                                 " (every mesh up to 64^3 in either precision, with option deposit_xcd = 0)");
+  S->sc_host = c->lane0.sc_host;  // pinned host mirror for the error read-back
+  return OCSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocsc_apply_noalloc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
+                       double E_GeV, double zstep, int random_mesh, double mesh_scale, double mesh_shift, int prec,
+                       int sync, void* stream) {
+  if (c && zstep == 0) return OCSC_OK;  // sc.py:210-212
+  int rc;
+  if ((rc = check_common(c, r, q, n, ld))) return rc;
+  if (!nmesh) return fail(OCSC_EINVAL, "null nmesh");
+  if (prec != OCSC_FP64 && prec != OCSC_MIXED) return fail(OCSC_EINVAL, "precision must be OCSC_FP64 or OCSC_MIXED");
+  if (!(E_GeV > 0) || !std::isfinite(zstep)) return fail(OCSC_EINVAL, "E_GeV must be > 0 and zstep finite");
+  HIPCHK(hipSetDevice(c->device));
+  Geometry g;
+  if ((rc = make_geometry(nmesh, g))) return rc;
+  Lane S;
+  if ((rc = scratch_lane(c, g, prec, &S))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, random_mesh, mesh_scale, mesh_shift);
+  if ((rc = enqueue_kick_prec(c, S, g, r, ld, q, in, prec, st))) return rc;
+  if (sync) {
+    int32_t err = 0;
+    if ((rc = read_errors(c, S, st, &err))) return rc;
     if (err) return value_error(err);
   }
   return OCSC_OK;
