@@ -314,9 +314,20 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * concurrently with the charge deposit (the FP64 Green's function is
  * arithmetic-bound, the deposit memory-bound), 0 runs it in line.  Both give
  * bit-identical results.  "kick_timing": 1 times the kick kernel of every
- * tracking step (see ocsc_kick_times), 0 (default) does not. */
+ * tracking step (see ocsc_kick_times), 0 (default) does not.
+ * "permute_rec": 1 (default) stages the permutations between the caller's
+ * order and the cell order of ocsc_track_drift through 64-byte particle
+ * records (one whole memory segment per particle on the random side), 0
+ * permutes row by row.  Both give bit-identical results. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
+
+/* The cell sort of the tracking path, for diagnostics and tests (no
+ * reference counterpart): order[j] = position of the j-th smallest of the n
+ * device keys, comparing their low end_bit bits, ties in position order
+ * (numpy.argsort(keys, kind="stable")).  Hand-written stable LSD radix sort,
+ * 8 bits per pass.  keys and order are device arrays; synchronous. */
+int ocsc_cell_sort(ocsc_ctx* ctx, const uint32_t* keys, int64_t n, int end_bit, uint32_t* order, void* stream);
 
 /* Stage timing for performance work: with profiling on, every kick issued
  * outside a graph capture records HIP events between its stages and waits

@@ -52,6 +52,7 @@ SIGNATURES = {
     "ocsc_moments": (_c_int, [_vp, _vp, _i64, _vp, _i64, _c_int, _dp, _dbl, _dbl, _dp, _c_int, _vp]),
     "ocsc_set_option": (_c_int, [_vp, ctypes.c_char_p, _i64]),
     "ocsc_get_option": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_i64)]),
+    "ocsc_cell_sort": (_c_int, [_vp, _vp, _i64, _c_int, _vp, _vp]),
     "ocsc_set_profiling": (_c_int, [_vp, _c_int]),
     "ocsc_stage_times": (_c_int, [_vp, _dp, _c_int, ctypes.POINTER(_i64)]),
     "ocsc_stage_name": (ctypes.c_char_p, [_c_int]),

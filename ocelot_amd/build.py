@@ -2,8 +2,7 @@
 
     python -m ocelot_amd.build          # or __graft_entry__.build()
 
-hipcc compiles the translation units (ocsc.hip: kernels and C ABI; sort.hip:
-the rocPRIM radix sort) with
+hipcc compiles the translation unit (ocsc.hip: kernels and C ABI) with
 floating-point contraction off (the particle arithmetic reproduces numpy's
 rounding order; fused multiply-adds are written explicitly where wanted) and
 packed-FP32 instructions disabled.
@@ -15,7 +14,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-SRCS = [os.path.join(HERE, "csrc", "ocsc.hip"), os.path.join(HERE, "csrc", "sort.hip")]
+SRCS = [os.path.join(HERE, "csrc", "ocsc.hip")]
 OBJDIR = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libocsc.so")
 ARCH = os.environ.get("OCSC_OFFLOAD_ARCH", "gfx950")

@@ -22,14 +22,31 @@ namespace ocsc {
 constexpr int kNT = 256;  // threads per workgroup for the mesh kernels
 
 // columns per workgroup: LDS footprint M*(T+1)*sizeof(cpx<F>) stays <= ~72 KB
+// (plus the twiddle table W_M^k, k < M, staged behind the tile when the
+// workgroup's LDS stays within 64 KB: every Stockham pass reads it)
 template <typename F, int M>
 struct Tile {
   static constexpr int cap = sizeof(F) == 4 ? 64 : 32;
   static constexpr int want = (sizeof(F) == 4 ? 4096 : 2048) / M;
   static constexpr int T = want < 2 ? 2 : (want > cap ? cap : want);
   static constexpr int S = T + 1;
-  static constexpr size_t lds_bytes = (size_t)M * S * sizeof(cpx<F>);
+  static constexpr size_t data_bytes = (size_t)M * S * sizeof(cpx<F>);
+  static constexpr bool stw = data_bytes + M * sizeof(cpx<F>) <= 65536;
+  static constexpr size_t lds_bytes = data_bytes + (stw ? M * sizeof(cpx<F>) : 0);
 };
+
+// Copy the twiddle table into LDS at dst (ON) and return the table the
+// transforms should read; the kernel's barrier before its first transform
+// publishes the copy.
+template <typename F, int M, int NT, bool ON>
+__device__ __forceinline__ const cpx<F>* stage_tw(cpx<F>* dst, const cpx<F>* __restrict__ tw) {
+  if constexpr (ON) {
+    for (int i = threadIdx.x; i < M; i += NT) dst[i] = tw[i];
+    return dst;
+  } else {
+    return tw;
+  }
+}
 
 struct MeshDims {
   int nx, ny, nz;
@@ -82,6 +99,7 @@ __global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
+  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   const int64_t ncell = R * nz;
@@ -101,7 +119,7 @@ __global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<
     p[rl & 1] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, -1>(s, tw);
+  lds_fft<F, M, T, kNT, -1>(s, twl);
   for (int i = threadIdx.x; i < T * Hz; i += kNT) {
     const int t = i / Hz, k = i - t * Hz;
     const int64_t ra = row0 + 2 * t, rb = ra + 1;
@@ -123,6 +141,7 @@ __global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
+  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
   const int kz0 = blockIdx.x * T;
   const int x = blockIdx.y;
@@ -134,7 +153,7 @@ __global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<
     s[y * S + t] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, -1>(s, tw);
+  lds_fft<F, M, T, kNT, -1>(s, twl);
   for (int i = threadIdx.x; i < M * T; i += kNT) {
     const int t = i % T, ky = i / T;
     const int kz = kz0 + t;
@@ -152,6 +171,7 @@ __global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
+  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
   const int kz0 = blockIdx.x * T;
   const int ky = blockIdx.y;
@@ -164,7 +184,7 @@ __global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* 
     s[x * S + t] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, -1>(s, tw);
+  lds_fft<F, M, T, kNT, -1>(s, twl);
   const int fy = ky <= My - ky ? ky : My - ky;
   for (int i = threadIdx.x; i < M * T; i += kNT) {
     const int t = i % T, kx = i / T;
@@ -174,7 +194,7 @@ __global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* 
     s[kx * S + t] = cscale(s[kx * S + t], g);
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, +1>(s, tw);
+  lds_fft<F, M, T, kNT, +1>(s, twl);
   for (int i = threadIdx.x; i < nx * T; i += kNT) {
     const int t = i % T, x = i / T;
     const int kz = kz0 + t;
@@ -191,6 +211,7 @@ __global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
+  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
   const int kz0 = blockIdx.x * T;
   const int x = blockIdx.y;
@@ -202,7 +223,7 @@ __global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<
     s[ky * S + t] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, +1>(s, tw);
+  lds_fft<F, M, T, kNT, +1>(s, twl);
   for (int i = threadIdx.x; i < ny * T; i += kNT) {
     const int t = i % T, y = i / T;
     const int kz = kz0 + t;
@@ -217,6 +238,7 @@ __device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __res
                                           const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
                                           const cpx<double>* __restrict__ twd, cpx<F>* s) {
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
@@ -235,7 +257,7 @@ __device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __res
     if (k > 0 && k < M / 2) s[(M - k) * S + t] = cpx<F>{a.x + b.y, b.x - a.y};
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, +1>(s, tw);
+  lds_fft<F, M, T, kNT, +1>(s, twl);
   for (int i = threadIdx.x; i < 2 * T * nz; i += kNT) {
     const int rl = i / nz, z = i - rl * nz;
     const int64_t row = row0 + rl;
@@ -283,7 +305,8 @@ struct FieldTile {
   static constexpr int T = Tile<float, M>::T;  // columns per workgroup
   static constexpr int S = T + 1;
   static constexpr int PER = (M * T + NT - 1) / NT;  // spectrum values held per thread
-  static constexpr size_t lds = (size_t)M * S * sizeof(cpx<float>);
+  static constexpr bool stw = Tile<float, M>::stw;
+  static constexpr size_t lds = Tile<float, M>::lds_bytes;
 };
 
 // Mixed form, x pass: forward FFT along x, times the kernel spectrum, then
@@ -302,6 +325,7 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
+  const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
   const int kz0 = blockIdx.x * T;
   const int ky = blockIdx.y;
@@ -314,7 +338,7 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
     s[x * S + t] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, -1, S>(s, tw);
+  lds_fft<F, M, T, kNTF, -1, S>(s, twl);
   const int fy = ky <= My - ky ? ky : My - ky;
   const double inv_hx = 1.0 / sc->steps[0];
   cpx<F> v[PER];
@@ -340,7 +364,7 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
     }
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, tw);
+  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
   for (int i = threadIdx.x; i < nx * T; i += kNTF) {
     const int t = i % T, x = i / T;
     const int kz = kz0 + t;
@@ -353,7 +377,7 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
     if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + i % T] = v[k];
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, tw);
+  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
   for (int i = threadIdx.x; i < nx * T; i += kNTF) {
     const int t = i % T, x = i / T;
     const int kz = kz0 + t;
@@ -376,6 +400,7 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
+  const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
   const int kz0 = blockIdx.x * T;
   const int x = blockIdx.y;
@@ -398,7 +423,7 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
     }
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, tw);
+  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
   auto store = [&](cpx<F>* dst) {
     for (int i = threadIdx.x; i < ny * T; i += kNTF) {
       const int t = i % T, y = i / T;
@@ -414,7 +439,7 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
     if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + i % T] = cmul(p[k], dfac<F>(twd, i / T, inv_hy));
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, tw);
+  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
   store(Ay);
   __syncthreads();
 #pragma unroll
@@ -423,48 +448,50 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
     if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + i % T] = p[k];
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, tw);
+  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
   store(Ap);
 }
 
 // ---------------------------------------------------------------------------
 // Integrated Green's function (ocelot/cpbd/sc.py:109-133), FP64.
 // K[i,j,k] = 8-corner alternating difference of F at the cell faces
-// x_i = i*hx - hx/2 (i = 0..N).  A workgroup computes a 4x8x8 tile of cells
-// from a 5x9x9 tile of F values held in LDS.
+// x_i = i*hx - hx/2 (i = 0..N).
 __device__ __forceinline__ double igf_F(double x, double y, double z) {
   const double r = __builtin_sqrt(x * x + y * y + z * z);
   return -x * x * 0.5 * atan(y * z / (x * r)) + y * z * log(x + r) - y * y * 0.5 * atan(z * x / (y * r)) +
          z * x * log(y + r) - z * z * 0.5 * atan(x * y / (z * r)) + x * y * log(z + r);
 }
 
-constexpr int kIgfTx = 4, kIgfTy = 8, kIgfTz = 8;
-
-__global__ void __launch_bounds__(256) k_igf(double* __restrict__ K, MeshDims d, const KickScalars* __restrict__ sc,
-                                             int force) {
-  __shared__ double F[(kIgfTx + 1) * (kIgfTy + 1) * (kIgfTz + 1)];
+// Two passes: F once at every node (i, j, k) <= (nx, ny, nz) of the cell
+// faces, then the 8-corner difference per cell from those values (the
+// neighbours come from L2).  A tiled single pass (4x8x8 cells from a 5x9x9
+// F tile in LDS) evaluates 58% of F twice and leaves most threads idle while
+// the tile is filled.
+__global__ void __launch_bounds__(256) k_igf_nodes(double* __restrict__ Fn, MeshDims d,
+                                                   const KickScalars* __restrict__ sc, int force) {
   if (sc->err || !(force || sc->rebuild)) return;
   const double hx = sc->steps[0], hy = sc->steps[1], hz = sc->steps[2];
-  const int bz = (d.nz + kIgfTz - 1) / kIgfTz, by = (d.ny + kIgfTy - 1) / kIgfTy;
-  const int b = blockIdx.x;
-  const int i0 = (b / (bz * by)) * kIgfTx, j0 = ((b / bz) % by) * kIgfTy, k0 = (b % bz) * kIgfTz;
-  constexpr int FX = kIgfTx + 1, FY = kIgfTy + 1, FZ = kIgfTz + 1;
-  for (int f = threadIdx.x; f < FX * FY * FZ; f += 256) {
-    const int a = f / (FY * FZ), bb = (f / FZ) % FY, c = f % FZ;
-    const int i = i0 + a, j = j0 + bb, k = k0 + c;
-    double v = 0;
-    if (i <= d.nx && j <= d.ny && k <= d.nz) v = igf_F(hx * i - hx / 2, hy * j - hy / 2, hz * k - hz / 2);
-    F[f] = v;
+  const int sz = d.nz + 1, sy = d.ny + 1;
+  const int64_t nn = (int64_t)(d.nx + 1) * sy * sz;
+  for (int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x; f < nn; f += (int64_t)gridDim.x * 256) {
+    const int k = (int)(f % sz), j = (int)((f / sz) % sy), i = (int)(f / ((int64_t)sz * sy));
+    Fn[f] = igf_F(hx * i - hx / 2, hy * j - hy / 2, hz * k - hz / 2);
   }
-  __syncthreads();
-  const int a = threadIdx.x / (kIgfTy * kIgfTz), bb = (threadIdx.x / kIgfTz) % kIgfTy, c = threadIdx.x % kIgfTz;
-  const int i = i0 + a, j = j0 + bb, k = k0 + c;
-  if (i >= d.nx || j >= d.ny || k >= d.nz) return;
-#define FAT(p, q, r) F[((a + (p)) * FY + (bb + (q))) * FZ + (c + (r))]
-  const double kv = FAT(1, 1, 1) - FAT(0, 1, 1) - FAT(1, 0, 1) + FAT(0, 0, 1) - FAT(1, 1, 0) + FAT(0, 1, 0) +
-                    FAT(1, 0, 0) - FAT(0, 0, 0);
+}
+
+__global__ void __launch_bounds__(256) k_igf_diff(const double* __restrict__ Fn, double* __restrict__ K, MeshDims d,
+                                                  const KickScalars* __restrict__ sc, int force) {
+  if (sc->err || !(force || sc->rebuild)) return;
+  const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
+  const int64_t sy = d.nz + 1, sx = (int64_t)(d.ny + 1) * (d.nz + 1);
+  for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < nc; o += (int64_t)gridDim.x * 256) {
+    const int k = (int)(o % d.nz), j = (int)((o / d.nz) % d.ny), i = (int)(o / ((int64_t)d.nz * d.ny));
+    const double* f0 = Fn + i * sx + j * sy + k;
+#define FAT(p, q, r) f0[(p) * sx + (q) * sy + (r)]
+    K[o] = FAT(1, 1, 1) - FAT(0, 1, 1) - FAT(1, 0, 1) + FAT(0, 0, 1) - FAT(1, 1, 0) + FAT(0, 1, 0) + FAT(1, 0, 0) -
+           FAT(0, 0, 0);
 #undef FAT
-  K[((int64_t)i * d.ny + j) * d.nz + k] = kv;
+  }
 }
 
 // Even transform along z of the real kernel, two rows per complex FFT:
@@ -479,6 +506,7 @@ __global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, doub
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err || !(force || sc->rebuild)) return;
+  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
@@ -492,7 +520,7 @@ __global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, doub
     if (m > 0) ((F*)&s[(M - m) * S + (rl >> 1)])[rl & 1] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, -1>(s, tw);
+  lds_fft<F, M, T, kNT, -1>(s, twl);
   for (int i = threadIdx.x; i < 2 * T * Hz; i += kNT) {
     const int rl = i / Hz, k = i - rl * Hz;
     const int64_t row = row0 + rl;
@@ -515,6 +543,7 @@ __global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO*
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err || !(force || sc->rebuild)) return;
+  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int c0 = blockIdx.x * 2 * T;  // first real inner column of this tile
   const int o = blockIdx.y;            // outer index
   for (int i = threadIdx.x; i < M * T; i += kNT) s[(i / T) * S + (i % T)] = cpx<F>{0, 0};
@@ -527,7 +556,7 @@ __global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO*
     if (m > 0) ((F*)&s[(M - m) * S + (cl >> 1)])[cl & 1] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, -1>(s, tw);
+  lds_fft<F, M, T, kNT, -1>(s, twl);
   double g = 1.0;
   if (last) {
     // potential scale 1/(4 pi eps0 hx hy hz) (sc.py:167) and the 1/(Mx My Mz)

@@ -16,12 +16,7 @@
 #include "moments.hpp"
 #include "lsc.hpp"
 #include "csr.hpp"
-
-namespace ocsc {
-// sort.hip
-hipError_t sort_pairs_u32(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                          const uint32_t* vals_in, uint32_t* vals_out, int64_t n, int end_bit, hipStream_t st);
-}  // namespace ocsc
+#include "sort.hpp"
 
 using namespace ocsc;
 
@@ -164,7 +159,7 @@ struct Lane {
   KickScalars* sc_host = nullptr;  // pinned mirror
   Buf part;                        // per-block partial sums
   Buf grid, A, B, phi, G;
-  Buf KA, KB;                      // kernel-spectrum build scratch (its own, so it can overlap the FFTs)
+  Buf KA, KB, KF;                  // kernel-spectrum build scratch (its own, so it can overlap the FFTs)
   Buf E64;                         // FP64 form: the three difference fields of phi
   hipStream_t stream = nullptr;    // extra lanes only
   hipEvent_t done = nullptr;       // extra lanes only
@@ -193,8 +188,10 @@ struct ocsc_ctx {
   int deposit_xcd = 0;  // 1: per-XCD grid copies with L2-local atomics
   int red_blocks = kMaxRedBlocks;  // blocks of the strided reduction passes
   int sort_every = 10;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
-  // tracking workspace: two cell-ordered copies of the bunch, sort buffers
-  Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp;
+  int permute_rec = 1;  // tracking: permutations staged through 64-byte particle records (0: row by row)
+  // tracking workspace: two cell-ordered copies of the bunch, sort buffers,
+  // particle records for the permutations
+  Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp, rec;
   Buf draws;  // random-mesh draws of a tracking call and their counter
   // beam moments: device state and its pinned host mirror
   MomState* mom = nullptr;
@@ -259,7 +256,7 @@ int make_geometry(const int32_t* nmesh, Geometry& g) {
 
 // bytes of each mesh workspace of one kick (E64: FP64 form only)
 struct MeshSizes {
-  size_t grid, A, B, KB, KA, phi, G, E64;
+  size_t grid, A, B, KB, KA, KF, phi, G, E64;
 };
 
 template <typename F>
@@ -277,6 +274,7 @@ MeshSizes mesh_sizes(const Geometry& g, int nword, int deposit_xcd) {
   // kernel-spectrum scratch: the IGF grid / its y pass, and its z pass
   m.KB = std::max((size_t)g.ncell, (size_t)d.nx * d.hy * d.hz) * 8;
   m.KA = (size_t)d.nx * d.ny * d.hz * 8;
+  m.KF = (size_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1) * 8;  // the Green's-function primitive at the nodes
   m.phi = (size_t)g.ncell * std::max(sizeof(double), 3 * sizeof(float));
   m.G = (size_t)d.hx * d.hy * d.hz * sizeof(double);
   m.E64 = sizeof(F) == 8 ? (size_t)3 * g.ncell * sizeof(double) : 0;
@@ -292,6 +290,7 @@ int ensure_mesh(ocsc_ctx* c, Lane& L, const Geometry& g, int nword) {
   if ((rc = ensure(L.B, m.B))) return rc;
   if ((rc = ensure(L.KB, m.KB))) return rc;
   if ((rc = ensure(L.KA, m.KA))) return rc;
+  if ((rc = ensure(L.KF, m.KF))) return rc;
   if ((rc = ensure(L.phi, m.phi))) return rc;
   if ((rc = ensure(L.G, m.G))) return rc;
   if (m.E64 && (rc = ensure(L.E64, m.E64))) return rc;
@@ -333,8 +332,13 @@ int launch_kernel_spectrum(ocsc_ctx* c, Lane& L, const Geometry& g, hipStream_t 
   double* K = (double*)L.KB.p;
   double* Kz = (double*)L.KA.p;
   double* Kzy = (double*)L.KB.p;
-  const int nb_igf = ((d.nx + kIgfTx - 1) / kIgfTx) * ((d.ny + kIgfTy - 1) / kIgfTy) * ((d.nz + kIgfTz - 1) / kIgfTz);
-  k_igf<<<nb_igf, 256, 0, st>>>(K, d, L.sc, force);
+  {
+    // F once per node, then the 8-corner differences
+    const int64_t nn = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+    k_igf_nodes<<<(int)std::min<int64_t>((nn + 255) / 256, 65536), 256, 0, st>>>((double*)L.KF.p, d, L.sc, force);
+    k_igf_diff<<<(int)std::min<int64_t>((g.ncell + 255) / 256, 65536), 256, 0, st>>>((const double*)L.KF.p, K, d,
+                                                                                     L.sc, force);
+  }
   LAUNCHCHK("k_igf");
   const int64_t R = (int64_t)d.nx * d.ny;
   OCSC_SWITCH_M(d.mz, {
@@ -715,7 +719,7 @@ int init_lane(Lane& L, bool own_stream) {
 }
 
 void free_lane(Lane& L) {
-  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.KA, &L.KB, &L.phi, &L.G, &L.E64})
+  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.KA, &L.KB, &L.KF, &L.phi, &L.G, &L.E64})
     if (b->p) (void)hipFree(b->p);
   if (L.sc) (void)hipFree(L.sc);
   if (L.sc_host) (void)hipHostFree(L.sc_host);
@@ -794,10 +798,12 @@ int ocsc_destroy(ocsc_ctx* c) {
   Lane& L = c->lane0;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
-  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.KA, &L.KB, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0], &c->qs[1],
+  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.KA, &L.KB, &L.KF, &L.phi, &L.G, &c->rs[0], &c->rs[1], &c->qs[0],
+                 &c->qs[1],
                  &c->keys[0],
                  &L.E64, &c->draws, &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
-                 &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->lsc_fac, &c->csr_bins})
+                 &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->lsc_fac, &c->csr_bins,
+                 &c->rec})
     if (b->p) (void)hipFree(b->p);
   if (c->lsc) (void)hipFree(c->lsc);
   if (c->lsc_plan) (void)hipFree(c->lsc_plan);
@@ -922,6 +928,7 @@ int scratch_lane(ocsc_ctx* c, const Geometry& g, int prec, Lane* S) {
   take(S->B, m.B);
   take(S->KB, m.KB);
   take(S->KA, m.KA);
+  take(S->KF, m.KF);
   take(S->phi, m.phi);
   take(S->G, m.G);
   take(S->E64, m.E64);
@@ -1044,8 +1051,12 @@ int ocsc_sym_kernel(ocsc_ctx* c, const int32_t nmesh[3], const double h[3], doub
   d.nx = nmesh[0];
   d.ny = nmesh[1];
   d.nz = nmesh[2];
-  const int nb = ((d.nx + kIgfTx - 1) / kIgfTx) * ((d.ny + kIgfTy - 1) / kIgfTy) * ((d.nz + kIgfTz - 1) / kIgfTz);
-  k_igf<<<nb, 256, 0, st>>>(K, d, L.sc, 1);
+  int rc;
+  const int64_t nn = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+  if ((rc = ensure(L.KF, (size_t)nn * 8))) return rc;
+  const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
+  k_igf_nodes<<<(int)std::min<int64_t>((nn + 255) / 256, 65536), 256, 0, st>>>((double*)L.KF.p, d, L.sc, 1);
+  k_igf_diff<<<(int)std::min<int64_t>((nc + 255) / 256, 65536), 256, 0, st>>>((const double*)L.KF.p, K, d, L.sc, 1);
   LAUNCHCHK("k_igf");
   HIPCHK(hipStreamSynchronize(st));
   return OCSC_OK;
@@ -1451,27 +1462,36 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
       if ((rc = ensure(c->perm[k], (size_t)n * 4))) return rc;
     }
     if ((rc = ensure(c->vals, (size_t)2 * n * 4))) return rc;
-    size_t tb = 0;
-    HIPCHK(sort_pairs_u32(nullptr, &tb, nullptr, nullptr, nullptr, nullptr, n, end_bit, st));
-    if ((rc = ensure(c->sort_tmp, tb + 256))) return rc;
+    if ((rc = ensure(c->sort_tmp, rs_hist_bytes(n)))) return rc;
+    if (c->permute_rec && (rc = ensure(c->rec, (size_t)n * sizeof(PRec)))) return rc;
   }
+  const bool via_rec = sorted && c->permute_rec;
   // sort src (row stride lds) by NGP cell under the current kick scalars into
-  // the copy `to`; perm[to] = order relative to the caller's particles
-  auto resort = [&](const double* src, int64_t lds, const double* qsrc, int to, const uint32_t* perm_prev) -> int {
-    uint32_t* keys0 = (uint32_t*)c->keys[0].p;
-    uint32_t* keys1 = (uint32_t*)c->keys[1].p;
-    uint32_t* vin = (uint32_t*)c->vals.p;
-    uint32_t* vout = vin + n;  // order of src
-    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, L.sc, keys0, vin);
+  // the copy `to`; perm[to] = order relative to the caller's particles.
+  // drift: apply the first step's drift to the sorted copy (initial sort).
+  auto resort = [&](const double* src, int64_t lds, const double* qsrc, int to, const uint32_t* perm_prev,
+                    bool drift) -> int {
+    uint32_t* kb[2] = {(uint32_t*)c->keys[0].p, (uint32_t*)c->keys[1].p};
+    uint32_t* vb[2] = {(uint32_t*)c->vals.p, (uint32_t*)c->vals.p + n};
+    PRec* rec = via_rec ? (PRec*)c->rec.p : nullptr;
+    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, qsrc, L.sc, kb[0], rec);
     LAUNCHCHK("k_sort_keys");
-    size_t tb = c->sort_tmp.cap;
-    HIPCHK(sort_pairs_u32(c->sort_tmp.p, &tb, keys0, keys1, vin, vout, n, end_bit, st));
+    const uint32_t* order = vb[rs_sort(kb, vb, n, end_bit, (uint32_t*)c->sort_tmp.p, st)];  // order of src
+    LAUNCHCHK("cell sort");
     if (perm_prev)
-      k_compose<<<nb, 256, 0, st>>>(perm_prev, vout, (uint32_t*)c->perm[to].p, n);
+      k_compose<<<nb, 256, 0, st>>>(perm_prev, order, (uint32_t*)c->perm[to].p, n);
     else
-      HIPCHK(hipMemcpyAsync(c->perm[to].p, vout, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
-    k_permute_gather<<<dim3(nb, 7), 256, 0, st>>>(src, lds, qsrc, vout, (double*)c->rs[to].p, n, (double*)c->qs[to].p, n);
-    LAUNCHCHK("k_permute_gather");
+      HIPCHK(hipMemcpyAsync(c->perm[to].p, order, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    double* dst = (double*)c->rs[to].p;
+    double* qdst = (double*)c->qs[to].p;
+    if (via_rec && drift)
+      k_gather_rec<true><<<nb, 256, 0, st>>>(rec, order, dst, n, qdst, n, DriftArgs{Ld, r56});
+    else if (via_rec)
+      k_gather_rec<false><<<nb, 256, 0, st>>>(rec, order, dst, n, qdst, n, DriftArgs{0, 0});
+    else
+      k_permute_gather<<<dim3(nb, 7), 256, 0, st>>>(src, lds, qsrc, order, dst, n, qdst, n);
+    LAUNCHCHK("cell-order gather");
+    if (drift && !via_rec) k_drift<<<nb, 256, 0, st>>>(dst, n, n, Ld, r56);
     return OCSC_OK;
   };
 
@@ -1488,7 +1508,8 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     k_passB<false><<<nblk, kPT, 0, st>>>(Rows{r, ld}, q, L.sc, (double*)L.part.p);
     k_reduceB<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk, prec);
     LAUNCHCHK("initial frame");
-    if ((rc = resort(r, ld, q, 0, nullptr))) return rc;
+    // the first step's drift rides on the gather (sorted implies zstep != 0)
+    if ((rc = resort(r, ld, q, 0, nullptr, nsteps > 0))) return rc;
     rr = (double*)c->rs[0].p;
     lr = n;
     qq = (const double*)c->qs[0].p;
@@ -1534,8 +1555,8 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
       if (lq && (rc = enqueue_lsc(c, rr, lr, qq, n, lq->k, lq->b0, lq->b1, st))) return rc;
       if (tw && (rc = enqueue_moments(c, Rows{rr, lr}, qq, n, *tw, st))) return rc;
     }
-  } else if (nsteps > 0) {
-    k_drift<<<nb, 256, 0, st>>>(rr, lr, n, Ld, r56);
+  } else if (nsteps > 0 && !sorted) {
+    k_drift<<<nb, 256, 0, st>>>(rr, lr, n, Ld, r56);  // (sorted: applied by the initial gather)
   }
   // one tracking step after the drift: the kick and, with per-step moments,
   // the moments and the next drift
@@ -1565,7 +1586,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   };
   for (int step = 0; step < nsteps && zstep != 0; ++step) {
     if (sorted && step > 0 && step % c->sort_every == 0) {
-      if ((rc = resort(rr, n, qq, 1 - cur, (const uint32_t*)c->perm[cur].p))) {
+      if ((rc = resort(rr, n, qq, 1 - cur, (const uint32_t*)c->perm[cur].p, false))) {
         cleanup();
         return rc;
       }
@@ -1641,7 +1662,12 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
       return fail(OCSC_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
     }
   }
-  if (sorted) {
+  if (via_rec) {
+    // back to the caller's order: whole records to their places, then rows
+    k_scatter_rec<<<nb, 256, 0, st>>>(rr, n, (const uint32_t*)c->perm[cur].p, (PRec*)c->rec.p, n);
+    k_rec_to_rows<<<nb, 256, 0, st>>>((const PRec*)c->rec.p, r, ld, n);
+    LAUNCHCHK("caller-order scatter");
+  } else if (sorted) {
     k_permute_scatter<<<dim3(nb, 6), 256, 0, st>>>(rr, n, (const uint32_t*)c->perm[cur].p, r, ld, n);
     LAUNCHCHK("k_permute_scatter");
   }
@@ -1912,6 +1938,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->overlap = (int)value;
     return OCSC_OK;
   }
+  if (n == "permute_rec") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "permute_rec must be 0 or 1");
+    c->permute_rec = (int)value;
+    return OCSC_OK;
+  }
   return fail(OCSC_EINVAL, "unknown option " + n);
 }
 
@@ -1934,7 +1965,34 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
     *value = c->kick_timing;
     return OCSC_OK;
   }
+  if (n == "permute_rec") {
+    *value = c->permute_rec;
+    return OCSC_OK;
+  }
   return fail(OCSC_EINVAL, "unknown option " + n);
+}
+
+int ocsc_cell_sort(ocsc_ctx* c, const uint32_t* keys, int64_t n, int end_bit, uint32_t* order, void* stream) {
+  if (!c) return fail(OCSC_EINVAL, "null context");
+  if (n < 0 || n > (int64_t)0xFFFFFFFF) return fail(OCSC_EINVAL, "n must be in [0, 2^32)");
+  if (end_bit < 1 || end_bit > 32) return fail(OCSC_EINVAL, "end_bit must be in [1, 32]");
+  if (n == 0) return OCSC_OK;
+  if (!keys || !order) return fail(OCSC_EINVAL, "null keys or order");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  for (int k = 0; k < 2; ++k)
+    if ((rc = ensure(c->keys[k], (size_t)n * 4))) return rc;
+  if ((rc = ensure(c->vals, (size_t)2 * n * 4))) return rc;
+  if ((rc = ensure(c->sort_tmp, rs_hist_bytes(n)))) return rc;
+  uint32_t* kb[2] = {(uint32_t*)c->keys[0].p, (uint32_t*)c->keys[1].p};
+  uint32_t* vb[2] = {(uint32_t*)c->vals.p, (uint32_t*)c->vals.p + n};
+  HIPCHK(hipMemcpyAsync(kb[0], keys, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  const int res = rs_sort(kb, vb, n, end_bit, (uint32_t*)c->sort_tmp.p, st);
+  LAUNCHCHK("cell sort");
+  HIPCHK(hipMemcpyAsync(order, vb[res], (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return OCSC_OK;
 }
 
 int ocsc_set_profiling(ocsc_ctx* c, int on) {

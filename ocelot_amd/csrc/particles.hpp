@@ -32,6 +32,11 @@ struct Rows {
   int64_t ld;
 };
 
+// first-order drift x += L x', y += L y', tau += r56 delta
+struct DriftArgs {
+  double L, r56;
+};
+
 // coord_transform.py:57-96, one particle.  xp = common-time positions (m) and
 // momenta (eV/c).
 __device__ __forceinline__ void to_xp(double r0, double r1, double r2, double r3, double r4, double r5,
@@ -526,6 +531,33 @@ __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __rest
 }
 
 // ---------------------------------------------------------------------------
+// Permutations between the caller's particle order and the cell order.  A
+// permutation of the six SoA rows and the charge reads (or writes) seven
+// 8-byte values per particle at random places, each costing a whole memory
+// segment.  Staged through one 64-byte record per particle (rows, charge,
+// pad: one aligned segment), the random side moves one segment per particle
+// and every other access is sequential.
+struct alignas(64) PRec {
+  double v[8];  // x, x', y, y', tau, delta, q, 0
+};
+
+__device__ __forceinline__ void store_rec(PRec* __restrict__ dst, const double* v) {
+  double2* d = (double2*)dst->v;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) d[k] = double2{v[2 * k], v[2 * k + 1]};
+}
+
+__device__ __forceinline__ void load_rec(const PRec* __restrict__ src, double* v) {
+  const double2* s = (const double2*)src->v;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double2 a = s[k];
+    v[2 * k] = a.x;
+    v[2 * k + 1] = a.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Cell ordering of a resident bunch (multi-kick tracking): sort keys are the
 // NGP cells under the scalars of the last kick; particles move little between
 // re-sorts, so consecutive particles keep sharing cells.
@@ -540,14 +572,24 @@ __host__ __device__ inline int64_t brick_key_count(int nx, int ny, int nz) {
   return nbx * nby * nbz * (kBx * kBy * kBz);
 }
 
-__global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const KickScalars* __restrict__ scp,
-                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+// rec (may be null): also write each particle's record (rows and charge q),
+// the sequential half of the permutation into cell order
+__global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restrict__ q,
+                                                   const KickScalars* __restrict__ scp, uint32_t* __restrict__ keys,
+                                                   PRec* __restrict__ rec) {
   const KickScalars s = *scp;
   const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
   if (p >= s.n) return;
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = ldnt(R.r + k * R.ld + p);
+  if (rec) {
+    v[6] = ldnt(q + p);
+    v[7] = 0.0;
+    store_rec(rec + p, v);
+  }
   double xp[6], xyz[3], X[3];
-  to_xp_fast(R.r[p], R.r[R.ld + p], R.r[2 * R.ld + p], R.r[3 * R.ld + p], R.r[4 * R.ld + p], R.r[5 * R.ld + p], s,
-             xp);
+  to_xp_fast(v[0], v[1], v[2], v[3], v[4], v[5], s, xp);
   rot(xp, s.T, xyz);
   xyz[2] = xyz[2] * s.gamma0;
   grid_coords_fast(xyz, s, X);
@@ -561,7 +603,52 @@ __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const KickScalars* __
     key = (uint32_t)(brick * (kBx * kBy * kBz) + ((f0 % kBx) * kBy + f1 % kBy) * kBz + f2 % kBz);
   }
   keys[p] = key;
-  vals[p] = (uint32_t)p;
+}
+
+// dst[:, j] = rec[perm[j]] (six rows and the charge), optionally applying a
+// drift to the rows it writes (as k_drift rounds it)
+template <bool DRIFT>
+__global__ void __launch_bounds__(256) k_gather_rec(const PRec* __restrict__ rec, const uint32_t* __restrict__ perm,
+                                                    double* __restrict__ dst, int64_t ldd, double* __restrict__ qd,
+                                                    int64_t n, DriftArgs dr) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  double v[8];
+  load_rec(rec + perm[j], v);
+  if constexpr (DRIFT) {
+    v[0] = __builtin_fma(dr.L, v[1], v[0]);
+    v[2] = __builtin_fma(dr.L, v[3], v[2]);
+    v[4] = __builtin_fma(dr.r56, v[5], v[4]);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) stnt(v[k], dst + k * ldd + j);
+  stnt(v[6], qd + j);
+}
+
+// rec[perm[j]] = src[:, j] (six rows; the random half of the way back to the
+// caller's order: whole-segment writes)
+__global__ void __launch_bounds__(256) k_scatter_rec(const double* __restrict__ src, int64_t lds_,
+                                                     const uint32_t* __restrict__ perm, PRec* __restrict__ rec,
+                                                     int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = ldnt(src + k * lds_ + j);
+  v[6] = 0.0;
+  v[7] = 0.0;
+  store_rec(rec + perm[j], v);
+}
+
+// dst[:, i] = rec[i] for the six rows (sequential)
+__global__ void __launch_bounds__(256) k_rec_to_rows(const PRec* __restrict__ rec, double* __restrict__ dst,
+                                                     int64_t ldd, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double v[8];
+  load_rec(rec + i, v);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) stnt(v[k], dst + k * ldd + i);
 }
 
 // dst[:, j] = src[:, perm[j]] for the six rows and the charge
@@ -859,10 +946,6 @@ __device__ __forceinline__ void gather_field64(const double* __restrict__ E3, co
 // (x += L x', y += L y', tau += r56 delta, as numpy.dot(R, X) rounds it);
 // SUMP stores per-block sums of the momenta the next kick's pass A would
 // compute from the written rows (a drift leaves x', y', delta unchanged).
-struct DriftArgs {
-  double L, r56;
-};
-
 // threads per workgroup of the kick, one particle each (and one momentum
 // partial per workgroup); 128 and 512 measured no faster
 constexpr int kKickPT = 256;
