@@ -298,9 +298,13 @@ typedef struct ocsc_bunch {
  * size, mesh and energy (each round kicks every bunch once).  Bunches are
  * spread over up to 8 workspaces with their own streams so small bunches run
  * concurrently; with use_graph != 0 one round is captured in a HIP graph and
- * replayed.  Per bunch the result equals ocsc_apply.  random_mesh requires
- * nkicks == 1.  Synchronises at the end to report input errors.  No
- * reference counterpart beyond calling apply per bunch. */
+ * replayed.  With option "batch_sort" = 1 (default) each bunch is kicked as a
+ * cell-ordered copy kept across the rounds and written back to the caller's
+ * order at the end: one round equals ocsc_apply bit for bit; in later rounds
+ * the frame sums run in cell order (results equal nkicks calls of ocsc_apply
+ * to round-off).  With "batch_sort" = 0 every round equals ocsc_apply bit for
+ * bit.  random_mesh requires nkicks == 1.  Synchronises at the end to report
+ * input errors.  No reference counterpart beyond calling apply per bunch. */
 int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int precision, int nkicks, int use_graph,
                        void* stream);
 
@@ -318,7 +322,20 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * "permute_rec": 1 (default) stages the permutations between the caller's
  * order and the cell order of ocsc_track_drift through 64-byte particle
  * records (one whole memory segment per particle on the random side), 0
- * permutes row by row.  Both give bit-identical results. */
+ * permutes row by row.  Both give bit-identical results.  "deposit_agg": 1
+ * sums runs of particles in the same cell across the wavefront before the
+ * deposit's LDS table (fewer same-slot LDS atomics in cell order), 0
+ * (default) does not; bit-identical results.  "apply_sort": one-shot kicks
+ * (ocsc_apply) deposit and kick a cell-ordered copy of the bunch and write
+ * the result back in the caller's order: 0 never, 1 always, 2 (default) in
+ * the FP64 form from 1000000 particles on; bit-identical results
+ * (ocsc_apply_cells and ocsc_apply_noalloc never sort).  "deposit_tile": particles per thread and
+ * LDS hash slots of the deposit, 0 (default) 8 and 2048 (FP64: 1024), 1 half
+ * of both, 2 twice both; bit-identical results.  "batch_sort": see
+ * ocsc_apply_batched.  "fft_plan": the mixed form's transform plan, 0
+ * (default) radix 16 where every thread gets a butterfly (else 8), 1 radix 8
+ * in every pass, 2 half as many columns per workgroup; the results differ in
+ * FP32 round-off only. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 

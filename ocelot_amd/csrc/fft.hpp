@@ -136,21 +136,22 @@ __device__ __forceinline__ void stockham_pass(cpx<F>* s, int Ns, const cpx<F>* _
   __syncthreads();
 }
 
-template <typename F, int M, int T, int NT, int DIR, int S, int P = 0>
+template <typename F, int M, int T, int NT, int DIR, int S, int RSEL, int P = 0>
 __device__ __forceinline__ void lds_fft_from(cpx<F>* s, int Ns, const cpx<F>* __restrict__ tw) {
-  using PL = Plan<M, radix_for<M, T, NT>()>;
+  using PL = Plan<M, RSEL ? RSEL : radix_for<M, T, NT>()>;
   if constexpr (P < PL::npass) {
     constexpr int R = PL::radix(P);
     stockham_pass<F, M, R, T, NT, DIR, S>(s, Ns, tw);
-    lds_fft_from<F, M, T, NT, DIR, S, P + 1>(s, Ns * R, tw);
+    lds_fft_from<F, M, T, NT, DIR, S, RSEL, P + 1>(s, Ns * R, tw);
   }
 }
 
 // In-place FFT of T columns held in LDS as s[m*S + t] (caller synchronises
 // before).  DIR = -1: forward exp(-i...), DIR = +1: inverse without 1/M.
-template <typename F, int M, int T, int NT, int DIR, int S = T + 1>
+// RSEL: largest radix of the plan (0: radix_for's choice, or 8 / 16).
+template <typename F, int M, int T, int NT, int DIR, int S = T + 1, int RSEL = 0>
 __device__ __forceinline__ void lds_fft(cpx<F>* s, const cpx<F>* __restrict__ tw) {
-  lds_fft_from<F, M, T, NT, DIR, S, 0>(s, 1, tw);
+  lds_fft_from<F, M, T, NT, DIR, S, RSEL, 0>(s, 1, tw);
 }
 
 }  // namespace ocsc

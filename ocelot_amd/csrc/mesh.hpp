@@ -24,11 +24,15 @@ constexpr int kNT = 256;  // threads per workgroup for the mesh kernels
 // columns per workgroup: LDS footprint M*(T+1)*sizeof(cpx<F>) stays <= ~72 KB
 // (plus the twiddle table W_M^k, k < M, staged behind the tile when the
 // workgroup's LDS stays within 64 KB: every Stockham pass reads it)
-template <typename F, int M>
+// V selects the plan (option "fft_plan"): 0 the default, 1 radix 8 in every
+// pass, 2 half as many columns per workgroup (less LDS, more workgroups)
+template <typename F, int M, int V = 0>
 struct Tile {
   static constexpr int cap = sizeof(F) == 4 ? 64 : 32;
   static constexpr int want = (sizeof(F) == 4 ? 4096 : 2048) / M;
-  static constexpr int T = want < 2 ? 2 : (want > cap ? cap : want);
+  static constexpr int T0 = want < 2 ? 2 : (want > cap ? cap : want);
+  static constexpr int T = V == 2 ? (T0 / 2 < 2 ? 2 : T0 / 2) : T0;
+  static constexpr int RSEL = V == 1 ? 8 : 0;
   static constexpr int S = T + 1;
   static constexpr size_t data_bytes = (size_t)M * S * sizeof(cpx<F>);
   static constexpr bool stw = data_bytes + M * sizeof(cpx<F>) <= 65536;
@@ -91,15 +95,16 @@ __device__ __forceinline__ cpx<F> dfac(const cpx<double>* __restrict__ twd, int 
 
 // ---------------------------------------------------------------------------
 // z pass: pairs of real rows -> one complex FFT of length Mz -> two half spectra
-template <typename F, int M, int CIN>
+template <typename F, int M, int CIN, int V = 0>
 __global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d,
                                               const KickScalars* __restrict__ sc,
                                               const cpx<F>* __restrict__ tw, int ncopy = 1) {
-  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  using TL = Tile<F, M, V>;
+  constexpr int T = TL::T, S = TL::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
-  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
+  const cpx<F>* twl = stage_tw<F, M, kNT, TL::stw>(s + (size_t)M * S, tw);
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   const int64_t ncell = R * nz;
@@ -119,7 +124,7 @@ __global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<
     p[rl & 1] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, -1>(s, twl);
+  lds_fft<F, M, T, kNT, -1, S, TL::RSEL>(s, twl);
   for (int i = threadIdx.x; i < T * Hz; i += kNT) {
     const int t = i / Hz, k = i - t * Hz;
     const int64_t ra = row0 + 2 * t, rb = ra + 1;
@@ -133,15 +138,16 @@ __global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<
 }
 
 // y pass forward: A[x][y<ny][kz] -> B[x][ky<My][kz]
-template <typename F, int M>
+template <typename F, int M, int V = 0>
 __global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d,
                                               const KickScalars* __restrict__ sc,
                                               const cpx<F>* __restrict__ tw) {
-  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
+  using TL = Tile<F, M, V>;
+  constexpr int T = TL::T, S = TL::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
-  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
+  const cpx<F>* twl = stage_tw<F, M, kNT, TL::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
   const int kz0 = blockIdx.x * T;
   const int x = blockIdx.y;
@@ -153,7 +159,7 @@ __global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<
     s[y * S + t] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, -1>(s, twl);
+  lds_fft<F, M, T, kNT, -1, S, TL::RSEL>(s, twl);
   for (int i = threadIdx.x; i < M * T; i += kNT) {
     const int t = i % T, ky = i / T;
     const int kz = kz0 + t;
@@ -233,12 +239,13 @@ __global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<
 
 // z pass inverse: two Hermitian half spectra -> one complex inverse FFT ->
 // two real rows of the potential, z < nz
-template <typename F, int M, bool DERIV>
+template <typename F, int M, bool DERIV, int V = 0>
 __device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __restrict__ phi, const MeshDims& d,
                                           const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
                                           const cpx<double>* __restrict__ twd, cpx<F>* s) {
-  constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
-  const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
+  using TL = Tile<F, M, V>;
+  constexpr int T = TL::T, S = TL::S;
+  const cpx<F>* twl = stage_tw<F, M, kNT, TL::stw>(s + (size_t)M * S, tw);
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
@@ -257,7 +264,7 @@ __device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __res
     if (k > 0 && k < M / 2) s[(M - k) * S + t] = cpx<F>{a.x + b.y, b.x - a.y};
   }
   __syncthreads();
-  lds_fft<F, M, T, kNT, +1>(s, twl);
+  lds_fft<F, M, T, kNT, +1, S, TL::RSEL>(s, twl);
   for (int i = threadIdx.x; i < 2 * T * nz; i += kNT) {
     const int rl = i / nz, z = i - rl * nz;
     const int64_t row = row0 + rl;
@@ -279,7 +286,7 @@ __global__ void __launch_bounds__(kNT) k_zinv(const cpx<F>* __restrict__ A, F* _
 
 // Mixed form: the three field components in one launch (blockIdx.y), the
 // third (z) with the spectral difference factor
-template <int M>
+template <int M, int V = 0>
 __global__ void __launch_bounds__(kNT) k_zinv3(const cpx<float>* __restrict__ A, int64_t nyz, float* __restrict__ E,
                                                int64_t ncell, MeshDims d, const KickScalars* __restrict__ sc,
                                                const cpx<float>* __restrict__ tw,
@@ -289,9 +296,9 @@ __global__ void __launch_bounds__(kNT) k_zinv3(const cpx<float>* __restrict__ A,
   const int comp = blockIdx.y;
   cpx<float>* s = (cpx<float>*)smem_raw;
   if (comp == 2)
-    zinv_rows<float, M, true>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s);
+    zinv_rows<float, M, true, V>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s);
   else
-    zinv_rows<float, M, false>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s);
+    zinv_rows<float, M, false, V>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s);
 }
 
 // columns per workgroup of the kernels that carry two or three fields
@@ -300,13 +307,14 @@ __global__ void __launch_bounds__(kNT) k_zinv3(const cpx<float>* __restrict__ A,
 // step at 127^3: 0.894 -> 0.887 ms).
 constexpr int kNTF = 512;
 
-template <int M, int NT = kNT>
+template <int M, int NT = kNT, int V = 0>
 struct FieldTile {
-  static constexpr int T = Tile<float, M>::T;  // columns per workgroup
+  static constexpr int T = Tile<float, M, V>::T;  // columns per workgroup
   static constexpr int S = T + 1;
   static constexpr int PER = (M * T + NT - 1) / NT;  // spectrum values held per thread
-  static constexpr bool stw = Tile<float, M>::stw;
-  static constexpr size_t lds = Tile<float, M>::lds_bytes;
+  static constexpr bool stw = Tile<float, M, V>::stw;
+  static constexpr int RSEL = Tile<float, M, V>::RSEL;
+  static constexpr size_t lds = Tile<float, M, V>::lds_bytes;
 };
 
 // Mixed form, x pass: forward FFT along x, times the kernel spectrum, then
@@ -314,18 +322,18 @@ struct FieldTile {
 // potential, from which the y and z fields follow by their own factors).
 // The product S*G stays in registers between the two inverse transforms, so
 // LDS holds one tile of T columns (several workgroups per CU).
-template <int M>
+template <int M, int V = 0>
 __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
                                                      cpx<float>* __restrict__ P, const float* __restrict__ G,
                                                      MeshDims d, const KickScalars* __restrict__ sc,
                                                      const cpx<float>* __restrict__ tw,
                                                      const cpx<double>* __restrict__ twd) {
   using F = float;
-  constexpr int T = FieldTile<M, kNTF>::T, S = FieldTile<M, kNTF>::S, PER = FieldTile<M, kNTF>::PER;
+  constexpr int T = FieldTile<M, kNTF, V>::T, S = FieldTile<M, kNTF, V>::S, PER = FieldTile<M, kNTF, V>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
-  const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF>::stw>(s + (size_t)M * S, tw);
+  const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF, V>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
   const int kz0 = blockIdx.x * T;
   const int ky = blockIdx.y;
@@ -338,7 +346,7 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
     s[x * S + t] = v;
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, -1, S>(s, twl);
+  lds_fft<F, M, T, kNTF, -1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
   const int fy = ky <= My - ky ? ky : My - ky;
   const double inv_hx = 1.0 / sc->steps[0];
   cpx<F> v[PER];
@@ -364,7 +372,7 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
     }
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
+  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
   for (int i = threadIdx.x; i < nx * T; i += kNTF) {
     const int t = i % T, x = i / T;
     const int kz = kz0 + t;
@@ -377,7 +385,7 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
     if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + i % T] = v[k];
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
+  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
   for (int i = threadIdx.x; i < nx * T; i += kNTF) {
     const int t = i % T, x = i / T;
     const int kz = kz0 + t;
@@ -388,7 +396,7 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
 // Mixed form, y inverse: Ex' = IFFT_y(Qx), Ey' = IFFT_y(P * Dy), Phi' = IFFT_y(P)
 // (all [nx][ny][Hz]); the z field is Dz * Phi', applied in the z inverse.
 // Three transforms of one LDS tile in turn, P held in registers.
-template <int M>
+template <int M, int V = 0>
 __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restrict__ Qx,
                                                     const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax,
                                                     cpx<float>* __restrict__ Ay, cpx<float>* __restrict__ Ap,
@@ -396,11 +404,11 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
                                                     const cpx<float>* __restrict__ tw,
                                                     const cpx<double>* __restrict__ twd) {
   using F = float;
-  constexpr int T = FieldTile<M, kNTF>::T, S = FieldTile<M, kNTF>::S, PER = FieldTile<M, kNTF>::PER;
+  constexpr int T = FieldTile<M, kNTF, V>::T, S = FieldTile<M, kNTF, V>::S, PER = FieldTile<M, kNTF, V>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
-  const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF>::stw>(s + (size_t)M * S, tw);
+  const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF, V>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
   const int kz0 = blockIdx.x * T;
   const int x = blockIdx.y;
@@ -423,7 +431,7 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
     }
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
+  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
   auto store = [&](cpx<F>* dst) {
     for (int i = threadIdx.x; i < ny * T; i += kNTF) {
       const int t = i % T, y = i / T;
@@ -439,7 +447,7 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
     if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + i % T] = cmul(p[k], dfac<F>(twd, i / T, inv_hy));
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
+  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
   store(Ay);
   __syncthreads();
 #pragma unroll
@@ -448,7 +456,7 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
     if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + i % T] = p[k];
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S>(s, twl);
+  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
   store(Ap);
 }
 

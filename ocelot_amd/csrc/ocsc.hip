@@ -161,6 +161,7 @@ struct Lane {
   Buf grid, A, B, phi, G;
   Buf KA, KB, KF;                  // kernel-spectrum build scratch (its own, so it can overlap the FFTs)
   Buf E64;                         // FP64 form: the three difference fields of phi
+  Buf sk[2], sv, sh, srec;         // batched calls: sort keys, values, histograms, records (cell-order copies)
   hipStream_t stream = nullptr;    // extra lanes only
   hipEvent_t done = nullptr;       // extra lanes only
 };
@@ -186,9 +187,19 @@ struct ocsc_ctx {
   int profiling = 0;
   // tuning options (ocsc_set_option)
   int deposit_xcd = 0;  // 1: per-XCD grid copies with L2-local atomics
+  int deposit_agg = 0;  // 1: runs of equal cells summed across the wave before the LDS table
+  int deposit_tile = 0;  // deposit particles per thread and hash slots (DepTile)
+  int fft_plan = 0;      // mixed-form transform plan (Tile's V): 0 default, 1 radix 8, 2 half-width tiles
   int red_blocks = kMaxRedBlocks;  // blocks of the strided reduction passes
   int sort_every = 10;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
   int permute_rec = 1;  // tracking: permutations staged through 64-byte particle records (0: row by row)
+  int apply_sort = 2;   // one-shot kicks on a cell-ordered copy: 0 never, 1 always, 2 FP64 from kApplySortMin particles
+  int batch_sort = 1;   // batched calls keep a cell-ordered copy of each bunch across the kicks of a call
+  // cell-ordered copy of bunch i of a batched call: six rows, charges, permutation
+  struct BunchCopy {
+    Buf rs, qs, perm;
+  };
+  std::vector<BunchCopy*> bcopy;
   // tracking workspace: two cell-ordered copies of the bunch, sort buffers,
   // particle records for the permutations
   Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp, rec;
@@ -416,9 +427,11 @@ int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho,
 
 // Mixed form: the three field components on the mesh, E = -grad phi as the
 // forward differences of the reference (sc.py:195-200) taken in Fourier
-// space, written as three FP32 grids one after the other at E.
-int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, float* E, hipStream_t st,
-                  hipEvent_t* ev = nullptr, hipEvent_t join = nullptr) {
+// space, written as three FP32 grids one after the other at E.  V: the FFT
+// plan variant (option "fft_plan").
+template <int V>
+int launch_fields_v(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, float* E, hipStream_t st,
+                    hipEvent_t* ev, hipEvent_t join) {
   const MeshDims& d = g.d;
   const int64_t nyz = (int64_t)d.nx * d.ny * d.hz;
   cpx<float>* A = (cpx<float>*)L.A.p;
@@ -426,44 +439,56 @@ int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, floa
   cpx<float>* B2 = B + (int64_t)d.nx * d.my * d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   OCSC_SWITCH_M(d.mz, {
-    using TL = Tile<float, MM>;
+    using TL = Tile<float, MM, V>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zfwd<float, MM, kChargeFix1><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, L.sc, c->twf + MM,
-                                                                    c->deposit_xcd ? kNumXcd : 1);
+    k_zfwd<float, MM, kChargeFix1, V><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, L.sc, c->twf + MM,
+                                                                       c->deposit_xcd ? kNumXcd : 1);
   });
   LAUNCHCHK("k_zfwd");
   if (ev) (void)hipEventRecord(ev[1], st);
   OCSC_SWITCH_M(d.my, {
-    using TL = Tile<float, MM>;
+    using TL = Tile<float, MM, V>;
     dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
-    k_yfwd<float, MM><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, L.sc, c->twf + MM);
+    k_yfwd<float, MM, V><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, L.sc, c->twf + MM);
   });
   LAUNCHCHK("k_yfwd");
   if (ev) (void)hipEventRecord(ev[2], st);
   if (join) HIPCHK(hipStreamWaitEvent(st, join, 0));  // the kernel spectrum, built on another stream
   OCSC_SWITCH_M(d.mx, {
-    using FT = FieldTile<MM, kNTF>;
+    using FT = FieldTile<MM, kNTF, V>;
     dim3 grid((d.hz + FT::T - 1) / FT::T, d.my);
-    k_xconv_field<MM><<<grid, kNTF, FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
-                                                   c->twd + MM);
+    k_xconv_field<MM, V><<<grid, kNTF, FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
+                                                      c->twd + MM);
   });
   LAUNCHCHK("k_xconv_field");
   if (ev) (void)hipEventRecord(ev[3], st);
   OCSC_SWITCH_M(d.my, {
-    using FT = FieldTile<MM, kNTF>;
+    using FT = FieldTile<MM, kNTF, V>;
     dim3 grid((d.hz + FT::T - 1) / FT::T, d.nx);
-    k_yinv_field<MM><<<grid, kNTF, FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
-                                                  c->twd + MM);
+    k_yinv_field<MM, V><<<grid, kNTF, FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
+                                                     c->twd + MM);
   });
   LAUNCHCHK("k_yinv_field");
   if (ev) (void)hipEventRecord(ev[4], st);
   OCSC_SWITCH_M(d.mz, {
-    using TL = Tile<float, MM>;
+    using TL = Tile<float, MM, V>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zinv3<MM><<<dim3(nb, 3), kNT, TL::lds_bytes, st>>>(A, nyz, E, g.ncell, d, L.sc, c->twf + MM, c->twd + MM);
+    k_zinv3<MM, V><<<dim3(nb, 3), kNT, TL::lds_bytes, st>>>(A, nyz, E, g.ncell, d, L.sc, c->twf + MM, c->twd + MM);
   });
   LAUNCHCHK("k_zinv(fields)");
   return OCSC_OK;
+}
+
+int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, float* E, hipStream_t st,
+                  hipEvent_t* ev = nullptr, hipEvent_t join = nullptr) {
+  switch (c->fft_plan) {
+    case 1:
+      return launch_fields_v<1>(c, L, g, rho, E, st, ev, join);
+    case 2:
+      return launch_fields_v<2>(c, L, g, rho, E, st, ev, join);
+    default:
+      return launch_fields_v<0>(c, L, g, rho, E, st, ev, join);
+  }
 }
 
 KickInputs kick_inputs(const int32_t* nmesh, int64_t n, double E_GeV, double zstep, int random_mesh,
@@ -514,6 +539,8 @@ int flush_all(ocsc_ctx* c) {
 }
 
 // the launch sequence of one kick (no host synchronisation, graph-capturable)
+struct SortBufs;
+
 // options of one kick inside a tracking loop
 struct StepOpts {
   bool passA = true;   // false: the previous kick left its momentum sums in the partials
@@ -524,7 +551,46 @@ struct StepOpts {
   hipEvent_t kev0 = nullptr, kev1 = nullptr;  // recorded around the kick kernel (kick timing)
   bool kev_external = false;                  // capturing: the graph gets event-record nodes instead
   int32_t* cells = nullptr;                   // deposit writes each particle's NGP cell here (or null)
+  // deposit and kick a cell-ordered copy of the bunch (frame and extent still
+  // from the caller's rows), null: in place
+  const SortBufs* sort = nullptr;
 };
+
+// buffers of a kick on a cell-ordered copy: sort keys and values (ping-pong),
+// histograms, particle records, the copy itself; order_out (may be null)
+// receives the permutation (copy position -> caller position); scatter_back
+// writes the kicked copy back to the caller's rows
+struct SortBufs {
+  uint32_t* k[2];
+  uint32_t* v[2];
+  uint32_t* hist;
+  PRec* rec;
+  double* rs;  // six rows, row stride n
+  double* qs;
+  uint32_t* order_out;
+  bool scatter_back;
+};
+
+template <int NWORD, bool XCD, bool AGG, int TILE>
+void launch_deposit_t(Rows R, const double* q, KickScalars* sc, unsigned long long* grid, int32_t* cells, int64_t n,
+                      hipStream_t st) {
+  constexpr int per = kDepPT * DepTile<TILE>::items;
+  k_deposit<NWORD, XCD, AGG, TILE><<<(int)((n + per - 1) / per), kDepPT, 0, st>>>(R, q, sc, grid, cells);
+}
+
+// the deposit variant the context's options select
+template <int NWORD>
+void launch_deposit(const ocsc_ctx* c, Rows R, const double* q, KickScalars* sc, unsigned long long* grid,
+                    int32_t* cells, int64_t n, hipStream_t st) {
+  const int v = (c->deposit_xcd ? 4 : 0) + (c->deposit_agg ? 2 : 0);
+  const int t = c->deposit_tile;
+#define OCSC_DEP(V, T)                                                                                 \
+  if (v == V && t == T) return launch_deposit_t<NWORD, (V & 4) != 0, (V & 2) != 0, T>(R, q, sc, grid, cells, n, st);
+#define OCSC_DEP3(V) OCSC_DEP(V, 0) OCSC_DEP(V, 1) OCSC_DEP(V, 2)
+  OCSC_DEP3(0) OCSC_DEP3(2) OCSC_DEP3(4) OCSC_DEP3(6)
+#undef OCSC_DEP3
+#undef OCSC_DEP
+}
 
 template <typename F, bool DRIFT, bool SUMP>
 void launch_kick(ocsc_ctx* c, Lane& L, int npart, double* r, int64_t ld, const StepOpts& o, hipStream_t st) {
@@ -599,11 +665,24 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   } else {
     zero_grid(st);
   }
-  const int ndep = (int)((n + kDepPT * kDepItems - 1) / (kDepPT * kDepItems));
-  if (c->deposit_xcd)
-    k_deposit<NWORD, true><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p, o.cells);
-  else
-    k_deposit<NWORD, false><<<ndep, kDepPT, 0, st>>>(R, q, L.sc, (unsigned long long*)L.grid.p, o.cells);
+  // one-shot kick on a cell-ordered copy: the frame and extent above came
+  // from the caller's order, the deposit sums integers and the kick is per
+  // particle, so the result equals the caller-order kick bit for bit
+  const uint32_t* order = nullptr;
+  const SortBufs* sb = o.sort;
+  if (sb) {
+    const int nb = (int)((n + 255) / 256);
+    int end_bit = 1;
+    while ((int64_t(1) << end_bit) <= brick_key_count(g.d.nx, g.d.ny, g.d.nz)) ++end_bit;
+    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(R, q, L.sc, sb->k[0], sb->rec);
+    order = sb->v[rs_sort(sb->k, sb->v, n, end_bit, sb->hist, st)];
+    k_gather_rec<false><<<nb, 256, 0, st>>>(sb->rec, order, sb->rs, n, sb->qs, n, DriftArgs{0, 0});
+    if (sb->order_out) HIPCHK(hipMemcpyAsync(sb->order_out, order, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    LAUNCHCHK("cell-order copy");
+    R = Rows{sb->rs, n};
+    q = sb->qs;
+  }
+  launch_deposit<NWORD>(c, R, q, L.sc, (unsigned long long*)L.grid.p, o.cells, n, st);
   LAUNCHCHK("deposit");
   mark(3);
   if (!fork && (rc = launch_kernel_spectrum<F>(c, L, g, st, 0))) return rc;
@@ -622,15 +701,25 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   }
   mark(9);
   if (o.kev0 && !o.kev_external) HIPCHK(hipEventRecord(o.kev0, st));
+  double* rk = sb ? sb->rs : r;
+  const int64_t ldk = sb ? n : ld;
   if (o.drift && o.sump)
-    launch_kick<F, true, true>(c, L, npart, r, ld, o, st);
+    launch_kick<F, true, true>(c, L, npart, rk, ldk, o, st);
   else if (o.drift)
-    launch_kick<F, true, false>(c, L, npart, r, ld, o, st);
+    launch_kick<F, true, false>(c, L, npart, rk, ldk, o, st);
   else if (o.sump)
-    launch_kick<F, false, true>(c, L, npart, r, ld, o, st);
+    launch_kick<F, false, true>(c, L, npart, rk, ldk, o, st);
   else
-    launch_kick<F, false, false>(c, L, npart, r, ld, o, st);
+    launch_kick<F, false, false>(c, L, npart, rk, ldk, o, st);
   LAUNCHCHK("k_kick");
+  if (sb && sb->scatter_back) {
+    // back to the caller's order through whole records; a bunch the kick
+    // rejected (error flag) was not written, so nothing is copied back
+    const int nb = (int)((n + 255) / 256);
+    k_scatter_rec<<<nb, 256, 0, st>>>(rk, n, order, sb->rec, n);
+    k_rec_to_rows_checked<<<nb, 256, 0, st>>>(sb->rec, r, ld, n, L.sc);
+    LAUNCHCHK("caller-order copy");
+  }
   if (o.kev1 && !o.kev_external) HIPCHK(hipEventRecord(o.kev1, st));
   mark(10);
   if (prof) {
@@ -701,6 +790,11 @@ int check_common(ocsc_ctx* c, const void* r, const double* q, int64_t n, int64_t
 namespace {
 
 constexpr int kMaxLanes = 8;
+// one-shot FP64 kicks of at least this many particles go through a
+// cell-ordered copy (measured at 10M / 127^3: FP64 2.58 -> 2.42 ms, where the
+// two-word deposit's random atomics dominate; mixed 1.77 -> 2.29 ms, so the
+// mixed form stays in the caller's order)
+constexpr int64_t kApplySortMin = 1000000;
 
 // device address of the no-allocation workspace on `device` (current)
 int scratch_for_device(int device, void** out);
@@ -719,7 +813,8 @@ int init_lane(Lane& L, bool own_stream) {
 }
 
 void free_lane(Lane& L) {
-  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.KA, &L.KB, &L.KF, &L.phi, &L.G, &L.E64})
+  for (Buf* b : {&L.part, &L.grid, &L.A, &L.B, &L.KA, &L.KB, &L.KF, &L.phi, &L.G, &L.E64, &L.sk[0], &L.sk[1], &L.sv,
+                 &L.sh, &L.srec})
     if (b->p) (void)hipFree(b->p);
   if (L.sc) (void)hipFree(L.sc);
   if (L.sc_host) (void)hipHostFree(L.sc_host);
@@ -831,6 +926,11 @@ int ocsc_destroy(ocsc_ctx* c) {
     free_lane(*l);
     delete l;
   }
+  for (ocsc_ctx::BunchCopy* bc : c->bcopy) {
+    for (Buf* b : {&bc->rs, &bc->qs, &bc->perm})
+      if (b->p) (void)hipFree(b->p);
+    delete bc;
+  }
   delete c;
   return OCSC_OK;
 }
@@ -861,6 +961,23 @@ int ocsc_apply_cells(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   KickInputs in = kick_inputs(nmesh, n, E_GeV, zstep, random_mesh, mesh_scale, mesh_shift);
   StepOpts o;
   o.cells = cells;
+  // cell-ordered copy for the deposit and the kick (not with cells: those
+  // come in the caller's order)
+  SortBufs sb{};
+  if (!cells && n <= (int64_t)0xFFFFFFFF &&
+      (c->apply_sort == 1 || (c->apply_sort == 2 && prec == OCSC_FP64 && n >= kApplySortMin))) {
+    for (int k = 0; k < 2; ++k)
+      if ((rc = ensure(c->keys[k], (size_t)n * 4))) return rc;
+    if ((rc = ensure(c->vals, (size_t)2 * n * 4)) || (rc = ensure(c->sort_tmp, rs_hist_bytes(n))) ||
+        (rc = ensure(c->rec, (size_t)n * sizeof(PRec))) || (rc = ensure(c->rs[0], (size_t)6 * n * 8)) ||
+        (rc = ensure(c->qs[0], (size_t)n * 8)))
+      return rc;
+    sb = SortBufs{{(uint32_t*)c->keys[0].p, (uint32_t*)c->keys[1].p},
+                  {(uint32_t*)c->vals.p, (uint32_t*)c->vals.p + n},
+                  (uint32_t*)c->sort_tmp.p, (PRec*)c->rec.p, (double*)c->rs[0].p, (double*)c->qs[0].p,
+                  nullptr, true};
+    o.sort = &sb;
+  }
   if ((rc = enqueue_kick_prec(c, L, g, r, ld, q, in, prec, st, o))) return rc;
   if (sync) {
     int32_t err = 0;
@@ -1849,36 +1966,111 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
     if ((rc = ensure_prec(c, L, geo[i], prec))) return rc;
     if ((rc = ensure_part(c, L, b[i].n))) return rc;
   }
-  // one round: every bunch kicked once, bunch i on lane i % nl
+  // Cell-ordered copies: the first kick of each bunch takes its frame and
+  // extent from the caller's rows (as ocsc_apply), sorts the bunch into its
+  // copy and kicks the copy; later kicks work on the copy; the copies go back
+  // to the caller's order at the end.  One kick per call equals ocsc_apply
+  // bit for bit; later kicks reduce the frame sums in cell order.
+  std::vector<char> sorted(nb, 0);
+  std::vector<SortBufs> sbs(nb);
+  bool any_sorted = false;
+  if (c->batch_sort) {
+    while ((int)c->bcopy.size() < nb) c->bcopy.push_back(new ocsc_ctx::BunchCopy());
+    for (int i = 0; i < nb; ++i) {
+      const int64_t n = b[i].n;
+      if (b[i].zstep == 0 || n > (int64_t)0xFFFFFFFF) continue;
+      ocsc_ctx::BunchCopy& bc = *c->bcopy[i];
+      Lane& L = *c->lanes[i % nl];
+      if ((rc = ensure(bc.rs, (size_t)6 * n * 8)) || (rc = ensure(bc.qs, (size_t)n * 8)) ||
+          (rc = ensure(bc.perm, (size_t)n * 4)) || (rc = ensure(L.sk[0], (size_t)n * 4)) ||
+          (rc = ensure(L.sk[1], (size_t)n * 4)) || (rc = ensure(L.sv, (size_t)2 * n * 4)) ||
+          (rc = ensure(L.sh, rs_hist_bytes(n))) || (rc = ensure(L.srec, (size_t)n * sizeof(PRec))))
+        return rc;
+      sorted[i] = 1;
+      any_sorted = true;
+    }
+    for (int i = 0; i < nb; ++i) {
+      if (!sorted[i]) continue;
+      Lane& L = *c->lanes[i % nl];
+      ocsc_ctx::BunchCopy& bc = *c->bcopy[i];
+      sbs[i] = SortBufs{{(uint32_t*)L.sk[0].p, (uint32_t*)L.sk[1].p},
+                        {(uint32_t*)L.sv.p, (uint32_t*)L.sv.p + b[i].n},
+                        (uint32_t*)L.sh.p, (PRec*)L.srec.p, (double*)bc.rs.p, (double*)bc.qs.p,
+                        (uint32_t*)bc.perm.p, false};
+    }
+  }
+  // one round: every bunch kicked once, bunch i on lane i % nl; first: the
+  // round that makes the cell-ordered copies
   const int prof = c->profiling, ovl = c->overlap;
   c->profiling = 0;  // stage events and the side stream are per context
   c->overlap = 0;
-  auto round = [&](hipStream_t origin) -> int {
+  auto fork = [&](hipStream_t origin) -> int {
     HIPCHK(hipEventRecord(c->ev_fork, origin));
     for (int l = 0; l < nl; ++l) HIPCHK(hipStreamWaitEvent(c->lanes[l]->stream, c->ev_fork, 0));
-    for (int i = 0; i < nb; ++i) {
-      Lane& L = *c->lanes[i % nl];
-      if (b[i].zstep == 0) continue;
-      int r2 = enqueue_kick_prec(c, L, geo[i], b[i].rparticles, b[i].ld, b[i].q_array, ins[i], prec, L.stream);
-      if (r2) return r2;
-    }
+    return OCSC_OK;
+  };
+  auto join = [&](hipStream_t origin) -> int {
     for (int l = 0; l < nl; ++l) {
       HIPCHK(hipEventRecord(c->lanes[l]->done, c->lanes[l]->stream));
       HIPCHK(hipStreamWaitEvent(origin, c->lanes[l]->done, 0));
     }
     return OCSC_OK;
   };
+  auto round = [&](hipStream_t origin, bool first) -> int {
+    int r2;
+    if ((r2 = fork(origin))) return r2;
+    for (int i = 0; i < nb; ++i) {
+      Lane& L = *c->lanes[i % nl];
+      if (b[i].zstep == 0) continue;
+      StepOpts o;
+      double* rr = b[i].rparticles;
+      int64_t ldr = b[i].ld;
+      const double* qq = b[i].q_array;
+      if (sorted[i] && first) {
+        o.sort = &sbs[i];
+      } else if (sorted[i]) {
+        rr = sbs[i].rs;
+        ldr = b[i].n;
+        qq = sbs[i].qs;
+      }
+      if ((r2 = enqueue_kick_prec(c, L, geo[i], rr, ldr, qq, ins[i], prec, L.stream, o))) return r2;
+    }
+    return join(origin);
+  };
+  // the copies back to the callers' rows (whole records, then rows)
+  auto unsort = [&](hipStream_t origin) -> int {
+    int r2;
+    if ((r2 = fork(origin))) return r2;
+    for (int i = 0; i < nb; ++i) {
+      if (!sorted[i]) continue;
+      Lane& L = *c->lanes[i % nl];
+      const int64_t n = b[i].n;
+      const int nbk = (int)((n + 255) / 256);
+      k_scatter_rec<<<nbk, 256, 0, L.stream>>>(sbs[i].rs, n, sbs[i].order_out, sbs[i].rec, n);
+      k_rec_to_rows<<<nbk, 256, 0, L.stream>>>(sbs[i].rec, b[i].rparticles, b[i].ld, n);
+      LAUNCHCHK("batched caller-order copy");
+    }
+    return join(origin);
+  };
   auto restore = [&]() {
     c->profiling = prof;
     c->overlap = ovl;
   };
+  if (!use_graph || any_sorted) {
+    // the copying round eagerly; with graphs the later rounds are captured
+    if ((rc = round(st, true))) {
+      restore();
+      return rc;
+    }
+  }
+  const int first_graph_round = (!use_graph || any_sorted) ? 1 : 0;
   if (!use_graph) {
-    for (int k = 0; k < nkicks; ++k)
-      if ((rc = round(st))) {
+    for (int k = 1; k < nkicks; ++k)
+      if ((rc = round(st, false))) {
         restore();
         return rc;
       }
-  } else {
+  } else if (nkicks > first_graph_round) {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipError_t e = hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal);
@@ -1886,7 +2078,7 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
       restore();
       return fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
     }
-    int r2 = round(c->cap_stream);
+    int r2 = round(c->cap_stream, !any_sorted);
     e = hipStreamEndCapture(c->cap_stream, &graph);
     if (r2 || e != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -1894,7 +2086,7 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
       return r2 ? r2 : fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
     }
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    for (int k = 0; k < nkicks && e == hipSuccess; ++k) e = hipGraphLaunch(exec, st);
+    for (int k = first_graph_round; k < nkicks && e == hipSuccess; ++k) e = hipGraphLaunch(exec, st);
     if (exec) (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
@@ -1903,6 +2095,7 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
     }
   }
   restore();
+  if (any_sorted && (rc = unsort(st))) return rc;
   int32_t err_all = 0;
   for (int l = 0; l < nl; ++l) {
     int32_t err = 0;
@@ -1943,6 +2136,31 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->permute_rec = (int)value;
     return OCSC_OK;
   }
+  if (n == "deposit_agg") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "deposit_agg must be 0 or 1");
+    c->deposit_agg = (int)value;
+    return OCSC_OK;
+  }
+  if (n == "apply_sort") {
+    if (value < 0 || value > 2) return fail(OCSC_EINVAL, "apply_sort must be 0, 1 or 2");
+    c->apply_sort = (int)value;
+    return OCSC_OK;
+  }
+  if (n == "deposit_tile") {
+    if (value < 0 || value > 2) return fail(OCSC_EINVAL, "deposit_tile must be 0, 1 or 2");
+    c->deposit_tile = (int)value;
+    return OCSC_OK;
+  }
+  if (n == "batch_sort") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "batch_sort must be 0 or 1");
+    c->batch_sort = (int)value;
+    return OCSC_OK;
+  }
+  if (n == "fft_plan") {
+    if (value < 0 || value > 2) return fail(OCSC_EINVAL, "fft_plan must be 0, 1 or 2");
+    c->fft_plan = (int)value;
+    return OCSC_OK;
+  }
   return fail(OCSC_EINVAL, "unknown option " + n);
 }
 
@@ -1967,6 +2185,26 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "permute_rec") {
     *value = c->permute_rec;
+    return OCSC_OK;
+  }
+  if (n == "deposit_agg") {
+    *value = c->deposit_agg;
+    return OCSC_OK;
+  }
+  if (n == "apply_sort") {
+    *value = c->apply_sort;
+    return OCSC_OK;
+  }
+  if (n == "deposit_tile") {
+    *value = c->deposit_tile;
+    return OCSC_OK;
+  }
+  if (n == "batch_sort") {
+    *value = c->batch_sort;
+    return OCSC_OK;
+  }
+  if (n == "fft_plan") {
+    *value = c->fft_plan;
     return OCSC_OK;
   }
   return fail(OCSC_EINVAL, "unknown option " + n);

@@ -451,6 +451,18 @@ constexpr int kDepSlots = 2048;              // hash slots (power of two), one-w
 constexpr int kDepSlots2 = 1024;             // two-word (FP64) form: half the LDS per workgroup, more
                                              // workgroups per CU (FP64 step 0.964 -> 0.950 ms)
 
+// deposit tiles (option "deposit_tile"): 0 the sizes above, 1 half the
+// particles per thread and half the slots, 2 twice both
+template <int TILE>
+struct DepTile {
+  static constexpr int items = TILE == 1 ? kDepItems / 2 : (TILE == 2 ? kDepItems * 2 : kDepItems);
+  template <int NWORD>
+  static constexpr int slots() {
+    constexpr int base = NWORD == 2 ? kDepSlots2 : kDepSlots;
+    return TILE == 1 ? base / 2 : (TILE == 2 ? base * 2 : base);
+  }
+};
+
 template <bool XCD>
 __device__ __forceinline__ void flush_cell(unsigned long long* g, int64_t ncell, int64_t idx, long long hi,
                                            long long lo, int nword) {
@@ -460,10 +472,35 @@ __device__ __forceinline__ void flush_cell(unsigned long long* g, int64_t ncell,
 
 // cells (may be null): receives the flat cell index each particle's charge
 // went to (-1 outside the mesh), in the order of the rows
-template <int NWORD, bool XCD = false>
+// one (cell, fixed-point value) into the workgroup's LDS hash table (linear
+// probing; cell indices fit in 31 bits: mesh <= 1024^3), or straight to the
+// grid when the table is crowded
+template <int NWORD, bool XCD, int kSlots>
+__device__ __forceinline__ void hash_add(int* hkey, long long (*hval)[kSlots], unsigned long long* g, int64_t ncell,
+                                         int64_t idx, long long hi, long long lo) {
+  unsigned slot = ((unsigned)idx * 2654435761u) >> (32 - ilog2(kSlots));
+  for (int probe = 0; probe < 32; ++probe) {
+    const int k = atomicCAS(&hkey[slot], -1, (int)idx);
+    if (k == -1 || k == (int)idx) {
+      atomicAdd((unsigned long long*)&hval[0][slot], (unsigned long long)hi);
+      if constexpr (NWORD == 2) atomicAdd((unsigned long long*)&hval[1][slot], (unsigned long long)lo);
+      return;
+    }
+    slot = (slot + 1) & (kSlots - 1);
+  }
+  flush_cell<XCD>(g, ncell, idx, hi, lo, NWORD);
+}
+
+// Wave-level pre-sum (AGG): in cell order, neighbouring lanes usually carry
+// the same cell, and their LDS atomics on one slot serialise.  Runs of equal
+// cells in lane order are summed first (segmented inclusive scan with
+// shuffles, exact in integers); only the last lane of each run goes to the
+// table.  Cells that recur in separate runs are simply added twice.
+template <int NWORD, bool XCD = false, bool AGG = false, int TILE = 0>
 __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
                                                  unsigned long long* __restrict__ grid, int32_t* __restrict__ cells) {
-  constexpr int kSlots = NWORD == 2 ? kDepSlots2 : kDepSlots;
+  constexpr int kSlots = DepTile<TILE>::template slots<NWORD>();
+  constexpr int kItems = DepTile<TILE>::items;
   __shared__ int hkey[kSlots];
   __shared__ long long hval[NWORD][kSlots];
   const KickScalars& s0 = *scp;
@@ -478,7 +515,8 @@ __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __rest
     if constexpr (NWORD == 2) hval[1][i] = 0;
   }
   __syncthreads();
-  const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (kDepPT * kDepItems);
+  const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (kDepPT * kItems);
+  const int lane = threadIdx.x & 63;
   int bad = 0;
   // the rows of item it + 1 are loaded before item it goes through the table
   double cur[7], nxt[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -489,38 +527,53 @@ __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __rest
   };
   int64_t p = base + threadIdx.x;
   if (p < s.n) load(p, cur);
-  for (int it = 0; it < kDepItems; ++it, p += kDepPT) {
-    if (p >= s.n) break;
-    const int64_t pn = p + kDepPT;
-    if (it + 1 < kDepItems && pn < s.n) load(pn, nxt);
-    const int64_t idx = ngp_cell<NWORD == 1>(cur, s);  // one word: the mixed form
-    if (cells) cells[p] = (int32_t)idx;
-    const double qv = cur[6];
+  for (int it = 0; it < kItems; ++it, p += kDepPT) {
+    const bool valid = p < s.n;
+    if constexpr (AGG) {
+      if (__ballot(valid) == 0) break;  // uniform across the wave: every lane takes part in the scan
+    } else {
+      if (!valid) break;
+    }
+    int64_t idx = -1;
+    long long hi = 0, lo = 0;
+    if (valid) {
+      const int64_t pn = p + kDepPT;
+      if (it + 1 < kItems && pn < s.n) load(pn, nxt);
+      idx = ngp_cell<NWORD == 1>(cur, s);  // one word: the mixed form
+      if (cells) cells[p] = (int32_t)idx;
+      const double qv = cur[6];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) cur[k] = nxt[k];
-    if (idx < 0) {
-      bad = 1;
-      continue;
-    }
-    const double v = qv * s.qscale;  // exact: qscale is a power of two
-    const double h = __builtin_rint(v);
-    const long long hi = (long long)h;
-    long long lo = 0;
-    if constexpr (NWORD == 2) lo = (long long)__builtin_rint((v - h) * 0x1p30);
-    // linear probing; cell indices fit in 31 bits (mesh <= 1024^3)
-    unsigned slot = ((unsigned)idx * 2654435761u) >> (32 - ilog2(kSlots));
-    bool done = false;
-    for (int probe = 0; probe < 32; ++probe) {
-      const int k = atomicCAS(&hkey[slot], -1, (int)idx);
-      if (k == -1 || k == (int)idx) {
-        atomicAdd((unsigned long long*)&hval[0][slot], (unsigned long long)hi);
-        if constexpr (NWORD == 2) atomicAdd((unsigned long long*)&hval[1][slot], (unsigned long long)lo);
-        done = true;
-        break;
+      for (int k = 0; k < 7; ++k) cur[k] = nxt[k];
+      if (idx < 0) {
+        bad = 1;
+      } else {
+        const double v = qv * s.qscale;  // exact: qscale is a power of two
+        const double h = __builtin_rint(v);
+        hi = (long long)h;
+        if constexpr (NWORD == 2) lo = (long long)__builtin_rint((v - h) * 0x1p30);
       }
-      slot = (slot + 1) & (kSlots - 1);
     }
-    if (!done) flush_cell<XCD>(g, ncell, idx, hi, lo, NWORD);  // crowded table: straight to the grid
+    if constexpr (AGG) {
+      const int key = valid && idx >= 0 ? (int)idx : -1;
+      const int kprev = __shfl_up(key, 1);
+      const uint64_t heads = __ballot(lane == 0 || key != kprev);
+      const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+      const int start = 63 - __clzll(heads & upto);  // first lane of this lane's run
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const long long h2 = __shfl_up(hi, off);
+        long long l2 = 0;
+        if constexpr (NWORD == 2) l2 = __shfl_up(lo, off);
+        if (lane - off >= start) {
+          hi += h2;
+          if constexpr (NWORD == 2) lo += l2;
+        }
+      }
+      const int knext = __shfl_down(key, 1);
+      if (key >= 0 && (lane == 63 || knext != key)) hash_add<NWORD, XCD, kSlots>(hkey, hval, g, ncell, key, hi, lo);
+    } else {
+      if (idx >= 0) hash_add<NWORD, XCD, kSlots>(hkey, hval, g, ncell, idx, hi, lo);
+    }
   }
   if (bad) atomicOr(&scp->err, (int)kErrCell);
   __syncthreads();
@@ -643,6 +696,20 @@ __global__ void __launch_bounds__(256) k_scatter_rec(const double* __restrict__ 
 // dst[:, i] = rec[i] for the six rows (sequential)
 __global__ void __launch_bounds__(256) k_rec_to_rows(const PRec* __restrict__ rec, double* __restrict__ dst,
                                                      int64_t ldd, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double v[8];
+  load_rec(rec + i, v);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) stnt(v[k], dst + k * ldd + i);
+}
+
+// k_rec_to_rows unless the kick flagged the bunch as invalid (then the
+// caller's rows stay as they were)
+__global__ void __launch_bounds__(256) k_rec_to_rows_checked(const PRec* __restrict__ rec, double* __restrict__ dst,
+                                                             int64_t ldd, int64_t n,
+                                                             const KickScalars* __restrict__ scp) {
+  if (scp->err) return;
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   double v[8];

@@ -7,19 +7,59 @@ from ocelot_amd import autotune
 
 
 def test_search_finds_coordinate_minimum():
-    # cost: sort interval optimum at 7, per-XCD copies slower, overlap faster
+    # cost: sort interval optimum at 7, per-XCD copies slower, overlap faster,
+    # the wave pre-sum slower, the small deposit tile faster, records faster
     def cost(o):
-        return (o["sort_every"] - 7) ** 2 * 0.01 + 1.0 + 0.1 * o["deposit_xcd"] - 0.05 * o["overlap"]
+        return ((o["sort_every"] - 7) ** 2 * 0.01 + 1.0 + 0.1 * o["deposit_xcd"] - 0.05 * o["overlap"]
+                + 0.02 * o["deposit_agg"] - 0.03 * (o["deposit_tile"] == 1) - 0.04 * o["permute_rec"]
+                - 0.01 * (o["fft_plan"] == 2))
     calls = []
 
     def measure(o):
         calls.append(dict(o))
         return cost(o), None
-    best, timings, _ = autotune.search(autotune.DEFAULT_SPACE, measure, {"sort_every": 10, "deposit_xcd": 1,
-                                                                        "overlap": 0})
-    assert best == {"sort_every": 7, "deposit_xcd": 0, "overlap": 1}
-    assert set(timings) == set(autotune.DEFAULT_SPACE)
+    start = {"sort_every": 10, "deposit_xcd": 1, "deposit_agg": 1, "deposit_tile": 0, "permute_rec": 0, "overlap": 0,
+             "fft_plan": 0}
+    best, timings, results = autotune.search(autotune.DEFAULT_SPACE, measure, start)
+    assert best == {"sort_every": 7, "deposit_xcd": 0, "deposit_agg": 0, "deposit_tile": 1, "permute_rec": 1,
+                    "overlap": 1, "fft_plan": 2}
+    assert set(timings) == set(autotune.DEFAULT_SPACE) == set(results)
     assert len(calls) == 1 + sum(len(v) for v in autotune.DEFAULT_SPACE.values())
+
+
+def test_identity_check_compares_within_each_knobs_loop():
+    """A bit-identical knob is checked between the results of its own loop
+    (the other knobs held fixed), even when a later knob changes the best
+    setting; a knob whose values differ is reported."""
+    # results depend on sort_every (not bit-identical) and, wrongly, on overlap
+    def measure(o):
+        return 1.0 + 0.01 * (o["sort_every"] != 5), (o["sort_every"], o["overlap"] if o["overlap"] else 0)
+    space = {"overlap": (0, 1), "sort_every": (5, 10)}
+    _, _, results = autotune.search(space, measure, {"overlap": 0, "sort_every": 10})
+    with pytest.raises(RuntimeError, match="overlap"):
+        autotune.check_identical(results, ["overlap"])
+    good = lambda o: (1.0 + 0.01 * (o["sort_every"] != 5), (o["sort_every"],))  # noqa: E731
+    _, _, results = autotune.search(space, good, {"overlap": 0, "sort_every": 10})
+    autotune.check_identical(results, ["overlap"])  # the later sort_every loop does not matter
+
+
+def test_file_cache_round_trip(tmp_path, monkeypatch):
+    """Entries are keyed by architecture, mesh, precision and the power-of-two
+    bucket of the bunch size, written atomically, and read back."""
+    from ocelot_amd import _lib
+    path = str(tmp_path / "sub" / "autotune.json")
+    monkeypatch.setenv("OCSC_AUTOTUNE_CACHE", path)
+    assert autotune.cache_path() == path
+    k1 = autotune.file_key("gfx950", (127, 127, 127), _lib.OCSC_MIXED, 10_000_000)
+    k2 = autotune.file_key("gfx950", (127, 127, 127), _lib.OCSC_MIXED, 9_000_000)
+    assert k1 == k2 == "gfx950|127x127x127|mixed|n2^23"
+    assert autotune.file_key("gfx950", (63, 63, 63), _lib.OCSC_FP64, 2_000_000) == "gfx950|63x63x63|fp64|n2^21"
+    autotune.store_file(k1, {"options": {"sort_every": 7}, "ms_per_step": 0.8})
+    autotune.store_file("other", {"options": {}})
+    data = autotune.load_file()
+    assert data[k1]["options"] == {"sort_every": 7} and "other" in data
+    monkeypatch.setenv("OCSC_AUTOTUNE_CACHE", "")
+    assert autotune.cache_path() is None and autotune.load_file() == {}
 
 
 def test_search_keeps_start_when_nothing_is_faster():
@@ -41,6 +81,9 @@ def test_tune_tracking_on_gpu():
     d = DeviceParticleArray.from_host(p, dev)
     sc = SpaceCharge(nmesh_xyz=[31, 31, 63], precision="mixed")
     old = {k: _lib.get_option(dev, k) for k in autotune.DEFAULT_SPACE}
+    import os
+    import tempfile
+    os.environ["OCSC_AUTOTUNE_CACHE"] = os.path.join(tempfile.mkdtemp(), "autotune.json")
     try:
         res = autotune.tune_tracking(d, sc, dz, dz, steps=8)
         print("autotune:", res)
@@ -48,8 +91,12 @@ def test_tune_tracking_on_gpu():
         for k, v in res["options"].items():
             assert _lib.get_option(dev, k) == v
         again = autotune.tune_tracking(d, sc, dz, dz)
-        assert again["cached"] and again["options"] == res["options"]
+        assert again["cached"] and again["source"] == "memory" and again["options"] == res["options"]
+        autotune._cache.clear()
+        from_file = autotune.tune_tracking(d, sc, dz, dz)
+        assert from_file["source"] == "file" and from_file["options"] == res["options"]
     finally:
+        os.environ.pop("OCSC_AUTOTUNE_CACHE", None)
         for k, v in old.items():
             _lib.set_option(dev, k, v)
 

@@ -497,6 +497,99 @@ def test_per_xcd_deposit_is_bit_identical(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("opt,vals", [("deposit_agg", (0, 1)), ("permute_rec", (0, 1))])
+def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
+    """The deposit's wave pre-sum of equal cells (integer sums in another
+    grouping), with one grid and with per-XCD copies, and the record-staged
+    permutations of the tracking copy (same values moved another way) give the
+    same bunch bit for bit: one caller-order kick, then 12 tracking steps with
+    a re-sort every 5."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=400_000, seed=15)
+    dev = torch.cuda.current_device()
+    saved = {k: _lib.get_option(dev, k) for k in (opt, "sort_every", "deposit_xcd")}
+    outs = {}
+    try:
+        _lib.set_option(dev, "sort_every", 5)
+        for xcd in (0, 1) if opt == "deposit_agg" else (0,):
+            _lib.set_option(dev, "deposit_xcd", xcd)
+            for v in vals:
+                _lib.set_option(dev, opt, v)
+                d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+                sc = SpaceCharge(nmesh_xyz=[47, 47, 63], precision=prec)
+                sc.apply(d, dz)
+                track_drift(d, sc, dz, dz, 12, use_graph=True)
+                outs[(xcd, v)] = d.rparticles.cpu().numpy()
+    finally:
+        for k, v in saved.items():
+            _lib.set_option(dev, k, v)
+    ref = outs[(0, vals[0])]
+    for key, o in outs.items():
+        assert np.array_equal(o, ref), key
+
+
+@pytest.mark.parametrize("plan", [0, 1, 2])
+@pytest.mark.parametrize("nmesh", [(31, 31, 63), (127, 15, 15), (15, 15, 255)])
+def test_fft_plans_agree_with_fp64(plan, nmesh):
+    """The mixed form's transform plans (default, radix 8 throughout,
+    half-width tiles) change only FP32 round-off: each kick is within 1e-5
+    RMS of the FP64 kick and reproducible bit for bit."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, _, dz = make_bunch("flattop_10m_127", n=200_000, seed=18)
+    dev = torch.cuda.current_device()
+    d64 = DeviceParticleArray.from_host(parray(r, q, E), dev)
+    SpaceCharge(nmesh_xyz=list(nmesh)).apply(d64, dz)
+    ref = d64.rparticles.cpu().numpy()
+    old = _lib.get_option(dev, "fft_plan")
+    outs = []
+    try:
+        _lib.set_option(dev, "fft_plan", plan)
+        for _ in range(2):
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            SpaceCharge(nmesh_xyz=list(nmesh), precision="mixed").apply(d, dz)
+            outs.append(d.rparticles.cpu().numpy())
+    finally:
+        _lib.set_option(dev, "fft_plan", old)
+    kick = max(rel_rms(outs[0][i] - r[i], ref[i] - r[i]) for i in (1, 3, 5))
+    print(f"fft_plan {plan} mesh {nmesh}: mixed vs fp64 kick rms {kick:.3e}")
+    assert kick <= 1e-5
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_sorted_one_shot_apply_is_bit_identical(prec):
+    """A one-shot kick through a cell-ordered copy (frame and extent from the
+    caller's order, integer deposit, per-particle kick, copy written back)
+    equals the caller-order kick bit for bit; a bunch with a non-finite
+    coordinate raises ValueError and stays as it was."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("gauss_2m_63", n=300_000, seed=16)
+    dev = torch.cuda.current_device()
+    old = _lib.get_option(dev, "apply_sort")
+    outs = {}
+    try:
+        for mode in (0, 1):
+            _lib.set_option(dev, "apply_sort", mode)
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            sc = SpaceCharge(nmesh_xyz=[47, 47, 63], precision=prec)
+            sc.apply(d, dz)
+            sc.apply(d, dz)
+            outs[mode] = d.rparticles.cpu().numpy()
+        bad = r.copy()
+        bad[4, 123] = np.inf
+        d = DeviceParticleArray.from_host(parray(bad, q, E), dev)
+        with pytest.raises(ValueError):
+            SpaceCharge(nmesh_xyz=[47, 47, 63], precision=prec).apply(d, dz)
+        assert np.array_equal(d.rparticles.cpu().numpy(), bad)
+    finally:
+        _lib.set_option(dev, "apply_sort", old)
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_overlapped_kernel_spectrum_is_bit_identical(prec):
     """The Green's-function spectrum built on a second stream next to the
     deposit (overlap on, joined before the first FFT pass) gives the same
@@ -645,24 +738,52 @@ def _batch(n_bunches, seed, scale=1.0):
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_batched_equals_single_kicks(prec):
-    """Bunches of uneven size, mesh (31/47/63) and energy kicked together,
-    twice, equal the same bunches kicked one by one (graph and eager)."""
-    from ocelot_amd import DeviceParticleArray, SpaceCharge, apply_batched
+    """Bunches of uneven size, mesh (31/47/63) and energy kicked together
+    equal the same bunches kicked one by one (graph and eager): bit for bit
+    without cell-ordered copies (two rounds) and with them for one round;
+    with copies kept over three rounds the later rounds sum the frame in cell
+    order: the cell sizes move in the last bits, which the Green's function's
+    cancellation on the long rest-frame cells of the high-energy bunches
+    (gamma up to 1.6e3) turns into kick changes of ~1e-8 (the reference's own
+    FP64 kick is 9e-9 from the exact one there), so FP64 rows agree to
+    1e-12 and kicks to 1e-7 of their size; in the mixed form a last-bit move
+    of the grid can put a particle on the other side of a cell face, and the
+    FP32 fields carry it: kicks agree to 1e-5 RMS (the mixed form's field
+    tolerance)."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib, apply_batched
     dev = torch.cuda.current_device()
     bunches = _batch(12, seed=4, scale=0.1)
-    ref = []
-    for b in bunches:
-        d = DeviceParticleArray.from_host(parray(b["r"], b["q"], b["E"]), dev)
-        sc = SpaceCharge(nmesh_xyz=list(b["nmesh"]), precision=prec)
-        sc.apply(d, 0.1)
-        sc.apply(d, 0.1)
-        ref.append(d.rparticles.cpu().numpy())
-    for graph in (False, True):
-        ds = [DeviceParticleArray.from_host(parray(b["r"], b["q"], b["E"]), dev) for b in bunches]
-        apply_batched([(d, b["nmesh"], 0.1) for d, b in zip(ds, bunches)], precision=prec, nkicks=2,
-                      use_graph=graph)
-        for d, rr in zip(ds, ref):
-            assert np.array_equal(d.rparticles.cpu().numpy(), rr)
+    refs = {}
+    for nk in (1, 2, 3):
+        refs[nk] = []
+        for b in bunches:
+            d = DeviceParticleArray.from_host(parray(b["r"], b["q"], b["E"]), dev)
+            sc = SpaceCharge(nmesh_xyz=list(b["nmesh"]), precision=prec)
+            for _ in range(nk):
+                sc.apply(d, 0.1)
+            refs[nk].append(d.rparticles.cpu().numpy())
+    old = _lib.get_option(dev, "batch_sort")
+    try:
+        for bs, nk, exact in ((0, 2, True), (1, 1, True), (1, 3, False)):
+            _lib.set_option(dev, "batch_sort", bs)
+            for graph in (False, True):
+                ds = [DeviceParticleArray.from_host(parray(b["r"], b["q"], b["E"]), dev) for b in bunches]
+                apply_batched([(d, b["nmesh"], 0.1) for d, b in zip(ds, bunches)], precision=prec, nkicks=nk,
+                              use_graph=graph)
+                for d, rr, b in zip(ds, refs[nk], bunches):
+                    got = d.rparticles.cpu().numpy()
+                    if exact:
+                        assert np.array_equal(got, rr), (bs, nk, graph)
+                    else:
+                        row = max(rel_max(got[i], rr[i]) for i in range(6))
+                        kick = max(kick_err(got, b["r"], rr, i) for i in (1, 3, 5))
+                        if prec == "fp64":
+                            assert row <= 1e-12 and kick <= 1e-7, (graph, row, kick)
+                        else:
+                            krms = max(rel_rms(got[i] - b["r"][i], rr[i] - b["r"][i]) for i in (1, 3, 5))
+                            assert krms <= 1e-5, (graph, row, kick, krms)
+    finally:
+        _lib.set_option(dev, "batch_sort", old)
 
 
 def test_batched_64_bunches_vs_oracle():
