@@ -335,7 +335,10 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * ocsc_apply_batched.  "fft_plan": the mixed form's transform plan, 0
  * (default) radix 16 where every thread gets a butterfly (else 8), 1 radix 8
  * in every pass, 2 half as many columns per workgroup; the results differ in
- * FP32 round-off only. */
+ * FP32 round-off only.  "sort_cells": sort keys of the cell-ordered copies
+ * are bricks and the cells inside them (1, default) or bricks only (0: fewer
+ * radix passes, cells inside a brick stay in their previous order); both give
+ * the same particles up to the order of reduction sums. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 

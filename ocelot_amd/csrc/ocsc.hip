@@ -195,6 +195,7 @@ struct ocsc_ctx {
   int permute_rec = 1;  // tracking: permutations staged through 64-byte particle records (0: row by row)
   int apply_sort = 2;   // one-shot kicks on a cell-ordered copy: 0 never, 1 always, 2 FP64 from kApplySortMin particles
   int batch_sort = 1;   // batched calls keep a cell-ordered copy of each bunch across the kicks of a call
+  int sort_cells = 1;   // sort keys: bricks and the cells inside them (1), or bricks only (0)
   // cell-ordered copy of bunch i of a batched call: six rows, charges, permutation
   struct BunchCopy {
     Buf rs, qs, perm;
@@ -554,6 +555,10 @@ struct StepOpts {
   // deposit and kick a cell-ordered copy of the bunch (frame and extent still
   // from the caller's rows), null: in place
   const SortBufs* sort = nullptr;
+  // the kick writes records to rec_out[rec_perm[p]] instead of rows (the last
+  // kick of a tracking call on a cell-ordered copy)
+  const uint32_t* rec_perm = nullptr;
+  PRec* rec_out = nullptr;
 };
 
 // buffers of a kick on a cell-ordered copy: sort keys and values (ping-pong),
@@ -596,7 +601,11 @@ template <typename F, bool DRIFT, bool SUMP>
 void launch_kick(ocsc_ctx* c, Lane& L, int npart, double* r, int64_t ld, const StepOpts& o, hipStream_t st) {
   // mixed form: the three FP32 field grids; FP64 form: the k_diff64 fields
   const F* field = sizeof(F) == 8 ? (const F*)L.E64.p : (const F*)L.phi.p;
-  k_kick<F, DRIFT, SUMP><<<npart, kKickPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p);
+  if (!DRIFT && !SUMP && o.rec_out)
+    k_kick<F, false, false, true><<<npart, kKickPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p, o.rec_perm,
+                                                             o.rec_out);
+  else
+    k_kick<F, DRIFT, SUMP><<<npart, kKickPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p);
 }
 
 template <typename F, int NWORD>
@@ -672,9 +681,8 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   const SortBufs* sb = o.sort;
   if (sb) {
     const int nb = (int)((n + 255) / 256);
-    int end_bit = 1;
-    while ((int64_t(1) << end_bit) <= brick_key_count(g.d.nx, g.d.ny, g.d.nz)) ++end_bit;
-    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(R, q, L.sc, sb->k[0], sb->rec);
+    const int end_bit = sort_end_bit(g.d.nx, g.d.ny, g.d.nz, c->sort_cells);
+    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(R, q, L.sc, sb->k[0], sb->rec, c->sort_cells);
     order = sb->v[rs_sort(sb->k, sb->v, n, end_bit, sb->hist, st)];
     k_gather_rec<false><<<nb, 256, 0, st>>>(sb->rec, order, sb->rs, n, sb->qs, n, DriftArgs{0, 0});
     if (sb->order_out) HIPCHK(hipMemcpyAsync(sb->order_out, order, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
@@ -703,20 +711,26 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   if (o.kev0 && !o.kev_external) HIPCHK(hipEventRecord(o.kev0, st));
   double* rk = sb ? sb->rs : r;
   const int64_t ldk = sb ? n : ld;
+  StepOpts ok = o;
+  if (sb && sb->scatter_back && !o.drift && !o.sump) {
+    // the kick writes each record straight to its caller position
+    ok.rec_perm = order;
+    ok.rec_out = sb->rec;
+  }
   if (o.drift && o.sump)
-    launch_kick<F, true, true>(c, L, npart, rk, ldk, o, st);
+    launch_kick<F, true, true>(c, L, npart, rk, ldk, ok, st);
   else if (o.drift)
-    launch_kick<F, true, false>(c, L, npart, rk, ldk, o, st);
+    launch_kick<F, true, false>(c, L, npart, rk, ldk, ok, st);
   else if (o.sump)
-    launch_kick<F, false, true>(c, L, npart, rk, ldk, o, st);
+    launch_kick<F, false, true>(c, L, npart, rk, ldk, ok, st);
   else
-    launch_kick<F, false, false>(c, L, npart, rk, ldk, o, st);
+    launch_kick<F, false, false>(c, L, npart, rk, ldk, ok, st);
   LAUNCHCHK("k_kick");
   if (sb && sb->scatter_back) {
     // back to the caller's order through whole records; a bunch the kick
     // rejected (error flag) was not written, so nothing is copied back
     const int nb = (int)((n + 255) / 256);
-    k_scatter_rec<<<nb, 256, 0, st>>>(rk, n, order, sb->rec, n);
+    if (!ok.rec_out) k_scatter_rec<<<nb, 256, 0, st>>>(rk, n, order, sb->rec, n);
     k_rec_to_rows_checked<<<nb, 256, 0, st>>>(sb->rec, r, ld, n, L.sc);
     LAUNCHCHK("caller-order copy");
   }
@@ -1569,8 +1583,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
 
   // cell-ordered resident copy (sort_every > 0)
   const bool sorted = c->sort_every > 0 && zstep != 0;
-  int end_bit = 1;  // keys 0..nkeys (nkeys: outside the mesh)
-  while ((int64_t(1) << end_bit) <= brick_key_count(g.d.nx, g.d.ny, g.d.nz)) ++end_bit;
+  const int end_bit = sort_end_bit(g.d.nx, g.d.ny, g.d.nz, c->sort_cells);  // keys 0..nkeys (nkeys: outside)
   if (sorted) {
     for (int k = 0; k < 2; ++k) {
       if ((rc = ensure(c->rs[k], (size_t)6 * n * 8))) return rc;
@@ -1591,7 +1604,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     uint32_t* kb[2] = {(uint32_t*)c->keys[0].p, (uint32_t*)c->keys[1].p};
     uint32_t* vb[2] = {(uint32_t*)c->vals.p, (uint32_t*)c->vals.p + n};
     PRec* rec = via_rec ? (PRec*)c->rec.p : nullptr;
-    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, qsrc, L.sc, kb[0], rec);
+    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, qsrc, L.sc, kb[0], rec, c->sort_cells);
     LAUNCHCHK("k_sort_keys");
     const uint32_t* order = vb[rs_sort(kb, vb, n, end_bit, (uint32_t*)c->sort_tmp.p, st)];  // order of src
     LAUNCHCHK("cell sort");
@@ -1711,7 +1724,14 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
       rr = (double*)c->rs[cur].p;
       qq = (const double*)c->qs[cur].p;
     }
-    const StepOpts o = opts_for(step);
+    StepOpts o = opts_for(step);
+    // the last kick writes its records straight to the caller's positions
+    // (no moments or LSC kick read the rows after it)
+    const bool rec_last = via_rec && !tw && !lq && step == nsteps - 1;
+    if (rec_last) {
+      o.rec_perm = (const uint32_t*)c->perm[cur].p;
+      o.rec_out = (PRec*)c->rec.p;
+    }
     const bool middle = step > 0 && step < nsteps - 1;
     if (!use_graph || c->profiling || !middle) {
       if ((rc = step_body(step, o, st))) {
@@ -1780,8 +1800,14 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     }
   }
   if (via_rec) {
-    // back to the caller's order: whole records to their places, then rows
-    k_scatter_rec<<<nb, 256, 0, st>>>(rr, n, (const uint32_t*)c->perm[cur].p, (PRec*)c->rec.p, n);
+    // back to the caller's order: whole records to their places (done by the
+    // last kick unless it was skipped for an invalid bunch or other work read
+    // the rows after it), then rows
+    const bool rec_last = !tw && !lq && nsteps > 0;
+    if (rec_last)
+      k_scatter_rec_if_err<<<nb, 256, 0, st>>>(rr, n, (const uint32_t*)c->perm[cur].p, (PRec*)c->rec.p, n, L.sc);
+    else
+      k_scatter_rec<<<nb, 256, 0, st>>>(rr, n, (const uint32_t*)c->perm[cur].p, (PRec*)c->rec.p, n);
     k_rec_to_rows<<<nb, 256, 0, st>>>((const PRec*)c->rec.p, r, ld, n);
     LAUNCHCHK("caller-order scatter");
   } else if (sorted) {
@@ -2161,6 +2187,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->fft_plan = (int)value;
     return OCSC_OK;
   }
+  if (n == "sort_cells") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "sort_cells must be 0 or 1");
+    c->sort_cells = (int)value;
+    return OCSC_OK;
+  }
   return fail(OCSC_EINVAL, "unknown option " + n);
 }
 
@@ -2205,6 +2236,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "fft_plan") {
     *value = c->fft_plan;
+    return OCSC_OK;
+  }
+  if (n == "sort_cells") {
+    *value = c->sort_cells;
     return OCSC_OK;
   }
   return fail(OCSC_EINVAL, "unknown option " + n);

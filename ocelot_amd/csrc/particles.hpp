@@ -620,16 +620,25 @@ __device__ __forceinline__ void load_rec(const PRec* __restrict__ src, double* v
 // transforms are used; out-of-mesh particles sort last.
 constexpr int kBx = 8, kBy = 8, kBz = 4;
 
-__host__ __device__ inline int64_t brick_key_count(int nx, int ny, int nz) {
+// number of keys (the out-of-mesh key): bricks times cells per brick, or
+// bricks only (cells == 0: the order inside a brick is left as it was)
+__host__ __device__ inline int64_t brick_key_count(int nx, int ny, int nz, int cells = 1) {
   const int64_t nbx = (nx + kBx - 1) / kBx, nby = (ny + kBy - 1) / kBy, nbz = (nz + kBz - 1) / kBz;
-  return nbx * nby * nbz * (kBx * kBy * kBz);
+  return nbx * nby * nbz * (cells ? kBx * kBy * kBz : 1);
+}
+
+// key bits the radix sort must look at
+__host__ __device__ inline int sort_end_bit(int nx, int ny, int nz, int cells) {
+  int b = 1;
+  while ((int64_t(1) << b) <= brick_key_count(nx, ny, nz, cells)) ++b;
+  return b;
 }
 
 // rec (may be null): also write each particle's record (rows and charge q),
 // the sequential half of the permutation into cell order
 __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restrict__ q,
                                                    const KickScalars* __restrict__ scp, uint32_t* __restrict__ keys,
-                                                   PRec* __restrict__ rec) {
+                                                   PRec* __restrict__ rec, int cells) {
   const KickScalars s = *scp;
   const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
   if (p >= s.n) return;
@@ -648,12 +657,13 @@ __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restr
   grid_coords_fast(xyz, s, X);
   const int nx = s.nmesh[0], ny = s.nmesh[1], nz = s.nmesh[2];
   const double g0 = __builtin_floor(X[0]) + 1, g1 = __builtin_floor(X[1]) + 1, g2 = __builtin_floor(X[2]) + 1;
-  uint32_t key = (uint32_t)brick_key_count(nx, ny, nz);
+  uint32_t key = (uint32_t)brick_key_count(nx, ny, nz, cells);
   if (g0 >= 0 && g0 < nx && g1 >= 0 && g1 < ny && g2 >= 0 && g2 < nz) {
     const int f0 = (int)g0, f1 = (int)g1, f2 = (int)g2;
     const int nby = (ny + kBy - 1) / kBy, nbz = (nz + kBz - 1) / kBz;
     const int64_t brick = ((int64_t)(f0 / kBx) * nby + f1 / kBy) * nbz + f2 / kBz;
-    key = (uint32_t)(brick * (kBx * kBy * kBz) + ((f0 % kBx) * kBy + f1 % kBy) * kBz + f2 % kBz);
+    key = cells ? (uint32_t)(brick * (kBx * kBy * kBz) + ((f0 % kBx) * kBy + f1 % kBy) * kBz + f2 % kBz)
+                : (uint32_t)brick;
   }
   keys[p] = key;
 }
@@ -702,6 +712,22 @@ __global__ void __launch_bounds__(256) k_rec_to_rows(const PRec* __restrict__ re
   load_rec(rec + i, v);
 #pragma unroll
   for (int k = 0; k < 6; ++k) stnt(v[k], dst + k * ldd + i);
+}
+
+// k_scatter_rec only if the kick flagged the bunch as invalid (that kick
+// then wrote no records; the copy goes back as it stands)
+__global__ void __launch_bounds__(256) k_scatter_rec_if_err(const double* __restrict__ src, int64_t lds_,
+                                                            const uint32_t* __restrict__ perm, PRec* __restrict__ rec,
+                                                            int64_t n, const KickScalars* __restrict__ scp) {
+  if (!scp->err) return;
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = src[k * lds_ + j];
+  v[6] = 0.0;
+  v[7] = 0.0;
+  store_rec(rec + perm[j], v);
 }
 
 // k_rec_to_rows unless the kick flagged the bunch as invalid (then the
@@ -1017,10 +1043,15 @@ __device__ __forceinline__ void gather_field64(const double* __restrict__ E3, co
 // partial per workgroup); 128 and 512 measured no faster
 constexpr int kKickPT = 256;
 
-template <typename F, bool DRIFT = false, bool SUMP = false>
+// RECOUT: the rows are written as records to rec[perm[p]] (the caller's
+// position of the cell-ordered particle p) instead of in place: the last kick
+// of a tracking call and the way back to the caller's order in one pass.
+template <typename F, bool DRIFT = false, bool SUMP = false, bool RECOUT = false>
 __global__ void __launch_bounds__(kKickPT) k_kick(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
                                                   const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
-                                                  double* __restrict__ part = nullptr) {
+                                                  double* __restrict__ part = nullptr,
+                                                  const uint32_t* __restrict__ perm = nullptr,
+                                                  PRec* __restrict__ rec = nullptr) {
   __shared__ double lds[SUMP ? (kKickPT / 64) * 3 : 1];
   const KickScalars& s0 = *scp;
   if (s0.err) return;  // uniform across the grid
@@ -1033,7 +1064,7 @@ __global__ void __launch_bounds__(kKickPT) k_kick(double* __restrict__ r, int64_
   // FP64 step 1.035 -> 0.952 ms at 10M / 127^3); the FP64 form keeps the exact
   // grid coordinates, the FP64 field and the exact pass-A momentum sums
   constexpr bool FAST = sizeof(F) == 4;
-  double xp[6], xyz[3], X[3], pr[3], E[3], pb[3], o[6];
+  double xp[6], xyz[3], X[3], pr[3], E[3], pb[3], o[8];
   double m[3] = {0, 0, 0};
   const int nn[3] = {s.nmesh[0], s.nmesh[1], s.nmesh[2]};
   if (active) {
@@ -1065,8 +1096,14 @@ __global__ void __launch_bounds__(kKickPT) k_kick(double* __restrict__ r, int64_
       o[2] = __builtin_fma(dr.L, o[3], o[2]);
       o[4] = __builtin_fma(dr.r56, o[5], o[4]);
     }
+    if constexpr (RECOUT) {
+      o[6] = 0.0;
+      o[7] = 0.0;
+      store_rec(rec + perm[p], o);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) stnt(o[i], r + i * ld + p);
+      for (int i = 0; i < 6; ++i) stnt(o[i], r + i * ld + p);
+    }
     if constexpr (SUMP) {
       // next kick's pass A: the momentum of the written rows (a drift keeps
       // x', y', delta); the mixed form takes the lab momentum it just wrote
