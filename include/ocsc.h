@@ -338,7 +338,11 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * FP32 round-off only.  "sort_cells": sort keys of the cell-ordered copies
  * are bricks and the cells inside them (1, default) or bricks only (0: fewer
  * radix passes, cells inside a brick stay in their previous order); both give
- * the same particles up to the order of reduction sums. */
+ * the same particles up to the order of reduction sums.  "sort_ahead": 1
+ * (default) takes the tracking sort keys at each particle's drifted position
+ * half-way through the steps until the next re-sort (the copy then stays
+ * closer to cell order on average), 0 at its present position; same
+ * particles up to the order of reduction sums. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 

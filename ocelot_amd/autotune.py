@@ -9,6 +9,8 @@ Knobs (``ocsc_set_option``):
 * ``sort_every``: re-sort interval of the cell-ordered copy (a sort costs a
   radix sort and a permutation of the bunch, and buys deposit/gather
   locality that decays as particles drift);
+* ``sort_ahead``: sort keys at the drifted position half-way through the
+  interval, or at the present one;
 * ``deposit_xcd``: one charge grid, or one per XCD with L2-local atomics;
 * ``deposit_agg``: runs of equal cells summed across the wavefront before
   the deposit's LDS table;
@@ -24,8 +26,8 @@ Knobs (``ocsc_set_option``):
 
 The search is coordinate-wise: each knob in turn over its candidates with the
 others fixed, timing ``steps`` graph-captured steps on a copy of the bunch.
-Every knob but ``sort_every`` and ``fft_plan`` (which change reduction
-orders and FP32 round-off) must leave the particles bit-identical; this is
+Every knob but ``sort_every``, ``sort_ahead`` and ``fft_plan`` (which change
+reduction orders and FP32 round-off) must leave the particles bit-identical; this is
 checked between the values of each knob's own loop (the other knobs held as
 they were during that loop).  Choices are kept in memory per (n, nmesh,
 precision, device) and in a JSON file keyed by GPU architecture, mesh,
@@ -46,8 +48,9 @@ import time
 
 from . import _lib
 
-DEFAULT_SPACE = {"sort_every": (5, 7, 10, 14, 20), "deposit_xcd": (0, 1), "deposit_agg": (0, 1),
-                 "deposit_tile": (0, 1, 2), "permute_rec": (0, 1), "overlap": (0, 1), "fft_plan": (0, 1, 2)}
+DEFAULT_SPACE = {"sort_every": (5, 7, 10, 14, 20), "sort_ahead": (0, 1), "deposit_xcd": (0, 1),
+                 "deposit_agg": (0, 1), "deposit_tile": (0, 1, 2), "permute_rec": (0, 1), "overlap": (0, 1),
+                 "fft_plan": (0, 1, 2)}
 BIT_IDENTICAL = ("deposit_xcd", "deposit_agg", "deposit_tile", "permute_rec", "overlap")
 
 _cache = {}

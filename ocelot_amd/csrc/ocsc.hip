@@ -196,6 +196,7 @@ struct ocsc_ctx {
   int apply_sort = 2;   // one-shot kicks on a cell-ordered copy: 0 never, 1 always, 2 FP64 from kApplySortMin particles
   int batch_sort = 1;   // batched calls keep a cell-ordered copy of each bunch across the kicks of a call
   int sort_cells = 1;   // sort keys: bricks and the cells inside them (1), or bricks only (0)
+  int sort_ahead = 1;   // tracking sort keys at the position half-way through the re-sort interval
   // cell-ordered copy of bunch i of a batched call: six rows, charges, permutation
   struct BunchCopy {
     Buf rs, qs, perm;
@@ -682,7 +683,8 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   if (sb) {
     const int nb = (int)((n + 255) / 256);
     const int end_bit = sort_end_bit(g.d.nx, g.d.ny, g.d.nz, c->sort_cells);
-    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(R, q, L.sc, sb->k[0], sb->rec, c->sort_cells);
+    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(R, q, L.sc, sb->k[0], sb->rec, c->sort_cells,
+                                                            DriftArgs{0, 0});
     order = sb->v[rs_sort(sb->k, sb->v, n, end_bit, sb->hist, st)];
     k_gather_rec<false><<<nb, 256, 0, st>>>(sb->rec, order, sb->rs, n, sb->qs, n, DriftArgs{0, 0});
     if (sb->order_out) HIPCHK(hipMemcpyAsync(sb->order_out, order, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
@@ -1599,12 +1601,20 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   // sort src (row stride lds) by NGP cell under the current kick scalars into
   // the copy `to`; perm[to] = order relative to the caller's particles.
   // drift: apply the first step's drift to the sorted copy (initial sort).
+  // keys at the drifted position half-way through the steps this sort serves
+  // (option sort_ahead)
+  auto ahead_for = [&](int from_step) {
+    const int left = std::min(c->sort_every, nsteps - from_step);
+    const double h = c->sort_ahead ? 0.5 * std::max(left - 1, 0) + (from_step == 0 ? 1.0 : 0.0) : 0.0;
+    return DriftArgs{Ld * h, r56 * h};
+  };
   auto resort = [&](const double* src, int64_t lds, const double* qsrc, int to, const uint32_t* perm_prev,
-                    bool drift) -> int {
+                    bool drift, DriftArgs ahead) -> int {
     uint32_t* kb[2] = {(uint32_t*)c->keys[0].p, (uint32_t*)c->keys[1].p};
     uint32_t* vb[2] = {(uint32_t*)c->vals.p, (uint32_t*)c->vals.p + n};
     PRec* rec = via_rec ? (PRec*)c->rec.p : nullptr;
-    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, qsrc, L.sc, kb[0], rec, c->sort_cells);
+    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, qsrc, L.sc, kb[0], rec, c->sort_cells,
+                                                            ahead);
     LAUNCHCHK("k_sort_keys");
     const uint32_t* order = vb[rs_sort(kb, vb, n, end_bit, (uint32_t*)c->sort_tmp.p, st)];  // order of src
     LAUNCHCHK("cell sort");
@@ -1639,7 +1649,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     k_reduceB<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk, prec);
     LAUNCHCHK("initial frame");
     // the first step's drift rides on the gather (sorted implies zstep != 0)
-    if ((rc = resort(r, ld, q, 0, nullptr, nsteps > 0))) return rc;
+    if ((rc = resort(r, ld, q, 0, nullptr, nsteps > 0, ahead_for(0)))) return rc;
     rr = (double*)c->rs[0].p;
     lr = n;
     qq = (const double*)c->qs[0].p;
@@ -1716,7 +1726,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   };
   for (int step = 0; step < nsteps && zstep != 0; ++step) {
     if (sorted && step > 0 && step % c->sort_every == 0) {
-      if ((rc = resort(rr, n, qq, 1 - cur, (const uint32_t*)c->perm[cur].p, false))) {
+      if ((rc = resort(rr, n, qq, 1 - cur, (const uint32_t*)c->perm[cur].p, false, ahead_for(step)))) {
         cleanup();
         return rc;
       }
@@ -2192,6 +2202,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->sort_cells = (int)value;
     return OCSC_OK;
   }
+  if (n == "sort_ahead") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "sort_ahead must be 0 or 1");
+    c->sort_ahead = (int)value;
+    return OCSC_OK;
+  }
   return fail(OCSC_EINVAL, "unknown option " + n);
 }
 
@@ -2240,6 +2255,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "sort_cells") {
     *value = c->sort_cells;
+    return OCSC_OK;
+  }
+  if (n == "sort_ahead") {
+    *value = c->sort_ahead;
     return OCSC_OK;
   }
   return fail(OCSC_EINVAL, "unknown option " + n);

@@ -635,10 +635,13 @@ __host__ __device__ inline int sort_end_bit(int nx, int ny, int nz, int cells) {
 }
 
 // rec (may be null): also write each particle's record (rows and charge q),
-// the sequential half of the permutation into cell order
+// the sequential half of the permutation into cell order.  ahead: the keys
+// are taken at the position after a drift of that length (the middle of the
+// coming re-sort interval, so particles stay as close to their sorted place
+// before it as after it); the rows are not changed.
 __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restrict__ q,
                                                    const KickScalars* __restrict__ scp, uint32_t* __restrict__ keys,
-                                                   PRec* __restrict__ rec, int cells) {
+                                                   PRec* __restrict__ rec, int cells, DriftArgs ahead) {
   const KickScalars s = *scp;
   const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
   if (p >= s.n) return;
@@ -651,7 +654,9 @@ __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restr
     store_rec(rec + p, v);
   }
   double xp[6], xyz[3], X[3];
-  to_xp_fast(v[0], v[1], v[2], v[3], v[4], v[5], s, xp);
+  const double x = __builtin_fma(ahead.L, v[1], v[0]), y = __builtin_fma(ahead.L, v[3], v[2]),
+               t = __builtin_fma(ahead.r56, v[5], v[4]);
+  to_xp_fast(x, v[1], y, v[3], t, v[5], s, xp);
   rot(xp, s.T, xyz);
   xyz[2] = xyz[2] * s.gamma0;
   grid_coords_fast(xyz, s, X);

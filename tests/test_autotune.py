@@ -12,17 +12,17 @@ def test_search_finds_coordinate_minimum():
     def cost(o):
         return ((o["sort_every"] - 7) ** 2 * 0.01 + 1.0 + 0.1 * o["deposit_xcd"] - 0.05 * o["overlap"]
                 + 0.02 * o["deposit_agg"] - 0.03 * (o["deposit_tile"] == 1) - 0.04 * o["permute_rec"]
-                - 0.01 * (o["fft_plan"] == 2))
+                - 0.01 * (o["fft_plan"] == 2) - 0.05 * o["sort_ahead"])
     calls = []
 
     def measure(o):
         calls.append(dict(o))
         return cost(o), None
-    start = {"sort_every": 10, "deposit_xcd": 1, "deposit_agg": 1, "deposit_tile": 0, "permute_rec": 0, "overlap": 0,
-             "fft_plan": 0}
+    start = {"sort_every": 10, "sort_ahead": 0, "deposit_xcd": 1, "deposit_agg": 1, "deposit_tile": 0,
+             "permute_rec": 0, "overlap": 0, "fft_plan": 0}
     best, timings, results = autotune.search(autotune.DEFAULT_SPACE, measure, start)
-    assert best == {"sort_every": 7, "deposit_xcd": 0, "deposit_agg": 0, "deposit_tile": 1, "permute_rec": 1,
-                    "overlap": 1, "fft_plan": 2}
+    assert best == {"sort_every": 7, "sort_ahead": 1, "deposit_xcd": 0, "deposit_agg": 0, "deposit_tile": 1,
+                    "permute_rec": 1, "overlap": 1, "fft_plan": 2}
     assert set(timings) == set(autotune.DEFAULT_SPACE) == set(results)
     assert len(calls) == 1 + sum(len(v) for v in autotune.DEFAULT_SPACE.values())
 
