@@ -479,22 +479,23 @@ __global__ void __launch_bounds__(256) k_igf_nodes(double* __restrict__ Fn, Mesh
                                                    const KickScalars* __restrict__ sc, int force) {
   if (sc->err || !(force || sc->rebuild)) return;
   const double hx = sc->steps[0], hy = sc->steps[1], hz = sc->steps[2];
-  const int sz = d.nz + 1, sy = d.ny + 1;
-  const int64_t nn = (int64_t)(d.nx + 1) * sy * sz;
-  for (int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x; f < nn; f += (int64_t)gridDim.x * 256) {
-    const int k = (int)(f % sz), j = (int)((f / sz) % sy), i = (int)(f / ((int64_t)sz * sy));
-    Fn[f] = igf_F(hx * i - hx / 2, hy * j - hy / 2, hz * k - hz / 2);
+  // 32-bit index arithmetic: meshes have at most 513^3 nodes
+  const unsigned sz = d.nz + 1, sy = d.ny + 1;
+  const unsigned nn = (unsigned)(d.nx + 1) * sy * sz;
+  for (unsigned f = blockIdx.x * 256u + threadIdx.x; f < nn; f += gridDim.x * 256u) {
+    const unsigned row = f / sz, k = f - row * sz, i = row / sy, j = row - i * sy;
+    Fn[f] = igf_F(hx * (int)i - hx / 2, hy * (int)j - hy / 2, hz * (int)k - hz / 2);
   }
 }
 
 __global__ void __launch_bounds__(256) k_igf_diff(const double* __restrict__ Fn, double* __restrict__ K, MeshDims d,
                                                   const KickScalars* __restrict__ sc, int force) {
   if (sc->err || !(force || sc->rebuild)) return;
-  const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
-  const int64_t sy = d.nz + 1, sx = (int64_t)(d.ny + 1) * (d.nz + 1);
-  for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < nc; o += (int64_t)gridDim.x * 256) {
-    const int k = (int)(o % d.nz), j = (int)((o / d.nz) % d.ny), i = (int)(o / ((int64_t)d.nz * d.ny));
-    const double* f0 = Fn + i * sx + j * sy + k;
+  const unsigned nc = (unsigned)d.nx * d.ny * d.nz;
+  const unsigned sy = d.nz + 1, sx = (unsigned)(d.ny + 1) * (d.nz + 1);
+  for (unsigned o = blockIdx.x * 256u + threadIdx.x; o < nc; o += gridDim.x * 256u) {
+    const unsigned row = o / d.nz, k = o - row * d.nz, i = row / d.ny, j = row - i * d.ny;
+    const double* f0 = Fn + (i * sx + j * sy + k);
 #define FAT(p, q, r) f0[(p) * sx + (q) * sy + (r)]
     K[o] = FAT(1, 1, 1) - FAT(0, 1, 1) - FAT(1, 0, 1) + FAT(0, 0, 1) - FAT(1, 1, 0) + FAT(0, 1, 0) + FAT(1, 0, 0) -
            FAT(0, 0, 0);

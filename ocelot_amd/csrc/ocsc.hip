@@ -197,6 +197,7 @@ struct ocsc_ctx {
   int batch_sort = 1;   // batched calls keep a cell-ordered copy of each bunch across the kicks of a call
   int sort_cells = 1;   // sort keys: bricks and the cells inside them (1), or bricks only (0)
   int sort_ahead = 1;   // tracking sort keys at the position half-way through the re-sort interval
+  int resort_near = 0;  // re-sorts of the copy: one gather of rows, charges and permutation (1), or records
   // cell-ordered copy of bunch i of a batched call: six rows, charges, permutation
   struct BunchCopy {
     Buf rs, qs, perm;
@@ -1612,18 +1613,26 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
                     bool drift, DriftArgs ahead) -> int {
     uint32_t* kb[2] = {(uint32_t*)c->keys[0].p, (uint32_t*)c->keys[1].p};
     uint32_t* vb[2] = {(uint32_t*)c->vals.p, (uint32_t*)c->vals.p + n};
-    PRec* rec = via_rec ? (PRec*)c->rec.p : nullptr;
+    // records for the first sort (from the caller's order); a re-sort of the
+    // copy permutes nearby rows directly
+    const bool resort_near = perm_prev && c->resort_near;
+    PRec* rec = via_rec && !resort_near ? (PRec*)c->rec.p : nullptr;
     k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(Rows{src, lds}, qsrc, L.sc, kb[0], rec, c->sort_cells,
                                                             ahead);
     LAUNCHCHK("k_sort_keys");
     const uint32_t* order = vb[rs_sort(kb, vb, n, end_bit, (uint32_t*)c->sort_tmp.p, st)];  // order of src
     LAUNCHCHK("cell sort");
+    double* dst = (double*)c->rs[to].p;
+    double* qdst = (double*)c->qs[to].p;
+    if (resort_near) {
+      k_resort_gather<<<nb, 256, 0, st>>>(src, qsrc, order, perm_prev, dst, qdst, (uint32_t*)c->perm[to].p, n);
+      LAUNCHCHK("re-sort gather");
+      return OCSC_OK;
+    }
     if (perm_prev)
       k_compose<<<nb, 256, 0, st>>>(perm_prev, order, (uint32_t*)c->perm[to].p, n);
     else
       HIPCHK(hipMemcpyAsync(c->perm[to].p, order, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
-    double* dst = (double*)c->rs[to].p;
-    double* qdst = (double*)c->qs[to].p;
     if (via_rec && drift)
       k_gather_rec<true><<<nb, 256, 0, st>>>(rec, order, dst, n, qdst, n, DriftArgs{Ld, r56});
     else if (via_rec)
@@ -2207,6 +2216,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->sort_ahead = (int)value;
     return OCSC_OK;
   }
+  if (n == "resort_near") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "resort_near must be 0 or 1");
+    c->resort_near = (int)value;
+    return OCSC_OK;
+  }
   return fail(OCSC_EINVAL, "unknown option " + n);
 }
 
@@ -2259,6 +2273,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "sort_ahead") {
     *value = c->sort_ahead;
+    return OCSC_OK;
+  }
+  if (n == "resort_near") {
+    *value = c->resort_near;
     return OCSC_OK;
   }
   return fail(OCSC_EINVAL, "unknown option " + n);

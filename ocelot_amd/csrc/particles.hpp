@@ -749,6 +749,30 @@ __global__ void __launch_bounds__(256) k_rec_to_rows_checked(const PRec* __restr
   for (int k = 0; k < 6; ++k) stnt(v[k], dst + k * ldd + i);
 }
 
+// Re-sort of an already cell-ordered copy: dst[:, j] = src[:, order[j]] for
+// the six rows and the charge, and perm_dst[j] = perm_src[order[j]] (the
+// caller's index of the particle now at j), in one pass.  The particles moved
+// a few cells since the last sort, so order[j] stays near j and the reads of
+// a workgroup fall in a small window of each row.
+__global__ void __launch_bounds__(256) k_resort_gather(const double* __restrict__ src, const double* __restrict__ qs,
+                                                       const uint32_t* __restrict__ order,
+                                                       const uint32_t* __restrict__ perm_src, double* __restrict__ dst,
+                                                       double* __restrict__ qd, uint32_t* __restrict__ perm_dst,
+                                                       int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int64_t i = order[j];
+  double v[7];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = src[k * n + i];
+  v[6] = qs[i];
+  const uint32_t pi = perm_src[i];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) stnt(v[k], dst + k * n + j);
+  stnt(v[6], qd + j);
+  perm_dst[j] = pi;
+}
+
 // dst[:, j] = src[:, perm[j]] for the six rows and the charge
 // One row per blockIdx.y (rows 0-5, then q as row 6): workgroups are
 // dispatched in x-major order, so a pass's random reads (gather) or writes

@@ -497,13 +497,14 @@ def test_per_xcd_deposit_is_bit_identical(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-@pytest.mark.parametrize("opt,vals", [("deposit_agg", (0, 1)), ("permute_rec", (0, 1))])
+@pytest.mark.parametrize("opt,vals", [("deposit_agg", (0, 1)), ("permute_rec", (0, 1)), ("resort_near", (0, 1)),
+                                      ("deposit_tile", (0, 1, 2))])
 def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
-    """The deposit's wave pre-sum of equal cells (integer sums in another
-    grouping), with one grid and with per-XCD copies, and the record-staged
-    permutations of the tracking copy (same values moved another way) give the
-    same bunch bit for bit: one caller-order kick, then 12 tracking steps with
-    a re-sort every 5."""
+    """The deposit's wave pre-sum of equal cells and its tile sizes (integer
+    sums in another grouping), with one grid and with per-XCD copies, and the
+    record-staged permutations and one-pass re-sorts of the tracking copy
+    (same values moved another way) give the same bunch bit for bit: one
+    caller-order kick, then 12 tracking steps with a re-sort every 5."""
     from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib, track_drift
     from ocelot_amd.workloads import make_bunch
     r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=400_000, seed=15)
