@@ -60,7 +60,43 @@ def parse():
     ap.add_argument("--lsc-only", action="store_true",
                     help="only time tracking with an LSC kick after every space-charge kick (--warmup, then "
                          "--steps graph-captured steps) and two one-shot LSC kicks; one JSON line")
+    ap.add_argument("--apply-only", choices=["fp64", "mixed"], default=None,
+                    help="only time the drop-in SpaceCharge.apply on the resident bunch of --workload in its "
+                         "caller's order (--warmup, then --steps kicks, then per-stage events over --steps more); "
+                         "one JSON line")
     return ap.parse_args()
+
+
+def apply_only(a, torch, host, nmesh, dz, dev):
+    """Context run for profiling the per-kick drop-in path that Ocelot's
+    track() drives (one SpaceCharge.apply per step, ref track.py:475-477)."""
+    import ctypes
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
+    lib, ctx = _lib.load(), _lib.context(dev)
+    d = DeviceParticleArray.from_host(host, dev)
+    sc = SpaceCharge(nmesh_xyz=list(nmesh), precision=a.apply_only)
+    for _ in range(a.warmup):
+        sc.apply(d, dz)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sc.apply(d, dz)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(a.steps, 1)
+    _lib.check(lib.ocsc_set_profiling(ctx.handle, 1))
+    for _ in range(a.steps):
+        sc.apply(d, dz)
+    ms = (ctypes.c_double * 16)()
+    calls = ctypes.c_int64()
+    ns = lib.ocsc_stage_times(ctx.handle, ms, 16, ctypes.byref(calls))
+    _lib.check(lib.ocsc_set_profiling(ctx.handle, 0))
+    n = d.n
+    opts = {k: _lib.get_option(dev, k) for k in ("apply_sort", "deposit_xcd", "deposit_tile", "overlap")}
+    print(json.dumps({"metric": "SpaceCharge.apply on a resident bunch (caller's order)", "workload": a.workload,
+                      "particles": n, "nmesh": list(nmesh), "precision": a.apply_only, "kicks": a.steps,
+                      "warmup": a.warmup, "ms_per_kick": round(dt * 1e3, 4), "value": round(n / dt, 1),
+                      "unit": "macroparticle-kicks/s", "options": opts,
+                      "stages_ms": {lib.ocsc_stage_name(i).decode(): round(ms[i], 4) for i in range(ns)}}))
 
 
 def lsc_only(a, torch, host, nmesh, dz, dev):
@@ -396,6 +432,8 @@ def main():
     L = dz
     if a.lsc_only:
         return lsc_only(a, torch, host, nmesh, dz, dev)
+    if a.apply_only:
+        return apply_only(a, torch, host, nmesh, dz, dev)
 
     # the library's autotuner picks the speed-only options (re-sort interval,
     # deposit mode, Green's-function overlap) for this bunch and mesh; options

@@ -331,7 +331,7 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * the FP64 form from 1000000 particles on; bit-identical results
  * (ocsc_apply_cells and ocsc_apply_noalloc never sort).  "deposit_tile": particles per thread and
  * LDS hash slots of the deposit, 0 (default) 8 and 2048 (FP64: 1024), 1 half
- * of both, 2 twice both; bit-identical results.  "batch_sort": see
+ * of both, 2 twice both, 3 no table (one global atomic per particle); bit-identical results.  "batch_sort": see
  * ocsc_apply_batched.  "fft_plan": the mixed form's transform plan, 0
  * (default) radix 16 where every thread gets a butterfly (else 8), 1 radix 8
  * in every pass, 2 half as many columns per workgroup; the results differ in
@@ -345,7 +345,14 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * particles up to the order of reduction sums.  "resort_near": 1 re-sorts
  * the copy with one gather of rows, charges and permutation (the copy is
  * nearly in order), 0 (default) through particle records as the first sort
- * (measured faster); bit-identical results. */
+ * (measured faster); bit-identical results.  "apply_graph": 1 (default)
+ * replays the launch sequence of ocsc_apply from a HIP graph captured at the
+ * first call and kept while the particle and charge pointers, sizes, mesh,
+ * precision, options and workspaces stay the same (energy, step and
+ * random-mesh draws are updated in place), 0 launches kernel by kernel;
+ * bit-identical results.  The graph holds the caller's pointers: a bunch
+ * freed and another allocated at the same address with the same sizes is
+ * kicked correctly, since only addresses and sizes enter the sequence. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 

@@ -93,6 +93,28 @@ __device__ __forceinline__ cpx<F> dfac(const cpx<double>* __restrict__ twd, int 
   return cpx<F>{(F)((1.0 - w.x) * inv_h), (F)(w.y * inv_h)};
 }
 
+// The y and x passes transform columns (a, b) of a [a][m][b] array along m,
+// T columns per workgroup.  Columns are numbered flat, c = a * nb + b, so
+// every workgroup but the last gets T of them: with b running over Hz = M/2+1
+// spectrum planes, tiles of b alone would leave one plane (b = M/2) to a tile
+// of its own in every row a.  A thread touches one column in every loop of
+// these kernels (NT is a multiple of T), so it splits c once.
+struct Col {
+  int t, a, b;
+  bool ok;
+};
+template <int T, int NT>
+__device__ __forceinline__ Col flat_col(int ncol, int nb) {
+  static_assert(NT % T == 0, "a thread must keep its column in every loop");
+  Col c;
+  c.t = threadIdx.x % T;
+  const int col = blockIdx.x * T + c.t;
+  c.ok = col < ncol;
+  c.a = c.ok ? col / nb : 0;
+  c.b = c.ok ? col - c.a * nb : 0;
+  return c;
+}
+
 // ---------------------------------------------------------------------------
 // z pass: pairs of real rows -> one complex FFT of length Mz -> two half spectra
 template <typename F, int M, int CIN, int V = 0>
@@ -149,21 +171,21 @@ __global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<
   if (sc->err) return;
   const cpx<F>* twl = stage_tw<F, M, kNT, TL::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
-  const int kz0 = blockIdx.x * T;
-  const int x = blockIdx.y;
+  const Col c = flat_col<T, kNT>(d.nx * Hz, Hz);
+  const cpx<F>* src = A + (int64_t)c.a * ny * Hz + c.b;
   for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int t = i % T, y = i / T;
-    const int kz = kz0 + t;
+    const int y = i / T;
     cpx<F> v{0, 0};
-    if (y < ny && kz < Hz) v = A[((int64_t)x * ny + y) * Hz + kz];
-    s[y * S + t] = v;
+    if (y < ny && c.ok) v = src[(int64_t)y * Hz];
+    s[y * S + c.t] = v;
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, -1, S, TL::RSEL>(s, twl);
+  if (!c.ok) return;
+  cpx<F>* dst = B + (int64_t)c.a * M * Hz + c.b;
   for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int t = i % T, ky = i / T;
-    const int kz = kz0 + t;
-    if (kz < Hz) B[((int64_t)x * M + ky) * Hz + kz] = s[ky * S + t];
+    const int ky = i / T;
+    dst[(int64_t)ky * Hz] = s[ky * S + c.t];
   }
 }
 
@@ -179,32 +201,30 @@ __global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* 
   if (sc->err) return;
   const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
-  const int kz0 = blockIdx.x * T;
-  const int ky = blockIdx.y;
   const int64_t cs = (int64_t)My * Hz;  // stride between x planes
+  const Col c = flat_col<T, kNT>(My * Hz, Hz);  // a = ky, b = kz
+  cpx<F>* col = B + (int64_t)c.a * Hz + c.b;
   for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int t = i % T, x = i / T;
-    const int kz = kz0 + t;
+    const int x = i / T;
     cpx<F> v{0, 0};
-    if (x < nx && kz < Hz) v = B[x * cs + (int64_t)ky * Hz + kz];
-    s[x * S + t] = v;
+    if (x < nx && c.ok) v = col[x * cs];
+    s[x * S + c.t] = v;
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, -1>(s, twl);
-  const int fy = ky <= My - ky ? ky : My - ky;
+  const int fy = c.a <= My - c.a ? c.a : My - c.a;
   for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int t = i % T, kx = i / T;
-    const int kz = kz0 + t;
+    const int kx = i / T;
     const int fx = kx <= M - kx ? kx : M - kx;
-    const F g = kz < Hz ? G[((int64_t)fx * Hy + fy) * Hz + kz] : (F)0;
-    s[kx * S + t] = cscale(s[kx * S + t], g);
+    const F g = c.ok ? G[((int64_t)fx * Hy + fy) * Hz + c.b] : (F)0;
+    s[kx * S + c.t] = cscale(s[kx * S + c.t], g);
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, +1>(s, twl);
+  if (!c.ok) return;
   for (int i = threadIdx.x; i < nx * T; i += kNT) {
-    const int t = i % T, x = i / T;
-    const int kz = kz0 + t;
-    if (kz < Hz) B[x * cs + (int64_t)ky * Hz + kz] = s[x * S + t];
+    const int x = i / T;
+    col[x * cs] = s[x * S + c.t];
   }
 }
 
@@ -219,21 +239,21 @@ __global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<
   if (sc->err) return;
   const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
-  const int kz0 = blockIdx.x * T;
-  const int x = blockIdx.y;
+  const Col c = flat_col<T, kNT>(d.nx * Hz, Hz);  // a = x, b = kz
+  const cpx<F>* src = B + (int64_t)c.a * M * Hz + c.b;
   for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int t = i % T, ky = i / T;
-    const int kz = kz0 + t;
+    const int ky = i / T;
     cpx<F> v{0, 0};
-    if (kz < Hz) v = B[((int64_t)x * M + ky) * Hz + kz];
-    s[ky * S + t] = v;
+    if (c.ok) v = src[(int64_t)ky * Hz];
+    s[ky * S + c.t] = v;
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, +1>(s, twl);
+  if (!c.ok) return;
+  cpx<F>* dst = A + (int64_t)c.a * ny * Hz + c.b;
   for (int i = threadIdx.x; i < ny * T; i += kNT) {
-    const int t = i % T, y = i / T;
-    const int kz = kz0 + t;
-    if (kz < Hz) A[((int64_t)x * ny + y) * Hz + kz] = s[y * S + t];
+    const int y = i / T;
+    dst[(int64_t)y * Hz] = s[y * S + c.t];
   }
 }
 
@@ -335,19 +355,18 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
   if (sc->err) return;
   const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF, V>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
-  const int kz0 = blockIdx.x * T;
-  const int ky = blockIdx.y;
   const int64_t cs = (int64_t)My * Hz;
+  const Col c = flat_col<T, kNTF>(My * Hz, Hz);  // a = ky, b = kz
+  const int64_t off = (int64_t)c.a * Hz + c.b;
   for (int i = threadIdx.x; i < M * T; i += kNTF) {
-    const int t = i % T, x = i / T;
-    const int kz = kz0 + t;
+    const int x = i / T;
     cpx<F> v{0, 0};
-    if (x < nx && kz < Hz) v = B[x * cs + (int64_t)ky * Hz + kz];
-    s[x * S + t] = v;
+    if (x < nx && c.ok) v = B[x * cs + off];
+    s[x * S + c.t] = v;
   }
   __syncthreads();
   lds_fft<F, M, T, kNTF, -1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
-  const int fy = ky <= My - ky ? ky : My - ky;
+  const int fy = c.a <= My - c.a ? c.a : My - c.a;
   const double inv_hx = 1.0 / sc->steps[0];
   cpx<F> v[PER];
 #pragma unroll
@@ -355,11 +374,10 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
     const int i = threadIdx.x + k * kNTF;
     v[k] = cpx<F>{0, 0};
     if (M * T % kNTF == 0 || i < M * T) {
-      const int t = i % T, kx = i / T;
-      const int kz = kz0 + t;
+      const int kx = i / T;
       const int fx = kx <= M - kx ? kx : M - kx;
-      const F g = kz < Hz ? G[((int64_t)fx * Hy + fy) * Hz + kz] : (F)0;
-      v[k] = cscale(s[kx * S + t], g);
+      const F g = c.ok ? G[((int64_t)fx * Hy + fy) * Hz + c.b] : (F)0;
+      v[k] = cscale(s[kx * S + c.t], g);
     }
   }
   __syncthreads();
@@ -367,30 +385,30 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
   for (int k = 0; k < PER; ++k) {
     const int i = threadIdx.x + k * kNTF;
     if (M * T % kNTF == 0 || i < M * T) {
-      const int t = i % T, kx = i / T;
-      s[kx * S + t] = cmul(v[k], dfac<F>(twd, kx, inv_hx));
+      const int kx = i / T;
+      s[kx * S + c.t] = cmul(v[k], dfac<F>(twd, kx, inv_hx));
     }
   }
   __syncthreads();
   lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
-  for (int i = threadIdx.x; i < nx * T; i += kNTF) {
-    const int t = i % T, x = i / T;
-    const int kz = kz0 + t;
-    if (kz < Hz) Qx[x * cs + (int64_t)ky * Hz + kz] = s[x * S + t];
-  }
+  if (c.ok)
+    for (int i = threadIdx.x; i < nx * T; i += kNTF) {
+      const int x = i / T;
+      Qx[x * cs + off] = s[x * S + c.t];
+    }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int i = threadIdx.x + k * kNTF;
-    if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + i % T] = v[k];
+    if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + c.t] = v[k];
   }
   __syncthreads();
   lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
-  for (int i = threadIdx.x; i < nx * T; i += kNTF) {
-    const int t = i % T, x = i / T;
-    const int kz = kz0 + t;
-    if (kz < Hz) P[x * cs + (int64_t)ky * Hz + kz] = s[x * S + t];
-  }
+  if (c.ok)
+    for (int i = threadIdx.x; i < nx * T; i += kNTF) {
+      const int x = i / T;
+      P[x * cs + off] = s[x * S + c.t];
+    }
 }
 
 // Mixed form, y inverse: Ex' = IFFT_y(Qx), Ey' = IFFT_y(P * Dy), Phi' = IFFT_y(P)
@@ -410,8 +428,8 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
   if (sc->err) return;
   const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF, V>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
-  const int kz0 = blockIdx.x * T;
-  const int x = blockIdx.y;
+  const Col c = flat_col<T, kNTF>(d.nx * Hz, Hz);  // a = x, b = kz
+  const int64_t src = (int64_t)c.a * M * Hz + c.b;
   const double inv_hy = 1.0 / sc->steps[1];
   cpx<F> p[PER];
 #pragma unroll
@@ -419,24 +437,24 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
     const int i = threadIdx.x + k * kNTF;
     p[k] = cpx<F>{0, 0};
     if (M * T % kNTF == 0 || i < M * T) {
-      const int t = i % T, ky = i / T;
-      const int kz = kz0 + t;
+      const int ky = i / T;
       cpx<F> q{0, 0};
-      if (kz < Hz) {
-        const int64_t o = ((int64_t)x * M + ky) * Hz + kz;
+      if (c.ok) {
+        const int64_t o = src + (int64_t)ky * Hz;
         q = Qx[o];
         p[k] = P[o];
       }
-      s[ky * S + t] = q;
+      s[ky * S + c.t] = q;
     }
   }
   __syncthreads();
   lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
+  const int64_t dst0 = (int64_t)c.a * ny * Hz + c.b;
   auto store = [&](cpx<F>* dst) {
+    if (!c.ok) return;
     for (int i = threadIdx.x; i < ny * T; i += kNTF) {
-      const int t = i % T, y = i / T;
-      const int kz = kz0 + t;
-      if (kz < Hz) dst[((int64_t)x * ny + y) * Hz + kz] = s[y * S + t];
+      const int y = i / T;
+      dst[dst0 + (int64_t)y * Hz] = s[y * S + c.t];
     }
   };
   store(Ax);
@@ -444,7 +462,7 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int i = threadIdx.x + k * kNTF;
-    if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + i % T] = cmul(p[k], dfac<F>(twd, i / T, inv_hy));
+    if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + c.t] = cmul(p[k], dfac<F>(twd, i / T, inv_hy));
   }
   __syncthreads();
   lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
@@ -453,7 +471,7 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int i = threadIdx.x + k * kNTF;
-    if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + i % T] = p[k];
+    if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + c.t] = p[k];
   }
   __syncthreads();
   lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
@@ -544,7 +562,7 @@ __global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, doub
 // The last (x) pass scales by gscale and writes the FP32 or FP64 spectrum.
 template <int M, typename FO>
 __global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO* __restrict__ out, int n, int inner,
-                                              int H, const KickScalars* __restrict__ sc,
+                                              int H, int nouter, const KickScalars* __restrict__ sc,
                                               const cpx<double>* __restrict__ tw, int force, int last,
                                               double inv_m3) {
   using F = double;
@@ -553,14 +571,18 @@ __global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO*
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err || !(force || sc->rebuild)) return;
   const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
-  const int c0 = blockIdx.x * 2 * T;  // first real inner column of this tile
-  const int o = blockIdx.y;            // outer index
+  // real columns (outer, col), numbered flat over nouter * inner, 2T per
+  // tile; this thread's column is the same in every loop below
+  static_assert(kNT % (2 * T) == 0, "a thread must keep its column in every loop");
+  const int cl = threadIdx.x % (2 * T);
+  const int fc = blockIdx.x * 2 * T + cl;
+  const bool ok = fc < nouter * inner;
+  const int o = ok ? fc / inner : 0, col = ok ? fc - o * inner : 0;
   for (int i = threadIdx.x; i < M * T; i += kNT) s[(i / T) * S + (i % T)] = cpx<F>{0, 0};
   __syncthreads();
   for (int i = threadIdx.x; i < n * 2 * T; i += kNT) {
-    const int cl = i % (2 * T), m = i / (2 * T);
-    const int col = c0 + cl;
-    const F v = col < inner ? in[((int64_t)o * n + m) * inner + col] : 0;
+    const int m = i / (2 * T);
+    const F v = ok ? in[((int64_t)o * n + m) * inner + col] : 0;
     ((F*)&s[m * S + (cl >> 1)])[cl & 1] = v;
     if (m > 0) ((F*)&s[(M - m) * S + (cl >> 1)])[cl & 1] = v;
   }
@@ -574,10 +596,9 @@ __global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO*
     const double denom = 4 * kPi * kEps0 * hx * hy * hz;
     g = inv_m3 / denom;
   }
+  if (!ok) return;
   for (int i = threadIdx.x; i < H * 2 * T; i += kNT) {
-    const int cl = i % (2 * T), k = i / (2 * T);
-    const int col = c0 + cl;
-    if (col >= inner) continue;
+    const int k = i / (2 * T);
     const cpx<F> v = s[k * S + (cl >> 1)];
     const double r = (cl & 1) ? v.y : v.x;
     out[((int64_t)o * H + k) * inner + col] = (FO)(r * g);

@@ -2,11 +2,12 @@
 // sequences.  Declarations and contracts are in include/ocsc.h.
 #include "ocsc.h"
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <string>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -46,8 +47,13 @@ struct Buf {
   size_t cap = 0;
 };
 
+// counts device (re)allocations of workspaces: a captured launch sequence
+// holds the addresses it was captured with and is valid while this is unchanged
+std::atomic<uint64_t> g_alloc_gen{0};
+
 int ensure(Buf& b, size_t bytes) {
   if (b.cap >= bytes) return OCSC_OK;
+  g_alloc_gen.fetch_add(1);
   if (b.p) {
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipFree(b.p));
@@ -198,6 +204,18 @@ struct ocsc_ctx {
   int sort_cells = 1;   // sort keys: bricks and the cells inside them (1), or bricks only (0)
   int sort_ahead = 1;   // tracking sort keys at the position half-way through the re-sort interval
   int resort_near = 0;  // re-sorts of the copy: one gather of rows, charges and permutation (1), or records
+  // one-shot kicks (ocsc_apply) replayed from a captured graph (option
+  // "apply_graph"): the launch sequence of the last call, the set-up node
+  // whose inputs change from call to call, and what it was captured for
+  int apply_graph = 1;
+  struct ApplyGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t setup = nullptr;
+    KickInputs in{};
+    std::vector<int64_t> key;
+    hipEvent_t done = nullptr;  // after the last replay: node updates wait for it
+  } ag;
   // cell-ordered copy of bunch i of a batched call: six rows, charges, permutation
   struct BunchCopy {
     Buf rs, qs, perm;
@@ -363,16 +381,17 @@ int launch_kernel_spectrum(ocsc_ctx* c, Lane& L, const Geometry& g, hipStream_t 
   LAUNCHCHK("k_zcos");
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<double, MM>;
-    dim3 grid((d.hz + 2 * TL::T - 1) / (2 * TL::T), d.nx);
-    k_scos<MM, double><<<grid, kNT, TL::lds_bytes, st>>>(Kz, Kzy, d.ny, d.hz, d.hy, L.sc, c->twd + MM, force, 0, 1.0);
+    const int grid = (d.nx * d.hz + 2 * TL::T - 1) / (2 * TL::T);
+    k_scos<MM, double><<<grid, kNT, TL::lds_bytes, st>>>(Kz, Kzy, d.ny, d.hz, d.hy, d.nx, L.sc, c->twd + MM, force, 0,
+                                                         1.0);
   });
   LAUNCHCHK("k_scos(y)");
   const double inv_m3 = 1.0 / ((double)d.mx * (double)d.my * (double)d.mz);
   OCSC_SWITCH_M(d.mx, {
     using TL = Tile<double, MM>;
     const int inner = d.hy * d.hz;
-    dim3 grid((inner + 2 * TL::T - 1) / (2 * TL::T), 1);
-    k_scos<MM, F><<<grid, kNT, TL::lds_bytes, st>>>(Kzy, (F*)L.G.p, d.nx, inner, d.hx, L.sc, c->twd + MM, force, 1,
+    const int grid = (inner + 2 * TL::T - 1) / (2 * TL::T);
+    k_scos<MM, F><<<grid, kNT, TL::lds_bytes, st>>>(Kzy, (F*)L.G.p, d.nx, inner, d.hx, 1, L.sc, c->twd + MM, force, 1,
                                                      inv_m3);
   });
   LAUNCHCHK("k_scos(x)");
@@ -399,7 +418,7 @@ int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho,
   if (ev) (void)hipEventRecord(ev[1], st);
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<F, MM>;
-    dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
+    const int grid = (d.nx * d.hz + TL::T - 1) / TL::T;
     k_yfwd<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, L.sc, tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_yfwd");
@@ -407,14 +426,14 @@ int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho,
   if (join) HIPCHK(hipStreamWaitEvent(st, join, 0));
   OCSC_SWITCH_M(d.mx, {
     using TL = Tile<F, MM>;
-    dim3 grid((d.hz + TL::T - 1) / TL::T, d.my);
+    const int grid = (d.my * d.hz + TL::T - 1) / TL::T;
     k_xconv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, (const F*)L.G.p, d, L.sc, tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_xconv");
   if (ev) (void)hipEventRecord(ev[3], st);
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<F, MM>;
-    dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
+    const int grid = (d.nx * d.hz + TL::T - 1) / TL::T;
     k_yinv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, A, d, L.sc, tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_yinv");
@@ -451,7 +470,7 @@ int launch_fields_v(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, fl
   if (ev) (void)hipEventRecord(ev[1], st);
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<float, MM, V>;
-    dim3 grid((d.hz + TL::T - 1) / TL::T, d.nx);
+    const int grid = (d.nx * d.hz + TL::T - 1) / TL::T;
     k_yfwd<float, MM, V><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, L.sc, c->twf + MM);
   });
   LAUNCHCHK("k_yfwd");
@@ -459,7 +478,7 @@ int launch_fields_v(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, fl
   if (join) HIPCHK(hipStreamWaitEvent(st, join, 0));  // the kernel spectrum, built on another stream
   OCSC_SWITCH_M(d.mx, {
     using FT = FieldTile<MM, kNTF, V>;
-    dim3 grid((d.hz + FT::T - 1) / FT::T, d.my);
+    const int grid = (d.my * d.hz + FT::T - 1) / FT::T;
     k_xconv_field<MM, V><<<grid, kNTF, FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
                                                       c->twd + MM);
   });
@@ -467,7 +486,7 @@ int launch_fields_v(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, fl
   if (ev) (void)hipEventRecord(ev[3], st);
   OCSC_SWITCH_M(d.my, {
     using FT = FieldTile<MM, kNTF, V>;
-    dim3 grid((d.hz + FT::T - 1) / FT::T, d.nx);
+    const int grid = (d.nx * d.hz + FT::T - 1) / FT::T;
     k_yinv_field<MM, V><<<grid, kNTF, FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
                                                      c->twd + MM);
   });
@@ -593,7 +612,7 @@ void launch_deposit(const ocsc_ctx* c, Rows R, const double* q, KickScalars* sc,
   const int t = c->deposit_tile;
 #define OCSC_DEP(V, T)                                                                                 \
   if (v == V && t == T) return launch_deposit_t<NWORD, (V & 4) != 0, (V & 2) != 0, T>(R, q, sc, grid, cells, n, st);
-#define OCSC_DEP3(V) OCSC_DEP(V, 0) OCSC_DEP(V, 1) OCSC_DEP(V, 2)
+#define OCSC_DEP3(V) OCSC_DEP(V, 0) OCSC_DEP(V, 1) OCSC_DEP(V, 2) OCSC_DEP(V, 3)
   OCSC_DEP3(0) OCSC_DEP3(2) OCSC_DEP3(4) OCSC_DEP3(6)
 #undef OCSC_DEP3
 #undef OCSC_DEP
@@ -806,6 +825,99 @@ int check_common(ocsc_ctx* c, const void* r, const double* q, int64_t n, int64_t
 
 namespace {
 
+// bitwise equality of two kick inputs (the values baked into a set-up node)
+bool same_inputs(const KickInputs& a, const KickInputs& b) {
+  auto eq = [](const void* x, const void* y, size_t nbytes) { return std::memcmp(x, y, nbytes) == 0; };
+  return eq(&a.gamref, &b.gamref, 8) && eq(&a.betaref, &b.betaref, 8) && eq(&a.pref, &b.pref, 8) &&
+         eq(&a.gbref, &b.gbref, 8) && eq(&a.cdT, &b.cdT, 8) && eq(&a.mesh_scale, &b.mesh_scale, 8) &&
+         eq(&a.mesh_shift, &b.mesh_shift, 8) && eq(&a.gamma0, &b.gamma0, 8) && eq(a.steps, b.steps, 24) &&
+         a.random_mesh == b.random_mesh && a.mode == b.mode && eq(a.nmesh, b.nmesh, 12) && a.n == b.n &&
+         a.draws == b.draws && a.draw_ctr == b.draw_ctr && a.ndraws == b.ndraws;
+}
+
+void drop_apply_graph(ocsc_ctx* c) {
+  if (c->ag.done) (void)hipEventSynchronize(c->ag.done);
+  if (c->ag.exec) (void)hipGraphExecDestroy(c->ag.exec);
+  if (c->ag.graph) (void)hipGraphDestroy(c->ag.graph);
+  if (c->ag.done) (void)hipEventDestroy(c->ag.done);
+  c->ag = ocsc_ctx::ApplyGraph{};
+}
+
+// One-shot kick from a captured graph.  A kick is some 30 launches, many of
+// them a few microseconds long; launched one by one after the host has waited
+// for the previous kick, the GPU idles between them.  The sequence depends on
+// the caller's pointers and sizes, the mesh, the precision, the options and
+// the workspace addresses; while those are unchanged the graph is replayed,
+// with only the set-up node's inputs (energy, step, random-mesh draws)
+// replaced when they differ.
+int apply_via_graph(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld, const double* q,
+                    const KickInputs& in, int prec, hipStream_t st, const StepOpts& o) {
+  auto& A = c->ag;
+  const std::vector<int64_t> key = {(int64_t)(intptr_t)r, ld, (int64_t)(intptr_t)q, in.n, g.d.nx, g.d.ny, g.d.nz,
+                                    prec, o.sort != nullptr, c->overlap, c->deposit_xcd, c->deposit_agg,
+                                    c->deposit_tile, c->fft_plan, c->sort_cells, c->red_blocks,
+                                    (int64_t)g_alloc_gen.load()};
+  if (!A.exec || A.key != key) {
+    drop_apply_graph(c);
+    HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_kick_prec(c, L, g, r, ld, q, in, prec, c->cap_stream, o);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->cap_stream, &graph);
+    if (rc || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc ? rc : fail(OCSC_EHIP, std::string("apply capture: ") + hipGetErrorString(e));
+    }
+    A.graph = graph;
+    size_t nn = 0;
+    HIPCHK(hipGraphGetNodes(graph, nullptr, &nn));
+    std::vector<hipGraphNode_t> nodes(nn);
+    HIPCHK(hipGraphGetNodes(graph, nodes.data(), &nn));
+    for (hipGraphNode_t nd : nodes) {
+      hipGraphNodeType t;
+      if (hipGraphNodeGetType(nd, &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
+      hipKernelNodeParams p{};
+      if (hipGraphKernelNodeGetParams(nd, &p) == hipSuccess && p.func == (void*)k_setup) A.setup = nd;
+    }
+    if (!A.setup) {
+      // the runtime did not expose the node: kick eagerly from now on
+      drop_apply_graph(c);
+      c->apply_graph = 0;
+      return enqueue_kick_prec(c, L, g, r, ld, q, in, prec, st, o);
+    }
+    const hipError_t ei = hipGraphInstantiate(&A.exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      drop_apply_graph(c);
+      return fail(OCSC_EHIP, std::string("apply instantiate: ") + hipGetErrorString(ei));
+    }
+    HIPCHK(hipEventCreateWithFlags(&A.done, hipEventDisableTiming));
+    A.in = in;
+    A.key = key;
+  } else if (!same_inputs(A.in, in)) {
+    // a replay still in flight (calls without sync) keeps its inputs
+    HIPCHK(hipEventSynchronize(A.done));
+    hipKernelNodeParams p{};
+    HIPCHK(hipGraphKernelNodeGetParams(A.setup, &p));
+    KickScalars* scp = L.sc;
+    KickInputs inv = in;
+    void* args[2] = {&scp, &inv};
+    p.kernelParams = args;
+    p.extra = nullptr;
+    const hipError_t e = hipGraphExecKernelNodeSetParams(A.exec, A.setup, &p);
+    if (e != hipSuccess) {
+      drop_apply_graph(c);
+      return fail(OCSC_EHIP, std::string("apply graph update: ") + hipGetErrorString(e));
+    }
+    A.in = in;
+  }
+  const hipError_t e = hipGraphLaunch(A.exec, st);
+  if (e != hipSuccess) {
+    drop_apply_graph(c);
+    return fail(OCSC_EHIP, std::string("apply graph launch: ") + hipGetErrorString(e));
+  }
+  HIPCHK(hipEventRecord(A.done, st));
+  return OCSC_OK;
+}
+
 constexpr int kMaxLanes = 8;
 // one-shot FP64 kicks of at least this many particles go through a
 // cell-ordered copy (measured at 10M / 127^3: FP64 2.58 -> 2.42 ms, where the
@@ -929,6 +1041,7 @@ int ocsc_destroy(ocsc_ctx* c) {
   if (c->mom_host) (void)hipHostFree(c->mom_host);
   if (c->twf) (void)hipFree(c->twf);
   if (c->twd) (void)hipFree(c->twd);
+  drop_apply_graph(c);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -995,7 +1108,11 @@ int ocsc_apply_cells(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
                   nullptr, true};
     o.sort = &sb;
   }
-  if ((rc = enqueue_kick_prec(c, L, g, r, ld, q, in, prec, st, o))) return rc;
+  if (c->apply_graph && !cells && !c->profiling)
+    rc = apply_via_graph(c, L, g, r, ld, q, in, prec, st, o);
+  else
+    rc = enqueue_kick_prec(c, L, g, r, ld, q, in, prec, st, o);
+  if (rc) return rc;
   if (sync) {
     int32_t err = 0;
     if ((rc = read_errors(c, L, st, &err))) return rc;
@@ -2191,8 +2308,14 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->apply_sort = (int)value;
     return OCSC_OK;
   }
+  if (n == "apply_graph") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "apply_graph must be 0 or 1");
+    c->apply_graph = (int)value;
+    if (!value) drop_apply_graph(c);
+    return OCSC_OK;
+  }
   if (n == "deposit_tile") {
-    if (value < 0 || value > 2) return fail(OCSC_EINVAL, "deposit_tile must be 0, 1 or 2");
+    if (value < 0 || value > 3) return fail(OCSC_EINVAL, "deposit_tile must be 0, 1, 2 or 3");
     c->deposit_tile = (int)value;
     return OCSC_OK;
   }
@@ -2253,6 +2376,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "apply_sort") {
     *value = c->apply_sort;
+    return OCSC_OK;
+  }
+  if (n == "apply_graph") {
+    *value = c->apply_graph;
     return OCSC_OK;
   }
   if (n == "deposit_tile") {

@@ -452,14 +452,18 @@ constexpr int kDepSlots2 = 1024;             // two-word (FP64) form: half the L
                                              // workgroups per CU (FP64 step 0.964 -> 0.950 ms)
 
 // deposit tiles (option "deposit_tile"): 0 the sizes above, 1 half the
-// particles per thread and half the slots, 2 twice both
+// particles per thread and half the slots, 2 twice both, 3 no table (every
+// particle one global atomic: a bunch in random order spreads a workgroup's
+// particles over far more cells than a table holds, and each insert then
+// probes a nearly full table before it falls through)
 template <int TILE>
 struct DepTile {
+  static constexpr bool direct = TILE == 3;
   static constexpr int items = TILE == 1 ? kDepItems / 2 : (TILE == 2 ? kDepItems * 2 : kDepItems);
   template <int NWORD>
   static constexpr int slots() {
     constexpr int base = NWORD == 2 ? kDepSlots2 : kDepSlots;
-    return TILE == 1 ? base / 2 : (TILE == 2 ? base * 2 : base);
+    return TILE == 1 ? base / 2 : (TILE == 2 ? base * 2 : (TILE == 3 ? 1 : base));
   }
 };
 
@@ -509,12 +513,15 @@ __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __rest
   const int64_t ncell = (int64_t)s.nmesh[0] * s.nmesh[1] * s.nmesh[2];
   unsigned long long* g = grid;
   if constexpr (XCD) g += (int64_t)xcc_id() * NWORD * ncell;
-  for (int i = threadIdx.x; i < kSlots; i += kDepPT) {
-    hkey[i] = -1;
-    hval[0][i] = 0;
-    if constexpr (NWORD == 2) hval[1][i] = 0;
+  constexpr bool kDirect = DepTile<TILE>::direct;
+  if constexpr (!kDirect) {
+    for (int i = threadIdx.x; i < kSlots; i += kDepPT) {
+      hkey[i] = -1;
+      hval[0][i] = 0;
+      if constexpr (NWORD == 2) hval[1][i] = 0;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (kDepPT * kItems);
   const int lane = threadIdx.x & 63;
   int bad = 0;
@@ -570,16 +577,26 @@ __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __rest
         }
       }
       const int knext = __shfl_down(key, 1);
-      if (key >= 0 && (lane == 63 || knext != key)) hash_add<NWORD, XCD, kSlots>(hkey, hval, g, ncell, key, hi, lo);
-    } else {
-      if (idx >= 0) hash_add<NWORD, XCD, kSlots>(hkey, hval, g, ncell, idx, hi, lo);
+      if (key >= 0 && (lane == 63 || knext != key)) {
+        if constexpr (kDirect)
+          flush_cell<XCD>(g, ncell, key, hi, lo, NWORD);
+        else
+          hash_add<NWORD, XCD, kSlots>(hkey, hval, g, ncell, key, hi, lo);
+      }
+    } else if (idx >= 0) {
+      if constexpr (kDirect)
+        flush_cell<XCD>(g, ncell, idx, hi, lo, NWORD);
+      else
+        hash_add<NWORD, XCD, kSlots>(hkey, hval, g, ncell, idx, hi, lo);
     }
   }
   if (bad) atomicOr(&scp->err, (int)kErrCell);
-  __syncthreads();
-  for (int i = threadIdx.x; i < kSlots; i += kDepPT) {
-    const int k = hkey[i];
-    if (k >= 0) flush_cell<XCD>(g, ncell, k, hval[0][i], NWORD == 2 ? hval[NWORD - 1][i] : 0, NWORD);
+  if constexpr (!kDirect) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kSlots; i += kDepPT) {
+      const int k = hkey[i];
+      if (k >= 0) flush_cell<XCD>(g, ncell, k, hval[0][i], NWORD == 2 ? hval[NWORD - 1][i] : 0, NWORD);
+    }
   }
 }
 

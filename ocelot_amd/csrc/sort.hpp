@@ -18,7 +18,10 @@
 // takes its 16 items in order; lanes holding the same digit find each other
 // with eight ballots, and a per-wavefront running count per digit (LDS) gives
 // the rank, so ties keep their input order.  The per-digit counts of the four
-// wavefronts are then prefix-summed.
+// wavefronts are then prefix-summed.  The tile is sorted by digit in LDS
+// first and written out in that order, so consecutive threads store to
+// consecutive addresses inside each digit's run (written straight from the
+// ranks, each store would land in a run of 16 keys on average).
 #pragma once
 #include "common.hpp"
 
@@ -103,6 +106,8 @@ __global__ void __launch_bounds__(kRsPT) k_rs_scatter(const uint32_t* __restrict
   __shared__ uint32_t base[kRsDigits];
   __shared__ uint32_t wrun[kRsWaves][kRsDigits];
   __shared__ uint32_t scan_lds[kRsPT];
+  __shared__ uint32_t lstart[kRsDigits];  // first tile-local slot of each digit
+  __shared__ uint32_t lk[kRsTile], lv[kRsTile];
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   // digit bases: exclusive scan of this pass's totals, plus this tile's offset
   {
@@ -140,7 +145,8 @@ __global__ void __launch_bounds__(kRsPT) k_rs_scatter(const uint32_t* __restrict
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
-  // prefix of the per-wavefront counts for each digit (thread t: digit t)
+  // prefix of the per-wavefront counts for each digit (thread t: digit t),
+  // and the digit's first slot in the tile
   {
     uint32_t s = 0;
     for (int ww = 0; ww < kRsWaves; ++ww) {
@@ -148,16 +154,29 @@ __global__ void __launch_bounds__(kRsPT) k_rs_scatter(const uint32_t* __restrict
       wrun[ww][t] = s;
       s += c;
     }
+    uint32_t all;
+    lstart[t] = rs_block_scan(s, scan_lds, &all);
   }
   __syncthreads();
+  // Keys past n carry digit kRsDigits - 1 at every shift and come last in
+  // the input order, so they take the tile's last slots.
 #pragma unroll
   for (int i = 0; i < kRsItems; ++i) {
     const int64_t p = tile0 + rs_pos(w, i, l);
     if (p >= n) continue;
     const uint32_t d = (key[i] >> shift) & (kRsDigits - 1);
-    const uint32_t dst = base[d] + wrun[w][d] + rank[i];
-    keys_out[dst] = key[i];
-    vals_out[dst] = val[i];
+    const uint32_t lp = lstart[d] + wrun[w][d] + rank[i];
+    lk[lp] = key[i];
+    lv[lp] = val[i];
+  }
+  __syncthreads();
+  const int nvalid = n - tile0 < kRsTile ? (int)(n - tile0) : kRsTile;
+  for (int j = t; j < nvalid; j += kRsPT) {
+    const uint32_t k = lk[j];
+    const uint32_t d = (k >> shift) & (kRsDigits - 1);
+    const uint32_t dst = base[d] + ((uint32_t)j - lstart[d]);
+    keys_out[dst] = k;
+    vals_out[dst] = lv[j];
   }
 }
 

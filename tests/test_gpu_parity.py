@@ -498,7 +498,7 @@ def test_per_xcd_deposit_is_bit_identical(prec):
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("opt,vals", [("deposit_agg", (0, 1)), ("permute_rec", (0, 1)), ("resort_near", (0, 1)),
-                                      ("deposit_tile", (0, 1, 2))])
+                                      ("deposit_tile", (0, 1, 2, 3))])
 def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
     """The deposit's wave pre-sum of equal cells and its tile sizes (integer
     sums in another grouping), with one grid and with per-XCD copies, and the
@@ -588,6 +588,52 @@ def test_sorted_one_shot_apply_is_bit_identical(prec):
     finally:
         _lib.set_option(dev, "apply_sort", old)
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("sort", [0, 1])
+def test_apply_graph_replays_equal_eager_kicks(prec, sort):
+    """One-shot kicks replayed from the cached launch graph (option
+    apply_graph) equal eager kicks bit for bit, kick after kick: repeated
+    kicks of one bunch (replays), a changed step and random-mesh draws (the
+    set-up node's inputs replaced), another bunch (a new capture), a bad
+    bunch that raises and stays unchanged, and the first bunch again."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("gauss_2m_63", n=300_000, seed=21)
+    r2, q2, _, _, _ = make_bunch("gauss_2m_63", n=200_000, seed=22)
+    bad = r2.copy()
+    bad[0, 77] = np.nan
+    dev = torch.cuda.current_device()
+    saved = {k: _lib.get_option(dev, k) for k in ("apply_graph", "apply_sort")}
+    outs = {}
+    try:
+        _lib.set_option(dev, "apply_sort", sort)
+        for mode in (0, 1):
+            _lib.set_option(dev, "apply_graph", mode)
+            np.random.seed(5)
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            d2 = DeviceParticleArray.from_host(parray(r2, q2, E), dev)
+            db = DeviceParticleArray.from_host(parray(bad, q2, E), dev)
+            sc = SpaceCharge(nmesh_xyz=[47, 47, 63], precision=prec)
+            scr = SpaceCharge(nmesh_xyz=[47, 47, 63], precision=prec, random_mesh=True)
+            seq = []
+            for proc, dd, step in ((sc, d, dz), (sc, d, dz), (sc, d, 0.5 * dz), (scr, d, dz), (scr, d, dz),
+                                   (sc, d2, dz), (sc, d, dz)):
+                proc.apply(dd, step)
+                seq.append(dd.rparticles.cpu().numpy())
+                torch.cuda.synchronize()
+            with pytest.raises(ValueError):
+                sc.apply(db, dz)
+            assert np.array_equal(db.rparticles.cpu().numpy(), bad, equal_nan=True)
+            sc.apply(d, dz)
+            seq.append(d.rparticles.cpu().numpy())
+            outs[mode] = seq
+    finally:
+        for k, v in saved.items():
+            _lib.set_option(dev, k, v)
+    for i, (a, b) in enumerate(zip(outs[0], outs[1])):
+        assert np.array_equal(a, b), i
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
