@@ -64,7 +64,34 @@ def parse():
                     help="only time the drop-in SpaceCharge.apply on the resident bunch of --workload in its "
                          "caller's order (--warmup, then --steps kicks, then per-stage events over --steps more); "
                          "one JSON line")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only time the 64-bunch batched kicks (mixed, --warmup kicks, then one call of --steps "
+                         "kicks); one JSON line")
     return ap.parse_args()
+
+
+def batch_only(a, torch, dev):
+    """Context run for profiling the batched kicks of 64 bunches of uneven
+    size and mesh (not the headline)."""
+    from ocelot_amd import DeviceParticleArray, ParticleArray, apply_batched
+    from ocelot_amd.workloads import batch64
+    bunches = batch64(seed=0)
+    ntot = sum(b["r"].shape[1] for b in bunches)
+    ds = []
+    for b in bunches:
+        p = ParticleArray()
+        p.rparticles, p.q_array, p.E = b["r"], b["q"], b["E"]
+        ds.append(DeviceParticleArray.from_host(p, dev))
+    items = [(d, b["nmesh"], 0.1) for d, b in zip(ds, bunches)]
+    apply_batched(items, precision="mixed", nkicks=max(a.warmup, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    apply_batched(items, precision="mixed", nkicks=a.steps)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(a.steps, 1)
+    print(json.dumps({"metric": "batched kicks of 64 bunches", "particles": ntot, "bunches": len(bunches),
+                      "kicks": a.steps, "ms_per_round": round(dt * 1e3, 4), "value": round(ntot / dt, 1),
+                      "unit": "macroparticle-kicks/s", "precision": "mixed"}))
 
 
 def apply_only(a, torch, host, nmesh, dz, dev):
@@ -424,6 +451,8 @@ def main():
         torch.add(x, 1.0, out=y)
         torch.cuda.synchronize()
         del x, y
+    if a.batch_only:
+        return batch_only(a, torch, torch.cuda.current_device())
     r, q, E, nmesh, dz = make_bunch(a.workload, n=a.particles)
     n = r.shape[1]
     host = ParticleArray()

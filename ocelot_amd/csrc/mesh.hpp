@@ -115,6 +115,13 @@ __device__ __forceinline__ Col flat_col(int ncol, int nb) {
   return c;
 }
 
+// Items per thread of a loop over `count` items strided by NT threads.  The
+// global loads of a tile are issued in unrolled loops of this many items,
+// all before the first use, so a thread has its loads in flight together
+// instead of waiting for each before the next (the loop counts depend on
+// run-time sizes, so the compiler would otherwise keep them rolled).
+constexpr int per_thread(int count, int nt) { return (count + nt - 1) / nt; }
+
 // ---------------------------------------------------------------------------
 // z pass: pairs of real rows -> one complex FFT of length Mz -> two half spectra
 template <typename F, int M, int CIN, int V = 0>
@@ -137,17 +144,32 @@ __global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<
     const int t = i % T, m = nz + i / T;
     s[m * S + t] = cpx<F>{0, 0};
   }
-  for (int i = threadIdx.x; i < 2 * T * nz; i += kNT) {
-    const int rl = i / nz, m = i - rl * nz;
-    const int64_t row = row0 + rl;
-    F v = 0;
-    if (row < R) v = (F)charge_at<CIN, F>(rho, row * nz + m, ncell, iq, ncopy);
-    F* p = (F*)&s[m * S + (rl >> 1)];
-    p[rl & 1] = v;
+  // 2T rows of nz <= M/2 charges
+  constexpr int NL = per_thread(T * M, kNT);
+  F v[NL];
+  int off[NL];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int i = threadIdx.x + k * kNT;
+    v[k] = 0;
+    off[k] = -1;
+    if (i < 2 * T * nz) {
+      const int rl = i / nz, m = i - rl * nz;
+      const int64_t row = row0 + rl;
+      if (row < R) v[k] = (F)charge_at<CIN, F>(rho, row * nz + m, ncell, iq, ncopy);
+      off[k] = (m * S + (rl >> 1)) * 2 + (rl & 1);
+    }
   }
+#pragma unroll
+  for (int k = 0; k < NL; ++k)
+    if (off[k] >= 0) ((F*)s)[off[k]] = v[k];
   __syncthreads();
   lds_fft<F, M, T, kNT, -1, S, TL::RSEL>(s, twl);
-  for (int i = threadIdx.x; i < T * Hz; i += kNT) {
+  constexpr int NS = per_thread(T * (M / 2 + 1), kNT);
+#pragma unroll
+  for (int k2 = 0; k2 < NS; ++k2) {
+    const int i = threadIdx.x + k2 * kNT;
+    if (i >= T * Hz) break;
     const int t = i / Hz, k = i - t * Hz;
     const int64_t ra = row0 + 2 * t, rb = ra + 1;
     if (ra >= R) continue;
@@ -173,19 +195,27 @@ __global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<
   const int Hz = d.hz, ny = d.ny;
   const Col c = flat_col<T, kNT>(d.nx * Hz, Hz);
   const cpx<F>* src = A + (int64_t)c.a * ny * Hz + c.b;
-  for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int y = i / T;
-    cpx<F> v{0, 0};
-    if (y < ny && c.ok) v = src[(int64_t)y * Hz];
-    s[y * S + c.t] = v;
+  constexpr int NL = per_thread(M * T, kNT);
+  cpx<F> v[NL];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int y = (threadIdx.x + k * kNT) / T;
+    v[k] = cpx<F>{0, 0};
+    if (y < ny && c.ok) v[k] = src[(int64_t)y * Hz];
+  }
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int y = (threadIdx.x + k * kNT) / T;
+    if (y < M) s[y * S + c.t] = v[k];
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, -1, S, TL::RSEL>(s, twl);
   if (!c.ok) return;
   cpx<F>* dst = B + (int64_t)c.a * M * Hz + c.b;
-  for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int ky = i / T;
-    dst[(int64_t)ky * Hz] = s[ky * S + c.t];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int ky = (threadIdx.x + k * kNT) / T;
+    if (ky < M) dst[(int64_t)ky * Hz] = s[ky * S + c.t];
   }
 }
 
@@ -204,27 +234,42 @@ __global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* 
   const int64_t cs = (int64_t)My * Hz;  // stride between x planes
   const Col c = flat_col<T, kNT>(My * Hz, Hz);  // a = ky, b = kz
   cpx<F>* col = B + (int64_t)c.a * Hz + c.b;
-  for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int x = i / T;
-    cpx<F> v{0, 0};
-    if (x < nx && c.ok) v = col[x * cs];
-    s[x * S + c.t] = v;
+  constexpr int NL = per_thread(M * T, kNT);
+  cpx<F> v[NL];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int x = (threadIdx.x + k * kNT) / T;
+    v[k] = cpx<F>{0, 0};
+    if (x < nx && c.ok) v[k] = col[x * cs];
+  }
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int x = (threadIdx.x + k * kNT) / T;
+    if (x < M) s[x * S + c.t] = v[k];
+  }
+  // the kernel spectrum for this thread's values, loaded while the FFT runs
+  const int fy = c.a <= My - c.a ? c.a : My - c.a;
+  F g[NL];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int kx = (threadIdx.x + k * kNT) / T;
+    const int fx = kx <= M - kx ? kx : M - kx;
+    g[k] = (c.ok && kx < M) ? G[((int64_t)fx * Hy + fy) * Hz + c.b] : (F)0;
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, -1>(s, twl);
-  const int fy = c.a <= My - c.a ? c.a : My - c.a;
-  for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int kx = i / T;
-    const int fx = kx <= M - kx ? kx : M - kx;
-    const F g = c.ok ? G[((int64_t)fx * Hy + fy) * Hz + c.b] : (F)0;
-    s[kx * S + c.t] = cscale(s[kx * S + c.t], g);
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int kx = (threadIdx.x + k * kNT) / T;
+    if (kx < M) s[kx * S + c.t] = cscale(s[kx * S + c.t], g[k]);
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, +1>(s, twl);
   if (!c.ok) return;
-  for (int i = threadIdx.x; i < nx * T; i += kNT) {
-    const int x = i / T;
-    col[x * cs] = s[x * S + c.t];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int x = (threadIdx.x + k * kNT) / T;
+    if (x < nx) col[x * cs] = s[x * S + c.t];
   }
 }
 
@@ -241,19 +286,27 @@ __global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<
   const int Hz = d.hz, ny = d.ny;
   const Col c = flat_col<T, kNT>(d.nx * Hz, Hz);  // a = x, b = kz
   const cpx<F>* src = B + (int64_t)c.a * M * Hz + c.b;
-  for (int i = threadIdx.x; i < M * T; i += kNT) {
-    const int ky = i / T;
-    cpx<F> v{0, 0};
-    if (c.ok) v = src[(int64_t)ky * Hz];
-    s[ky * S + c.t] = v;
+  constexpr int NL = per_thread(M * T, kNT);
+  cpx<F> v[NL];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int ky = (threadIdx.x + k * kNT) / T;
+    v[k] = cpx<F>{0, 0};
+    if (c.ok && ky < M) v[k] = src[(int64_t)ky * Hz];
+  }
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int ky = (threadIdx.x + k * kNT) / T;
+    if (ky < M) s[ky * S + c.t] = v[k];
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, +1>(s, twl);
   if (!c.ok) return;
   cpx<F>* dst = A + (int64_t)c.a * ny * Hz + c.b;
-  for (int i = threadIdx.x; i < ny * T; i += kNT) {
-    const int y = i / T;
-    dst[(int64_t)y * Hz] = s[y * S + c.t];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int y = (threadIdx.x + k * kNT) / T;
+    if (y < ny) dst[(int64_t)y * Hz] = s[y * S + c.t];
   }
 }
 
@@ -269,23 +322,42 @@ __device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __res
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
-  for (int i = threadIdx.x; i < T * Hz; i += kNT) {
+  // T pairs of half spectra (Hz = M/2 + 1 each)
+  constexpr int NL = per_thread(T * (M / 2 + 1), kNT);
+  cpx<F> a[NL], b[NL];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    const int i = threadIdx.x + j * kNT;
+    a[j] = cpx<F>{0, 0};
+    b[j] = cpx<F>{0, 0};
+    if (i < T * Hz) {
+      const int t = i / Hz, k = i - t * Hz;
+      const int64_t ra = row0 + 2 * t, rb = ra + 1;
+      if (ra < R) a[j] = A[ra * Hz + k];
+      if (rb < R) b[j] = A[rb * Hz + k];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    const int i = threadIdx.x + j * kNT;
+    if (i >= T * Hz) break;
     const int t = i / Hz, k = i - t * Hz;
-    const int64_t ra = row0 + 2 * t, rb = ra + 1;
-    cpx<F> a{0, 0}, b{0, 0};
-    if (ra < R) a = A[ra * Hz + k];
-    if (rb < R) b = A[rb * Hz + k];
+    cpx<F> aa = a[j], bb = b[j];
     if constexpr (DERIV) {
       const cpx<F> f = dfac<F>(twd, k, 1.0 / sc->steps[2]);
-      a = cmul(a, f);
-      b = cmul(b, f);
+      aa = cmul(aa, f);
+      bb = cmul(bb, f);
     }
-    s[k * S + t] = cpx<F>{a.x - b.y, a.y + b.x};
-    if (k > 0 && k < M / 2) s[(M - k) * S + t] = cpx<F>{a.x + b.y, b.x - a.y};
+    s[k * S + t] = cpx<F>{aa.x - bb.y, aa.y + bb.x};
+    if (k > 0 && k < M / 2) s[(M - k) * S + t] = cpx<F>{aa.x + bb.y, bb.x - aa.y};
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, +1, S, TL::RSEL>(s, twl);
-  for (int i = threadIdx.x; i < 2 * T * nz; i += kNT) {
+  constexpr int NS = per_thread(T * M, kNT);  // 2T rows of nz <= M/2
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int i = threadIdx.x + j * kNT;
+    if (i >= 2 * T * nz) break;
     const int rl = i / nz, z = i - rl * nz;
     const int64_t row = row0 + rl;
     if (row >= R) continue;
@@ -358,11 +430,19 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
   const int64_t cs = (int64_t)My * Hz;
   const Col c = flat_col<T, kNTF>(My * Hz, Hz);  // a = ky, b = kz
   const int64_t off = (int64_t)c.a * Hz + c.b;
-  for (int i = threadIdx.x; i < M * T; i += kNTF) {
-    const int x = i / T;
-    cpx<F> v{0, 0};
-    if (x < nx && c.ok) v = B[x * cs + off];
-    s[x * S + c.t] = v;
+  {
+    cpx<F> u[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int x = (threadIdx.x + k * kNTF) / T;
+      u[k] = cpx<F>{0, 0};
+      if (x < nx && c.ok) u[k] = B[x * cs + off];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int x = (threadIdx.x + k * kNTF) / T;
+      if (x < M) s[x * S + c.t] = u[k];
+    }
   }
   __syncthreads();
   lds_fft<F, M, T, kNTF, -1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
@@ -391,11 +471,13 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
   }
   __syncthreads();
   lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
-  if (c.ok)
-    for (int i = threadIdx.x; i < nx * T; i += kNTF) {
-      const int x = i / T;
-      Qx[x * cs + off] = s[x * S + c.t];
+  if (c.ok) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int x = (threadIdx.x + k * kNTF) / T;
+      if (x < nx) Qx[x * cs + off] = s[x * S + c.t];
     }
+  }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -404,11 +486,13 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
   }
   __syncthreads();
   lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
-  if (c.ok)
-    for (int i = threadIdx.x; i < nx * T; i += kNTF) {
-      const int x = i / T;
-      P[x * cs + off] = s[x * S + c.t];
+  if (c.ok) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int x = (threadIdx.x + k * kNTF) / T;
+      if (x < nx) P[x * cs + off] = s[x * S + c.t];
     }
+  }
 }
 
 // Mixed form, y inverse: Ex' = IFFT_y(Qx), Ey' = IFFT_y(P * Dy), Phi' = IFFT_y(P)
@@ -452,9 +536,10 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
   const int64_t dst0 = (int64_t)c.a * ny * Hz + c.b;
   auto store = [&](cpx<F>* dst) {
     if (!c.ok) return;
-    for (int i = threadIdx.x; i < ny * T; i += kNTF) {
-      const int y = i / T;
-      dst[dst0 + (int64_t)y * Hz] = s[y * S + c.t];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int y = (threadIdx.x + k * kNTF) / T;
+      if (y < ny) dst[dst0 + (int64_t)y * Hz] = s[y * S + c.t];
     }
   };
   store(Ax);
@@ -539,21 +624,38 @@ __global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, doub
   const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
   for (int i = threadIdx.x; i < M * T; i += kNT) s[(i / T) * S + (i % T)] = cpx<F>{0, 0};
   __syncthreads();
-  for (int i = threadIdx.x; i < 2 * T * nz; i += kNT) {
+  constexpr int NL = per_thread(T * M, kNT);  // 2T rows of nz <= M/2 values
+  F v[NL];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    const int i = threadIdx.x + j * kNT;
+    v[j] = 0;
+    if (i < 2 * T * nz) {
+      const int rl = i / nz, m = i - rl * nz;
+      const int64_t row = row0 + rl;
+      if (row < R) v[j] = K[row * nz + m];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    const int i = threadIdx.x + j * kNT;
+    if (i >= 2 * T * nz) break;
     const int rl = i / nz, m = i - rl * nz;
-    const int64_t row = row0 + rl;
-    const F v = row < R ? K[row * nz + m] : 0;
-    ((F*)&s[m * S + (rl >> 1)])[rl & 1] = v;
-    if (m > 0) ((F*)&s[(M - m) * S + (rl >> 1)])[rl & 1] = v;
+    ((F*)&s[m * S + (rl >> 1)])[rl & 1] = v[j];
+    if (m > 0) ((F*)&s[(M - m) * S + (rl >> 1)])[rl & 1] = v[j];
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, -1>(s, twl);
-  for (int i = threadIdx.x; i < 2 * T * Hz; i += kNT) {
+  constexpr int NS = per_thread(2 * T * (M / 2 + 1), kNT);
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int i = threadIdx.x + j * kNT;
+    if (i >= 2 * T * Hz) break;
     const int rl = i / Hz, k = i - rl * Hz;
     const int64_t row = row0 + rl;
     if (row >= R) continue;
-    const cpx<F> v = s[k * S + (rl >> 1)];
-    out[row * Hz + k] = (rl & 1) ? v.y : v.x;
+    const cpx<F> w = s[k * S + (rl >> 1)];
+    out[row * Hz + k] = (rl & 1) ? w.y : w.x;
   }
 }
 
@@ -580,11 +682,19 @@ __global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO*
   const int o = ok ? fc / inner : 0, col = ok ? fc - o * inner : 0;
   for (int i = threadIdx.x; i < M * T; i += kNT) s[(i / T) * S + (i % T)] = cpx<F>{0, 0};
   __syncthreads();
-  for (int i = threadIdx.x; i < n * 2 * T; i += kNT) {
-    const int m = i / (2 * T);
-    const F v = ok ? in[((int64_t)o * n + m) * inner + col] : 0;
-    ((F*)&s[m * S + (cl >> 1)])[cl & 1] = v;
-    if (m > 0) ((F*)&s[(M - m) * S + (cl >> 1)])[cl & 1] = v;
+  constexpr int NL = per_thread(T * M, kNT);  // n <= M/2 values of 2T columns
+  F v[NL];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    const int m = (threadIdx.x + j * kNT) / (2 * T);
+    v[j] = (ok && m < n) ? in[((int64_t)o * n + m) * inner + col] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    const int m = (threadIdx.x + j * kNT) / (2 * T);
+    if (m >= n) break;
+    ((F*)&s[m * S + (cl >> 1)])[cl & 1] = v[j];
+    if (m > 0) ((F*)&s[(M - m) * S + (cl >> 1)])[cl & 1] = v[j];
   }
   __syncthreads();
   lds_fft<F, M, T, kNT, -1>(s, twl);
@@ -597,10 +707,13 @@ __global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO*
     g = inv_m3 / denom;
   }
   if (!ok) return;
-  for (int i = threadIdx.x; i < H * 2 * T; i += kNT) {
-    const int k = i / (2 * T);
-    const cpx<F> v = s[k * S + (cl >> 1)];
-    const double r = (cl & 1) ? v.y : v.x;
+  constexpr int NS = per_thread((M / 2 + 1) * 2 * T, kNT);
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int k = (threadIdx.x + j * kNT) / (2 * T);
+    if (k >= H) break;
+    const cpx<F> w = s[k * S + (cl >> 1)];
+    const double r = (cl & 1) ? w.y : w.x;
     out[((int64_t)o * H + k) * inner + col] = (FO)(r * g);
   }
 }

@@ -47,10 +47,16 @@ __global__ void __launch_bounds__(kRsPT) k_rs_hist(const uint32_t* __restrict__ 
   __syncthreads();
   const int w = threadIdx.x >> 6;
   const int64_t base = (int64_t)blockIdx.x * kRsTile;
+  // all of a thread's keys in flight before the first count
+  uint32_t k[kRsItems];
+#pragma unroll
   for (int i = 0; i < kRsItems; ++i) {
     const int64_t p = base + i * kRsPT + threadIdx.x;
-    if (p < n) atomicAdd(&h[w][(keys[p] >> shift) & (kRsDigits - 1)], 1u);
+    k[i] = p < n ? keys[p] : 0xFFFFFFFFu;
   }
+#pragma unroll
+  for (int i = 0; i < kRsItems; ++i)
+    if (base + i * kRsPT + threadIdx.x < n) atomicAdd(&h[w][(k[i] >> shift) & (kRsDigits - 1)], 1u);
   __syncthreads();
   const int d = threadIdx.x;  // kRsPT == kRsDigits
   uint32_t c = 0;
