@@ -295,10 +295,14 @@ typedef struct ocsc_bunch {
 } ocsc_bunch;
 
 /* nkicks rounds of SpaceCharge.apply over nb independent bunches of uneven
- * size, mesh and energy (each round kicks every bunch once).  Bunches are
- * spread over up to 8 workspaces with their own streams so small bunches run
- * concurrently; with use_graph != 0 one round is captured in a HIP graph and
- * replayed.  With option "batch_sort" = 1 (default) each bunch is kicked as a
+ * size, mesh and energy (each round kicks every bunch once).  With option
+ * "batch_fused" = 1 (default) every pass of a round is one launch over all
+ * bunches (a device table maps each workgroup to its bunch; the transform
+ * passes launch once per FFT length along their axis), each bunch with a
+ * workspace of its own; with 0 the bunches are spread over up to 8
+ * workspaces with their own streams; both give bit-identical results.  With
+ * use_graph != 0 one round is captured in a HIP graph and replayed.  With
+ * option "batch_sort" = 1 (default) each bunch is kicked as a
  * cell-ordered copy kept across the rounds and written back to the caller's
  * order at the end: one round equals ocsc_apply bit for bit; in later rounds
  * the frame sums run in cell order (results equal nkicks calls of ocsc_apply

@@ -104,11 +104,11 @@ struct Col {
   bool ok;
 };
 template <int T, int NT>
-__device__ __forceinline__ Col flat_col(int ncol, int nb) {
+__device__ __forceinline__ Col flat_col(int ncol, int nb, int bx) {
   static_assert(NT % T == 0, "a thread must keep its column in every loop");
   Col c;
   c.t = threadIdx.x % T;
-  const int col = blockIdx.x * T + c.t;
+  const int col = bx * T + c.t;
   c.ok = col < ncol;
   c.a = c.ok ? col / nb : 0;
   c.b = c.ok ? col - c.a * nb : 0;
@@ -124,10 +124,8 @@ constexpr int per_thread(int count, int nt) { return (count + nt - 1) / nt; }
 
 // ---------------------------------------------------------------------------
 // z pass: pairs of real rows -> one complex FFT of length Mz -> two half spectra
-template <typename F, int M, int CIN, int V = 0>
-__global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d,
-                                              const KickScalars* __restrict__ sc,
-                                              const cpx<F>* __restrict__ tw, int ncopy = 1) {
+template <typename F, int M, int CIN, int V>
+__device__ __forceinline__ void zfwd_body(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, int ncopy, const int bx, const int nbx) {
   using TL = Tile<F, M, V>;
   constexpr int T = TL::T, S = TL::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -137,7 +135,7 @@ __global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
   const int64_t ncell = R * nz;
-  const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
+  const int64_t row0 = (int64_t)bx * (2 * T);
   const double iq = sc->inv_qscale;
   // zero the padding rows m in [nz, M)
   for (int i = threadIdx.x; i < (M - nz) * T; i += kNT) {
@@ -181,11 +179,16 @@ __global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<
   }
 }
 
-// y pass forward: A[x][y<ny][kz] -> B[x][ky<My][kz]
-template <typename F, int M, int V = 0>
-__global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d,
+template <typename F, int M, int CIN, int V = 0>
+__global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d,
                                               const KickScalars* __restrict__ sc,
-                                              const cpx<F>* __restrict__ tw) {
+                                              const cpx<F>* __restrict__ tw, int ncopy = 1) {
+  zfwd_body<F, M, CIN, V>(rho, A, d, sc, tw, ncopy, blockIdx.x, gridDim.x);
+}
+
+// y pass forward: A[x][y<ny][kz] -> B[x][ky<My][kz]
+template <typename F, int M, int V>
+__device__ __forceinline__ void yfwd_body(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, const int bx, const int nbx) {
   using TL = Tile<F, M, V>;
   constexpr int T = TL::T, S = TL::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -193,7 +196,7 @@ __global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<
   if (sc->err) return;
   const cpx<F>* twl = stage_tw<F, M, kNT, TL::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
-  const Col c = flat_col<T, kNT>(d.nx * Hz, Hz);
+  const Col c = flat_col<T, kNT>(d.nx * Hz, Hz, bx);
   const cpx<F>* src = A + (int64_t)c.a * ny * Hz + c.b;
   constexpr int NL = per_thread(M * T, kNT);
   cpx<F> v[NL];
@@ -219,12 +222,17 @@ __global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<
   }
 }
 
+template <typename F, int M, int V = 0>
+__global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d,
+                                              const KickScalars* __restrict__ sc,
+                                              const cpx<F>* __restrict__ tw) {
+  yfwd_body<F, M, V>(A, B, d, sc, tw, blockIdx.x, gridDim.x);
+}
+
 // x pass: forward FFT along x, multiply by the kernel spectrum, inverse FFT,
 // keep x < nx.  In place on B.
 template <typename F, int M>
-__global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* __restrict__ G, MeshDims d,
-                                               const KickScalars* __restrict__ sc,
-                                               const cpx<F>* __restrict__ tw) {
+__device__ __forceinline__ void xconv_body(cpx<F>* __restrict__ B, const F* __restrict__ G, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, const int bx, const int nbx) {
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
@@ -232,7 +240,7 @@ __global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* 
   const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
   const int64_t cs = (int64_t)My * Hz;  // stride between x planes
-  const Col c = flat_col<T, kNT>(My * Hz, Hz);  // a = ky, b = kz
+  const Col c = flat_col<T, kNT>(My * Hz, Hz, bx);  // a = ky, b = kz
   cpx<F>* col = B + (int64_t)c.a * Hz + c.b;
   constexpr int NL = per_thread(M * T, kNT);
   cpx<F> v[NL];
@@ -273,18 +281,23 @@ __global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* 
   }
 }
 
+template <typename F, int M>
+__global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* __restrict__ G, MeshDims d,
+                                               const KickScalars* __restrict__ sc,
+                                               const cpx<F>* __restrict__ tw) {
+  xconv_body<F, M>(B, G, d, sc, tw, blockIdx.x, gridDim.x);
+}
+
 // y pass inverse: B[x][ky][kz] -> A[x][y<ny][kz]
 template <typename F, int M>
-__global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<F>* __restrict__ A, MeshDims d,
-                                              const KickScalars* __restrict__ sc,
-                                              const cpx<F>* __restrict__ tw) {
+__device__ __forceinline__ void yinv_body(const cpx<F>* __restrict__ B, cpx<F>* __restrict__ A, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, const int bx, const int nbx) {
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
   const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
-  const Col c = flat_col<T, kNT>(d.nx * Hz, Hz);  // a = x, b = kz
+  const Col c = flat_col<T, kNT>(d.nx * Hz, Hz, bx);  // a = x, b = kz
   const cpx<F>* src = B + (int64_t)c.a * M * Hz + c.b;
   constexpr int NL = per_thread(M * T, kNT);
   cpx<F> v[NL];
@@ -310,18 +323,25 @@ __global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<
   }
 }
 
+template <typename F, int M>
+__global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<F>* __restrict__ A, MeshDims d,
+                                              const KickScalars* __restrict__ sc,
+                                              const cpx<F>* __restrict__ tw) {
+  yinv_body<F, M>(B, A, d, sc, tw, blockIdx.x, gridDim.x);
+}
+
 // z pass inverse: two Hermitian half spectra -> one complex inverse FFT ->
 // two real rows of the potential, z < nz
 template <typename F, int M, bool DERIV, int V = 0>
 __device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __restrict__ phi, const MeshDims& d,
                                           const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
-                                          const cpx<double>* __restrict__ twd, cpx<F>* s) {
+                                          const cpx<double>* __restrict__ twd, cpx<F>* s, const int bx) {
   using TL = Tile<F, M, V>;
   constexpr int T = TL::T, S = TL::S;
   const cpx<F>* twl = stage_tw<F, M, kNT, TL::stw>(s + (size_t)M * S, tw);
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
-  const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
+  const int64_t row0 = (int64_t)bx * (2 * T);
   // T pairs of half spectra (Hz = M/2 + 1 each)
   constexpr int NL = per_thread(T * (M / 2 + 1), kNT);
   cpx<F> a[NL], b[NL];
@@ -366,31 +386,43 @@ __device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __res
   }
 }
 
+template <typename F, int M, bool DERIV>
+__device__ __forceinline__ void zinv_body(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (sc->err) return;
+  zinv_rows<F, M, DERIV>(A, phi, d, sc, tw, twd, (cpx<F>*)smem_raw, bx);
+}
+
 template <typename F, int M, bool DERIV = false>
 __global__ void __launch_bounds__(kNT) k_zinv(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d,
                                               const KickScalars* __restrict__ sc,
                                               const cpx<F>* __restrict__ tw,
                                               const cpx<double>* __restrict__ twd = nullptr) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  if (sc->err) return;
-  zinv_rows<F, M, DERIV>(A, phi, d, sc, tw, twd, (cpx<F>*)smem_raw);
+  zinv_body<F, M, DERIV>(A, phi, d, sc, tw, twd, blockIdx.x, gridDim.x);
 }
 
 // Mixed form: the three field components in one launch (blockIdx.y), the
 // third (z) with the spectral difference factor
+template <int M, int V>
+__device__ __forceinline__ void zinv3_body(const cpx<float>* __restrict__ A, int64_t nyz, float* __restrict__ E,
+                                           int64_t ncell, const MeshDims& d, const KickScalars* __restrict__ sc,
+                                           const cpx<float>* __restrict__ tw, const cpx<double>* __restrict__ twd,
+                                           const int comp, const int bx) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (sc->err) return;
+  cpx<float>* s = (cpx<float>*)smem_raw;
+  if (comp == 2)
+    zinv_rows<float, M, true, V>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s, bx);
+  else
+    zinv_rows<float, M, false, V>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s, bx);
+}
+
 template <int M, int V = 0>
 __global__ void __launch_bounds__(kNT) k_zinv3(const cpx<float>* __restrict__ A, int64_t nyz, float* __restrict__ E,
                                                int64_t ncell, MeshDims d, const KickScalars* __restrict__ sc,
                                                const cpx<float>* __restrict__ tw,
                                                const cpx<double>* __restrict__ twd) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  if (sc->err) return;
-  const int comp = blockIdx.y;
-  cpx<float>* s = (cpx<float>*)smem_raw;
-  if (comp == 2)
-    zinv_rows<float, M, true, V>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s);
-  else
-    zinv_rows<float, M, false, V>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s);
+  zinv3_body<M, V>(A, nyz, E, ncell, d, sc, tw, twd, blockIdx.y, blockIdx.x);
 }
 
 // columns per workgroup of the kernels that carry two or three fields
@@ -414,12 +446,8 @@ struct FieldTile {
 // potential, from which the y and z fields follow by their own factors).
 // The product S*G stays in registers between the two inverse transforms, so
 // LDS holds one tile of T columns (several workgroups per CU).
-template <int M, int V = 0>
-__global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
-                                                     cpx<float>* __restrict__ P, const float* __restrict__ G,
-                                                     MeshDims d, const KickScalars* __restrict__ sc,
-                                                     const cpx<float>* __restrict__ tw,
-                                                     const cpx<double>* __restrict__ twd) {
+template <int M, int V>
+__device__ __forceinline__ void xconv_field_body(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx, cpx<float>* __restrict__ P, const float* __restrict__ G, MeshDims d, const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw, const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
   using F = float;
   constexpr int T = FieldTile<M, kNTF, V>::T, S = FieldTile<M, kNTF, V>::S, PER = FieldTile<M, kNTF, V>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -428,7 +456,7 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
   const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF, V>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
   const int64_t cs = (int64_t)My * Hz;
-  const Col c = flat_col<T, kNTF>(My * Hz, Hz);  // a = ky, b = kz
+  const Col c = flat_col<T, kNTF>(My * Hz, Hz, bx);  // a = ky, b = kz
   const int64_t off = (int64_t)c.a * Hz + c.b;
   {
     cpx<F> u[PER];
@@ -495,16 +523,20 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
   }
 }
 
+template <int M, int V = 0>
+__global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
+                                                     cpx<float>* __restrict__ P, const float* __restrict__ G,
+                                                     MeshDims d, const KickScalars* __restrict__ sc,
+                                                     const cpx<float>* __restrict__ tw,
+                                                     const cpx<double>* __restrict__ twd) {
+  xconv_field_body<M, V>(B, Qx, P, G, d, sc, tw, twd, blockIdx.x, gridDim.x);
+}
+
 // Mixed form, y inverse: Ex' = IFFT_y(Qx), Ey' = IFFT_y(P * Dy), Phi' = IFFT_y(P)
 // (all [nx][ny][Hz]); the z field is Dz * Phi', applied in the z inverse.
 // Three transforms of one LDS tile in turn, P held in registers.
-template <int M, int V = 0>
-__global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restrict__ Qx,
-                                                    const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax,
-                                                    cpx<float>* __restrict__ Ay, cpx<float>* __restrict__ Ap,
-                                                    MeshDims d, const KickScalars* __restrict__ sc,
-                                                    const cpx<float>* __restrict__ tw,
-                                                    const cpx<double>* __restrict__ twd) {
+template <int M, int V>
+__device__ __forceinline__ void yinv_field_body(const cpx<float>* __restrict__ Qx, const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax, cpx<float>* __restrict__ Ay, cpx<float>* __restrict__ Ap, MeshDims d, const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw, const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
   using F = float;
   constexpr int T = FieldTile<M, kNTF, V>::T, S = FieldTile<M, kNTF, V>::S, PER = FieldTile<M, kNTF, V>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -512,7 +544,7 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
   if (sc->err) return;
   const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF, V>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
-  const Col c = flat_col<T, kNTF>(d.nx * Hz, Hz);  // a = x, b = kz
+  const Col c = flat_col<T, kNTF>(d.nx * Hz, Hz, bx);  // a = x, b = kz
   const int64_t src = (int64_t)c.a * M * Hz + c.b;
   const double inv_hy = 1.0 / sc->steps[1];
   cpx<F> p[PER];
@@ -563,6 +595,16 @@ __global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restric
   store(Ap);
 }
 
+template <int M, int V = 0>
+__global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restrict__ Qx,
+                                                    const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax,
+                                                    cpx<float>* __restrict__ Ay, cpx<float>* __restrict__ Ap,
+                                                    MeshDims d, const KickScalars* __restrict__ sc,
+                                                    const cpx<float>* __restrict__ tw,
+                                                    const cpx<double>* __restrict__ twd) {
+  yinv_field_body<M, V>(Qx, P, Ax, Ay, Ap, d, sc, tw, twd, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Integrated Green's function (ocelot/cpbd/sc.py:109-133), FP64.
 // K[i,j,k] = 8-corner alternating difference of F at the cell faces
@@ -578,25 +620,28 @@ __device__ __forceinline__ double igf_F(double x, double y, double z) {
 // neighbours come from L2).  A tiled single pass (4x8x8 cells from a 5x9x9
 // F tile in LDS) evaluates 58% of F twice and leaves most threads idle while
 // the tile is filled.
-__global__ void __launch_bounds__(256) k_igf_nodes(double* __restrict__ Fn, MeshDims d,
-                                                   const KickScalars* __restrict__ sc, int force) {
+__device__ __forceinline__ void igf_nodes_body(double* __restrict__ Fn, MeshDims d, const KickScalars* __restrict__ sc, int force, const int bx, const int nbx) {
   if (sc->err || !(force || sc->rebuild)) return;
   const double hx = sc->steps[0], hy = sc->steps[1], hz = sc->steps[2];
   // 32-bit index arithmetic: meshes have at most 513^3 nodes
   const unsigned sz = d.nz + 1, sy = d.ny + 1;
   const unsigned nn = (unsigned)(d.nx + 1) * sy * sz;
-  for (unsigned f = blockIdx.x * 256u + threadIdx.x; f < nn; f += gridDim.x * 256u) {
+  for (unsigned f = bx * 256u + threadIdx.x; f < nn; f += nbx * 256u) {
     const unsigned row = f / sz, k = f - row * sz, i = row / sy, j = row - i * sy;
     Fn[f] = igf_F(hx * (int)i - hx / 2, hy * (int)j - hy / 2, hz * (int)k - hz / 2);
   }
 }
 
-__global__ void __launch_bounds__(256) k_igf_diff(const double* __restrict__ Fn, double* __restrict__ K, MeshDims d,
-                                                  const KickScalars* __restrict__ sc, int force) {
+__global__ void __launch_bounds__(256) k_igf_nodes(double* __restrict__ Fn, MeshDims d,
+                                                   const KickScalars* __restrict__ sc, int force) {
+  igf_nodes_body(Fn, d, sc, force, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void igf_diff_body(const double* __restrict__ Fn, double* __restrict__ K, MeshDims d, const KickScalars* __restrict__ sc, int force, const int bx, const int nbx) {
   if (sc->err || !(force || sc->rebuild)) return;
   const unsigned nc = (unsigned)d.nx * d.ny * d.nz;
   const unsigned sy = d.nz + 1, sx = (unsigned)(d.ny + 1) * (d.nz + 1);
-  for (unsigned o = blockIdx.x * 256u + threadIdx.x; o < nc; o += gridDim.x * 256u) {
+  for (unsigned o = bx * 256u + threadIdx.x; o < nc; o += nbx * 256u) {
     const unsigned row = o / d.nz, k = o - row * d.nz, i = row / d.ny, j = row - i * d.ny;
     const double* f0 = Fn + (i * sx + j * sy + k);
 #define FAT(p, q, r) f0[(p) * sx + (q) * sy + (r)]
@@ -606,13 +651,16 @@ __global__ void __launch_bounds__(256) k_igf_diff(const double* __restrict__ Fn,
   }
 }
 
+__global__ void __launch_bounds__(256) k_igf_diff(const double* __restrict__ Fn, double* __restrict__ K, MeshDims d,
+                                                  const KickScalars* __restrict__ sc, int force) {
+  igf_diff_body(Fn, K, d, sc, force, blockIdx.x, gridDim.x);
+}
+
 // Even transform along z of the real kernel, two rows per complex FFT:
 // e[m] = K[m] (m < nz), e[M-m] = K[m] (1 <= m < nz); both spectra are real,
 // so row a is the real part and row b the imaginary part of the output.
 template <int M>
-__global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, double* __restrict__ out, MeshDims d,
-                                              const KickScalars* __restrict__ sc,
-                                              const cpx<double>* __restrict__ tw, int force) {
+__device__ __forceinline__ void zcos_body(const double* __restrict__ K, double* __restrict__ out, MeshDims d, const KickScalars* __restrict__ sc, const cpx<double>* __restrict__ tw, int force, const int bx, const int nbx) {
   using F = double;
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -621,7 +669,7 @@ __global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, doub
   const cpx<F>* twl = stage_tw<F, M, kNT, Tile<F, M>::stw>(s + (size_t)M * S, tw);
   const int nz = d.nz, Hz = d.hz;
   const int64_t R = (int64_t)d.nx * d.ny;
-  const int64_t row0 = (int64_t)blockIdx.x * (2 * T);
+  const int64_t row0 = (int64_t)bx * (2 * T);
   for (int i = threadIdx.x; i < M * T; i += kNT) s[(i / T) * S + (i % T)] = cpx<F>{0, 0};
   __syncthreads();
   constexpr int NL = per_thread(T * M, kNT);  // 2T rows of nz <= M/2 values
@@ -659,14 +707,18 @@ __global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, doub
   }
 }
 
+template <int M>
+__global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, double* __restrict__ out, MeshDims d,
+                                              const KickScalars* __restrict__ sc,
+                                              const cpx<double>* __restrict__ tw, int force) {
+  zcos_body<M>(K, out, d, sc, tw, force, blockIdx.x, gridDim.x);
+}
+
 // Even transform along a strided axis of a real array [outer][n][inner] ->
 // [outer][H][inner]; adjacent inner columns are paired into one complex FFT.
 // The last (x) pass scales by gscale and writes the FP32 or FP64 spectrum.
 template <int M, typename FO>
-__global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO* __restrict__ out, int n, int inner,
-                                              int H, int nouter, const KickScalars* __restrict__ sc,
-                                              const cpx<double>* __restrict__ tw, int force, int last,
-                                              double inv_m3) {
+__device__ __forceinline__ void scos_body(const double* __restrict__ in, FO* __restrict__ out, int n, int inner, int H, int nouter, const KickScalars* __restrict__ sc, const cpx<double>* __restrict__ tw, int force, int last, double inv_m3, const int bx, const int nbx) {
   using F = double;
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -677,7 +729,7 @@ __global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO*
   // tile; this thread's column is the same in every loop below
   static_assert(kNT % (2 * T) == 0, "a thread must keep its column in every loop");
   const int cl = threadIdx.x % (2 * T);
-  const int fc = blockIdx.x * 2 * T + cl;
+  const int fc = bx * 2 * T + cl;
   const bool ok = fc < nouter * inner;
   const int o = ok ? fc / inner : 0, col = ok ? fc - o * inner : 0;
   for (int i = threadIdx.x; i < M * T; i += kNT) s[(i / T) * S + (i % T)] = cpx<F>{0, 0};
@@ -716,6 +768,14 @@ __global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO*
     const double r = (cl & 1) ? w.y : w.x;
     out[((int64_t)o * H + k) * inner + col] = (FO)(r * g);
   }
+}
+
+template <int M, typename FO>
+__global__ void __launch_bounds__(kNT) k_scos(const double* __restrict__ in, FO* __restrict__ out, int n, int inner,
+                                              int H, int nouter, const KickScalars* __restrict__ sc,
+                                              const cpx<double>* __restrict__ tw, int force, int last,
+                                              double inv_m3) {
+  scos_body<M, FO>(in, out, n, inner, H, nouter, sc, tw, force, last, inv_m3, blockIdx.x, gridDim.x);
 }
 
 }  // namespace ocsc

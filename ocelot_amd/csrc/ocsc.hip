@@ -2,6 +2,7 @@
 // sequences.  Declarations and contracts are in include/ocsc.h.
 #include "ocsc.h"
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include "lsc.hpp"
 #include "csr.hpp"
 #include "sort.hpp"
+#include "batch.hpp"
 
 using namespace ocsc;
 
@@ -88,8 +90,7 @@ struct KickInputs {
   int32_t ndraws;
 };
 
-__global__ void k_setup(KickScalars* __restrict__ s, KickInputs in) {
-  if (threadIdx.x != 0) return;
+__device__ __forceinline__ void setup_one(KickScalars* __restrict__ s, const KickInputs& in) {
   s->gamref = in.gamref;
   s->betaref = in.betaref;
   s->pref = in.pref;
@@ -126,6 +127,31 @@ __global__ void k_setup(KickScalars* __restrict__ s, KickInputs in) {
     s->key_prec = -1;
     s->inv_qscale = 1.0;
   }
+}
+
+__global__ void k_setup(KickScalars* __restrict__ s, KickInputs in) {
+  if (threadIdx.x == 0) setup_one(s, in);
+}
+
+// batched calls: the set-up of every job (thread j: job j)
+__global__ void kb_setup(const KickInputs* __restrict__ in, const BJob* __restrict__ J, int nb) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nb) setup_one(J[j].sc, in[j]);
+}
+
+// error bits of all jobs, or-ed; clear: reset them afterwards
+__global__ void kb_errors(const BJob* __restrict__ J, int nb, int* __restrict__ out, int clear) {
+  int e = 0;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+    KickScalars* s = J[j].sc;
+    e |= s->err | s->err_sticky;
+    if (clear) {
+      s->err = 0;
+      s->err_sticky = 0;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) e |= __shfl_down(e, off, 64);
+  if (threadIdx.x == 0) *out = e;
 }
 
 // x += L x', y += L y', tau += r56 delta (numpy.dot(R, X) accumulates with fma)
@@ -221,6 +247,11 @@ struct ocsc_ctx {
     Buf rs, qs, perm;
   };
   std::vector<BunchCopy*> bcopy;
+  // batched calls in one launch per pass (option "batch_fused"): a workspace
+  // per bunch and the device tables of jobs, inputs and launch segments
+  int batch_fused = 1;
+  std::vector<Lane*> flanes;
+  Buf btab;
   // tracking workspace: two cell-ordered copies of the bunch, sort buffers,
   // particle records for the permutations
   Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp, rec;
@@ -532,10 +563,13 @@ KickInputs kick_inputs(const int32_t* nmesh, int64_t n, double E_GeV, double zst
 }
 
 // grid of the strided reduction passes (A, B, moments): one full round of
-// resident workgroups (no tail of half-empty rounds), at most one per 256
-// particles
+// resident workgroups (no tail of half-empty rounds), at most one per 8 * 256
+// particles (every thread sums several particles before the workgroup's
+// reduction tree, which costs more than one particle's transform: batched
+// passes over bunches of a few 1e5 particles ran at half the bandwidth)
+constexpr int64_t kRedParticles = 8 * kPT;
 int nred_blocks(const ocsc_ctx* c, int64_t n) {
-  int64_t b = (n + kPT - 1) / kPT;
+  int64_t b = (n + kRedParticles - 1) / kRedParticles;
   return (int)(b < c->red_blocks ? (b < 1 ? 1 : b) : c->red_blocks);
 }
 
@@ -1056,6 +1090,11 @@ int ocsc_destroy(ocsc_ctx* c) {
     free_lane(*l);
     delete l;
   }
+  for (Lane* l : c->flanes) {
+    free_lane(*l);
+    delete l;
+  }
+  if (c->btab.p) (void)hipFree(c->btab.p);
   for (ocsc_ctx::BunchCopy* bc : c->bcopy) {
     for (Buf* b : {&bc->rs, &bc->qs, &bc->perm})
       if (b->p) (void)hipFree(b->p);
@@ -1990,6 +2029,320 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
 
 }  // namespace
 
+namespace {
+
+// Launch tables of a fused batched round: for every pass, the jobs it covers
+// and the prefix of their workgroup counts; the transform passes once per
+// FFT length along their axis.
+struct FusedPlan {
+  struct Launch {
+    BSeg seg{};
+    int64_t grid = 0;
+    int M = 0;
+  };
+  int nb = 0;
+  BJob* jobs = nullptr;
+  KickInputs* ins = nullptr;
+  int* err = nullptr;
+  Launch pab, dep, kick, zero, nodes, diff, d64;
+  std::vector<Launch> zcos, scos_y, scos_x, zf, yf, xc, yi, zi;
+};
+
+// columns per workgroup of the transform kernels (host side of Tile)
+int tile_cols(bool dbl, int M) {
+  switch (M) {
+#define OCSC_TC(V) \
+  case V:          \
+    return dbl ? Tile<double, V>::T : Tile<float, V>::T;
+    OCSC_TC(8) OCSC_TC(16) OCSC_TC(32) OCSC_TC(64) OCSC_TC(128) OCSC_TC(256) OCSC_TC(512) OCSC_TC(1024)
+#undef OCSC_TC
+    default:
+      return 0;
+  }
+}
+
+int dep_items(int tile) { return tile == 1 ? DepTile<1>::items : (tile == 2 ? DepTile<2>::items : DepTile<0>::items); }
+
+// Build the jobs (one per kicked bunch: rows rr[i], workspace c->flanes[i])
+// and the launch tables, and upload them; the tables stay valid while the
+// rows, sizes, meshes, options and workspaces are unchanged.
+int fused_plan(ocsc_ctx* c, const std::vector<int>& kicked, const std::vector<double*>& rr,
+               const std::vector<int64_t>& ldr, const std::vector<const double*>& qq, const std::vector<int64_t>& nn,
+               const std::vector<Geometry>& geo, const std::vector<KickInputs>& ins, int prec, hipStream_t st,
+               FusedPlan* P) {
+  const bool dbl = prec == OCSC_FP64;
+  const int nword = dbl ? 2 : 1;
+  const int ncopy = c->deposit_xcd ? kNumXcd : 1;
+  const int nbk = (int)kicked.size();
+  std::vector<BJob> jobs(nbk);
+  std::vector<KickInputs> jin(nbk);
+  for (int j = 0; j < nbk; ++j) {
+    const int i = kicked[j];
+    Lane& L = *c->flanes[j];
+    const Geometry& g = geo[i];
+    BJob& b = jobs[j];
+    b.r = rr[i];
+    b.ld = ldr[i];
+    b.q = qq[i];
+    b.n = nn[i];
+    b.sc = L.sc;
+    b.part = (double*)L.part.p;
+    b.nblk = nred_blocks(c, nn[i]);
+    b.d = g.d;
+    b.ncell = g.ncell;
+    b.nzero = (int64_t)ncopy * nword * g.ncell;
+    b.grid = (unsigned long long*)L.grid.p;
+    b.A = L.A.p;
+    b.B = L.B.p;
+    b.KF = (double*)L.KF.p;
+    b.KA = (double*)L.KA.p;
+    b.KB = (double*)L.KB.p;
+    b.G = L.G.p;
+    b.phi = L.phi.p;
+    b.E64 = (double*)L.E64.p;
+    jin[j] = ins[i];
+  }
+  // segments: host ids and prefixes, packed after the jobs and inputs
+  std::vector<int> ids;
+  std::vector<int64_t> pre, cnt;
+  struct Pend {
+    FusedPlan::Launch* l;
+    size_t ids_at, pre_at;
+  };
+  std::vector<Pend> pend;
+  // align: every job starts at a multiple of 8 workgroups (the hardware hands
+  // workgroup b to XCD b mod 8, and the deposit and the kick give each XCD a
+  // contiguous run of a job's particles by that rule); the padding
+  // workgroups exit at once
+  auto add = [&](FusedPlan::Launch* l, const std::vector<int>& js, const std::vector<int64_t>& blocks, int M,
+                 bool align = false) {
+    l->M = M;
+    l->seg.nj = (int)js.size();
+    pend.push_back({l, ids.size(), pre.size()});
+    int64_t acc = 0;
+    pre.push_back(0);
+    for (size_t k = 0; k < js.size(); ++k) {
+      ids.push_back(js[k]);
+      cnt.push_back(blocks[k]);
+      acc += align ? (blocks[k] + kNumXcd - 1) / kNumXcd * kNumXcd : blocks[k];
+      pre.push_back(acc);
+    }
+    l->grid = acc;
+  };
+  auto all = [&](auto blocks_of, FusedPlan::Launch* l, bool align = false) {
+    std::vector<int> js(nbk);
+    std::vector<int64_t> bl(nbk);
+    for (int j = 0; j < nbk; ++j) {
+      js[j] = j;
+      bl[j] = blocks_of(jobs[j]);
+    }
+    add(l, js, bl, 0, align);
+  };
+  // per FFT length along one axis
+  auto by_m = [&](auto m_of, auto blocks_of, std::vector<FusedPlan::Launch>* out) {
+    std::vector<int> ms;
+    for (int j = 0; j < nbk; ++j)
+      if (std::find(ms.begin(), ms.end(), m_of(jobs[j].d)) == ms.end()) ms.push_back(m_of(jobs[j].d));
+    out->assign(ms.size(), FusedPlan::Launch{});
+    for (size_t k = 0; k < ms.size(); ++k) {
+      std::vector<int> js;
+      std::vector<int64_t> bl;
+      for (int j = 0; j < nbk; ++j)
+        if (m_of(jobs[j].d) == ms[k]) {
+          js.push_back(j);
+          bl.push_back(blocks_of(jobs[j], ms[k]));
+        }
+      add(&(*out)[k], js, bl, ms[k]);
+    }
+  };
+  const int items = dep_items(c->deposit_tile);
+  all([&](const BJob& b) { return (int64_t)b.nblk; }, &P->pab);
+  all([&](const BJob& b) { return (b.n + kDepPT * items - 1) / (kDepPT * items); }, &P->dep, true);
+  all([&](const BJob& b) { return (b.n + kKickPT - 1) / kKickPT; }, &P->kick, true);
+  all([&](const BJob& b) { return std::min<int64_t>((b.nzero + 255) / 256, 8192); }, &P->zero);
+  all([&](const BJob& b) {
+    const int64_t nodes = (int64_t)(b.d.nx + 1) * (b.d.ny + 1) * (b.d.nz + 1);
+    return std::min<int64_t>((nodes + 255) / 256, 65536);
+  }, &P->nodes);
+  all([&](const BJob& b) { return std::min<int64_t>((b.ncell + 255) / 256, 65536); }, &P->diff);
+  all([&](const BJob& b) { return (b.ncell + 255) / 256; }, &P->d64);
+  auto mz = [](const MeshDims& d) { return d.mz; };
+  auto my = [](const MeshDims& d) { return d.my; };
+  auto mx = [](const MeshDims& d) { return d.mx; };
+  auto rows_pairs = [&](bool d64) {
+    return [&, d64](const BJob& b, int M) {
+      const int T = tile_cols(d64, M);
+      return ((int64_t)b.d.nx * b.d.ny + 2 * T - 1) / (2 * T);
+    };
+  };
+  by_m(mz, rows_pairs(true), &P->zcos);
+  by_m(my, [&](const BJob& b, int M) {
+    const int T = tile_cols(true, M);
+    return ((int64_t)b.d.nx * b.d.hz + 2 * T - 1) / (2 * T);
+  }, &P->scos_y);
+  by_m(mx, [&](const BJob& b, int M) {
+    const int T = tile_cols(true, M);
+    return ((int64_t)b.d.hy * b.d.hz + 2 * T - 1) / (2 * T);
+  }, &P->scos_x);
+  by_m(mz, rows_pairs(dbl), &P->zf);
+  by_m(my, [&](const BJob& b, int M) {
+    const int T = tile_cols(dbl, M);
+    return ((int64_t)b.d.nx * b.d.hz + T - 1) / T;
+  }, &P->yf);
+  by_m(mx, [&](const BJob& b, int M) {
+    const int T = tile_cols(dbl, M);
+    return ((int64_t)b.d.my * b.d.hz + T - 1) / T;
+  }, &P->xc);
+  by_m(my, [&](const BJob& b, int M) {
+    const int T = tile_cols(dbl, M);
+    return ((int64_t)b.d.nx * b.d.hz + T - 1) / T;
+  }, &P->yi);
+  by_m(mz, rows_pairs(dbl), &P->zi);
+  // one device table: jobs, inputs, error word, ids, prefixes
+  auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t o_in = al(nbk * sizeof(BJob));
+  const size_t o_err = o_in + al(nbk * sizeof(KickInputs));
+  const size_t o_ids = o_err + 256;
+  const size_t o_pre = o_ids + al(ids.size() * sizeof(int));
+  const size_t o_cnt = o_pre + al(pre.size() * sizeof(int64_t));
+  const size_t total = o_cnt + cnt.size() * sizeof(int64_t);
+  int rc;
+  if ((rc = ensure(c->btab, total))) return rc;
+  std::vector<unsigned char> host(total, 0);
+  std::memcpy(host.data(), jobs.data(), nbk * sizeof(BJob));
+  std::memcpy(host.data() + o_in, jin.data(), nbk * sizeof(KickInputs));
+  std::memcpy(host.data() + o_ids, ids.data(), ids.size() * sizeof(int));
+  std::memcpy(host.data() + o_pre, pre.data(), pre.size() * sizeof(int64_t));
+  std::memcpy(host.data() + o_cnt, cnt.data(), cnt.size() * sizeof(int64_t));
+  HIPCHK(hipMemcpyAsync(c->btab.p, host.data(), total, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  unsigned char* base = (unsigned char*)c->btab.p;
+  P->nb = nbk;
+  P->jobs = (BJob*)base;
+  P->ins = (KickInputs*)(base + o_in);
+  P->err = (int*)(base + o_err);
+  for (const Pend& q : pend) {
+    q.l->seg.ids = (const int*)(base + o_ids) + q.ids_at;
+    q.l->seg.pre = (const int64_t*)(base + o_pre) + q.pre_at;
+    q.l->seg.cnt = (const int64_t*)(base + o_cnt) + q.ids_at;
+  }
+  return OCSC_OK;
+}
+
+template <int NWORD>
+int launch_deposit_b(const ocsc_ctx* c, const FusedPlan& P, hipStream_t st) {
+  const int v = (c->deposit_xcd ? 4 : 0) + (c->deposit_agg ? 2 : 0);
+  const int t = c->deposit_tile;
+#define OCSC_DEPB(V, T)                                                                                   \
+  if (v == V && t == T) {                                                                                 \
+    kb_deposit<NWORD, (V & 4) != 0, (V & 2) != 0, T><<<(unsigned)P.dep.grid, kDepPT, 0, st>>>(P.jobs, P.dep.seg); \
+    return OCSC_OK;                                                                                       \
+  }
+#define OCSC_DEPB4(V) OCSC_DEPB(V, 0) OCSC_DEPB(V, 1) OCSC_DEPB(V, 2) OCSC_DEPB(V, 3)
+  OCSC_DEPB4(0) OCSC_DEPB4(2) OCSC_DEPB4(4) OCSC_DEPB4(6)
+#undef OCSC_DEPB4
+#undef OCSC_DEPB
+  return fail(OCSC_EINVAL, "deposit options out of range");
+}
+
+// One kick of every job: the passes of enqueue_kick, each one launch (per
+// FFT length for the transforms) over all jobs.  Kernel spectrum in line.
+template <typename F, int NWORD>
+int fused_round(ocsc_ctx* c, const FusedPlan& P, int prec, hipStream_t st) {
+  const BJob* J = P.jobs;
+  kb_setup<<<(P.nb + 63) / 64, 64, 0, st>>>(P.ins, J, P.nb);
+  kb_passA<<<(unsigned)P.pab.grid, kPT, 0, st>>>(J, P.pab.seg);
+  kb_reduceA<<<P.nb, 1024, 0, st>>>(J);
+  if (prec == OCSC_MIXED)
+    kb_passB<true><<<(unsigned)P.pab.grid, kPT, 0, st>>>(J, P.pab.seg);
+  else
+    kb_passB<false><<<(unsigned)P.pab.grid, kPT, 0, st>>>(J, P.pab.seg);
+  kb_reduceB<<<P.nb, 1024, 0, st>>>(J, prec);
+  kb_zero<<<(unsigned)P.zero.grid, 256, 0, st>>>(J, P.zero.seg);
+  LAUNCHCHK("batched frame/extent passes");
+  kb_igf_nodes<<<(unsigned)P.nodes.grid, 256, 0, st>>>(J, P.nodes.seg);
+  kb_igf_diff<<<(unsigned)P.diff.grid, 256, 0, st>>>(J, P.diff.seg);
+  for (const auto& l : P.zcos) {
+    OCSC_SWITCH_M(l.M, {
+      kb_zcos<MM><<<(unsigned)l.grid, kNT, Tile<double, MM>::lds_bytes, st>>>(J, l.seg, c->twd + MM);
+    });
+  }
+  for (const auto& l : P.scos_y) {
+    OCSC_SWITCH_M(l.M, {
+      kb_scos_y<MM><<<(unsigned)l.grid, kNT, Tile<double, MM>::lds_bytes, st>>>(J, l.seg, c->twd + MM);
+    });
+  }
+  for (const auto& l : P.scos_x) {
+    OCSC_SWITCH_M(l.M, {
+      kb_scos_x<MM, F><<<(unsigned)l.grid, kNT, Tile<double, MM>::lds_bytes, st>>>(J, l.seg, c->twd + MM);
+    });
+  }
+  LAUNCHCHK("batched kernel spectrum");
+  int rc;
+  if ((rc = launch_deposit_b<NWORD>(c, P, st))) return rc;
+  LAUNCHCHK("batched deposit");
+  const int ncopy = c->deposit_xcd ? kNumXcd : 1;
+  for (const auto& l : P.zf) {
+    OCSC_SWITCH_M(l.M, {
+      kb_zfwd<F, MM, NWORD == 2 ? kChargeFix2 : kChargeFix1>
+          <<<(unsigned)l.grid, kNT, Tile<F, MM>::lds_bytes, st>>>(J, l.seg, tw_for<F>(c, MM), ncopy);
+    });
+  }
+  for (const auto& l : P.yf) {
+    OCSC_SWITCH_M(l.M, {
+      kb_yfwd<F, MM><<<(unsigned)l.grid, kNT, Tile<F, MM>::lds_bytes, st>>>(J, l.seg, tw_for<F>(c, MM));
+    });
+  }
+  if constexpr (sizeof(F) == 4) {
+    for (const auto& l : P.xc) {
+      OCSC_SWITCH_M(l.M, {
+        kb_xconv_field<MM><<<(unsigned)l.grid, kNTF, FieldTile<MM, kNTF, 0>::lds, st>>>(J, l.seg, c->twf + MM,
+                                                                                       c->twd + MM);
+      });
+    }
+    for (const auto& l : P.yi) {
+      OCSC_SWITCH_M(l.M, {
+        kb_yinv_field<MM><<<(unsigned)l.grid, kNTF, FieldTile<MM, kNTF, 0>::lds, st>>>(J, l.seg, c->twf + MM,
+                                                                                      c->twd + MM);
+      });
+    }
+    for (const auto& l : P.zi) {
+      OCSC_SWITCH_M(l.M, {
+        kb_zinv3<MM><<<dim3((unsigned)l.grid, 3), kNT, Tile<float, MM>::lds_bytes, st>>>(J, l.seg, c->twf + MM,
+                                                                                         c->twd + MM);
+      });
+    }
+  } else {
+    for (const auto& l : P.xc) {
+      OCSC_SWITCH_M(l.M, {
+        kb_xconv64<MM><<<(unsigned)l.grid, kNT, Tile<double, MM>::lds_bytes, st>>>(J, l.seg, c->twd + MM);
+      });
+    }
+    for (const auto& l : P.yi) {
+      OCSC_SWITCH_M(l.M, {
+        kb_yinv64<MM><<<(unsigned)l.grid, kNT, Tile<double, MM>::lds_bytes, st>>>(J, l.seg, c->twd + MM);
+      });
+    }
+    for (const auto& l : P.zi) {
+      OCSC_SWITCH_M(l.M, {
+        kb_zinv64<MM><<<(unsigned)l.grid, kNT, Tile<double, MM>::lds_bytes, st>>>(J, l.seg, c->twd + MM);
+      });
+    }
+    kb_diff64<<<(unsigned)P.d64.grid, 256, 0, st>>>(J, P.d64.seg);
+  }
+  LAUNCHCHK("batched transforms");
+  kb_kick<F><<<(unsigned)P.kick.grid, kKickPT, 0, st>>>(J, P.kick.seg);
+  LAUNCHCHK("batched kick");
+  return OCSC_OK;
+}
+
+int fused_round_prec(ocsc_ctx* c, const FusedPlan& P, int prec, hipStream_t st) {
+  if (prec == OCSC_FP64) return fused_round<double, 2>(c, P, prec, st);
+  return fused_round<float, 1>(c, P, prec, st);
+}
+
+}  // namespace
+
 extern "C" {
 
 int ocsc_track_drift_lsc(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, const int32_t nmesh[3],
@@ -2218,9 +2571,61 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
     c->profiling = prof;
     c->overlap = ovl;
   };
+  // Fused rounds: every pass one launch over all kicked bunches, each bunch
+  // with a workspace of its own (the copying first round stays per bunch)
+  FusedPlan plan;
+  const bool fused = c->batch_fused != 0;
+  if (fused) {
+    std::vector<int> kicked;
+    for (int i = 0; i < nb; ++i)
+      if (b[i].zstep != 0) kicked.push_back(i);
+    if (kicked.empty()) {
+      restore();
+      return OCSC_OK;
+    }
+    while (c->flanes.size() < kicked.size()) {
+      Lane* l = new Lane();
+      if ((rc = init_lane(*l, false))) {
+        free_lane(*l);
+        delete l;
+        restore();
+        return rc;
+      }
+      c->flanes.push_back(l);
+    }
+    std::vector<double*> rr(nb);
+    std::vector<int64_t> ldr(nb), nn(nb);
+    std::vector<const double*> qq(nb);
+    for (size_t j = 0; j < kicked.size(); ++j) {
+      const int i = kicked[j];
+      if ((rc = ensure_prec(c, *c->flanes[j], geo[i], prec)) || (rc = ensure_part(c, *c->flanes[j], b[i].n))) {
+        restore();
+        return rc;
+      }
+    }
+    for (int i = 0; i < nb; ++i) {
+      nn[i] = b[i].n;
+      if (sorted[i]) {
+        rr[i] = sbs[i].rs;
+        ldr[i] = b[i].n;
+        qq[i] = sbs[i].qs;
+      } else {
+        rr[i] = b[i].rparticles;
+        ldr[i] = b[i].ld;
+        qq[i] = b[i].q_array;
+      }
+    }
+    if ((rc = fused_plan(c, kicked, rr, ldr, qq, nn, geo, ins, prec, st, &plan))) {
+      restore();
+      return rc;
+    }
+  }
+  auto later_round = [&](hipStream_t origin) -> int {
+    return fused ? fused_round_prec(c, plan, prec, origin) : round(origin, false);
+  };
   if (!use_graph || any_sorted) {
-    // the copying round eagerly; with graphs the later rounds are captured
-    if ((rc = round(st, true))) {
+    // the copying round eagerly (per bunch); with graphs the later rounds are captured
+    if ((rc = any_sorted || !fused ? round(st, true) : later_round(st))) {
       restore();
       return rc;
     }
@@ -2228,7 +2633,7 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
   const int first_graph_round = (!use_graph || any_sorted) ? 1 : 0;
   if (!use_graph) {
     for (int k = 1; k < nkicks; ++k)
-      if ((rc = round(st, false))) {
+      if ((rc = later_round(st))) {
         restore();
         return rc;
       }
@@ -2240,7 +2645,7 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
       restore();
       return fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
     }
-    int r2 = round(c->cap_stream, !any_sorted);
+    int r2 = fused ? later_round(c->cap_stream) : round(c->cap_stream, !any_sorted);
     e = hipStreamEndCapture(c->cap_stream, &graph);
     if (r2 || e != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -2262,6 +2667,14 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
   for (int l = 0; l < nl; ++l) {
     int32_t err = 0;
     if ((rc = read_errors(c, *c->lanes[l], st, &err))) return rc;
+    err_all |= err;
+  }
+  if (fused) {
+    int32_t err = 0;
+    kb_errors<<<1, 64, 0, st>>>(plan.jobs, plan.nb, plan.err, 1);
+    LAUNCHCHK("batched errors");
+    HIPCHK(hipMemcpyAsync(&err, plan.err, sizeof(err), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     err_all |= err;
   }
   if (err_all) return value_error(err_all);
@@ -2306,6 +2719,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
   if (n == "apply_sort") {
     if (value < 0 || value > 2) return fail(OCSC_EINVAL, "apply_sort must be 0, 1 or 2");
     c->apply_sort = (int)value;
+    return OCSC_OK;
+  }
+  if (n == "batch_fused") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "batch_fused must be 0 or 1");
+    c->batch_fused = (int)value;
     return OCSC_OK;
   }
   if (n == "apply_graph") {
@@ -2380,6 +2798,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "apply_graph") {
     *value = c->apply_graph;
+    return OCSC_OK;
+  }
+  if (n == "batch_fused") {
+    *value = c->batch_fused;
     return OCSC_OK;
   }
   if (n == "deposit_tile") {

@@ -229,12 +229,12 @@ __global__ void __launch_bounds__(256) k_fold(const double* __restrict__ part, i
 }
 
 // pass A: per-block sums of the momenta (sc.py:224 numerator)
-__global__ void __launch_bounds__(kPT) k_passA(Rows R, KickScalars* __restrict__ scp, double* __restrict__ part) {
+__device__ __forceinline__ void passA_body(Rows R, KickScalars* __restrict__ scp, double* __restrict__ part, const int bx, const int nbx) {
   __shared__ double lds[(kPT / 64) * 3];
   const KickScalars s = *scp;
   double acc[3] = {0, 0, 0};
   const int64_t n = s.n;
-  for (int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x; p < n; p += (int64_t)gridDim.x * kPT) {
+  for (int64_t p = (int64_t)bx * kPT + threadIdx.x; p < n; p += (int64_t)nbx * kPT) {
     double m[3];
     to_p(R.r[R.ld + p], R.r[3 * R.ld + p], R.r[5 * R.ld + p], s, m);
     acc[0] += m[0];
@@ -243,12 +243,15 @@ __global__ void __launch_bounds__(kPT) k_passA(Rows R, KickScalars* __restrict__
   }
   block_reduce<3, 0>(acc, lds);
   if (threadIdx.x == 0)
-    for (int c = 0; c < 3; ++c) part[c * gridDim.x + blockIdx.x] = acc[c];
+    for (int c = 0; c < 3; ++c) part[c * nbx + bx] = acc[c];
+}
+
+__global__ void __launch_bounds__(kPT) k_passA(Rows R, KickScalars* __restrict__ scp, double* __restrict__ part) {
+  passA_body(R, scp, part, blockIdx.x, gridDim.x);
 }
 
 // frame from the mean momentum (sc.py:224-239)
-__global__ void __launch_bounds__(1024) k_reduceA(KickScalars* __restrict__ scp, const double* __restrict__ part,
-                                                  int nblk) {
+__device__ __forceinline__ void reduceA_body(KickScalars* __restrict__ scp, const double* __restrict__ part, int nblk, const int bx, const int nbx) {
   __shared__ double lds[16 * 3];
   double acc[3];
   reduce_partials<3, 0>(part, nblk, acc, lds);
@@ -276,10 +279,14 @@ __global__ void __launch_bounds__(1024) k_reduceA(KickScalars* __restrict__ scp,
   if (!(Pav > 0) || !__builtin_isfinite(Pav)) s.err |= kErrNonFinite;
 }
 
+__global__ void __launch_bounds__(1024) k_reduceA(KickScalars* __restrict__ scp, const double* __restrict__ part,
+                                                  int nblk) {
+  reduceA_body(scp, part, nblk, blockIdx.x, gridDim.x);
+}
+
 // pass B: rest-frame min/max, total charge, charge-weighted sums
 template <bool FAST>
-__global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
-                                               double* __restrict__ part) {
+__device__ __forceinline__ void passB_body(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp, double* __restrict__ part, const int bx, const int nbx) {
   __shared__ double lds[(kPT / 64) * kNPartB];
   const KickScalars s = *scp;
   const double inf = __builtin_inf();
@@ -289,7 +296,7 @@ __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict_
   const int64_t n = s.n;
   // two particles per iteration, both loaded before either is used (more
   // loads in flight); accumulated in the order of the plain strided loop
-  const int64_t stride = (int64_t)gridDim.x * kPT;
+  const int64_t stride = (int64_t)nbx * kPT;
   auto body = [&](const double* v, double qq) {
     double xp[6], xyz[3];
     if constexpr (FAST)
@@ -309,7 +316,7 @@ __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict_
     sq += qq;
     saq += __builtin_fabs(qq);
   };
-  for (int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x; p < n; p += 2 * stride) {
+  for (int64_t p = (int64_t)bx * kPT + threadIdx.x; p < n; p += 2 * stride) {
     const int64_t p2 = p + stride;
     const bool two = p2 < n;
     double a[6], b[6], qa, qb = 0;
@@ -328,13 +335,18 @@ __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict_
   double vals[kNPartB] = {mn[0], mn[1], mn[2], mx[0], mx[1], mx[2], sq, saq, sqx[0], sqx[1], sqx[2]};
   block_reduce<kNPartB, 1>(vals, lds);
   if (threadIdx.x == 0)
-    for (int c = 0; c < kNPartB; ++c) part[c * gridDim.x + blockIdx.x] = vals[c];
+    for (int c = 0; c < kNPartB; ++c) part[c * nbx + bx] = vals[c];
+}
+
+template <bool FAST>
+__global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
+                                               double* __restrict__ part) {
+  passB_body<FAST>(R, q, scp, part, blockIdx.x, gridDim.x);
 }
 
 // cell sizes, grid offset, charge scale and kernel-cache decision
 // (sc.py:172-186)
-__global__ void __launch_bounds__(1024) k_reduceB(KickScalars* __restrict__ scp, const double* __restrict__ part,
-                                                  int nblk, int prec) {
+__device__ __forceinline__ void reduceB_body(KickScalars* __restrict__ scp, const double* __restrict__ part, int nblk, int prec, const int bx, const int nbx) {
   __shared__ double lds[16 * kNPartB];
   double out[kNPartB];
   reduce_partials<kNPartB, 1>(part, nblk, out, lds);
@@ -381,6 +393,11 @@ __global__ void __launch_bounds__(1024) k_reduceB(KickScalars* __restrict__ scp,
   }
   if (err) s.rebuild = 1;
   s.err = err;
+}
+
+__global__ void __launch_bounds__(1024) k_reduceB(KickScalars* __restrict__ scp, const double* __restrict__ part,
+                                                  int nblk, int prec) {
+  reduceB_body(scp, part, nblk, prec, blockIdx.x, gridDim.x);
 }
 
 // pass C: nearest-grid-point deposit as exact fixed-point sums (sc.py:187-193).
@@ -500,9 +517,8 @@ __device__ __forceinline__ void hash_add(int* hkey, long long (*hval)[kSlots], u
 // cells in lane order are summed first (segmented inclusive scan with
 // shuffles, exact in integers); only the last lane of each run goes to the
 // table.  Cells that recur in separate runs are simply added twice.
-template <int NWORD, bool XCD = false, bool AGG = false, int TILE = 0>
-__global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
-                                                 unsigned long long* __restrict__ grid, int32_t* __restrict__ cells) {
+template <int NWORD, bool XCD, bool AGG, int TILE>
+__device__ __forceinline__ void deposit_body(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp, unsigned long long* __restrict__ grid, int32_t* __restrict__ cells, const int bx, const int nbx) {
   constexpr int kSlots = DepTile<TILE>::template slots<NWORD>();
   constexpr int kItems = DepTile<TILE>::items;
   __shared__ int hkey[kSlots];
@@ -522,7 +538,7 @@ __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __rest
     }
     __syncthreads();
   }
-  const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (kDepPT * kItems);
+  const int64_t base = xcd_block(bx, nbx) * (kDepPT * kItems);
   const int lane = threadIdx.x & 63;
   int bad = 0;
   // the rows of item it + 1 are loaded before item it goes through the table
@@ -598,6 +614,12 @@ __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __rest
       if (k >= 0) flush_cell<XCD>(g, ncell, k, hval[0][i], NWORD == 2 ? hval[NWORD - 1][i] : 0, NWORD);
     }
   }
+}
+
+template <int NWORD, bool XCD = false, bool AGG = false, int TILE = 0>
+__global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
+                                                 unsigned long long* __restrict__ grid, int32_t* __restrict__ cells) {
+  deposit_body<NWORD, XCD, AGG, TILE>(R, q, scp, grid, cells, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1018,11 +1040,10 @@ __device__ __forceinline__ void gather_field(const F* __restrict__ phi, const Ki
 // from the potential, the arrays the reference interpolates (sc.py:198-204);
 // the kick then reads 8 corner values per component instead of 16 potential
 // values and divides nothing.  Stored one after the other, C order.
-__global__ void __launch_bounds__(256) k_diff64(const double* __restrict__ phi, const KickScalars* __restrict__ scp,
-                                                double* __restrict__ E) {
+__device__ __forceinline__ void diff64_body(const double* __restrict__ phi, const KickScalars* __restrict__ scp, double* __restrict__ E, const int bx, const int nbx) {
   const int nx = scp->nmesh[0], ny = scp->nmesh[1], nz = scp->nmesh[2];
   const int64_t nc = (int64_t)nx * ny * nz;
-  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t o = (int64_t)bx * 256 + threadIdx.x;
   if (o >= nc) return;
   const int64_t sx = (int64_t)ny * nz;
   const int i = (int)(o / sx), j = (int)((o / nz) % ny), k = (int)(o % nz);
@@ -1030,6 +1051,11 @@ __global__ void __launch_bounds__(256) k_diff64(const double* __restrict__ phi, 
   E[o] = i < nx - 1 ? (p0 - phi[o + sx]) / scp->steps[0] : 0.0;
   E[nc + o] = j < ny - 1 ? (p0 - phi[o + nz]) / scp->steps[1] : 0.0;
   E[2 * nc + o] = k < nz - 1 ? (p0 - phi[o + 1]) / scp->steps[2] : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_diff64(const double* __restrict__ phi, const KickScalars* __restrict__ scp,
+                                                double* __restrict__ E) {
+  diff64_body(phi, scp, E, blockIdx.x, gridDim.x);
 }
 
 // trilinear interpolation of a k_diff64 component, in gather_diff's order of
@@ -1092,17 +1118,13 @@ constexpr int kKickPT = 256;
 // RECOUT: the rows are written as records to rec[perm[p]] (the caller's
 // position of the cell-ordered particle p) instead of in place: the last kick
 // of a tracking call and the way back to the caller's order in one pass.
-template <typename F, bool DRIFT = false, bool SUMP = false, bool RECOUT = false>
-__global__ void __launch_bounds__(kKickPT) k_kick(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
-                                                  const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
-                                                  double* __restrict__ part = nullptr,
-                                                  const uint32_t* __restrict__ perm = nullptr,
-                                                  PRec* __restrict__ rec = nullptr) {
+template <typename F, bool DRIFT, bool SUMP, bool RECOUT>
+__device__ __forceinline__ void kick_body(double* __restrict__ r, int64_t ld, const F* __restrict__ phi, const KickScalars* __restrict__ scp, DriftArgs dr, double* __restrict__ part, const uint32_t* __restrict__ perm, PRec* __restrict__ rec, const int bx, const int nbx) {
   __shared__ double lds[SUMP ? (kKickPT / 64) * 3 : 1];
   const KickScalars& s0 = *scp;
   if (s0.err) return;  // uniform across the grid
   const KickScalars s = s0;
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t blk = xcd_block(bx, nbx);
   const int64_t p = blk * kKickPT + threadIdx.x;
   const bool active = p < s.n;
   // both forms take the fast transforms (to_xp_fast, to_ocelot_fast: the
@@ -1166,8 +1188,17 @@ __global__ void __launch_bounds__(kKickPT) k_kick(double* __restrict__ r, int64_
   if constexpr (SUMP) {
     block_reduce<3, 0>(m, lds);
     if (threadIdx.x == 0)
-      for (int c = 0; c < 3; ++c) part[c * gridDim.x + blk] = m[c];
+      for (int c = 0; c < 3; ++c) part[c * nbx + blk] = m[c];
   }
+}
+
+template <typename F, bool DRIFT = false, bool SUMP = false, bool RECOUT = false>
+__global__ void __launch_bounds__(kKickPT) k_kick(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
+                                                  const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
+                                                  double* __restrict__ part = nullptr,
+                                                  const uint32_t* __restrict__ perm = nullptr,
+                                                  PRec* __restrict__ rec = nullptr) {
+  kick_body<F, DRIFT, SUMP, RECOUT>(r, ld, phi, scp, dr, part, perm, rec, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -833,6 +833,62 @@ def test_batched_equals_single_kicks(prec):
         _lib.set_option(dev, "batch_sort", old)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fused_batch_equals_per_bunch_launches(prec):
+    """Batched kicks with one launch per pass over all bunches (option
+    batch_fused) equal the per-bunch launch sequences bit for bit: bunches of
+    uneven size on cubic and non-cubic meshes (several FFT lengths per axis,
+    so several transform launches per pass), with and without cell-ordered
+    copies, eager and in graphs, two rounds; and one round with random-mesh
+    draws."""
+    from ocelot_amd import DeviceParticleArray, _lib, apply_batched
+    from ocelot_amd.workloads import gaussian_bunch
+    dev = torch.cuda.current_device()
+    bunches = _batch(7, seed=9, scale=0.05)
+    meshes = [(15, 31, 47), (31, 31, 31), (63, 15, 31), (47, 47, 47), (15, 15, 15), (31, 63, 15), (63, 63, 63)]
+    for b, m in zip(bunches, meshes):
+        b["nmesh"] = m
+    saved = {k: _lib.get_option(dev, k) for k in ("batch_fused", "batch_sort")}
+    try:
+        for bs in (0, 1):
+            _lib.set_option(dev, "batch_sort", bs)
+            for graph in (False, True):
+                outs = {}
+                for fused in (0, 1):
+                    _lib.set_option(dev, "batch_fused", fused)
+                    ds = [DeviceParticleArray.from_host(parray(b["r"], b["q"], b["E"]), dev) for b in bunches]
+                    apply_batched([(d, b["nmesh"], 0.1) for d, b in zip(ds, bunches)], precision=prec, nkicks=2,
+                                  use_graph=graph)
+                    outs[fused] = [d.rparticles.cpu().numpy() for d in ds]
+                for i, (a, f) in enumerate(zip(outs[0], outs[1])):
+                    assert np.array_equal(a, f), (bs, graph, i)
+        _lib.set_option(dev, "batch_sort", 0)
+        outs = {}
+        for fused in (0, 1):
+            _lib.set_option(dev, "batch_fused", fused)
+            np.random.seed(3)
+            ds = [DeviceParticleArray.from_host(parray(b["r"], b["q"], b["E"]), dev) for b in bunches]
+            apply_batched([(d, b["nmesh"], 0.1) for d, b in zip(ds, bunches)], precision=prec, nkicks=1,
+                          random_mesh=True)
+            outs[fused] = [d.rparticles.cpu().numpy() for d in ds]
+        for i, (a, f) in enumerate(zip(outs[0], outs[1])):
+            assert np.array_equal(a, f), ("random mesh", i)
+        # a bunch outside any sensible range raises for the whole batch
+        bad = gaussian_bunch(2000, seed=1)
+        bad[0, 5] = np.inf
+        _lib.set_option(dev, "batch_fused", 1)
+        ds = [DeviceParticleArray.from_host(parray(b["r"], b["q"], b["E"]), dev) for b in bunches[:2]]
+        ds.append(DeviceParticleArray.from_host(parray(bad, np.full(2000, 1e-15), 0.5), dev))
+        with pytest.raises(ValueError):
+            apply_batched([(d, m, 0.1) for d, m in zip(ds, [(31, 31, 31)] * 3)], precision=prec, nkicks=2)
+        # the context reports no stale error afterwards
+        d = DeviceParticleArray.from_host(parray(bunches[0]["r"], bunches[0]["q"], bunches[0]["E"]), dev)
+        apply_batched([(d, (31, 31, 31), 0.1)], precision=prec, nkicks=2)
+    finally:
+        for k, v in saved.items():
+            _lib.set_option(dev, k, v)
+
+
 def test_batched_64_bunches_vs_oracle():
     """The 64-bunch configuration at full size, one batched FP64 kick; four
     bunches (one per mesh size and the extremes of the size range) are
