@@ -38,6 +38,13 @@ struct BJob {
   void* G;    // kernel spectrum
   void* phi;  // mixed: three FP32 field grids; FP64: the potential
   double* E64;  // FP64: the three difference fields
+  // cell-ordered copy kept across the rounds of a call: rows (ld n),
+  // charges, caller index of each copy position; offset of the job's
+  // particles in the call-wide sort arrays
+  double* crs;
+  double* cqs;
+  uint32_t* cperm;
+  int64_t off;
 };
 
 // one launch over some of the jobs: their ids, the prefix of the workgroups
@@ -216,6 +223,60 @@ __global__ void __launch_bounds__(kNT) kb_zinv64(const BJob* __restrict__ J, BSe
 __global__ void __launch_bounds__(256) kb_diff64(const BJob* __restrict__ J, BSeg g) {
   OCSC_BJOB();
   diff64_body((const double*)b.phi, b.sc, b.E64, bx, nbx);
+}
+
+// Cell-ordered copies of all jobs with one sort: every job's keys carry its
+// index above the cell key (tag), so one stable radix sort over the
+// concatenated keys orders the particles by job, then by cell, and each job's
+// run is its own permutation.  Keys and records from the job's rows (its
+// scalars give the frame and grid of the kick just done).
+__global__ void __launch_bounds__(kPT) kb_sort_keys(const BJob* __restrict__ J, BSeg g, uint32_t* __restrict__ keys,
+                                                    PRec* __restrict__ rec, int cells, int jshift) {
+  OCSC_BJOB();
+  sort_keys_body(Rows{b.r, b.ld}, b.q, b.sc, keys + b.off, rec + b.off, cells, DriftArgs{0, 0},
+                 (uint32_t)j_ << jshift, bx, nbx);
+}
+
+// copy[:, i] = record perm[off + i] (a call-wide index into the job's own
+// records), cperm[i] = its caller index
+__global__ void __launch_bounds__(256) kb_gather_copy(const BJob* __restrict__ J, BSeg g,
+                                                      const uint32_t* __restrict__ perm,
+                                                      const PRec* __restrict__ rec) {
+  OCSC_BJOB();
+  const int64_t i = (int64_t)bx * 256 + threadIdx.x;
+  if (i >= b.n) return;
+  const uint32_t src = perm[b.off + i];
+  double v[8];
+  load_rec(rec + src, v);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) stnt(v[k], b.crs + k * b.n + i);
+  stnt(v[6], b.cqs + i);
+  b.cperm[i] = (uint32_t)(src - (uint32_t)b.off);
+}
+
+// back to the caller's order: each copy position's record to its caller
+// position (whole segments), then the records as rows
+__global__ void __launch_bounds__(256) kb_copy_to_rec(const BJob* __restrict__ J, BSeg g, PRec* __restrict__ rec) {
+  OCSC_BJOB();
+  const int64_t i = (int64_t)bx * 256 + threadIdx.x;
+  if (i >= b.n) return;
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = ldnt(b.crs + k * b.n + i);
+  v[6] = 0.0;
+  v[7] = 0.0;
+  store_rec(rec + b.off + b.cperm[i], v);
+}
+
+__global__ void __launch_bounds__(256) kb_rec_to_rows(const BJob* __restrict__ J, BSeg g,
+                                                      const PRec* __restrict__ rec) {
+  OCSC_BJOB();
+  const int64_t i = (int64_t)bx * 256 + threadIdx.x;
+  if (i >= b.n) return;
+  double v[8];
+  load_rec(rec + b.off + i, v);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) stnt(v[k], b.r + k * b.ld + i);
 }
 
 #undef OCSC_BJOB

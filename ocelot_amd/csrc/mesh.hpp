@@ -430,6 +430,12 @@ __global__ void __launch_bounds__(kNT) k_zinv3(const cpx<float>* __restrict__ A,
 // thread of a 256-thread workgroup, so twice the waves fit a CU (tracking
 // step at 127^3: 0.894 -> 0.887 ms).
 constexpr int kNTF = 512;
+// plan 3: 256 threads, so every thread gets a radix-16 butterfly (two passes
+// for 256 points instead of three of radix 8)
+template <int V>
+constexpr int field_threads() {
+  return V == 3 ? 256 : kNTF;
+}
 
 template <int M, int NT = kNT, int V = 0>
 struct FieldTile {
@@ -448,40 +454,41 @@ struct FieldTile {
 // LDS holds one tile of T columns (several workgroups per CU).
 template <int M, int V>
 __device__ __forceinline__ void xconv_field_body(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx, cpx<float>* __restrict__ P, const float* __restrict__ G, MeshDims d, const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw, const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
+  constexpr int NTF = field_threads<V>();
   using F = float;
-  constexpr int T = FieldTile<M, kNTF, V>::T, S = FieldTile<M, kNTF, V>::S, PER = FieldTile<M, kNTF, V>::PER;
+  constexpr int T = FieldTile<M, NTF, V>::T, S = FieldTile<M, NTF, V>::S, PER = FieldTile<M, NTF, V>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
-  const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF, V>::stw>(s + (size_t)M * S, tw);
+  const cpx<F>* twl = stage_tw<F, M, NTF, FieldTile<M, NTF, V>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
   const int64_t cs = (int64_t)My * Hz;
-  const Col c = flat_col<T, kNTF>(My * Hz, Hz, bx);  // a = ky, b = kz
+  const Col c = flat_col<T, NTF>(My * Hz, Hz, bx);  // a = ky, b = kz
   const int64_t off = (int64_t)c.a * Hz + c.b;
   {
     cpx<F> u[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int x = (threadIdx.x + k * kNTF) / T;
+      const int x = (threadIdx.x + k * NTF) / T;
       u[k] = cpx<F>{0, 0};
       if (x < nx && c.ok) u[k] = B[x * cs + off];
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int x = (threadIdx.x + k * kNTF) / T;
+      const int x = (threadIdx.x + k * NTF) / T;
       if (x < M) s[x * S + c.t] = u[k];
     }
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, -1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
+  lds_fft<F, M, T, NTF, -1, S, FieldTile<M, NTF, V>::RSEL>(s, twl);
   const int fy = c.a <= My - c.a ? c.a : My - c.a;
   const double inv_hx = 1.0 / sc->steps[0];
   cpx<F> v[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
-    const int i = threadIdx.x + k * kNTF;
+    const int i = threadIdx.x + k * NTF;
     v[k] = cpx<F>{0, 0};
-    if (M * T % kNTF == 0 || i < M * T) {
+    if (M * T % NTF == 0 || i < M * T) {
       const int kx = i / T;
       const int fx = kx <= M - kx ? kx : M - kx;
       const F g = c.ok ? G[((int64_t)fx * Hy + fy) * Hz + c.b] : (F)0;
@@ -491,40 +498,40 @@ __device__ __forceinline__ void xconv_field_body(const cpx<float>* __restrict__ 
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
-    const int i = threadIdx.x + k * kNTF;
-    if (M * T % kNTF == 0 || i < M * T) {
+    const int i = threadIdx.x + k * NTF;
+    if (M * T % NTF == 0 || i < M * T) {
       const int kx = i / T;
       s[kx * S + c.t] = cmul(v[k], dfac<F>(twd, kx, inv_hx));
     }
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
+  lds_fft<F, M, T, NTF, +1, S, FieldTile<M, NTF, V>::RSEL>(s, twl);
   if (c.ok) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int x = (threadIdx.x + k * kNTF) / T;
+      const int x = (threadIdx.x + k * NTF) / T;
       if (x < nx) Qx[x * cs + off] = s[x * S + c.t];
     }
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
-    const int i = threadIdx.x + k * kNTF;
-    if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + c.t] = v[k];
+    const int i = threadIdx.x + k * NTF;
+    if (M * T % NTF == 0 || i < M * T) s[(i / T) * S + c.t] = v[k];
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
+  lds_fft<F, M, T, NTF, +1, S, FieldTile<M, NTF, V>::RSEL>(s, twl);
   if (c.ok) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int x = (threadIdx.x + k * kNTF) / T;
+      const int x = (threadIdx.x + k * NTF) / T;
       if (x < nx) P[x * cs + off] = s[x * S + c.t];
     }
   }
 }
 
 template <int M, int V = 0>
-__global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
+__global__ void __launch_bounds__(field_threads<V>()) k_xconv_field(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
                                                      cpx<float>* __restrict__ P, const float* __restrict__ G,
                                                      MeshDims d, const KickScalars* __restrict__ sc,
                                                      const cpx<float>* __restrict__ tw,
@@ -537,22 +544,23 @@ __global__ void __launch_bounds__(kNTF) k_xconv_field(const cpx<float>* __restri
 // Three transforms of one LDS tile in turn, P held in registers.
 template <int M, int V>
 __device__ __forceinline__ void yinv_field_body(const cpx<float>* __restrict__ Qx, const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax, cpx<float>* __restrict__ Ay, cpx<float>* __restrict__ Ap, MeshDims d, const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw, const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
+  constexpr int NTF = field_threads<V>();
   using F = float;
-  constexpr int T = FieldTile<M, kNTF, V>::T, S = FieldTile<M, kNTF, V>::S, PER = FieldTile<M, kNTF, V>::PER;
+  constexpr int T = FieldTile<M, NTF, V>::T, S = FieldTile<M, NTF, V>::S, PER = FieldTile<M, NTF, V>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
   if (sc->err) return;
-  const cpx<F>* twl = stage_tw<F, M, kNTF, FieldTile<M, kNTF, V>::stw>(s + (size_t)M * S, tw);
+  const cpx<F>* twl = stage_tw<F, M, NTF, FieldTile<M, NTF, V>::stw>(s + (size_t)M * S, tw);
   const int Hz = d.hz, ny = d.ny;
-  const Col c = flat_col<T, kNTF>(d.nx * Hz, Hz, bx);  // a = x, b = kz
+  const Col c = flat_col<T, NTF>(d.nx * Hz, Hz, bx);  // a = x, b = kz
   const int64_t src = (int64_t)c.a * M * Hz + c.b;
   const double inv_hy = 1.0 / sc->steps[1];
   cpx<F> p[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
-    const int i = threadIdx.x + k * kNTF;
+    const int i = threadIdx.x + k * NTF;
     p[k] = cpx<F>{0, 0};
-    if (M * T % kNTF == 0 || i < M * T) {
+    if (M * T % NTF == 0 || i < M * T) {
       const int ky = i / T;
       cpx<F> q{0, 0};
       if (c.ok) {
@@ -564,13 +572,13 @@ __device__ __forceinline__ void yinv_field_body(const cpx<float>* __restrict__ Q
     }
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
+  lds_fft<F, M, T, NTF, +1, S, FieldTile<M, NTF, V>::RSEL>(s, twl);
   const int64_t dst0 = (int64_t)c.a * ny * Hz + c.b;
   auto store = [&](cpx<F>* dst) {
     if (!c.ok) return;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int y = (threadIdx.x + k * kNTF) / T;
+      const int y = (threadIdx.x + k * NTF) / T;
       if (y < ny) dst[dst0 + (int64_t)y * Hz] = s[y * S + c.t];
     }
   };
@@ -578,25 +586,25 @@ __device__ __forceinline__ void yinv_field_body(const cpx<float>* __restrict__ Q
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
-    const int i = threadIdx.x + k * kNTF;
-    if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + c.t] = cmul(p[k], dfac<F>(twd, i / T, inv_hy));
+    const int i = threadIdx.x + k * NTF;
+    if (M * T % NTF == 0 || i < M * T) s[(i / T) * S + c.t] = cmul(p[k], dfac<F>(twd, i / T, inv_hy));
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
+  lds_fft<F, M, T, NTF, +1, S, FieldTile<M, NTF, V>::RSEL>(s, twl);
   store(Ay);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
-    const int i = threadIdx.x + k * kNTF;
-    if (M * T % kNTF == 0 || i < M * T) s[(i / T) * S + c.t] = p[k];
+    const int i = threadIdx.x + k * NTF;
+    if (M * T % NTF == 0 || i < M * T) s[(i / T) * S + c.t] = p[k];
   }
   __syncthreads();
-  lds_fft<F, M, T, kNTF, +1, S, FieldTile<M, kNTF, V>::RSEL>(s, twl);
+  lds_fft<F, M, T, NTF, +1, S, FieldTile<M, NTF, V>::RSEL>(s, twl);
   store(Ap);
 }
 
 template <int M, int V = 0>
-__global__ void __launch_bounds__(kNTF) k_yinv_field(const cpx<float>* __restrict__ Qx,
+__global__ void __launch_bounds__(field_threads<V>()) k_yinv_field(const cpx<float>* __restrict__ Qx,
                                                     const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax,
                                                     cpx<float>* __restrict__ Ay, cpx<float>* __restrict__ Ap,
                                                     MeshDims d, const KickScalars* __restrict__ sc,

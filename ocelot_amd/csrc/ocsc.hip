@@ -221,7 +221,8 @@ struct ocsc_ctx {
   int deposit_xcd = 0;  // 1: per-XCD grid copies with L2-local atomics
   int deposit_agg = 0;  // 1: runs of equal cells summed across the wave before the LDS table
   int deposit_tile = 0;  // deposit particles per thread and hash slots (DepTile)
-  int fft_plan = 0;      // mixed-form transform plan (Tile's V): 0 default, 1 radix 8, 2 half-width tiles
+  int fft_plan = 0;  // mixed-form transform plan (Tile's V): 0 default, 1 radix 8, 2 half-width tiles,
+                     // 3 field passes with 256 threads (radix 16)
   int red_blocks = kMaxRedBlocks;  // blocks of the strided reduction passes
   int sort_every = 10;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
   int permute_rec = 1;  // tracking: permutations staged through 64-byte particle records (0: row by row)
@@ -251,7 +252,7 @@ struct ocsc_ctx {
   // per bunch and the device tables of jobs, inputs and launch segments
   int batch_fused = 1;
   std::vector<Lane*> flanes;
-  Buf btab;
+  Buf btab, btab2;
   // tracking workspace: two cell-ordered copies of the bunch, sort buffers,
   // particle records for the permutations
   Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp, rec;
@@ -508,17 +509,17 @@ int launch_fields_v(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, fl
   if (ev) (void)hipEventRecord(ev[2], st);
   if (join) HIPCHK(hipStreamWaitEvent(st, join, 0));  // the kernel spectrum, built on another stream
   OCSC_SWITCH_M(d.mx, {
-    using FT = FieldTile<MM, kNTF, V>;
+    using FT = FieldTile<MM, field_threads<V>(), V>;
     const int grid = (d.my * d.hz + FT::T - 1) / FT::T;
-    k_xconv_field<MM, V><<<grid, kNTF, FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
+    k_xconv_field<MM, V><<<grid, field_threads<V>(), FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
                                                       c->twd + MM);
   });
   LAUNCHCHK("k_xconv_field");
   if (ev) (void)hipEventRecord(ev[3], st);
   OCSC_SWITCH_M(d.my, {
-    using FT = FieldTile<MM, kNTF, V>;
+    using FT = FieldTile<MM, field_threads<V>(), V>;
     const int grid = (d.nx * d.hz + FT::T - 1) / FT::T;
-    k_yinv_field<MM, V><<<grid, kNTF, FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
+    k_yinv_field<MM, V><<<grid, field_threads<V>(), FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
                                                      c->twd + MM);
   });
   LAUNCHCHK("k_yinv_field");
@@ -539,6 +540,8 @@ int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, floa
       return launch_fields_v<1>(c, L, g, rho, E, st, ev, join);
     case 2:
       return launch_fields_v<2>(c, L, g, rho, E, st, ev, join);
+    case 3:
+      return launch_fields_v<3>(c, L, g, rho, E, st, ev, join);
     default:
       return launch_fields_v<0>(c, L, g, rho, E, st, ev, join);
   }
@@ -1095,6 +1098,7 @@ int ocsc_destroy(ocsc_ctx* c) {
     delete l;
   }
   if (c->btab.p) (void)hipFree(c->btab.p);
+  if (c->btab2.p) (void)hipFree(c->btab2.p);
   for (ocsc_ctx::BunchCopy* bc : c->bcopy) {
     for (Buf* b : {&bc->rs, &bc->qs, &bc->perm})
       if (b->p) (void)hipFree(b->p);
@@ -2069,7 +2073,7 @@ int dep_items(int tile) { return tile == 1 ? DepTile<1>::items : (tile == 2 ? De
 int fused_plan(ocsc_ctx* c, const std::vector<int>& kicked, const std::vector<double*>& rr,
                const std::vector<int64_t>& ldr, const std::vector<const double*>& qq, const std::vector<int64_t>& nn,
                const std::vector<Geometry>& geo, const std::vector<KickInputs>& ins, int prec, hipStream_t st,
-               FusedPlan* P) {
+               Buf& tab, bool copies, FusedPlan* P) {
   const bool dbl = prec == OCSC_FP64;
   const int nword = dbl ? 2 : 1;
   const int ncopy = c->deposit_xcd ? kNumXcd : 1;
@@ -2100,6 +2104,10 @@ int fused_plan(ocsc_ctx* c, const std::vector<int>& kicked, const std::vector<do
     b.G = L.G.p;
     b.phi = L.phi.p;
     b.E64 = (double*)L.E64.p;
+    b.crs = copies ? (double*)c->bcopy[j]->rs.p : nullptr;
+    b.cqs = copies ? (double*)c->bcopy[j]->qs.p : nullptr;
+    b.cperm = copies ? (uint32_t*)c->bcopy[j]->perm.p : nullptr;
+    b.off = j ? jobs[j - 1].off + jobs[j - 1].n : 0;
     jin[j] = ins[i];
   }
   // segments: host ids and prefixes, packed after the jobs and inputs
@@ -2207,16 +2215,16 @@ int fused_plan(ocsc_ctx* c, const std::vector<int>& kicked, const std::vector<do
   const size_t o_cnt = o_pre + al(pre.size() * sizeof(int64_t));
   const size_t total = o_cnt + cnt.size() * sizeof(int64_t);
   int rc;
-  if ((rc = ensure(c->btab, total))) return rc;
+  if ((rc = ensure(tab, total))) return rc;
   std::vector<unsigned char> host(total, 0);
   std::memcpy(host.data(), jobs.data(), nbk * sizeof(BJob));
   std::memcpy(host.data() + o_in, jin.data(), nbk * sizeof(KickInputs));
   std::memcpy(host.data() + o_ids, ids.data(), ids.size() * sizeof(int));
   std::memcpy(host.data() + o_pre, pre.data(), pre.size() * sizeof(int64_t));
   std::memcpy(host.data() + o_cnt, cnt.data(), cnt.size() * sizeof(int64_t));
-  HIPCHK(hipMemcpyAsync(c->btab.p, host.data(), total, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(tab.p, host.data(), total, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
-  unsigned char* base = (unsigned char*)c->btab.p;
+  unsigned char* base = (unsigned char*)tab.p;
   P->nb = nbk;
   P->jobs = (BJob*)base;
   P->ins = (KickInputs*)(base + o_in);
@@ -2246,9 +2254,11 @@ int launch_deposit_b(const ocsc_ctx* c, const FusedPlan& P, hipStream_t st) {
 }
 
 // One kick of every job: the passes of enqueue_kick, each one launch (per
-// FFT length for the transforms) over all jobs.  Kernel spectrum in line.
-template <typename F, int NWORD>
-int fused_round(ocsc_ctx* c, const FusedPlan& P, int prec, hipStream_t st) {
+// FFT length for the transforms) over all jobs; kernel spectrum in line.
+// Front: set-up, frame and extent; back: clear, Green's function, deposit,
+// transforms, kick (a round that builds the cell-ordered copies runs the
+// front on the callers' rows and the back on the copies).
+int fused_front(const FusedPlan& P, int prec, hipStream_t st) {
   const BJob* J = P.jobs;
   kb_setup<<<(P.nb + 63) / 64, 64, 0, st>>>(P.ins, J, P.nb);
   kb_passA<<<(unsigned)P.pab.grid, kPT, 0, st>>>(J, P.pab.seg);
@@ -2258,8 +2268,14 @@ int fused_round(ocsc_ctx* c, const FusedPlan& P, int prec, hipStream_t st) {
   else
     kb_passB<false><<<(unsigned)P.pab.grid, kPT, 0, st>>>(J, P.pab.seg);
   kb_reduceB<<<P.nb, 1024, 0, st>>>(J, prec);
-  kb_zero<<<(unsigned)P.zero.grid, 256, 0, st>>>(J, P.zero.seg);
   LAUNCHCHK("batched frame/extent passes");
+  return OCSC_OK;
+}
+
+template <typename F, int NWORD>
+int fused_back(ocsc_ctx* c, const FusedPlan& P, hipStream_t st) {
+  const BJob* J = P.jobs;
+  kb_zero<<<(unsigned)P.zero.grid, 256, 0, st>>>(J, P.zero.seg);
   kb_igf_nodes<<<(unsigned)P.nodes.grid, 256, 0, st>>>(J, P.nodes.seg);
   kb_igf_diff<<<(unsigned)P.diff.grid, 256, 0, st>>>(J, P.diff.seg);
   for (const auto& l : P.zcos) {
@@ -2336,9 +2352,138 @@ int fused_round(ocsc_ctx* c, const FusedPlan& P, int prec, hipStream_t st) {
   return OCSC_OK;
 }
 
+int fused_back_prec(ocsc_ctx* c, const FusedPlan& P, int prec, hipStream_t st) {
+  if (prec == OCSC_FP64) return fused_back<double, 2>(c, P, st);
+  return fused_back<float, 1>(c, P, st);
+}
+
 int fused_round_prec(ocsc_ctx* c, const FusedPlan& P, int prec, hipStream_t st) {
-  if (prec == OCSC_FP64) return fused_round<double, 2>(c, P, prec, st);
-  return fused_round<float, 1>(c, P, prec, st);
+  int rc;
+  if ((rc = fused_front(P, prec, st))) return rc;
+  return fused_back_prec(c, P, prec, st);
+}
+
+// nkicks rounds in one launch per pass.  With cell-ordered copies (option
+// batch_sort, more than one round): the first round takes frame and extent
+// from the callers' rows, then sorts all bunches into their copies with one
+// radix sort (keys tagged by bunch) and kicks the copies; later rounds work on
+// the copies; at the end the copies go back to the callers' order.  Same
+// arithmetic and the same orders as the per-bunch path, so the same bits.
+// *handled = false: the call does not fit (more than 2^32 particles in all
+// or keys wider than 32 bits with the bunch tag) and is left to that path.
+int apply_batched_fused(ocsc_ctx* c, const ocsc_bunch* b, int nb, const std::vector<Geometry>& geo,
+                        const std::vector<KickInputs>& ins, int prec, int nkicks, int use_graph, hipStream_t st,
+                        bool* handled) {
+  *handled = false;
+  std::vector<int> kicked;
+  for (int i = 0; i < nb; ++i)
+    if (b[i].zstep != 0) kicked.push_back(i);
+  const int nk = (int)kicked.size();
+  const bool copy = c->batch_sort && nkicks > 1;
+  int64_t N = 0;
+  int kbits = 0, jbits = 0;
+  for (int i : kicked) {
+    N += b[i].n;
+    kbits = std::max(kbits, sort_end_bit(geo[i].d.nx, geo[i].d.ny, geo[i].d.nz, c->sort_cells));
+  }
+  while ((1 << jbits) < nk) ++jbits;
+  if (copy && (N > (int64_t)0xFFFFFFFF || kbits + jbits > 32)) return OCSC_OK;
+  *handled = true;
+  if (nk == 0) return OCSC_OK;
+  int rc;
+  while ((int)c->flanes.size() < nk) {
+    Lane* l = new Lane();
+    if ((rc = init_lane(*l, false))) {
+      free_lane(*l);
+      delete l;
+      return rc;
+    }
+    c->flanes.push_back(l);
+  }
+  for (int j = 0; j < nk; ++j) {
+    const int i = kicked[j];
+    if ((rc = ensure_prec(c, *c->flanes[j], geo[i], prec)) || (rc = ensure_part(c, *c->flanes[j], b[i].n)))
+      return rc;
+  }
+  if (copy) {
+    while ((int)c->bcopy.size() < nk) c->bcopy.push_back(new ocsc_ctx::BunchCopy());
+    for (int j = 0; j < nk; ++j) {
+      const int64_t n = b[kicked[j]].n;
+      ocsc_ctx::BunchCopy& bc = *c->bcopy[j];
+      if ((rc = ensure(bc.rs, (size_t)6 * n * 8)) || (rc = ensure(bc.qs, (size_t)n * 8)) ||
+          (rc = ensure(bc.perm, (size_t)n * 4)))
+        return rc;
+    }
+    if ((rc = ensure(c->keys[0], (size_t)N * 4)) || (rc = ensure(c->keys[1], (size_t)N * 4)) ||
+        (rc = ensure(c->vals, (size_t)2 * N * 4)) || (rc = ensure(c->sort_tmp, rs_hist_bytes(N))) ||
+        (rc = ensure(c->rec, (size_t)N * sizeof(PRec))))
+      return rc;
+  }
+  std::vector<double*> rr(nb), rc_rows(nb);
+  std::vector<int64_t> ldr(nb), nn(nb), ldc(nb);
+  std::vector<const double*> qq(nb), qc(nb);
+  for (int i = 0; i < nb; ++i) {
+    nn[i] = b[i].n;
+    rr[i] = b[i].rparticles;
+    ldr[i] = b[i].ld;
+    qq[i] = b[i].q_array;
+  }
+  for (int j = 0; j < nk && copy; ++j) {
+    const int i = kicked[j];
+    rc_rows[i] = (double*)c->bcopy[j]->rs.p;
+    ldc[i] = b[i].n;
+    qc[i] = (const double*)c->bcopy[j]->qs.p;
+  }
+  FusedPlan pc, ps;  // callers' rows, copies
+  if ((rc = fused_plan(c, kicked, rr, ldr, qq, nn, geo, ins, prec, st, c->btab, copy, &pc))) return rc;
+  if (copy && (rc = fused_plan(c, kicked, rc_rows, ldc, qc, nn, geo, ins, prec, st, c->btab2, true, &ps)))
+    return rc;
+  const FusedPlan& later = copy ? ps : pc;
+  int first = 0;  // rounds done before the replayed ones
+  if (copy) {
+    // first round: frame and extent from the callers' rows, then the copies
+    if ((rc = fused_front(pc, prec, st))) return rc;
+    kb_sort_keys<<<(unsigned)pc.kick.grid, kPT, 0, st>>>(pc.jobs, pc.kick.seg, (uint32_t*)c->keys[0].p,
+                                                        (PRec*)c->rec.p, c->sort_cells, kbits);
+    uint32_t* const kk[2] = {(uint32_t*)c->keys[0].p, (uint32_t*)c->keys[1].p};
+    uint32_t* const vv[2] = {(uint32_t*)c->vals.p, (uint32_t*)c->vals.p + N};
+    const uint32_t* order = vv[rs_sort(kk, vv, N, kbits + jbits, (uint32_t*)c->sort_tmp.p, st)];
+    kb_gather_copy<<<(unsigned)pc.kick.grid, 256, 0, st>>>(pc.jobs, pc.kick.seg, order, (const PRec*)c->rec.p);
+    LAUNCHCHK("batched cell-ordered copies");
+    if ((rc = fused_back_prec(c, ps, prec, st))) return rc;
+    first = 1;
+  }
+  if (!use_graph || nkicks - first < 2) {
+    for (int k = first; k < nkicks; ++k)
+      if ((rc = fused_round_prec(c, later, prec, st))) return rc;
+  } else {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int r2 = fused_round_prec(c, later, prec, c->cap_stream);
+    hipError_t e = hipStreamEndCapture(c->cap_stream, &graph);
+    if (r2 || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r2 ? r2 : fail(OCSC_EHIP, std::string("batched capture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    for (int k = first; k < nkicks && e == hipSuccess; ++k) e = hipGraphLaunch(exec, st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(OCSC_EHIP, std::string("batched graph: ") + hipGetErrorString(e));
+  }
+  if (copy) {
+    kb_copy_to_rec<<<(unsigned)pc.kick.grid, 256, 0, st>>>(pc.jobs, pc.kick.seg, (PRec*)c->rec.p);
+    kb_rec_to_rows<<<(unsigned)pc.kick.grid, 256, 0, st>>>(pc.jobs, pc.kick.seg, (const PRec*)c->rec.p);
+    LAUNCHCHK("batched caller-order copy");
+  }
+  int32_t err = 0;
+  kb_errors<<<1, 64, 0, st>>>(pc.jobs, pc.nb, pc.err, 1);
+  LAUNCHCHK("batched errors");
+  HIPCHK(hipMemcpyAsync(&err, pc.err, sizeof(err), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (err) return value_error(err);
+  return OCSC_OK;
 }
 
 }  // namespace
@@ -2466,6 +2611,11 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
   }
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
+  if (c->batch_fused) {
+    bool handled = false;
+    rc = apply_batched_fused(c, b, nb, geo, ins, prec, nkicks, use_graph, st, &handled);
+    if (handled) return rc;
+  }
   const int nl = nb < kMaxLanes ? nb : kMaxLanes;
   while ((int)c->lanes.size() < nl) {
     Lane* l = new Lane();
@@ -2571,61 +2721,9 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
     c->profiling = prof;
     c->overlap = ovl;
   };
-  // Fused rounds: every pass one launch over all kicked bunches, each bunch
-  // with a workspace of its own (the copying first round stays per bunch)
-  FusedPlan plan;
-  const bool fused = c->batch_fused != 0;
-  if (fused) {
-    std::vector<int> kicked;
-    for (int i = 0; i < nb; ++i)
-      if (b[i].zstep != 0) kicked.push_back(i);
-    if (kicked.empty()) {
-      restore();
-      return OCSC_OK;
-    }
-    while (c->flanes.size() < kicked.size()) {
-      Lane* l = new Lane();
-      if ((rc = init_lane(*l, false))) {
-        free_lane(*l);
-        delete l;
-        restore();
-        return rc;
-      }
-      c->flanes.push_back(l);
-    }
-    std::vector<double*> rr(nb);
-    std::vector<int64_t> ldr(nb), nn(nb);
-    std::vector<const double*> qq(nb);
-    for (size_t j = 0; j < kicked.size(); ++j) {
-      const int i = kicked[j];
-      if ((rc = ensure_prec(c, *c->flanes[j], geo[i], prec)) || (rc = ensure_part(c, *c->flanes[j], b[i].n))) {
-        restore();
-        return rc;
-      }
-    }
-    for (int i = 0; i < nb; ++i) {
-      nn[i] = b[i].n;
-      if (sorted[i]) {
-        rr[i] = sbs[i].rs;
-        ldr[i] = b[i].n;
-        qq[i] = sbs[i].qs;
-      } else {
-        rr[i] = b[i].rparticles;
-        ldr[i] = b[i].ld;
-        qq[i] = b[i].q_array;
-      }
-    }
-    if ((rc = fused_plan(c, kicked, rr, ldr, qq, nn, geo, ins, prec, st, &plan))) {
-      restore();
-      return rc;
-    }
-  }
-  auto later_round = [&](hipStream_t origin) -> int {
-    return fused ? fused_round_prec(c, plan, prec, origin) : round(origin, false);
-  };
   if (!use_graph || any_sorted) {
     // the copying round eagerly (per bunch); with graphs the later rounds are captured
-    if ((rc = any_sorted || !fused ? round(st, true) : later_round(st))) {
+    if ((rc = round(st, true))) {
       restore();
       return rc;
     }
@@ -2633,7 +2731,7 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
   const int first_graph_round = (!use_graph || any_sorted) ? 1 : 0;
   if (!use_graph) {
     for (int k = 1; k < nkicks; ++k)
-      if ((rc = later_round(st))) {
+      if ((rc = round(st, false))) {
         restore();
         return rc;
       }
@@ -2645,7 +2743,7 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
       restore();
       return fail(OCSC_EHIP, std::string("capture: ") + hipGetErrorString(e));
     }
-    int r2 = fused ? later_round(c->cap_stream) : round(c->cap_stream, !any_sorted);
+    int r2 = round(c->cap_stream, !any_sorted);
     e = hipStreamEndCapture(c->cap_stream, &graph);
     if (r2 || e != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -2669,14 +2767,7 @@ int ocsc_apply_batched(ocsc_ctx* c, const ocsc_bunch* b, int nb, int prec, int n
     if ((rc = read_errors(c, *c->lanes[l], st, &err))) return rc;
     err_all |= err;
   }
-  if (fused) {
-    int32_t err = 0;
-    kb_errors<<<1, 64, 0, st>>>(plan.jobs, plan.nb, plan.err, 1);
-    LAUNCHCHK("batched errors");
-    HIPCHK(hipMemcpyAsync(&err, plan.err, sizeof(err), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    err_all |= err;
-  }
+
   if (err_all) return value_error(err_all);
   return OCSC_OK;
 }
@@ -2743,7 +2834,7 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     return OCSC_OK;
   }
   if (n == "fft_plan") {
-    if (value < 0 || value > 2) return fail(OCSC_EINVAL, "fft_plan must be 0, 1 or 2");
+    if (value < 0 || value > 3) return fail(OCSC_EINVAL, "fft_plan must be 0, 1, 2 or 3");
     c->fft_plan = (int)value;
     return OCSC_OK;
   }

@@ -678,11 +678,9 @@ __host__ __device__ inline int sort_end_bit(int nx, int ny, int nz, int cells) {
 // are taken at the position after a drift of that length (the middle of the
 // coming re-sort interval, so particles stay as close to their sorted place
 // before it as after it); the rows are not changed.
-__global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restrict__ q,
-                                                   const KickScalars* __restrict__ scp, uint32_t* __restrict__ keys,
-                                                   PRec* __restrict__ rec, int cells, DriftArgs ahead) {
+__device__ __forceinline__ void sort_keys_body(Rows R, const double* __restrict__ q, const KickScalars* __restrict__ scp, uint32_t* __restrict__ keys, PRec* __restrict__ rec, int cells, DriftArgs ahead, uint32_t tag, const int bx, const int nbx) {
   const KickScalars s = *scp;
-  const int64_t p = (int64_t)blockIdx.x * kPT + threadIdx.x;
+  const int64_t p = (int64_t)bx * kPT + threadIdx.x;
   if (p >= s.n) return;
   double v[8];
 #pragma unroll
@@ -709,7 +707,14 @@ __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restr
     key = cells ? (uint32_t)(brick * (kBx * kBy * kBz) + ((f0 % kBx) * kBy + f1 % kBy) * kBz + f2 % kBz)
                 : (uint32_t)brick;
   }
-  keys[p] = key;
+  keys[p] = tag | key;
+}
+
+__global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restrict__ q,
+                                                   const KickScalars* __restrict__ scp, uint32_t* __restrict__ keys,
+                                                   PRec* __restrict__ rec, int cells, DriftArgs ahead,
+                                                   uint32_t tag = 0) {
+  sort_keys_body(R, q, scp, keys, rec, cells, ahead, tag, blockIdx.x, gridDim.x);
 }
 
 // dst[:, j] = rec[perm[j]] (six rows and the charge), optionally applying a

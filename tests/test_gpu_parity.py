@@ -530,12 +530,13 @@ def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
         assert np.array_equal(o, ref), key
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3])
 @pytest.mark.parametrize("nmesh", [(31, 31, 63), (127, 15, 15), (15, 15, 255)])
 def test_fft_plans_agree_with_fp64(plan, nmesh):
     """The mixed form's transform plans (default, radix 8 throughout,
-    half-width tiles) change only FP32 round-off: each kick is within 1e-5
-    RMS of the FP64 kick and reproducible bit for bit."""
+    half-width tiles, 256-thread field passes) change only FP32 round-off:
+    each kick is within 1e-5 RMS of the FP64 kick and reproducible bit for
+    bit."""
     from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
     from ocelot_amd.workloads import make_bunch
     r, q, E, _, dz = make_bunch("flattop_10m_127", n=200_000, seed=18)
