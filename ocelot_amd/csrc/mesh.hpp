@@ -125,7 +125,9 @@ constexpr int per_thread(int count, int nt) { return (count + nt - 1) / nt; }
 // ---------------------------------------------------------------------------
 // z pass: pairs of real rows -> one complex FFT of length Mz -> two half spectra
 template <typename F, int M, int CIN, int V>
-__device__ __forceinline__ void zfwd_body(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, int ncopy, const int bx, const int nbx) {
+__device__ __forceinline__ void zfwd_body(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d,
+                                          const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, int ncopy,
+                                          const int bx, const int nbx) {
   using TL = Tile<F, M, V>;
   constexpr int T = TL::T, S = TL::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -188,7 +190,9 @@ __global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<
 
 // y pass forward: A[x][y<ny][kz] -> B[x][ky<My][kz]
 template <typename F, int M, int V>
-__device__ __forceinline__ void yfwd_body(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, const int bx, const int nbx) {
+__device__ __forceinline__ void yfwd_body(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d,
+                                          const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
+                                          const int bx, const int nbx) {
   using TL = Tile<F, M, V>;
   constexpr int T = TL::T, S = TL::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -232,7 +236,9 @@ __global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<
 // x pass: forward FFT along x, multiply by the kernel spectrum, inverse FFT,
 // keep x < nx.  In place on B.
 template <typename F, int M>
-__device__ __forceinline__ void xconv_body(cpx<F>* __restrict__ B, const F* __restrict__ G, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, const int bx, const int nbx) {
+__device__ __forceinline__ void xconv_body(cpx<F>* __restrict__ B, const F* __restrict__ G, MeshDims d,
+                                           const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
+                                           const int bx, const int nbx) {
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
@@ -290,7 +296,9 @@ __global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* 
 
 // y pass inverse: B[x][ky][kz] -> A[x][y<ny][kz]
 template <typename F, int M>
-__device__ __forceinline__ void yinv_body(const cpx<F>* __restrict__ B, cpx<F>* __restrict__ A, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, const int bx, const int nbx) {
+__device__ __forceinline__ void yinv_body(const cpx<F>* __restrict__ B, cpx<F>* __restrict__ A, MeshDims d,
+                                          const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
+                                          const int bx, const int nbx) {
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
@@ -387,7 +395,9 @@ __device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __res
 }
 
 template <typename F, int M, bool DERIV>
-__device__ __forceinline__ void zinv_body(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d, const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
+__device__ __forceinline__ void zinv_body(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d,
+                                          const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
+                                          const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   if (sc->err) return;
   zinv_rows<F, M, DERIV>(A, phi, d, sc, tw, twd, (cpx<F>*)smem_raw, bx);
@@ -453,7 +463,10 @@ struct FieldTile {
 // The product S*G stays in registers between the two inverse transforms, so
 // LDS holds one tile of T columns (several workgroups per CU).
 template <int M, int V>
-__device__ __forceinline__ void xconv_field_body(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx, cpx<float>* __restrict__ P, const float* __restrict__ G, MeshDims d, const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw, const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
+__device__ __forceinline__ void xconv_field_body(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
+                                                 cpx<float>* __restrict__ P, const float* __restrict__ G, MeshDims d,
+                                                 const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+                                                 const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
   constexpr int NTF = field_threads<V>();
   using F = float;
   constexpr int T = FieldTile<M, NTF, V>::T, S = FieldTile<M, NTF, V>::S, PER = FieldTile<M, NTF, V>::PER;
@@ -543,7 +556,11 @@ __global__ void __launch_bounds__(field_threads<V>()) k_xconv_field(const cpx<fl
 // (all [nx][ny][Hz]); the z field is Dz * Phi', applied in the z inverse.
 // Three transforms of one LDS tile in turn, P held in registers.
 template <int M, int V>
-__device__ __forceinline__ void yinv_field_body(const cpx<float>* __restrict__ Qx, const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax, cpx<float>* __restrict__ Ay, cpx<float>* __restrict__ Ap, MeshDims d, const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw, const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
+__device__ __forceinline__ void yinv_field_body(const cpx<float>* __restrict__ Qx, const cpx<float>* __restrict__ P,
+                                                cpx<float>* __restrict__ Ax, cpx<float>* __restrict__ Ay,
+                                                cpx<float>* __restrict__ Ap, MeshDims d,
+                                                const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+                                                const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
   constexpr int NTF = field_threads<V>();
   using F = float;
   constexpr int T = FieldTile<M, NTF, V>::T, S = FieldTile<M, NTF, V>::S, PER = FieldTile<M, NTF, V>::PER;
@@ -628,7 +645,8 @@ __device__ __forceinline__ double igf_F(double x, double y, double z) {
 // neighbours come from L2).  A tiled single pass (4x8x8 cells from a 5x9x9
 // F tile in LDS) evaluates 58% of F twice and leaves most threads idle while
 // the tile is filled.
-__device__ __forceinline__ void igf_nodes_body(double* __restrict__ Fn, MeshDims d, const KickScalars* __restrict__ sc, int force, const int bx, const int nbx) {
+__device__ __forceinline__ void igf_nodes_body(double* __restrict__ Fn, MeshDims d, const KickScalars* __restrict__ sc,
+                                               int force, const int bx, const int nbx) {
   if (sc->err || !(force || sc->rebuild)) return;
   const double hx = sc->steps[0], hy = sc->steps[1], hz = sc->steps[2];
   // 32-bit index arithmetic: meshes have at most 513^3 nodes
@@ -645,7 +663,9 @@ __global__ void __launch_bounds__(256) k_igf_nodes(double* __restrict__ Fn, Mesh
   igf_nodes_body(Fn, d, sc, force, blockIdx.x, gridDim.x);
 }
 
-__device__ __forceinline__ void igf_diff_body(const double* __restrict__ Fn, double* __restrict__ K, MeshDims d, const KickScalars* __restrict__ sc, int force, const int bx, const int nbx) {
+__device__ __forceinline__ void igf_diff_body(const double* __restrict__ Fn, double* __restrict__ K, MeshDims d,
+                                              const KickScalars* __restrict__ sc, int force, const int bx,
+                                              const int nbx) {
   if (sc->err || !(force || sc->rebuild)) return;
   const unsigned nc = (unsigned)d.nx * d.ny * d.nz;
   const unsigned sy = d.nz + 1, sx = (unsigned)(d.ny + 1) * (d.nz + 1);
@@ -668,7 +688,9 @@ __global__ void __launch_bounds__(256) k_igf_diff(const double* __restrict__ Fn,
 // e[m] = K[m] (m < nz), e[M-m] = K[m] (1 <= m < nz); both spectra are real,
 // so row a is the real part and row b the imaginary part of the output.
 template <int M>
-__device__ __forceinline__ void zcos_body(const double* __restrict__ K, double* __restrict__ out, MeshDims d, const KickScalars* __restrict__ sc, const cpx<double>* __restrict__ tw, int force, const int bx, const int nbx) {
+__device__ __forceinline__ void zcos_body(const double* __restrict__ K, double* __restrict__ out, MeshDims d,
+                                          const KickScalars* __restrict__ sc, const cpx<double>* __restrict__ tw,
+                                          int force, const int bx, const int nbx) {
   using F = double;
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -726,7 +748,10 @@ __global__ void __launch_bounds__(kNT) k_zcos(const double* __restrict__ K, doub
 // [outer][H][inner]; adjacent inner columns are paired into one complex FFT.
 // The last (x) pass scales by gscale and writes the FP32 or FP64 spectrum.
 template <int M, typename FO>
-__device__ __forceinline__ void scos_body(const double* __restrict__ in, FO* __restrict__ out, int n, int inner, int H, int nouter, const KickScalars* __restrict__ sc, const cpx<double>* __restrict__ tw, int force, int last, double inv_m3, const int bx, const int nbx) {
+__device__ __forceinline__ void scos_body(const double* __restrict__ in, FO* __restrict__ out, int n, int inner, int H,
+                                          int nouter, const KickScalars* __restrict__ sc,
+                                          const cpx<double>* __restrict__ tw, int force, int last, double inv_m3,
+                                          const int bx, const int nbx) {
   using F = double;
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];

@@ -229,7 +229,8 @@ __global__ void __launch_bounds__(256) k_fold(const double* __restrict__ part, i
 }
 
 // pass A: per-block sums of the momenta (sc.py:224 numerator)
-__device__ __forceinline__ void passA_body(Rows R, KickScalars* __restrict__ scp, double* __restrict__ part, const int bx, const int nbx) {
+__device__ __forceinline__ void passA_body(Rows R, KickScalars* __restrict__ scp, double* __restrict__ part,
+                                           const int bx, const int nbx) {
   __shared__ double lds[(kPT / 64) * 3];
   const KickScalars s = *scp;
   double acc[3] = {0, 0, 0};
@@ -251,7 +252,8 @@ __global__ void __launch_bounds__(kPT) k_passA(Rows R, KickScalars* __restrict__
 }
 
 // frame from the mean momentum (sc.py:224-239)
-__device__ __forceinline__ void reduceA_body(KickScalars* __restrict__ scp, const double* __restrict__ part, int nblk, const int bx, const int nbx) {
+__device__ __forceinline__ void reduceA_body(KickScalars* __restrict__ scp, const double* __restrict__ part, int nblk,
+                                             const int bx, const int nbx) {
   __shared__ double lds[16 * 3];
   double acc[3];
   reduce_partials<3, 0>(part, nblk, acc, lds);
@@ -286,7 +288,8 @@ __global__ void __launch_bounds__(1024) k_reduceA(KickScalars* __restrict__ scp,
 
 // pass B: rest-frame min/max, total charge, charge-weighted sums
 template <bool FAST>
-__device__ __forceinline__ void passB_body(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp, double* __restrict__ part, const int bx, const int nbx) {
+__device__ __forceinline__ void passB_body(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
+                                           double* __restrict__ part, const int bx, const int nbx) {
   __shared__ double lds[(kPT / 64) * kNPartB];
   const KickScalars s = *scp;
   const double inf = __builtin_inf();
@@ -346,7 +349,8 @@ __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict_
 
 // cell sizes, grid offset, charge scale and kernel-cache decision
 // (sc.py:172-186)
-__device__ __forceinline__ void reduceB_body(KickScalars* __restrict__ scp, const double* __restrict__ part, int nblk, int prec, const int bx, const int nbx) {
+__device__ __forceinline__ void reduceB_body(KickScalars* __restrict__ scp, const double* __restrict__ part, int nblk,
+                                             int prec, const int bx, const int nbx) {
   __shared__ double lds[16 * kNPartB];
   double out[kNPartB];
   reduce_partials<kNPartB, 1>(part, nblk, out, lds);
@@ -518,7 +522,9 @@ __device__ __forceinline__ void hash_add(int* hkey, long long (*hval)[kSlots], u
 // shuffles, exact in integers); only the last lane of each run goes to the
 // table.  Cells that recur in separate runs are simply added twice.
 template <int NWORD, bool XCD, bool AGG, int TILE>
-__device__ __forceinline__ void deposit_body(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp, unsigned long long* __restrict__ grid, int32_t* __restrict__ cells, const int bx, const int nbx) {
+__device__ __forceinline__ void deposit_body(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
+                                             unsigned long long* __restrict__ grid, int32_t* __restrict__ cells,
+                                             const int bx, const int nbx) {
   constexpr int kSlots = DepTile<TILE>::template slots<NWORD>();
   constexpr int kItems = DepTile<TILE>::items;
   __shared__ int hkey[kSlots];
@@ -678,7 +684,10 @@ __host__ __device__ inline int sort_end_bit(int nx, int ny, int nz, int cells) {
 // are taken at the position after a drift of that length (the middle of the
 // coming re-sort interval, so particles stay as close to their sorted place
 // before it as after it); the rows are not changed.
-__device__ __forceinline__ void sort_keys_body(Rows R, const double* __restrict__ q, const KickScalars* __restrict__ scp, uint32_t* __restrict__ keys, PRec* __restrict__ rec, int cells, DriftArgs ahead, uint32_t tag, const int bx, const int nbx) {
+__device__ __forceinline__ void sort_keys_body(Rows R, const double* __restrict__ q,
+                                               const KickScalars* __restrict__ scp, uint32_t* __restrict__ keys,
+                                               PRec* __restrict__ rec, int cells, DriftArgs ahead, uint32_t tag,
+                                               const int bx, const int nbx) {
   const KickScalars s = *scp;
   const int64_t p = (int64_t)bx * kPT + threadIdx.x;
   if (p >= s.n) return;
@@ -1045,7 +1054,8 @@ __device__ __forceinline__ void gather_field(const F* __restrict__ phi, const Ki
 // from the potential, the arrays the reference interpolates (sc.py:198-204);
 // the kick then reads 8 corner values per component instead of 16 potential
 // values and divides nothing.  Stored one after the other, C order.
-__device__ __forceinline__ void diff64_body(const double* __restrict__ phi, const KickScalars* __restrict__ scp, double* __restrict__ E, const int bx, const int nbx) {
+__device__ __forceinline__ void diff64_body(const double* __restrict__ phi, const KickScalars* __restrict__ scp,
+                                            double* __restrict__ E, const int bx, const int nbx) {
   const int nx = scp->nmesh[0], ny = scp->nmesh[1], nz = scp->nmesh[2];
   const int64_t nc = (int64_t)nx * ny * nz;
   const int64_t o = (int64_t)bx * 256 + threadIdx.x;
@@ -1124,7 +1134,10 @@ constexpr int kKickPT = 256;
 // position of the cell-ordered particle p) instead of in place: the last kick
 // of a tracking call and the way back to the caller's order in one pass.
 template <typename F, bool DRIFT, bool SUMP, bool RECOUT>
-__device__ __forceinline__ void kick_body(double* __restrict__ r, int64_t ld, const F* __restrict__ phi, const KickScalars* __restrict__ scp, DriftArgs dr, double* __restrict__ part, const uint32_t* __restrict__ perm, PRec* __restrict__ rec, const int bx, const int nbx) {
+__device__ __forceinline__ void kick_body(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
+                                          const KickScalars* __restrict__ scp, DriftArgs dr, double* __restrict__ part,
+                                          const uint32_t* __restrict__ perm, PRec* __restrict__ rec, const int bx,
+                                          const int nbx) {
   __shared__ double lds[SUMP ? (kKickPT / 64) * 3 : 1];
   const KickScalars& s0 = *scp;
   if (s0.err) return;  // uniform across the grid
