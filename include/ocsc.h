@@ -357,7 +357,10 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * random-mesh draws are updated in place), 0 launches kernel by kernel;
  * bit-identical results.  The graph holds the caller's pointers: a bunch
  * freed and another allocated at the same address with the same sizes is
- * kicked correctly, since only addresses and sizes enter the sequence. */
+ * kicked correctly, since only addresses and sizes enter the sequence.
+ * "grid_clear": the charge grid is cleared by a kernel of its own on the
+ * second stream (1, default; the deposit waits on that stream) or by the
+ * frame/extent pass B (0); bit-identical results. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 

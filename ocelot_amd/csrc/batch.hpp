@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(kPT) kb_passA(const BJob* __restrict__ J, BSeg
 template <bool FAST>
 __global__ void __launch_bounds__(kPT) kb_passB(const BJob* __restrict__ J, BSeg g) {
   OCSC_BJOB();
-  passB_body<FAST>(Rows{b.r, b.ld}, b.q, b.sc, b.part, bx, nbx);
+  passB_body<FAST>(Rows{b.r, b.ld}, b.q, b.sc, b.part, b.grid, b.nzero, bx, nbx);
 }
 
 // one workgroup per job
@@ -99,11 +99,6 @@ __global__ void __launch_bounds__(1024) kb_reduceA(const BJob* __restrict__ J) {
 __global__ void __launch_bounds__(1024) kb_reduceB(const BJob* __restrict__ J, int prec) {
   const BJob& b = J[blockIdx.x];
   reduceB_body(b.sc, b.part, b.nblk, prec, 0, 1);
-}
-
-__global__ void __launch_bounds__(256) kb_zero(const BJob* __restrict__ J, BSeg g) {
-  OCSC_BJOB();
-  for (int64_t i = (int64_t)bx * 256 + threadIdx.x; i < b.nzero; i += (int64_t)nbx * 256) b.grid[i] = 0ull;
 }
 
 template <int NWORD, bool XCD, bool AGG, int TILE>

@@ -213,7 +213,7 @@ struct ocsc_ctx {
   hipStream_t cap_stream = nullptr;  // stream used for graph capture
   hipStream_t side = nullptr;        // second stream: kernel spectrum next to the deposit
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  hipEvent_t ev_fork0 = nullptr, ev_zero = nullptr;  // charge-grid clear on the side stream
+  hipEvent_t ev_fork0 = nullptr, ev_zero = nullptr;  // charge-grid clear on the side stream (grid_clear 1)
   int overlap = 1;                   // 1: build the kernel spectrum concurrently with the deposit
   // stage timing (ocsc_set_profiling): events between the stages of a kick
   int profiling = 0;
@@ -231,6 +231,7 @@ struct ocsc_ctx {
   int sort_cells = 1;   // sort keys: bricks and the cells inside them (1), or bricks only (0)
   int sort_ahead = 1;   // tracking sort keys at the position half-way through the re-sort interval
   int resort_near = 0;  // re-sorts of the copy: one gather of rows, charges and permutation (1), or records
+  int grid_clear = 1;   // charge grid cleared by a kernel on the side stream (1) or by pass B (0)
   // one-shot kicks (ocsc_apply) replayed from a captured graph (option
   // "apply_graph"): the launch sequence of the last call, the set-up node
   // whose inputs change from call to call, and what it was captured for
@@ -253,6 +254,15 @@ struct ocsc_ctx {
   int batch_fused = 1;
   std::vector<Lane*> flanes;
   Buf btab, btab2;
+  // step graphs of the last plain tracking call (no moments, LSC kicks or
+  // random-mesh draws) and what they were captured for: the next call with
+  // the same launch sequence replays them instead of capturing again
+  struct TrackGraphs {
+    std::vector<int64_t> key;
+    hipGraph_t graphs[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraphNode_t knode[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  } tg;
   // tracking workspace: two cell-ordered copies of the bunch, sort buffers,
   // particle records for the permutations
   Buf rs[2], qs[2], keys[2], vals, perm[2], sort_tmp, rec;
@@ -684,20 +694,24 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
     if (prof) (void)hipEventRecord(ev[i], st);
   };
   // Side-stream work (not when profiling, so the stage times stay separable):
-  // the charge grid is cleared while the frame and extent passes run (the
-  // previous kick's z pass, earlier on this stream, was its last reader), and
   // the kernel spectrum, which needs only the cell sizes, is built while the
-  // deposit and the z and y passes run; the main stream waits for each only
-  // before its first reader.
+  // deposit and the z and y passes run; the main stream waits for it only
+  // before the x pass, its first reader.
   const bool fork = c->overlap && !prof;
+  // the charge grid (all copies, all words): cleared by its own kernel on the
+  // side stream while the frame passes run, the deposit waiting for it on the
+  // second queue (option grid_clear 1), or by pass B (0; also when profiling
+  // or without the second stream).  10M / 127^3 mixed, 20-step calls on one
+  // GPU: 0.857 vs 0.870 ms per step (pass B's stores cost more than the
+  // cross-queue wait)
   const int64_t nz64 = (c->deposit_xcd ? kNumXcd : 1) * NWORD * g.ncell;
-  auto zero_grid = [&](hipStream_t s) {
-    k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, s>>>((unsigned long long*)L.grid.p, nz64);
-  };
-  if (fork) {
+  const bool side_clear = c->grid_clear && fork;
+  unsigned long long* const zg = side_clear ? nullptr : (unsigned long long*)L.grid.p;
+  if (side_clear) {
     HIPCHK(hipEventRecord(c->ev_fork0, st));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork0, 0));
-    zero_grid(c->side);
+    k_zero64<<<(int)std::min<int64_t>((nz64 + 255) / 256, 8192), 256, 0, c->side>>>(
+        (unsigned long long*)L.grid.p, nz64);
     HIPCHK(hipEventRecord(c->ev_zero, c->side));
   }
   mark(0);
@@ -716,9 +730,9 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   }
   mark(1);
   if (prec == OCSC_MIXED)
-    k_passB<true><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p);
+    k_passB<true><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p, zg, zg ? nz64 : 0);
   else
-    k_passB<false><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p);
+    k_passB<false><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p, zg, zg ? nz64 : 0);
   k_reduceB<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk, prec);
   LAUNCHCHK("frame/extent passes");
   mark(2);
@@ -728,9 +742,7 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     if ((rc = launch_kernel_spectrum<F>(c, L, g, c->side, 0))) return rc;
     HIPCHK(hipEventRecord(c->ev_join, c->side));
-    HIPCHK(hipStreamWaitEvent(st, c->ev_zero, 0));
-  } else {
-    zero_grid(st);
+    if (side_clear) HIPCHK(hipStreamWaitEvent(st, c->ev_zero, 0));
   }
   // one-shot kick on a cell-ordered copy: the frame and extent above came
   // from the caller's order, the deposit sums integers and the kick is per
@@ -872,6 +884,14 @@ bool same_inputs(const KickInputs& a, const KickInputs& b) {
          a.draws == b.draws && a.draw_ctr == b.draw_ctr && a.ndraws == b.ndraws;
 }
 
+void drop_track_graphs(ocsc_ctx* c) {
+  for (int k = 0; k < 2; ++k) {
+    if (c->tg.exec[k]) (void)hipGraphExecDestroy(c->tg.exec[k]);
+    if (c->tg.graphs[k]) (void)hipGraphDestroy(c->tg.graphs[k]);
+  }
+  c->tg = ocsc_ctx::TrackGraphs{};
+}
+
 void drop_apply_graph(ocsc_ctx* c) {
   if (c->ag.done) (void)hipEventSynchronize(c->ag.done);
   if (c->ag.exec) (void)hipGraphExecDestroy(c->ag.exec);
@@ -892,7 +912,7 @@ int apply_via_graph(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t 
   auto& A = c->ag;
   const std::vector<int64_t> key = {(int64_t)(intptr_t)r, ld, (int64_t)(intptr_t)q, in.n, g.d.nx, g.d.ny, g.d.nz,
                                     prec, o.sort != nullptr, c->overlap, c->deposit_xcd, c->deposit_agg,
-                                    c->deposit_tile, c->fft_plan, c->sort_cells, c->red_blocks,
+                                    c->deposit_tile, c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear,
                                     (int64_t)g_alloc_gen.load()};
   if (!A.exec || A.key != key) {
     drop_apply_graph(c);
@@ -1079,6 +1099,7 @@ int ocsc_destroy(ocsc_ctx* c) {
   if (c->twf) (void)hipFree(c->twf);
   if (c->twd) (void)hipFree(c->twd);
   drop_apply_graph(c);
+  drop_track_graphs(c);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -1891,8 +1912,35 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     for (int k = 0; k < 2; ++k) {
       if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
       if (graphs[k]) (void)hipGraphDestroy(graphs[k]);
+      exec[k] = nullptr;
+      graphs[k] = nullptr;
     }
   };
+  // The step graphs hold pointers, sizes, step and energy constants and the
+  // options in their launches; a plain call with the same ones (and the same
+  // workspaces) replays the previous call's graphs.
+  const bool cacheable = use_graph && !tw && !lq && !draws && !c->profiling && zstep != 0;
+  std::vector<int64_t> tkey;
+  if (cacheable) {
+    auto bits = [](double v) {
+      int64_t b;
+      std::memcpy(&b, &v, sizeof(b));
+      return b;
+    };
+    tkey = {(int64_t)(intptr_t)r, ld, (int64_t)(intptr_t)q, n, g.d.nx, g.d.ny, g.d.nz, prec, sorted, via_rec,
+            bits(E_GeV), bits(zstep), bits(Ld), ktime, c->overlap, c->deposit_xcd, c->deposit_agg, c->deposit_tile,
+            c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear, (int64_t)g_alloc_gen.load()};
+    if (c->tg.key == tkey) {
+      for (int k = 0; k < 2; ++k) {
+        exec[k] = c->tg.exec[k];
+        graphs[k] = c->tg.graphs[k];
+        knode[k][0] = c->tg.knode[k][0];
+        knode[k][1] = c->tg.knode[k][1];
+      }
+      c->tg = ocsc_ctx::TrackGraphs{};  // owned by this call now
+    }
+  }
+  drop_track_graphs(c);
   for (int step = 0; step < nsteps && zstep != 0; ++step) {
     if (sorted && step > 0 && step % c->sort_every == 0) {
       if ((rc = resort(rr, n, qq, 1 - cur, (const uint32_t*)c->perm[cur].p, false, ahead_for(step)))) {
@@ -1995,6 +2043,18 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
   }
   int32_t err = 0;
   rc = read_errors(c, L, st, &err);
+  if (cacheable && !rc && (exec[0] || exec[1])) {
+    // keep the step graphs for the next call
+    c->tg.key = tkey;
+    for (int k = 0; k < 2; ++k) {
+      c->tg.exec[k] = exec[k];
+      c->tg.graphs[k] = graphs[k];
+      c->tg.knode[k][0] = knode[k][0];
+      c->tg.knode[k][1] = knode[k][1];
+      exec[k] = nullptr;
+      graphs[k] = nullptr;
+    }
+  }
   cleanup();
   if (rc) return rc;
   if (ktime) {
@@ -2048,7 +2108,7 @@ struct FusedPlan {
   BJob* jobs = nullptr;
   KickInputs* ins = nullptr;
   int* err = nullptr;
-  Launch pab, dep, kick, zero, nodes, diff, d64;
+  Launch pab, dep, kick, nodes, diff, d64;
   std::vector<Launch> zcos, scos_y, scos_x, zf, yf, xc, yi, zi;
 };
 
@@ -2167,7 +2227,6 @@ int fused_plan(ocsc_ctx* c, const std::vector<int>& kicked, const std::vector<do
   all([&](const BJob& b) { return (int64_t)b.nblk; }, &P->pab);
   all([&](const BJob& b) { return (b.n + kDepPT * items - 1) / (kDepPT * items); }, &P->dep, true);
   all([&](const BJob& b) { return (b.n + kKickPT - 1) / kKickPT; }, &P->kick, true);
-  all([&](const BJob& b) { return std::min<int64_t>((b.nzero + 255) / 256, 8192); }, &P->zero);
   all([&](const BJob& b) {
     const int64_t nodes = (int64_t)(b.d.nx + 1) * (b.d.ny + 1) * (b.d.nz + 1);
     return std::min<int64_t>((nodes + 255) / 256, 65536);
@@ -2275,7 +2334,6 @@ int fused_front(const FusedPlan& P, int prec, hipStream_t st) {
 template <typename F, int NWORD>
 int fused_back(ocsc_ctx* c, const FusedPlan& P, hipStream_t st) {
   const BJob* J = P.jobs;
-  kb_zero<<<(unsigned)P.zero.grid, 256, 0, st>>>(J, P.zero.seg);
   kb_igf_nodes<<<(unsigned)P.nodes.grid, 256, 0, st>>>(J, P.nodes.seg);
   kb_igf_diff<<<(unsigned)P.diff.grid, 256, 0, st>>>(J, P.diff.seg);
   for (const auto& l : P.zcos) {
@@ -2812,6 +2870,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->apply_sort = (int)value;
     return OCSC_OK;
   }
+  if (n == "grid_clear") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "grid_clear must be 0 or 1");
+    c->grid_clear = (int)value;
+    return OCSC_OK;
+  }
   if (n == "batch_fused") {
     if (value != 0 && value != 1) return fail(OCSC_EINVAL, "batch_fused must be 0 or 1");
     c->batch_fused = (int)value;
@@ -2893,6 +2956,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "batch_fused") {
     *value = c->batch_fused;
+    return OCSC_OK;
+  }
+  if (n == "grid_clear") {
+    *value = c->grid_clear;
     return OCSC_OK;
   }
   if (n == "deposit_tile") {

@@ -289,8 +289,13 @@ __global__ void __launch_bounds__(1024) k_reduceA(KickScalars* __restrict__ scp,
 // pass B: rest-frame min/max, total charge, charge-weighted sums
 template <bool FAST>
 __device__ __forceinline__ void passB_body(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
-                                           double* __restrict__ part, const int bx, const int nbx) {
+                                           double* __restrict__ part, unsigned long long* __restrict__ zg,
+                                           int64_t zn, const int bx, const int nbx) {
   __shared__ double lds[(kPT / 64) * kNPartB];
+  // the charge grid the deposit after this pass adds to is cleared here (its
+  // last reader, the previous kick's z pass, ran earlier on the stream): no
+  // launch and no cross-stream wait of its own
+  for (int64_t i = (int64_t)bx * kPT + threadIdx.x; i < zn; i += (int64_t)nbx * kPT) zg[i] = 0ull;
   const KickScalars s = *scp;
   const double inf = __builtin_inf();
   double mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
@@ -343,8 +348,9 @@ __device__ __forceinline__ void passB_body(Rows R, const double* __restrict__ q,
 
 template <bool FAST>
 __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
-                                               double* __restrict__ part) {
-  passB_body<FAST>(R, q, scp, part, blockIdx.x, gridDim.x);
+                                               double* __restrict__ part, unsigned long long* __restrict__ zg = nullptr,
+                                               int64_t zn = 0) {
+  passB_body<FAST>(R, q, scp, part, zg, zn, blockIdx.x, gridDim.x);
 }
 
 // cell sizes, grid offset, charge scale and kernel-cache decision

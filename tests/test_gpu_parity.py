@@ -444,6 +444,38 @@ def test_track_drift_graph_eager_and_oracle():
     assert err <= 1e-11 and kick <= 1e-8
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_repeated_tracking_calls_replay_kept_graphs(prec):
+    """A tracking call with the same bunch, sizes, step and options as the
+    previous one replays that call's step graphs: consecutive graph-captured
+    calls (with re-sorts), with another bunch and a changed option in
+    between, equal the same calls run eagerly, bit for bit."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=60_000, seed=23)
+    r2, q2, _, _, _ = make_bunch("flattop_10m_127", n=60_000, seed=24)
+    dev = torch.cuda.current_device()
+    saved = {k: _lib.get_option(dev, k) for k in ("sort_every", "deposit_tile")}
+    outs = {}
+    try:
+        _lib.set_option(dev, "sort_every", 4)
+        for graph in (False, True):
+            sc = SpaceCharge(nmesh_xyz=[15, 15, 31], precision=prec)
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            d2 = DeviceParticleArray.from_host(parray(r2, q2, E), dev)
+            seq = []
+            for step, (dd, tile) in enumerate(((d, 0), (d, 0), (d2, 0), (d, 0), (d, 1), (d, 0), (d, 0))):
+                _lib.set_option(dev, "deposit_tile", tile)
+                track_drift(dd, sc, 0.1, dz, 6, use_graph=graph)
+                seq.append(dd.rparticles.cpu().numpy())
+            outs[graph] = seq
+    finally:
+        for k, v in saved.items():
+            _lib.set_option(dev, k, v)
+    for i, (a, b) in enumerate(zip(outs[False], outs[True])):
+        assert np.array_equal(a, b), i
+
+
 def test_track_drift_random_mesh_against_oracle_and_rng_order():
     """random_mesh inside graph-captured tracking: every kick takes the two
     global numpy draws the reference takes (scale, then shift), passed to the
@@ -498,7 +530,7 @@ def test_per_xcd_deposit_is_bit_identical(prec):
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("opt,vals", [("deposit_agg", (0, 1)), ("permute_rec", (0, 1)), ("resort_near", (0, 1)),
-                                      ("deposit_tile", (0, 1, 2, 3))])
+                                      ("deposit_tile", (0, 1, 2, 3)), ("grid_clear", (0, 1))])
 def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
     """The deposit's wave pre-sum of equal cells and its tile sizes (integer
     sums in another grouping), with one grid and with per-XCD copies, and the
