@@ -20,10 +20,12 @@ Knobs (``ocsc_set_option``):
   particle records or row by row;
 * ``overlap``: Green's-function spectrum on a second stream next to the
   deposit;
-* ``fft_plan``: the mixed form's transform plan (0: radix 16 where every
-  thread gets a butterfly, else 8; 1: radix 8 in every pass; 2: half as many
-  columns per workgroup, less LDS and more workgroups; 3: the two field passes
-  with 256 threads, radix 16 in every pass).
+* ``fft_plan``: the mixed form's transform plan (0: LDS Stockham passes,
+  radix 16 where every thread gets a butterfly, else 8; 1: radix 8 in every
+  pass; 2: half as many columns per workgroup, less LDS and more workgroups;
+  3: the two field passes with 256 threads, radix 16 in every pass; 4, the
+  default: register-resident transforms, one LDS transpose per transform, in
+  every pass whose length is 16, 64 or 256).
 
 The search is coordinate-wise: each knob in turn over its candidates with the
 others fixed, timing ``steps`` graph-captured steps on a copy of the bunch.
@@ -51,7 +53,7 @@ from . import _lib
 
 DEFAULT_SPACE = {"sort_every": (5, 7, 10, 14, 20), "sort_ahead": (0, 1), "deposit_xcd": (0, 1),
                  "deposit_agg": (0, 1), "deposit_tile": (0, 1, 2), "permute_rec": (0, 1), "overlap": (0, 1),
-                 "fft_plan": (0, 1, 2, 3)}
+                 "fft_plan": (0, 1, 2, 3, 4)}
 BIT_IDENTICAL = ("deposit_xcd", "deposit_agg", "deposit_tile", "permute_rec", "overlap")
 
 _cache = {}

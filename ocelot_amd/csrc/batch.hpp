@@ -148,31 +148,32 @@ __global__ void __launch_bounds__(kNT) kb_scos_x(const BJob* __restrict__ J, BSe
 }
 
 // transform passes
-template <typename F, int M, int CIN>
-__global__ void __launch_bounds__(kNT) kb_zfwd(const BJob* __restrict__ J, BSeg g, const cpx<F>* __restrict__ tw,
-                                               int ncopy) {
+template <typename F, int M, int CIN, int V>
+__global__ void __launch_bounds__((pass_threads<F, M, V>())) kb_zfwd(const BJob* __restrict__ J, BSeg g,
+                                                                     const cpx<F>* __restrict__ tw, int ncopy) {
   OCSC_BJOB();
-  zfwd_body<F, M, CIN, 0>(b.grid, (cpx<F>*)b.A, b.d, b.sc, tw, ncopy, bx, nbx);
+  zfwd_body<F, M, CIN, V>(b.grid, (cpx<F>*)b.A, b.d, b.sc, tw, ncopy, bx, nbx);
 }
 
-template <typename F, int M>
-__global__ void __launch_bounds__(kNT) kb_yfwd(const BJob* __restrict__ J, BSeg g, const cpx<F>* __restrict__ tw) {
+template <typename F, int M, int V>
+__global__ void __launch_bounds__((pass_threads<F, M, V>())) kb_yfwd(const BJob* __restrict__ J, BSeg g,
+                                                                     const cpx<F>* __restrict__ tw) {
   OCSC_BJOB();
-  yfwd_body<F, M, 0>((const cpx<F>*)b.A, (cpx<F>*)b.B, b.d, b.sc, tw, bx, nbx);
+  yfwd_body<F, M, V>((const cpx<F>*)b.A, (cpx<F>*)b.B, b.d, b.sc, tw, bx, nbx);
 }
 
-template <int M>
-__global__ void __launch_bounds__(kNTF) kb_xconv_field(const BJob* __restrict__ J, BSeg g,
+template <int M, int V>
+__global__ void __launch_bounds__((field_threads<M, V>())) kb_xconv_field(const BJob* __restrict__ J, BSeg g,
                                                        const cpx<float>* __restrict__ tw,
                                                        const cpx<double>* __restrict__ twd) {
   OCSC_BJOB();
   cpx<float>* B = (cpx<float>*)b.B;
   cpx<float>* B2 = B + (int64_t)b.d.nx * b.d.my * b.d.hz;
-  xconv_field_body<M, 0>(B, B, B2, (const float*)b.G, b.d, b.sc, tw, twd, bx, nbx);
+  xconv_field_body<M, V>(B, B, B2, (const float*)b.G, b.d, b.sc, tw, twd, bx, nbx);
 }
 
-template <int M>
-__global__ void __launch_bounds__(kNTF) kb_yinv_field(const BJob* __restrict__ J, BSeg g,
+template <int M, int V>
+__global__ void __launch_bounds__((field_threads<M, V>())) kb_yinv_field(const BJob* __restrict__ J, BSeg g,
                                                       const cpx<float>* __restrict__ tw,
                                                       const cpx<double>* __restrict__ twd) {
   OCSC_BJOB();
@@ -180,17 +181,18 @@ __global__ void __launch_bounds__(kNTF) kb_yinv_field(const BJob* __restrict__ J
   const cpx<float>* B2 = B + (int64_t)b.d.nx * b.d.my * b.d.hz;
   cpx<float>* A = (cpx<float>*)b.A;
   const int64_t nyz = (int64_t)b.d.nx * b.d.ny * b.d.hz;
-  yinv_field_body<M, 0>(B, B2, A, A + nyz, A + 2 * nyz, b.d, b.sc, tw, twd, bx, nbx);
+  yinv_field_body<M, V>(B, B2, A, A + nyz, A + 2 * nyz, b.d, b.sc, tw, twd, bx, nbx);
 }
 
 // blockIdx.y: the field component
-template <int M>
-__global__ void __launch_bounds__(kNT) kb_zinv3(const BJob* __restrict__ J, BSeg g, const cpx<float>* __restrict__ tw,
-                                                const cpx<double>* __restrict__ twd) {
+template <int M, int V>
+__global__ void __launch_bounds__((pass_threads<float, M, V>())) kb_zinv3(const BJob* __restrict__ J, BSeg g,
+                                                                          const cpx<float>* __restrict__ tw,
+                                                                          const cpx<double>* __restrict__ twd) {
   OCSC_BJOB();
   (void)nbx;
   const int64_t nyz = (int64_t)b.d.nx * b.d.ny * b.d.hz;
-  zinv3_body<M, 0>((const cpx<float>*)b.A, nyz, (float*)b.phi, b.ncell, b.d, b.sc, tw, twd, blockIdx.y, bx);
+  zinv3_body<M, V>((const cpx<float>*)b.A, nyz, (float*)b.phi, b.ncell, b.d, b.sc, tw, twd, blockIdx.y, bx);
 }
 
 // FP64 form: convolution to the potential, then the difference fields

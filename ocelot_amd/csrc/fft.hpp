@@ -154,4 +154,87 @@ __device__ __forceinline__ void lds_fft(cpx<F>* s, const cpx<F>* __restrict__ tw
   lds_fft_from<F, M, T, NT, DIR, S, RSEL, 0>(s, 1, tw);
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident transforms of length M = R*R (R = 4, 8 or 16): the
+// four-step factorisation X[k + R*q] = sum_j W_R^{jq} W_M^{jk} sum_n
+// x[R*n + j] W_R^{nk}, with R threads per column.  Thread j of a column
+// holds x[n] = element R*n + j on entry; it takes the R-point DFT over n in
+// registers, multiplies by W_M^{jk}, and one transpose through LDS hands
+// every thread the R values of one k, whose R-point DFT leaves element
+// j + R*q in x[q].  That output layout is the input layout of the next
+// transform, so chained transforms (forward, pointwise product, inverse)
+// touch LDS once each instead of once per radix pass.
+//
+// Exchange buffer for T columns: R blocks of R*T entries plus padding.
+// Lanes take the column index t fastest (JT = false: columns side by side in
+// memory) or the in-column index j fastest (JT = true: rows contiguous in
+// memory).  JT = false: thread (t, j) writes its value k to [k][j][t] and
+// reads [j][n][t] for n < R; consecutive lanes touch consecutive entries in
+// both phases, and the block stride R*T + 16 (16 entries past a multiple of
+// 512 bytes when R*T is a multiple of 32) puts the blocks one read
+// instruction touches into alternate halves of the banks.  JT = true: writes
+// go to [k][t][j] (consecutive) and reads to [j][t][n], whose lanes step by
+// the block stride R*T + 1, odd, so 32 lanes hit 32 different bank pairs.
+constexpr int kRegPad = 16;
+
+template <int R, int T, bool JT = false>
+struct RegX {
+  static constexpr int blk = R * T + (JT ? 1 : kRegPad);
+  static constexpr int entries = R * blk;
+};
+
+// b[K] = a[K] * exp(DIR 2 pi i K / R) for K < R/2 (constant twiddles)
+template <int R, int DIR, typename F, int K = 0>
+__device__ __forceinline__ void ramp_half(cpx<F>* b, const cpx<F>* a) {
+  if constexpr (K < R / 2) {
+    b[K] = twc<R, K, DIR>(a[K]);
+    ramp_half<R, DIR, F, K + 1>(b, a);
+  }
+}
+
+// R-point DFT of v in place; NIN = R/2 when v[n] = 0 for n >= R/2: then
+// X[2m] = DFT_{R/2}(v)[m] and X[2m+1] = DFT_{R/2}(v[n] W_R^n)[m].
+template <int R, int DIR, typename F, int NIN>
+__device__ __forceinline__ void dft_pruned(cpx<F>* v) {
+  if constexpr (NIN * 2 == R && R >= 4) {
+    cpx<F> a[R / 2], b[R / 2];
+#pragma unroll
+    for (int n = 0; n < R / 2; ++n) a[n] = v[n];
+    ramp_half<R, DIR, F>(b, a);
+    Dft<R / 2, DIR, F>::run(a);
+    Dft<R / 2, DIR, F>::run(b);
+#pragma unroll
+    for (int m = 0; m < R / 2; ++m) {
+      v[2 * m] = a[m];
+      v[2 * m + 1] = b[m];
+    }
+  } else {
+    Dft<R, DIR, F>::run(v);
+  }
+}
+
+// One transform of the column (t, j) as described above.  DIR = -1 forward,
+// +1 inverse without 1/M; tw[k] = exp(-2 pi i k / M), k < M.  Begins with a
+// barrier (the buffer may still be read by the previous transform); the
+// caller publishes tw before the first call.
+template <typename F, int R, int T, int DIR, int NIN, bool JT = false>
+__device__ __forceinline__ void reg_fft(cpx<F> (&x)[R], cpx<F>* s, const int t, const int j,
+                                        const cpx<F>* __restrict__ tw) {
+  constexpr int BLK = RegX<R, T, JT>::blk;
+  dft_pruned<R, DIR, F, NIN>(x);
+#pragma unroll
+  for (int k = 1; k < R; ++k) {
+    cpx<F> w = tw[j * k];
+    if (DIR > 0) w.y = -w.y;
+    x[k] = cmul(x[k], w);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < R; ++k) s[k * BLK + (JT ? t * R + j : j * T + t)] = x[k];
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < R; ++n) x[n] = s[j * BLK + (JT ? t * R + n : n * T + t)];
+  Dft<R, DIR, F>::run(x);
+}
+
 }  // namespace ocsc

@@ -122,12 +122,128 @@ __device__ __forceinline__ Col flat_col(int ncol, int nb, int bx) {
 // run-time sizes, so the compiler would otherwise keep them rolled).
 constexpr int per_thread(int count, int nt) { return (count + nt - 1) / nt; }
 
+// Plan 4 (option "fft_plan"): register-resident transforms (reg_fft) for the
+// square lengths M = 16, 64, 256 in every FP32 pass (z and y forward, the two
+// field passes, z inverse), sqrt(M) threads per column; other lengths and
+// FP64 as plan 0.
+template <int M>
+constexpr int sqrt_len() {
+  return M == 16 ? 4 : (M == 64 ? 8 : (M == 256 ? 16 : 0));
+}
+template <typename F, int M, int V>
+constexpr bool reg_plan() {
+  return V == 4 && sizeof(F) == 4 && sqrt_len<M>() > 0;
+}
+// threads and LDS bytes per workgroup of the z and y passes (plan 4: the
+// larger exchange buffer of the two lane orders, twiddles, difference factors)
+template <typename F, int M, int V>
+constexpr int pass_threads() {
+  return reg_plan<F, M, V>() ? sqrt_len<M>() * Tile<F, M>::T : kNT;
+}
+template <typename F, int M, int V>
+constexpr size_t pass_lds() {
+  constexpr int R = reg_plan<F, M, V>() ? sqrt_len<M>() : 1;
+  return reg_plan<F, M, V>() ? (size_t)(RegX<R, Tile<F, M>::T>::entries + 2 * M) * sizeof(cpx<F>)
+                             : Tile<F, M, V>::lds_bytes;
+}
+
+// Plan 4 z forward: T pairs of real rows (2t, 2t+1) as T complex columns,
+// lanes j fastest (rows are contiguous in memory): thread (t, j) holds
+// z = R*n + j of its pair; z < nz <= M/2, so only n < R/2 can be non-zero.
+// The spectrum then goes to LDS in natural order and is split into the two
+// rows' half spectra as in zfwd_body.
+template <int M, int CIN>
+__device__ __forceinline__ void zfwd_reg(const void* __restrict__ rho, cpx<float>* __restrict__ A, MeshDims d,
+                                         const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+                                         int ncopy, const int bx) {
+  using F = float;
+  constexpr int R = sqrt_len<M>(), T = Tile<F, M>::T, NT = R * T;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  cpx<F>* twl = s + RegX<R, T>::entries;
+  if (sc->err) return;
+  const int nz = d.nz, Hz = d.hz;
+  const int64_t nrow = (int64_t)d.nx * d.ny;
+  const int64_t ncell = nrow * nz;
+  const int64_t row0 = (int64_t)bx * (2 * T);
+  const double iq = sc->inv_qscale;
+  const int j = threadIdx.x % R, t = threadIdx.x / R;
+  const int64_t ra = row0 + 2 * t, rb = ra + 1;
+  cpx<F> x[R];
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    const int m = R * n + j;
+    x[n] = cpx<F>{0, 0};
+    if (n < R / 2 && m < nz) {
+      if (ra < nrow) x[n].x = (F)charge_at<CIN, F>(rho, ra * nz + m, ncell, iq, ncopy);
+      if (rb < nrow) x[n].y = (F)charge_at<CIN, F>(rho, rb * nz + m, ncell, iq, ncopy);
+    }
+  }
+  for (int i = threadIdx.x; i < M; i += NT) twl[i] = tw[i];
+  __syncthreads();
+  reg_fft<F, R, T, -1, R / 2, true>(x, s, t, j, twl);
+  __syncthreads();  // the exchange buffer is read; now it holds the spectra
+#pragma unroll
+  for (int q = 0; q < R; ++q) s[t * M + j + R * q] = x[q];
+  __syncthreads();
+  constexpr int NS = per_thread(T * (M / 2 + 1), NT);
+#pragma unroll
+  for (int k2 = 0; k2 < NS; ++k2) {
+    const int i = threadIdx.x + k2 * NT;
+    if (i >= T * Hz) break;
+    const int tt = i / Hz, k = i - tt * Hz;
+    const int64_t a = row0 + 2 * tt, b = a + 1;
+    if (a >= nrow) continue;
+    const cpx<F> z = s[tt * M + k];
+    const cpx<F> w = s[tt * M + ((M - k) & (M - 1))];
+    const F h = (F)0.5;
+    A[a * Hz + k] = cpx<F>{(z.x + w.x) * h, (z.y - w.y) * h};
+    if (b < nrow) A[b * Hz + k] = cpx<F>{(z.y + w.y) * h, (w.x - z.x) * h};
+  }
+}
+
+// Plan 4 y forward: thread j of column t holds rows y = R*n + j; y < ny <=
+// M/2, so only n < R/2 can be non-zero; all M outputs are kept.
+template <int M>
+__device__ __forceinline__ void yfwd_reg(const cpx<float>* __restrict__ A, cpx<float>* __restrict__ B, MeshDims d,
+                                         const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+                                         const int bx) {
+  using F = float;
+  constexpr int R = sqrt_len<M>(), T = Tile<F, M>::T, NT = R * T;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  cpx<F>* twl = s + RegX<R, T>::entries;
+  if (sc->err) return;
+  const int Hz = d.hz, ny = d.ny;
+  const Col c = flat_col<T, NT>(d.nx * Hz, Hz, bx);
+  const int j = threadIdx.x / T;
+  const cpx<F>* src = A + (int64_t)c.a * ny * Hz + c.b;
+  cpx<F> x[R];
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    const int y = R * n + j;
+    x[n] = cpx<F>{0, 0};
+    if (n < R / 2 && c.ok && y < ny) x[n] = src[(int64_t)y * Hz];
+  }
+  for (int i = threadIdx.x; i < M; i += NT) twl[i] = tw[i];
+  __syncthreads();
+  reg_fft<F, R, T, -1, R / 2>(x, s, c.t, j, twl);
+  if (!c.ok) return;
+  cpx<F>* dst = B + (int64_t)c.a * M * Hz + c.b;
+#pragma unroll
+  for (int q = 0; q < R; ++q) dst[(int64_t)(j + R * q) * Hz] = x[q];
+}
+
 // ---------------------------------------------------------------------------
 // z pass: pairs of real rows -> one complex FFT of length Mz -> two half spectra
 template <typename F, int M, int CIN, int V>
 __device__ __forceinline__ void zfwd_body(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d,
                                           const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, int ncopy,
                                           const int bx, const int nbx) {
+  if constexpr (reg_plan<F, M, V>()) {
+    zfwd_reg<M, CIN>(rho, A, d, sc, tw, ncopy, bx);
+    return;
+  }
   using TL = Tile<F, M, V>;
   constexpr int T = TL::T, S = TL::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -182,7 +298,7 @@ __device__ __forceinline__ void zfwd_body(const void* __restrict__ rho, cpx<F>* 
 }
 
 template <typename F, int M, int CIN, int V = 0>
-__global__ void __launch_bounds__(kNT) k_zfwd(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d,
+__global__ void __launch_bounds__((pass_threads<F, M, V>())) k_zfwd(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d,
                                               const KickScalars* __restrict__ sc,
                                               const cpx<F>* __restrict__ tw, int ncopy = 1) {
   zfwd_body<F, M, CIN, V>(rho, A, d, sc, tw, ncopy, blockIdx.x, gridDim.x);
@@ -193,6 +309,10 @@ template <typename F, int M, int V>
 __device__ __forceinline__ void yfwd_body(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d,
                                           const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
                                           const int bx, const int nbx) {
+  if constexpr (reg_plan<F, M, V>()) {
+    yfwd_reg<M>(A, B, d, sc, tw, bx);
+    return;
+  }
   using TL = Tile<F, M, V>;
   constexpr int T = TL::T, S = TL::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -227,7 +347,7 @@ __device__ __forceinline__ void yfwd_body(const cpx<F>* __restrict__ A, cpx<F>* 
 }
 
 template <typename F, int M, int V = 0>
-__global__ void __launch_bounds__(kNT) k_yfwd(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d,
+__global__ void __launch_bounds__((pass_threads<F, M, V>())) k_yfwd(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d,
                                               const KickScalars* __restrict__ sc,
                                               const cpx<F>* __restrict__ tw) {
   yfwd_body<F, M, V>(A, B, d, sc, tw, blockIdx.x, gridDim.x);
@@ -411,6 +531,64 @@ __global__ void __launch_bounds__(kNT) k_zinv(const cpx<F>* __restrict__ A, F* _
   zinv_body<F, M, DERIV>(A, phi, d, sc, tw, twd, blockIdx.x, gridDim.x);
 }
 
+// Plan 4 z inverse: thread (t, j) (lanes j fastest) assembles the values
+// m = R*n + j of the complex column a + i b (half spectra of rows 2t, 2t+1)
+// straight from global memory (m > M/2 from the conjugates at M - m, as
+// zinv_rows mirrors them), the z field's difference factor applied first;
+// rows z = j + R*q < nz <= M/2 are written.
+template <int M, bool DERIV>
+__device__ __forceinline__ void zinv_rows_reg(const cpx<float>* __restrict__ A, float* __restrict__ phi,
+                                              const MeshDims& d, const KickScalars* __restrict__ sc,
+                                              const cpx<float>* __restrict__ tw, const cpx<double>* __restrict__ twd,
+                                              cpx<float>* s, const int bx) {
+  using F = float;
+  constexpr int R = sqrt_len<M>(), T = Tile<F, M>::T, NT = R * T;
+  cpx<F>* twl = s + RegX<R, T>::entries;
+  cpx<F>* dl = twl + M;
+  const int nz = d.nz, Hz = d.hz;
+  const int64_t nrow = (int64_t)d.nx * d.ny;
+  const int64_t row0 = (int64_t)bx * (2 * T);
+  const int j = threadIdx.x % R, t = threadIdx.x / R;
+  const int64_t ra = row0 + 2 * t, rb = ra + 1;
+  cpx<F> a[R], b[R];
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    const int m = R * n + j;
+    const int k = m <= M / 2 ? m : M - m;
+    a[n] = cpx<F>{0, 0};
+    b[n] = cpx<F>{0, 0};
+    if (ra < nrow) a[n] = A[ra * Hz + k];
+    if (rb < nrow) b[n] = A[rb * Hz + k];
+  }
+  for (int i = threadIdx.x; i < M; i += NT) {
+    twl[i] = tw[i];
+    if constexpr (DERIV) dl[i] = dfac<F>(twd, i, 1.0 / sc->steps[2]);
+  }
+  __syncthreads();
+  cpx<F> x[R];
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    const int m = R * n + j;
+    const int k = m <= M / 2 ? m : M - m;
+    cpx<F> aa = a[n], bb = b[n];
+    if constexpr (DERIV) {
+      const cpx<F> f = dl[k];
+      aa = cmul(aa, f);
+      bb = cmul(bb, f);
+    }
+    x[n] = m <= M / 2 ? cpx<F>{aa.x - bb.y, aa.y + bb.x} : cpx<F>{aa.x + bb.y, bb.x - aa.y};
+  }
+  reg_fft<F, R, T, +1, R, true>(x, s, t, j, twl);
+#pragma unroll
+  for (int q = 0; q < R / 2; ++q) {
+    const int z = j + R * q;
+    if (z < nz) {
+      if (ra < nrow) phi[ra * nz + z] = x[q].x;
+      if (rb < nrow) phi[rb * nz + z] = x[q].y;
+    }
+  }
+}
+
 // Mixed form: the three field components in one launch (blockIdx.y), the
 // third (z) with the spectral difference factor
 template <int M, int V>
@@ -421,14 +599,21 @@ __device__ __forceinline__ void zinv3_body(const cpx<float>* __restrict__ A, int
   extern __shared__ __align__(16) unsigned char smem_raw[];
   if (sc->err) return;
   cpx<float>* s = (cpx<float>*)smem_raw;
-  if (comp == 2)
-    zinv_rows<float, M, true, V>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s, bx);
-  else
-    zinv_rows<float, M, false, V>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s, bx);
+  if constexpr (reg_plan<float, M, V>()) {
+    if (comp == 2)
+      zinv_rows_reg<M, true>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s, bx);
+    else
+      zinv_rows_reg<M, false>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s, bx);
+  } else {
+    if (comp == 2)
+      zinv_rows<float, M, true, V>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s, bx);
+    else
+      zinv_rows<float, M, false, V>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s, bx);
+  }
 }
 
 template <int M, int V = 0>
-__global__ void __launch_bounds__(kNT) k_zinv3(const cpx<float>* __restrict__ A, int64_t nyz, float* __restrict__ E,
+__global__ void __launch_bounds__((pass_threads<float, M, V>())) k_zinv3(const cpx<float>* __restrict__ A, int64_t nyz, float* __restrict__ E,
                                                int64_t ncell, MeshDims d, const KickScalars* __restrict__ sc,
                                                const cpx<float>* __restrict__ tw,
                                                const cpx<double>* __restrict__ twd) {
@@ -441,21 +626,153 @@ __global__ void __launch_bounds__(kNT) k_zinv3(const cpx<float>* __restrict__ A,
 // step at 127^3: 0.894 -> 0.887 ms).
 constexpr int kNTF = 512;
 // plan 3: 256 threads, so every thread gets a radix-16 butterfly (two passes
-// for 256 points instead of three of radix 8)
-template <int V>
+// for 256 points instead of three of radix 8).
+// plan 4: see reg_plan.
+template <int M, int V>
+constexpr bool reg_field() {
+  return reg_plan<float, M, V>();
+}
+template <int M, int V>
 constexpr int field_threads() {
-  return V == 3 ? 256 : kNTF;
+  return reg_field<M, V>() ? sqrt_len<M>() * Tile<float, M>::T : (V == 3 ? 256 : kNTF);
 }
 
 template <int M, int NT = kNT, int V = 0>
 struct FieldTile {
+  static constexpr bool reg = reg_field<M, V>();
   static constexpr int T = Tile<float, M, V>::T;  // columns per workgroup
   static constexpr int S = T + 1;
   static constexpr int PER = (M * T + NT - 1) / NT;  // spectrum values held per thread
   static constexpr bool stw = Tile<float, M, V>::stw;
   static constexpr int RSEL = Tile<float, M, V>::RSEL;
-  static constexpr size_t lds = Tile<float, M, V>::lds_bytes;
+  static constexpr int R = reg ? sqrt_len<M>() : 1;
+  // plan 4: the exchange buffer, the twiddle table, the difference factors
+  static constexpr size_t lds =
+      reg ? (size_t)(RegX<R, T>::entries + 2 * M) * sizeof(cpx<float>) : Tile<float, M, V>::lds_bytes;
 };
+
+// Plan 4 set-up: twiddles W_M^k and difference factors dfac(k) for k < M
+// into LDS (one evaluation per workgroup instead of one per element)
+template <int M, int NT>
+__device__ __forceinline__ void stage_reg_tables(cpx<float>* twl, cpx<float>* dl, const cpx<float>* __restrict__ tw,
+                                                 const cpx<double>* __restrict__ twd, double inv_h) {
+  for (int i = threadIdx.x; i < M; i += NT) {
+    twl[i] = tw[i];
+    dl[i] = dfac<float>(twd, i, inv_h);
+  }
+}
+
+// Plan 4 x pass (see xconv_field_body): thread j of column t holds rows
+// x = R*n + j; x < nx <= M/2 (M >= 2 nx - 1 and even), so only n < R/2 can
+// be non-zero and only outputs x = j + R*q with q < R/2 are kept.
+template <int M>
+__device__ __forceinline__ void xconv_field_reg(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
+                                                cpx<float>* __restrict__ P, const float* __restrict__ G, MeshDims d,
+                                                const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+                                                const cpx<double>* __restrict__ twd, const int bx) {
+  using F = float;
+  using FT = FieldTile<M, field_threads<M, 4>(), 4>;
+  constexpr int R = FT::R, T = FT::T, NT = field_threads<M, 4>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  cpx<F>* twl = s + RegX<R, T>::entries;
+  cpx<F>* dl = twl + M;
+  if (sc->err) return;
+  const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
+  const int64_t cs = (int64_t)My * Hz;
+  const Col c = flat_col<T, NT>(My * Hz, Hz, bx);  // a = ky, b = kz
+  const int j = threadIdx.x / T;
+  const int64_t off = (int64_t)c.a * Hz + c.b;
+  cpx<F> x[R];
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    const int row = R * n + j;
+    x[n] = cpx<F>{0, 0};
+    if (n < R / 2 && c.ok && row < nx) x[n] = B[row * cs + off];
+  }
+  stage_reg_tables<M, NT>(twl, dl, tw, twd, 1.0 / sc->steps[0]);
+  __syncthreads();
+  reg_fft<F, R, T, -1, R / 2>(x, s, c.t, j, twl);
+  // x[q]: the spectrum at kx = j + R*q; v = S*G kept for the potential
+  const int fy = c.a <= My - c.a ? c.a : My - c.a;
+  cpx<F> v[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int kx = j + R * q;
+    const int fx = kx <= M - kx ? kx : M - kx;
+    const F g = c.ok ? G[((int64_t)fx * Hy + fy) * Hz + c.b] : (F)0;
+    v[q] = cscale(x[q], g);
+    x[q] = cmul(v[q], dl[kx]);
+  }
+  reg_fft<F, R, T, +1, R>(x, s, c.t, j, twl);
+  if (c.ok) {
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+      const int row = j + R * q;
+      if (row < nx) Qx[row * cs + off] = x[q];
+    }
+  }
+  reg_fft<F, R, T, +1, R>(v, s, c.t, j, twl);
+  if (c.ok) {
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+      const int row = j + R * q;
+      if (row < nx) P[row * cs + off] = v[q];
+    }
+  }
+}
+
+// Plan 4 y inverse (see yinv_field_body): all M spectrum rows are inputs,
+// rows y = j + R*q < ny <= M/2 are outputs.
+template <int M>
+__device__ __forceinline__ void yinv_field_reg(const cpx<float>* __restrict__ Qx, const cpx<float>* __restrict__ P,
+                                               cpx<float>* __restrict__ Ax, cpx<float>* __restrict__ Ay,
+                                               cpx<float>* __restrict__ Ap, MeshDims d,
+                                               const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+                                               const cpx<double>* __restrict__ twd, const int bx) {
+  using F = float;
+  using FT = FieldTile<M, field_threads<M, 4>(), 4>;
+  constexpr int R = FT::R, T = FT::T, NT = field_threads<M, 4>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  cpx<F>* twl = s + RegX<R, T>::entries;
+  cpx<F>* dl = twl + M;
+  if (sc->err) return;
+  const int Hz = d.hz, ny = d.ny;
+  const Col c = flat_col<T, NT>(d.nx * Hz, Hz, bx);  // a = x, b = kz
+  const int j = threadIdx.x / T;
+  const int64_t src = (int64_t)c.a * M * Hz + c.b;
+  cpx<F> q[R], p[R];
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    q[n] = cpx<F>{0, 0};
+    p[n] = cpx<F>{0, 0};
+    if (c.ok) {
+      const int64_t o = src + (int64_t)(R * n + j) * Hz;
+      q[n] = Qx[o];
+      p[n] = P[o];
+    }
+  }
+  stage_reg_tables<M, NT>(twl, dl, tw, twd, 1.0 / sc->steps[1]);
+  __syncthreads();
+  const int64_t dst0 = (int64_t)c.a * ny * Hz + c.b;
+  auto store = [&](cpx<F>* dst, const cpx<F>(&v)[R]) {
+    if (!c.ok) return;
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      const int y = j + R * k;
+      if (y < ny) dst[dst0 + (int64_t)y * Hz] = v[k];
+    }
+  };
+  reg_fft<F, R, T, +1, R>(q, s, c.t, j, twl);
+  store(Ax, q);
+#pragma unroll
+  for (int n = 0; n < R; ++n) q[n] = cmul(p[n], dl[R * n + j]);
+  reg_fft<F, R, T, +1, R>(q, s, c.t, j, twl);
+  store(Ay, q);
+  reg_fft<F, R, T, +1, R>(p, s, c.t, j, twl);
+  store(Ap, p);
+}
 
 // Mixed form, x pass: forward FFT along x, times the kernel spectrum, then
 // two inverse FFTs: Qx = IFFT(S * Dx) (the x field) and P = IFFT(S) (the
@@ -463,11 +780,11 @@ struct FieldTile {
 // The product S*G stays in registers between the two inverse transforms, so
 // LDS holds one tile of T columns (several workgroups per CU).
 template <int M, int V>
-__device__ __forceinline__ void xconv_field_body(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
+__device__ __forceinline__ void xconv_field_lds(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
                                                  cpx<float>* __restrict__ P, const float* __restrict__ G, MeshDims d,
                                                  const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
                                                  const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
-  constexpr int NTF = field_threads<V>();
+  constexpr int NTF = field_threads<M, V>();
   using F = float;
   constexpr int T = FieldTile<M, NTF, V>::T, S = FieldTile<M, NTF, V>::S, PER = FieldTile<M, NTF, V>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -543,8 +860,19 @@ __device__ __forceinline__ void xconv_field_body(const cpx<float>* __restrict__ 
   }
 }
 
+template <int M, int V>
+__device__ __forceinline__ void xconv_field_body(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
+                                                 cpx<float>* __restrict__ P, const float* __restrict__ G, MeshDims d,
+                                                 const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+                                                 const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
+  if constexpr (reg_field<M, V>())
+    xconv_field_reg<M>(B, Qx, P, G, d, sc, tw, twd, bx);
+  else
+    xconv_field_lds<M, V>(B, Qx, P, G, d, sc, tw, twd, bx, nbx);
+}
+
 template <int M, int V = 0>
-__global__ void __launch_bounds__(field_threads<V>()) k_xconv_field(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
+__global__ void __launch_bounds__((field_threads<M, V>())) k_xconv_field(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
                                                      cpx<float>* __restrict__ P, const float* __restrict__ G,
                                                      MeshDims d, const KickScalars* __restrict__ sc,
                                                      const cpx<float>* __restrict__ tw,
@@ -556,12 +884,12 @@ __global__ void __launch_bounds__(field_threads<V>()) k_xconv_field(const cpx<fl
 // (all [nx][ny][Hz]); the z field is Dz * Phi', applied in the z inverse.
 // Three transforms of one LDS tile in turn, P held in registers.
 template <int M, int V>
-__device__ __forceinline__ void yinv_field_body(const cpx<float>* __restrict__ Qx, const cpx<float>* __restrict__ P,
+__device__ __forceinline__ void yinv_field_lds(const cpx<float>* __restrict__ Qx, const cpx<float>* __restrict__ P,
                                                 cpx<float>* __restrict__ Ax, cpx<float>* __restrict__ Ay,
                                                 cpx<float>* __restrict__ Ap, MeshDims d,
                                                 const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
                                                 const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
-  constexpr int NTF = field_threads<V>();
+  constexpr int NTF = field_threads<M, V>();
   using F = float;
   constexpr int T = FieldTile<M, NTF, V>::T, S = FieldTile<M, NTF, V>::S, PER = FieldTile<M, NTF, V>::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -620,8 +948,20 @@ __device__ __forceinline__ void yinv_field_body(const cpx<float>* __restrict__ Q
   store(Ap);
 }
 
+template <int M, int V>
+__device__ __forceinline__ void yinv_field_body(const cpx<float>* __restrict__ Qx, const cpx<float>* __restrict__ P,
+                                                cpx<float>* __restrict__ Ax, cpx<float>* __restrict__ Ay,
+                                                cpx<float>* __restrict__ Ap, MeshDims d,
+                                                const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+                                                const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
+  if constexpr (reg_field<M, V>())
+    yinv_field_reg<M>(Qx, P, Ax, Ay, Ap, d, sc, tw, twd, bx);
+  else
+    yinv_field_lds<M, V>(Qx, P, Ax, Ay, Ap, d, sc, tw, twd, bx, nbx);
+}
+
 template <int M, int V = 0>
-__global__ void __launch_bounds__(field_threads<V>()) k_yinv_field(const cpx<float>* __restrict__ Qx,
+__global__ void __launch_bounds__((field_threads<M, V>())) k_yinv_field(const cpx<float>* __restrict__ Qx,
                                                     const cpx<float>* __restrict__ P, cpx<float>* __restrict__ Ax,
                                                     cpx<float>* __restrict__ Ay, cpx<float>* __restrict__ Ap,
                                                     MeshDims d, const KickScalars* __restrict__ sc,

@@ -221,8 +221,12 @@ struct ocsc_ctx {
   int deposit_xcd = 0;  // 1: per-XCD grid copies with L2-local atomics
   int deposit_agg = 0;  // 1: runs of equal cells summed across the wave before the LDS table
   int deposit_tile = 0;  // deposit particles per thread and hash slots (DepTile)
-  int fft_plan = 0;  // mixed-form transform plan (Tile's V): 0 default, 1 radix 8, 2 half-width tiles,
-                     // 3 field passes with 256 threads (radix 16)
+  // mixed-form transform plan (Tile's V): 0 LDS Stockham passes (radix 16 or
+  // 8), 1 radix 8, 2 half-width tiles, 3 field passes with 256 threads
+  // (radix 16), 4 (default) register-resident transforms where the length is
+  // 16, 64 or 256 (10M / 127^3 tracking step: 0.854 -> 0.816 ms, transform
+  // chain 172 -> 128 us), else as 0
+  int fft_plan = 4;
   int red_blocks = kMaxRedBlocks;  // blocks of the strided reduction passes
   int sort_every = 10;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
   int permute_rec = 1;  // tracking: permutations staged through 64-byte particle records (0: row by row)
@@ -505,31 +509,32 @@ int launch_fields_v(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, fl
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<float, MM, V>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zfwd<float, MM, kChargeFix1, V><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, L.sc, c->twf + MM,
-                                                                       c->deposit_xcd ? kNumXcd : 1);
+    k_zfwd<float, MM, kChargeFix1, V><<<nb, pass_threads<float, MM, V>(), pass_lds<float, MM, V>(), st>>>(
+        rho, A, d, L.sc, c->twf + MM, c->deposit_xcd ? kNumXcd : 1);
   });
   LAUNCHCHK("k_zfwd");
   if (ev) (void)hipEventRecord(ev[1], st);
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<float, MM, V>;
     const int grid = (d.nx * d.hz + TL::T - 1) / TL::T;
-    k_yfwd<float, MM, V><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, L.sc, c->twf + MM);
+    k_yfwd<float, MM, V><<<grid, pass_threads<float, MM, V>(), pass_lds<float, MM, V>(), st>>>(A, B, d, L.sc,
+                                                                                           c->twf + MM);
   });
   LAUNCHCHK("k_yfwd");
   if (ev) (void)hipEventRecord(ev[2], st);
   if (join) HIPCHK(hipStreamWaitEvent(st, join, 0));  // the kernel spectrum, built on another stream
   OCSC_SWITCH_M(d.mx, {
-    using FT = FieldTile<MM, field_threads<V>(), V>;
+    using FT = FieldTile<MM, field_threads<MM, V>(), V>;
     const int grid = (d.my * d.hz + FT::T - 1) / FT::T;
-    k_xconv_field<MM, V><<<grid, field_threads<V>(), FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
+    k_xconv_field<MM, V><<<grid, field_threads<MM, V>(), FT::lds, st>>>(B, B, B2, (const float*)L.G.p, d, L.sc, c->twf + MM,
                                                       c->twd + MM);
   });
   LAUNCHCHK("k_xconv_field");
   if (ev) (void)hipEventRecord(ev[3], st);
   OCSC_SWITCH_M(d.my, {
-    using FT = FieldTile<MM, field_threads<V>(), V>;
+    using FT = FieldTile<MM, field_threads<MM, V>(), V>;
     const int grid = (d.nx * d.hz + FT::T - 1) / FT::T;
-    k_yinv_field<MM, V><<<grid, field_threads<V>(), FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
+    k_yinv_field<MM, V><<<grid, field_threads<MM, V>(), FT::lds, st>>>(B, B2, A, A + nyz, A + 2 * nyz, d, L.sc, c->twf + MM,
                                                      c->twd + MM);
   });
   LAUNCHCHK("k_yinv_field");
@@ -537,7 +542,8 @@ int launch_fields_v(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, fl
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<float, MM, V>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zinv3<MM, V><<<dim3(nb, 3), kNT, TL::lds_bytes, st>>>(A, nyz, E, g.ncell, d, L.sc, c->twf + MM, c->twd + MM);
+    k_zinv3<MM, V><<<dim3(nb, 3), pass_threads<float, MM, V>(), pass_lds<float, MM, V>(), st>>>(
+        A, nyz, E, g.ncell, d, L.sc, c->twf + MM, c->twd + MM);
   });
   LAUNCHCHK("k_zinv(fields)");
   return OCSC_OK;
@@ -552,6 +558,8 @@ int launch_fields(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, floa
       return launch_fields_v<2>(c, L, g, rho, E, st, ev, join);
     case 3:
       return launch_fields_v<3>(c, L, g, rho, E, st, ev, join);
+    case 4:
+      return launch_fields_v<4>(c, L, g, rho, E, st, ev, join);
     default:
       return launch_fields_v<0>(c, L, g, rho, E, st, ev, join);
   }
@@ -2356,36 +2364,62 @@ int fused_back(ocsc_ctx* c, const FusedPlan& P, hipStream_t st) {
   if ((rc = launch_deposit_b<NWORD>(c, P, st))) return rc;
   LAUNCHCHK("batched deposit");
   const int ncopy = c->deposit_xcd ? kNumXcd : 1;
-  for (const auto& l : P.zf) {
-    OCSC_SWITCH_M(l.M, {
-      kb_zfwd<F, MM, NWORD == 2 ? kChargeFix2 : kChargeFix1>
-          <<<(unsigned)l.grid, kNT, Tile<F, MM>::lds_bytes, st>>>(J, l.seg, tw_for<F>(c, MM), ncopy);
-    });
-  }
-  for (const auto& l : P.yf) {
-    OCSC_SWITCH_M(l.M, {
-      kb_yfwd<F, MM><<<(unsigned)l.grid, kNT, Tile<F, MM>::lds_bytes, st>>>(J, l.seg, tw_for<F>(c, MM));
-    });
+  // forward passes: plan 4 (register-resident square transforms, FP32) when
+  // chosen, else the default plan
+  auto fwd_passes = [&](auto plan) -> int {
+    constexpr int V = decltype(plan)::value;
+    for (const auto& l : P.zf) {
+      OCSC_SWITCH_M(l.M, {
+        kb_zfwd<F, MM, NWORD == 2 ? kChargeFix2 : kChargeFix1, V>
+            <<<(unsigned)l.grid, pass_threads<F, MM, V>(), pass_lds<F, MM, V>(), st>>>(J, l.seg, tw_for<F>(c, MM),
+                                                                                       ncopy);
+      });
+    }
+    for (const auto& l : P.yf) {
+      OCSC_SWITCH_M(l.M, {
+        kb_yfwd<F, MM, V><<<(unsigned)l.grid, pass_threads<F, MM, V>(), pass_lds<F, MM, V>(), st>>>(
+            J, l.seg, tw_for<F>(c, MM));
+      });
+    }
+    return OCSC_OK;
+  };
+  if constexpr (sizeof(F) == 4) {
+    if ((rc = c->fft_plan == 4 ? fwd_passes(std::integral_constant<int, 4>())
+                               : fwd_passes(std::integral_constant<int, 0>())))
+      return rc;
+  } else if ((rc = fwd_passes(std::integral_constant<int, 0>()))) {
+    return rc;
   }
   if constexpr (sizeof(F) == 4) {
-    for (const auto& l : P.xc) {
-      OCSC_SWITCH_M(l.M, {
-        kb_xconv_field<MM><<<(unsigned)l.grid, kNTF, FieldTile<MM, kNTF, 0>::lds, st>>>(J, l.seg, c->twf + MM,
-                                                                                       c->twd + MM);
-      });
-    }
-    for (const auto& l : P.yi) {
-      OCSC_SWITCH_M(l.M, {
-        kb_yinv_field<MM><<<(unsigned)l.grid, kNTF, FieldTile<MM, kNTF, 0>::lds, st>>>(J, l.seg, c->twf + MM,
-                                                                                      c->twd + MM);
-      });
-    }
-    for (const auto& l : P.zi) {
-      OCSC_SWITCH_M(l.M, {
-        kb_zinv3<MM><<<dim3((unsigned)l.grid, 3), kNT, Tile<float, MM>::lds_bytes, st>>>(J, l.seg, c->twf + MM,
+    // field passes and z inverse: plan 4 (register-resident square
+    // transforms) when chosen, else the default plan
+    auto field_passes = [&](auto plan) -> int {
+      constexpr int V = decltype(plan)::value;
+      for (const auto& l : P.xc) {
+        OCSC_SWITCH_M(l.M, {
+          constexpr int NT = field_threads<MM, V>();
+          kb_xconv_field<MM, V><<<(unsigned)l.grid, NT, FieldTile<MM, NT, V>::lds, st>>>(J, l.seg, c->twf + MM,
                                                                                          c->twd + MM);
-      });
-    }
+        });
+      }
+      for (const auto& l : P.yi) {
+        OCSC_SWITCH_M(l.M, {
+          constexpr int NT = field_threads<MM, V>();
+          kb_yinv_field<MM, V><<<(unsigned)l.grid, NT, FieldTile<MM, NT, V>::lds, st>>>(J, l.seg, c->twf + MM,
+                                                                                        c->twd + MM);
+        });
+      }
+      for (const auto& l : P.zi) {
+        OCSC_SWITCH_M(l.M, {
+          kb_zinv3<MM, V><<<dim3((unsigned)l.grid, 3), pass_threads<float, MM, V>(), pass_lds<float, MM, V>(), st>>>(
+              J, l.seg, c->twf + MM, c->twd + MM);
+        });
+      }
+      return OCSC_OK;
+    };
+    if ((rc = c->fft_plan == 4 ? field_passes(std::integral_constant<int, 4>())
+                               : field_passes(std::integral_constant<int, 0>())))
+      return rc;
   } else {
     for (const auto& l : P.xc) {
       OCSC_SWITCH_M(l.M, {
@@ -2897,7 +2931,7 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     return OCSC_OK;
   }
   if (n == "fft_plan") {
-    if (value < 0 || value > 3) return fail(OCSC_EINVAL, "fft_plan must be 0, 1, 2 or 3");
+    if (value < 0 || value > 4) return fail(OCSC_EINVAL, "fft_plan must be 0, 1, 2, 3 or 4");
     c->fft_plan = (int)value;
     return OCSC_OK;
   }

@@ -562,13 +562,15 @@ def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
         assert np.array_equal(o, ref), key
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2, 3])
-@pytest.mark.parametrize("nmesh", [(31, 31, 63), (127, 15, 15), (15, 15, 255)])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("nmesh", [(31, 31, 63), (127, 15, 15), (15, 15, 255), (7, 127, 7), (127, 127, 31)])
 def test_fft_plans_agree_with_fp64(plan, nmesh):
-    """The mixed form's transform plans (default, radix 8 throughout,
-    half-width tiles, 256-thread field passes) change only FP32 round-off:
-    each kick is within 1e-5 RMS of the FP64 kick and reproducible bit for
-    bit."""
+    """The mixed form's transform plans (LDS Stockham passes, radix 8
+    throughout, half-width tiles, 256-thread field passes, register-resident
+    square transforms) change only FP32 round-off: each kick is within 1e-5
+    RMS of the FP64 kick and reproducible bit for bit.  The meshes give every
+    pass lengths with and without a register form (16, 64 and 256; 32, 128
+    and 512)."""
     from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
     from ocelot_amd.workloads import make_bunch
     r, q, E, _, dz = make_bunch("flattop_10m_127", n=200_000, seed=18)
@@ -917,6 +919,30 @@ def test_fused_batch_equals_per_bunch_launches(prec):
         # the context reports no stale error afterwards
         d = DeviceParticleArray.from_host(parray(bunches[0]["r"], bunches[0]["q"], bunches[0]["E"]), dev)
         apply_batched([(d, (31, 31, 31), 0.1)], precision=prec, nkicks=2)
+    finally:
+        for k, v in saved.items():
+            _lib.set_option(dev, k, v)
+
+
+def test_fused_batch_register_plan_equals_per_bunch_launches():
+    """With the register-resident transforms (fft_plan 4) the fused batch
+    equals the per-bunch launch sequences bit for bit (meshes with lengths 16,
+    64 and 256 and others, on every axis)."""
+    from ocelot_amd import DeviceParticleArray, _lib, apply_batched
+    dev = torch.cuda.current_device()
+    bunches = _batch(5, seed=12, scale=0.05)
+    meshes = [(7, 31, 63), (31, 63, 127), (127, 15, 31), (63, 63, 63), (15, 7, 127)]
+    saved = {k: _lib.get_option(dev, k) for k in ("batch_fused", "fft_plan")}
+    try:
+        _lib.set_option(dev, "fft_plan", 4)
+        outs = {}
+        for fused in (0, 1):
+            _lib.set_option(dev, "batch_fused", fused)
+            ds = [DeviceParticleArray.from_host(parray(b["r"], b["q"], b["E"]), dev) for b in bunches]
+            apply_batched([(d, m, 0.1) for d, m in zip(ds, meshes)], precision="mixed", nkicks=2)
+            outs[fused] = [d.rparticles.cpu().numpy() for d in ds]
+        for i, (a, f) in enumerate(zip(outs[0], outs[1])):
+            assert np.array_equal(a, f), i
     finally:
         for k, v in saved.items():
             _lib.set_option(dev, k, v)
