@@ -335,7 +335,8 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * the FP64 form from 1000000 particles on; bit-identical results
  * (ocsc_apply_cells and ocsc_apply_noalloc never sort).  "deposit_tile": particles per thread and
  * LDS hash slots of the deposit, 0 (default) 8 and 2048 (FP64: 1024), 1 half
- * of both, 2 twice both, 3 no table (one global atomic per particle); bit-identical results.  "batch_sort": see
+ * of both, 2 twice both, 3 no table (one global atomic per particle), 4 as 0
+ * with all of a thread's particle loads issued first; bit-identical results.  "batch_sort": see
  * ocsc_apply_batched.  "fft_plan": the mixed form's transform plan, 0 LDS
  * Stockham passes, radix 16 where every thread gets a butterfly (else 8), 1
  * radix 8 in every pass, 2 half as many columns per workgroup, 3 the two
@@ -361,8 +362,9 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * freed and another allocated at the same address with the same sizes is
  * kicked correctly, since only addresses and sizes enter the sequence.
  * "grid_clear": the charge grid is cleared by a kernel of its own on the
- * second stream (1, default; the deposit waits on that stream) or by the
- * frame/extent pass B (0); bit-identical results. */
+ * second stream (1, default; the deposit waits on that stream), by the
+ * frame/extent pass B (0), or in tracking by each step's kick for the next
+ * step (2; the first step of a call as 1); bit-identical results. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 

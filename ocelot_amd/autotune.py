@@ -15,7 +15,8 @@ Knobs (``ocsc_set_option``):
 * ``deposit_agg``: runs of equal cells summed across the wavefront before
   the deposit's LDS table;
 * ``deposit_tile``: particles per thread and LDS hash slots of the deposit
-  (0: 8 x 2048, 1: 4 x 1024, 2: 16 x 4096);
+  (0: 8 x 2048, 1: 4 x 1024, 2: 16 x 4096, 4: as 0 with every particle's
+  loads issued first);
 * ``permute_rec``: permutations of the tracking copy through 64-byte
   particle records or row by row;
 * ``overlap``: Green's-function spectrum on a second stream next to the
@@ -52,7 +53,7 @@ import time
 from . import _lib
 
 DEFAULT_SPACE = {"sort_every": (5, 7, 10, 14, 20), "sort_ahead": (0, 1), "deposit_xcd": (0, 1),
-                 "deposit_agg": (0, 1), "deposit_tile": (0, 1, 2), "permute_rec": (0, 1), "overlap": (0, 1),
+                 "deposit_agg": (0, 1), "deposit_tile": (0, 1, 2, 4), "permute_rec": (0, 1), "overlap": (0, 1),
                  "fft_plan": (0, 1, 2, 3, 4)}
 BIT_IDENTICAL = ("deposit_xcd", "deposit_agg", "deposit_tile", "permute_rec", "overlap")
 

@@ -111,7 +111,8 @@ template <typename F>
 __global__ void __launch_bounds__(kKickPT) kb_kick(const BJob* __restrict__ J, BSeg g) {
   OCSC_BJOB();
   const F* field = sizeof(F) == 8 ? (const F*)b.E64 : (const F*)b.phi;
-  kick_body<F, false, false, false>(b.r, b.ld, field, b.sc, DriftArgs{0, 0}, nullptr, nullptr, nullptr, bx, nbx);
+  kick_body<F, false, false, false>(b.r, b.ld, field, b.sc, DriftArgs{0, 0}, nullptr, nullptr, nullptr, nullptr, 0, bx,
+                                    nbx);
 }
 
 // Green's function and its cosine transforms (the buffers of the single

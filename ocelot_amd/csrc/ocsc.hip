@@ -235,7 +235,9 @@ struct ocsc_ctx {
   int sort_cells = 1;   // sort keys: bricks and the cells inside them (1), or bricks only (0)
   int sort_ahead = 1;   // tracking sort keys at the position half-way through the re-sort interval
   int resort_near = 0;  // re-sorts of the copy: one gather of rows, charges and permutation (1), or records
-  int grid_clear = 1;   // charge grid cleared by a kernel on the side stream (1) or by pass B (0)
+  // charge grid cleared by a kernel on the side stream (1), by pass B (0), or
+  // in tracking by the previous step's kick (2; the first step as 1)
+  int grid_clear = 1;
   // one-shot kicks (ocsc_apply) replayed from a captured graph (option
   // "apply_graph"): the launch sequence of the last call, the set-up node
   // whose inputs change from call to call, and what it was captured for
@@ -635,6 +637,11 @@ struct StepOpts {
   // kick of a tracking call on a cell-ordered copy)
   const uint32_t* rec_perm = nullptr;
   PRec* rec_out = nullptr;
+  // tracking: the charge grid is zero on entry (the previous step's kick
+  // cleared it), and this step's kick clears it for the next step (the
+  // z pass, its last reader, runs before the kick)
+  bool grid_zeroed = false;
+  bool zero_next = false;
 };
 
 // buffers of a kick on a cell-ordered copy: sort keys and values (ping-pong),
@@ -667,21 +674,24 @@ void launch_deposit(const ocsc_ctx* c, Rows R, const double* q, KickScalars* sc,
   const int t = c->deposit_tile;
 #define OCSC_DEP(V, T)                                                                                 \
   if (v == V && t == T) return launch_deposit_t<NWORD, (V & 4) != 0, (V & 2) != 0, T>(R, q, sc, grid, cells, n, st);
-#define OCSC_DEP3(V) OCSC_DEP(V, 0) OCSC_DEP(V, 1) OCSC_DEP(V, 2) OCSC_DEP(V, 3)
+#define OCSC_DEP3(V) OCSC_DEP(V, 0) OCSC_DEP(V, 1) OCSC_DEP(V, 2) OCSC_DEP(V, 3) OCSC_DEP(V, 4)
   OCSC_DEP3(0) OCSC_DEP3(2) OCSC_DEP3(4) OCSC_DEP3(6)
 #undef OCSC_DEP3
 #undef OCSC_DEP
 }
 
 template <typename F, bool DRIFT, bool SUMP>
-void launch_kick(ocsc_ctx* c, Lane& L, int npart, double* r, int64_t ld, const StepOpts& o, hipStream_t st) {
+void launch_kick(ocsc_ctx* c, Lane& L, int npart, double* r, int64_t ld, const StepOpts& o, int64_t nz64,
+                 hipStream_t st) {
   // mixed form: the three FP32 field grids; FP64 form: the k_diff64 fields
   const F* field = sizeof(F) == 8 ? (const F*)L.E64.p : (const F*)L.phi.p;
+  unsigned long long* zg = o.zero_next ? (unsigned long long*)L.grid.p : nullptr;
   if (!DRIFT && !SUMP && o.rec_out)
     k_kick<F, false, false, true><<<npart, kKickPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p, o.rec_perm,
-                                                             o.rec_out);
+                                                             o.rec_out, zg, zg ? nz64 : 0);
   else
-    k_kick<F, DRIFT, SUMP><<<npart, kKickPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p);
+    k_kick<F, DRIFT, SUMP><<<npart, kKickPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p, nullptr, nullptr,
+                                                      zg, zg ? nz64 : 0);
 }
 
 template <typename F, int NWORD>
@@ -711,10 +721,11 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   // second queue (option grid_clear 1), or by pass B (0; also when profiling
   // or without the second stream).  10M / 127^3 mixed, 20-step calls on one
   // GPU: 0.857 vs 0.870 ms per step (pass B's stores cost more than the
-  // cross-queue wait)
+  // cross-queue wait).  Tracking with grid_clear 2: the previous step's kick
+  // cleared it (o.grid_zeroed), no clear and no wait here.
   const int64_t nz64 = (c->deposit_xcd ? kNumXcd : 1) * NWORD * g.ncell;
-  const bool side_clear = c->grid_clear && fork;
-  unsigned long long* const zg = side_clear ? nullptr : (unsigned long long*)L.grid.p;
+  const bool side_clear = c->grid_clear && fork && !o.grid_zeroed;
+  unsigned long long* const zg = side_clear || o.grid_zeroed ? nullptr : (unsigned long long*)L.grid.p;
   if (side_clear) {
     HIPCHK(hipEventRecord(c->ev_fork0, st));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork0, 0));
@@ -791,19 +802,20 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   double* rk = sb ? sb->rs : r;
   const int64_t ldk = sb ? n : ld;
   StepOpts ok = o;
+  ok.zero_next = o.zero_next && !(sb && sb->scatter_back && !o.drift && !o.sump);
   if (sb && sb->scatter_back && !o.drift && !o.sump) {
     // the kick writes each record straight to its caller position
     ok.rec_perm = order;
     ok.rec_out = sb->rec;
   }
   if (o.drift && o.sump)
-    launch_kick<F, true, true>(c, L, npart, rk, ldk, ok, st);
+    launch_kick<F, true, true>(c, L, npart, rk, ldk, ok, nz64, st);
   else if (o.drift)
-    launch_kick<F, true, false>(c, L, npart, rk, ldk, ok, st);
+    launch_kick<F, true, false>(c, L, npart, rk, ldk, ok, nz64, st);
   else if (o.sump)
-    launch_kick<F, false, true>(c, L, npart, rk, ldk, ok, st);
+    launch_kick<F, false, true>(c, L, npart, rk, ldk, ok, nz64, st);
   else
-    launch_kick<F, false, false>(c, L, npart, rk, ldk, ok, st);
+    launch_kick<F, false, false>(c, L, npart, rk, ldk, ok, nz64, st);
   LAUNCHCHK("k_kick");
   if (sb && sb->scatter_back) {
     // back to the caller's order through whole records; a bunch the kick
@@ -1879,6 +1891,8 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
       o.prev_nblk = nred_blocks(c, n);
     }
     o.dr = DriftArgs{Ld, r56};
+    o.grid_zeroed = c->grid_clear == 2 && step > 0;
+    o.zero_next = c->grid_clear == 2 && step < nsteps - 1;
     if (ktime) {
       o.kev0 = c->kick_ev[2 * step];
       o.kev1 = c->kick_ev[2 * step + 1];
@@ -2313,7 +2327,7 @@ int launch_deposit_b(const ocsc_ctx* c, const FusedPlan& P, hipStream_t st) {
     kb_deposit<NWORD, (V & 4) != 0, (V & 2) != 0, T><<<(unsigned)P.dep.grid, kDepPT, 0, st>>>(P.jobs, P.dep.seg); \
     return OCSC_OK;                                                                                       \
   }
-#define OCSC_DEPB4(V) OCSC_DEPB(V, 0) OCSC_DEPB(V, 1) OCSC_DEPB(V, 2) OCSC_DEPB(V, 3)
+#define OCSC_DEPB4(V) OCSC_DEPB(V, 0) OCSC_DEPB(V, 1) OCSC_DEPB(V, 2) OCSC_DEPB(V, 3) OCSC_DEPB(V, 4)
   OCSC_DEPB4(0) OCSC_DEPB4(2) OCSC_DEPB4(4) OCSC_DEPB4(6)
 #undef OCSC_DEPB4
 #undef OCSC_DEPB
@@ -2905,7 +2919,7 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     return OCSC_OK;
   }
   if (n == "grid_clear") {
-    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "grid_clear must be 0 or 1");
+    if (value < 0 || value > 2) return fail(OCSC_EINVAL, "grid_clear must be 0, 1 or 2");
     c->grid_clear = (int)value;
     return OCSC_OK;
   }
@@ -2921,7 +2935,7 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     return OCSC_OK;
   }
   if (n == "deposit_tile") {
-    if (value < 0 || value > 3) return fail(OCSC_EINVAL, "deposit_tile must be 0, 1, 2 or 3");
+    if (value < 0 || value > 4) return fail(OCSC_EINVAL, "deposit_tile must be 0, 1, 2, 3 or 4");
     c->deposit_tile = (int)value;
     return OCSC_OK;
   }

@@ -482,10 +482,13 @@ constexpr int kDepSlots2 = 1024;             // two-word (FP64) form: half the L
 // particles per thread and half the slots, 2 twice both, 3 no table (every
 // particle one global atomic: a bunch in random order spreads a workgroup's
 // particles over far more cells than a table holds, and each insert then
-// probes a nearly full table before it falls through)
+// probes a nearly full table before it falls through), 4 as 0 with the rows
+// of all of a thread's particles loaded before the first is deposited (tile
+// 0 loads one particle ahead; the deposit waits on memory most of its time)
 template <int TILE>
 struct DepTile {
   static constexpr bool direct = TILE == 3;
+  static constexpr bool all_loads = TILE == 4;  // as 0, every item's loads issued first
   static constexpr int items = TILE == 1 ? kDepItems / 2 : (TILE == 2 ? kDepItems * 2 : kDepItems);
   template <int NWORD>
   static constexpr int slots() {
@@ -553,36 +556,22 @@ __device__ __forceinline__ void deposit_body(Rows R, const double* __restrict__ 
   const int64_t base = xcd_block(bx, nbx) * (kDepPT * kItems);
   const int lane = threadIdx.x & 63;
   int bad = 0;
-  // the rows of item it + 1 are loaded before item it goes through the table
-  double cur[7], nxt[7] = {0, 0, 0, 0, 0, 0, 0};
   auto load = [&](int64_t p, double* v) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) v[k] = ldnt(R.r + k * R.ld + p);
     v[6] = ldnt(q + p);
   };
-  int64_t p = base + threadIdx.x;
-  if (p < s.n) load(p, cur);
-  for (int it = 0; it < kItems; ++it, p += kDepPT) {
-    const bool valid = p < s.n;
-    if constexpr (AGG) {
-      if (__ballot(valid) == 0) break;  // uniform across the wave: every lane takes part in the scan
-    } else {
-      if (!valid) break;
-    }
+  // one particle (valid: p < n) through the table; cur holds its rows
+  auto item = [&](int64_t p, bool valid, const double* cur) {
     int64_t idx = -1;
     long long hi = 0, lo = 0;
     if (valid) {
-      const int64_t pn = p + kDepPT;
-      if (it + 1 < kItems && pn < s.n) load(pn, nxt);
       idx = ngp_cell<NWORD == 1>(cur, s);  // one word: the mixed form
       if (cells) cells[p] = (int32_t)idx;
-      const double qv = cur[6];
-#pragma unroll
-      for (int k = 0; k < 7; ++k) cur[k] = nxt[k];
       if (idx < 0) {
         bad = 1;
       } else {
-        const double v = qv * s.qscale;  // exact: qscale is a power of two
+        const double v = cur[6] * s.qscale;  // exact: qscale is a power of two
         const double h = __builtin_rint(v);
         hi = (long long)h;
         if constexpr (NWORD == 2) lo = (long long)__builtin_rint((v - h) * 0x1p30);
@@ -616,6 +605,46 @@ __device__ __forceinline__ void deposit_body(Rows R, const double* __restrict__ 
         flush_cell<XCD>(g, ncell, idx, hi, lo, NWORD);
       else
         hash_add<NWORD, XCD, kSlots>(hkey, hval, g, ncell, idx, hi, lo);
+    }
+  };
+  if constexpr (DepTile<TILE>::all_loads) {
+    // every item's rows in flight before the first goes through the table
+    double v[kItems][7];
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+      const int64_t p = base + threadIdx.x + (int64_t)it * kDepPT;
+      if (p < s.n) load(p, v[it]);
+    }
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+      const int64_t p = base + threadIdx.x + (int64_t)it * kDepPT;
+      const bool valid = p < s.n;
+      if constexpr (AGG) {
+        if (__ballot(valid) == 0) break;  // uniform across the wave: every lane takes part in the scan
+      } else {
+        if (!valid) break;
+      }
+      item(p, valid, v[it]);
+    }
+  } else {
+    // the rows of item it + 1 are loaded before item it goes through the table
+    double cur[7], nxt[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t p = base + threadIdx.x;
+    if (p < s.n) load(p, cur);
+    for (int it = 0; it < kItems; ++it, p += kDepPT) {
+      const bool valid = p < s.n;
+      if constexpr (AGG) {
+        if (__ballot(valid) == 0) break;  // uniform across the wave: every lane takes part in the scan
+      } else {
+        if (!valid) break;
+      }
+      if (valid) {
+        const int64_t pn = p + kDepPT;
+        if (it + 1 < kItems && pn < s.n) load(pn, nxt);
+      }
+      item(p, valid, cur);
+#pragma unroll
+      for (int k = 0; k < 7; ++k) cur[k] = nxt[k];
     }
   }
   if (bad) atomicOr(&scp->err, (int)kErrCell);
@@ -1139,14 +1168,18 @@ constexpr int kKickPT = 256;
 // RECOUT: the rows are written as records to rec[perm[p]] (the caller's
 // position of the cell-ordered particle p) instead of in place: the last kick
 // of a tracking call and the way back to the caller's order in one pass.
+// zg, zn (tracking): zn words of the charge grid to clear for the next step's
+// deposit (the z pass that read them ran before this kick).
 template <typename F, bool DRIFT, bool SUMP, bool RECOUT>
 __device__ __forceinline__ void kick_body(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
                                           const KickScalars* __restrict__ scp, DriftArgs dr, double* __restrict__ part,
-                                          const uint32_t* __restrict__ perm, PRec* __restrict__ rec, const int bx,
+                                          const uint32_t* __restrict__ perm, PRec* __restrict__ rec,
+                                          unsigned long long* __restrict__ zg, int64_t zn, const int bx,
                                           const int nbx) {
   __shared__ double lds[SUMP ? (kKickPT / 64) * 3 : 1];
   const KickScalars& s0 = *scp;
   if (s0.err) return;  // uniform across the grid
+  for (int64_t i = (int64_t)bx * kKickPT + threadIdx.x; i < zn; i += (int64_t)nbx * kKickPT) zg[i] = 0ull;
   const KickScalars s = s0;
   const int64_t blk = xcd_block(bx, nbx);
   const int64_t p = blk * kKickPT + threadIdx.x;
@@ -1221,8 +1254,9 @@ __global__ void __launch_bounds__(kKickPT) k_kick(double* __restrict__ r, int64_
                                                   const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
                                                   double* __restrict__ part = nullptr,
                                                   const uint32_t* __restrict__ perm = nullptr,
-                                                  PRec* __restrict__ rec = nullptr) {
-  kick_body<F, DRIFT, SUMP, RECOUT>(r, ld, phi, scp, dr, part, perm, rec, blockIdx.x, gridDim.x);
+                                                  PRec* __restrict__ rec = nullptr,
+                                                  unsigned long long* __restrict__ zg = nullptr, int64_t zn = 0) {
+  kick_body<F, DRIFT, SUMP, RECOUT>(r, ld, phi, scp, dr, part, perm, rec, zg, zn, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------

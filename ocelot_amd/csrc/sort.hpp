@@ -8,7 +8,8 @@
 // Per pass (the tiles of a pass hold the keys in the order the previous pass
 // left them):
 //   k_rs_hist     per 4096-key tile, the histogram of the pass's digit,
-//                 stored digit-major H[digit][tile]
+//                 stored digit-major H[digit][tile] (runs of equal digits
+//                 counted in registers before the LDS atomics)
 //   k_rs_scan     per digit: exclusive scan of H over the tiles, and the
 //                 digit's total
 //   k_rs_scatter  digit bases from the totals, stable rank of each key
@@ -30,6 +31,7 @@ namespace ocsc {
 constexpr int kRsPT = 256;                      // threads per workgroup
 constexpr int kRsItems = 16;                    // keys per thread
 constexpr int kRsTile = kRsPT * kRsItems;       // keys per tile
+static_assert(kRsItems % 4 == 0, "the histogram loads a thread's keys 16 bytes at a time");
 constexpr int kRsWaves = kRsPT / 64;
 constexpr int kRsBits = 8;
 constexpr int kRsDigits = 1 << kRsBits;
@@ -46,17 +48,42 @@ __global__ void __launch_bounds__(kRsPT) k_rs_hist(const uint32_t* __restrict__ 
   for (int i = threadIdx.x; i < kRsWaves * kRsDigits; i += kRsPT) (&h[0][0])[i] = 0;
   __syncthreads();
   const int w = threadIdx.x >> 6;
-  const int64_t base = (int64_t)blockIdx.x * kRsTile;
-  // all of a thread's keys in flight before the first count
+  // A thread counts kRsItems consecutive keys (the histogram does not depend
+  // on which thread counts which key).  Keys of a bunch kept in cell order
+  // arrive nearly sorted: in the passes over the high digits a whole tile
+  // often shares one digit, and the runs of equal digits are counted in
+  // registers first, one LDS atomic per run instead of one per key.
+  const int64_t p0 = (int64_t)blockIdx.x * kRsTile + (int64_t)threadIdx.x * kRsItems;
   uint32_t k[kRsItems];
+  if (p0 + kRsItems <= n && ((uintptr_t)keys & 15) == 0) {
+    const uint4* src = (const uint4*)(keys + p0);  // 16-byte aligned: p0 is a multiple of 16
+#pragma unroll
+    for (int i = 0; i < kRsItems / 4; ++i) {
+      const uint4 v = src[i];
+      k[4 * i] = v.x;
+      k[4 * i + 1] = v.y;
+      k[4 * i + 2] = v.z;
+      k[4 * i + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kRsItems; ++i) k[i] = p0 + i < n ? keys[p0 + i] : 0u;
+  }
+  const int cnt = p0 >= n ? 0 : (n - p0 < kRsItems ? (int)(n - p0) : kRsItems);
+  uint32_t dprev = 0, run = 0;
 #pragma unroll
   for (int i = 0; i < kRsItems; ++i) {
-    const int64_t p = base + i * kRsPT + threadIdx.x;
-    k[i] = p < n ? keys[p] : 0xFFFFFFFFu;
+    if (i < cnt) {
+      const uint32_t d = (k[i] >> shift) & (kRsDigits - 1);
+      if (run && d != dprev) {
+        atomicAdd(&h[w][dprev], run);
+        run = 0;
+      }
+      dprev = d;
+      ++run;
+    }
   }
-#pragma unroll
-  for (int i = 0; i < kRsItems; ++i)
-    if (base + i * kRsPT + threadIdx.x < n) atomicAdd(&h[w][(k[i] >> shift) & (kRsDigits - 1)], 1u);
+  if (run) atomicAdd(&h[w][dprev], run);
   __syncthreads();
   const int d = threadIdx.x;  // kRsPT == kRsDigits
   uint32_t c = 0;
