@@ -337,13 +337,14 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * LDS hash slots of the deposit, 0 (default) 8 and 2048 (FP64: 1024), 1 half
  * of both, 2 twice both, 3 no table (one global atomic per particle), 4 as 0
  * with all of a thread's particle loads issued first; bit-identical results.  "batch_sort": see
- * ocsc_apply_batched.  "fft_plan": the mixed form's transform plan, 0 LDS
- * Stockham passes, radix 16 where every thread gets a butterfly (else 8), 1
- * radix 8 in every pass, 2 half as many columns per workgroup, 3 the two
- * field passes with 256 threads per workgroup (radix 16 throughout), 4
- * (default) register-resident transforms (one LDS transpose per transform)
- * in every pass whose length is 16, 64 or 256, other lengths as 0; the
- * results differ in FP32 round-off only.  "sort_cells": sort keys of the cell-ordered copies
+ * ocsc_apply_batched.  "fft_plan": the transform plan, 0 LDS Stockham
+ * passes, radix 16 where every thread gets a butterfly (else 8), 1 radix 8
+ * in every pass, 2 half as many columns per workgroup, 3 the two field
+ * passes with 256 threads per workgroup (radix 16 throughout) (1 to 3: mixed
+ * form only, FP64 as 0), 4 (default) register-resident transforms (one LDS
+ * transpose per transform) in every pass whose length is 16 to 512 (in the
+ * FP64 form not the z forward and y inverse passes of lengths 256 and 512),
+ * other lengths as 0; the results differ in round-off only.  "sort_cells": sort keys of the cell-ordered copies
  * are bricks and the cells inside them (1, default) or bricks only (0: fewer
  * radix passes, cells inside a brick stay in their previous order); both give
  * the same particles up to the order of reduction sums.  "sort_ahead": 1

@@ -21,12 +21,12 @@ Knobs (``ocsc_set_option``):
   particle records or row by row;
 * ``overlap``: Green's-function spectrum on a second stream next to the
   deposit;
-* ``fft_plan``: the mixed form's transform plan (0: LDS Stockham passes,
-  radix 16 where every thread gets a butterfly, else 8; 1: radix 8 in every
-  pass; 2: half as many columns per workgroup, less LDS and more workgroups;
-  3: the two field passes with 256 threads, radix 16 in every pass; 4, the
-  default: register-resident transforms, one LDS transpose per transform, in
-  every pass whose length is 16, 64 or 256).
+* ``fft_plan``: the transform plan (0: LDS Stockham passes, radix 16 where
+  every thread gets a butterfly, else 8; 1: radix 8 in every pass; 2: half as
+  many columns per workgroup, less LDS and more workgroups; 3: the two field
+  passes with 256 threads, radix 16 in every pass (1 to 3 in the mixed form
+  only); 4, the default: register-resident transforms, one LDS transpose per
+  transform, in every pass whose length is 16 to 512).
 
 The search is coordinate-wise: each knob in turn over its candidates with the
 others fixed, timing ``steps`` graph-captured steps on a copy of the bunch.

@@ -197,25 +197,25 @@ __global__ void __launch_bounds__((pass_threads<float, M, V>())) kb_zinv3(const 
 }
 
 // FP64 form: convolution to the potential, then the difference fields
-template <int M>
-__global__ void __launch_bounds__(kNT) kb_xconv64(const BJob* __restrict__ J, BSeg g,
-                                                  const cpx<double>* __restrict__ tw) {
+template <int M, int V>
+__global__ void __launch_bounds__((pass_threads<double, M, V>())) kb_xconv64(const BJob* __restrict__ J, BSeg g,
+                                                                             const cpx<double>* __restrict__ tw) {
   OCSC_BJOB();
-  xconv_body<double, M>((cpx<double>*)b.B, (const double*)b.G, b.d, b.sc, tw, bx, nbx);
+  xconv_body<double, M, V>((cpx<double>*)b.B, (const double*)b.G, b.d, b.sc, tw, bx, nbx);
 }
 
-template <int M>
-__global__ void __launch_bounds__(kNT) kb_yinv64(const BJob* __restrict__ J, BSeg g,
-                                                 const cpx<double>* __restrict__ tw) {
+template <int M, int V>
+__global__ void __launch_bounds__((pass_threads<double, M, V>())) kb_yinv64(const BJob* __restrict__ J, BSeg g,
+                                                                            const cpx<double>* __restrict__ tw) {
   OCSC_BJOB();
-  yinv_body<double, M>((const cpx<double>*)b.B, (cpx<double>*)b.A, b.d, b.sc, tw, bx, nbx);
+  yinv_body<double, M, V>((const cpx<double>*)b.B, (cpx<double>*)b.A, b.d, b.sc, tw, bx, nbx);
 }
 
-template <int M>
-__global__ void __launch_bounds__(kNT) kb_zinv64(const BJob* __restrict__ J, BSeg g,
-                                                 const cpx<double>* __restrict__ tw) {
+template <int M, int V>
+__global__ void __launch_bounds__((pass_threads<double, M, V>())) kb_zinv64(const BJob* __restrict__ J, BSeg g,
+                                                                            const cpx<double>* __restrict__ tw) {
   OCSC_BJOB();
-  zinv_body<double, M, false>((const cpx<double>*)b.A, (double*)b.phi, b.d, b.sc, tw, nullptr, bx, nbx);
+  zinv_body<double, M, false, V>((const cpx<double>*)b.A, (double*)b.phi, b.d, b.sc, tw, nullptr, bx, nbx);
 }
 
 __global__ void __launch_bounds__(256) kb_diff64(const BJob* __restrict__ J, BSeg g) {

@@ -21,9 +21,9 @@ __device__ __forceinline__ constexpr double w16c(int k) {
 }
 __device__ __forceinline__ constexpr double w16s(int k) { return w16c(k - 4); }
 
-// y = x * exp(DIR * 2 pi i * k / R) with constant k/R; DIR = -1 forward
+// y = x * exp(DIR * 2 pi i * k / R) for k/R a multiple of 1/16
 template <int R, int K, int DIR, typename F>
-__device__ __forceinline__ cpx<F> twc(cpx<F> v) {
+__device__ __forceinline__ cpx<F> twc16(cpx<F> v) {
   constexpr int k16 = ((K * 16 / R) % 16 + 16) % 16;
   if constexpr (k16 == 0) {
     return v;
@@ -37,6 +37,30 @@ __device__ __forceinline__ cpx<F> twc(cpx<F> v) {
     const F c = (F)w16c(k16);
     const F s = (F)(DIR * w16s(k16));
     return cmul(v, cpx<F>{c, s});
+  }
+}
+
+// cos of 2 pi k / 32 for odd k (the even k are w16c's)
+__device__ __forceinline__ constexpr double w32c_odd(int k) {
+  constexpr double c[16] = {0.980785280403230449126,  0.831469612302545237079,  0.555570233019602224743,
+                            0.195090322016128267848,  -0.195090322016128267848, -0.555570233019602224743,
+                            -0.831469612302545237079, -0.980785280403230449126, -0.980785280403230449126,
+                            -0.831469612302545237079, -0.555570233019602224743, -0.195090322016128267848,
+                            0.195090322016128267848,  0.555570233019602224743,  0.831469612302545237079,
+                            0.980785280403230449126};
+  return c[(k & 31) >> 1];
+}
+
+// y = x * exp(DIR * 2 pi i * k / R) with constant k/R (R <= 32); DIR = -1 forward
+template <int R, int K, int DIR, typename F>
+__device__ __forceinline__ cpx<F> twc(cpx<F> v) {
+  constexpr int k32 = ((K * 32 / R) % 32 + 32) % 32;
+  if constexpr (k32 % 2 == 1) {
+    const F c = (F)w32c_odd(k32);
+    const F s = (F)(DIR * w32c_odd(k32 + 24));  // sin(a) = cos(a - pi/2)
+    return cmul(v, cpx<F>{c, s});
+  } else {
+    return twc16<R, K, DIR, F>(v);
   }
 }
 
@@ -155,31 +179,34 @@ __device__ __forceinline__ void lds_fft(cpx<F>* s, const cpx<F>* __restrict__ tw
 }
 
 // ---------------------------------------------------------------------------
-// Register-resident transforms of length M = R*R (R = 4, 8 or 16): the
-// four-step factorisation X[k + R*q] = sum_j W_R^{jq} W_M^{jk} sum_n
-// x[R*n + j] W_R^{nk}, with R threads per column.  Thread j of a column
-// holds x[n] = element R*n + j on entry; it takes the R-point DFT over n in
-// registers, multiplies by W_M^{jk}, and one transpose through LDS hands
-// every thread the R values of one k, whose R-point DFT leaves element
-// j + R*q in x[q].  That output layout is the input layout of the next
-// transform, so chained transforms (forward, pointwise product, inverse)
-// touch LDS once each instead of once per radix pass.
+// Register-resident transforms of length M = R*Q with Q = R (M = 16, 64,
+// 256) or Q = 2R (M = 32, 128, 512), Q threads per column, from the four-step
+// factorisation X[k + R*q] = sum_j W_Q^{jq} W_M^{jk} sum_n x[Q*n + j] W_R^{nk}
+// (n, k < R; j, q < Q).  Thread j of a column holds x[n] = element Q*n + j on
+// entry; it takes the R-point DFT over n in registers and multiplies by
+// W_M^{jk}; one transpose through LDS then hands thread j' = k + R*h the Q
+// values of its k.  For Q = R it takes their R-point DFT; for Q = 2R the half
+// of the Q-point DFT with outputs of parity h: DFT_R(v[i] + v[i+R]) (h = 0)
+// or DFT_R((v[i] - v[i+R]) W_Q^i) (h = 1).  Either way element j' + Q*m ends
+// in x[m]: the output layout is the input layout of the next transform, so
+// chained transforms (forward, pointwise product, inverse) touch LDS once
+// each instead of once per radix pass.
 //
-// Exchange buffer for T columns: R blocks of R*T entries plus padding.
+// Exchange buffer for T columns: R blocks of Q*T entries plus padding.
 // Lanes take the column index t fastest (JT = false: columns side by side in
 // memory) or the in-column index j fastest (JT = true: rows contiguous in
 // memory).  JT = false: thread (t, j) writes its value k to [k][j][t] and
-// reads [j][n][t] for n < R; consecutive lanes touch consecutive entries in
-// both phases, and the block stride R*T + 16 (16 entries past a multiple of
-// 512 bytes when R*T is a multiple of 32) puts the blocks one read
+// reads [k'][i][t]; consecutive lanes touch consecutive entries in both
+// phases, and the block stride Q*T + 16 (16 entries past a multiple of 512
+// bytes for FP32 when Q*T is a multiple of 32) puts the blocks one read
 // instruction touches into alternate halves of the banks.  JT = true: writes
-// go to [k][t][j] (consecutive) and reads to [j][t][n], whose lanes step by
-// the block stride R*T + 1, odd, so 32 lanes hit 32 different bank pairs.
+// go to [k][t][j] (consecutive) and reads to [k'][t][i], whose lanes step by
+// the block stride Q*T + 1, odd, so 32 lanes hit 32 different bank pairs.
 constexpr int kRegPad = 16;
 
-template <int R, int T, bool JT = false>
+template <int R, int Q, int T, bool JT = false>
 struct RegX {
-  static constexpr int blk = R * T + (JT ? 1 : kRegPad);
+  static constexpr int blk = Q * T + (JT ? 1 : kRegPad);
   static constexpr int entries = R * blk;
 };
 
@@ -213,14 +240,26 @@ __device__ __forceinline__ void dft_pruned(cpx<F>* v) {
   }
 }
 
+// v[I] = odd ? (a[I] - b[I]) W_{2R}^I : a[I] + b[I] (per-lane select)
+template <int R, int DIR, typename F, int I = 0>
+__device__ __forceinline__ void half_split(cpx<F>* v, const cpx<F>* a, const cpx<F>* b, const bool odd) {
+  if constexpr (I < R) {
+    const cpx<F> d = twc<2 * R, I, DIR>(csub(a[I], b[I]));
+    const cpx<F> e = cadd(a[I], b[I]);
+    v[I] = odd ? d : e;
+    half_split<R, DIR, F, I + 1>(v, a, b, odd);
+  }
+}
+
 // One transform of the column (t, j) as described above.  DIR = -1 forward,
 // +1 inverse without 1/M; tw[k] = exp(-2 pi i k / M), k < M.  Begins with a
 // barrier (the buffer may still be read by the previous transform); the
 // caller publishes tw before the first call.
-template <typename F, int R, int T, int DIR, int NIN, bool JT = false>
+template <typename F, int R, int Q, int T, int DIR, int NIN, bool JT = false>
 __device__ __forceinline__ void reg_fft(cpx<F> (&x)[R], cpx<F>* s, const int t, const int j,
                                         const cpx<F>* __restrict__ tw) {
-  constexpr int BLK = RegX<R, T, JT>::blk;
+  static_assert(Q == R || Q == 2 * R, "Q threads per column: R or 2R");
+  constexpr int BLK = RegX<R, Q, T, JT>::blk;
   dft_pruned<R, DIR, F, NIN>(x);
 #pragma unroll
   for (int k = 1; k < R; ++k) {
@@ -230,10 +269,21 @@ __device__ __forceinline__ void reg_fft(cpx<F> (&x)[R], cpx<F>* s, const int t, 
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < R; ++k) s[k * BLK + (JT ? t * R + j : j * T + t)] = x[k];
+  for (int k = 0; k < R; ++k) s[k * BLK + (JT ? t * Q + j : j * T + t)] = x[k];
   __syncthreads();
+  const int k1 = j & (R - 1);
+  if constexpr (Q == R) {
 #pragma unroll
-  for (int n = 0; n < R; ++n) x[n] = s[j * BLK + (JT ? t * R + n : n * T + t)];
+    for (int i = 0; i < R; ++i) x[i] = s[k1 * BLK + (JT ? t * Q + i : i * T + t)];
+  } else {
+    cpx<F> a[R], b[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      a[i] = s[k1 * BLK + (JT ? t * Q + i : i * T + t)];
+      b[i] = s[k1 * BLK + (JT ? t * Q + i + R : (i + R) * T + t)];
+    }
+    half_split<R, DIR, F>(x, a, b, j >= R);
+  }
   Dft<R, DIR, F>::run(x);
 }
 

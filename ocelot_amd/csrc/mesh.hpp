@@ -122,57 +122,69 @@ __device__ __forceinline__ Col flat_col(int ncol, int nb, int bx) {
 // run-time sizes, so the compiler would otherwise keep them rolled).
 constexpr int per_thread(int count, int nt) { return (count + nt - 1) / nt; }
 
-// Plan 4 (option "fft_plan"): register-resident transforms (reg_fft) for the
-// square lengths M = 16, 64, 256 in every FP32 pass (z and y forward, the two
-// field passes, z inverse), sqrt(M) threads per column; other lengths and
-// FP64 as plan 0.
+// Plan 4 (option "fft_plan"): register-resident transforms (reg_fft) in
+// every pass whose length is M = R*Q, Q = R (16, 64, 256) or Q = 2R (32, 128,
+// 512), Q threads per column, both forms; other lengths as plan 0.
 template <int M>
-constexpr int sqrt_len() {
-  return M == 16 ? 4 : (M == 64 ? 8 : (M == 256 ? 16 : 0));
+constexpr int reg_r() {
+  return M == 16 || M == 32 ? 4 : (M == 64 || M == 128 ? 8 : (M == 256 || M == 512 ? 16 : 0));
+}
+template <int M>
+constexpr int reg_q() {
+  return reg_r<M>() ? M / reg_r<M>() : 1;
 }
 template <typename F, int M, int V>
 constexpr bool reg_plan() {
-  return V == 4 && sizeof(F) == 4 && sqrt_len<M>() > 0;
+  return V == 4 && reg_r<M>() > 0;
 }
+// The plan the FP64 z forward and y inverse passes take under plan V: at
+// M >= 256 a register-resident FP64 column (16 complex doubles per thread)
+// leaves too few waves per CU for those two passes, and they keep plan 0
+// (127^3, one MI355X: z forward 26.8 vs 39.0 us, y inverse 30.1 vs 31.7 us;
+// the x pass 58.3 -> 46.9 us and the z inverse 17.7 -> 15.4 us do gain).
+template <typename F, int M, int V>
+constexpr int plan_zf_yi() {
+  return sizeof(F) == 8 && M >= 256 ? 0 : V;
+}
+
 // threads and LDS bytes per workgroup of the z and y passes (plan 4: the
 // larger exchange buffer of the two lane orders, twiddles, difference factors)
 template <typename F, int M, int V>
 constexpr int pass_threads() {
-  return reg_plan<F, M, V>() ? sqrt_len<M>() * Tile<F, M>::T : kNT;
+  return reg_plan<F, M, V>() ? reg_q<M>() * Tile<F, M>::T : kNT;
 }
 template <typename F, int M, int V>
 constexpr size_t pass_lds() {
-  constexpr int R = reg_plan<F, M, V>() ? sqrt_len<M>() : 1;
-  return reg_plan<F, M, V>() ? (size_t)(RegX<R, Tile<F, M>::T>::entries + 2 * M) * sizeof(cpx<F>)
+  constexpr int R = reg_plan<F, M, V>() ? reg_r<M>() : 1;
+  return reg_plan<F, M, V>() ? (size_t)(RegX<R, reg_q<M>(), Tile<F, M>::T>::entries + 2 * M) * sizeof(cpx<F>)
                              : Tile<F, M, V>::lds_bytes;
 }
 
 // Plan 4 z forward: T pairs of real rows (2t, 2t+1) as T complex columns,
 // lanes j fastest (rows are contiguous in memory): thread (t, j) holds
-// z = R*n + j of its pair; z < nz <= M/2, so only n < R/2 can be non-zero.
+// z = Q*n + j of its pair; z < nz <= M/2, so only n < R/2 can be non-zero.
 // The spectrum then goes to LDS in natural order and is split into the two
 // rows' half spectra as in zfwd_body.
-template <int M, int CIN>
-__device__ __forceinline__ void zfwd_reg(const void* __restrict__ rho, cpx<float>* __restrict__ A, MeshDims d,
-                                         const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+template <typename F, int M, int CIN>
+__device__ __forceinline__ void zfwd_reg(const void* __restrict__ rho, cpx<F>* __restrict__ A, MeshDims d,
+                                         const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
                                          int ncopy, const int bx) {
-  using F = float;
-  constexpr int R = sqrt_len<M>(), T = Tile<F, M>::T, NT = R * T;
+  constexpr int R = reg_r<M>(), Q = reg_q<M>(), T = Tile<F, M>::T, NT = Q * T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
-  cpx<F>* twl = s + RegX<R, T>::entries;
+  cpx<F>* twl = s + RegX<R, Q, T>::entries;
   if (sc->err) return;
   const int nz = d.nz, Hz = d.hz;
   const int64_t nrow = (int64_t)d.nx * d.ny;
   const int64_t ncell = nrow * nz;
   const int64_t row0 = (int64_t)bx * (2 * T);
   const double iq = sc->inv_qscale;
-  const int j = threadIdx.x % R, t = threadIdx.x / R;
+  const int j = threadIdx.x % Q, t = threadIdx.x / Q;
   const int64_t ra = row0 + 2 * t, rb = ra + 1;
   cpx<F> x[R];
 #pragma unroll
   for (int n = 0; n < R; ++n) {
-    const int m = R * n + j;
+    const int m = Q * n + j;
     x[n] = cpx<F>{0, 0};
     if (n < R / 2 && m < nz) {
       if (ra < nrow) x[n].x = (F)charge_at<CIN, F>(rho, ra * nz + m, ncell, iq, ncopy);
@@ -181,10 +193,10 @@ __device__ __forceinline__ void zfwd_reg(const void* __restrict__ rho, cpx<float
   }
   for (int i = threadIdx.x; i < M; i += NT) twl[i] = tw[i];
   __syncthreads();
-  reg_fft<F, R, T, -1, R / 2, true>(x, s, t, j, twl);
+  reg_fft<F, R, Q, T, -1, R / 2, true>(x, s, t, j, twl);
   __syncthreads();  // the exchange buffer is read; now it holds the spectra
 #pragma unroll
-  for (int q = 0; q < R; ++q) s[t * M + j + R * q] = x[q];
+  for (int q = 0; q < R; ++q) s[t * M + j + Q * q] = x[q];
   __syncthreads();
   constexpr int NS = per_thread(T * (M / 2 + 1), NT);
 #pragma unroll
@@ -202,17 +214,16 @@ __device__ __forceinline__ void zfwd_reg(const void* __restrict__ rho, cpx<float
   }
 }
 
-// Plan 4 y forward: thread j of column t holds rows y = R*n + j; y < ny <=
+// Plan 4 y forward: thread j of column t holds rows y = Q*n + j; y < ny <=
 // M/2, so only n < R/2 can be non-zero; all M outputs are kept.
-template <int M>
-__device__ __forceinline__ void yfwd_reg(const cpx<float>* __restrict__ A, cpx<float>* __restrict__ B, MeshDims d,
-                                         const KickScalars* __restrict__ sc, const cpx<float>* __restrict__ tw,
+template <typename F, int M>
+__device__ __forceinline__ void yfwd_reg(const cpx<F>* __restrict__ A, cpx<F>* __restrict__ B, MeshDims d,
+                                         const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
                                          const int bx) {
-  using F = float;
-  constexpr int R = sqrt_len<M>(), T = Tile<F, M>::T, NT = R * T;
+  constexpr int R = reg_r<M>(), Q = reg_q<M>(), T = Tile<F, M>::T, NT = Q * T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
-  cpx<F>* twl = s + RegX<R, T>::entries;
+  cpx<F>* twl = s + RegX<R, Q, T>::entries;
   if (sc->err) return;
   const int Hz = d.hz, ny = d.ny;
   const Col c = flat_col<T, NT>(d.nx * Hz, Hz, bx);
@@ -221,17 +232,17 @@ __device__ __forceinline__ void yfwd_reg(const cpx<float>* __restrict__ A, cpx<f
   cpx<F> x[R];
 #pragma unroll
   for (int n = 0; n < R; ++n) {
-    const int y = R * n + j;
+    const int y = Q * n + j;
     x[n] = cpx<F>{0, 0};
     if (n < R / 2 && c.ok && y < ny) x[n] = src[(int64_t)y * Hz];
   }
   for (int i = threadIdx.x; i < M; i += NT) twl[i] = tw[i];
   __syncthreads();
-  reg_fft<F, R, T, -1, R / 2>(x, s, c.t, j, twl);
+  reg_fft<F, R, Q, T, -1, R / 2>(x, s, c.t, j, twl);
   if (!c.ok) return;
   cpx<F>* dst = B + (int64_t)c.a * M * Hz + c.b;
 #pragma unroll
-  for (int q = 0; q < R; ++q) dst[(int64_t)(j + R * q) * Hz] = x[q];
+  for (int q = 0; q < R; ++q) dst[(int64_t)(j + Q * q) * Hz] = x[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -241,7 +252,7 @@ __device__ __forceinline__ void zfwd_body(const void* __restrict__ rho, cpx<F>* 
                                           const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw, int ncopy,
                                           const int bx, const int nbx) {
   if constexpr (reg_plan<F, M, V>()) {
-    zfwd_reg<M, CIN>(rho, A, d, sc, tw, ncopy, bx);
+    zfwd_reg<F, M, CIN>(rho, A, d, sc, tw, ncopy, bx);
     return;
   }
   using TL = Tile<F, M, V>;
@@ -310,7 +321,7 @@ __device__ __forceinline__ void yfwd_body(const cpx<F>* __restrict__ A, cpx<F>* 
                                           const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
                                           const int bx, const int nbx) {
   if constexpr (reg_plan<F, M, V>()) {
-    yfwd_reg<M>(A, B, d, sc, tw, bx);
+    yfwd_reg<F, M>(A, B, d, sc, tw, bx);
     return;
   }
   using TL = Tile<F, M, V>;
@@ -353,12 +364,93 @@ __global__ void __launch_bounds__((pass_threads<F, M, V>())) k_yfwd(const cpx<F>
   yfwd_body<F, M, V>(A, B, d, sc, tw, blockIdx.x, gridDim.x);
 }
 
+// Plan 4 x pass of the potential: thread j of column t holds rows x = Q*n + j
+// (n < R/2 non-zero: nx <= M/2), forward transform, times the kernel
+// spectrum, inverse, rows x = j + Q*q < nx kept.  In place on B.
+template <typename F, int M>
+__device__ __forceinline__ void xconv_reg(cpx<F>* __restrict__ B, const F* __restrict__ G, MeshDims d,
+                                          const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
+                                          const int bx) {
+  constexpr int R = reg_r<M>(), Q = reg_q<M>(), T = Tile<F, M>::T, NT = Q * T;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  cpx<F>* twl = s + RegX<R, Q, T>::entries;
+  if (sc->err) return;
+  const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
+  const int64_t cs = (int64_t)My * Hz;  // stride between x planes
+  const Col c = flat_col<T, NT>(My * Hz, Hz, bx);  // a = ky, b = kz
+  const int j = threadIdx.x / T;
+  cpx<F>* col = B + (int64_t)c.a * Hz + c.b;
+  cpx<F> x[R];
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    const int row = Q * n + j;
+    x[n] = cpx<F>{0, 0};
+    if (n < R / 2 && c.ok && row < nx) x[n] = col[row * cs];
+  }
+  for (int i = threadIdx.x; i < M; i += NT) twl[i] = tw[i];
+  __syncthreads();
+  reg_fft<F, R, Q, T, -1, R / 2>(x, s, c.t, j, twl);
+  const int fy = c.a <= My - c.a ? c.a : My - c.a;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int kx = j + Q * q;
+    const int fx = kx <= M - kx ? kx : M - kx;
+    const F g = c.ok ? G[((int64_t)fx * Hy + fy) * Hz + c.b] : (F)0;
+    x[q] = cscale(x[q], g);
+  }
+  reg_fft<F, R, Q, T, +1, R>(x, s, c.t, j, twl);
+  if (!c.ok) return;
+#pragma unroll
+  for (int q = 0; q < R / 2; ++q) {
+    const int row = j + Q * q;
+    if (row < nx) col[row * cs] = x[q];
+  }
+}
+
+// Plan 4 y inverse of the potential: all M spectrum rows in, rows
+// y = j + Q*q < ny out.
+template <typename F, int M>
+__device__ __forceinline__ void yinv_reg(const cpx<F>* __restrict__ B, cpx<F>* __restrict__ A, MeshDims d,
+                                         const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
+                                         const int bx) {
+  constexpr int R = reg_r<M>(), Q = reg_q<M>(), T = Tile<F, M>::T, NT = Q * T;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<F>* s = (cpx<F>*)smem_raw;
+  cpx<F>* twl = s + RegX<R, Q, T>::entries;
+  if (sc->err) return;
+  const int Hz = d.hz, ny = d.ny;
+  const Col c = flat_col<T, NT>(d.nx * Hz, Hz, bx);  // a = x, b = kz
+  const int j = threadIdx.x / T;
+  const cpx<F>* src = B + (int64_t)c.a * M * Hz + c.b;
+  cpx<F> x[R];
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    x[n] = cpx<F>{0, 0};
+    if (c.ok) x[n] = src[(int64_t)(Q * n + j) * Hz];
+  }
+  for (int i = threadIdx.x; i < M; i += NT) twl[i] = tw[i];
+  __syncthreads();
+  reg_fft<F, R, Q, T, +1, R>(x, s, c.t, j, twl);
+  if (!c.ok) return;
+  cpx<F>* dst = A + (int64_t)c.a * ny * Hz + c.b;
+#pragma unroll
+  for (int q = 0; q < R / 2; ++q) {
+    const int y = j + Q * q;
+    if (y < ny) dst[(int64_t)y * Hz] = x[q];
+  }
+}
+
 // x pass: forward FFT along x, multiply by the kernel spectrum, inverse FFT,
 // keep x < nx.  In place on B.
-template <typename F, int M>
+template <typename F, int M, int V = 0>
 __device__ __forceinline__ void xconv_body(cpx<F>* __restrict__ B, const F* __restrict__ G, MeshDims d,
                                            const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
                                            const int bx, const int nbx) {
+  if constexpr (reg_plan<F, M, V>()) {
+    xconv_reg<F, M>(B, G, d, sc, tw, bx);
+    return;
+  }
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
@@ -407,18 +499,22 @@ __device__ __forceinline__ void xconv_body(cpx<F>* __restrict__ B, const F* __re
   }
 }
 
-template <typename F, int M>
-__global__ void __launch_bounds__(kNT) k_xconv(cpx<F>* __restrict__ B, const F* __restrict__ G, MeshDims d,
-                                               const KickScalars* __restrict__ sc,
-                                               const cpx<F>* __restrict__ tw) {
-  xconv_body<F, M>(B, G, d, sc, tw, blockIdx.x, gridDim.x);
+template <typename F, int M, int V = 0>
+__global__ void __launch_bounds__((pass_threads<F, M, V>())) k_xconv(cpx<F>* __restrict__ B, const F* __restrict__ G,
+                                                                     MeshDims d, const KickScalars* __restrict__ sc,
+                                                                     const cpx<F>* __restrict__ tw) {
+  xconv_body<F, M, V>(B, G, d, sc, tw, blockIdx.x, gridDim.x);
 }
 
 // y pass inverse: B[x][ky][kz] -> A[x][y<ny][kz]
-template <typename F, int M>
+template <typename F, int M, int V = 0>
 __device__ __forceinline__ void yinv_body(const cpx<F>* __restrict__ B, cpx<F>* __restrict__ A, MeshDims d,
                                           const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
                                           const int bx, const int nbx) {
+  if constexpr (reg_plan<F, M, V>()) {
+    yinv_reg<F, M>(B, A, d, sc, tw, bx);
+    return;
+  }
   constexpr int T = Tile<F, M>::T, S = Tile<F, M>::S;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
@@ -451,11 +547,12 @@ __device__ __forceinline__ void yinv_body(const cpx<F>* __restrict__ B, cpx<F>* 
   }
 }
 
-template <typename F, int M>
-__global__ void __launch_bounds__(kNT) k_yinv(const cpx<F>* __restrict__ B, cpx<F>* __restrict__ A, MeshDims d,
-                                              const KickScalars* __restrict__ sc,
-                                              const cpx<F>* __restrict__ tw) {
-  yinv_body<F, M>(B, A, d, sc, tw, blockIdx.x, gridDim.x);
+template <typename F, int M, int V = 0>
+__global__ void __launch_bounds__((pass_threads<F, M, V>())) k_yinv(const cpx<F>* __restrict__ B,
+                                                                    cpx<F>* __restrict__ A, MeshDims d,
+                                                                    const KickScalars* __restrict__ sc,
+                                                                    const cpx<F>* __restrict__ tw) {
+  yinv_body<F, M, V>(B, A, d, sc, tw, blockIdx.x, gridDim.x);
 }
 
 // z pass inverse: two Hermitian half spectra -> one complex inverse FFT ->
@@ -514,46 +611,27 @@ __device__ __forceinline__ void zinv_rows(const cpx<F>* __restrict__ A, F* __res
   }
 }
 
-template <typename F, int M, bool DERIV>
-__device__ __forceinline__ void zinv_body(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d,
-                                          const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
-                                          const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  if (sc->err) return;
-  zinv_rows<F, M, DERIV>(A, phi, d, sc, tw, twd, (cpx<F>*)smem_raw, bx);
-}
-
-template <typename F, int M, bool DERIV = false>
-__global__ void __launch_bounds__(kNT) k_zinv(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d,
-                                              const KickScalars* __restrict__ sc,
-                                              const cpx<F>* __restrict__ tw,
-                                              const cpx<double>* __restrict__ twd = nullptr) {
-  zinv_body<F, M, DERIV>(A, phi, d, sc, tw, twd, blockIdx.x, gridDim.x);
-}
-
 // Plan 4 z inverse: thread (t, j) (lanes j fastest) assembles the values
-// m = R*n + j of the complex column a + i b (half spectra of rows 2t, 2t+1)
+// m = Q*n + j of the complex column a + i b (half spectra of rows 2t, 2t+1)
 // straight from global memory (m > M/2 from the conjugates at M - m, as
-// zinv_rows mirrors them), the z field's difference factor applied first;
-// rows z = j + R*q < nz <= M/2 are written.
-template <int M, bool DERIV>
-__device__ __forceinline__ void zinv_rows_reg(const cpx<float>* __restrict__ A, float* __restrict__ phi,
-                                              const MeshDims& d, const KickScalars* __restrict__ sc,
-                                              const cpx<float>* __restrict__ tw, const cpx<double>* __restrict__ twd,
-                                              cpx<float>* s, const int bx) {
-  using F = float;
-  constexpr int R = sqrt_len<M>(), T = Tile<F, M>::T, NT = R * T;
-  cpx<F>* twl = s + RegX<R, T>::entries;
+// zinv_rows mirrors them), the z field's difference factor applied first
+// (DERIV); rows z = j + Q*q < nz <= M/2 are written.
+template <typename F, int M, bool DERIV>
+__device__ __forceinline__ void zinv_rows_reg(const cpx<F>* __restrict__ A, F* __restrict__ phi, const MeshDims& d,
+                                              const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
+                                              const cpx<double>* __restrict__ twd, cpx<F>* s, const int bx) {
+  constexpr int R = reg_r<M>(), Q = reg_q<M>(), T = Tile<F, M>::T, NT = Q * T;
+  cpx<F>* twl = s + RegX<R, Q, T>::entries;
   cpx<F>* dl = twl + M;
   const int nz = d.nz, Hz = d.hz;
   const int64_t nrow = (int64_t)d.nx * d.ny;
   const int64_t row0 = (int64_t)bx * (2 * T);
-  const int j = threadIdx.x % R, t = threadIdx.x / R;
+  const int j = threadIdx.x % Q, t = threadIdx.x / Q;
   const int64_t ra = row0 + 2 * t, rb = ra + 1;
   cpx<F> a[R], b[R];
 #pragma unroll
   for (int n = 0; n < R; ++n) {
-    const int m = R * n + j;
+    const int m = Q * n + j;
     const int k = m <= M / 2 ? m : M - m;
     a[n] = cpx<F>{0, 0};
     b[n] = cpx<F>{0, 0};
@@ -568,7 +646,7 @@ __device__ __forceinline__ void zinv_rows_reg(const cpx<float>* __restrict__ A, 
   cpx<F> x[R];
 #pragma unroll
   for (int n = 0; n < R; ++n) {
-    const int m = R * n + j;
+    const int m = Q * n + j;
     const int k = m <= M / 2 ? m : M - m;
     cpx<F> aa = a[n], bb = b[n];
     if constexpr (DERIV) {
@@ -578,15 +656,36 @@ __device__ __forceinline__ void zinv_rows_reg(const cpx<float>* __restrict__ A, 
     }
     x[n] = m <= M / 2 ? cpx<F>{aa.x - bb.y, aa.y + bb.x} : cpx<F>{aa.x + bb.y, bb.x - aa.y};
   }
-  reg_fft<F, R, T, +1, R, true>(x, s, t, j, twl);
+  reg_fft<F, R, Q, T, +1, R, true>(x, s, t, j, twl);
 #pragma unroll
   for (int q = 0; q < R / 2; ++q) {
-    const int z = j + R * q;
+    const int z = j + Q * q;
     if (z < nz) {
       if (ra < nrow) phi[ra * nz + z] = x[q].x;
       if (rb < nrow) phi[rb * nz + z] = x[q].y;
     }
   }
+}
+
+template <typename F, int M, bool DERIV, int V = 0>
+__device__ __forceinline__ void zinv_body(const cpx<F>* __restrict__ A, F* __restrict__ phi, MeshDims d,
+                                          const KickScalars* __restrict__ sc, const cpx<F>* __restrict__ tw,
+                                          const cpx<double>* __restrict__ twd, const int bx, const int nbx) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (sc->err) return;
+  if constexpr (reg_plan<F, M, V>())
+    zinv_rows_reg<F, M, DERIV>(A, phi, d, sc, tw, twd, (cpx<F>*)smem_raw, bx);
+  else
+    zinv_rows<F, M, DERIV>(A, phi, d, sc, tw, twd, (cpx<F>*)smem_raw, bx);
+}
+
+template <typename F, int M, bool DERIV = false, int V = 0>
+__global__ void __launch_bounds__((pass_threads<F, M, V>())) k_zinv(const cpx<F>* __restrict__ A,
+                                                                    F* __restrict__ phi, MeshDims d,
+                                                                    const KickScalars* __restrict__ sc,
+                                                                    const cpx<F>* __restrict__ tw,
+                                                                    const cpx<double>* __restrict__ twd = nullptr) {
+  zinv_body<F, M, DERIV, V>(A, phi, d, sc, tw, twd, blockIdx.x, gridDim.x);
 }
 
 // Mixed form: the three field components in one launch (blockIdx.y), the
@@ -601,9 +700,9 @@ __device__ __forceinline__ void zinv3_body(const cpx<float>* __restrict__ A, int
   cpx<float>* s = (cpx<float>*)smem_raw;
   if constexpr (reg_plan<float, M, V>()) {
     if (comp == 2)
-      zinv_rows_reg<M, true>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s, bx);
+      zinv_rows_reg<float, M, true>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s, bx);
     else
-      zinv_rows_reg<M, false>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s, bx);
+      zinv_rows_reg<float, M, false>(A + comp * nyz, E + comp * ncell, d, sc, tw, twd, s, bx);
   } else {
     if (comp == 2)
       zinv_rows<float, M, true, V>(A + 2 * nyz, E + 2 * ncell, d, sc, tw, twd, s, bx);
@@ -634,7 +733,7 @@ constexpr bool reg_field() {
 }
 template <int M, int V>
 constexpr int field_threads() {
-  return reg_field<M, V>() ? sqrt_len<M>() * Tile<float, M>::T : (V == 3 ? 256 : kNTF);
+  return reg_field<M, V>() ? reg_q<M>() * Tile<float, M>::T : (V == 3 ? 256 : kNTF);
 }
 
 template <int M, int NT = kNT, int V = 0>
@@ -645,10 +744,11 @@ struct FieldTile {
   static constexpr int PER = (M * T + NT - 1) / NT;  // spectrum values held per thread
   static constexpr bool stw = Tile<float, M, V>::stw;
   static constexpr int RSEL = Tile<float, M, V>::RSEL;
-  static constexpr int R = reg ? sqrt_len<M>() : 1;
+  static constexpr int R = reg ? reg_r<M>() : 1;
+  static constexpr int Q = reg ? reg_q<M>() : 1;
   // plan 4: the exchange buffer, the twiddle table, the difference factors
   static constexpr size_t lds =
-      reg ? (size_t)(RegX<R, T>::entries + 2 * M) * sizeof(cpx<float>) : Tile<float, M, V>::lds_bytes;
+      reg ? (size_t)(RegX<R, Q, T>::entries + 2 * M) * sizeof(cpx<float>) : Tile<float, M, V>::lds_bytes;
 };
 
 // Plan 4 set-up: twiddles W_M^k and difference factors dfac(k) for k < M
@@ -663,8 +763,8 @@ __device__ __forceinline__ void stage_reg_tables(cpx<float>* twl, cpx<float>* dl
 }
 
 // Plan 4 x pass (see xconv_field_body): thread j of column t holds rows
-// x = R*n + j; x < nx <= M/2 (M >= 2 nx - 1 and even), so only n < R/2 can
-// be non-zero and only outputs x = j + R*q with q < R/2 are kept.
+// x = Q*n + j; x < nx <= M/2 (M >= 2 nx - 1 and even), so only n < R/2 can
+// be non-zero and only outputs x = j + Q*q with q < R/2 are kept.
 template <int M>
 __device__ __forceinline__ void xconv_field_reg(const cpx<float>* __restrict__ B, cpx<float>* __restrict__ Qx,
                                                 cpx<float>* __restrict__ P, const float* __restrict__ G, MeshDims d,
@@ -672,10 +772,10 @@ __device__ __forceinline__ void xconv_field_reg(const cpx<float>* __restrict__ B
                                                 const cpx<double>* __restrict__ twd, const int bx) {
   using F = float;
   using FT = FieldTile<M, field_threads<M, 4>(), 4>;
-  constexpr int R = FT::R, T = FT::T, NT = field_threads<M, 4>();
+  constexpr int R = FT::R, Q = FT::Q, T = FT::T, NT = field_threads<M, 4>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
-  cpx<F>* twl = s + RegX<R, T>::entries;
+  cpx<F>* twl = s + RegX<R, Q, T>::entries;
   cpx<F>* dl = twl + M;
   if (sc->err) return;
   const int Hz = d.hz, nx = d.nx, My = d.my, Hy = d.hy;
@@ -686,44 +786,44 @@ __device__ __forceinline__ void xconv_field_reg(const cpx<float>* __restrict__ B
   cpx<F> x[R];
 #pragma unroll
   for (int n = 0; n < R; ++n) {
-    const int row = R * n + j;
+    const int row = Q * n + j;
     x[n] = cpx<F>{0, 0};
     if (n < R / 2 && c.ok && row < nx) x[n] = B[row * cs + off];
   }
   stage_reg_tables<M, NT>(twl, dl, tw, twd, 1.0 / sc->steps[0]);
   __syncthreads();
-  reg_fft<F, R, T, -1, R / 2>(x, s, c.t, j, twl);
-  // x[q]: the spectrum at kx = j + R*q; v = S*G kept for the potential
+  reg_fft<F, R, Q, T, -1, R / 2>(x, s, c.t, j, twl);
+  // x[q]: the spectrum at kx = j + Q*q; v = S*G kept for the potential
   const int fy = c.a <= My - c.a ? c.a : My - c.a;
   cpx<F> v[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
-    const int kx = j + R * q;
+    const int kx = j + Q * q;
     const int fx = kx <= M - kx ? kx : M - kx;
     const F g = c.ok ? G[((int64_t)fx * Hy + fy) * Hz + c.b] : (F)0;
     v[q] = cscale(x[q], g);
     x[q] = cmul(v[q], dl[kx]);
   }
-  reg_fft<F, R, T, +1, R>(x, s, c.t, j, twl);
+  reg_fft<F, R, Q, T, +1, R>(x, s, c.t, j, twl);
   if (c.ok) {
 #pragma unroll
     for (int q = 0; q < R / 2; ++q) {
-      const int row = j + R * q;
+      const int row = j + Q * q;
       if (row < nx) Qx[row * cs + off] = x[q];
     }
   }
-  reg_fft<F, R, T, +1, R>(v, s, c.t, j, twl);
+  reg_fft<F, R, Q, T, +1, R>(v, s, c.t, j, twl);
   if (c.ok) {
 #pragma unroll
     for (int q = 0; q < R / 2; ++q) {
-      const int row = j + R * q;
+      const int row = j + Q * q;
       if (row < nx) P[row * cs + off] = v[q];
     }
   }
 }
 
 // Plan 4 y inverse (see yinv_field_body): all M spectrum rows are inputs,
-// rows y = j + R*q < ny <= M/2 are outputs.
+// rows y = j + Q*q < ny <= M/2 are outputs.
 template <int M>
 __device__ __forceinline__ void yinv_field_reg(const cpx<float>* __restrict__ Qx, const cpx<float>* __restrict__ P,
                                                cpx<float>* __restrict__ Ax, cpx<float>* __restrict__ Ay,
@@ -732,10 +832,10 @@ __device__ __forceinline__ void yinv_field_reg(const cpx<float>* __restrict__ Qx
                                                const cpx<double>* __restrict__ twd, const int bx) {
   using F = float;
   using FT = FieldTile<M, field_threads<M, 4>(), 4>;
-  constexpr int R = FT::R, T = FT::T, NT = field_threads<M, 4>();
+  constexpr int R = FT::R, Q = FT::Q, T = FT::T, NT = field_threads<M, 4>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<F>* s = (cpx<F>*)smem_raw;
-  cpx<F>* twl = s + RegX<R, T>::entries;
+  cpx<F>* twl = s + RegX<R, Q, T>::entries;
   cpx<F>* dl = twl + M;
   if (sc->err) return;
   const int Hz = d.hz, ny = d.ny;
@@ -748,7 +848,7 @@ __device__ __forceinline__ void yinv_field_reg(const cpx<float>* __restrict__ Qx
     q[n] = cpx<F>{0, 0};
     p[n] = cpx<F>{0, 0};
     if (c.ok) {
-      const int64_t o = src + (int64_t)(R * n + j) * Hz;
+      const int64_t o = src + (int64_t)(Q * n + j) * Hz;
       q[n] = Qx[o];
       p[n] = P[o];
     }
@@ -760,17 +860,17 @@ __device__ __forceinline__ void yinv_field_reg(const cpx<float>* __restrict__ Qx
     if (!c.ok) return;
 #pragma unroll
     for (int k = 0; k < R / 2; ++k) {
-      const int y = j + R * k;
+      const int y = j + Q * k;
       if (y < ny) dst[dst0 + (int64_t)y * Hz] = v[k];
     }
   };
-  reg_fft<F, R, T, +1, R>(q, s, c.t, j, twl);
+  reg_fft<F, R, Q, T, +1, R>(q, s, c.t, j, twl);
   store(Ax, q);
 #pragma unroll
-  for (int n = 0; n < R; ++n) q[n] = cmul(p[n], dl[R * n + j]);
-  reg_fft<F, R, T, +1, R>(q, s, c.t, j, twl);
+  for (int n = 0; n < R; ++n) q[n] = cmul(p[n], dl[Q * n + j]);
+  reg_fft<F, R, Q, T, +1, R>(q, s, c.t, j, twl);
   store(Ay, q);
-  reg_fft<F, R, T, +1, R>(p, s, c.t, j, twl);
+  reg_fft<F, R, Q, T, +1, R>(p, s, c.t, j, twl);
   store(Ap, p);
 }
 

@@ -221,11 +221,11 @@ struct ocsc_ctx {
   int deposit_xcd = 0;  // 1: per-XCD grid copies with L2-local atomics
   int deposit_agg = 0;  // 1: runs of equal cells summed across the wave before the LDS table
   int deposit_tile = 0;  // deposit particles per thread and hash slots (DepTile)
-  // mixed-form transform plan (Tile's V): 0 LDS Stockham passes (radix 16 or
-  // 8), 1 radix 8, 2 half-width tiles, 3 field passes with 256 threads
-  // (radix 16), 4 (default) register-resident transforms where the length is
-  // 16, 64 or 256 (10M / 127^3 tracking step: 0.854 -> 0.816 ms, transform
-  // chain 172 -> 128 us), else as 0
+  // transform plan (Tile's V): 0 LDS Stockham passes (radix 16 or 8), 1
+  // radix 8, 2 half-width tiles, 3 field passes with 256 threads (radix 16)
+  // (1-3 mixed form only), 4 (default) register-resident transforms where the
+  // length is 16 to 512, else as 0 (10M / 127^3 tracking step, mixed: 0.854
+  // -> 0.816 ms, transform chain 172 -> 128 us; FP64: 0.983 -> 0.970 ms)
   int fft_plan = 4;
   int red_blocks = kMaxRedBlocks;  // blocks of the strided reduction passes
   int sort_every = 10;  // tracking: re-sort the resident bunch by cell every k steps (0: never)
@@ -446,10 +446,11 @@ int launch_kernel_spectrum(ocsc_ctx* c, Lane& L, const Geometry& g, hipStream_t 
   return OCSC_OK;
 }
 
-// phi = G * rho on the doubled grid; rho read through charge_at<CIN>
-template <typename F, int CIN>
-int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, F* phi, hipStream_t st,
-                       hipEvent_t* ev = nullptr, hipEvent_t join = nullptr) {
+// phi = G * rho on the doubled grid; rho read through charge_at<CIN>.  V:
+// the transform plan (option "fft_plan": 4 or the default passes).
+template <typename F, int CIN, int V>
+int launch_convolution_v(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, F* phi, hipStream_t st,
+                         hipEvent_t* ev, hipEvent_t join) {
   // ev[0] is recorded by the caller before the z pass; ev[1..4] after z, y, x, y.
   // join: the kernel spectrum built on another stream, awaited before the x pass
   const MeshDims& d = g.d;
@@ -459,15 +460,16 @@ int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho,
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<F, MM>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zfwd<F, MM, CIN><<<nb, kNT, TL::lds_bytes, st>>>(rho, A, d, L.sc, tw_for<F>(c, MM),
-                                                        c->deposit_xcd ? kNumXcd : 1);
+    constexpr int VZ = plan_zf_yi<F, MM, V>();
+    k_zfwd<F, MM, CIN, VZ><<<nb, pass_threads<F, MM, VZ>(), pass_lds<F, MM, VZ>(), st>>>(
+        rho, A, d, L.sc, tw_for<F>(c, MM), c->deposit_xcd ? kNumXcd : 1);
   });
   LAUNCHCHK("k_zfwd");
   if (ev) (void)hipEventRecord(ev[1], st);
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<F, MM>;
     const int grid = (d.nx * d.hz + TL::T - 1) / TL::T;
-    k_yfwd<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(A, B, d, L.sc, tw_for<F>(c, MM));
+    k_yfwd<F, MM, V><<<grid, pass_threads<F, MM, V>(), pass_lds<F, MM, V>(), st>>>(A, B, d, L.sc, tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_yfwd");
   if (ev) (void)hipEventRecord(ev[2], st);
@@ -475,24 +477,35 @@ int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho,
   OCSC_SWITCH_M(d.mx, {
     using TL = Tile<F, MM>;
     const int grid = (d.my * d.hz + TL::T - 1) / TL::T;
-    k_xconv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, (const F*)L.G.p, d, L.sc, tw_for<F>(c, MM));
+    k_xconv<F, MM, V><<<grid, pass_threads<F, MM, V>(), pass_lds<F, MM, V>(), st>>>(B, (const F*)L.G.p, d, L.sc,
+                                                                                   tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_xconv");
   if (ev) (void)hipEventRecord(ev[3], st);
   OCSC_SWITCH_M(d.my, {
     using TL = Tile<F, MM>;
     const int grid = (d.nx * d.hz + TL::T - 1) / TL::T;
-    k_yinv<F, MM><<<grid, kNT, TL::lds_bytes, st>>>(B, A, d, L.sc, tw_for<F>(c, MM));
+    constexpr int VY = plan_zf_yi<F, MM, V>();
+    k_yinv<F, MM, VY><<<grid, pass_threads<F, MM, VY>(), pass_lds<F, MM, VY>(), st>>>(B, A, d, L.sc,
+                                                                                     tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_yinv");
   if (ev) (void)hipEventRecord(ev[4], st);
   OCSC_SWITCH_M(d.mz, {
     using TL = Tile<F, MM>;
     const int nb = (int)((R + 2 * TL::T - 1) / (2 * TL::T));
-    k_zinv<F, MM><<<nb, kNT, TL::lds_bytes, st>>>(A, phi, d, L.sc, tw_for<F>(c, MM));
+    k_zinv<F, MM, false, V><<<nb, pass_threads<F, MM, V>(), pass_lds<F, MM, V>(), st>>>(A, phi, d, L.sc,
+                                                                                        tw_for<F>(c, MM));
   });
   LAUNCHCHK("k_zinv");
   return OCSC_OK;
+}
+
+template <typename F, int CIN>
+int launch_convolution(ocsc_ctx* c, Lane& L, const Geometry& g, const void* rho, F* phi, hipStream_t st,
+                       hipEvent_t* ev = nullptr, hipEvent_t join = nullptr) {
+  if (c->fft_plan == 4) return launch_convolution_v<F, CIN, 4>(c, L, g, rho, phi, st, ev, join);
+  return launch_convolution_v<F, CIN, 0>(c, L, g, rho, phi, st, ev, join);
 }
 
 // Mixed form: the three field components on the mesh, E = -grad phi as the
@@ -2378,15 +2391,16 @@ int fused_back(ocsc_ctx* c, const FusedPlan& P, hipStream_t st) {
   if ((rc = launch_deposit_b<NWORD>(c, P, st))) return rc;
   LAUNCHCHK("batched deposit");
   const int ncopy = c->deposit_xcd ? kNumXcd : 1;
-  // forward passes: plan 4 (register-resident square transforms, FP32) when
-  // chosen, else the default plan
+  // forward passes: plan 4 (register-resident transforms) when chosen, else
+  // the default plan
   auto fwd_passes = [&](auto plan) -> int {
     constexpr int V = decltype(plan)::value;
     for (const auto& l : P.zf) {
       OCSC_SWITCH_M(l.M, {
-        kb_zfwd<F, MM, NWORD == 2 ? kChargeFix2 : kChargeFix1, V>
-            <<<(unsigned)l.grid, pass_threads<F, MM, V>(), pass_lds<F, MM, V>(), st>>>(J, l.seg, tw_for<F>(c, MM),
-                                                                                       ncopy);
+        constexpr int VZ = plan_zf_yi<F, MM, V>();
+        kb_zfwd<F, MM, NWORD == 2 ? kChargeFix2 : kChargeFix1, VZ>
+            <<<(unsigned)l.grid, pass_threads<F, MM, VZ>(), pass_lds<F, MM, VZ>(), st>>>(J, l.seg, tw_for<F>(c, MM),
+                                                                                         ncopy);
       });
     }
     for (const auto& l : P.yf) {
@@ -2397,13 +2411,9 @@ int fused_back(ocsc_ctx* c, const FusedPlan& P, hipStream_t st) {
     }
     return OCSC_OK;
   };
-  if constexpr (sizeof(F) == 4) {
-    if ((rc = c->fft_plan == 4 ? fwd_passes(std::integral_constant<int, 4>())
-                               : fwd_passes(std::integral_constant<int, 0>())))
-      return rc;
-  } else if ((rc = fwd_passes(std::integral_constant<int, 0>()))) {
+  if ((rc = c->fft_plan == 4 ? fwd_passes(std::integral_constant<int, 4>())
+                             : fwd_passes(std::integral_constant<int, 0>())))
     return rc;
-  }
   if constexpr (sizeof(F) == 4) {
     // field passes and z inverse: plan 4 (register-resident square
     // transforms) when chosen, else the default plan
@@ -2435,21 +2445,32 @@ int fused_back(ocsc_ctx* c, const FusedPlan& P, hipStream_t st) {
                                : field_passes(std::integral_constant<int, 0>())))
       return rc;
   } else {
-    for (const auto& l : P.xc) {
-      OCSC_SWITCH_M(l.M, {
-        kb_xconv64<MM><<<(unsigned)l.grid, kNT, Tile<double, MM>::lds_bytes, st>>>(J, l.seg, c->twd + MM);
-      });
-    }
-    for (const auto& l : P.yi) {
-      OCSC_SWITCH_M(l.M, {
-        kb_yinv64<MM><<<(unsigned)l.grid, kNT, Tile<double, MM>::lds_bytes, st>>>(J, l.seg, c->twd + MM);
-      });
-    }
-    for (const auto& l : P.zi) {
-      OCSC_SWITCH_M(l.M, {
-        kb_zinv64<MM><<<(unsigned)l.grid, kNT, Tile<double, MM>::lds_bytes, st>>>(J, l.seg, c->twd + MM);
-      });
-    }
+    auto conv_passes = [&](auto plan) -> int {
+      constexpr int V = decltype(plan)::value;
+      for (const auto& l : P.xc) {
+        OCSC_SWITCH_M(l.M, {
+          kb_xconv64<MM, V><<<(unsigned)l.grid, pass_threads<double, MM, V>(), pass_lds<double, MM, V>(), st>>>(
+              J, l.seg, c->twd + MM);
+        });
+      }
+      for (const auto& l : P.yi) {
+        OCSC_SWITCH_M(l.M, {
+          constexpr int VY = plan_zf_yi<double, MM, V>();
+          kb_yinv64<MM, VY><<<(unsigned)l.grid, pass_threads<double, MM, VY>(), pass_lds<double, MM, VY>(), st>>>(
+              J, l.seg, c->twd + MM);
+        });
+      }
+      for (const auto& l : P.zi) {
+        OCSC_SWITCH_M(l.M, {
+          kb_zinv64<MM, V><<<(unsigned)l.grid, pass_threads<double, MM, V>(), pass_lds<double, MM, V>(), st>>>(
+              J, l.seg, c->twd + MM);
+        });
+      }
+      return OCSC_OK;
+    };
+    if ((rc = c->fft_plan == 4 ? conv_passes(std::integral_constant<int, 4>())
+                               : conv_passes(std::integral_constant<int, 0>())))
+      return rc;
     kb_diff64<<<(unsigned)P.d64.grid, 256, 0, st>>>(J, P.d64.seg);
   }
   LAUNCHCHK("batched transforms");

@@ -595,6 +595,34 @@ def test_fft_plans_agree_with_fp64(plan, nmesh):
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("nmesh", [(7, 31, 127), (15, 63, 255), (255, 15, 7)])
+def test_fp64_register_plan_agrees_with_lds_plan(nmesh):
+    """The FP64 form's register-resident transforms (fft_plan 4, every
+    length 16 to 512 on some axis of these meshes) and its LDS Stockham
+    passes (plan 0) differ in FP64 round-off only: rows to 1e-13 of their
+    largest value, kicks to 1e-10 of their RMS."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, _, dz = make_bunch("flattop_10m_127", n=150_000, seed=31)
+    dev = torch.cuda.current_device()
+    old = _lib.get_option(dev, "fft_plan")
+    outs = {}
+    try:
+        for plan in (0, 4):
+            _lib.set_option(dev, "fft_plan", plan)
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            SpaceCharge(nmesh_xyz=list(nmesh)).apply(d, dz)
+            outs[plan] = d.rparticles.cpu().numpy()
+    finally:
+        _lib.set_option(dev, "fft_plan", old)
+    a, b = outs[0], outs[4]
+    rows = max(float(np.max(np.abs(a[i] - b[i])) / np.max(np.abs(a[i]))) for i in range(6))
+    kick = max(rel_rms(b[i] - r[i], a[i] - r[i]) for i in (1, 3, 5))
+    print(f"mesh {nmesh}: plan 4 vs 0 rows {rows:.2e} kick {kick:.2e}")
+    assert rows <= 1e-13
+    assert kick <= 1e-10
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_sorted_one_shot_apply_is_bit_identical(prec):
     """A one-shot kick through a cell-ordered copy (frame and extent from the
