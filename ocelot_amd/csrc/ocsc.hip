@@ -238,6 +238,10 @@ struct ocsc_ctx {
   // charge grid cleared by a kernel on the side stream (1), by pass B (0), or
   // in tracking by the previous step's kick (2; the first step as 1)
   int grid_clear = 1;
+  // workgroups of the Green's-function node and difference kernels (grid-
+  // stride loops; 0: one thread per node or cell): fewer leave more of the
+  // GPU to the deposit they overlap
+  int igf_blocks = 0;
   // one-shot kicks (ocsc_apply) replayed from a captured graph (option
   // "apply_graph"): the launch sequence of the last call, the set-up node
   // whose inputs change from call to call, and what it was captured for
@@ -415,9 +419,10 @@ int launch_kernel_spectrum(ocsc_ctx* c, Lane& L, const Geometry& g, hipStream_t 
   {
     // F once per node, then the 8-corner differences
     const int64_t nn = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
-    k_igf_nodes<<<(int)std::min<int64_t>((nn + 255) / 256, 65536), 256, 0, st>>>((double*)L.KF.p, d, L.sc, force);
-    k_igf_diff<<<(int)std::min<int64_t>((g.ncell + 255) / 256, 65536), 256, 0, st>>>((const double*)L.KF.p, K, d,
-                                                                                     L.sc, force);
+    const int64_t cap = c->igf_blocks > 0 ? c->igf_blocks : 65536;
+    k_igf_nodes<<<(int)std::min<int64_t>((nn + 255) / 256, cap), 256, 0, st>>>((double*)L.KF.p, d, L.sc, force);
+    k_igf_diff<<<(int)std::min<int64_t>((g.ncell + 255) / 256, cap), 256, 0, st>>>((const double*)L.KF.p, K, d,
+                                                                                   L.sc, force);
   }
   LAUNCHCHK("k_igf");
   const int64_t R = (int64_t)d.nx * d.ny;
@@ -946,7 +951,7 @@ int apply_via_graph(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t 
   const std::vector<int64_t> key = {(int64_t)(intptr_t)r, ld, (int64_t)(intptr_t)q, in.n, g.d.nx, g.d.ny, g.d.nz,
                                     prec, o.sort != nullptr, c->overlap, c->deposit_xcd, c->deposit_agg,
                                     c->deposit_tile, c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear,
-                                    (int64_t)g_alloc_gen.load()};
+                                    c->igf_blocks, (int64_t)g_alloc_gen.load()};
   if (!A.exec || A.key != key) {
     drop_apply_graph(c);
     HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
@@ -1964,7 +1969,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     };
     tkey = {(int64_t)(intptr_t)r, ld, (int64_t)(intptr_t)q, n, g.d.nx, g.d.ny, g.d.nz, prec, sorted, via_rec,
             bits(E_GeV), bits(zstep), bits(Ld), ktime, c->overlap, c->deposit_xcd, c->deposit_agg, c->deposit_tile,
-            c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear, (int64_t)g_alloc_gen.load()};
+            c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear, c->igf_blocks, (int64_t)g_alloc_gen.load()};
     if (c->tg.key == tkey) {
       for (int k = 0; k < 2; ++k) {
         exec[k] = c->tg.exec[k];
@@ -2944,6 +2949,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->grid_clear = (int)value;
     return OCSC_OK;
   }
+  if (n == "igf_blocks") {
+    if (value < 0 || value > 65536) return fail(OCSC_EINVAL, "igf_blocks must be 0..65536");
+    c->igf_blocks = (int)value;
+    return OCSC_OK;
+  }
   if (n == "batch_fused") {
     if (value != 0 && value != 1) return fail(OCSC_EINVAL, "batch_fused must be 0 or 1");
     c->batch_fused = (int)value;
@@ -3029,6 +3039,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "grid_clear") {
     *value = c->grid_clear;
+    return OCSC_OK;
+  }
+  if (n == "igf_blocks") {
+    *value = c->igf_blocks;
     return OCSC_OK;
   }
   if (n == "deposit_tile") {
