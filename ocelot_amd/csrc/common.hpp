@@ -94,6 +94,9 @@ struct KickScalars {
   // 4 = cell index out of range, 8 = non-positive/non-finite cell size
   int32_t err;
   int32_t err_sticky;  // errors of earlier kicks not yet reported
+  // workgroups of the fused fold (k_fold_frame) that have finished; the last
+  // one resets it
+  uint32_t fold_count;
 };
 
 enum ErrBits : int32_t { kErrNonFinite = 1, kErrZeroCharge = 2, kErrCell = 4, kErrSteps = 8 };

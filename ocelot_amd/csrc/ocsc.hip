@@ -133,6 +133,40 @@ __global__ void k_setup(KickScalars* __restrict__ s, KickInputs in) {
   if (threadIdx.x == 0) setup_one(s, in);
 }
 
+// Tracking steps: the previous kick's per-workgroup momentum sums folded by
+// gridDim.x workgroups (k_fold), and the last workgroup to finish runs this
+// kick's set-up and the frame reduction (k_setup, k_reduceA): one launch
+// instead of three.  The same sums bit for bit: each workgroup folds the same
+// chunk, and reducing the gridDim.x (<= 256) folded values with 256 threads
+// takes the tree that 1024 threads would (the extra lanes add only zeros).
+__global__ void __launch_bounds__(256) k_fold_frame(const double* __restrict__ part, int nblk,
+                                                    double* __restrict__ folded, KickScalars* __restrict__ sc,
+                                                    KickInputs in) {
+  __shared__ double lds[16 * 3];
+  __shared__ int last;
+  const int nf = gridDim.x, chunk = (nblk + nf - 1) / nf;
+  const int b0 = blockIdx.x * chunk, b1 = b0 + chunk < nblk ? b0 + chunk : nblk;
+  double v[3] = {0, 0, 0};
+  for (int b = b0 + threadIdx.x; b < b1; b += 256)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] += part[c * nblk + b];
+  block_reduce<3, 0>(v, lds);
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < 3; ++c) folded[c * nf + blockIdx.x] = v[c];
+    __threadfence();  // the folded values before the count
+    last = atomicAdd(&sc->fold_count, 1u) == (unsigned)nf - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // every workgroup's folded values visible here
+  if (threadIdx.x == 0) {
+    sc->fold_count = 0;
+    setup_one(sc, in);
+  }
+  __syncthreads();
+  reduceA_body(sc, folded, nf, 0, 1);
+}
+
 // batched calls: the set-up of every job (thread j: job j)
 __global__ void kb_setup(const KickInputs* __restrict__ in, const BJob* __restrict__ J, int nb) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -242,6 +276,8 @@ struct ocsc_ctx {
   // stride loops; 0: one thread per node or cell): fewer leave more of the
   // GPU to the deposit they overlap
   int igf_blocks = 0;
+  // tracking: set-up, fold and frame reduction in one launch (1) or three (0)
+  int fold_frame = 1;
   // one-shot kicks (ocsc_apply) replayed from a captured graph (option
   // "apply_graph"): the launch sequence of the last call, the set-up node
   // whose inputs change from call to call, and what it was captured for
@@ -752,9 +788,15 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
     HIPCHK(hipEventRecord(c->ev_zero, c->side));
   }
   mark(0);
-  k_setup<<<1, 64, 0, st>>>(L.sc, in);
-  LAUNCHCHK("k_setup");
-  if (o.passA) {
+  const bool fold_frame = !o.passA && o.prev_nblk > kMaxRedBlocks && c->fold_frame;
+  if (!fold_frame) {
+    k_setup<<<1, 64, 0, st>>>(L.sc, in);
+    LAUNCHCHK("k_setup");
+  }
+  if (fold_frame) {
+    double* folded = (double*)L.part.p + (size_t)3 * o.prev_nblk;
+    k_fold_frame<<<kFoldBlocks, 256, 0, st>>>((const double*)L.part.p, o.prev_nblk, folded, L.sc, in);
+  } else if (o.passA) {
     k_passA<<<nblk, kPT, 0, st>>>(R, L.sc, (double*)L.part.p);
     k_reduceA<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk);
   } else if (o.prev_nblk > kMaxRedBlocks) {
@@ -1969,7 +2011,8 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     };
     tkey = {(int64_t)(intptr_t)r, ld, (int64_t)(intptr_t)q, n, g.d.nx, g.d.ny, g.d.nz, prec, sorted, via_rec,
             bits(E_GeV), bits(zstep), bits(Ld), ktime, c->overlap, c->deposit_xcd, c->deposit_agg, c->deposit_tile,
-            c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear, c->igf_blocks, (int64_t)g_alloc_gen.load()};
+            c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear, c->igf_blocks, c->fold_frame,
+            (int64_t)g_alloc_gen.load()};
     if (c->tg.key == tkey) {
       for (int k = 0; k < 2; ++k) {
         exec[k] = c->tg.exec[k];
@@ -2954,6 +2997,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->igf_blocks = (int)value;
     return OCSC_OK;
   }
+  if (n == "fold_frame") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "fold_frame must be 0 or 1");
+    c->fold_frame = (int)value;
+    return OCSC_OK;
+  }
   if (n == "batch_fused") {
     if (value != 0 && value != 1) return fail(OCSC_EINVAL, "batch_fused must be 0 or 1");
     c->batch_fused = (int)value;
@@ -3043,6 +3091,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "igf_blocks") {
     *value = c->igf_blocks;
+    return OCSC_OK;
+  }
+  if (n == "fold_frame") {
+    *value = c->fold_frame;
     return OCSC_OK;
   }
   if (n == "deposit_tile") {

@@ -531,12 +531,13 @@ def test_per_xcd_deposit_is_bit_identical(prec):
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("opt,vals", [("deposit_agg", (0, 1)), ("permute_rec", (0, 1)), ("resort_near", (0, 1)),
                                       ("deposit_tile", (0, 1, 2, 3, 4)), ("grid_clear", (0, 1, 2)),
-                                      ("igf_blocks", (0, 64, 512))])
+                                      ("igf_blocks", (0, 64, 512)), ("fold_frame", (0, 1))])
 def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
     """The deposit's wave pre-sum of equal cells and its tile sizes (integer
     sums in another grouping), the three ways of clearing the charge grid
     (each with one grid and with per-XCD copies where it matters), the
-    Green's-function kernels on fewer workgroups (same values), and the
+    Green's-function kernels on fewer workgroups (same values), the frame
+    reduction of tracking steps in one launch or three (same tree), and the
     record-staged permutations and one-pass re-sorts of the tracking copy
     (same values moved another way) give the same bunch bit for bit: one
     caller-order kick, then 12 tracking steps with a re-sort every 5."""
