@@ -92,21 +92,30 @@ __global__ void __launch_bounds__(kRsPT) k_rs_hist(const uint32_t* __restrict__ 
 }
 
 // exclusive block scan of one value per thread (kRsPT threads); returns the
-// prefix, *total receives the sum
+// prefix, *total receives the sum.  Each wavefront scans its 64 values with
+// shuffles, then every thread adds the totals of the wavefronts before its
+// own (two barriers instead of two per doubling step; integer sums, so the
+// result does not depend on the order)
 __device__ __forceinline__ uint32_t rs_block_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
-  const int t = threadIdx.x;
-  lds[t] = v;
-  __syncthreads();
-  for (int off = 1; off < kRsPT; off <<= 1) {
-    const uint32_t a = t >= off ? lds[t - off] : 0u;
-    __syncthreads();
-    lds[t] += a;
-    __syncthreads();
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
   }
-  const uint32_t incl = lds[t];
-  *total = lds[kRsPT - 1];
+  if (lane == 63) lds[w] = x;
   __syncthreads();
-  return incl - v;
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kRsWaves; ++i) {
+    const uint32_t sw = lds[i];
+    if (i < w) pre += sw;
+    tot += sw;
+  }
+  *total = tot;
+  __syncthreads();  // lds may be written again after the return
+  return pre + x - v;
 }
 
 // grid kRsDigits: exclusive scan of H[digit][0..ntiles) in place;
