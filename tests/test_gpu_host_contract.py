@@ -75,3 +75,29 @@ def test_two_threads_on_one_device_equal_sequential_kicks():
     assert not errors, errors
     for d, rr in zip(ds, ref):
         assert torch.equal(d.rparticles.cpu(), rr)
+
+
+@pytest.mark.parametrize("name,good,bad", [
+    ("fft_plan", (0, 1, 2, 3, 4), (-1, 5)),
+    ("deposit_tile", (0, 1, 2, 3, 4), (-1, 5)),
+    ("grid_clear", (0, 1, 2), (-1, 3)),
+    ("igf_blocks", (0, 1, 65536), (-1, 65537)),
+    ("fold_frame", (0, 1), (-1, 2)),
+])
+def test_options_round_trip_and_reject_out_of_range(name, good, bad):
+    """Every value an option documents is kept and read back; values outside
+    its range raise ValueError and leave the option as it was."""
+    from ocelot_amd import _lib
+    dev = torch.cuda.current_device()
+    old = _lib.get_option(dev, name)
+    try:
+        for v in good:
+            _lib.set_option(dev, name, v)
+            assert _lib.get_option(dev, name) == v
+        _lib.set_option(dev, name, good[0])
+        for v in bad:
+            with pytest.raises(ValueError):
+                _lib.set_option(dev, name, v)
+            assert _lib.get_option(dev, name) == good[0]
+    finally:
+        _lib.set_option(dev, name, old)
