@@ -457,8 +457,8 @@ int launch_kernel_spectrum(ocsc_ctx* c, Lane& L, const Geometry& g, hipStream_t 
     const int64_t nn = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
     const int64_t cap = c->igf_blocks > 0 ? c->igf_blocks : 65536;
     k_igf_nodes<<<(int)std::min<int64_t>((nn + 255) / 256, cap), 256, 0, st>>>((double*)L.KF.p, d, L.sc, force);
-    k_igf_diff<<<(int)std::min<int64_t>((g.ncell + 255) / 256, cap), 256, 0, st>>>((const double*)L.KF.p, K, d,
-                                                                                   L.sc, force);
+    k_igf_diff<<<(int)std::min<int64_t>((igf_diff_items(d) + 255) / 256, cap), 256, 0, st>>>(
+        (const double*)L.KF.p, K, d, L.sc, force);
   }
   LAUNCHCHK("k_igf");
   const int64_t R = (int64_t)d.nx * d.ny;
@@ -1449,9 +1449,9 @@ int ocsc_sym_kernel(ocsc_ctx* c, const int32_t nmesh[3], const double h[3], doub
   int rc;
   const int64_t nn = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
   if ((rc = ensure(L.KF, (size_t)nn * 8))) return rc;
-  const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
   k_igf_nodes<<<(int)std::min<int64_t>((nn + 255) / 256, 65536), 256, 0, st>>>((double*)L.KF.p, d, L.sc, 1);
-  k_igf_diff<<<(int)std::min<int64_t>((nc + 255) / 256, 65536), 256, 0, st>>>((const double*)L.KF.p, K, d, L.sc, 1);
+  k_igf_diff<<<(int)std::min<int64_t>((igf_diff_items(d) + 255) / 256, 65536), 256, 0, st>>>((const double*)L.KF.p,
+                                                                                              K, d, L.sc, 1);
   LAUNCHCHK("k_igf");
   HIPCHK(hipStreamSynchronize(st));
   return OCSC_OK;
@@ -2314,7 +2314,7 @@ int fused_plan(ocsc_ctx* c, const std::vector<int>& kicked, const std::vector<do
     const int64_t nodes = (int64_t)(b.d.nx + 1) * (b.d.ny + 1) * (b.d.nz + 1);
     return std::min<int64_t>((nodes + 255) / 256, 65536);
   }, &P->nodes);
-  all([&](const BJob& b) { return std::min<int64_t>((b.ncell + 255) / 256, 65536); }, &P->diff);
+  all([&](const BJob& b) { return std::min<int64_t>((igf_diff_items(b.d) + 255) / 256, 65536); }, &P->diff);
   all([&](const BJob& b) { return (b.ncell + 255) / 256; }, &P->d64);
   auto mz = [](const MeshDims& d) { return d.mz; };
   auto my = [](const MeshDims& d) { return d.my; };
