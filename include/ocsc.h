@@ -365,7 +365,12 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * "grid_clear": the charge grid is cleared by a kernel of its own on the
  * second stream (1, default; the deposit waits on that stream), by the
  * frame/extent pass B (0), or in tracking by each step's kick for the next
- * step (2; the first step of a call as 1); bit-identical results. */
+ * step (2; the first step of a call as 1); bit-identical results.
+ * "igf_blocks": workgroups of the Green's-function node and difference
+ * kernels, 0 (default) as many as their work items, 1..65536 a cap (their
+ * loops stride over the rest); bit-identical results.  "fold_frame":
+ * tracking steps fold the previous kick's momentum sums, set up and reduce
+ * the frame in one launch (1, default) or three (0); bit-identical results. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 
