@@ -115,6 +115,10 @@ __device__ __forceinline__ void setup_one(KickScalars* __restrict__ s, const Kic
   s->n = in.n;
   s->err_sticky |= s->err;
   s->err = 0;
+  // k_fold_frame's count: zero after every set-up (its last workgroup runs
+  // this one; the first step of a call runs it in k_setup), so a count left
+  // by an interrupted launch does not reach into later calls
+  s->fold_count = 0;
   if (in.mode == 1) {
     s->gamma0 = in.gamma0;
     for (int k = 0; k < 9; ++k) s->T[k] = (k % 4 == 0) ? 1.0 : 0.0;
@@ -159,10 +163,7 @@ __global__ void __launch_bounds__(256) k_fold_frame(const double* __restrict__ p
   __syncthreads();
   if (!last) return;
   __threadfence();  // every workgroup's folded values visible here
-  if (threadIdx.x == 0) {
-    sc->fold_count = 0;
-    setup_one(sc, in);
-  }
+  if (threadIdx.x == 0) setup_one(sc, in);  // also resets the count
   __syncthreads();
   reduceA_body(sc, folded, nf, 0, 1);
 }
