@@ -370,7 +370,11 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * kernels, 0 (default) as many as their work items, 1..65536 a cap (their
  * loops stride over the rest); bit-identical results.  "fold_frame":
  * tracking steps fold the previous kick's momentum sums, set up and reduce
- * the frame in one launch (1, default) or three (0); bit-identical results. */
+ * the frame in one launch (1, default) or three (0); bit-identical results.
+ * "q_uniform": a tracking call checks its charges once; when every charge
+ * equals the first bit for bit, the steps' extent and deposit passes take
+ * that value instead of loading a charge per particle (1, default), or they
+ * always load them (0); bit-identical results. */
 int ocsc_set_option(ocsc_ctx* ctx, const char* name, int64_t value);
 int ocsc_get_option(ocsc_ctx* ctx, const char* name, int64_t* value);
 

@@ -97,6 +97,11 @@ struct KickScalars {
   // workgroups of the fused fold (k_fold_frame) that have finished; the last
   // one resets it
   uint32_t fold_count;
+  // tracking: every charge of the bunch equals q_val bit for bit (checked at
+  // the start of the call); the extent and deposit passes then take q_val
+  // instead of loading the charge.  Zero after every other set-up.
+  int32_t q_const;
+  double q_val;
 };
 
 enum ErrBits : int32_t { kErrNonFinite = 1, kErrZeroCharge = 2, kErrCell = 4, kErrSteps = 8 };

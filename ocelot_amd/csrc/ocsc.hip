@@ -88,6 +88,9 @@ struct KickInputs {
   const double* draws;
   int* draw_ctr;
   int32_t ndraws;
+  // keep the equal-charge flag of a tracking call (KickScalars::q_const)
+  // instead of clearing it
+  int32_t q_keep;
 };
 
 __device__ __forceinline__ void setup_one(KickScalars* __restrict__ s, const KickInputs& in) {
@@ -119,6 +122,7 @@ __device__ __forceinline__ void setup_one(KickScalars* __restrict__ s, const Kic
   // this one; the first step of a call runs it in k_setup), so a count left
   // by an interrupted launch does not reach into later calls
   s->fold_count = 0;
+  if (!in.q_keep) s->q_const = 0;
   if (in.mode == 1) {
     s->gamma0 = in.gamma0;
     for (int k = 0; k < 9; ++k) s->T[k] = (k % 4 == 0) ? 1.0 : 0.0;
@@ -166,6 +170,25 @@ __global__ void __launch_bounds__(256) k_fold_frame(const double* __restrict__ p
   if (threadIdx.x == 0) setup_one(sc, in);  // also resets the count
   __syncthreads();
   reduceA_body(sc, folded, nf, 0, 1);
+}
+
+// Tracking calls: whether every charge equals the first one bit for bit
+// (KickScalars::q_const and q_val, kept by the steps' set-ups); the check
+// follows the init on the same stream.
+__global__ void k_q_const_init(const double* __restrict__ q, int64_t n, KickScalars* __restrict__ s) {
+  if (threadIdx.x == 0) {
+    s->q_val = n > 0 ? q[0] : 0.0;
+    s->q_const = n > 0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_q_const_check(const double* __restrict__ q, int64_t n,
+                                                       KickScalars* __restrict__ s) {
+  const long long q0 = __double_as_longlong(s->q_val);
+  bool diff = false;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256)
+    diff |= __double_as_longlong(q[p]) != q0;
+  if (diff) s->q_const = 0;  // every writer stores the same value
 }
 
 // batched calls: the set-up of every job (thread j: job j)
@@ -279,6 +302,10 @@ struct ocsc_ctx {
   int igf_blocks = 0;
   // tracking: set-up, fold and frame reduction in one launch (1) or three (0)
   int fold_frame = 1;
+  // tracking: charges checked at the start of a call; when all are equal, the
+  // extent and deposit passes take the charge from the kick scalars (1)
+  // instead of loading it per particle (0)
+  int q_uniform = 1;
   // one-shot kicks (ocsc_apply) replayed from a captured graph (option
   // "apply_graph"): the launch sequence of the last call, the set-up node
   // whose inputs change from call to call, and what it was captured for
@@ -1925,6 +1952,14 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     lr = n;
     qq = (const double*)c->qs[0].p;
   }
+  // equal charges (Ocelot's generated bunches): the steps' extent and deposit
+  // passes take the charge from the kick scalars, the same value bit for bit
+  if (c->q_uniform && nsteps > 0 && zstep != 0) {
+    k_q_const_init<<<1, 64, 0, st>>>(q, n, L.sc);
+    k_q_const_check<<<(int)std::min<int64_t>(nb, 4096), 256, 0, st>>>(q, n, L.sc);
+    LAUNCHCHK("charge check");
+    in.q_keep = 1;
+  }
   // Step loop.  The drift of step s+1 and the momentum sums of kick s+1 are
   // fused into kick s, so the sequence is drift_0, then kicks 0..n-1 with
   // kick s < n-1 drifting (first kick: own pass A; later kicks: partials).
@@ -2012,7 +2047,7 @@ int track_impl(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_t n, c
     };
     tkey = {(int64_t)(intptr_t)r, ld, (int64_t)(intptr_t)q, n, g.d.nx, g.d.ny, g.d.nz, prec, sorted, via_rec,
             bits(E_GeV), bits(zstep), bits(Ld), ktime, c->overlap, c->deposit_xcd, c->deposit_agg, c->deposit_tile,
-            c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear, c->igf_blocks, c->fold_frame,
+            c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear, c->igf_blocks, c->fold_frame, c->q_uniform,
             (int64_t)g_alloc_gen.load()};
     if (c->tg.key == tkey) {
       for (int k = 0; k < 2; ++k) {
@@ -3003,6 +3038,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->fold_frame = (int)value;
     return OCSC_OK;
   }
+  if (n == "q_uniform") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "q_uniform must be 0 or 1");
+    c->q_uniform = (int)value;
+    return OCSC_OK;
+  }
   if (n == "batch_fused") {
     if (value != 0 && value != 1) return fail(OCSC_EINVAL, "batch_fused must be 0 or 1");
     c->batch_fused = (int)value;
@@ -3096,6 +3136,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "fold_frame") {
     *value = c->fold_frame;
+    return OCSC_OK;
+  }
+  if (n == "q_uniform") {
+    *value = c->q_uniform;
     return OCSC_OK;
   }
   if (n == "deposit_tile") {

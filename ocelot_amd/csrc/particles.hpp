@@ -330,11 +330,11 @@ __device__ __forceinline__ void passB_body(Rows R, const double* __restrict__ q,
     double a[6], b[6], qa, qb = 0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) a[k] = ldnt(R.r + k * R.ld + p);
-    qa = ldnt(q + p);
+    qa = s.q_const ? s.q_val : ldnt(q + p);
     if (two) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) b[k] = ldnt(R.r + k * R.ld + p2);
-      qb = ldnt(q + p2);
+      qb = s.q_const ? s.q_val : ldnt(q + p2);
     }
     body(a, qa);
     if (two) body(b, qb);
@@ -559,7 +559,7 @@ __device__ __forceinline__ void deposit_body(Rows R, const double* __restrict__ 
   auto load = [&](int64_t p, double* v) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) v[k] = ldnt(R.r + k * R.ld + p);
-    v[6] = ldnt(q + p);
+    v[6] = s.q_const ? s.q_val : ldnt(q + p);
   };
   // one particle (valid: p < n) through the table; cur holds its rows
   auto item = [&](int64_t p, bool valid, const double* cur) {

@@ -83,6 +83,7 @@ def test_two_threads_on_one_device_equal_sequential_kicks():
     ("grid_clear", (0, 1, 2), (-1, 3)),
     ("igf_blocks", (0, 1, 65536), (-1, 65537)),
     ("fold_frame", (0, 1), (-1, 2)),
+    ("q_uniform", (0, 1), (-1, 2)),
 ])
 def test_options_round_trip_and_reject_out_of_range(name, good, bad):
     """Every value an option documents is kept and read back; values outside

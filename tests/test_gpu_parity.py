@@ -531,13 +531,15 @@ def test_per_xcd_deposit_is_bit_identical(prec):
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("opt,vals", [("deposit_agg", (0, 1)), ("permute_rec", (0, 1)), ("resort_near", (0, 1)),
                                       ("deposit_tile", (0, 1, 2, 3, 4)), ("grid_clear", (0, 1, 2)),
-                                      ("igf_blocks", (0, 64, 512)), ("fold_frame", (0, 1))])
+                                      ("igf_blocks", (0, 64, 512)), ("fold_frame", (0, 1)),
+                                      ("q_uniform", (0, 1))])
 def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
     """The deposit's wave pre-sum of equal cells and its tile sizes (integer
     sums in another grouping), the three ways of clearing the charge grid
     (each with one grid and with per-XCD copies where it matters), the
     Green's-function kernels on fewer workgroups (same values), the frame
-    reduction of tracking steps in one launch or three (same tree), and the
+    reduction of tracking steps in one launch or three (same tree), equal
+    charges taken from the kick scalars instead of loaded (same value), and the
     record-staged permutations and one-pass re-sorts of the tracking copy
     (same values moved another way) give the same bunch bit for bit: one
     caller-order kick, then 12 tracking steps with a re-sort every 5."""
@@ -564,6 +566,41 @@ def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
     ref = outs[(0, vals[0])]
     for key, o in outs.items():
         assert np.array_equal(o, ref), key
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("odd", [0, -1, None])
+def test_equal_charge_check_sees_every_charge(prec, odd):
+    """A tracking call takes the charge from the kick scalars only when every
+    charge equals the first: with one charge doubled (the first, the last) or
+    none, tracking with the check on gives the bunch of tracking with the
+    check off bit for bit; the doubled charge changes the result."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=300_000, seed=23)
+    q = q.copy()
+    if odd is not None:
+        q[odd] *= 2.0
+    dev = torch.cuda.current_device()
+    saved = {k: _lib.get_option(dev, k) for k in ("q_uniform", "sort_every")}
+    outs = []
+    try:
+        _lib.set_option(dev, "sort_every", 4)
+        runs = [(0, q), (1, q)]
+        if odd is not None:
+            runs.append((1, np.full_like(q, q[1 if odd == 0 else 0])))
+        for v, qv in runs:
+            _lib.set_option(dev, "q_uniform", v)
+            d = DeviceParticleArray.from_host(parray(r, qv, E), dev)
+            sc = SpaceCharge(nmesh_xyz=[31, 31, 47], precision=prec)
+            track_drift(d, sc, dz, dz, 9, use_graph=True)
+            outs.append(d.rparticles.cpu().numpy())
+    finally:
+        for k, v in saved.items():
+            _lib.set_option(dev, k, v)
+    assert np.array_equal(outs[0], outs[1])
+    if odd is not None:
+        assert not np.array_equal(outs[2], outs[1])
 
 
 @pytest.mark.parametrize("plan", [0, 1, 2, 3, 4])
