@@ -333,7 +333,12 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * (ocsc_apply) deposit and kick a cell-ordered copy of the bunch and write
  * the result back in the caller's order: 0 never, 1 always, 2 (default) in
  * the FP64 form from 1000000 particles on; bit-identical results
- * (ocsc_apply_cells and ocsc_apply_noalloc never sort).  "deposit_tile": particles per thread and
+ * (ocsc_apply_cells and ocsc_apply_noalloc never sort).  "apply_resort":
+ * such a kick sorts on the first of every apply_resort calls (1..1000,
+ * default 10) on the same rows (pointer, particle count and row stride) and
+ * the calls in between re-use that call's cell order (any order gives the
+ * same kick; a recent one keeps the locality): 1 sorts on every call;
+ * bit-identical results.  "deposit_tile": particles per thread and
  * LDS hash slots of the deposit, 0 (default) 8 and 2048 (FP64: 1024), 1 half
  * of both, 2 twice both, 3 no table (one global atomic per particle), 4 as 0
  * with all of a thread's particle loads issued first; bit-identical results.  "batch_sort": see

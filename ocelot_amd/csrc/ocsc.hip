@@ -317,7 +317,16 @@ struct ocsc_ctx {
     KickInputs in{};
     std::vector<int64_t> key;
     hipEvent_t done = nullptr;  // after the last replay: node updates wait for it
-  } ag;
+  } ag[2];  // launch sequences with a fresh cell sort (0) and with a re-used cell order (1)
+  // one-shot kicks on a cell-ordered copy: the cell order of the last sorting
+  // call, re-used by the next apply_resort - 1 calls on the same rows (any
+  // permutation gives the same kick bit for bit; a recent one keeps the
+  // deposit's and the kick's locality), and the rows it was sorted for
+  int apply_resort = 10;
+  Buf apply_order;
+  const double* ao_r = nullptr;
+  int64_t ao_n = 0, ao_ld = 0;
+  int ao_age = 0;  // calls since the sort, that one included; 0: no order kept
   // cell-ordered copy of bunch i of a batched call: six rows, charges, permutation
   struct BunchCopy {
     Buf rs, qs, perm;
@@ -739,6 +748,9 @@ struct SortBufs {
   double* qs;
   uint32_t* order_out;
   bool scatter_back;
+  // a cell order to re-use instead of sorting (copy position -> caller
+  // position, a permutation of 0..n-1), or null
+  const uint32_t* order_in = nullptr;
 };
 
 template <int NWORD, bool XCD, bool AGG, int TILE>
@@ -858,10 +870,16 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   const SortBufs* sb = o.sort;
   if (sb) {
     const int nb = (int)((n + 255) / 256);
-    const int end_bit = sort_end_bit(g.d.nx, g.d.ny, g.d.nz, c->sort_cells);
-    k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(R, q, L.sc, sb->k[0], sb->rec, c->sort_cells,
-                                                            DriftArgs{0, 0});
-    order = sb->v[rs_sort(sb->k, sb->v, n, end_bit, sb->hist, st)];
+    if (sb->order_in) {
+      // an earlier call's cell order: records in the caller's order, no sort
+      k_rows_to_rec<<<nb, 256, 0, st>>>(R, q, sb->rec, n);
+      order = sb->order_in;
+    } else {
+      const int end_bit = sort_end_bit(g.d.nx, g.d.ny, g.d.nz, c->sort_cells);
+      k_sort_keys<<<(int)((n + kPT - 1) / kPT), kPT, 0, st>>>(R, q, L.sc, sb->k[0], sb->rec, c->sort_cells,
+                                                              DriftArgs{0, 0});
+      order = sb->v[rs_sort(sb->k, sb->v, n, end_bit, sb->hist, st)];
+    }
     k_gather_rec<false><<<nb, 256, 0, st>>>(sb->rec, order, sb->rs, n, sb->qs, n, DriftArgs{0, 0});
     if (sb->order_out) HIPCHK(hipMemcpyAsync(sb->order_out, order, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
     LAUNCHCHK("cell-order copy");
@@ -1000,12 +1018,16 @@ void drop_track_graphs(ocsc_ctx* c) {
   c->tg = ocsc_ctx::TrackGraphs{};
 }
 
+void drop_apply_slot(ocsc_ctx::ApplyGraph& A) {
+  if (A.done) (void)hipEventSynchronize(A.done);
+  if (A.exec) (void)hipGraphExecDestroy(A.exec);
+  if (A.graph) (void)hipGraphDestroy(A.graph);
+  if (A.done) (void)hipEventDestroy(A.done);
+  A = ocsc_ctx::ApplyGraph{};
+}
+
 void drop_apply_graph(ocsc_ctx* c) {
-  if (c->ag.done) (void)hipEventSynchronize(c->ag.done);
-  if (c->ag.exec) (void)hipGraphExecDestroy(c->ag.exec);
-  if (c->ag.graph) (void)hipGraphDestroy(c->ag.graph);
-  if (c->ag.done) (void)hipEventDestroy(c->ag.done);
-  c->ag = ocsc_ctx::ApplyGraph{};
+  for (auto& A : c->ag) drop_apply_slot(A);
 }
 
 // One-shot kick from a captured graph.  A kick is some 30 launches, many of
@@ -1017,13 +1039,15 @@ void drop_apply_graph(ocsc_ctx* c) {
 // replaced when they differ.
 int apply_via_graph(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld, const double* q,
                     const KickInputs& in, int prec, hipStream_t st, const StepOpts& o) {
-  auto& A = c->ag;
+  const bool reuse = o.sort && o.sort->order_in;
+  auto& A = c->ag[reuse ? 1 : 0];
   const std::vector<int64_t> key = {(int64_t)(intptr_t)r, ld, (int64_t)(intptr_t)q, in.n, g.d.nx, g.d.ny, g.d.nz,
-                                    prec, o.sort != nullptr, c->overlap, c->deposit_xcd, c->deposit_agg,
+                                    prec, o.sort != nullptr, reuse, o.sort && o.sort->order_out, c->overlap,
+                                    c->deposit_xcd, c->deposit_agg,
                                     c->deposit_tile, c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear,
                                     c->igf_blocks, (int64_t)g_alloc_gen.load()};
   if (!A.exec || A.key != key) {
-    drop_apply_graph(c);
+    drop_apply_slot(A);  // the other slot's sequence stays cached
     HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_kick_prec(c, L, g, r, ld, q, in, prec, c->cap_stream, o);
     hipGraph_t graph = nullptr;
@@ -1192,7 +1216,7 @@ int ocsc_destroy(ocsc_ctx* c) {
                  &c->keys[0],
                  &L.E64, &c->draws, &c->keys[1], &c->vals, &c->perm[0], &c->perm[1], &c->sort_tmp, &c->cic,
                  &c->mom_part, &c->mom_tab, &c->mom_ctr, &c->lsc_bins, &c->lsc_cic, &c->lsc_fac, &c->csr_bins,
-                 &c->rec})
+                 &c->rec, &c->apply_order})
     if (b->p) (void)hipFree(b->p);
   if (c->lsc) (void)hipFree(c->lsc);
   if (c->lsc_plan) (void)hipFree(c->lsc_plan);
@@ -1266,6 +1290,7 @@ int ocsc_apply_cells(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
   // cell-ordered copy for the deposit and the kick (not with cells: those
   // come in the caller's order)
   SortBufs sb{};
+  bool reuse = false, keep = false;
   if (!cells && n <= (int64_t)0xFFFFFFFF &&
       (c->apply_sort == 1 || (c->apply_sort == 2 && prec == OCSC_FP64 && n >= kApplySortMin))) {
     for (int k = 0; k < 2; ++k)
@@ -1274,17 +1299,36 @@ int ocsc_apply_cells(ocsc_ctx* c, double* r, int64_t ld, const double* q, int64_
         (rc = ensure(c->rec, (size_t)n * sizeof(PRec))) || (rc = ensure(c->rs[0], (size_t)6 * n * 8)) ||
         (rc = ensure(c->qs[0], (size_t)n * 8)))
       return rc;
+    // the cell order of a recent call on the same rows, or a sort whose order
+    // is kept for the next calls
+    reuse = c->apply_resort > 1 && c->ao_age > 0 && c->ao_age < c->apply_resort && c->ao_r == r && c->ao_n == n &&
+            c->ao_ld == ld;
+    keep = !reuse && c->apply_resort > 1;
+    if (keep && (rc = ensure(c->apply_order, (size_t)n * 4))) return rc;
+    c->ao_age = reuse ? c->ao_age : 0;  // set again once this call is enqueued
     sb = SortBufs{{(uint32_t*)c->keys[0].p, (uint32_t*)c->keys[1].p},
                   {(uint32_t*)c->vals.p, (uint32_t*)c->vals.p + n},
                   (uint32_t*)c->sort_tmp.p, (PRec*)c->rec.p, (double*)c->rs[0].p, (double*)c->qs[0].p,
-                  nullptr, true};
+                  keep ? (uint32_t*)c->apply_order.p : nullptr, true};
+    sb.order_in = reuse ? (const uint32_t*)c->apply_order.p : nullptr;
     o.sort = &sb;
   }
   if (c->apply_graph && !cells && !c->profiling)
     rc = apply_via_graph(c, L, g, r, ld, q, in, prec, st, o);
   else
     rc = enqueue_kick_prec(c, L, g, r, ld, q, in, prec, st, o);
-  if (rc) return rc;
+  if (rc) {
+    c->ao_age = 0;
+    return rc;
+  }
+  if (reuse) {
+    c->ao_age += 1;
+  } else if (keep) {
+    c->ao_r = r;
+    c->ao_n = n;
+    c->ao_ld = ld;
+    c->ao_age = 1;
+  }
   if (sync) {
     int32_t err = 0;
     if ((rc = read_errors(c, L, st, &err))) return rc;
@@ -3043,6 +3087,12 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->q_uniform = (int)value;
     return OCSC_OK;
   }
+  if (n == "apply_resort") {
+    if (value < 1 || value > 1000) return fail(OCSC_EINVAL, "apply_resort must be 1..1000");
+    c->apply_resort = (int)value;
+    c->ao_age = 0;  // the next sorting call starts a new interval
+    return OCSC_OK;
+  }
   if (n == "batch_fused") {
     if (value != 0 && value != 1) return fail(OCSC_EINVAL, "batch_fused must be 0 or 1");
     c->batch_fused = (int)value;
@@ -3140,6 +3190,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "q_uniform") {
     *value = c->q_uniform;
+    return OCSC_OK;
+  }
+  if (n == "apply_resort") {
+    *value = c->apply_resort;
     return OCSC_OK;
   }
   if (n == "deposit_tile") {

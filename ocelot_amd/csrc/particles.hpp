@@ -761,6 +761,20 @@ __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restr
   sort_keys_body(R, q, scp, keys, rec, cells, ahead, tag, blockIdx.x, gridDim.x);
 }
 
+// rec[j] = six rows and the charge of particle j (sequential): the records a
+// re-used cell order gathers from
+__global__ void __launch_bounds__(256) k_rows_to_rec(Rows R, const double* __restrict__ q, PRec* __restrict__ rec,
+                                                     int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  double v[8];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = ldnt(R.r + k * R.ld + j);
+  v[6] = ldnt(q + j);
+  v[7] = 0.0;
+  store_rec(rec + j, v);
+}
+
 // dst[:, j] = rec[perm[j]] (six rows and the charge), optionally applying a
 // drift to the rows it writes (as k_drift rounds it)
 template <bool DRIFT>

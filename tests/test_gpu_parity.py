@@ -569,6 +569,48 @@ def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_apply_reuses_cell_order_bit_identically(prec):
+    """One-shot kicks on a cell-ordered copy sort on the first of every
+    apply_resort calls on the same rows and re-use that call's order in
+    between.  Six kicks with a drift after each, and the caller's particles
+    (rows and charges) reordered in place before the fourth, give the bunch of
+    kicks that sort every time, bit for bit, after every kick (the re-using
+    runs before and after the sorting one, in the same context: the kept
+    sequences of both kinds of call are replayed or captured anew)."""
+    from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("gauss_2m_63", n=300_000, seed=29)
+    rng = np.random.default_rng(3)
+    q = q * (1.0 + 0.5 * rng.random(q.shape[0]))
+    dev = torch.cuda.current_device()
+    perm = torch.from_numpy(rng.permutation(r.shape[1])).to(f"cuda:{dev}")
+    saved = {k: _lib.get_option(dev, k) for k in ("apply_sort", "apply_resort")}
+    outs = []
+    try:
+        _lib.set_option(dev, "apply_sort", 1)
+        for every in (3, 1, 3):
+            _lib.set_option(dev, "apply_resort", every)
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            sc = SpaceCharge(nmesh_xyz=list(nmesh), precision=prec)
+            seq = []
+            for k in range(6):
+                if k == 3:
+                    d.rparticles.copy_(d.rparticles[:, perm])
+                    d.q_array.copy_(d.q_array[perm])
+                sc.apply(d, dz)
+                seq.append(d.rparticles.cpu().numpy())
+                d.rparticles[0] += 0.1 * d.rparticles[1]
+                d.rparticles[2] += 0.1 * d.rparticles[3]
+            outs.append(seq)
+    finally:
+        for k, v in saved.items():
+            _lib.set_option(dev, k, v)
+    for run in (0, 2):
+        for k, (a, b) in enumerate(zip(outs[run], outs[1])):
+            assert np.array_equal(a, b), (run, k)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("odd", [0, -1, None])
 def test_equal_charge_check_sees_every_charge(prec, odd):
     """A tracking call takes the charge from the kick scalars only when every
