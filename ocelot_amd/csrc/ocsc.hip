@@ -848,10 +848,12 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
     k_reduceA<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, o.prev_nblk);
   }
   mark(1);
+  // a kick that re-uses a cell order takes its particle records from pass B
+  PRec* const recB = o.sort && o.sort->order_in ? o.sort->rec : nullptr;
   if (prec == OCSC_MIXED)
-    k_passB<true><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p, zg, zg ? nz64 : 0);
+    k_passB<true><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p, zg, zg ? nz64 : 0, recB);
   else
-    k_passB<false><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p, zg, zg ? nz64 : 0);
+    k_passB<false><<<nblk, kPT, 0, st>>>(R, q, L.sc, (double*)L.part.p, zg, zg ? nz64 : 0, recB);
   k_reduceB<<<1, 1024, 0, st>>>(L.sc, (double*)L.part.p, nblk, prec);
   LAUNCHCHK("frame/extent passes");
   mark(2);
@@ -871,8 +873,7 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   if (sb) {
     const int nb = (int)((n + 255) / 256);
     if (sb->order_in) {
-      // an earlier call's cell order: records in the caller's order, no sort
-      k_rows_to_rec<<<nb, 256, 0, st>>>(R, q, sb->rec, n);
+      // an earlier call's cell order (the records came from pass B): no sort
       order = sb->order_in;
     } else {
       const int end_bit = sort_end_bit(g.d.nx, g.d.ny, g.d.nz, c->sort_cells);

@@ -228,6 +228,33 @@ __global__ void __launch_bounds__(256) k_fold(const double* __restrict__ part, i
     for (int c = 0; c < NV; ++c) out[c * nf + blockIdx.x] = v[c];
 }
 
+// ---------------------------------------------------------------------------
+// Permutations between the caller's particle order and the cell order.  A
+// permutation of the six SoA rows and the charge reads (or writes) seven
+// 8-byte values per particle at random places, each costing a whole memory
+// segment.  Staged through one 64-byte record per particle (rows, charge,
+// pad: one aligned segment), the random side moves one segment per particle
+// and every other access is sequential.
+struct alignas(64) PRec {
+  double v[8];  // x, x', y, y', tau, delta, q, 0
+};
+
+__device__ __forceinline__ void store_rec(PRec* __restrict__ dst, const double* v) {
+  double2* d = (double2*)dst->v;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) d[k] = double2{v[2 * k], v[2 * k + 1]};
+}
+
+__device__ __forceinline__ void load_rec(const PRec* __restrict__ src, double* v) {
+  const double2* s = (const double2*)src->v;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double2 a = s[k];
+    v[2 * k] = a.x;
+    v[2 * k + 1] = a.y;
+  }
+}
+
 // pass A: per-block sums of the momenta (sc.py:224 numerator)
 __device__ __forceinline__ void passA_body(Rows R, KickScalars* __restrict__ scp, double* __restrict__ part,
                                            const int bx, const int nbx) {
@@ -286,11 +313,13 @@ __global__ void __launch_bounds__(1024) k_reduceA(KickScalars* __restrict__ scp,
   reduceA_body(scp, part, nblk, blockIdx.x, gridDim.x);
 }
 
-// pass B: rest-frame min/max, total charge, charge-weighted sums
+// pass B: rest-frame min/max, total charge, charge-weighted sums.
+// rec (or null): each particle's record (rows, charge) is written as well,
+// rec[p] for the caller's particle p
 template <bool FAST>
 __device__ __forceinline__ void passB_body(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
                                            double* __restrict__ part, unsigned long long* __restrict__ zg,
-                                           int64_t zn, const int bx, const int nbx) {
+                                           int64_t zn, const int bx, const int nbx, PRec* __restrict__ rec = nullptr) {
   __shared__ double lds[(kPT / 64) * kNPartB];
   // the charge grid the deposit after this pass adds to is cleared here (its
   // last reader, the previous kick's z pass, ran earlier on the stream): no
@@ -336,6 +365,14 @@ __device__ __forceinline__ void passB_body(Rows R, const double* __restrict__ q,
       for (int k = 0; k < 6; ++k) b[k] = ldnt(R.r + k * R.ld + p2);
       qb = s.q_const ? s.q_val : ldnt(q + p2);
     }
+    if (rec) {
+      const double va[8] = {a[0], a[1], a[2], a[3], a[4], a[5], qa, 0.0};
+      store_rec(rec + p, va);
+      if (two) {
+        const double vb[8] = {b[0], b[1], b[2], b[3], b[4], b[5], qb, 0.0};
+        store_rec(rec + p2, vb);
+      }
+    }
     body(a, qa);
     if (two) body(b, qb);
   }
@@ -349,8 +386,8 @@ __device__ __forceinline__ void passB_body(Rows R, const double* __restrict__ q,
 template <bool FAST>
 __global__ void __launch_bounds__(kPT) k_passB(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
                                                double* __restrict__ part, unsigned long long* __restrict__ zg = nullptr,
-                                               int64_t zn = 0) {
-  passB_body<FAST>(R, q, scp, part, zg, zn, blockIdx.x, gridDim.x);
+                                               int64_t zn = 0, PRec* __restrict__ rec = nullptr) {
+  passB_body<FAST>(R, q, scp, part, zg, zn, blockIdx.x, gridDim.x, rec);
 }
 
 // cell sizes, grid offset, charge scale and kernel-cache decision
@@ -664,33 +701,6 @@ __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Permutations between the caller's particle order and the cell order.  A
-// permutation of the six SoA rows and the charge reads (or writes) seven
-// 8-byte values per particle at random places, each costing a whole memory
-// segment.  Staged through one 64-byte record per particle (rows, charge,
-// pad: one aligned segment), the random side moves one segment per particle
-// and every other access is sequential.
-struct alignas(64) PRec {
-  double v[8];  // x, x', y, y', tau, delta, q, 0
-};
-
-__device__ __forceinline__ void store_rec(PRec* __restrict__ dst, const double* v) {
-  double2* d = (double2*)dst->v;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) d[k] = double2{v[2 * k], v[2 * k + 1]};
-}
-
-__device__ __forceinline__ void load_rec(const PRec* __restrict__ src, double* v) {
-  const double2* s = (const double2*)src->v;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const double2 a = s[k];
-    v[2 * k] = a.x;
-    v[2 * k + 1] = a.y;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Cell ordering of a resident bunch (multi-kick tracking): sort keys are the
 // NGP cells under the scalars of the last kick; particles move little between
 // re-sorts, so consecutive particles keep sharing cells.
@@ -759,20 +769,6 @@ __global__ void __launch_bounds__(kPT) k_sort_keys(Rows R, const double* __restr
                                                    PRec* __restrict__ rec, int cells, DriftArgs ahead,
                                                    uint32_t tag = 0) {
   sort_keys_body(R, q, scp, keys, rec, cells, ahead, tag, blockIdx.x, gridDim.x);
-}
-
-// rec[j] = six rows and the charge of particle j (sequential): the records a
-// re-used cell order gathers from
-__global__ void __launch_bounds__(256) k_rows_to_rec(Rows R, const double* __restrict__ q, PRec* __restrict__ rec,
-                                                     int64_t n) {
-  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (j >= n) return;
-  double v[8];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) v[k] = ldnt(R.r + k * R.ld + j);
-  v[6] = ldnt(q + j);
-  v[7] = 0.0;
-  store_rec(rec + j, v);
 }
 
 // dst[:, j] = rec[perm[j]] (six rows and the charge), optionally applying a
