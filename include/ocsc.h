@@ -338,7 +338,10 @@ int ocsc_apply_batched(ocsc_ctx* ctx, const ocsc_bunch* bunches, int nb, int pre
  * default 10) on the same rows (pointer, particle count and row stride) and
  * the calls in between re-use that call's cell order (any order gives the
  * same kick; a recent one keeps the locality): 1 sorts on every call;
- * bit-identical results.  "deposit_tile": particles per thread and
+ * bit-identical results.  "rec_direct": such a kick with the default
+ * deposit layout reads the particle records through the cell order in the
+ * deposit and the kick (1, default) or gathers them into a cell-ordered
+ * copy of the rows first (0); bit-identical results.  "deposit_tile": particles per thread and
  * LDS hash slots of the deposit, 0 (default) 8 and 2048 (FP64: 1024), 1 half
  * of both, 2 twice both, 3 no table (one global atomic per particle), 4 as 0
  * with all of a thread's particle loads issued first; bit-identical results.  "batch_sort": see

@@ -323,6 +323,9 @@ struct ocsc_ctx {
   // permutation gives the same kick bit for bit; a recent one keeps the
   // deposit's and the kick's locality), and the rows it was sorted for
   int apply_resort = 10;
+  // such kicks read the records through the cell order in the deposit and
+  // the kick (1) or gather them into a cell-ordered copy of the rows first (0)
+  int rec_direct = 1;
   Buf apply_order;
   const double* ao_r = nullptr;
   int64_t ao_n = 0, ao_ld = 0;
@@ -728,6 +731,8 @@ struct StepOpts {
   // kick of a tracking call on a cell-ordered copy)
   const uint32_t* rec_perm = nullptr;
   PRec* rec_out = nullptr;
+  // and reads them from there too (rec_out[rec_perm[p]]) instead of rows
+  bool rec_in = false;
   // tracking: the charge grid is zero on entry (the previous step's kick
   // cleared it), and this step's kick clears it for the next step (the
   // z pass, its last reader, runs before the kick)
@@ -780,7 +785,10 @@ void launch_kick(ocsc_ctx* c, Lane& L, int npart, double* r, int64_t ld, const S
   // mixed form: the three FP32 field grids; FP64 form: the k_diff64 fields
   const F* field = sizeof(F) == 8 ? (const F*)L.E64.p : (const F*)L.phi.p;
   unsigned long long* zg = o.zero_next ? (unsigned long long*)L.grid.p : nullptr;
-  if (!DRIFT && !SUMP && o.rec_out)
+  if (!DRIFT && !SUMP && o.rec_out && o.rec_in)
+    k_kick<F, false, false, true, true><<<npart, kKickPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p,
+                                                                   o.rec_perm, o.rec_out, zg, zg ? nz64 : 0);
+  else if (!DRIFT && !SUMP && o.rec_out)
     k_kick<F, false, false, true><<<npart, kKickPT, 0, st>>>(r, ld, field, L.sc, o.dr, (double*)L.part.p, o.rec_perm,
                                                              o.rec_out, zg, zg ? nz64 : 0);
   else
@@ -870,6 +878,10 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   // particle, so the result equals the caller-order kick bit for bit
   const uint32_t* order = nullptr;
   const SortBufs* sb = o.sort;
+  // a one-shot kick on the cell order with the default deposit reads the
+  // records through the order in the deposit and the kick: no sorted copy
+  const bool rec_direct = sb && sb->scatter_back && !o.drift && !o.sump && !c->deposit_xcd && !c->deposit_agg &&
+                          c->deposit_tile == 0 && c->rec_direct;
   if (sb) {
     const int nb = (int)((n + 255) / 256);
     if (sb->order_in) {
@@ -881,13 +893,22 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
                                                               DriftArgs{0, 0});
       order = sb->v[rs_sort(sb->k, sb->v, n, end_bit, sb->hist, st)];
     }
-    k_gather_rec<false><<<nb, 256, 0, st>>>(sb->rec, order, sb->rs, n, sb->qs, n, DriftArgs{0, 0});
+    if (!rec_direct)
+      k_gather_rec<false><<<nb, 256, 0, st>>>(sb->rec, order, sb->rs, n, sb->qs, n, DriftArgs{0, 0});
     if (sb->order_out) HIPCHK(hipMemcpyAsync(sb->order_out, order, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
     LAUNCHCHK("cell-order copy");
-    R = Rows{sb->rs, n};
-    q = sb->qs;
+    if (!rec_direct) {
+      R = Rows{sb->rs, n};
+      q = sb->qs;
+    }
   }
-  launch_deposit<NWORD>(c, R, q, L.sc, (unsigned long long*)L.grid.p, o.cells, n, st);
+  if (rec_direct) {
+    constexpr int per = kDepPT * DepTile<0>::items;
+    k_deposit<NWORD, false, false, 0, true><<<(int)((n + per - 1) / per), kDepPT, 0, st>>>(
+        R, q, L.sc, (unsigned long long*)L.grid.p, nullptr, sb->rec, order);
+  } else {
+    launch_deposit<NWORD>(c, R, q, L.sc, (unsigned long long*)L.grid.p, o.cells, n, st);
+  }
   LAUNCHCHK("deposit");
   mark(3);
   if (!fork && (rc = launch_kernel_spectrum<F>(c, L, g, st, 0))) return rc;
@@ -911,9 +932,11 @@ int enqueue_kick(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t ld,
   StepOpts ok = o;
   ok.zero_next = o.zero_next && !(sb && sb->scatter_back && !o.drift && !o.sump);
   if (sb && sb->scatter_back && !o.drift && !o.sump) {
-    // the kick writes each record straight to its caller position
+    // the kick writes each record straight to its caller position (and, with
+    // rec_direct, reads it from there)
     ok.rec_perm = order;
     ok.rec_out = sb->rec;
+    ok.rec_in = rec_direct;
   }
   if (o.drift && o.sump)
     launch_kick<F, true, true>(c, L, npart, rk, ldk, ok, nz64, st);
@@ -1043,7 +1066,8 @@ int apply_via_graph(ocsc_ctx* c, Lane& L, const Geometry& g, double* r, int64_t 
   const bool reuse = o.sort && o.sort->order_in;
   auto& A = c->ag[reuse ? 1 : 0];
   const std::vector<int64_t> key = {(int64_t)(intptr_t)r, ld, (int64_t)(intptr_t)q, in.n, g.d.nx, g.d.ny, g.d.nz,
-                                    prec, o.sort != nullptr, reuse, o.sort && o.sort->order_out, c->overlap,
+                                    prec, o.sort != nullptr, reuse, o.sort && o.sort->order_out, c->rec_direct,
+                                    c->overlap,
                                     c->deposit_xcd, c->deposit_agg,
                                     c->deposit_tile, c->fft_plan, c->sort_cells, c->red_blocks, c->grid_clear,
                                     c->igf_blocks, (int64_t)g_alloc_gen.load()};
@@ -3088,6 +3112,11 @@ int ocsc_set_option(ocsc_ctx* c, const char* name, int64_t value) {
     c->q_uniform = (int)value;
     return OCSC_OK;
   }
+  if (n == "rec_direct") {
+    if (value != 0 && value != 1) return fail(OCSC_EINVAL, "rec_direct must be 0 or 1");
+    c->rec_direct = (int)value;
+    return OCSC_OK;
+  }
   if (n == "apply_resort") {
     if (value < 1 || value > 1000) return fail(OCSC_EINVAL, "apply_resort must be 1..1000");
     c->apply_resort = (int)value;
@@ -3195,6 +3224,10 @@ int ocsc_get_option(ocsc_ctx* c, const char* name, int64_t* value) {
   }
   if (n == "apply_resort") {
     *value = c->apply_resort;
+    return OCSC_OK;
+  }
+  if (n == "rec_direct") {
+    *value = c->rec_direct;
     return OCSC_OK;
   }
   if (n == "deposit_tile") {

@@ -567,10 +567,13 @@ __device__ __forceinline__ void hash_add(int* hkey, long long (*hval)[kSlots], u
 // cells in lane order are summed first (segmented inclusive scan with
 // shuffles, exact in integers); only the last lane of each run goes to the
 // table.  Cells that recur in separate runs are simply added twice.
-template <int NWORD, bool XCD, bool AGG, int TILE>
+// RECIN: particle p of the cell order is the record rin[ord[p]] (rows and
+// charge) instead of column p of R and q
+template <int NWORD, bool XCD, bool AGG, int TILE, bool RECIN = false>
 __device__ __forceinline__ void deposit_body(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
                                              unsigned long long* __restrict__ grid, int32_t* __restrict__ cells,
-                                             const int bx, const int nbx) {
+                                             const int bx, const int nbx, const PRec* __restrict__ rin = nullptr,
+                                             const uint32_t* __restrict__ ord = nullptr) {
   constexpr int kSlots = DepTile<TILE>::template slots<NWORD>();
   constexpr int kItems = DepTile<TILE>::items;
   __shared__ int hkey[kSlots];
@@ -594,9 +597,17 @@ __device__ __forceinline__ void deposit_body(Rows R, const double* __restrict__ 
   const int lane = threadIdx.x & 63;
   int bad = 0;
   auto load = [&](int64_t p, double* v) {
+    if constexpr (RECIN) {
+      double t[8];
+      load_rec(rin + ord[p], t);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = ldnt(R.r + k * R.ld + p);
-    v[6] = s.q_const ? s.q_val : ldnt(q + p);
+      for (int k = 0; k < 6; ++k) v[k] = t[k];
+      v[6] = s.q_const ? s.q_val : t[6];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) v[k] = ldnt(R.r + k * R.ld + p);
+      v[6] = s.q_const ? s.q_val : ldnt(q + p);
+    }
   };
   // one particle (valid: p < n) through the table; cur holds its rows
   auto item = [&](int64_t p, bool valid, const double* cur) {
@@ -694,10 +705,12 @@ __device__ __forceinline__ void deposit_body(Rows R, const double* __restrict__ 
   }
 }
 
-template <int NWORD, bool XCD = false, bool AGG = false, int TILE = 0>
+template <int NWORD, bool XCD = false, bool AGG = false, int TILE = 0, bool RECIN = false>
 __global__ void __launch_bounds__(kDepPT) k_deposit(Rows R, const double* __restrict__ q, KickScalars* __restrict__ scp,
-                                                 unsigned long long* __restrict__ grid, int32_t* __restrict__ cells) {
-  deposit_body<NWORD, XCD, AGG, TILE>(R, q, scp, grid, cells, blockIdx.x, gridDim.x);
+                                                 unsigned long long* __restrict__ grid, int32_t* __restrict__ cells,
+                                                 const PRec* __restrict__ rin = nullptr,
+                                                 const uint32_t* __restrict__ ord = nullptr) {
+  deposit_body<NWORD, XCD, AGG, TILE, RECIN>(R, q, scp, grid, cells, blockIdx.x, gridDim.x, rin, ord);
 }
 
 // ---------------------------------------------------------------------------
@@ -1180,7 +1193,9 @@ constexpr int kKickPT = 256;
 // of a tracking call and the way back to the caller's order in one pass.
 // zg, zn (tracking): zn words of the charge grid to clear for the next step's
 // deposit (the z pass that read them ran before this kick).
-template <typename F, bool DRIFT, bool SUMP, bool RECOUT>
+// RECIN (with RECOUT): particle p is read from the record rec[perm[p]] that
+// it is written back to, instead of column p of r
+template <typename F, bool DRIFT, bool SUMP, bool RECOUT, bool RECIN = false>
 __device__ __forceinline__ void kick_body(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
                                           const KickScalars* __restrict__ scp, DriftArgs dr, double* __restrict__ part,
                                           const uint32_t* __restrict__ perm, PRec* __restrict__ rec,
@@ -1202,9 +1217,16 @@ __device__ __forceinline__ void kick_body(double* __restrict__ r, int64_t ld, co
   double xp[6], xyz[3], X[3], pr[3], E[3], pb[3], o[8];
   double m[3] = {0, 0, 0};
   const int nn[3] = {s.nmesh[0], s.nmesh[1], s.nmesh[2]};
+  static_assert(!RECIN || RECOUT, "records in only with records out");
   if (active) {
-    to_xp_fast(ldnt(r + p), ldnt(r + ld + p), ldnt(r + 2 * ld + p), ldnt(r + 3 * ld + p), ldnt(r + 4 * ld + p),
-               ldnt(r + 5 * ld + p), s, xp);
+    if constexpr (RECIN) {
+      double t[8];
+      load_rec(rec + perm[p], t);
+      to_xp_fast(t[0], t[1], t[2], t[3], t[4], t[5], s, xp);
+    } else {
+      to_xp_fast(ldnt(r + p), ldnt(r + ld + p), ldnt(r + 2 * ld + p), ldnt(r + 3 * ld + p), ldnt(r + 4 * ld + p),
+                 ldnt(r + 5 * ld + p), s, xp);
+    }
     rot(xp, s.T, xyz);
     xyz[2] = xyz[2] * s.gamma0;
     if constexpr (FAST)
@@ -1259,14 +1281,14 @@ __device__ __forceinline__ void kick_body(double* __restrict__ r, int64_t ld, co
   }
 }
 
-template <typename F, bool DRIFT = false, bool SUMP = false, bool RECOUT = false>
+template <typename F, bool DRIFT = false, bool SUMP = false, bool RECOUT = false, bool RECIN = false>
 __global__ void __launch_bounds__(kKickPT) k_kick(double* __restrict__ r, int64_t ld, const F* __restrict__ phi,
                                                   const KickScalars* __restrict__ scp, DriftArgs dr = {0, 0},
                                                   double* __restrict__ part = nullptr,
                                                   const uint32_t* __restrict__ perm = nullptr,
                                                   PRec* __restrict__ rec = nullptr,
                                                   unsigned long long* __restrict__ zg = nullptr, int64_t zn = 0) {
-  kick_body<F, DRIFT, SUMP, RECOUT>(r, ld, phi, scp, dr, part, perm, rec, zg, zn, blockIdx.x, gridDim.x);
+  kick_body<F, DRIFT, SUMP, RECOUT, RECIN>(r, ld, phi, scp, dr, part, perm, rec, zg, zn, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------

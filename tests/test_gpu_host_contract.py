@@ -85,6 +85,7 @@ def test_two_threads_on_one_device_equal_sequential_kicks():
     ("fold_frame", (0, 1), (-1, 2)),
     ("q_uniform", (0, 1), (-1, 2)),
     ("apply_resort", (1, 2, 1000), (0, 1001)),
+    ("rec_direct", (0, 1), (-1, 2)),
 ])
 def test_options_round_trip_and_reject_out_of_range(name, good, bad):
     """Every value an option documents is kept and read back; values outside

@@ -572,11 +572,13 @@ def test_speed_options_leave_tracking_bit_identical(prec, opt, vals):
 def test_apply_reuses_cell_order_bit_identically(prec):
     """One-shot kicks on a cell-ordered copy sort on the first of every
     apply_resort calls on the same rows and re-use that call's order in
-    between.  Six kicks with a drift after each, and the caller's particles
-    (rows and charges) reordered in place before the fourth, give the bunch of
-    kicks that sort every time, bit for bit, after every kick (the re-using
-    runs before and after the sorting one, in the same context: the kept
-    sequences of both kinds of call are replayed or captured anew)."""
+    between; with rec_direct the deposit and the kick read the particle
+    records through the order instead of a sorted copy of the rows.  Six kicks
+    with a drift after each, and the caller's particles (rows and charges)
+    reordered in place before the fourth, give the bunch of kicks that sort
+    every time into a copy, bit for bit, after every kick (the other runs
+    before and after that one, in the same context: the kept sequences of
+    each kind of call are replayed or captured anew)."""
     from ocelot_amd import DeviceParticleArray, SpaceCharge, _lib
     from ocelot_amd.workloads import make_bunch
     r, q, E, nmesh, dz = make_bunch("gauss_2m_63", n=300_000, seed=29)
@@ -584,11 +586,13 @@ def test_apply_reuses_cell_order_bit_identically(prec):
     q = q * (1.0 + 0.5 * rng.random(q.shape[0]))
     dev = torch.cuda.current_device()
     perm = torch.from_numpy(rng.permutation(r.shape[1])).to(f"cuda:{dev}")
-    saved = {k: _lib.get_option(dev, k) for k in ("apply_sort", "apply_resort")}
+    saved = {k: _lib.get_option(dev, k) for k in ("apply_sort", "apply_resort", "rec_direct")}
+    runs = [(1, 3), (0, 1), (1, 3), (1, 1), (0, 3)]  # (rec_direct, apply_resort); run 1 is the reference
     outs = []
     try:
         _lib.set_option(dev, "apply_sort", 1)
-        for every in (3, 1, 3):
+        for direct, every in runs:
+            _lib.set_option(dev, "rec_direct", direct)
             _lib.set_option(dev, "apply_resort", every)
             d = DeviceParticleArray.from_host(parray(r, q, E), dev)
             sc = SpaceCharge(nmesh_xyz=list(nmesh), precision=prec)
@@ -605,9 +609,9 @@ def test_apply_reuses_cell_order_bit_identically(prec):
     finally:
         for k, v in saved.items():
             _lib.set_option(dev, k, v)
-    for run in (0, 2):
+    for run in (0, 2, 3, 4):
         for k, (a, b) in enumerate(zip(outs[run], outs[1])):
-            assert np.array_equal(a, b), (run, k)
+            assert np.array_equal(a, b), (runs[run], k)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
