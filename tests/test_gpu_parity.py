@@ -649,6 +649,36 @@ def test_equal_charge_check_sees_every_charge(prec, odd):
         assert not np.array_equal(outs[2], outs[1])
 
 
+@pytest.mark.parametrize("mode", ["tws", "lsc"])
+def test_equal_charge_shortcut_in_tws_and_lsc_tracking(mode):
+    """The equal-charge shortcut in the tracking variants that share the
+    kick's set-up: with per-step Twiss (calc_tws) and with an LSC kick after
+    every step, tracking with the check on gives the bunch (and the Twiss
+    list) of tracking with it off, bit for bit."""
+    from ocelot_amd import LSC, DeviceParticleArray, SpaceCharge, _lib, track_drift
+    from ocelot_amd.workloads import make_bunch
+    r, q, E, nmesh, dz = make_bunch("flattop_10m_127", n=300_000, seed=37)
+    dev = torch.cuda.current_device()
+    old = _lib.get_option(dev, "q_uniform")
+    outs, twss = [], []
+    try:
+        for v in (0, 1):
+            _lib.set_option(dev, "q_uniform", v)
+            d = DeviceParticleArray.from_host(parray(r, q, E), dev)
+            sc = SpaceCharge(nmesh_xyz=[31, 31, 47], precision="mixed")
+            if mode == "tws":
+                tws = track_drift(d, sc, dz, dz, 6, use_graph=True, calc_tws=True)
+                twss.append([(t.emit_x, t.beta_x, t.emit_y, t.s) for t in tws])
+            else:
+                track_drift(d, sc, dz, dz, 6, use_graph=True, lsc=LSC())
+            outs.append(d.rparticles.cpu().numpy())
+    finally:
+        _lib.set_option(dev, "q_uniform", old)
+    assert np.array_equal(outs[0], outs[1])
+    if twss:
+        assert twss[0] == twss[1]
+
+
 @pytest.mark.parametrize("plan", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("nmesh", [(31, 31, 63), (127, 15, 15), (15, 15, 255), (7, 127, 7), (127, 127, 31)])
 def test_fft_plans_agree_with_fp64(plan, nmesh):
